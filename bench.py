@@ -1,0 +1,199 @@
+"""Benchmark: CKKS HMult+relin per second at N=2^16, L=35 (12 special primes,
+dnum=3), bit-exact path, on one MI355X; plus the NTT's HBM throughput.
+
+A step is one pass of the hot path over one batch: B ciphertext pairs, each
+multiplied and relinearised with the same resident relinearisation key
+(plain path: one ciphertext pair per call). Inputs are synthetic, seeded,
+uniform residues already resident in HBM when timing starts.
+
+Prints one JSON line (rank 0). `--gpus N` with N > 1 exits: this build runs
+on a single GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "phantom-fhe_amd"), os.path.join(ROOT, "tests"), ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+CONFIG2_BITS = [60] + [50] * 34 + [60] * 12
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_context(ph, N, bits, size_P):
+    mods = ph.coeff_modulus_create(N, bits)
+    parms = ph.EncryptionParameters(ph.SCHEME_CKKS)
+    parms.set_poly_modulus_degree(N)
+    parms.set_coeff_modulus(mods)
+    parms.set_special_modulus_size(size_P)
+    return ph.context(parms), mods
+
+
+def rand_ct(rng, mods, N):
+    return np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in mods]) for _ in range(2)])
+
+
+def bench_hmult(ph, ctx, mods, N, batch, steps, warmup, stream):
+    rng = np.random.default_rng(1234)
+    Ql = ctx.size_Q
+    key = np.stack([np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in mods])
+                              for _ in range(2)]) for _ in range(ctx.dnum)])
+    rlk = ph.relin_key(ctx, host=key)
+    del key
+    a = [ph.ciphertext(ctx, 1, 2) for _ in range(batch)]
+    b = [ph.ciphertext(ctx, 1, 2) for _ in range(batch)]
+    for i in range(batch):
+        a[i].upload(rand_ct(rng, mods[:Ql], N))
+        b[i].upload(rand_ct(rng, mods[:Ql], N))
+
+    def step():
+        for i in range(batch):
+            ph.multiply_and_relin_inplace(ctx, a[i], b[i], rlk, stream)
+
+    for _ in range(warmup):
+        step()
+    ph.synchronize(stream)
+    ev0, ev1 = ph.Event(), ph.Event()
+    ph.device_synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        step()
+    ev1.record(stream)
+    ph.synchronize(stream)
+    ph.device_synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev0.elapsed_ms(ev1)
+    return ms, wall
+
+
+def bench_ntt(ph, ctx, N, L, npolys, iters, stream):
+    """Forward NTT over npolys x L limbs in one launch pair (pass 1 + pass 2)."""
+    rng = np.random.default_rng(7)
+    mods = ctx.coeff_modulus[:L]
+    host = np.concatenate([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in mods])
+                           for _ in range(npolys)])
+    buf = ph.DeviceBuffer.from_array(host)
+    del host
+    for _ in range(3):
+        ph.ntt_forward_batch(ctx, buf, L, 0, npolys, L * N, stream)
+    ph.synchronize(stream)
+    evs = [ph.Event() for _ in range(iters + 1)]
+    evs[0].record(stream)
+    for i in range(iters):
+        ph.ntt_forward_batch(ctx, buf, L, 0, npolys, L * N, stream)
+        evs[i + 1].record(stream)
+    ph.synchronize(stream)
+    per = [evs[i].elapsed_ms(evs[i + 1]) for i in range(iters)]
+    return float(np.mean(per)), float(np.min(per))
+
+
+def cpu_baseline(N, bits, size_P, budget_s):
+    import oracle_bind as ob
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    ob.set_threads(threads)
+    mods = ob.coeff_modulus_create(N, bits)
+    octx = ob.Ctx(N.bit_length() - 1, mods, size_P)
+    rng = np.random.default_rng(99)
+    Ql = len(bits) - size_P
+    dnum = (Ql + size_P - 1) // size_P
+    a, b = rand_ct(rng, mods[:Ql], N), rand_ct(rng, mods[:Ql], N)
+    key = np.stack([np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in mods])
+                              for _ in range(2)]) for _ in range(dnum)])
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        octx.multiply_and_relin(Ql, a, b, key)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return {"value": n / el, "unit": "HMult+relin/s", "cores": threads, "kind": "port",
+            "sample": f"{n} HMult+relin at N=2^16, L=35+12, dnum=3 on the oracle (CPU restatement of the "
+                      f"reference algorithm, {threads} threads), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--ntt-polys", type=int, default=16)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import pyphantom as ph
+
+    N = 1 << 16
+    ctx, mods = make_context(ph, N, CONFIG2_BITS, 12)
+    stream = ph.stream_create()
+    log(f"bench: HMult+relin N=2^16 L=35 P=12 dnum={ctx.dnum} batch={args.batch} steps={args.steps}")
+    ms, wall = bench_hmult(ph, ctx, mods, N, args.batch, args.steps, args.warmup, stream)
+    ops = args.batch * args.steps
+    value = ops / (ms / 1e3)
+    log(f"bench: {ops} ops in {ms:.2f} ms (events), {wall * 1e3:.2f} ms wall -> {value:.1f} op/s")
+
+    # dominant kernel family: the forward NTT (two passes) over 47 x P limbs
+    L = ctx.size_Q + ctx.size_P
+    ntt_ms, ntt_min = bench_ntt(ph, ctx, N, L, args.ntt_polys, 10, stream)
+    ntt_bytes = 2 * N * 8 * L * args.ntt_polys
+    achieved = ntt_bytes / (ntt_ms / 1e3) / 1e9
+    log(f"bench: NTT {L}x{args.ntt_polys} limbs: {ntt_ms:.3f} ms -> {achieved:.1f} GB/s algorithmic")
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "ntt_fwd_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU baseline (oracle) ...")
+        cpu = cpu_baseline(N, CONFIG2_BITS, 12, args.cpu_seconds)
+
+    # algorithmic bytes of one HMult+relin with the key streamed per op:
+    # (4*35 in + 2*35 out + 3*2*47 key) limbs x N x 8 B
+    op_bytes = (4 * 35 + 2 * 35 + 3 * 2 * 47) * N * 8
+    out = {
+        "metric": "HMult+relin/sec at N=2^16 L=35 (dnum=3), bit-exact",
+        "value": round(value, 2),
+        "unit": "HMult+relin/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded uniform residues, resident in HBM)",
+        "config": {"workload": "CKKS HMult+relin, N=65536, 35 data limbs (60+34x50 bit) + 12 special (60 bit), "
+                               "dnum=3, plain path (one ciphertext pair per call)",
+                   "batch_per_step": args.batch, "mode": "plain"},
+        "op_algorithmic_GBps": round(op_bytes * value / 1e9, 1),
+        "ntt": {"limbs": L * args.ntt_polys, "N": N, "ms": round(ntt_ms, 4), "ms_min": round(ntt_min, 4),
+                "GBps_algorithmic": round(achieved, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "forward NTT (k_fwd_cols + k_fwd_rows), algorithmic bytes 2*N*8 per limb"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,187 @@
+/*
+ * phantom_amd.h -- C ABI of the MI355X (gfx950) implementation of
+ * PhantomFHE's RNS polynomial hot path: negacyclic NTT/INTT, RNS base
+ * conversion (modup/moddown) and hybrid key switching (relinearisation,
+ * Galois rotation), plus CKKS rescale.
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the PhantomFHE source tree). All functions return PHX_OK (0)
+ * or a negative status; phx_last_error() returns the thread's last message.
+ * Status codes map to the reference's exception types:
+ *   PHX_EINVAL   -> std::invalid_argument
+ *   PHX_ELOGIC   -> std::logic_error
+ *   PHX_ERUNTIME -> std::runtime_error (HIP / allocation failures)
+ *
+ * Device buffers hold u64 residues laid out like the reference:
+ *   ciphertext [poly][limb][N], key [digit][2][QP][N], t_mod_up [digit][QlP][N].
+ * phx_stream is a hipStream_t; NULL selects the per-thread default stream
+ * (the reference runs on cudaStreamPerThread, src/evaluate.cu:1079).
+ */
+#ifndef PHANTOM_AMD_H
+#define PHANTOM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHX_OK 0
+#define PHX_EINVAL (-1)
+#define PHX_ELOGIC (-2)
+#define PHX_ERUNTIME (-3)
+
+/* scheme_type / mul_tech_type values of include/host/encryptionparams.h:14-35 */
+#define PHX_SCHEME_BGV 1
+#define PHX_SCHEME_BFV 2
+#define PHX_SCHEME_CKKS 3
+#define PHX_MUL_TECH_NONE 0
+#define PHX_MUL_TECH_BEHZ 1
+#define PHX_MUL_TECH_HPS 2
+
+typedef struct phx_context phx_context;
+typedef struct phx_ciphertext phx_ciphertext;
+typedef struct phx_secret_key phx_secret_key;
+typedef struct phx_relin_key phx_relin_key;
+typedef struct phx_galois_key phx_galois_key;
+typedef void *phx_stream;
+
+/* thread-local message of the last failed call */
+const char *phx_last_error(void);
+
+/* EncryptionParameters (include/host/encryptionparams.h:37-245) */
+typedef struct {
+    int scheme;
+    int mul_tech;
+    uint64_t poly_modulus_degree;
+    const uint64_t *coeff_modulus;   /* size_Q data primes, then the special primes */
+    size_t coeff_modulus_size;
+    size_t special_modulus_size;
+    uint64_t plain_modulus;
+    const uint32_t *galois_elts;     /* may be NULL */
+    size_t galois_elts_size;
+} phx_params;
+
+/* ---- parameter maths ---------------------------------------------------- */
+/* CoeffModulus::Create (src/host/modulus.cu:80-111) */
+int phx_coeff_modulus_create(uint64_t poly_modulus_degree, const int *bit_sizes, size_t n, uint64_t *out);
+/* get_elt_from_step (include/galois.cuh:16-49) */
+int phx_galois_elt_from_step(int step, uint64_t poly_modulus_degree, uint32_t *out);
+
+/* ---- context ------------------------------------------------------------ */
+/* PhantomContext::PhantomContext (src/context.cu:121-232) */
+int phx_context_create(const phx_params *params, phx_context **out);
+int phx_context_destroy(phx_context *ctx);
+/* sizes: N, size_Q, size_P, dnum (= ceil(size_Q / size_P)) */
+int phx_context_info(const phx_context *ctx, uint64_t *N, size_t *size_Q, size_t *size_P, size_t *dnum);
+/* limbs of chain index c (context_data_[c], src/context.cu:145-159) */
+int phx_context_limbs_at(const phx_context *ctx, size_t chain_index, size_t *limbs);
+
+/* ---- ciphertext (include/ciphertext.h:7-308) ------------------------------ */
+int phx_ciphertext_create(phx_ciphertext **out);
+int phx_ciphertext_destroy(phx_ciphertext *ct);
+/* PhantomCiphertext::resize(context, chain_index, size, stream) (include/ciphertext.h:44-72) */
+int phx_ciphertext_resize(const phx_context *ctx, phx_ciphertext *ct, size_t chain_index, size_t size,
+                          phx_stream stream);
+int phx_ciphertext_upload(phx_ciphertext *ct, const uint64_t *host, size_t n_elems, phx_stream stream);
+int phx_ciphertext_download(const phx_ciphertext *ct, uint64_t *host, size_t n_elems, phx_stream stream);
+int phx_ciphertext_copy(const phx_context *ctx, phx_ciphertext *dst, const phx_ciphertext *src, phx_stream stream);
+int phx_ciphertext_info(const phx_ciphertext *ct, size_t *chain_index, size_t *size, size_t *coeff_modulus_size,
+                        double *scale, int *is_ntt_form);
+int phx_ciphertext_set_scale(phx_ciphertext *ct, double scale);
+int phx_ciphertext_data(const phx_ciphertext *ct, uint64_t **dev_ptr);
+
+/* ---- keys (include/secretkey.h) ------------------------------------------ */
+/* PhantomSecretKey(context): ternary secret from a seed, stored in NTT form */
+int phx_secret_key_create(const phx_context *ctx, uint64_t seed, phx_secret_key **out);
+/* secret from explicit ternary coefficients (N int8 values in {-1,0,1}) */
+int phx_secret_key_create_from_coeffs(const phx_context *ctx, const int8_t *coeffs, phx_secret_key **out);
+int phx_secret_key_download(const phx_context *ctx, const phx_secret_key *sk, uint64_t *host_qp_n);
+int phx_secret_key_destroy(phx_secret_key *sk);
+/* PhantomSecretKey::gen_relinkey (src/secretkey.cu:396-419), dnum = ceil(size_Q/size_P) */
+int phx_relin_key_create(const phx_context *ctx, phx_secret_key *sk, uint64_t seed, phx_relin_key **out);
+/* load a key laid out [dnum][2][QP][N] (PhantomRelinKey::load, include/secretkey.h:129-162) */
+int phx_relin_key_create_from_host(const phx_context *ctx, const uint64_t *host, size_t dnum, phx_relin_key **out);
+int phx_relin_key_download(const phx_context *ctx, const phx_relin_key *key, uint64_t *host);
+int phx_relin_key_destroy(phx_relin_key *key);
+/* PhantomSecretKey::create_galois_keys (src/secretkey.cu:421-461) */
+int phx_galois_key_create(const phx_context *ctx, const phx_secret_key *sk, const uint32_t *elts, size_t n,
+                          uint64_t seed, phx_galois_key **out);
+/* load keys [n][dnum][2][QP][N] for the listed elements (include/secretkey.h:194-219) */
+int phx_galois_key_create_from_host(const phx_context *ctx, const uint32_t *elts, size_t n, const uint64_t *host,
+                                    size_t dnum, phx_galois_key **out);
+int phx_galois_key_destroy(phx_galois_key *key);
+
+/* ---- symmetric encryption / decryption (CKKS, NTT form) ------------------- */
+/* PhantomSecretKey::encrypt_symmetric (src/secretkey.cu:463-531); plain = [Ql][N] coefficient residues */
+int phx_encrypt_symmetric(const phx_context *ctx, const phx_secret_key *sk, const uint64_t *plain_host,
+                          size_t chain_index, uint64_t seed, phx_ciphertext *out, phx_stream stream);
+/* PhantomSecretKey::ckks_decrypt (src/secretkey.cu:533-569) followed by the INTT: out = [Ql][N] */
+int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext *ct, uint64_t *out_host,
+                phx_stream stream);
+
+/* ---- evaluator: the hot path (include/evaluate.cuh) ------------------------ */
+/* multiply_and_relin_inplace (include/evaluate.cuh:115-116, src/evaluate.cu:1061-1104) */
+int phx_multiply_and_relin_inplace(const phx_context *ctx, phx_ciphertext *ct1, const phx_ciphertext *ct2,
+                                   const phx_relin_key *rlk, phx_stream stream);
+/* multiply_inplace (include/evaluate.cuh:104-105): tensor only, result size 3 */
+int phx_multiply_inplace(const phx_context *ctx, phx_ciphertext *ct1, const phx_ciphertext *ct2, phx_stream stream);
+/* relinearize_inplace (include/evaluate.cuh:125-126, src/evaluate.cu:1342-1374) */
+int phx_relinearize_inplace(const phx_context *ctx, phx_ciphertext *ct, const phx_relin_key *rlk, phx_stream stream);
+/* rescale_to_next (include/evaluate.cuh:137, src/evaluate.cu:1545-1565) */
+int phx_rescale_to_next(const phx_context *ctx, const phx_ciphertext *in, phx_ciphertext *out, phx_stream stream);
+/* apply_galois_inplace (include/evaluate.cuh:215-216, src/evaluate.cu:1567-1630) */
+int phx_apply_galois_inplace(const phx_context *ctx, phx_ciphertext *ct, uint32_t galois_elt,
+                             const phx_galois_key *gk, phx_stream stream);
+/* rotate_inplace (include/evaluate.cuh:225-226, src/evaluate.cu:1633-1668) */
+int phx_rotate_inplace(const phx_context *ctx, phx_ciphertext *ct, int step, const phx_galois_key *gk,
+                       phx_stream stream);
+/* keyswitch_inplace (include/evaluate.cuh:29-32, src/eval_key_switch.cu:95-182):
+ * ct[0..1] += keyswitch(c2) with c2 = [Ql][N] (NTT form) in device memory */
+int phx_keyswitch_inplace(const phx_context *ctx, phx_ciphertext *ct, const uint64_t *c2_dev,
+                          const phx_relin_key *key, int is_relin, phx_stream stream);
+
+/* ---- low-level RNS kernels (device pointers) ------------------------------- */
+/* nwt_2d_radix8_forward_inplace (include/ntt.cuh:173-226, src/ntt/fntt_2d.cu:620-653):
+ * limbs [0, L) of dev use QP table rows [start, start + L) */
+int phx_ntt_forward(const phx_context *ctx, uint64_t *dev, size_t L, size_t start, phx_stream stream);
+/* nwt_2d_radix8_backward_inplace (src/ntt/intt_2d.cu:724-757) */
+int phx_ntt_inverse(const phx_context *ctx, uint64_t *dev, size_t L, size_t start, phx_stream stream);
+/* the same transforms over npolys polynomials of L limbs each (poly_stride elements apart) in one launch;
+ * the reference's benchmark drives its NTT the same way over batch x limbs (benchmark/ntt_bench.cu:84-152) */
+int phx_ntt_forward_batch(const phx_context *ctx, uint64_t *dev, size_t L, size_t start, size_t npolys,
+                          size_t poly_stride, phx_stream stream);
+int phx_ntt_inverse_batch(const phx_context *ctx, uint64_t *dev, size_t L, size_t start, size_t npolys,
+                          size_t poly_stride, phx_stream stream);
+/* DRNSTool::modup (include/rns.cuh:156-163, src/rns_bconv.cu:530-628):
+ * c2 [Ql][N] NTT form -> t_mod_up [beta][QlP][N] */
+int phx_modup(const phx_context *ctx, size_t chain_index, const uint64_t *c2_dev, uint64_t *t_mod_up_dev,
+              phx_stream stream);
+/* key_switch_inner_prod (include/evaluate.cuh:25-27): t_mod_up -> cx [2][QlP][N] */
+int phx_key_inner_product(const phx_context *ctx, size_t chain_index, const uint64_t *t_mod_up_dev,
+                          const phx_relin_key *key, uint64_t *cx_dev, phx_stream stream);
+/* DRNSTool::moddown_from_NTT (src/rns_bconv.cu:776-828) on npolys polynomials of cx [npolys][QlP][N];
+ * out [npolys][Ql][N]; cx's special limbs are overwritten (scratch) */
+int phx_moddown_from_ntt(const phx_context *ctx, size_t chain_index, uint64_t *cx_dev, uint64_t *out_dev,
+                         size_t npolys, phx_stream stream);
+
+/* ---- device memory and timing helpers ------------------------------------- */
+int phx_malloc(size_t bytes, void **out);
+int phx_free(void *p);
+int phx_memcpy_htod(void *dst, const void *src, size_t bytes, phx_stream stream);
+int phx_memcpy_dtoh(void *dst, const void *src, size_t bytes, phx_stream stream);
+int phx_stream_synchronize(phx_stream stream);
+int phx_device_synchronize(void);
+int phx_stream_create(phx_stream *out);
+int phx_stream_destroy(phx_stream s);
+int phx_event_create(void **out);
+int phx_event_record(void *ev, phx_stream stream);
+int phx_event_elapsed_ms(void *start, void *stop, float *ms);
+int phx_event_destroy(void *ev);
+int phx_device_count(int *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHANTOM_AMD_H */

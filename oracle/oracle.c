@@ -1,0 +1,750 @@
+/*
+ * oracle.c -- CPU restatement of PhantomFHE's RNS polynomial hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke entry and the
+ * benchmark's CPU-baseline leg as the checker. The product never uses it.
+ *
+ * Each function restates the reference algorithm and cites the reference
+ * file:line it follows (paths relative to the reference repository root).
+ * Arithmetic is exact (unsigned __int128); every output is a canonical
+ * residue in [0, q), as the reference's final csub steps make it.
+ */
+#include "oracle.h"
+
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+typedef uint64_t u64;
+
+/* ------------------------------------------------------------------ */
+/* threading: limb-parallel loops                                      */
+/* ------------------------------------------------------------------ */
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+int orc_get_threads(void) { return g_threads; }
+
+typedef void (*orc_task_fn)(void *arg, size_t i);
+typedef struct {
+    orc_task_fn fn;
+    void *arg;
+    size_t n;
+    size_t next;
+    pthread_mutex_t mu;
+} orc_pool;
+
+static void *orc_worker(void *p) {
+    orc_pool *pool = (orc_pool *)p;
+    for (;;) {
+        pthread_mutex_lock(&pool->mu);
+        size_t i = pool->next++;
+        pthread_mutex_unlock(&pool->mu);
+        if (i >= pool->n) break;
+        pool->fn(pool->arg, i);
+    }
+    return NULL;
+}
+
+static void parallel_for(size_t n, orc_task_fn fn, void *arg) {
+    int nt = g_threads;
+    if (nt <= 1 || n <= 1) {
+        for (size_t i = 0; i < n; i++) fn(arg, i);
+        return;
+    }
+    if ((size_t)nt > n) nt = (int)n;
+    orc_pool pool = {fn, arg, n, 0, PTHREAD_MUTEX_INITIALIZER};
+    pthread_t th[256];
+    if (nt > 256) nt = 256;
+    for (int t = 0; t < nt; t++) pthread_create(&th[t], NULL, orc_worker, &pool);
+    for (int t = 0; t < nt; t++) pthread_join(th[t], NULL);
+}
+
+/* ------------------------------------------------------------------ */
+/* scalar modular arithmetic                                           */
+/* ------------------------------------------------------------------ */
+static inline u64 mulmod(u64 a, u64 b, u64 q) { return (u64)(((u128)a * b) % q); }
+static inline u64 addmod(u64 a, u64 b, u64 q) { u64 r = a + b; return r >= q ? r - q : r; }
+static inline u64 submod(u64 a, u64 b, u64 q) { return a >= b ? a - b : a + q - b; }
+
+static u64 powmod(u64 a, u64 e, u64 q) {
+    u64 r = 1 % q;
+    a %= q;
+    while (e) {
+        if (e & 1) r = mulmod(r, a, q);
+        a = mulmod(a, a, q);
+        e >>= 1;
+    }
+    return r;
+}
+
+/* inverse modulo a prime (every modulus on the path is prime) */
+static u64 invmod(u64 a, u64 q) { return powmod(a % q, q - 2, q); }
+
+/* compute_shoup: floor(w * 2^64 / q)  (include/host/uintarithsmallmod.h:119-124) */
+u64 orc_shoup(u64 w, u64 q) { return (u64)(((u128)w << 64) / q); }
+
+/* Harvey/Shoup lazy product in [0, 2q)  (include/uintmodmath.cuh:223-231) */
+static inline u64 shoup_lazy(u64 x, u64 w, u64 ws, u64 q) {
+    u64 hi = (u64)(((u128)x * ws) >> 64);
+    return x * w - hi * q;
+}
+/* csub_q  (include/uintmodmath.cuh:18-21) */
+static inline u64 csub(u64 x, u64 q) { return x >= q ? x - q : x; }
+
+/* ------------------------------------------------------------------ */
+/* primes and roots                                                    */
+/* ------------------------------------------------------------------ */
+
+/* Deterministic Miller-Rabin for 64-bit inputs. The reference runs 40 random
+ * bases (src/host/numth.cu:129-205); for n < 3.3e24 the fixed base set below
+ * decides primality exactly, so both agree on every 64-bit value. */
+int orc_is_prime(u64 n) {
+    static const u64 small[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    if (n < 2) return 0;
+    for (int i = 0; i < 12; i++) {
+        if (n == small[i]) return 1;
+        if (n % small[i] == 0) return 0;
+    }
+    u64 d = n - 1;
+    int r = 0;
+    while ((d & 1) == 0) { d >>= 1; r++; }
+    for (int i = 0; i < 12; i++) {
+        u64 x = powmod(small[i], d, n);
+        if (x == 1 || x == n - 1) continue;
+        int comp = 1;
+        for (int k = 1; k < r; k++) {
+            x = mulmod(x, x, n);
+            if (x == n - 1) { comp = 0; break; }
+        }
+        if (comp) return 0;
+    }
+    return 1;
+}
+
+/* get_primes: descending primes p = 1 (mod 2N) starting from 2^bits - 2N + 1
+ * and stepping by 2N while p > 2^(bits-1)   (src/host/numth.cu:207-233) */
+int orc_get_primes(u64 N, int bits, size_t count, u64 *out) {
+    u64 factor = 2 * N;
+    u64 value = ((u64)1 << bits) - factor + 1;
+    u64 lower = (u64)1 << (bits - 1);
+    size_t got = 0;
+    while (got < count && value > lower) {
+        if (orc_is_prime(value)) out[got++] = value;
+        value -= factor;
+    }
+    return got == count ? 0 : -1;
+}
+
+/* CoeffModulus::Create: for each bit size, the primes of that size are drawn
+ * from get_primes' descending list by taking .back() each time, so equal sizes
+ * come out in ascending order   (src/host/modulus.cu:80-111) */
+int orc_coeff_modulus_create(u64 N, const int *bits, size_t n, u64 *out) {
+    for (size_t i = 0; i < n; i++) {
+        size_t cnt = 0, rank = 0;
+        for (size_t j = 0; j < n; j++) {
+            if (bits[j] == bits[i]) {
+                cnt++;
+                if (j < i) rank++;
+            }
+        }
+        u64 *tmp = (u64 *)malloc(cnt * sizeof(u64));
+        if (orc_get_primes(N, bits[i], cnt, tmp) != 0) { free(tmp); return -1; }
+        out[i] = tmp[cnt - 1 - rank];
+        free(tmp);
+    }
+    return 0;
+}
+
+/* try_minimal_primitive_root: smallest primitive degree-th root of unity mod q.
+ * The reference starts from a random primitive root and scans all its odd
+ * powers keeping the minimum (src/host/numth.cu:309-331); the set of odd
+ * powers of any primitive root is the set of all primitive roots, so the
+ * minimum does not depend on the start. Here the start is found by a
+ * deterministic search instead of random_device. */
+u64 orc_minimal_root(u64 q, u64 degree) {
+    if ((q - 1) % degree != 0) return 0;
+    u64 quot = (q - 1) / degree;
+    u64 g = 0;
+    for (u64 x = 2; x < q; x++) {
+        u64 c = powmod(x, quot, q);
+        if (powmod(c, degree >> 1, q) == q - 1) { g = c; break; }
+    }
+    if (!g) return 0;
+    u64 g2 = mulmod(g, g, q), cur = g, best = g;
+    for (u64 i = 0; i < degree; i++) {
+        if (cur < best) best = cur;
+        cur = mulmod(cur, g2, q);
+    }
+    return best;
+}
+
+static inline u64 brv(u64 x, int bits) {
+    u64 r = 0;
+    for (int i = 0; i < bits; i++) { r = (r << 1) | (x & 1); x >>= 1; }
+    return r;
+}
+
+/* NTT::NTT tables (src/host/ntt.cu:11-56): root_powers[brv(i)] = psi^i,
+ * inv_root_powers[brv(i)] = psi^-i, entry [1] of the inverse table
+ * pre-multiplied by n^-1; Shoup companions for all. */
+void orc_ntt_tables(u64 q, int logN, u64 *rp, u64 *rps, u64 *irp, u64 *irps, u64 *ninv) {
+    u64 N = (u64)1 << logN;
+    u64 psi = orc_minimal_root(q, 2 * N);
+    u64 ipsi = invmod(psi, q);
+    u64 p = psi;
+    for (u64 i = 1; i < N; i++) {
+        rp[brv(i, logN)] = p;
+        p = mulmod(p, psi, q);
+    }
+    rp[0] = 1;
+    p = ipsi;
+    for (u64 i = 1; i < N; i++) {
+        irp[brv(i, logN)] = p;
+        p = mulmod(p, ipsi, q);
+    }
+    irp[0] = 1;
+    u64 ni = invmod(N % q, q);
+    irp[1] = mulmod(irp[1], ni, q);
+    for (u64 i = 0; i < N; i++) {
+        rps[i] = orc_shoup(rp[i], q);
+        irps[i] = orc_shoup(irp[i], q);
+    }
+    *ninv = ni;
+}
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+struct orc_ctx {
+    int logN;
+    size_t N, size_QP, size_P, size_Q;
+    u64 *q;                       /* [QP] */
+    u64 *rp, *rps, *irp, *irps;   /* [QP][N] */
+    u64 *ninv;                    /* [QP] */
+};
+
+orc_ctx *orc_ctx_create(int logN, const u64 *moduli, size_t size_QP, size_t size_P) {
+    orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+    c->logN = logN;
+    c->N = (size_t)1 << logN;
+    c->size_QP = size_QP;
+    c->size_P = size_P;
+    c->size_Q = size_QP - size_P;
+    c->q = (u64 *)malloc(size_QP * sizeof(u64));
+    memcpy(c->q, moduli, size_QP * sizeof(u64));
+    size_t tot = size_QP * c->N;
+    c->rp = (u64 *)malloc(tot * 8);
+    c->rps = (u64 *)malloc(tot * 8);
+    c->irp = (u64 *)malloc(tot * 8);
+    c->irps = (u64 *)malloc(tot * 8);
+    c->ninv = (u64 *)malloc(size_QP * 8);
+    for (size_t i = 0; i < size_QP; i++)
+        orc_ntt_tables(c->q[i], logN, c->rp + i * c->N, c->rps + i * c->N, c->irp + i * c->N,
+                       c->irps + i * c->N, c->ninv + i);
+    return c;
+}
+
+void orc_ctx_destroy(orc_ctx *c) {
+    if (!c) return;
+    free(c->q); free(c->rp); free(c->rps); free(c->irp); free(c->irps); free(c->ninv);
+    free(c);
+}
+
+const u64 *orc_ctx_root_powers(const orc_ctx *c, size_t i) { return c->rp + i * c->N; }
+const u64 *orc_ctx_inv_root_powers(const orc_ctx *c, size_t i) { return c->irp + i * c->N; }
+
+/* ------------------------------------------------------------------ */
+/* NTT / INTT of one limb                                              */
+/* ------------------------------------------------------------------ */
+
+/* Forward negacyclic NTT, natural order in, bit-reversed order out: CT
+ * butterflies with lazy [0,4q) ranges (include/butterfly.cuh:10-22), stage
+ * order of the radix-2 kernel (src/ntt/ntt_1d.cu:17-76), final csub 2q, q. */
+static void ntt_fwd_limb(u64 *a, size_t N, u64 q, const u64 *rp, const u64 *rps) {
+    u64 q2 = 2 * q;
+    for (size_t m = 1, t = N >> 1; m < N; m <<= 1, t >>= 1) {
+        for (size_t i = 0; i < m; i++) {
+            u64 w = rp[m + i], ws = rps[m + i];
+            u64 *x = a + 2 * i * t, *y = x + t;
+            for (size_t j = 0; j < t; j++) {
+                u64 tw_y = shoup_lazy(y[j], w, ws, q);
+                u64 xx = csub(x[j], q2);
+                y[j] = xx + q2 - tw_y;
+                x[j] = xx + tw_y;
+            }
+        }
+    }
+    for (size_t j = 0; j < N; j++) a[j] = csub(csub(a[j], q2), q);
+}
+
+/* Inverse: GS butterflies (include/butterfly.cuh:28-37) in the order of
+ * src/ntt/ntt_1d.cu:194-256; the last stage's twiddle carries n^-1 for the
+ * odd output, the even output is multiplied by n^-1 explicitly. */
+static void ntt_inv_limb(u64 *a, size_t N, u64 q, const u64 *irp, const u64 *irps, u64 ninv) {
+    u64 q2 = 2 * q;
+    u64 ninv_s = orc_shoup(ninv, q);
+    for (size_t m = N >> 1, t = 1; m >= 1; m >>= 1, t <<= 1) {
+        for (size_t i = 0; i < m; i++) {
+            u64 w = irp[m + i], ws = irps[m + i];
+            u64 *x = a + 2 * i * t, *y = x + t;
+            for (size_t j = 0; j < t; j++) {
+                u64 tt = x[j] + q2 - y[j];
+                u64 s = csub(x[j] + y[j], q2);
+                x[j] = s;
+                y[j] = shoup_lazy(tt, w, ws, q);
+            }
+        }
+        if (m == 1) break;
+    }
+    size_t h = N >> 1;
+    for (size_t j = 0; j < h; j++) {
+        u64 x = csub(a[j], q);
+        u64 y = csub(a[j + h], q);
+        a[j] = csub(shoup_lazy(x, ninv, ninv_s, q), q);
+        a[j + h] = y;
+    }
+}
+
+typedef struct {
+    const orc_ctx *c;
+    u64 *data;
+    size_t start;
+    int inverse;
+} ntt_job;
+
+static void ntt_task(void *arg, size_t i) {
+    ntt_job *j = (ntt_job *)arg;
+    const orc_ctx *c = j->c;
+    size_t p = j->start + i;
+    u64 *a = j->data + i * c->N;
+    if (j->inverse)
+        ntt_inv_limb(a, c->N, c->q[p], c->irp + p * c->N, c->irps + p * c->N, c->ninv[p]);
+    else
+        ntt_fwd_limb(a, c->N, c->q[p], c->rp + p * c->N, c->rps + p * c->N);
+}
+
+void orc_ntt_forward(const orc_ctx *c, u64 *data, size_t L, size_t start) {
+    ntt_job j = {c, data, start, 0};
+    parallel_for(L, ntt_task, &j);
+}
+
+void orc_ntt_inverse(const orc_ctx *c, u64 *data, size_t L, size_t start) {
+    ntt_job j = {c, data, start, 1};
+    parallel_for(L, ntt_task, &j);
+}
+
+/* limb transforms where each limb names its own QP table row */
+typedef struct {
+    const orc_ctx *c;
+    u64 *data;
+    const size_t *rows;
+    int inverse;
+} ntt_rows_job;
+
+static void ntt_rows_task(void *arg, size_t i) {
+    ntt_rows_job *j = (ntt_rows_job *)arg;
+    const orc_ctx *c = j->c;
+    size_t p = j->rows[i];
+    u64 *a = j->data + i * c->N;
+    if (j->inverse)
+        ntt_inv_limb(a, c->N, c->q[p], c->irp + p * c->N, c->irps + p * c->N, c->ninv[p]);
+    else
+        ntt_fwd_limb(a, c->N, c->q[p], c->rp + p * c->N, c->rps + p * c->N);
+}
+
+static void ntt_rows(const orc_ctx *c, u64 *data, const size_t *rows, size_t L, int inverse) {
+    ntt_rows_job j = {c, data, rows, inverse};
+    parallel_for(L, ntt_rows_task, &j);
+}
+
+/* ------------------------------------------------------------------ */
+/* RNS base conversion                                                 */
+/* ------------------------------------------------------------------ */
+
+/* punctured product prod_{k != skip} base[k] reduced mod m; equals the
+ * reference's multiprecision punctured product reduced by modulo_uint
+ * (src/host/rns.cu:296-311, 447-453) */
+static u64 punctured_mod(const u64 *base, size_t n, size_t skip, u64 m) {
+    u64 r = 1 % m;
+    for (size_t k = 0; k < n; k++)
+        if (k != skip) r = mulmod(r, base[k] % m, m);
+    return r;
+}
+
+static u64 prod_mod(const u64 *base, size_t n, u64 m) {
+    u64 r = 1 % m;
+    for (size_t k = 0; k < n; k++) r = mulmod(r, base[k] % m, m);
+    return r;
+}
+
+typedef struct {
+    size_t N, ni, no;
+    const u64 *ibase, *obase;
+    const u64 *y;         /* [ni][N], already multiplied by qHatInv */
+    const u64 *mat;       /* [no][ni] qiHat mod pj */
+    u64 *out;             /* [no][N] */
+    const size_t *out_rows; /* output limb index for each obase prime (NULL = identity) */
+} bconv_job;
+
+/* FastBConv phase 2: out_p = (sum_i y_i * [qHat_i]_p) mod p with a 128-bit
+ * accumulator and one final reduction (src/rns_bconv.cu:143-168, 455-485;
+ * include/rns_bconv.cuh:127-143). No alpha correction. */
+static void bconv_task(void *arg, size_t o) {
+    bconv_job *j = (bconv_job *)arg;
+    u64 p = j->obase[o];
+    const u64 *row = j->mat + o * j->ni;
+    u64 *dst = j->out + (j->out_rows ? j->out_rows[o] : o) * j->N;
+    for (size_t k = 0; k < j->N; k++) {
+        u128 acc = 0;
+        for (size_t i = 0; i < j->ni; i++) acc += (u128)j->y[i * j->N + k] * row[i];
+        dst[k] = (u64)(acc % p);
+    }
+}
+
+/* FastBConv (phase 1 + phase 2): y_i = [x_i * qHatInv_i]_{q_i}
+ * (src/rns_bconv.cu:40-60), then phase 2 as above; DBaseConverter::bConv_BEHZ
+ * src/rns_bconv.cu:212-229 */
+int orc_fastbconv(size_t N, const u64 *ibase, size_t ni, const u64 *obase, size_t no,
+                  const u64 *in, u64 *out) {
+    u64 *y = (u64 *)malloc(ni * N * 8);
+    u64 *mat = (u64 *)malloc(no * ni * 8);
+    for (size_t i = 0; i < ni; i++) {
+        u64 q = ibase[i];
+        u64 hinv = invmod(punctured_mod(ibase, ni, i, q), q);
+        for (size_t k = 0; k < N; k++) y[i * N + k] = mulmod(in[i * N + k], hinv, q);
+    }
+    for (size_t o = 0; o < no; o++)
+        for (size_t i = 0; i < ni; i++) mat[o * ni + i] = punctured_mod(ibase, ni, i, obase[o]);
+    bconv_job j = {N, ni, no, ibase, obase, y, mat, out, NULL};
+    parallel_for(no, bconv_task, &j);
+    free(y);
+    free(mat);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* hybrid key switching                                                */
+/* ------------------------------------------------------------------ */
+
+/* beta = ceil(size_Ql / alpha), alpha = size_P (src/rns.cu:152) */
+size_t orc_beta(const orc_ctx *c, size_t size_Ql) {
+    return (size_Ql + c->size_P - 1) / c->size_P;
+}
+
+/* QlP prime at position pos and the QP table row it uses: data limbs map to
+ * themselves, special limbs to size_Q + (pos - size_Ql)
+ * (src/ntt/fntt_2d.cu:434-436, src/eval_key_switch.cu:21) */
+static size_t qlp_row(const orc_ctx *c, size_t size_Ql, size_t pos) {
+    return pos < size_Ql ? pos : c->size_Q + (pos - size_Ql);
+}
+
+/* DRNSTool::modup for CKKS (src/rns_bconv.cu:530-628) with the constants of
+ * the hybrid-KS part of the DRNSTool ctor (src/rns.cu:136-190):
+ *  1. t_cks = INTT(c2) * partQlHatInv  (fused scale, src/ntt/ntt_modup.cu:356-393)
+ *  2. per digit j: own limbs copied in NTT form (modup_copy_partQl_kernel,
+ *     src/rns_bconv.cu:522-528); complement limbs = FastBConv phase 2 of the
+ *     digit, written at their QlP positions (bconv_matmul_padded, 455-485)
+ *  3. NTT of the complement limbs (exclude-range NTT, src/ntt/ntt_modup.cu:610-657) */
+int orc_modup(const orc_ctx *c, size_t size_Ql, const u64 *c2, u64 *t_mod_up) {
+    size_t N = c->N, alpha = c->size_P, size_QlP = size_Ql + c->size_P;
+    size_t beta = orc_beta(c, size_Ql);
+    u64 *qlp = (u64 *)malloc(size_QlP * 8);
+    size_t *rows = (size_t *)malloc(size_QlP * sizeof(size_t));
+    for (size_t p = 0; p < size_QlP; p++) {
+        rows[p] = qlp_row(c, size_Ql, p);
+        qlp[p] = c->q[rows[p]];
+    }
+    u64 *t_cks = (u64 *)malloc(size_Ql * N * 8);
+    memcpy(t_cks, c2, size_Ql * N * 8);
+    orc_ntt_inverse(c, t_cks, size_Ql, 0);
+    for (size_t j = 0; j < beta; j++) {
+        size_t st = alpha * j;
+        size_t sz = (j == beta - 1) ? size_Ql - alpha * (beta - 1) : alpha;
+        for (size_t i = st; i < st + sz; i++) {
+            u64 q = qlp[i];
+            u64 hinv = invmod(punctured_mod(qlp + st, sz, i - st, q), q);
+            for (size_t k = 0; k < N; k++) t_cks[i * N + k] = mulmod(t_cks[i * N + k], hinv, q);
+        }
+    }
+    for (size_t j = 0; j < beta; j++) {
+        size_t st = alpha * j;
+        size_t sz = (j == beta - 1) ? size_Ql - alpha * (beta - 1) : alpha;
+        u64 *dst = t_mod_up + j * size_QlP * N;
+        memcpy(dst + st * N, c2 + st * N, sz * N * 8);
+        size_t no = size_QlP - sz;
+        u64 *obase = (u64 *)malloc(no * 8);
+        size_t *orow = (size_t *)malloc(no * sizeof(size_t));
+        size_t *ntt_rows_v = (size_t *)malloc(no * sizeof(size_t));
+        for (size_t o = 0; o < no; o++) {
+            size_t pos = o < st ? o : o + sz;
+            obase[o] = qlp[pos];
+            orow[o] = pos;
+        }
+        u64 *mat = (u64 *)malloc(no * sz * 8);
+        for (size_t o = 0; o < no; o++)
+            for (size_t i = 0; i < sz; i++) mat[o * sz + i] = punctured_mod(qlp + st, sz, i, obase[o]);
+        bconv_job bj = {N, sz, no, qlp + st, obase, t_cks + st * N, mat, dst, orow};
+        parallel_for(no, bconv_task, &bj);
+        /* forward NTT of every complement limb with its own table row */
+        u64 *tmp = (u64 *)malloc(no * N * 8);
+        for (size_t o = 0; o < no; o++) {
+            memcpy(tmp + o * N, dst + orow[o] * N, N * 8);
+            ntt_rows_v[o] = rows[orow[o]];
+        }
+        ntt_rows(c, tmp, ntt_rows_v, no, 0);
+        for (size_t o = 0; o < no; o++) memcpy(dst + orow[o] * N, tmp + o * N, N * 8);
+        free(tmp); free(mat); free(obase); free(orow); free(ntt_rows_v);
+    }
+    free(t_cks); free(qlp); free(rows);
+    return 0;
+}
+
+typedef struct {
+    const orc_ctx *c;
+    size_t size_Ql, size_QlP, beta;
+    const u64 *t, *key;
+    u64 *cx;
+} ip_job;
+
+/* key_switch_inner_prod_c2_and_evk (src/eval_key_switch.cu:14-69):
+ * cx0[i] = sum_j t[j][i]*evk_j.b[row(i)], cx1[i] = sum_j t[j][i]*evk_j.a[row(i)],
+ * 128-bit accumulation (the mid-loop reduction never fires: the threshold
+ * computed at src/eval_key_switch.cu:156-157 is nonzero), one final reduction. */
+static void ip_task(void *arg, size_t pos) {
+    ip_job *j = (ip_job *)arg;
+    const orc_ctx *c = j->c;
+    size_t N = c->N, QPN = c->size_QP * N;
+    size_t row = qlp_row(c, j->size_Ql, pos);
+    u64 q = c->q[row];
+    for (size_t k = 0; k < N; k++) {
+        u128 a0 = 0, a1 = 0;
+        for (size_t b = 0; b < j->beta; b++) {
+            u64 x = j->t[b * j->size_QlP * N + pos * N + k];
+            const u64 *kb = j->key + b * 2 * QPN;
+            a0 += (u128)x * kb[row * N + k];
+            a1 += (u128)x * kb[QPN + row * N + k];
+        }
+        j->cx[pos * N + k] = (u64)(a0 % q);
+        j->cx[j->size_QlP * N + pos * N + k] = (u64)(a1 % q);
+    }
+}
+
+int orc_inner_product(const orc_ctx *c, size_t size_Ql, const u64 *t_mod_up, const u64 *key,
+                      size_t dnum, u64 *cx) {
+    size_t beta = orc_beta(c, size_Ql);
+    if (beta > dnum) return -1;
+    ip_job j = {c, size_Ql, size_Ql + c->size_P, beta, t_mod_up, key, cx};
+    parallel_for(size_Ql + c->size_P, ip_task, &j);
+    return 0;
+}
+
+/* DRNSTool::moddown_from_NTT for CKKS (src/rns_bconv.cu:776-828):
+ *  1. INTT of the P limbs of cx_i (include_special_mod INTT)
+ *  2. delta = FastBConv P -> Ql (bConv_BEHZ with base_P_to_Ql_conv, src/rns.cu:193-194)
+ *  3. out = (cx_i[Ql] - NTT(delta)) * P^-1 mod q_i
+ *     (fused phase 2, src/ntt/ntt_moddown.cu:106-216; sub_negate_const_mult
+ *      include/uintmodmath.cuh:234-242) */
+int orc_moddown(const orc_ctx *c, size_t size_Ql, const u64 *cx_i, u64 *out) {
+    size_t N = c->N, P = c->size_P;
+    u64 *xp = (u64 *)malloc(P * N * 8);
+    memcpy(xp, cx_i + size_Ql * N, P * N * 8);
+    orc_ntt_inverse(c, xp, P, c->size_Q);
+    u64 *delta = (u64 *)malloc(size_Ql * N * 8);
+    orc_fastbconv(N, c->q + c->size_Q, P, c->q, size_Ql, xp, delta);
+    orc_ntt_forward(c, delta, size_Ql, 0);
+    for (size_t i = 0; i < size_Ql; i++) {
+        u64 q = c->q[i];
+        u64 pinv = invmod(prod_mod(c->q + c->size_Q, P, q), q);
+        for (size_t k = 0; k < N; k++) {
+            u64 d = submod(cx_i[i * N + k], delta[i * N + k], q);
+            out[i * N + k] = mulmod(d, pinv, q);
+        }
+    }
+    free(xp);
+    free(delta);
+    return 0;
+}
+
+/* keyswitch_inplace core (src/eval_key_switch.cu:95-182): modup, inner
+ * product, moddown of both halves. out2 = [2][Ql][N], the values the
+ * reference adds into ct[0], ct[1] (add_to_ct_kernel). */
+int orc_keyswitch(const orc_ctx *c, size_t size_Ql, const u64 *c2, const u64 *key, size_t dnum,
+                  u64 *out2) {
+    size_t N = c->N, size_QlP = size_Ql + c->size_P;
+    size_t beta = orc_beta(c, size_Ql);
+    if (beta > dnum) return -1;
+    u64 *t = (u64 *)malloc(beta * size_QlP * N * 8);
+    u64 *cx = (u64 *)malloc(2 * size_QlP * N * 8);
+    orc_modup(c, size_Ql, c2, t);
+    orc_inner_product(c, size_Ql, t, key, dnum, cx);
+    orc_moddown(c, size_Ql, cx, out2);
+    orc_moddown(c, size_Ql, cx + size_QlP * N, out2 + size_Ql * N);
+    free(t);
+    free(cx);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* evaluator                                                           */
+/* ------------------------------------------------------------------ */
+
+/* tensor_prod_2x2_rns_poly (src/polymath.cu:463-498): d0 = c0 c0',
+ * d2 = c1 c1', d1 = (c0+c1)(c0'+c1') - d0 - d2 (= c0 c1' + c1 c0') */
+void orc_tensor(const orc_ctx *c, size_t size_Ql, const u64 *ct1, const u64 *ct2, u64 *out3) {
+    size_t N = c->N, LN = size_Ql * N;
+    for (size_t i = 0; i < size_Ql; i++) {
+        u64 q = c->q[i];
+        for (size_t k = 0; k < N; k++) {
+            size_t o = i * N + k;
+            u64 a0 = ct1[o], a1 = ct1[LN + o], b0 = ct2[o], b1 = ct2[LN + o];
+            u64 d0 = mulmod(a0, b0, q), d2 = mulmod(a1, b1, q);
+            u64 d1 = mulmod(addmod(a0, a1, q), addmod(b0, b1, q), q);
+            d1 = submod(submod(d1, d0, q), d2, q);
+            out3[o] = d0;
+            out3[LN + o] = d1;
+            out3[2 * LN + o] = d2;
+        }
+    }
+}
+
+/* multiply_and_relin_inplace for CKKS (src/evaluate.cu:1061-1104):
+ * bgv_ckks_multiply (345-397) then relinearize_inplace (1342-1374) */
+int orc_multiply_and_relin(const orc_ctx *c, size_t size_Ql, const u64 *ct1, const u64 *ct2,
+                           const u64 *key, size_t dnum, u64 *out2) {
+    size_t N = c->N, LN = size_Ql * N;
+    u64 *d = (u64 *)malloc(3 * LN * 8);
+    u64 *ks = (u64 *)malloc(2 * LN * 8);
+    orc_tensor(c, size_Ql, ct1, ct2, d);
+    int rc = orc_keyswitch(c, size_Ql, d + 2 * LN, key, dnum, ks);
+    if (rc == 0) {
+        for (size_t i = 0; i < size_Ql; i++) {
+            u64 q = c->q[i];
+            for (size_t k = 0; k < N; k++) {
+                size_t o = i * N + k;
+                out2[o] = addmod(d[o], ks[o], q);
+                out2[LN + o] = addmod(d[LN + o], ks[LN + o], q);
+            }
+        }
+    }
+    free(d);
+    free(ks);
+    return rc;
+}
+
+/* rescale_to_next -> DRNSTool::divide_and_round_q_last_ntt
+ * (src/rns.cu:1128-1184): INTT of the last limb, reduce it mod every other
+ * q_j, NTT, then (c_j - NTT([c_last]_qj)) * q_last^-1 -- a floor division
+ * (no +q_last/2 term). */
+int orc_rescale(const orc_ctx *c, size_t size_Ql, const u64 *ct, size_t ct_size, u64 *out) {
+    size_t N = c->N, L1 = size_Ql - 1;
+    if (size_Ql < 2) return -1;
+    u64 *last = (u64 *)malloc(N * 8);
+    u64 *tmp = (u64 *)malloc(L1 * N * 8);
+    u64 ql = c->q[L1];
+    for (size_t s = 0; s < ct_size; s++) {
+        const u64 *ci = ct + s * size_Ql * N;
+        u64 *co = out + s * L1 * N;
+        memcpy(last, ci + L1 * N, N * 8);
+        orc_ntt_inverse(c, last, 1, L1);
+        for (size_t j = 0; j < L1; j++)
+            for (size_t k = 0; k < N; k++) tmp[j * N + k] = last[k] % c->q[j];
+        orc_ntt_forward(c, tmp, L1, 0);
+        for (size_t j = 0; j < L1; j++) {
+            u64 q = c->q[j];
+            u64 inv = invmod(ql % q, q);
+            for (size_t k = 0; k < N; k++)
+                co[j * N + k] = mulmod(submod(ci[j * N + k], tmp[j * N + k], q), inv, q);
+        }
+    }
+    free(last);
+    free(tmp);
+    return 0;
+}
+
+/* get_elt_from_step (include/galois.cuh:16-49): 5^step mod 2N, negative
+ * steps map to N/2 - |step|, step 0 to 2N - 1 (conjugation) */
+uint32_t orc_galois_elt_from_step(int step, size_t N) {
+    uint64_t m = 2 * (uint64_t)N;
+    if (step == 0) return (uint32_t)(m - 1);
+    int neg = step < 0;
+    uint32_t ps = (uint32_t)(neg ? -step : step);
+    if (ps >= N / 2) return 0;
+    int e = neg ? (int)(N / 2) - (int)ps : (int)ps;
+    uint64_t g = 1;
+    while (e--) g = (g * 5) & (m - 1);
+    return (uint32_t)g;
+}
+
+/* NTT-domain permutation table (include/galois.cuh:98-113) */
+void orc_galois_perm_table(int logN, uint32_t elt, uint32_t *out) {
+    size_t N = (size_t)1 << logN;
+    for (size_t i = N; i < 2 * N; i++) {
+        uint64_t r = brv(i, logN + 1);
+        uint64_t idx = ((uint64_t)elt * r) >> 1;
+        idx &= (N - 1);
+        out[i - N] = (uint32_t)brv(idx, logN);
+    }
+}
+
+/* apply_galois_inplace for CKKS (src/evaluate.cu:1567-1630): c0 <- pi(c0),
+ * t <- pi(c1), c1 <- 0, then keyswitch(t) added to (c0, c1). */
+int orc_apply_galois(const orc_ctx *c, size_t size_Ql, const u64 *ct, uint32_t elt,
+                     const u64 *key, size_t dnum, u64 *out2) {
+    size_t N = c->N, LN = size_Ql * N;
+    uint32_t *perm = (uint32_t *)malloc(N * 4);
+    orc_galois_perm_table(c->logN, elt, perm);
+    u64 *t = (u64 *)malloc(LN * 8);
+    u64 *ks = (u64 *)malloc(2 * LN * 8);
+    for (size_t i = 0; i < size_Ql; i++)
+        for (size_t k = 0; k < N; k++) {
+            out2[i * N + k] = ct[i * N + perm[k]];
+            t[i * N + k] = ct[LN + i * N + perm[k]];
+        }
+    int rc = orc_keyswitch(c, size_Ql, t, key, dnum, ks);
+    if (rc == 0) {
+        for (size_t i = 0; i < size_Ql; i++) {
+            u64 q = c->q[i];
+            for (size_t k = 0; k < N; k++) {
+                out2[i * N + k] = addmod(out2[i * N + k], ks[i * N + k], q);
+                out2[LN + i * N + k] = ks[LN + i * N + k];
+            }
+        }
+    }
+    free(perm); free(t); free(ks);
+    return rc;
+}
+
+/* generate_one_kswitch_key (src/secretkey.cu:297-341) with the digit count
+ * generalised to ceil(size_Q / alpha): digit j = (b_j, a_j) at the key level,
+ * b_j = -(a_j s + e_j) + P s2 on the limbs of digit j
+ * (encrypt_zero_symmetric src/secretkey.cu:232-289;
+ *  multiply_temp_mod_and_add_rns_poly src/polymath.cu:318-338). */
+int orc_make_kswitch_key(const orc_ctx *c, const u64 *s_ntt, const u64 *s2_ntt, const u64 *a,
+                         const u64 *e, size_t dnum, u64 *key) {
+    size_t N = c->N, QP = c->size_QP, QPN = QP * N, alpha = c->size_P;
+    u64 *en = (u64 *)malloc(QPN * 8);
+    for (size_t j = 0; j < dnum; j++) {
+        memcpy(en, e + j * QPN, QPN * 8);
+        orc_ntt_forward(c, en, QP, 0);
+        u64 *b = key + j * 2 * QPN;
+        u64 *ak = b + QPN;
+        const u64 *aj = a + j * QPN;
+        size_t st = alpha * j, en_idx = st + alpha;
+        if (en_idx > c->size_Q) en_idx = c->size_Q;
+        for (size_t i = 0; i < QP; i++) {
+            u64 q = c->q[i];
+            u64 pmod = (i >= st && i < en_idx) ? prod_mod(c->q + c->size_Q, alpha, q) : 0;
+            for (size_t k = 0; k < N; k++) {
+                size_t o = i * N + k;
+                u64 v = addmod(mulmod(aj[o], s_ntt[o], q), en[o], q);
+                v = v ? q - v : 0;
+                if (pmod) v = addmod(v, mulmod(s2_ntt[o], pmod, q), q);
+                b[o] = v;
+                ak[o] = aj[o];
+            }
+        }
+    }
+    free(en);
+    return 0;
+}

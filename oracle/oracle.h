@@ -1,0 +1,86 @@
+/*
+ * oracle.h -- CPU restatement of PhantomFHE's RNS polynomial hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: tests/, the smoke
+ * entry and the benchmark's CPU-baseline leg load it; the product library
+ * never links, loads or calls it.
+ *
+ * Every routine restates the reference algorithm step for step, with exact
+ * 128-bit integer arithmetic, and cites the reference file:line it follows.
+ * All outputs are canonical residues in [0, q).
+ *
+ * Layouts (identical to the reference):
+ *   polynomial limb set  [limb][N]            (u64)
+ *   ciphertext           [poly][limb][N]
+ *   relin / Galois key   [digit][2][QP][N]    (b part, then a part; NTT form)
+ *   t_mod_up             [digit][QlP][N]
+ */
+#ifndef PHANTOM_ORACLE_H
+#define PHANTOM_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ctx orc_ctx;
+
+/* thread count used by the limb-parallel loops (default 1) */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
+/* ---- host parameter maths ---- */
+int orc_is_prime(uint64_t v);
+int orc_get_primes(uint64_t N, int bits, size_t count, uint64_t *out);
+int orc_coeff_modulus_create(uint64_t N, const int *bits, size_t n, uint64_t *out);
+uint64_t orc_minimal_root(uint64_t q, uint64_t degree);
+uint64_t orc_shoup(uint64_t w, uint64_t q);
+void orc_ntt_tables(uint64_t q, int logN, uint64_t *rp, uint64_t *rps,
+                    uint64_t *irp, uint64_t *irps, uint64_t *ninv);
+
+/* ---- context (QP primes: size_Q data primes, then size_P special primes) ---- */
+orc_ctx *orc_ctx_create(int logN, const uint64_t *moduli, size_t size_QP, size_t size_P);
+void orc_ctx_destroy(orc_ctx *c);
+const uint64_t *orc_ctx_root_powers(const orc_ctx *c, size_t prime_idx);
+const uint64_t *orc_ctx_inv_root_powers(const orc_ctx *c, size_t prime_idx);
+
+/* ---- transforms on limbs [start, start+L) of the QP chain ---- */
+void orc_ntt_forward(const orc_ctx *c, uint64_t *data, size_t L, size_t start);
+void orc_ntt_inverse(const orc_ctx *c, uint64_t *data, size_t L, size_t start);
+
+/* ---- RNS base conversion ---- */
+int orc_fastbconv(size_t N, const uint64_t *ibase, size_t ni, const uint64_t *obase, size_t no,
+                  const uint64_t *in, uint64_t *out);
+
+/* ---- hybrid key switching (CKKS, NTT domain), level with size_Ql data limbs ---- */
+size_t orc_beta(const orc_ctx *c, size_t size_Ql);
+int orc_modup(const orc_ctx *c, size_t size_Ql, const uint64_t *c2, uint64_t *t_mod_up);
+int orc_inner_product(const orc_ctx *c, size_t size_Ql, const uint64_t *t_mod_up,
+                      const uint64_t *key, size_t dnum, uint64_t *cx);
+int orc_moddown(const orc_ctx *c, size_t size_Ql, const uint64_t *cx_i, uint64_t *out);
+int orc_keyswitch(const orc_ctx *c, size_t size_Ql, const uint64_t *c2, const uint64_t *key,
+                  size_t dnum, uint64_t *out2);
+
+/* ---- evaluator ---- */
+void orc_tensor(const orc_ctx *c, size_t size_Ql, const uint64_t *ct1, const uint64_t *ct2,
+                uint64_t *out3);
+int orc_multiply_and_relin(const orc_ctx *c, size_t size_Ql, const uint64_t *ct1,
+                           const uint64_t *ct2, const uint64_t *key, size_t dnum, uint64_t *out2);
+int orc_rescale(const orc_ctx *c, size_t size_Ql, const uint64_t *ct, size_t ct_size,
+                uint64_t *out);
+uint32_t orc_galois_elt_from_step(int step, size_t N);
+void orc_galois_perm_table(int logN, uint32_t elt, uint32_t *out);
+int orc_apply_galois(const orc_ctx *c, size_t size_Ql, const uint64_t *ct, uint32_t elt,
+                     const uint64_t *key, size_t dnum, uint64_t *out2);
+
+/* key material from explicit samples: s_ntt, s2_ntt [QP][N] (NTT), a [dnum][QP][N] (NTT),
+ * e [dnum][QP][N] (coefficient domain, canonical residues) */
+int orc_make_kswitch_key(const orc_ctx *c, const uint64_t *s_ntt, const uint64_t *s2_ntt,
+                         const uint64_t *a, const uint64_t *e, size_t dnum, uint64_t *key);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,617 @@
+// capi.cpp -- the extern "C" boundary (include/phantom_amd.h).
+#include "../../include/phantom_amd.h"
+
+#include <cstring>
+#include <string>
+
+#include "evaluator.h"
+
+using namespace phx;
+
+struct phx_context {
+    Context *ctx;
+};
+struct phx_ciphertext {
+    Ciphertext ct;
+};
+struct phx_secret_key {
+    SecretKey sk;
+};
+struct phx_relin_key {
+    KSwitchKey key;
+};
+struct phx_galois_key {
+    GaloisKeys gk;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guard(F &&f) {
+    try {
+        f();
+        return PHX_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::invalid_argument &e) {
+        g_last_error = e.what();
+        return PHX_EINVAL;
+    } catch (const std::logic_error &e) {
+        g_last_error = e.what();
+        return PHX_ELOGIC;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return PHX_ERUNTIME;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return PHX_ERUNTIME;
+    }
+}
+
+inline hipStream_t S(phx_stream s) { return s ? static_cast<hipStream_t>(s) : hipStreamPerThread; }
+inline void nn(const void *p, const char *what) {
+    if (!p) fail(kInvalidArgument, std::string(what) + " must not be null");
+}
+}  // namespace
+
+extern "C" {
+
+const char *phx_last_error(void) { return g_last_error.c_str(); }
+
+int phx_coeff_modulus_create(uint64_t N, const int *bits, size_t n, uint64_t *out) {
+    return guard([&] {
+        nn(bits, "bit_sizes");
+        nn(out, "out");
+        require(N >= 4096 && N <= 65536 && (N & (N - 1)) == 0, "poly_modulus_degree is invalid");
+        auto v = hm::coeff_modulus_create(N, std::vector<int>(bits, bits + n));
+        std::memcpy(out, v.data(), n * 8);
+    });
+}
+
+int phx_galois_elt_from_step(int step, uint64_t N, uint32_t *out) {
+    return guard([&] {
+        nn(out, "out");
+        *out = galois_elt_from_step(step, N);
+    });
+}
+
+int phx_context_create(const phx_params *p, phx_context **out) {
+    return guard([&] {
+        nn(p, "params");
+        nn(out, "out");
+        nn(p->coeff_modulus, "coeff_modulus");
+        Params q;
+        q.scheme = p->scheme;
+        q.mul_tech = p->mul_tech;
+        q.poly_degree = p->poly_modulus_degree;
+        q.coeff_modulus.assign(p->coeff_modulus, p->coeff_modulus + p->coeff_modulus_size);
+        q.special_modulus_size = p->special_modulus_size;
+        q.plain_modulus = p->plain_modulus;
+        if (p->galois_elts) q.galois_elts.assign(p->galois_elts, p->galois_elts + p->galois_elts_size);
+        require(q.scheme == PHX_SCHEME_CKKS || q.scheme == PHX_SCHEME_BFV || q.scheme == PHX_SCHEME_BGV,
+                "unsupported scheme");
+        auto *h = new phx_context{new Context(q)};
+        *out = h;
+    });
+}
+
+int phx_context_destroy(phx_context *ctx) {
+    return guard([&] {
+        if (!ctx) return;
+        delete ctx->ctx;
+        delete ctx;
+    });
+}
+
+int phx_context_info(const phx_context *ctx, uint64_t *N, size_t *sQ, size_t *sP, size_t *dnum) {
+    return guard([&] {
+        nn(ctx, "context");
+        if (N) *N = ctx->ctx->N();
+        if (sQ) *sQ = ctx->ctx->size_Q();
+        if (sP) *sP = ctx->ctx->size_P();
+        if (dnum) *dnum = ctx->ctx->dnum();
+    });
+}
+
+int phx_context_limbs_at(const phx_context *ctx, size_t c, size_t *limbs) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(limbs, "limbs");
+        require(c <= ctx->ctx->size_Q(), "invalid chain index");
+        *limbs = ctx->ctx->limbs_at((u32)c);
+    });
+}
+
+int phx_ciphertext_create(phx_ciphertext **out) {
+    return guard([&] {
+        nn(out, "out");
+        *out = new phx_ciphertext{};
+    });
+}
+
+int phx_ciphertext_destroy(phx_ciphertext *ct) {
+    return guard([&] {
+        if (!ct) return;
+        ct->ct.release(hipStreamPerThread);
+        delete ct;
+    });
+}
+
+int phx_ciphertext_resize(const phx_context *ctx, phx_ciphertext *ct, size_t chain, size_t size, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct, "ciphertext");
+        require(chain <= ctx->ctx->size_Q(), "invalid chain index");
+        require(size >= 1 && size <= 3, "ciphertext size must be 1..3");
+        ct->ct.resize(*ctx->ctx, (u32)chain, (u32)size, S(s));
+    });
+}
+
+int phx_ciphertext_upload(phx_ciphertext *ct, const uint64_t *host, size_t n, phx_stream s) {
+    return guard([&] {
+        nn(ct, "ciphertext");
+        nn(host, "host");
+        require(n == (size_t)ct->ct.size * ct->ct.coeff_modulus_size * ct->ct.poly_degree, "size mismatch");
+        PHX_HIP_CHECK(hipMemcpyAsync(ct->ct.data, host, n * 8, hipMemcpyHostToDevice, S(s)));
+        PHX_HIP_CHECK(hipStreamSynchronize(S(s)));
+    });
+}
+
+int phx_ciphertext_download(const phx_ciphertext *ct, uint64_t *host, size_t n, phx_stream s) {
+    return guard([&] {
+        nn(ct, "ciphertext");
+        nn(host, "host");
+        require(n == (size_t)ct->ct.size * ct->ct.coeff_modulus_size * ct->ct.poly_degree, "size mismatch");
+        PHX_HIP_CHECK(hipMemcpyAsync(host, ct->ct.data, n * 8, hipMemcpyDeviceToHost, S(s)));
+        PHX_HIP_CHECK(hipStreamSynchronize(S(s)));
+    });
+}
+
+int phx_ciphertext_copy(const phx_context *ctx, phx_ciphertext *dst, const phx_ciphertext *src, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(dst, "dst");
+        nn(src, "src");
+        if (dst == src) return;
+        const Ciphertext &a = src->ct;
+        dst->ct.resize(*ctx->ctx, a.chain_index, a.size, S(s));
+        PHX_HIP_CHECK(hipMemcpyAsync(dst->ct.data, a.data, (size_t)a.size * a.coeff_modulus_size * a.poly_degree * 8,
+                                     hipMemcpyDeviceToDevice, S(s)));
+        dst->ct.scale = a.scale;
+        dst->ct.is_ntt_form = a.is_ntt_form;
+        dst->ct.correction_factor = a.correction_factor;
+    });
+}
+
+int phx_ciphertext_info(const phx_ciphertext *ct, size_t *chain, size_t *size, size_t *L, double *scale, int *ntt) {
+    return guard([&] {
+        nn(ct, "ciphertext");
+        if (chain) *chain = ct->ct.chain_index;
+        if (size) *size = ct->ct.size;
+        if (L) *L = ct->ct.coeff_modulus_size;
+        if (scale) *scale = ct->ct.scale;
+        if (ntt) *ntt = ct->ct.is_ntt_form ? 1 : 0;
+    });
+}
+
+int phx_ciphertext_set_scale(phx_ciphertext *ct, double scale) {
+    return guard([&] {
+        nn(ct, "ciphertext");
+        ct->ct.scale = scale;
+    });
+}
+
+int phx_ciphertext_data(const phx_ciphertext *ct, uint64_t **p) {
+    return guard([&] {
+        nn(ct, "ciphertext");
+        nn(p, "out");
+        *p = ct->ct.data;
+    });
+}
+
+int phx_secret_key_create(const phx_context *ctx, uint64_t seed, phx_secret_key **out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(out, "out");
+        auto *k = new phx_secret_key{};
+        secret_key_generate(*ctx->ctx, k->sk, seed, hipStreamPerThread);
+        PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+        *out = k;
+    });
+}
+
+int phx_secret_key_create_from_coeffs(const phx_context *ctx, const int8_t *coeffs, phx_secret_key **out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(coeffs, "coeffs");
+        nn(out, "out");
+        auto *k = new phx_secret_key{};
+        secret_key_upload(*ctx->ctx, k->sk, coeffs, hipStreamPerThread);
+        PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+        *out = k;
+    });
+}
+
+int phx_secret_key_download(const phx_context *ctx, const phx_secret_key *sk, uint64_t *host) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(sk, "secret key");
+        nn(host, "host");
+        PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+        PHX_HIP_CHECK(hipMemcpy(host, sk->sk.s, (size_t)ctx->ctx->size_QP() * ctx->ctx->N() * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int phx_secret_key_destroy(phx_secret_key *sk) {
+    return guard([&] {
+        if (!sk) return;
+        dev_free(sk->sk.s, hipStreamPerThread);
+        dev_free(sk->sk.s2, hipStreamPerThread);
+        delete sk;
+    });
+}
+
+int phx_relin_key_create(const phx_context *ctx, phx_secret_key *sk, uint64_t seed, phx_relin_key **out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(sk, "secret key");
+        nn(out, "out");
+        auto *k = new phx_relin_key{};
+        relin_key_generate(*ctx->ctx, sk->sk, k->key, seed, hipStreamPerThread);
+        PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+        *out = k;
+    });
+}
+
+int phx_relin_key_create_from_host(const phx_context *ctx, const uint64_t *host, size_t dnum, phx_relin_key **out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(host, "host");
+        nn(out, "out");
+        require(dnum >= 1, "dnum must be positive");
+        const size_t n = dnum * 2 * (size_t)ctx->ctx->size_QP() * ctx->ctx->N();
+        auto *k = new phx_relin_key{};
+        k->key.dnum = (u32)dnum;
+        k->key.data = static_cast<u64 *>(dev_alloc(n * 8, hipStreamPerThread));
+        PHX_HIP_CHECK(hipMemcpyAsync(k->key.data, host, n * 8, hipMemcpyHostToDevice, hipStreamPerThread));
+        PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+        *out = k;
+    });
+}
+
+int phx_relin_key_download(const phx_context *ctx, const phx_relin_key *key, uint64_t *host) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(key, "key");
+        nn(host, "host");
+        const size_t n = (size_t)key->key.dnum * 2 * ctx->ctx->size_QP() * ctx->ctx->N();
+        PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+        PHX_HIP_CHECK(hipMemcpy(host, key->key.data, n * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int phx_relin_key_destroy(phx_relin_key *key) {
+    return guard([&] {
+        if (!key) return;
+        dev_free(key->key.data, hipStreamPerThread);
+        delete key;
+    });
+}
+
+int phx_galois_key_create(const phx_context *ctx, const phx_secret_key *sk, const uint32_t *elts, size_t n,
+                          uint64_t seed, phx_galois_key **out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(sk, "secret key");
+        nn(elts, "elts");
+        nn(out, "out");
+        auto *k = new phx_galois_key{};
+        for (size_t i = 0; i < n; ++i) {
+            KSwitchKey key;
+            galois_key_generate(*ctx->ctx, sk->sk, elts[i], key, seed + 0x1000 * (i + 1), hipStreamPerThread);
+            k->gk.elts.push_back(elts[i]);
+            k->gk.keys.push_back(key);
+        }
+        PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+        *out = k;
+    });
+}
+
+int phx_galois_key_create_from_host(const phx_context *ctx, const uint32_t *elts, size_t n, const uint64_t *host,
+                                    size_t dnum, phx_galois_key **out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(elts, "elts");
+        nn(host, "host");
+        nn(out, "out");
+        const size_t per = dnum * 2 * (size_t)ctx->ctx->size_QP() * ctx->ctx->N();
+        auto *k = new phx_galois_key{};
+        for (size_t i = 0; i < n; ++i) {
+            KSwitchKey key;
+            key.dnum = (u32)dnum;
+            key.data = static_cast<u64 *>(dev_alloc(per * 8, hipStreamPerThread));
+            PHX_HIP_CHECK(hipMemcpyAsync(key.data, host + i * per, per * 8, hipMemcpyHostToDevice, hipStreamPerThread));
+            k->gk.elts.push_back(elts[i]);
+            k->gk.keys.push_back(key);
+        }
+        PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+        *out = k;
+    });
+}
+
+int phx_galois_key_destroy(phx_galois_key *key) {
+    return guard([&] {
+        if (!key) return;
+        for (auto &k : key->gk.keys) dev_free(k.data, hipStreamPerThread);
+        delete key;
+    });
+}
+
+int phx_encrypt_symmetric(const phx_context *ctx, const phx_secret_key *sk, const uint64_t *plain, size_t chain,
+                          uint64_t seed, phx_ciphertext *out, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(sk, "secret key");
+        nn(plain, "plain");
+        nn(out, "out");
+        const Context &c = *ctx->ctx;
+        require(chain >= 1 && chain <= c.size_Q(), "invalid chain index");
+        const size_t n = (size_t)c.limbs_at((u32)chain) * c.N();
+        u64 *d = static_cast<u64 *>(dev_alloc(n * 8, S(s)));
+        PHX_HIP_CHECK(hipMemcpyAsync(d, plain, n * 8, hipMemcpyHostToDevice, S(s)));
+        encrypt_symmetric(c, sk->sk, d, (u32)chain, out->ct, seed, S(s));
+        dev_free(d, S(s));
+        PHX_HIP_CHECK(hipStreamSynchronize(S(s)));
+    });
+}
+
+int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext *ct, uint64_t *out, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(sk, "secret key");
+        nn(ct, "ciphertext");
+        nn(out, "out");
+        const Context &c = *ctx->ctx;
+        const size_t n = (size_t)ct->ct.coeff_modulus_size * c.N();
+        u64 *d = static_cast<u64 *>(dev_alloc(n * 8, S(s)));
+        decrypt(c, sk->sk, ct->ct, d, S(s));
+        PHX_HIP_CHECK(hipMemcpyAsync(out, d, n * 8, hipMemcpyDeviceToHost, S(s)));
+        dev_free(d, S(s));
+        PHX_HIP_CHECK(hipStreamSynchronize(S(s)));
+    });
+}
+
+int phx_multiply_and_relin_inplace(const phx_context *ctx, phx_ciphertext *ct1, const phx_ciphertext *ct2,
+                                   const phx_relin_key *rlk, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct1, "encrypted1");
+        nn(ct2, "encrypted2");
+        nn(rlk, "relin_keys");
+        multiply_and_relin(*ctx->ctx, ct1->ct, ct2->ct, rlk->key, S(s));
+    });
+}
+
+int phx_multiply_inplace(const phx_context *ctx, phx_ciphertext *ct1, const phx_ciphertext *ct2, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct1, "encrypted1");
+        nn(ct2, "encrypted2");
+        Ciphertext &a = ct1->ct;
+        const Ciphertext &b = ct2->ct;
+        require(a.chain_index == b.chain_index, "encrypted1 and encrypted2 parameter mismatch");
+        require(a.is_ntt_form && b.is_ntt_form, "encrypted1 and encrypted2 must be in NTT form");
+        require(a.size == 2 && b.size == 2, "only size-2 operands are supported");
+        const u64 LN = (u64)a.coeff_modulus_size * ctx->ctx->N();
+        const double sb = b.scale;
+        // keep ct2 readable when it aliases ct1: the tensor reads both operands
+        // before writing d0/d1 at the same index.
+        a.resize(*ctx->ctx, a.chain_index, 3, S(s));
+        const u64 *bd = (ct1 == ct2) ? a.data : b.data;
+        tensor_2x2(*ctx->ctx, a.coeff_modulus_size, a.data, bd, a.data, a.data + LN, a.data + 2 * LN, S(s));
+        a.scale *= sb;
+    });
+}
+
+int phx_relinearize_inplace(const phx_context *ctx, phx_ciphertext *ct, const phx_relin_key *rlk, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct, "encrypted");
+        nn(rlk, "relin_keys");
+        relinearize(*ctx->ctx, ct->ct, rlk->key, S(s));
+    });
+}
+
+int phx_rescale_to_next(const phx_context *ctx, const phx_ciphertext *in, phx_ciphertext *out, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(in, "encrypted");
+        nn(out, "destination");
+        if (in == out) {
+            Ciphertext tmp = in->ct;
+            Ciphertext res;
+            rescale_to_next(*ctx->ctx, tmp, res, S(s));
+            out->ct.release(S(s));
+            out->ct = res;
+        } else {
+            rescale_to_next(*ctx->ctx, in->ct, out->ct, S(s));
+        }
+    });
+}
+
+int phx_apply_galois_inplace(const phx_context *ctx, phx_ciphertext *ct, uint32_t elt, const phx_galois_key *gk,
+                             phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct, "encrypted");
+        nn(gk, "galois_keys");
+        apply_galois(*ctx->ctx, ct->ct, elt, gk->gk, S(s));
+    });
+}
+
+int phx_rotate_inplace(const phx_context *ctx, phx_ciphertext *ct, int step, const phx_galois_key *gk, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct, "encrypted");
+        nn(gk, "galois_keys");
+        rotate(*ctx->ctx, ct->ct, step, gk->gk, S(s));
+    });
+}
+
+int phx_keyswitch_inplace(const phx_context *ctx, phx_ciphertext *ct, const uint64_t *c2, const phx_relin_key *key,
+                          int /*is_relin*/, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct, "encrypted");
+        nn(c2, "c2");
+        nn(key, "key");
+        require(ct->ct.size >= 2, "ciphertext must have two polynomials");
+        const u64 LN = (u64)ct->ct.coeff_modulus_size * ctx->ctx->N();
+        keyswitch(*ctx->ctx, ct->ct.chain_index, c2, key->key, ct->ct.data, ct->ct.data, ct->ct.data + LN, S(s));
+    });
+}
+
+int phx_ntt_forward(const phx_context *ctx, uint64_t *dev, size_t L, size_t start, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(dev, "data");
+        require(start + L <= ctx->ctx->size_QP(), "limb range exceeds the modulus chain");
+        ntt_forward(ctx->ctx->tables(), limb_map(dev, (u32)start, (u32)L), (u32)L, 1, S(s));
+    });
+}
+
+int phx_ntt_inverse(const phx_context *ctx, uint64_t *dev, size_t L, size_t start, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(dev, "data");
+        require(start + L <= ctx->ctx->size_QP(), "limb range exceeds the modulus chain");
+        ntt_inverse(ctx->ctx->tables(), limb_map(dev, (u32)start, (u32)L), (u32)L, 1, S(s));
+    });
+}
+
+int phx_ntt_forward_batch(const phx_context *ctx, uint64_t *dev, size_t L, size_t start, size_t npolys,
+                          size_t poly_stride, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(dev, "data");
+        require(start + L <= ctx->ctx->size_QP(), "limb range exceeds the modulus chain");
+        require(npolys <= 65535, "too many polynomials in one launch");
+        ntt_forward(ctx->ctx->tables(), limb_map(dev, (u32)start, (u32)L, poly_stride), (u32)L, (u32)npolys, S(s));
+    });
+}
+
+int phx_ntt_inverse_batch(const phx_context *ctx, uint64_t *dev, size_t L, size_t start, size_t npolys,
+                          size_t poly_stride, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(dev, "data");
+        require(start + L <= ctx->ctx->size_QP(), "limb range exceeds the modulus chain");
+        require(npolys <= 65535, "too many polynomials in one launch");
+        ntt_inverse(ctx->ctx->tables(), limb_map(dev, (u32)start, (u32)L, poly_stride), (u32)L, (u32)npolys, S(s));
+    });
+}
+
+int phx_modup(const phx_context *ctx, size_t chain, const uint64_t *c2, uint64_t *t, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(c2, "c2");
+        nn(t, "t_mod_up");
+        modup(*ctx->ctx, (u32)chain, c2, t, S(s));
+    });
+}
+
+int phx_key_inner_product(const phx_context *ctx, size_t chain, const uint64_t *t, const phx_relin_key *key,
+                          uint64_t *cx, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(t, "t_mod_up");
+        nn(key, "key");
+        nn(cx, "cx");
+        inner_product(*ctx->ctx, (u32)chain, t, key->key, cx, S(s));
+    });
+}
+
+int phx_moddown_from_ntt(const phx_context *ctx, size_t chain, uint64_t *cx, uint64_t *out, size_t npolys,
+                         phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(cx, "cx");
+        nn(out, "out");
+        require(npolys >= 1 && npolys <= 2, "npolys must be 1 or 2");
+        moddown(*ctx->ctx, (u32)chain, cx, out, nullptr, nullptr, (u32)npolys, S(s));
+    });
+}
+
+int phx_malloc(size_t bytes, void **out) {
+    return guard([&] {
+        nn(out, "out");
+        PHX_HIP_CHECK(hipMalloc(out, bytes));
+    });
+}
+int phx_free(void *p) {
+    return guard([&] { PHX_HIP_CHECK(hipFree(p)); });
+}
+int phx_memcpy_htod(void *dst, const void *src, size_t bytes, phx_stream s) {
+    return guard([&] {
+        PHX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(s)));
+        PHX_HIP_CHECK(hipStreamSynchronize(S(s)));
+    });
+}
+int phx_memcpy_dtoh(void *dst, const void *src, size_t bytes, phx_stream s) {
+    return guard([&] {
+        PHX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(s)));
+        PHX_HIP_CHECK(hipStreamSynchronize(S(s)));
+    });
+}
+int phx_stream_synchronize(phx_stream s) {
+    return guard([&] { PHX_HIP_CHECK(hipStreamSynchronize(S(s))); });
+}
+int phx_device_synchronize(void) {
+    return guard([&] { PHX_HIP_CHECK(hipDeviceSynchronize()); });
+}
+int phx_stream_create(phx_stream *out) {
+    return guard([&] {
+        nn(out, "out");
+        hipStream_t st;
+        PHX_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        *out = st;
+    });
+}
+int phx_stream_destroy(phx_stream s) {
+    return guard([&] {
+        if (s) PHX_HIP_CHECK(hipStreamDestroy(static_cast<hipStream_t>(s)));
+    });
+}
+int phx_event_create(void **out) {
+    return guard([&] {
+        nn(out, "out");
+        hipEvent_t e;
+        PHX_HIP_CHECK(hipEventCreate(&e));
+        *out = e;
+    });
+}
+int phx_event_record(void *ev, phx_stream s) {
+    return guard([&] { PHX_HIP_CHECK(hipEventRecord(static_cast<hipEvent_t>(ev), S(s))); });
+}
+int phx_event_elapsed_ms(void *a, void *b, float *ms) {
+    return guard([&] {
+        nn(ms, "ms");
+        PHX_HIP_CHECK(hipEventSynchronize(static_cast<hipEvent_t>(b)));
+        PHX_HIP_CHECK(hipEventElapsedTime(ms, static_cast<hipEvent_t>(a), static_cast<hipEvent_t>(b)));
+    });
+}
+int phx_event_destroy(void *ev) {
+    return guard([&] {
+        if (ev) PHX_HIP_CHECK(hipEventDestroy(static_cast<hipEvent_t>(ev)));
+    });
+}
+int phx_device_count(int *n) {
+    return guard([&] {
+        nn(n, "n");
+        PHX_HIP_CHECK(hipGetDeviceCount(n));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,293 @@
+// context.cpp -- context construction: tables, per-level constants, pool.
+#include "context.h"
+
+#include <cstring>
+#include <mutex>
+#include <thread>
+
+namespace phx {
+
+namespace {
+std::once_flag g_pool_once;
+std::mutex g_galois_mu;
+
+// byte buffer with aligned appends; offsets are returned in bytes
+struct ConstBuilder {
+    std::vector<uint8_t> buf;
+    size_t align(size_t a) {
+        size_t off = (buf.size() + a - 1) / a * a;
+        buf.resize(off);
+        return off;
+    }
+    size_t push64(const std::vector<u64> &v) {
+        size_t off = align(16);
+        buf.resize(off + v.size() * 8);
+        std::memcpy(buf.data() + off, v.data(), v.size() * 8);
+        return off;
+    }
+    size_t push32(const std::vector<u32> &v) {
+        size_t off = align(16);
+        buf.resize(off + v.size() * 4);
+        std::memcpy(buf.data() + off, v.data(), v.size() * 4);
+        return off;
+    }
+    template <class T>
+    size_t push_struct(const std::vector<T> &v) {
+        size_t off = align(16);
+        buf.resize(off + v.size() * sizeof(T));
+        std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
+};
+}  // namespace
+
+void pool_init() {
+    std::call_once(g_pool_once, [] {
+        int dev = 0;
+        PHX_HIP_CHECK(hipGetDevice(&dev));
+        hipMemPool_t pool;
+        PHX_HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, dev));
+        uint64_t thr = UINT64_MAX;
+        PHX_HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+    });
+}
+
+void *dev_alloc(size_t bytes, hipStream_t s) {
+    if (bytes == 0) return nullptr;
+    void *p = nullptr;
+    PHX_HIP_CHECK(hipMallocAsync(&p, bytes, s));
+    return p;
+}
+
+void dev_free(void *p, hipStream_t s) {
+    if (p) (void)hipFreeAsync(p, s);
+}
+
+Context::Context(const Params &p) : params_(p) {
+    const size_t QP = p.coeff_modulus.size();
+    require(QP >= 2, "The coefficient modulus must be a vector of at least two primes");
+    require(p.special_modulus_size >= 1 && p.special_modulus_size < QP, "invalid special_modulus_size");
+    N_ = p.poly_degree;
+    logN_ = (u32)hm::bit_count(N_) - 1;
+    require(N_ == ((u64)1 << logN_) && logN_ >= 12 && logN_ <= 16,
+            "poly_modulus_degree must be a power of two in [2^12, 2^16]");
+    size_QP_ = (u32)QP;
+    size_P_ = (u32)p.special_modulus_size;
+    size_Q_ = size_QP_ - size_P_;
+    for (u64 q : p.coeff_modulus) {
+        require(q > 1 && (q >> 61) == 0, "value can be at most 61-bit and cannot be 1");
+        require(hm::is_prime(q), "coefficient modulus must be prime");
+        require(q % (2 * N_) == 1, "coefficient modulus must be 1 mod 2N");
+    }
+    for (size_t i = 0; i < QP; ++i)
+        for (size_t j = i + 1; j < QP; ++j) require(p.coeff_modulus[i] != p.coeff_modulus[j], "moduli must be distinct");
+    pool_init();
+
+    // ---- NTT tables (host, one thread per prime) ----
+    host_tables_.resize(QP);
+    {
+        unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 16));
+        std::vector<std::thread> th;
+        for (unsigned w = 0; w < nt; ++w)
+            th.emplace_back([&, w] {
+                for (size_t i = w; i < QP; i += nt) host_tables_[i] = hm::make_ntt_table(p.coeff_modulus[i], (int)logN_);
+            });
+        for (auto &t : th) t.join();
+    }
+    const size_t twb = QP * N_ * sizeof(TwPair);
+    const size_t modb = QP * sizeof(DevModulus);
+    const size_t icb = QP * 4 * sizeof(u64);
+    std::vector<uint8_t> h(2 * twb + modb + icb);
+    TwPair *htw = reinterpret_cast<TwPair *>(h.data());
+    TwPair *hitw = reinterpret_cast<TwPair *>(h.data() + twb);
+    DevModulus *hmod = reinterpret_cast<DevModulus *>(h.data() + 2 * twb);
+    u64 *hic = reinterpret_cast<u64 *>(h.data() + 2 * twb + modb);
+    for (size_t r = 0; r < QP; ++r) {
+        const auto &t = host_tables_[r];
+        for (u64 k = 0; k < N_; ++k) {
+            htw[r * N_ + k] = {t.fw[k], t.fws[k]};
+            hitw[r * N_ + k] = {t.iw[k], t.iws[k]};
+        }
+        u64 lo, hi;
+        hm::barrett_ratio(t.q, lo, hi);
+        hmod[r] = {t.q, lo, hi, 0};
+        const u64 s1 = hm::mulmod(t.ninv, t.iw[1], t.q);
+        hic[4 * r + 0] = t.ninv;
+        hic[4 * r + 1] = hm::shoup(t.ninv, t.q);
+        hic[4 * r + 2] = s1;
+        hic[4 * r + 3] = hm::shoup(s1, t.q);
+    }
+    PHX_HIP_CHECK(hipMalloc(&tw_buf_, h.size()));
+    PHX_HIP_CHECK(hipMemcpy(tw_buf_, h.data(), h.size(), hipMemcpyHostToDevice));
+    uint8_t *db = static_cast<uint8_t *>(tw_buf_);
+    tables_.tw = reinterpret_cast<const TwPair *>(db);
+    tables_.itw = reinterpret_cast<const TwPair *>(db + twb);
+    tables_.mod = reinterpret_cast<const DevModulus *>(db + 2 * twb);
+    tables_.inv_consts = reinterpret_cast<const u64 *>(db + 2 * twb + modb);
+    tables_.logN = logN_;
+    tables_.N = (u32)N_;
+
+    // ---- per-level key-switching constants ----
+    // reference: DRNSTool ctor, hybrid key-switching part (src/rns.cu:97-195)
+    const u64 *qp = p.coeff_modulus.data();
+    const u64 *Pp = qp + size_Q_;
+    const u32 alpha = size_P_;
+    ConstBuilder cb;
+    struct Offs {
+        size_t modup_scale, modup_bconv, moddown_scale, moddown_bconv, pinv, rescale;
+    };
+    std::vector<Offs> offs(size_Q_ + 1);
+    levels_.resize(size_Q_ + 1);
+    auto row_of = [&](u32 Ql, u32 pos) -> u32 { return pos < Ql ? pos : size_Q_ + (pos - Ql); };
+    auto scale_consts = [&](u32 row, u64 scale) {
+        const auto &t = host_tables_[row];
+        const u64 s0 = hm::mulmod(t.ninv, scale, t.q);
+        const u64 s1 = hm::mulmod(hm::mulmod(t.ninv, t.iw[1], t.q), scale, t.q);
+        return std::vector<u64>{s0, hm::shoup(s0, t.q), s1, hm::shoup(s1, t.q)};
+    };
+    for (u32 c = 1; c <= size_Q_; ++c) {
+        LevelConsts &L = levels_[c];
+        const u32 Ql = size_Q_ - (c - 1), QlP = Ql + size_P_;
+        const u32 beta = (Ql + alpha - 1) / alpha;
+        L.chain_index = c;
+        L.size_Ql = Ql;
+        L.size_QlP = QlP;
+        L.beta = beta;
+        std::vector<u64> qlp(QlP);
+        for (u32 i = 0; i < QlP; ++i) qlp[i] = qp[row_of(Ql, i)];
+
+        // modup: INTT scale partQlHatInv, then per digit FastBConv to the complement
+        std::vector<u64> msc;
+        std::vector<BconvDesc> descs;
+        for (u32 j = 0; j < beta; ++j) {
+            const u32 st = alpha * j, sz = (j == beta - 1) ? Ql - alpha * (beta - 1) : alpha;
+            for (u32 i = st; i < st + sz; ++i) {
+                const u64 hinv = hm::invmod(hm::punctured_mod(qlp.data() + st, sz, i - st, qlp[i]), qlp[i]);
+                auto v = scale_consts(i, hinv);
+                msc.insert(msc.end(), v.begin(), v.end());
+            }
+            const u32 no = QlP - sz;
+            std::vector<u64> mat((size_t)no * sz);
+            std::vector<u32> pos(no), rows(no);
+            for (u32 o = 0; o < no; ++o) {
+                pos[o] = o < st ? o : o + sz;
+                rows[o] = row_of(Ql, pos[o]);
+                for (u32 i = 0; i < sz; ++i) mat[(size_t)o * sz + i] = hm::punctured_mod(qlp.data() + st, sz, i, qlp[pos[o]]);
+            }
+            BconvDesc d{};
+            d.ni = sz;
+            d.no = no;
+            d.in_first = st;
+            d.own_first = st;
+            d.mat_off = cb.push64(mat) / 8;
+            d.pos_off = cb.push32(pos) / 4;
+            d.row_off = cb.push32(rows) / 4;
+            descs.push_back(d);
+        }
+        offs[c].modup_scale = cb.push64(msc);
+        offs[c].modup_bconv = cb.push_struct(descs);
+        L.h_modup_bconv = descs;
+
+        // moddown: INTT of the P limbs scaled by PHatInv, FastBConv P -> Ql, P^-1
+        std::vector<u64> dsc;
+        for (u32 j = 0; j < size_P_; ++j) {
+            const u64 hinv = hm::invmod(hm::punctured_mod(Pp, size_P_, j, Pp[j]), Pp[j]);
+            auto v = scale_consts(size_Q_ + j, hinv);
+            dsc.insert(dsc.end(), v.begin(), v.end());
+        }
+        offs[c].moddown_scale = cb.push64(dsc);
+        {
+            std::vector<u64> mat((size_t)Ql * size_P_);
+            std::vector<u32> pos(Ql), rows(Ql);
+            for (u32 i = 0; i < Ql; ++i) {
+                pos[i] = i;
+                rows[i] = i;
+                for (u32 j = 0; j < size_P_; ++j) mat[(size_t)i * size_P_ + j] = hm::punctured_mod(Pp, size_P_, j, qp[i]);
+            }
+            BconvDesc d{};
+            d.ni = size_P_;
+            d.no = Ql;
+            d.in_first = Ql;
+            d.own_first = 0xffffffffu;
+            d.mat_off = cb.push64(mat) / 8;
+            d.pos_off = cb.push32(pos) / 4;
+            d.row_off = cb.push32(rows) / 4;
+            offs[c].moddown_bconv = cb.push_struct(std::vector<BconvDesc>{d});
+            L.h_moddown_bconv = d;
+        }
+        std::vector<u64> pinv;
+        for (u32 i = 0; i < Ql; ++i) {
+            const u64 v = hm::invmod(hm::product_mod(Pp, size_P_, qp[i]), qp[i]);
+            pinv.push_back(v);
+            pinv.push_back(hm::shoup(v, qp[i]));
+        }
+        offs[c].pinv = cb.push64(pinv);
+        std::vector<u64> rq;
+        for (u32 j = 0; j + 1 < Ql; ++j) {
+            const u64 v = hm::invmod(qp[Ql - 1] % qp[j], qp[j]);
+            rq.push_back(v);
+            rq.push_back(hm::shoup(v, qp[j]));
+        }
+        if (rq.empty()) rq.push_back(0);
+        offs[c].rescale = cb.push64(rq);
+    }
+    // key level: P mod q_i (gadget factor of the key-switching keys)
+    std::vector<u64> bp;
+    for (u32 i = 0; i < size_Q_; ++i) {
+        const u64 v = hm::product_mod(Pp, size_P_, qp[i]);
+        bp.push_back(v);
+        bp.push_back(hm::shoup(v, qp[i]));
+    }
+    const size_t bp_off = cb.push64(bp);
+    cb.align(256);
+    PHX_HIP_CHECK(hipMalloc(&cbuf_, cb.buf.size()));
+    PHX_HIP_CHECK(hipMemcpy(cbuf_, cb.buf.data(), cb.buf.size(), hipMemcpyHostToDevice));
+    uint8_t *cbb = reinterpret_cast<uint8_t *>(cbuf_);
+    for (u32 c = 1; c <= size_Q_; ++c) {
+        LevelConsts &L = levels_[c];
+        L.modup_scale = reinterpret_cast<const u64 *>(cbb + offs[c].modup_scale);
+        L.modup_bconv = reinterpret_cast<const BconvDesc *>(cbb + offs[c].modup_bconv);
+        L.moddown_scale = reinterpret_cast<const u64 *>(cbb + offs[c].moddown_scale);
+        L.moddown_bconv = reinterpret_cast<const BconvDesc *>(cbb + offs[c].moddown_bconv);
+        L.pinv = reinterpret_cast<const u64 *>(cbb + offs[c].pinv);
+        L.rescale_qinv = reinterpret_cast<const u64 *>(cbb + offs[c].rescale);
+    }
+    bigP_mod_q_ = reinterpret_cast<u64 *>(cbb + bp_off);
+    levels_[0].chain_index = 0;
+    levels_[0].size_Ql = size_Q_;
+    levels_[0].size_QlP = size_QP_;
+}
+
+Context::~Context() {
+    (void)hipDeviceSynchronize();
+    for (auto &g : galois_tables_) (void)hipFree(g.second);
+    if (cbuf_) (void)hipFree(cbuf_);
+    if (tw_buf_) (void)hipFree(tw_buf_);
+}
+
+const LevelConsts &Context::level(u32 chain_index) const {
+    require(chain_index >= 1 && chain_index <= size_Q_, "invalid chain index");
+    return levels_[chain_index];
+}
+
+// reference: PhantomGaloisTool permutation tables (include/galois.cuh:98-113)
+const u32 *Context::galois_perm(u32 elt) const {
+    std::lock_guard<std::mutex> lk(g_galois_mu);
+    for (auto &g : galois_tables_)
+        if (g.first == elt) return g.second;
+    require((elt & 1) && elt < 2 * N_, "Galois element is not valid");
+    std::vector<u32> t(N_);
+    for (u64 i = N_; i < 2 * N_; ++i) {
+        const u64 r = hm::reverse_bits(i, (int)logN_ + 1);
+        u64 idx = ((u64)elt * r) >> 1;
+        idx &= (N_ - 1);
+        t[i - N_] = (u32)hm::reverse_bits(idx, (int)logN_);
+    }
+    u32 *d = nullptr;
+    PHX_HIP_CHECK(hipMalloc(&d, N_ * sizeof(u32)));
+    PHX_HIP_CHECK(hipMemcpy(d, t.data(), N_ * sizeof(u32), hipMemcpyHostToDevice));
+    galois_tables_.emplace_back(elt, d);
+    return d;
+}
+
+}  // namespace phx

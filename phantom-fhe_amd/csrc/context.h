@@ -1,0 +1,104 @@
+// context.h -- the device-resident context: modulus chain, NTT tables and
+// per-level hybrid key-switching constants.
+//
+// Chain semantics follow the reference's PhantomContext (src/context.cu:121-232):
+// chain index 0 is the key level (all QP primes), chain index c >= 1 is a data
+// level holding the first size_Q - (c - 1) data primes.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "common.h"
+#include "hostmath.h"
+#include "ntt.h"
+
+namespace phx {
+
+enum SchemeType : int { kSchemeNone = 0, kSchemeBGV = 1, kSchemeBFV = 2, kSchemeCKKS = 3 };
+enum MulTech : int { kMulNone = 0, kMulBEHZ = 1, kMulHPS = 2 };
+
+struct Params {
+    int scheme = kSchemeCKKS;
+    int mul_tech = kMulNone;
+    u64 poly_degree = 0;
+    std::vector<u64> coeff_modulus;   // size_Q data primes then size_P special primes
+    u64 special_modulus_size = 1;
+    u64 plain_modulus = 0;
+    std::vector<u32> galois_elts;
+};
+
+// Base conversion of one digit / one polynomial (FastBConv phase 2):
+// out[out_pos[o]] = (sum_i in[i] * mat[o][i]) mod omod[o]
+struct BconvDesc {
+    u32 ni, no;          // input limbs, output limbs
+    u32 in_first;        // first input limb (within the input limb set)
+    u32 own_first;       // modup: first output limb that copies the digit's own limbs
+    u64 mat_off;         // offset (u64 units) of the [no][ni] matrix in the constant buffer
+    u64 pos_off;         // offset (u32 units) of out_pos[no]
+    u64 row_off;         // offset (u32 units) of the table rows of the outputs
+};
+
+struct LevelConsts {
+    u32 chain_index = 0;
+    u32 size_Ql = 0, size_QlP = 0, beta = 0;
+    // device pointers into the context constant buffer
+    const u64 *modup_scale = nullptr;     // [Ql][4] INTT final-stage constants x partQlHatInv
+    const BconvDesc *modup_bconv = nullptr;  // [beta]
+    const u64 *moddown_scale = nullptr;   // [P][4] INTT final-stage constants x PHatInv
+    const BconvDesc *moddown_bconv = nullptr;  // [1]
+    const u64 *pinv = nullptr;            // [Ql][2] P^-1 mod q_i (Shoup)
+    const u64 *rescale_qinv = nullptr;    // [Ql-1][2] q_last^-1 mod q_j (Shoup)
+    std::vector<BconvDesc> h_modup_bconv; // host copies
+    BconvDesc h_moddown_bconv{};
+};
+
+class Context {
+public:
+    explicit Context(const Params &p);
+    ~Context();
+    Context(const Context &) = delete;
+    Context &operator=(const Context &) = delete;
+
+    const Params &params() const { return params_; }
+    u64 N() const { return N_; }
+    u32 logN() const { return logN_; }
+    u32 size_Q() const { return size_Q_; }
+    u32 size_P() const { return size_P_; }
+    u32 size_QP() const { return size_QP_; }
+    u32 dnum() const { return (size_Q_ + size_P_ - 1) / size_P_; }
+    u32 total_levels() const { return size_Q_ + 1; }
+    u32 limbs_at(u32 chain_index) const { return chain_index == 0 ? size_QP_ : size_Q_ - (chain_index - 1); }
+    const std::vector<u64> &moduli() const { return params_.coeff_modulus; }
+    const NttTables &tables() const { return tables_; }
+    const LevelConsts &level(u32 chain_index) const;
+    const DevModulus *dev_mod() const { return tables_.mod; }
+    const u64 *const_buf() const { return cbuf_; }
+    const u32 *const_buf32() const { return reinterpret_cast<const u32 *>(cbuf_); }
+    // P mod q_i for i < size_Q (keygen gadget factor), [size_Q][2] with Shoup
+    const u64 *bigP_mod_q() const { return bigP_mod_q_; }
+    const hm::NttTable &host_table(u32 row) const { return host_tables_[row]; }
+
+    // Galois permutation tables (NTT domain), created on first use
+    const u32 *galois_perm(u32 elt) const;
+    const std::vector<u32> &galois_elts() const { return params_.galois_elts; }
+
+private:
+    Params params_;
+    u64 N_ = 0;
+    u32 logN_ = 0, size_Q_ = 0, size_P_ = 0, size_QP_ = 0;
+    std::vector<hm::NttTable> host_tables_;
+    NttTables tables_{};
+    void *tw_buf_ = nullptr;     // forward + inverse twiddles, moduli, inverse consts
+    u64 *cbuf_ = nullptr;        // per-level constants
+    u64 *bigP_mod_q_ = nullptr;
+    std::vector<LevelConsts> levels_;
+    mutable std::vector<std::pair<u32, u32 *>> galois_tables_;
+};
+
+// stream-ordered device allocation (pooled, like the reference's cuda_auto_ptr)
+void *dev_alloc(size_t bytes, hipStream_t s);
+void dev_free(void *p, hipStream_t s);
+void pool_init();
+
+}  // namespace phx

@@ -1,0 +1,82 @@
+// evaluator.h -- ciphertext/key types and the hot-path operations.
+#pragma once
+
+#include <vector>
+
+#include "context.h"
+
+namespace phx {
+
+// Ciphertext: device buffer [size][coeff_modulus_size][N] (the reference's
+// PhantomCiphertext layout, include/ciphertext.h:15-25).
+struct Ciphertext {
+    u32 chain_index = 0;
+    u32 size = 0;
+    u64 poly_degree = 0;
+    u32 coeff_modulus_size = 0;
+    double scale = 1.0;
+    u64 correction_factor = 1;
+    u32 noise_scale_deg = 1;
+    bool is_ntt_form = true;
+    bool is_asymmetric = false;
+    u64 *data = nullptr;
+    size_t capacity = 0;  // elements
+    bool owns = true;
+
+    void resize(const Context &ctx, u32 chain, u32 sz, hipStream_t s);
+    void release(hipStream_t s);
+};
+
+// Key-switching key: [dnum][2][QP][N] (digit j = (b_j, a_j), NTT form),
+// the reference's PhantomRelinKey public_keys_ layout (src/secretkey.cu:297-341)
+// with dnum = ceil(size_Q / size_P).
+struct KSwitchKey {
+    u32 dnum = 0;
+    u64 *data = nullptr;
+};
+
+struct GaloisKeys {
+    std::vector<u32> elts;
+    std::vector<KSwitchKey> keys;
+};
+
+struct SecretKey {
+    u64 *s = nullptr;    // [QP][N] NTT form
+    u64 *s2 = nullptr;   // s^2, [QP][N] NTT form (lazily computed)
+};
+
+// ---- hot path ----
+void tensor_2x2(const Context &ctx, u32 Ql, const u64 *a, const u64 *b, u64 *d0, u64 *d1, u64 *d2, hipStream_t s);
+// ct[0..1] (Ql limbs each) = base0/base1 + keyswitch(c2); base may be null (zero)
+void keyswitch(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey &key, u64 *ct,
+               const u64 *base0, const u64 *base1, hipStream_t s);
+void multiply_and_relin(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, const KSwitchKey &rlk,
+                        hipStream_t s);
+void relinearize(const Context &ctx, Ciphertext &ct, const KSwitchKey &rlk, hipStream_t s);
+void rescale_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, hipStream_t s);
+void apply_galois(const Context &ctx, Ciphertext &ct, u32 elt, const GaloisKeys &gk, hipStream_t s);
+void rotate(const Context &ctx, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s);
+u32 galois_elt_from_step(int step, u64 N);
+
+// ---- pieces exposed for tests / the low-level API ----
+void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStream_t s);
+void inner_product(const Context &ctx, u32 chain, const u64 *t_mod_up, const KSwitchKey &key, u64 *cx,
+                   hipStream_t s);
+void moddown(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, const u64 *base1, u32 npolys,
+             hipStream_t s);
+
+// ---- keys and (test-side) encryption ----
+void secret_key_generate(const Context &ctx, SecretKey &sk, u64 seed, hipStream_t s);
+void secret_key_upload(const Context &ctx, SecretKey &sk, const int8_t *coeffs, hipStream_t s);
+void kswitch_key_generate(const Context &ctx, const SecretKey &sk, const u64 *new_key, KSwitchKey &key, u64 seed,
+                          hipStream_t s);
+void relin_key_generate(const Context &ctx, SecretKey &sk, KSwitchKey &key, u64 seed, hipStream_t s);
+void galois_key_generate(const Context &ctx, const SecretKey &sk, u32 elt, KSwitchKey &key, u64 seed,
+                         hipStream_t s);
+// plain: [Ql][N] coefficient-domain residues at chain index `chain`; result NTT form
+void encrypt_symmetric(const Context &ctx, const SecretKey &sk, const u64 *plain_dev, u32 chain, Ciphertext &ct,
+                       u64 seed, hipStream_t s);
+// out: [Ql][N] coefficient-domain residues of c0 + c1 s (+ c2 s^2)
+void decrypt(const Context &ctx, SecretKey &sk, const Ciphertext &ct, u64 *out_dev, hipStream_t s);
+
+}  // namespace phx

@@ -1,0 +1,546 @@
+// evaluator.hip -- hot-path kernels (base conversion, key inner product,
+// tensor, Galois permutation) and the orchestration of key switching,
+// relinearisation, rotation and rescale; key generation and symmetric
+// encryption for tests.
+#include <algorithm>
+#include <cstring>
+
+#include "evaluator.h"
+
+namespace phx {
+
+namespace {
+constexpr int kThreads = 256;
+constexpr int kMaxDigitLimbs = 16;   // FastBConv input limbs per digit (alpha)
+
+inline dim3 grid_for(u64 N, u32 y, u32 z = 1) { return dim3((u32)((N + kThreads - 1) / kThreads), y, z); }
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// FastBConv phase 2 (+ optional copy of the digit's own limbs), one thread per
+// coefficient, one grid row per job (digit or polynomial):
+//   out[pos[o]][k] = (sum_i in[in_first+i][k] * mat[o][i]) mod p_o
+// 128-bit accumulation, one Barrett reduction per output.
+// reference: bconv_matmul_padded_unroll2_kernel src/rns_bconv.cu:455-485,
+//            bconv_matmul_unroll2_kernel 143-168, modup_copy_partQl 522-528
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_bconv(const u64 *__restrict__ cbuf, const BconvDesc *__restrict__ descs,
+                                                    u32 desc_stride, const u64 *__restrict__ in, u64 in_job_stride,
+                                                    u64 *__restrict__ out, u64 out_job_stride,
+                                                    const u64 *__restrict__ copy_src, u64 copy_job_stride,
+                                                    const DevModulus *__restrict__ mod, u32 N) {
+    const u32 job = blockIdx.y;
+    const BconvDesc d = descs[job * desc_stride];
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 *inj = in + job * in_job_stride + (u64)d.in_first * N + k;
+    u64 *outj = out + job * out_job_stride + k;
+    const u64 *mat = cbuf + d.mat_off;
+    const u32 *pos = reinterpret_cast<const u32 *>(cbuf) + d.pos_off;
+    const u32 *rows = reinterpret_cast<const u32 *>(cbuf) + d.row_off;
+    u64 x[kMaxDigitLimbs];
+#pragma unroll
+    for (int i = 0; i < kMaxDigitLimbs; ++i) x[i] = (u32)i < d.ni ? inj[(u64)i * N] : 0;
+    for (u32 o = 0; o < d.no; ++o) {
+        const u64 *mo = mat + (u64)o * d.ni;
+        u128 acc = 0;
+#pragma unroll
+        for (int i = 0; i < kMaxDigitLimbs; ++i)
+            if ((u32)i < d.ni) acc += (u128)x[i] * mo[i];
+        const DevModulus m = mod[rows[o]];
+        outj[(u64)pos[o] * N] = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
+    }
+    if (copy_src) {
+        const u64 *cs = copy_src + job * copy_job_stride + (u64)d.in_first * N + k;
+        for (u32 i = 0; i < d.ni; ++i) outj[(u64)(d.own_first + i) * N] = cs[(u64)i * N];
+    }
+}
+
+// key_switch_inner_prod_c2_and_evk (src/eval_key_switch.cu:14-69):
+// cx0[pos] = sum_j t[j][pos] * key_j.b[row], cx1[pos] = sum_j t[j][pos] * key_j.a[row]
+__global__ __launch_bounds__(kThreads) void k_inner_product(const u64 *__restrict__ t, const u64 *__restrict__ key,
+                                                            u64 *__restrict__ cx, const DevModulus *__restrict__ mod,
+                                                            u32 N, u32 Ql, u32 QlP, u32 size_Q, u32 QP, u32 beta) {
+    const u32 pos = blockIdx.y;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u32 row = pos < Ql ? pos : size_Q + (pos - Ql);
+    const u64 QPN = (u64)QP * N;
+    u128 a0 = 0, a1 = 0;
+    for (u32 j = 0; j < beta; ++j) {
+        const u64 x = t[((u64)j * QlP + pos) * N + k];
+        const u64 *kb = key + (u64)j * 2 * QPN + (u64)row * N + k;
+        a0 += (u128)x * kb[0];
+        a1 += (u128)x * kb[QPN];
+    }
+    const DevModulus m = mod[row];
+    cx[(u64)pos * N + k] = barrett128(a0, m.value, m.ratio_lo, m.ratio_hi);
+    cx[((u64)QlP + pos) * N + k] = barrett128(a1, m.value, m.ratio_lo, m.ratio_hi);
+}
+
+// tensor_prod_2x2_rns_poly (src/polymath.cu:463-498), Karatsuba middle term
+__global__ __launch_bounds__(kThreads) void k_tensor(const u64 *__restrict__ a, const u64 *__restrict__ b,
+                                                     u64 *d0, u64 *d1, u64 *__restrict__ d2,
+                                                     const DevModulus *__restrict__ mod, u32 N, u64 LN) {
+    const u32 limb = blockIdx.y;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 o = (u64)limb * N + k;
+    const DevModulus m = mod[limb];
+    const u64 q = m.value;
+    const u64 a0 = a[o], a1 = a[LN + o], b0 = b[o], b1 = b[LN + o];
+    const u64 r0 = mul_mod(a0, b0, m);
+    const u64 r2 = mul_mod(a1, b1, m);
+    u64 r1 = mul_mod(add_mod(a0, a1, q), add_mod(b0, b1, q), m);
+    r1 = sub_mod(sub_mod(r1, r0, q), r2, q);
+    d0[o] = r0;
+    d1[o] = r1;
+    d2[o] = r2;
+}
+
+// apply_galois_ntt_permutation (src/galois.cu:11-18): dst[i] = src[perm[i]]
+__global__ __launch_bounds__(kThreads) void k_permute(const u64 *__restrict__ src, u64 *__restrict__ dst,
+                                                      const u32 *__restrict__ perm, u32 N) {
+    const u64 off = (u64)blockIdx.y * N;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    dst[off + k] = src[off + perm[k]];
+}
+
+// element-wise kernels used by key generation / encryption / decryption
+__global__ __launch_bounds__(kThreads) void k_mul_elementwise(const u64 *a, const u64 *b, u64 *c,
+                                                              const DevModulus *mod, u32 N, u32 row0) {
+    const u32 limb = blockIdx.y;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 o = (u64)limb * N + k;
+    c[o] = mul_mod(a[o], b[o], mod[row0 + limb]);
+}
+
+__device__ __forceinline__ u64 mix64(u64 x) {
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+__device__ __forceinline__ u64 rnd(u64 seed, u64 a, u64 b) {
+    return mix64(seed ^ mix64(a * 0x9e3779b97f4a7c15ull + b + 0x632be59bd9b4e019ull));
+}
+__device__ __forceinline__ u64 uniform_mod(u64 seed, u64 idx, u64 q) {
+    const int bits = 64 - __clzll(q);
+    const u64 mask = bits == 64 ? ~0ull : ((1ull << bits) - 1);
+    for (u64 t = 0; t < 128; ++t) {
+        const u64 r = rnd(seed, idx, t) & mask;
+        if (r < q) return r;
+    }
+    return rnd(seed, idx, 999) % q;
+}
+// centred binomial error, variance 21/2 (the reference's 21+21-bit CBD,
+// src/prng.cu:222-244)
+__device__ __forceinline__ long long cbd21(u64 seed, u64 k) {
+    const u64 r = rnd(seed, k, 7);
+    return (long long)__popcll(r & 0x1fffffull) - (long long)__popcll((r >> 21) & 0x1fffffull);
+}
+
+// ternary secret, identical integer across limbs
+__global__ void k_sample_ternary(u64 *s, const DevModulus *mod, u32 N, u64 seed) {
+    const u32 limb = blockIdx.y;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 q = mod[limb].value;
+    const u64 r = rnd(seed, k, 3) % 3;
+    s[(u64)limb * N + k] = r == 0 ? 0 : (r == 1 ? 1 : q - 1);
+}
+
+__global__ void k_signed_to_rns(const int8_t *c, u64 *s, const DevModulus *mod, u32 N) {
+    const u32 limb = blockIdx.y;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 q = mod[limb].value;
+    const int v = c[k];
+    s[(u64)limb * N + k] = v >= 0 ? (u64)v : q - (u64)(-v);
+}
+
+__global__ void k_sample_error(u64 *e, const DevModulus *mod, u32 N, u32 row0, u64 seed) {
+    const u32 limb = blockIdx.y;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 q = mod[row0 + limb].value;
+    const long long v = cbd21(seed, k);
+    e[(u64)limb * N + k] = v >= 0 ? (u64)v : q - (u64)(-v);
+}
+
+// c1 = a (uniform, NTT form), c0 = -(a s + e) + m (+ gadget * new_key on the
+// digit's limbs). Rows are data rows 0..L-1 (keys: all QP rows).
+// reference: encrypt_zero_symmetric src/secretkey.cu:232-289,
+//            multiply_temp_mod_and_add_rns_poly src/polymath.cu:318-338
+__global__ void k_rlwe(u64 *c0, u64 *c1, const u64 *s, const u64 *e_ntt, const u64 *m_ntt, const u64 *gadget_key,
+                       const u64 *bigP, u32 g_lo, u32 g_hi, const DevModulus *mod, u32 N, u64 seed) {
+    const u32 limb = blockIdx.y;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 o = (u64)limb * N + k;
+    const DevModulus md = mod[limb];
+    const u64 q = md.value;
+    const u64 a = uniform_mod(seed, o, q);
+    u64 v = add_mod(mul_mod(a, s[o], md), e_ntt[o], q);
+    v = v ? q - v : 0;
+    if (m_ntt) v = add_mod(v, m_ntt[o], q);
+    if (gadget_key && limb >= g_lo && limb < g_hi)
+        v = add_mod(v, mul_shoup(gadget_key[o], bigP[2 * limb], bigP[2 * limb + 1], q), q);
+    c0[o] = v;
+    c1[o] = a;
+}
+
+// decrypt: m = c0 + c1 s + c2 s^2 (NTT form)
+__global__ void k_dot_secret(const u64 *ct, const u64 *s, const u64 *s2, u64 *out, const DevModulus *mod, u32 N,
+                             u32 L, u32 size, u64 s_stride) {
+    const u32 limb = blockIdx.y;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 o = (u64)limb * N + k;
+    const u64 LN = (u64)L * N;
+    const DevModulus md = mod[limb];
+    u64 v = add_mod(ct[o], mul_mod(ct[LN + o], s[o], md), md.value);
+    if (size > 2) v = add_mod(v, mul_mod(ct[2 * LN + o], s2[o], md), md.value);
+    out[o] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+void Ciphertext::resize(const Context &ctx, u32 chain, u32 sz, hipStream_t s) {
+    const u32 L = ctx.limbs_at(chain);
+    const size_t need = (size_t)sz * L * ctx.N();
+    if (need > capacity || !owns) {
+        u64 *nd = static_cast<u64 *>(dev_alloc(need * 8, s));
+        const size_t old = (size_t)size * coeff_modulus_size * poly_degree;
+        if (data && old) PHX_HIP_CHECK(hipMemcpyAsync(nd, data, std::min(old, need) * 8, hipMemcpyDeviceToDevice, s));
+        if (owns) dev_free(data, s);
+        data = nd;
+        capacity = need;
+        owns = true;
+    }
+    chain_index = chain;
+    size = sz;
+    coeff_modulus_size = L;
+    poly_degree = ctx.N();
+}
+
+void Ciphertext::release(hipStream_t s) {
+    if (owns) dev_free(data, s);
+    data = nullptr;
+    capacity = 0;
+}
+
+void tensor_2x2(const Context &ctx, u32 Ql, const u64 *a, const u64 *b, u64 *d0, u64 *d1, u64 *d2, hipStream_t s) {
+    const u64 N = ctx.N();
+    hipLaunchKernelGGL(k_tensor, grid_for(N, Ql), dim3(kThreads), 0, s, a, b, d0, d1, d2, ctx.dev_mod(), (u32)N,
+                       (u64)Ql * N);
+}
+
+// DRNSTool::modup for CKKS (src/rns_bconv.cu:530-628)
+void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStream_t s) {
+    const LevelConsts &L = ctx.level(chain);
+    const u64 N = ctx.N();
+    const u32 Ql = L.size_Ql, QlP = L.size_QlP, beta = L.beta, alpha = ctx.size_P();
+    require(alpha <= (u32)kMaxDigitLimbs, "special_modulus_size above 16 is not supported");
+    u64 *t_cks = static_cast<u64 *>(dev_alloc((size_t)Ql * N * 8, s));
+    // INTT(c2) * partQlHatInv (out of place)
+    LimbMap mi = limb_map(t_cks, 0, Ql);
+    mi.src = c2;
+    mi.scale = L.modup_scale;
+    ntt_inverse(ctx.tables(), mi, Ql, 1, s);
+    // FastBConv of every digit to its complement, own limbs copied
+    hipLaunchKernelGGL(k_bconv, grid_for(N, beta), dim3(kThreads), 0, s, ctx.const_buf(), L.modup_bconv, 1u,
+                       (const u64 *)t_cks, (u64)0, t_mod_up, (u64)QlP * N, c2, (u64)0, ctx.dev_mod(), (u32)N);
+    PHX_HIP_CHECK(hipGetLastError());
+    // forward NTT of the complement limbs of every digit
+    LimbMap mf = limb_map(t_mod_up, 0, QlP, (u64)QlP * N);
+    mf.split = Ql;
+    mf.special_row = ctx.size_Q();
+    mf.skip_alpha = alpha;
+    mf.skip_cap = Ql;
+    mf.nlimbs = QlP;
+    const u32 last_sz = Ql - alpha * (beta - 1);
+    ntt_forward(ctx.tables(), mf, QlP - last_sz, beta, s);
+    dev_free(t_cks, s);
+}
+
+void inner_product(const Context &ctx, u32 chain, const u64 *t_mod_up, const KSwitchKey &key, u64 *cx,
+                   hipStream_t s) {
+    const LevelConsts &L = ctx.level(chain);
+    require(key.dnum >= L.beta, "key-switching key has too few digits for this level");
+    const u64 N = ctx.N();
+    hipLaunchKernelGGL(k_inner_product, grid_for(N, L.size_QlP), dim3(kThreads), 0, s, t_mod_up,
+                       (const u64 *)key.data, cx, ctx.dev_mod(), (u32)N, L.size_Ql, L.size_QlP, ctx.size_Q(),
+                       ctx.size_QP(), L.beta);
+    PHX_HIP_CHECK(hipGetLastError());
+}
+
+// DRNSTool::moddown_from_NTT for CKKS (src/rns_bconv.cu:776-828), both halves
+// in one launch per step, the tail fused into the last NTT pass.
+void moddown(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, const u64 *base1, u32 npolys,
+             hipStream_t s) {
+    const LevelConsts &L = ctx.level(chain);
+    const u64 N = ctx.N();
+    const u32 Ql = L.size_Ql, QlP = L.size_QlP, P = ctx.size_P();
+    // INTT of the P limbs, scaled by PHatInv (FastBConv phase 1)
+    LimbMap mi = limb_map(cx + (u64)Ql * N, ctx.size_Q(), P, (u64)QlP * N);
+    mi.scale = L.moddown_scale;
+    ntt_inverse(ctx.tables(), mi, P, npolys, s);
+    u64 *delta = static_cast<u64 *>(dev_alloc((size_t)npolys * Ql * N * 8, s));
+    hipLaunchKernelGGL(k_bconv, grid_for(N, npolys), dim3(kThreads), 0, s, ctx.const_buf(), L.moddown_bconv, 0u,
+                       (const u64 *)cx, (u64)QlP * N, delta, (u64)Ql * N, (const u64 *)nullptr, (u64)0,
+                       ctx.dev_mod(), (u32)N);
+    PHX_HIP_CHECK(hipGetLastError());
+    StoreModDown st{};
+    st.ct = ct;
+    st.base0 = base0;
+    st.base1 = base1;
+    st.cx = cx;
+    st.pinv = L.pinv;
+    st.ct_stride = (u64)Ql * N;
+    st.cx_stride = (u64)QlP * N;
+    st.N = (u32)N;
+    ntt_forward_store(ctx.tables(), limb_map(delta, 0, Ql, (u64)Ql * N), Ql, npolys, st, s);
+    dev_free(delta, s);
+}
+
+// keyswitch_inplace (src/eval_key_switch.cu:95-182)
+void keyswitch(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey &key, u64 *ct, const u64 *base0,
+               const u64 *base1, hipStream_t s) {
+    const LevelConsts &L = ctx.level(chain);
+    require(key.data != nullptr, "key-switching key is empty");
+    require(key.dnum >= L.beta, "key-switching key has too few digits for this level");
+    const u64 N = ctx.N();
+    u64 *t_mod_up = static_cast<u64 *>(dev_alloc((size_t)L.beta * L.size_QlP * N * 8, s));
+    u64 *cx = static_cast<u64 *>(dev_alloc((size_t)2 * L.size_QlP * N * 8, s));
+    modup(ctx, chain, c2, t_mod_up, s);
+    inner_product(ctx, chain, t_mod_up, key, cx, s);
+    moddown(ctx, chain, cx, ct, base0, base1, 2, s);
+    dev_free(cx, s);
+    dev_free(t_mod_up, s);
+}
+
+// multiply_and_relin_inplace for CKKS (src/evaluate.cu:1061-1104)
+void multiply_and_relin(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, const KSwitchKey &rlk,
+                        hipStream_t s) {
+    require(ctx.params().scheme == kSchemeCKKS, "multiply_and_relin: only CKKS is on this path");
+    require(ct1.chain_index == ct2.chain_index, "encrypted1 and encrypted2 parameter mismatch");
+    require(ct1.is_ntt_form == ct2.is_ntt_form, "NTT form mismatch");
+    require(ct1.is_ntt_form, "encrypted1 and encrypted2 must be in NTT form");
+    require(ct1.size == 2 && ct2.size == 2, "poly number mismatch");
+    const u32 Ql = ct1.coeff_modulus_size;
+    const u64 N = ctx.N(), LN = (u64)Ql * N;
+    u64 *c2 = static_cast<u64 *>(dev_alloc(LN * 8, s));
+    tensor_2x2(ctx, Ql, ct1.data, ct2.data, ct1.data, ct1.data + LN, c2, s);
+    keyswitch(ctx, ct1.chain_index, c2, rlk, ct1.data, ct1.data, ct1.data + LN, s);
+    dev_free(c2, s);
+    ct1.scale = ct1.scale * ct2.scale;
+}
+
+// relinearize_inplace (src/evaluate.cu:1342-1374)
+void relinearize(const Context &ctx, Ciphertext &ct, const KSwitchKey &rlk, hipStream_t s) {
+    require(ct.size == 3, "destination_size must be 3");
+    require(ct.is_ntt_form, "CKKS encrypted must be in NTT form");
+    const u64 LN = (u64)ct.coeff_modulus_size * ctx.N();
+    keyswitch(ctx, ct.chain_index, ct.data + 2 * LN, rlk, ct.data, ct.data, ct.data + LN, s);
+    ct.size = 2;
+}
+
+// rescale_to_next (src/evaluate.cu:1545-1565, 1376-1427; src/rns.cu:1160-1184)
+void rescale_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, hipStream_t s) {
+    require(ctx.params().scheme == kSchemeCKKS, "unsupported scheme");
+    require(in.chain_index >= 1, "invalid chain index");
+    require(in.chain_index < ctx.size_Q(), "end of modulus switching chain reached");
+    const LevelConsts &L = ctx.level(in.chain_index);
+    const u32 Ql = L.size_Ql, size = in.size;
+    const u64 N = ctx.N();
+    u64 *last = static_cast<u64 *>(dev_alloc((size_t)size * N * 8, s));
+    LimbMap mi = limb_map(last, Ql - 1, 1, N);
+    mi.src = in.data + (u64)(Ql - 1) * N;
+    mi.src_poly_stride = (u64)Ql * N;
+    ntt_inverse(ctx.tables(), mi, 1, size, s);
+    Ciphertext dst;
+    dst.resize(ctx, in.chain_index + 1, size, s);
+    LoadRescale ld{last, N};
+    StoreRescale st{in.data, dst.data, L.rescale_qinv, (u64)Ql * N, (u64)(Ql - 1) * N, (u32)N};
+    ntt_forward_rescale(ctx.tables(), limb_map(dst.data, 0, Ql - 1, (u64)(Ql - 1) * N), Ql - 1, size, ld, st, s);
+    dev_free(last, s);
+    dst.is_ntt_form = in.is_ntt_form;
+    dst.scale = in.scale / (double)ctx.moduli()[Ql - 1];
+    dst.correction_factor = in.correction_factor;
+    if (&out != &in) out.release(s);
+    out = dst;
+}
+
+// apply_galois_inplace (src/evaluate.cu:1567-1630)
+void apply_galois(const Context &ctx, Ciphertext &ct, u32 elt, const GaloisKeys &gk, hipStream_t s) {
+    require(ct.size <= 2, "encrypted size must be 2");
+    auto it = std::find(gk.elts.begin(), gk.elts.end(), elt);
+    require(it != gk.elts.end(), "Galois elt not present");
+    const KSwitchKey &key = gk.keys[it - gk.elts.begin()];
+    const u32 Ql = ct.coeff_modulus_size;
+    const u64 N = ctx.N(), LN = (u64)Ql * N;
+    const u32 *perm = ctx.galois_perm(elt);
+    u64 *tmp = static_cast<u64 *>(dev_alloc(2 * LN * 8, s));
+    hipLaunchKernelGGL(k_permute, grid_for(N, 2 * Ql), dim3(kThreads), 0, s, (const u64 *)ct.data, tmp, perm,
+                       (u32)N);
+    PHX_HIP_CHECK(hipGetLastError());
+    // (c0, c1) <- (pi(c0), 0) + keyswitch(pi(c1))
+    keyswitch(ctx, ct.chain_index, tmp + LN, key, ct.data, tmp, nullptr, s);
+    dev_free(tmp, s);
+}
+
+u32 galois_elt_from_step(int step, u64 N) {
+    // get_elt_from_step (include/galois.cuh:16-49)
+    const u64 m = 2 * N;
+    if (step == 0) return (u32)(m - 1);
+    const bool neg = step < 0;
+    const u64 ps = (u64)(neg ? -(long long)step : step);
+    require(ps < N / 2, "step count too large");
+    long long e = neg ? (long long)(N / 2) - (long long)ps : (long long)ps;
+    u64 g = 1;
+    while (e--) g = (g * 5) & (m - 1);
+    return (u32)g;
+}
+
+namespace {
+// non-adjacent form of step (reference: naf(), src/evaluate.cu rotate_internal)
+std::vector<int> naf(int value) {
+    std::vector<int> res;
+    const bool sign = value < 0;
+    value = std::abs(value);
+    for (int i = 0; value; ++i) {
+        const int zi = (value & 1) ? 2 - (value & 3) : 0;
+        value = (value - zi) >> 1;
+        if (zi) res.push_back((sign ? -zi : zi) * (1 << i));
+    }
+    return res;
+}
+}  // namespace
+
+// rotate_inplace -> rotate_internal (src/evaluate.cu:1633-1668)
+void rotate(const Context &ctx, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s) {
+    const u32 elt = galois_elt_from_step(step, ctx.N());
+    if (std::find(gk.elts.begin(), gk.elts.end(), elt) != gk.elts.end()) {
+        apply_galois(ctx, ct, elt, gk, s);
+        return;
+    }
+    const auto steps = naf(step);
+    require(steps.size() != 1, "Galois key not present");
+    for (int st : steps)
+        if ((u64)std::abs(st) != (ctx.N() >> 1)) rotate(ctx, ct, st, gk, s);
+}
+
+// ---------------------------------------------------------------------------
+// keys and encryption (test-side key material; seeded, reproducible)
+// ---------------------------------------------------------------------------
+void secret_key_generate(const Context &ctx, SecretKey &sk, u64 seed, hipStream_t s) {
+    const u64 N = ctx.N();
+    const u32 QP = ctx.size_QP();
+    sk.s = static_cast<u64 *>(dev_alloc((size_t)QP * N * 8, s));
+    hipLaunchKernelGGL(k_sample_ternary, grid_for(N, QP), dim3(kThreads), 0, s, sk.s, ctx.dev_mod(), (u32)N, seed);
+    ntt_forward(ctx.tables(), limb_map(sk.s, 0, QP), QP, 1, s);
+}
+
+void secret_key_upload(const Context &ctx, SecretKey &sk, const int8_t *coeffs, hipStream_t s) {
+    const u64 N = ctx.N();
+    const u32 QP = ctx.size_QP();
+    sk.s = static_cast<u64 *>(dev_alloc((size_t)QP * N * 8, s));
+    int8_t *d = static_cast<int8_t *>(dev_alloc(N, s));
+    PHX_HIP_CHECK(hipMemcpyAsync(d, coeffs, N, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_signed_to_rns, grid_for(N, QP), dim3(kThreads), 0, s, (const int8_t *)d, sk.s,
+                       ctx.dev_mod(), (u32)N);
+    ntt_forward(ctx.tables(), limb_map(sk.s, 0, QP), QP, 1, s);
+    dev_free(d, s);
+}
+
+// generate_one_kswitch_key (src/secretkey.cu:297-341), dnum = ceil(size_Q / size_P)
+void kswitch_key_generate(const Context &ctx, const SecretKey &sk, const u64 *new_key, KSwitchKey &key, u64 seed,
+                          hipStream_t s) {
+    const u64 N = ctx.N();
+    const u32 QP = ctx.size_QP(), alpha = ctx.size_P(), dnum = ctx.dnum();
+    const u64 QPN = (u64)QP * N;
+    key.dnum = dnum;
+    key.data = static_cast<u64 *>(dev_alloc((size_t)dnum * 2 * QPN * 8, s));
+    u64 *e = static_cast<u64 *>(dev_alloc(QPN * 8, s));
+    for (u32 j = 0; j < dnum; ++j) {
+        const u64 sj = seed * 0x100000001b3ull + j * 0x9e3779b97f4a7c15ull + 1;
+        hipLaunchKernelGGL(k_sample_error, grid_for(N, QP), dim3(kThreads), 0, s, e, ctx.dev_mod(), (u32)N, 0u,
+                           sj ^ 0x5555);
+        ntt_forward(ctx.tables(), limb_map(e, 0, QP), QP, 1, s);
+        u64 *b = key.data + (u64)j * 2 * QPN;
+        const u32 g_lo = alpha * j, g_hi = std::min(alpha * (j + 1), ctx.size_Q());
+        hipLaunchKernelGGL(k_rlwe, grid_for(N, QP), dim3(kThreads), 0, s, b, b + QPN, (const u64 *)sk.s,
+                           (const u64 *)e, (const u64 *)nullptr, new_key, ctx.bigP_mod_q(), g_lo, g_hi,
+                           ctx.dev_mod(), (u32)N, sj);
+    }
+    PHX_HIP_CHECK(hipGetLastError());
+    dev_free(e, s);
+}
+
+void relin_key_generate(const Context &ctx, SecretKey &sk, KSwitchKey &key, u64 seed, hipStream_t s) {
+    const u64 N = ctx.N();
+    const u32 QP = ctx.size_QP();
+    if (!sk.s2) {
+        sk.s2 = static_cast<u64 *>(dev_alloc((size_t)QP * N * 8, s));
+        hipLaunchKernelGGL(k_mul_elementwise, grid_for(N, QP), dim3(kThreads), 0, s, (const u64 *)sk.s,
+                           (const u64 *)sk.s, sk.s2, ctx.dev_mod(), (u32)N, 0u);
+    }
+    kswitch_key_generate(ctx, sk, sk.s2, key, seed, s);
+}
+
+// create_galois_keys (src/secretkey.cu:421-461): key for pi_elt(s)
+void galois_key_generate(const Context &ctx, const SecretKey &sk, u32 elt, KSwitchKey &key, u64 seed,
+                         hipStream_t s) {
+    const u64 N = ctx.N();
+    const u32 QP = ctx.size_QP();
+    u64 *rs = static_cast<u64 *>(dev_alloc((size_t)QP * N * 8, s));
+    hipLaunchKernelGGL(k_permute, grid_for(N, QP), dim3(kThreads), 0, s, (const u64 *)sk.s, rs, ctx.galois_perm(elt),
+                       (u32)N);
+    kswitch_key_generate(ctx, sk, rs, key, seed, s);
+    dev_free(rs, s);
+}
+
+void encrypt_symmetric(const Context &ctx, const SecretKey &sk, const u64 *plain_dev, u32 chain, Ciphertext &ct,
+                       u64 seed, hipStream_t s) {
+    const u64 N = ctx.N();
+    const u32 L = ctx.limbs_at(chain);
+    ct.resize(ctx, chain, 2, s);
+    ct.is_ntt_form = true;
+    u64 *m = static_cast<u64 *>(dev_alloc((size_t)L * N * 8, s));
+    u64 *e = static_cast<u64 *>(dev_alloc((size_t)L * N * 8, s));
+    PHX_HIP_CHECK(hipMemcpyAsync(m, plain_dev, (size_t)L * N * 8, hipMemcpyDeviceToDevice, s));
+    ntt_forward(ctx.tables(), limb_map(m, 0, L), L, 1, s);
+    hipLaunchKernelGGL(k_sample_error, grid_for(N, L), dim3(kThreads), 0, s, e, ctx.dev_mod(), (u32)N, 0u,
+                       seed ^ 0x7777);
+    ntt_forward(ctx.tables(), limb_map(e, 0, L), L, 1, s);
+    hipLaunchKernelGGL(k_rlwe, grid_for(N, L), dim3(kThreads), 0, s, ct.data, ct.data + (u64)L * N,
+                       (const u64 *)sk.s, (const u64 *)e, (const u64 *)m, (const u64 *)nullptr, (const u64 *)nullptr,
+                       0u, 0u, ctx.dev_mod(), (u32)N, seed);
+    PHX_HIP_CHECK(hipGetLastError());
+    dev_free(m, s);
+    dev_free(e, s);
+}
+
+void decrypt(const Context &ctx, SecretKey &sk, const Ciphertext &ct, u64 *out_dev, hipStream_t s) {
+    const u64 N = ctx.N();
+    const u32 L = ct.coeff_modulus_size;
+    require(ct.is_ntt_form, "encrypted must be in NTT form");
+    if (ct.size > 2 && !sk.s2) {
+        const u32 QP = ctx.size_QP();
+        sk.s2 = static_cast<u64 *>(dev_alloc((size_t)QP * N * 8, s));
+        hipLaunchKernelGGL(k_mul_elementwise, grid_for(N, QP), dim3(kThreads), 0, s, (const u64 *)sk.s,
+                           (const u64 *)sk.s, sk.s2, ctx.dev_mod(), (u32)N, 0u);
+    }
+    hipLaunchKernelGGL(k_dot_secret, grid_for(N, L), dim3(kThreads), 0, s, (const u64 *)ct.data, (const u64 *)sk.s,
+                       (const u64 *)sk.s2, out_dev, ctx.dev_mod(), (u32)N, L, ct.size, (u64)ctx.size_QP() * N);
+    PHX_HIP_CHECK(hipGetLastError());
+    ntt_inverse(ctx.tables(), limb_map(out_dev, 0, L), L, 1, s);
+}
+
+}  // namespace phx

@@ -1,0 +1,95 @@
+// ntt.h -- NTT launch interface and the load/store policies that fuse
+// neighbouring element-wise steps into the first/last NTT pass.
+#pragma once
+
+#include "ntt_kernels.h"
+
+namespace phx {
+
+struct LoadPlain {
+    __device__ __forceinline__ u64 operator()(const u64 *d, u32 i, u32, u32, const DevModulus &) const {
+        return d[i];
+    }
+};
+
+struct StorePlain {
+    __device__ __forceinline__ void operator()(u64 *d, u32 i, u64 v, u32, u32, const DevModulus &) const {
+        d[i] = v;
+    }
+};
+
+// Moddown tail fused into the forward NTT of delta (CKKS):
+//   ct[poly][limb] += (cx[poly][limb] - NTT(delta)) * P^-1   (mod q_limb)
+// (reference: inplace_fnwt_radix8_phase2_fuse_moddown, src/ntt/ntt_moddown.cu:106-216,
+//  followed by add_to_ct_kernel, src/rns_bconv.cu:763-769)
+struct StoreModDown {
+    u64 *ct;             // [poly][Ql][N] output
+    const u64 *base0;    // added to poly 0 (may alias ct; null = 0)
+    const u64 *base1;    // added to poly 1 (null = 0)
+    const u64 *cx;       // [poly][QlP][N]
+    const u64 *pinv;     // [Ql][2] (P^-1 mod q_i, Shoup)
+    u64 ct_stride, cx_stride;
+    u32 N;
+    u32 pad;
+    __device__ __forceinline__ void operator()(u64 *, u32 i, u64 v, u32 limb, u32 poly,
+                                               const DevModulus &m) const {
+        const u64 q = m.value;
+        const u64 off = (u64)limb * N + i;
+        const u64 d = sub_mod(cx[poly * cx_stride + off], v, q);
+        const u64 r = mul_shoup(d, pinv[2 * limb], pinv[2 * limb + 1], q);
+        const u64 *base = poly == 0 ? base0 : base1;
+        ct[poly * ct_stride + off] = base ? add_mod(base[off], r, q) : r;
+    }
+};
+
+// Rescale (CKKS, floor): the NTT input for limb j is [c_last]_{q_j} taken from
+// the coefficient-form last limb; the output is (c_j - NTT(.)) * q_last^-1.
+// (reference: DRNSTool::divide_and_round_q_last_ntt, src/rns.cu:1128-1184)
+struct LoadRescale {
+    const u64 *last;     // [poly][N] coefficient form, canonical mod q_last
+    u64 last_stride;
+    __device__ __forceinline__ u64 operator()(const u64 *, u32 i, u32, u32 poly, const DevModulus &m) const {
+        return barrett64(last[poly * last_stride + i], m.value, m.ratio_hi);
+    }
+};
+struct StoreRescale {
+    const u64 *in;       // [poly][Ql][N] input ciphertext (NTT form)
+    u64 *out;            // [poly][Ql-1][N]
+    const u64 *qinv;     // [Ql-1][2] q_last^-1 mod q_j, Shoup
+    u64 in_stride, out_stride;
+    u32 N;
+    __device__ __forceinline__ void operator()(u64 *, u32 i, u64 v, u32 limb, u32 poly,
+                                               const DevModulus &m) const {
+        const u64 q = m.value;
+        const u64 off = (u64)limb * N + i;
+        const u64 d = sub_mod(in[poly * in_stride + off], v, q);
+        out[poly * out_stride + off] = mul_shoup(d, qinv[2 * limb], qinv[2 * limb + 1], q);
+    }
+};
+
+void ntt_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s);
+void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s);
+void ntt_forward_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const StoreModDown &st,
+                       hipStream_t s);
+void ntt_forward_rescale(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const LoadRescale &ld,
+                         const StoreRescale &st, hipStream_t s);
+
+// contiguous limbs [0, L) of one polynomial with rows first_row + i
+inline LimbMap limb_map(u64 *data, u32 first_row, u32 L, u64 poly_stride = 0) {
+    LimbMap m{};
+    m.data = data;
+    m.src = nullptr;
+    m.scale = nullptr;
+    m.poly_stride = poly_stride;
+    m.src_poly_stride = 0;
+    m.first_row = first_row;
+    m.split = L;
+    m.special_row = 0;
+    m.skip_lo = m.skip_hi = L;
+    m.skip_alpha = 0;
+    m.skip_cap = L;
+    m.nlimbs = L;
+    return m;
+}
+
+}  // namespace phx

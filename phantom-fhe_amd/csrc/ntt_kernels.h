@@ -1,0 +1,154 @@
+// ntt_kernels.h -- two-pass negacyclic NTT / INTT for gfx950.
+//
+// A limb of N = R * 256 coefficients is viewed as R rows x 256 columns
+// (index i = r*256 + c). The forward transform (Cooley-Tukey, natural order
+// in, bit-reversed order out -- the reference's output order,
+// src/ntt/fntt_2d.cu:9-198) runs its first log2(R) stages on columns (pass
+// 1) and the last 8 stages on rows (pass 2). The inverse (Gentleman-Sande,
+// src/ntt/intt_2d.cu:9-207) mirrors it: rows first, then columns.
+//
+// Each workgroup owns a 4096-coefficient tile (256 threads x 16 registers).
+// Within a pass, stages are applied in register "rounds" of up to 4 stages
+// (radix-16) with one LDS exchange between rounds; each pass streams its
+// tile from HBM once and back once.
+//
+// Twiddles: forward table tw[row][k] = (psi^brv(k), Shoup), inverse
+// itw[row][k] = (psi^-brv(k), Shoup); stage with m groups and group g uses
+// entry m + g, as in the reference tables (src/host/ntt.cu:11-56).
+#pragma once
+
+#include "common.h"
+
+namespace phx {
+
+constexpr int kNttThreads = 256;
+constexpr int kNttRegs = 16;
+constexpr int kNttTile = kNttThreads * kNttRegs;  // 4096
+
+// Per-launch limb mapping: blockIdx.y selects a limb of the call, blockIdx.z
+// a polynomial (stride poly_stride elements). Limbs [skip_lo, skip_hi) are
+// not launched (the digit's own limbs in modup, already in NTT form).
+struct LimbMap {
+    u64 *data;          // limb 0 of poly 0
+    const u64 *src;     // out-of-place source for the first inverse pass (or null)
+    const u64 *scale;   // per-limb [s0, s0_shoup, s1, s1_shoup] for the final inverse stage (or null)
+    u64 poly_stride;    // elements between polynomials (blockIdx.z)
+    u64 src_poly_stride;
+    u32 first_row;      // table row of limb 0
+    u32 split;          // limbs >= split use special rows
+    u32 special_row;    // table row of limb `split`
+    u32 skip_lo, skip_hi;
+    u32 skip_alpha;     // nonzero: poly z skips limbs [alpha*z, min(alpha*(z+1), skip_cap))
+    u32 skip_cap;
+    u32 nlimbs;         // limbs >= nlimbs are not processed (ragged launches)
+
+    __device__ __forceinline__ u32 limb(u32 y, u32 z) const {
+        u32 lo = skip_lo, hi = skip_hi;
+        if (skip_alpha) {
+            lo = skip_alpha * z;
+            hi = lo + skip_alpha < skip_cap ? lo + skip_alpha : skip_cap;
+        }
+        return y < lo ? y : y + (hi - lo);
+    }
+    __device__ __forceinline__ u32 row(u32 l) const {
+        return l < split ? first_row + l : special_row + (l - split);
+    }
+};
+
+struct NttTables {
+    const TwPair *tw;        // [rows][N]
+    const TwPair *itw;       // [rows][N]
+    const DevModulus *mod;   // [rows]
+    const u64 *inv_consts;   // [rows][4]: n^-1, shoup, n^-1 * itw[1], shoup
+    u32 logN;
+    u32 N;
+};
+
+// ---- register rounds ------------------------------------------------------
+// v holds (16 >> NB) blocks of 2^NB values; value e of block b has global
+// index base[b] + (e << p). A round applies the NB stages whose pair distance
+// is 2^(p+NB-1) ... 2^p. Stage with distance t = 2^(p+jj) has
+// m = N / (2t) groups; the group of index i is i >> (p+jj+1).
+template <int NB>
+__device__ __forceinline__ void fwd_round(u64 (&v)[16], const u32 (&base)[16 >> NB], int p,
+                                          u32 logN, const TwPair *tw, u64 q) {
+    const u64 q2 = q << 1;
+#pragma unroll
+    for (int jj = NB - 1; jj >= 0; --jj) {
+        const int sh = p + jj + 1;
+        const u32 m = 1u << (logN - sh);
+#pragma unroll
+        for (int b = 0; b < (16 >> NB); ++b) {
+            const u32 g0 = m + (base[b] >> sh);
+#pragma unroll
+            for (int e = 0; e < (1 << NB); ++e) {
+                if (e & (1 << jj)) continue;
+                const TwPair w = tw[g0 + (e >> (jj + 1))];
+                ct_bfly(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], w.w, w.ws, q, q2);
+            }
+        }
+    }
+}
+
+template <int NB>
+__device__ __forceinline__ void inv_round(u64 (&v)[16], const u32 (&base)[16 >> NB], int p,
+                                          u32 logN, const TwPair *itw, u64 q) {
+    const u64 q2 = q << 1;
+#pragma unroll
+    for (int jj = 0; jj < NB; ++jj) {
+        const int sh = p + jj + 1;
+        const u32 m = 1u << (logN - sh);
+#pragma unroll
+        for (int b = 0; b < (16 >> NB); ++b) {
+            const u32 g0 = m + (base[b] >> sh);
+#pragma unroll
+            for (int e = 0; e < (1 << NB); ++e) {
+                if (e & (1 << jj)) continue;
+                const TwPair w = itw[g0 + (e >> (jj + 1))];
+                gs_bfly(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], w.w, w.ws, q, q2);
+            }
+        }
+    }
+}
+
+// Last inverse round: same as inv_round but the final stage (distance N/2,
+// m = 1) multiplies both outputs into canonical form by the per-limb
+// constants s0 (x output: n^-1 * scale) and s1 (y output: n^-1 * psi^-brv(1)
+// * scale). Only valid when the round contains the top stage.
+template <int NB>
+__device__ __forceinline__ void inv_round_last(u64 (&v)[16], const u32 (&base)[16 >> NB], int p,
+                                               u32 logN, const TwPair *itw, u64 q, u64 s0,
+                                               u64 s0s, u64 s1, u64 s1s) {
+    const u64 q2 = q << 1;
+#pragma unroll
+    for (int jj = 0; jj < NB - 1; ++jj) {
+        const int sh = p + jj + 1;
+        const u32 m = 1u << (logN - sh);
+#pragma unroll
+        for (int b = 0; b < (16 >> NB); ++b) {
+            const u32 g0 = m + (base[b] >> sh);
+#pragma unroll
+            for (int e = 0; e < (1 << NB); ++e) {
+                if (e & (1 << jj)) continue;
+                const TwPair w = itw[g0 + (e >> (jj + 1))];
+                gs_bfly(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], w.w, w.ws, q, q2);
+            }
+        }
+    }
+    constexpr int jj = NB - 1;
+#pragma unroll
+    for (int b = 0; b < (16 >> NB); ++b) {
+#pragma unroll
+        for (int e = 0; e < (1 << NB); ++e) {
+            if (e & (1 << jj)) continue;
+            u64 &x = v[b * (1 << NB) + e];
+            u64 &y = v[b * (1 << NB) + e + (1 << jj)];
+            const u64 sum = x + y;          // < 4q
+            const u64 dif = x + q2 - y;     // < 4q
+            x = mul_shoup(sum, s0, s0s, q);
+            y = mul_shoup(dif, s1, s1s, q);
+        }
+    }
+}
+
+}  // namespace phx

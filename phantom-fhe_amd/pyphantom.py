@@ -1,0 +1,509 @@
+"""pyphantom -- Python binding of libphantom_amd.so over its C ABI.
+
+Mirrors the names of the reference's pybind11 module (python/src/binding.cu:
+context, ciphertext, multiply_and_relin, relinearize, rescale_to_next,
+apply_galois, rotate) and the reference's exception types: a failed call
+raises ValueError (std::invalid_argument), RuntimeError otherwise.
+
+The module loads the in-tree HIP library and fails loudly when it is absent;
+there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libphantom_amd.so")
+
+SCHEME_BGV, SCHEME_BFV, SCHEME_CKKS = 1, 2, 3
+MUL_TECH_NONE, MUL_TECH_BEHZ, MUL_TECH_HPS = 0, 1, 2
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_lib = None
+
+
+class PhxError(RuntimeError):
+    pass
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [
+        ("scheme", ctypes.c_int),
+        ("mul_tech", ctypes.c_int),
+        ("poly_modulus_degree", ctypes.c_uint64),
+        ("coeff_modulus", _u64p),
+        ("coeff_modulus_size", _sz),
+        ("special_modulus_size", _sz),
+        ("plain_modulus", ctypes.c_uint64),
+        ("galois_elts", _u32p),
+        ("galois_elts_size", _sz),
+    ]
+
+
+_SIGS = {
+    "phx_last_error": (ctypes.c_char_p, []),
+    "phx_coeff_modulus_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), _sz, _u64p]),
+    "phx_galois_elt_from_step": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, _u32p]),
+    "phx_context_create": (ctypes.c_int, [ctypes.POINTER(_Params), ctypes.POINTER(_vp)]),
+    "phx_context_destroy": (ctypes.c_int, [_vp]),
+    "phx_context_info": (ctypes.c_int, [_vp, _u64p, ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
+    "phx_context_limbs_at": (ctypes.c_int, [_vp, _sz, ctypes.POINTER(_sz)]),
+    "phx_ciphertext_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "phx_ciphertext_destroy": (ctypes.c_int, [_vp]),
+    "phx_ciphertext_resize": (ctypes.c_int, [_vp, _vp, _sz, _sz, _vp]),
+    "phx_ciphertext_upload": (ctypes.c_int, [_vp, _u64p, _sz, _vp]),
+    "phx_ciphertext_download": (ctypes.c_int, [_vp, _u64p, _sz, _vp]),
+    "phx_ciphertext_copy": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "phx_ciphertext_info": (ctypes.c_int, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz), ctypes.POINTER(_sz),
+                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
+    "phx_ciphertext_set_scale": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "phx_ciphertext_data": (ctypes.c_int, [_vp, ctypes.POINTER(_u64p)]),
+    "phx_secret_key_create": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "phx_secret_key_create_from_coeffs": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int8), ctypes.POINTER(_vp)]),
+    "phx_secret_key_download": (ctypes.c_int, [_vp, _vp, _u64p]),
+    "phx_secret_key_destroy": (ctypes.c_int, [_vp]),
+    "phx_relin_key_create": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "phx_relin_key_create_from_host": (ctypes.c_int, [_vp, _u64p, _sz, ctypes.POINTER(_vp)]),
+    "phx_relin_key_download": (ctypes.c_int, [_vp, _vp, _u64p]),
+    "phx_relin_key_destroy": (ctypes.c_int, [_vp]),
+    "phx_galois_key_create": (ctypes.c_int, [_vp, _vp, _u32p, _sz, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "phx_galois_key_create_from_host": (ctypes.c_int, [_vp, _u32p, _sz, _u64p, _sz, ctypes.POINTER(_vp)]),
+    "phx_galois_key_destroy": (ctypes.c_int, [_vp]),
+    "phx_encrypt_symmetric": (ctypes.c_int, [_vp, _vp, _u64p, _sz, ctypes.c_uint64, _vp, _vp]),
+    "phx_decrypt": (ctypes.c_int, [_vp, _vp, _vp, _u64p, _vp]),
+    "phx_multiply_and_relin_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "phx_multiply_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "phx_relinearize_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "phx_rescale_to_next": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "phx_apply_galois_inplace": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "phx_rotate_inplace": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
+    "phx_keyswitch_inplace": (ctypes.c_int, [_vp, _vp, _u64p, _vp, ctypes.c_int, _vp]),
+    "phx_ntt_forward": (ctypes.c_int, [_vp, _u64p, _sz, _sz, _vp]),
+    "phx_ntt_inverse": (ctypes.c_int, [_vp, _u64p, _sz, _sz, _vp]),
+    "phx_ntt_forward_batch": (ctypes.c_int, [_vp, _u64p, _sz, _sz, _sz, _sz, _vp]),
+    "phx_ntt_inverse_batch": (ctypes.c_int, [_vp, _u64p, _sz, _sz, _sz, _sz, _vp]),
+    "phx_modup": (ctypes.c_int, [_vp, _sz, _u64p, _u64p, _vp]),
+    "phx_key_inner_product": (ctypes.c_int, [_vp, _sz, _u64p, _vp, _u64p, _vp]),
+    "phx_moddown_from_ntt": (ctypes.c_int, [_vp, _sz, _u64p, _u64p, _sz, _vp]),
+    "phx_malloc": (ctypes.c_int, [_sz, ctypes.POINTER(_vp)]),
+    "phx_free": (ctypes.c_int, [_vp]),
+    "phx_memcpy_htod": (ctypes.c_int, [_vp, _vp, _sz, _vp]),
+    "phx_memcpy_dtoh": (ctypes.c_int, [_vp, _vp, _sz, _vp]),
+    "phx_stream_synchronize": (ctypes.c_int, [_vp]),
+    "phx_device_synchronize": (ctypes.c_int, []),
+    "phx_stream_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "phx_stream_destroy": (ctypes.c_int, [_vp]),
+    "phx_event_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "phx_event_record": (ctypes.c_int, [_vp, _vp]),
+    "phx_event_elapsed_ms": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
+    "phx_event_destroy": (ctypes.c_int, [_vp]),
+    "phx_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+}
+
+
+def exported_symbols():
+    return list(_SIGS.keys())
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                              "(make -C phantom-fhe_amd); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().phx_last_error().decode()
+        if rc == -1:
+            raise ValueError(msg)
+        raise PhxError(msg)
+
+
+def _p(a):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_u64p)
+
+
+def _stream(s):
+    return None if s is None else s
+
+
+# ------------------------------------------------------------------ params
+def coeff_modulus_create(N, bit_sizes):
+    """CoeffModulus::Create (src/host/modulus.cu:80-111)."""
+    b = (ctypes.c_int * len(bit_sizes))(*bit_sizes)
+    out = np.zeros(len(bit_sizes), np.uint64)
+    _check(lib().phx_coeff_modulus_create(N, b, len(bit_sizes), _p(out)))
+    return [int(x) for x in out]
+
+
+def galois_elt_from_step(step, N):
+    out = ctypes.c_uint32()
+    _check(lib().phx_galois_elt_from_step(step, N, ctypes.byref(out)))
+    return out.value
+
+
+class EncryptionParameters:
+    def __init__(self, scheme=SCHEME_CKKS):
+        self.scheme = scheme
+        self.mul_tech = MUL_TECH_HPS if scheme == SCHEME_BFV else MUL_TECH_NONE
+        self.poly_modulus_degree = 0
+        self.coeff_modulus = []
+        self.special_modulus_size = 1
+        self.plain_modulus = 0
+        self.galois_elts = []
+
+    def set_poly_modulus_degree(self, n):
+        self.poly_modulus_degree = n
+
+    def set_coeff_modulus(self, m):
+        self.coeff_modulus = [int(x) for x in m]
+
+    def set_special_modulus_size(self, s):
+        self.special_modulus_size = s
+
+    def set_galois_elts(self, e):
+        self.galois_elts = list(e)
+
+    def set_mul_tech(self, t):
+        self.mul_tech = t
+
+    def set_plain_modulus(self, t):
+        self.plain_modulus = t
+
+
+class context:
+    """PhantomContext (include/context.cuh:133-273)."""
+
+    def __init__(self, parms):
+        self._mods = np.array(parms.coeff_modulus, np.uint64)
+        self._elts = np.array(parms.galois_elts or [0], np.uint32)
+        p = _Params(parms.scheme, parms.mul_tech, parms.poly_modulus_degree, _p(self._mods), len(self._mods),
+                    parms.special_modulus_size, parms.plain_modulus,
+                    self._elts.ctypes.data_as(_u32p) if parms.galois_elts else None, len(parms.galois_elts))
+        h = _vp()
+        _check(lib().phx_context_create(ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+        self.coeff_modulus = [int(x) for x in parms.coeff_modulus]
+        N = ctypes.c_uint64()
+        sq, sp, dn = _sz(), _sz(), _sz()
+        _check(lib().phx_context_info(h, ctypes.byref(N), ctypes.byref(sq), ctypes.byref(sp), ctypes.byref(dn)))
+        self.N, self.size_Q, self.size_P, self.dnum = N.value, sq.value, sp.value, dn.value
+        self.size_QP = self.size_Q + self.size_P
+
+    def limbs_at(self, chain_index):
+        out = _sz()
+        _check(lib().phx_context_limbs_at(self._h, chain_index, ctypes.byref(out)))
+        return out.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.phx_context_destroy(self._h)
+            self._h = None
+
+
+class ciphertext:
+    """PhantomCiphertext (include/ciphertext.h:7-308)."""
+
+    def __init__(self, ctx=None, chain_index=None, size=2, stream=None):
+        h = _vp()
+        _check(lib().phx_ciphertext_create(ctypes.byref(h)))
+        self._h = h
+        if ctx is not None and chain_index is not None:
+            self.resize(ctx, chain_index, size, stream)
+
+    def resize(self, ctx, chain_index, size, stream=None):
+        _check(lib().phx_ciphertext_resize(ctx._h, self._h, chain_index, size, _stream(stream)))
+
+    def info(self):
+        c, s, L = _sz(), _sz(), _sz()
+        sc = ctypes.c_double()
+        ntt = ctypes.c_int()
+        _check(lib().phx_ciphertext_info(self._h, ctypes.byref(c), ctypes.byref(s), ctypes.byref(L),
+                                         ctypes.byref(sc), ctypes.byref(ntt)))
+        return dict(chain_index=c.value, size=s.value, coeff_modulus_size=L.value, scale=sc.value,
+                    is_ntt_form=bool(ntt.value))
+
+    def chain_index(self):
+        return self.info()["chain_index"]
+
+    def size(self):
+        return self.info()["size"]
+
+    def coeff_modulus_size(self):
+        return self.info()["coeff_modulus_size"]
+
+    def scale(self):
+        return self.info()["scale"]
+
+    def set_scale(self, v):
+        _check(lib().phx_ciphertext_set_scale(self._h, v))
+
+    def upload(self, arr, stream=None):
+        a = np.ascontiguousarray(arr, np.uint64)
+        _check(lib().phx_ciphertext_upload(self._h, _p(a), a.size, _stream(stream)))
+
+    def download(self, N, stream=None):
+        i = self.info()
+        out = np.zeros((i["size"], i["coeff_modulus_size"], N), np.uint64)
+        _check(lib().phx_ciphertext_download(self._h, _p(out), out.size, _stream(stream)))
+        return out
+
+    def data_ptr(self):
+        p = _u64p()
+        _check(lib().phx_ciphertext_data(self._h, ctypes.byref(p)))
+        return p
+
+    def copy_from(self, ctx, other, stream=None):
+        _check(lib().phx_ciphertext_copy(ctx._h, self._h, other._h, _stream(stream)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.phx_ciphertext_destroy(self._h)
+            self._h = None
+
+
+class secret_key:
+    """PhantomSecretKey (include/secretkey.h): ternary secret, NTT form."""
+
+    def __init__(self, ctx, seed=1, coeffs=None):
+        h = _vp()
+        if coeffs is not None:
+            c = np.ascontiguousarray(coeffs, np.int8)
+            _check(lib().phx_secret_key_create_from_coeffs(ctx._h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                                                           ctypes.byref(h)))
+        else:
+            _check(lib().phx_secret_key_create(ctx._h, seed, ctypes.byref(h)))
+        self._h = h
+        self._ctx = ctx
+
+    def ntt_form(self):
+        out = np.zeros((self._ctx.size_QP, self._ctx.N), np.uint64)
+        _check(lib().phx_secret_key_download(self._ctx._h, self._h, _p(out)))
+        return out
+
+    def gen_relinkey(self, ctx, seed=2):
+        return relin_key(ctx, sk=self, seed=seed)
+
+    def create_galois_keys(self, ctx, elts, seed=3):
+        return galois_key(ctx, elts, sk=self, seed=seed)
+
+    def encrypt_symmetric(self, ctx, plain_limbs, chain_index, seed=4, stream=None):
+        a = np.ascontiguousarray(plain_limbs, np.uint64)
+        ct = ciphertext()
+        _check(lib().phx_encrypt_symmetric(ctx._h, self._h, _p(a), chain_index, seed, ct._h, _stream(stream)))
+        return ct
+
+    def decrypt(self, ctx, ct, stream=None):
+        L = ct.coeff_modulus_size()
+        out = np.zeros((L, ctx.N), np.uint64)
+        _check(lib().phx_decrypt(ctx._h, self._h, ct._h, _p(out), _stream(stream)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.phx_secret_key_destroy(self._h)
+            self._h = None
+
+
+class relin_key:
+    """PhantomRelinKey: [dnum][2][QP][N]."""
+
+    def __init__(self, ctx, sk=None, seed=2, host=None):
+        h = _vp()
+        if host is not None:
+            a = np.ascontiguousarray(host, np.uint64)
+            _check(lib().phx_relin_key_create_from_host(ctx._h, _p(a), a.shape[0], ctypes.byref(h)))
+            self.dnum = a.shape[0]
+        else:
+            _check(lib().phx_relin_key_create(ctx._h, sk._h, seed, ctypes.byref(h)))
+            self.dnum = ctx.dnum
+        self._h = h
+        self._ctx = ctx
+
+    def download(self):
+        out = np.zeros((self.dnum, 2, self._ctx.size_QP, self._ctx.N), np.uint64)
+        _check(lib().phx_relin_key_download(self._ctx._h, self._h, _p(out)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.phx_relin_key_destroy(self._h)
+            self._h = None
+
+
+class galois_key:
+    """PhantomGaloisKey: one key-switching key per Galois element."""
+
+    def __init__(self, ctx, elts, sk=None, seed=3, host=None):
+        e = np.array(elts, np.uint32)
+        h = _vp()
+        if host is not None:
+            a = np.ascontiguousarray(host, np.uint64)
+            _check(lib().phx_galois_key_create_from_host(ctx._h, e.ctypes.data_as(_u32p), len(e), _p(a),
+                                                         a.shape[1], ctypes.byref(h)))
+        else:
+            _check(lib().phx_galois_key_create(ctx._h, sk._h, e.ctypes.data_as(_u32p), len(e), seed,
+                                               ctypes.byref(h)))
+        self._h = h
+        self.elts = [int(x) for x in elts]
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.phx_galois_key_destroy(self._h)
+            self._h = None
+
+
+# ------------------------------------------------------------------ evaluator
+def multiply_and_relin_inplace(ctx, ct1, ct2, rlk, stream=None):
+    _check(lib().phx_multiply_and_relin_inplace(ctx._h, ct1._h, ct2._h, rlk._h, _stream(stream)))
+
+
+def multiply_and_relin(ctx, ct1, ct2, rlk, stream=None):
+    out = ciphertext()
+    out.copy_from(ctx, ct1, stream)
+    multiply_and_relin_inplace(ctx, out, ct2, rlk, stream)
+    return out
+
+
+def multiply_inplace(ctx, ct1, ct2, stream=None):
+    _check(lib().phx_multiply_inplace(ctx._h, ct1._h, ct2._h, _stream(stream)))
+
+
+def relinearize_inplace(ctx, ct, rlk, stream=None):
+    _check(lib().phx_relinearize_inplace(ctx._h, ct._h, rlk._h, _stream(stream)))
+
+
+def rescale_to_next(ctx, ct, stream=None):
+    out = ciphertext()
+    _check(lib().phx_rescale_to_next(ctx._h, ct._h, out._h, _stream(stream)))
+    return out
+
+
+def rescale_to_next_inplace(ctx, ct, stream=None):
+    _check(lib().phx_rescale_to_next(ctx._h, ct._h, ct._h, _stream(stream)))
+
+
+def apply_galois_inplace(ctx, ct, elt, gk, stream=None):
+    _check(lib().phx_apply_galois_inplace(ctx._h, ct._h, elt, gk._h, _stream(stream)))
+
+
+def rotate_inplace(ctx, ct, step, gk, stream=None):
+    _check(lib().phx_rotate_inplace(ctx._h, ct._h, step, gk._h, _stream(stream)))
+
+
+def keyswitch_inplace(ctx, ct, c2_dev, key, is_relin=True, stream=None):
+    _check(lib().phx_keyswitch_inplace(ctx._h, ct._h, c2_dev, key._h, int(is_relin), _stream(stream)))
+
+
+# ------------------------------------------------------------------ device buffers
+class DeviceBuffer:
+    """Plain device allocation of u64 elements (hipMalloc)."""
+
+    def __init__(self, n_elems):
+        h = _vp()
+        _check(lib().phx_malloc(max(1, n_elems) * 8, ctypes.byref(h)))
+        self._h = h
+        self.n = n_elems
+
+    @classmethod
+    def from_array(cls, arr, stream=None):
+        a = np.ascontiguousarray(arr, np.uint64)
+        b = cls(a.size)
+        _check(lib().phx_memcpy_htod(b._h, a.ctypes.data_as(_vp), a.size * 8, _stream(stream)))
+        return b
+
+    def ptr(self):
+        return ctypes.cast(self._h, _u64p)
+
+    def to_array(self, shape=None, stream=None):
+        out = np.zeros(self.n, np.uint64)
+        _check(lib().phx_memcpy_dtoh(out.ctypes.data_as(_vp), self._h, self.n * 8, _stream(stream)))
+        return out.reshape(shape) if shape is not None else out
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.phx_device_synchronize()
+            _lib.phx_free(self._h)
+            self._h = None
+
+
+def ntt_forward(ctx, buf, L, start=0, stream=None):
+    _check(lib().phx_ntt_forward(ctx._h, buf.ptr(), L, start, _stream(stream)))
+
+
+def ntt_inverse(ctx, buf, L, start=0, stream=None):
+    _check(lib().phx_ntt_inverse(ctx._h, buf.ptr(), L, start, _stream(stream)))
+
+
+def ntt_forward_batch(ctx, buf, L, start, npolys, poly_stride, stream=None):
+    _check(lib().phx_ntt_forward_batch(ctx._h, buf.ptr(), L, start, npolys, poly_stride, _stream(stream)))
+
+
+def ntt_inverse_batch(ctx, buf, L, start, npolys, poly_stride, stream=None):
+    _check(lib().phx_ntt_inverse_batch(ctx._h, buf.ptr(), L, start, npolys, poly_stride, _stream(stream)))
+
+
+def stream_create():
+    h = _vp()
+    _check(lib().phx_stream_create(ctypes.byref(h)))
+    return h
+
+
+def modup(ctx, chain_index, c2_buf, t_buf, stream=None):
+    _check(lib().phx_modup(ctx._h, chain_index, c2_buf.ptr(), t_buf.ptr(), _stream(stream)))
+
+
+def key_inner_product(ctx, chain_index, t_buf, key, cx_buf, stream=None):
+    _check(lib().phx_key_inner_product(ctx._h, chain_index, t_buf.ptr(), key._h, cx_buf.ptr(), _stream(stream)))
+
+
+def moddown_from_ntt(ctx, chain_index, cx_buf, out_buf, npolys=1, stream=None):
+    _check(lib().phx_moddown_from_ntt(ctx._h, chain_index, cx_buf.ptr(), out_buf.ptr(), npolys, _stream(stream)))
+
+
+def synchronize(stream=None):
+    _check(lib().phx_stream_synchronize(_stream(stream)))
+
+
+def device_synchronize():
+    _check(lib().phx_device_synchronize())
+
+
+class Event:
+    def __init__(self):
+        h = _vp()
+        _check(lib().phx_event_create(ctypes.byref(h)))
+        self._h = h
+
+    def record(self, stream=None):
+        _check(lib().phx_event_record(self._h, _stream(stream)))
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float()
+        _check(lib().phx_event_elapsed_ms(self._h, end._h, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.phx_event_destroy(self._h)
+            self._h = None
+
+
+def device_count():
+    n = ctypes.c_int()
+    rc = lib().phx_device_count(ctypes.byref(n))
+    return n.value if rc == 0 else 0

@@ -1,0 +1,216 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: only tests/, the smoke entry and the benchmark's CPU
+baseline import this module. It loads the oracle as the checker.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_sz = ctypes.c_size_t
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_ctx_create.restype = ctypes.c_void_p
+        L.orc_ctx_create.argtypes = [ctypes.c_int, _u64p, _sz, _sz]
+        L.orc_ctx_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_ntt_forward.argtypes = [ctypes.c_void_p, _u64p, _sz, _sz]
+        L.orc_ntt_inverse.argtypes = [ctypes.c_void_p, _u64p, _sz, _sz]
+        L.orc_get_primes.argtypes = [ctypes.c_uint64, ctypes.c_int, _sz, _u64p]
+        L.orc_coeff_modulus_create.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), _sz, _u64p]
+        L.orc_minimal_root.restype = ctypes.c_uint64
+        L.orc_minimal_root.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.orc_is_prime.argtypes = [ctypes.c_uint64]
+        L.orc_shoup.restype = ctypes.c_uint64
+        L.orc_shoup.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.orc_ntt_tables.argtypes = [ctypes.c_uint64, ctypes.c_int, _u64p, _u64p, _u64p, _u64p, _u64p]
+        L.orc_ctx_root_powers.restype = _u64p
+        L.orc_ctx_root_powers.argtypes = [ctypes.c_void_p, _sz]
+        L.orc_fastbconv.argtypes = [_sz, _u64p, _sz, _u64p, _sz, _u64p, _u64p]
+        L.orc_beta.restype = _sz
+        L.orc_beta.argtypes = [ctypes.c_void_p, _sz]
+        L.orc_modup.argtypes = [ctypes.c_void_p, _sz, _u64p, _u64p]
+        L.orc_inner_product.argtypes = [ctypes.c_void_p, _sz, _u64p, _u64p, _sz, _u64p]
+        L.orc_moddown.argtypes = [ctypes.c_void_p, _sz, _u64p, _u64p]
+        L.orc_keyswitch.argtypes = [ctypes.c_void_p, _sz, _u64p, _u64p, _sz, _u64p]
+        L.orc_tensor.argtypes = [ctypes.c_void_p, _sz, _u64p, _u64p, _u64p]
+        L.orc_multiply_and_relin.argtypes = [ctypes.c_void_p, _sz, _u64p, _u64p, _u64p, _sz, _u64p]
+        L.orc_rescale.argtypes = [ctypes.c_void_p, _sz, _u64p, _sz, _u64p]
+        L.orc_galois_elt_from_step.restype = ctypes.c_uint32
+        L.orc_galois_elt_from_step.argtypes = [ctypes.c_int, _sz]
+        L.orc_galois_perm_table.argtypes = [ctypes.c_int, ctypes.c_uint32, _u32p]
+        L.orc_apply_galois.argtypes = [ctypes.c_void_p, _sz, _u64p, ctypes.c_uint32, _u64p, _sz, _u64p]
+        L.orc_make_kswitch_key.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u64p, _u64p, _sz, _u64p]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(_u64p)
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def get_primes(N, bits, count):
+    out = np.zeros(count, np.uint64)
+    assert lib().orc_get_primes(N, bits, count, _p(out)) == 0
+    return [int(x) for x in out]
+
+
+def coeff_modulus_create(N, bits):
+    b = (ctypes.c_int * len(bits))(*bits)
+    out = np.zeros(len(bits), np.uint64)
+    assert lib().orc_coeff_modulus_create(N, b, len(bits), _p(out)) == 0
+    return [int(x) for x in out]
+
+
+def minimal_root(q, degree):
+    return int(lib().orc_minimal_root(q, degree))
+
+
+def is_prime(v):
+    return bool(lib().orc_is_prime(v))
+
+
+def ntt_tables(q, logN):
+    N = 1 << logN
+    rp, rps, irp, irps = (np.zeros(N, np.uint64) for _ in range(4))
+    ninv = np.zeros(1, np.uint64)
+    lib().orc_ntt_tables(q, logN, _p(rp), _p(rps), _p(irp), _p(irps), _p(ninv))
+    return rp, rps, irp, irps, int(ninv[0])
+
+
+def galois_elt_from_step(step, N):
+    return int(lib().orc_galois_elt_from_step(step, N))
+
+
+def galois_perm_table(logN, elt):
+    out = np.zeros(1 << logN, np.uint32)
+    lib().orc_galois_perm_table(logN, elt, out.ctypes.data_as(_u32p))
+    return out
+
+
+def fastbconv(N, ibase, obase, x):
+    ib = np.array(ibase, np.uint64)
+    ob = np.array(obase, np.uint64)
+    x = np.ascontiguousarray(x, np.uint64)
+    out = np.zeros((len(obase), N), np.uint64)
+    lib().orc_fastbconv(N, _p(ib), len(ibase), _p(ob), len(obase), _p(x), _p(out))
+    return out
+
+
+class Ctx:
+    """Oracle context over the QP chain (size_Q data primes then size_P special)."""
+
+    def __init__(self, logN, moduli, size_P):
+        self.logN = logN
+        self.N = 1 << logN
+        self.moduli = [int(m) for m in moduli]
+        self.size_QP = len(moduli)
+        self.size_P = size_P
+        self.size_Q = self.size_QP - size_P
+        m = np.array(self.moduli, np.uint64)
+        self._h = lib().orc_ctx_create(logN, _p(m), self.size_QP, size_P)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_ctx_destroy(self._h)
+            self._h = None
+
+    def beta(self, size_Ql):
+        return int(lib().orc_beta(self._h, size_Ql))
+
+    def ntt_forward(self, data, start=0):
+        a = np.ascontiguousarray(data, np.uint64).copy()
+        L = a.size // self.N
+        lib().orc_ntt_forward(self._h, _p(a), L, start)
+        return a
+
+    def ntt_inverse(self, data, start=0):
+        a = np.ascontiguousarray(data, np.uint64).copy()
+        L = a.size // self.N
+        lib().orc_ntt_inverse(self._h, _p(a), L, start)
+        return a
+
+    def root_powers(self, i):
+        ptr = lib().orc_ctx_root_powers(self._h, i)
+        return np.ctypeslib.as_array(ptr, shape=(self.N,)).copy()
+
+    def modup(self, size_Ql, c2):
+        beta = self.beta(size_Ql)
+        out = np.zeros((beta, size_Ql + self.size_P, self.N), np.uint64)
+        c2 = np.ascontiguousarray(c2, np.uint64)
+        assert lib().orc_modup(self._h, size_Ql, _p(c2), _p(out)) == 0
+        return out
+
+    def inner_product(self, size_Ql, t, key):
+        dnum = key.shape[0]
+        out = np.zeros((2, size_Ql + self.size_P, self.N), np.uint64)
+        assert lib().orc_inner_product(self._h, size_Ql, _p(np.ascontiguousarray(t)), _p(key), dnum, _p(out)) == 0
+        return out
+
+    def moddown(self, size_Ql, cx_i):
+        out = np.zeros((size_Ql, self.N), np.uint64)
+        assert lib().orc_moddown(self._h, size_Ql, _p(np.ascontiguousarray(cx_i)), _p(out)) == 0
+        return out
+
+    def keyswitch(self, size_Ql, c2, key):
+        out = np.zeros((2, size_Ql, self.N), np.uint64)
+        rc = lib().orc_keyswitch(self._h, size_Ql, _p(np.ascontiguousarray(c2)), _p(key), key.shape[0], _p(out))
+        assert rc == 0
+        return out
+
+    def tensor(self, size_Ql, ct1, ct2):
+        out = np.zeros((3, size_Ql, self.N), np.uint64)
+        lib().orc_tensor(self._h, size_Ql, _p(np.ascontiguousarray(ct1)), _p(np.ascontiguousarray(ct2)), _p(out))
+        return out
+
+    def multiply_and_relin(self, size_Ql, ct1, ct2, key):
+        out = np.zeros((2, size_Ql, self.N), np.uint64)
+        rc = lib().orc_multiply_and_relin(self._h, size_Ql, _p(np.ascontiguousarray(ct1)),
+                                          _p(np.ascontiguousarray(ct2)), _p(key), key.shape[0], _p(out))
+        assert rc == 0
+        return out
+
+    def rescale(self, size_Ql, ct):
+        ct = np.ascontiguousarray(ct, np.uint64)
+        size = ct.shape[0]
+        out = np.zeros((size, size_Ql - 1, self.N), np.uint64)
+        assert lib().orc_rescale(self._h, size_Ql, _p(ct), size, _p(out)) == 0
+        return out
+
+    def apply_galois(self, size_Ql, ct, elt, key):
+        out = np.zeros((2, size_Ql, self.N), np.uint64)
+        rc = lib().orc_apply_galois(self._h, size_Ql, _p(np.ascontiguousarray(ct)), elt, _p(key),
+                                    key.shape[0], _p(out))
+        assert rc == 0
+        return out
+
+    def make_kswitch_key(self, s_ntt, s2_ntt, a, e):
+        dnum = a.shape[0]
+        key = np.zeros((dnum, 2, self.size_QP, self.N), np.uint64)
+        lib().orc_make_kswitch_key(self._h, _p(np.ascontiguousarray(s_ntt)), _p(np.ascontiguousarray(s2_ntt)),
+                                   _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(e)), dnum, _p(key))
+        return key

@@ -1,0 +1,54 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol that
+include/phantom_amd.h declares, and its host-only entry points agree with the
+oracle. No GPU call is made here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import pyphantom as ph
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "phantom_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(phx_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_api():
+    syms = declared_symbols()
+    for must in ["phx_multiply_and_relin_inplace", "phx_rotate_inplace", "phx_keyswitch_inplace",
+                 "phx_ntt_forward", "phx_ntt_inverse", "phx_modup", "phx_moddown_from_ntt",
+                 "phx_context_create", "phx_rescale_to_next", "phx_apply_galois_inplace"]:
+        assert must in syms
+
+
+def test_library_exports_every_declared_symbol():
+    L = ph.lib()
+    for s in declared_symbols():
+        assert hasattr(L, s), s
+    assert set(declared_symbols()) == set(ph.exported_symbols())
+
+
+def test_coeff_modulus_create_matches_oracle():
+    for N, bits in [(4096, [60, 60, 60]), (65536, [60] + [50] * 34 + [60] * 12), (8192, [54] * 5)]:
+        assert ph.coeff_modulus_create(N, bits) == ob.coeff_modulus_create(N, bits)
+
+
+def test_galois_elt_matches_oracle():
+    for N in (4096, 65536):
+        for step in [0, 1, 2, 7, -1, -5, 100]:
+            assert ph.galois_elt_from_step(step, N) == ob.galois_elt_from_step(step, N)
+
+
+def test_invalid_arguments_raise_value_error():
+    with pytest.raises(ValueError):
+        ph.coeff_modulus_create(1000, [60])
+    with pytest.raises(ValueError):
+        ph.galois_elt_from_step(1 << 20, 4096)
+    assert "step count too large" in ph.lib().phx_last_error().decode()

@@ -1,0 +1,199 @@
+"""GPU parity of modup / key inner product / moddown / keyswitch /
+multiply_and_relin / rescale / apply_galois against the oracle, bit-exact
+on every output limb, on seeded uniform inputs."""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import pyphantom as ph
+from gpu_helpers import make_ctx, rand_ct, rand_key, rand_limbs, say, upload_ct
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(N=4096, bits=[60, 50, 50, 50, 50, 50, 60, 60], P=2)   # 6 data primes, alpha 2
+
+
+@pytest.fixture(scope="module")
+def small():
+    ob.set_threads(8)
+    ctx, octx, mods = make_ctx(SMALL["N"], SMALL["bits"], SMALL["P"])
+    rng = np.random.default_rng(100)
+    key = rand_key(rng, mods, SMALL["N"], ctx.dnum)
+    return ctx, octx, mods, key
+
+
+@pytest.mark.parametrize("chain", [1, 2, 3, 6])
+def test_modup_inner_moddown_pieces(small, chain):
+    ctx, octx, mods, key = small
+    N = ctx.N
+    Ql = ctx.limbs_at(chain)
+    say(f"case: pieces N={N} chain={chain} Ql={Ql}")
+    rng = np.random.default_rng(chain)
+    c2 = rand_limbs(rng, mods[:Ql], N)
+    beta = octx.beta(Ql)
+    QlP = Ql + ctx.size_P
+    c2b = ph.DeviceBuffer.from_array(c2)
+    tb = ph.DeviceBuffer(beta * QlP * N)
+    ph.modup(ctx, chain, c2b, tb)
+    ph.device_synchronize()
+    t = tb.to_array((beta, QlP, N))
+    assert np.array_equal(t, octx.modup(Ql, c2))
+    rk = ph.relin_key(ctx, host=key)
+    cxb = ph.DeviceBuffer(2 * QlP * N)
+    ph.key_inner_product(ctx, chain, tb, rk, cxb)
+    ph.device_synchronize()
+    cx = cxb.to_array((2, QlP, N))
+    assert np.array_equal(cx, octx.inner_product(Ql, t, key))
+    outb = ph.DeviceBuffer(2 * Ql * N)
+    ph.moddown_from_ntt(ctx, chain, cxb, outb, 2)
+    ph.device_synchronize()
+    out = outb.to_array((2, Ql, N))
+    assert np.array_equal(out[0], octx.moddown(Ql, cx[0]))
+    assert np.array_equal(out[1], octx.moddown(Ql, cx[1]))
+
+
+@pytest.mark.parametrize("chain", [1, 4, 6])
+def test_multiply_and_relin_small(small, chain):
+    ctx, octx, mods, key = small
+    N = ctx.N
+    Ql = ctx.limbs_at(chain)
+    say(f"case: mulrelin N={N} chain={chain}")
+    rng = np.random.default_rng(10 + chain)
+    a, b = rand_ct(rng, mods[:Ql], N), rand_ct(rng, mods[:Ql], N)
+    rk = ph.relin_key(ctx, host=key)
+    c1, c2 = upload_ct(ctx, a, chain), upload_ct(ctx, b, chain)
+    ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
+    got = c1.download(N)
+    assert got.shape == (2, Ql, N)
+    assert np.array_equal(got, octx.multiply_and_relin(Ql, a, b, key))
+
+
+def test_square_aliased_operands(small):
+    ctx, octx, mods, key = small
+    N = ctx.N
+    say("case: square (ct1 is ct2)")
+    rng = np.random.default_rng(77)
+    a = rand_ct(rng, mods[:6], N)
+    rk = ph.relin_key(ctx, host=key)
+    c1 = upload_ct(ctx, a, 1)
+    ph.multiply_and_relin_inplace(ctx, c1, c1, rk)
+    assert np.array_equal(c1.download(N), octx.multiply_and_relin(6, a, a, key))
+
+
+@pytest.mark.parametrize("chain", [1, 3, 5])
+def test_rescale_small(small, chain):
+    ctx, octx, mods, key = small
+    N = ctx.N
+    Ql = ctx.limbs_at(chain)
+    say(f"case: rescale chain={chain}")
+    rng = np.random.default_rng(20 + chain)
+    a = rand_ct(rng, mods[:Ql], N)
+    c = upload_ct(ctx, a, chain)
+    c.set_scale(2.0 ** 40)
+    r = ph.rescale_to_next(ctx, c)
+    assert r.chain_index() == chain + 1
+    assert np.array_equal(r.download(N), octx.rescale(Ql, a))
+    assert r.scale() == pytest.approx(2.0 ** 40 / mods[Ql - 1])
+
+
+def test_rescale_end_of_chain_raises(small):
+    ctx, octx, mods, key = small
+    c = upload_ct(ctx, rand_ct(np.random.default_rng(0), mods[:1], ctx.N), 6)
+    with pytest.raises(ValueError):
+        ph.rescale_to_next(ctx, c)
+
+
+@pytest.mark.parametrize("step", [1, -3, 0])
+def test_apply_galois_small(small, step):
+    ctx, octx, mods, key = small
+    N = ctx.N
+    say(f"case: galois step={step}")
+    elt = ph.galois_elt_from_step(step, N)
+    rng = np.random.default_rng(30 + step)
+    Ql = ctx.limbs_at(2)
+    a = rand_ct(rng, mods[:Ql], N)
+    gk = ph.galois_key(ctx, [elt], host=key[None])
+    c = upload_ct(ctx, a, 2)
+    ph.rotate_inplace(ctx, c, step, gk)
+    assert np.array_equal(c.download(N), octx.apply_galois(Ql, a, elt, key))
+
+
+def test_missing_galois_key_raises(small):
+    ctx, octx, mods, key = small
+    gk = ph.galois_key(ctx, [ph.galois_elt_from_step(1, ctx.N)], host=key[None])
+    c = upload_ct(ctx, rand_ct(np.random.default_rng(0), mods[:6], ctx.N), 1)
+    with pytest.raises(ValueError):
+        ph.apply_galois_inplace(ctx, c, ph.galois_elt_from_step(2, ctx.N), gk)
+
+
+# ---------------------------------------------------------------- full size
+CONFIG2_BITS = [60] + [50] * 34 + [60] * 12
+
+
+@pytest.fixture(scope="module")
+def config2():
+    ob.set_threads(16)
+    N = 1 << 16
+    ctx, octx, mods = make_ctx(N, CONFIG2_BITS, 12)
+    assert ctx.size_Q == 35 and ctx.dnum == 3
+    rng = np.random.default_rng(2026)
+    key = rand_key(rng, mods, N, 3)
+    return ctx, octx, mods, key
+
+
+def test_config2_multiply_and_relin_bit_exact(config2):
+    """N=2^16, 35 data limbs + 12 special, dnum = 3 (digits 12, 12, 11)."""
+    ctx, octx, mods, key = config2
+    N = ctx.N
+    say("case: config2 mulrelin top level")
+    rng = np.random.default_rng(5)
+    a, b = rand_ct(rng, mods[:35], N), rand_ct(rng, mods[:35], N)
+    rk = ph.relin_key(ctx, host=key)
+    c1, c2 = upload_ct(ctx, a, 1), upload_ct(ctx, b, 1)
+    ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
+    assert np.array_equal(c1.download(N), octx.multiply_and_relin(35, a, b, key))
+
+
+def test_config2_lower_level_and_rotation(config2):
+    ctx, octx, mods, key = config2
+    N = ctx.N
+    chain = 16   # 20 limbs: digits 12, 8
+    Ql = ctx.limbs_at(chain)
+    say(f"case: config2 chain={chain} mulrelin + galois")
+    rng = np.random.default_rng(6)
+    a, b = rand_ct(rng, mods[:Ql], N), rand_ct(rng, mods[:Ql], N)
+    rk = ph.relin_key(ctx, host=key)
+    c1, c2 = upload_ct(ctx, a, chain), upload_ct(ctx, b, chain)
+    ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
+    assert np.array_equal(c1.download(N), octx.multiply_and_relin(Ql, a, b, key))
+    elt = ph.galois_elt_from_step(5, N)
+    gk = ph.galois_key(ctx, [elt], host=key[None])
+    c3 = upload_ct(ctx, a, chain)
+    ph.rotate_inplace(ctx, c3, 5, gk)
+    assert np.array_equal(c3.download(N), octx.apply_galois(Ql, a, elt, key))
+
+
+def test_config2_rescale(config2):
+    ctx, octx, mods, key = config2
+    N = ctx.N
+    say("case: config2 rescale")
+    a = rand_ct(np.random.default_rng(8), mods[:35], N)
+    c = upload_ct(ctx, a, 1)
+    r = ph.rescale_to_next(ctx, c)
+    assert np.array_equal(r.download(N), octx.rescale(35, a))
+
+
+def test_reference_valid_shape_36_12():
+    """size_Q % size_P == 0 (the reference keygen's own assumption, src/secretkey.cu:309)."""
+    ob.set_threads(16)
+    N = 1 << 16
+    say("case: 36+12 mulrelin")
+    ctx, octx, mods = make_ctx(N, [60] + [50] * 35 + [60] * 12, 12)
+    assert ctx.dnum == 3
+    rng = np.random.default_rng(36)
+    key = rand_key(rng, mods, N, 3)
+    a, b = rand_ct(rng, mods[:36], N), rand_ct(rng, mods[:36], N)
+    rk = ph.relin_key(ctx, host=key)
+    c1, c2 = upload_ct(ctx, a, 1), upload_ct(ctx, b, 1)
+    ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
+    assert np.array_equal(c1.download(N), octx.multiply_and_relin(36, a, b, key))

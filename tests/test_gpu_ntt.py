@@ -1,0 +1,60 @@
+"""GPU parity of the NTT/INTT kernels against the oracle (bit-exact)."""
+import numpy as np
+import pytest
+
+import pyphantom as ph
+from gpu_helpers import make_ctx, rand_limbs, say
+
+pytestmark = pytest.mark.gpu
+
+
+def run_ntt(ctx, data, start, inverse=False):
+    buf = ph.DeviceBuffer.from_array(data)
+    L = data.shape[0]
+    if inverse:
+        ph.ntt_inverse(ctx, buf, L, start)
+    else:
+        ph.ntt_forward(ctx, buf, L, start)
+    ph.device_synchronize()
+    return buf.to_array(data.shape)
+
+
+def test_config1_ntt_then_intt_bit_exact():
+    """N=2^12, 3 limbs of the largest 60-bit primes = 1 mod 2N, uniform input."""
+    say("case: config1 N=4096 L=3")
+    N = 4096
+    ctx, octx, mods = make_ctx(N, [60, 60, 60], 1)
+    rng = np.random.default_rng(1)
+    a = rand_limbs(rng, mods, N)
+    fwd = run_ntt(ctx, a, 0)
+    assert np.array_equal(fwd, octx.ntt_forward(a, 0))
+    inv = run_ntt(ctx, fwd, 0, inverse=True)
+    assert np.array_equal(inv, a)
+
+
+@pytest.mark.parametrize("logN", [12, 13, 14, 15, 16])
+def test_ntt_parity_all_degrees(logN):
+    N = 1 << logN
+    say(f"case: ntt parity N=2^{logN}")
+    ctx, octx, mods = make_ctx(N, [60, 50, 50, 50, 40, 60, 60], 2)
+    rng = np.random.default_rng(logN)
+    a = rand_limbs(rng, mods, N)
+    got = run_ntt(ctx, a, 0)
+    assert np.array_equal(got, octx.ntt_forward(a, 0))
+    # inverse on arbitrary canonical input, with a table offset
+    b = rand_limbs(rng, mods[3:6], N)
+    got_i = run_ntt(ctx, b, 3, inverse=True)
+    assert np.array_equal(got_i, octx.ntt_inverse(b, 3))
+    assert np.array_equal(run_ntt(ctx, got_i, 3), b)
+
+
+def test_ntt_edge_values():
+    """all-zero, all q-1 and the reference's constant input (test/ntt_test.cu:102-121)"""
+    say("case: ntt edge values N=2^16")
+    N = 1 << 16
+    ctx, octx, mods = make_ctx(N, [60, 50, 60], 1)
+    qs = np.array(mods, np.uint64)[:, None]
+    for a in (np.zeros((3, N), np.uint64), np.broadcast_to(qs - 1, (3, N)).copy(), np.full((3, N), 2, np.uint64)):
+        f = run_ntt(ctx, a, 0)
+        assert np.array_equal(f, octx.ntt_forward(a, 0))
+        assert np.array_equal(run_ntt(ctx, f, 0, inverse=True), a)
