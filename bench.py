@@ -98,6 +98,23 @@ def bench_ntt(ph, ctx, N, L, npolys, iters, stream):
     return float(np.mean(per)), float(np.min(per))
 
 
+def bench_copy(ph, n_u64, iters, stream):
+    """Plain streaming copy kernel of known size: the bandwidth a kernel that
+    reads and writes each byte once actually reaches on this GPU."""
+    a = ph.DeviceBuffer(n_u64)
+    b = ph.DeviceBuffer(n_u64)
+    ph.copy_u64(b, a, n_u64, stream)
+    ph.synchronize(stream)
+    e0, e1 = ph.Event(), ph.Event()
+    e0.record(stream)
+    for _ in range(iters):
+        ph.copy_u64(b, a, n_u64, stream)
+    e1.record(stream)
+    ph.synchronize(stream)
+    ms = e0.elapsed_ms(e1) / iters
+    return 2 * n_u64 * 8 / (ms / 1e3) / 1e9
+
+
 def cpu_baseline(N, bits, size_P, budget_s):
     import oracle_bind as ob
 
@@ -155,6 +172,8 @@ def main():
     ntt_bytes = 2 * N * 8 * L * args.ntt_polys
     achieved = ntt_bytes / (ntt_ms / 1e3) / 1e9
     log(f"bench: NTT {L}x{args.ntt_polys} limbs: {ntt_ms:.3f} ms -> {achieved:.1f} GB/s algorithmic")
+    copy_gbs = bench_copy(ph, 1 << 27, 10, stream)
+    log(f"bench: copy kernel (1 GiB read + 1 GiB write): {copy_gbs:.1f} GB/s")
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "ntt_fwd_traffic.json")
     if os.path.exists(tpath):
@@ -187,6 +206,7 @@ def main():
         "op_algorithmic_GBps": round(op_bytes * value / 1e9, 1),
         "ntt": {"limbs": L * args.ntt_polys, "N": N, "ms": round(ntt_ms, 4), "ms_min": round(ntt_min, 4),
                 "GBps_algorithmic": round(achieved, 1)},
+        "copy_kernel_GBps": round(copy_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "forward NTT (k_fwd_cols + k_fwd_rows), algorithmic bytes 2*N*8 per limb"},

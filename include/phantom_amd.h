@@ -180,6 +180,16 @@ int phx_event_record(void *ev, phx_stream stream);
 int phx_event_elapsed_ms(void *start, void *stop, float *ms);
 int phx_event_destroy(void *ev);
 int phx_device_count(int *n);
+/* kernel-variant switches (no reference counterpart; used by tests and the autotuner):
+ *   "bconv_mfma" 1 = FastBConv on the matrix cores (default), 0 = VALU kernel
+ *   "ntt_fp64"   1 = exact FP64 butterflies for primes < 2^50 (default), 0 = 64-bit integer
+ *                only; takes effect for contexts created afterwards
+ * unknown keys return PHX_EINVAL */
+int phx_tune_set(const char *key, int value);
+int phx_tune_get(const char *key, int *value);
+/* streaming copy kernel of n u64 elements (no reference counterpart): the known-size
+ * copy that HBM counters and bandwidth figures are calibrated against */
+int phx_copy_u64(uint64_t *dst, const uint64_t *src, size_t n, phx_stream stream);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "evaluator.h"
+#include "tune.h"
 
 using namespace phx;
 
@@ -607,6 +608,25 @@ int phx_event_destroy(void *ev) {
         if (ev) PHX_HIP_CHECK(hipEventDestroy(static_cast<hipEvent_t>(ev)));
     });
 }
+int phx_tune_set(const char *key, int value) {
+    return guard([&] { require(tune_set(key, value), std::string("unknown tuning key: ") + (key ? key : "(null)")); });
+}
+
+int phx_tune_get(const char *key, int *value) {
+    return guard([&] {
+        nn(value, "value");
+        require(tune_get(key, value), std::string("unknown tuning key: ") + (key ? key : "(null)"));
+    });
+}
+
+int phx_copy_u64(uint64_t *dst, const uint64_t *src, size_t n, phx_stream s) {
+    return guard([&] {
+        nn(dst, "dst");
+        nn(src, "src");
+        copy_u64(dst, src, n, S(s));
+    });
+}
+
 int phx_device_count(int *n) {
     return guard([&] {
         nn(n, "n");

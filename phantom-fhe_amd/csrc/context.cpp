@@ -1,6 +1,8 @@
 // context.cpp -- context construction: tables, per-level constants, pool.
 #include "context.h"
+#include "tune.h"
 
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <thread>
@@ -40,6 +42,38 @@ struct ConstBuilder {
     }
 };
 }  // namespace
+
+std::vector<uint8_t> bconv_mfma_operands(const std::vector<u64> &mat, u32 ni, u32 no, const std::vector<u64> &omod,
+                                         u32 &rt) {
+    // 4 outputs per group of 3 tiles; tile 3T+s, row 4g+i holds output 4T+g, slice b = 4s+i
+    rt = 3 * ((no + 3) / 4);
+    std::vector<uint8_t> blob((size_t)rt * 2 * 64 * 16, 0);
+    // mp[o][i][a] = 2^(7a) * mat[o][i] mod p_o
+    std::vector<u64> mp((size_t)no * ni * 9);
+    for (u32 o = 0; o < no; ++o) {
+        const u64 p = omod[o];
+        for (u32 i = 0; i < ni; ++i) {
+            u64 v = mat[(size_t)o * ni + i] % p;
+            for (u32 a = 0; a < 9; ++a) {
+                mp[((size_t)o * ni + i) * 9 + a] = v;
+                v = hm::mulmod(v, 128, p);
+            }
+        }
+    }
+    // A[row][K]: row -> (output, slice) as above, K = i*9 + a
+    auto A = [&](u32 row, u32 K) -> uint8_t {
+        const u32 t = row / 16, rr = row % 16;
+        const u32 o = 4 * (t / 3) + rr / 4, b = 4 * (t % 3) + rr % 4, i = K / 9, a = K % 9;
+        if (o >= no || b >= 9 || i >= ni) return 0;
+        return (uint8_t)((mp[((size_t)o * ni + i) * 9 + a] >> (7 * b)) & 127);
+    };
+    for (u32 t = 0; t < rt; ++t)
+        for (u32 h = 0; h < 2; ++h)
+            for (u32 l = 0; l < 64; ++l)
+                for (u32 j = 0; j < 16; ++j)
+                    blob[(((size_t)t * 2 + h) * 64 + l) * 16 + j] = A(16 * t + l % 16, 64 * h + 16 * (l / 16) + j);
+    return blob;
+}
 
 void pool_init() {
     std::call_once(g_pool_once, [] {
@@ -94,23 +128,36 @@ Context::Context(const Params &p) : params_(p) {
             });
         for (auto &t : th) t.join();
     }
+    // Rows with q < 2^50 run the NTT on the FP64 pipe (exact, see ntt_arith.h)
+    // unless the ntt_fp64 switch is off (then every row uses 64-bit integers).
+    const bool int_only = tune_value("ntt_fp64") == 0;
     const size_t twb = QP * N_ * sizeof(TwPair);
+    const size_t twdb = QP * N_ * sizeof(double);
     const size_t modb = QP * sizeof(DevModulus);
     const size_t icb = QP * 4 * sizeof(u64);
-    std::vector<uint8_t> h(2 * twb + modb + icb);
+    std::vector<uint8_t> h(2 * twb + 2 * twdb + modb + icb);
     TwPair *htw = reinterpret_cast<TwPair *>(h.data());
     TwPair *hitw = reinterpret_cast<TwPair *>(h.data() + twb);
-    DevModulus *hmod = reinterpret_cast<DevModulus *>(h.data() + 2 * twb);
-    u64 *hic = reinterpret_cast<u64 *>(h.data() + 2 * twb + modb);
+    double *htwd = reinterpret_cast<double *>(h.data() + 2 * twb);
+    double *hitwd = reinterpret_cast<double *>(h.data() + 2 * twb + twdb);
+    DevModulus *hmod = reinterpret_cast<DevModulus *>(h.data() + 2 * twb + 2 * twdb);
+    u64 *hic = reinterpret_cast<u64 *>(h.data() + 2 * twb + 2 * twdb + modb);
     for (size_t r = 0; r < QP; ++r) {
         const auto &t = host_tables_[r];
         for (u64 k = 0; k < N_; ++k) {
             htw[r * N_ + k] = {t.fw[k], t.fws[k]};
             hitw[r * N_ + k] = {t.iw[k], t.iws[k]};
+            htwd[r * N_ + k] = (double)t.fw[k];
+            hitwd[r * N_ + k] = (double)t.iw[k];
         }
         u64 lo, hi;
         hm::barrett_ratio(t.q, lo, hi);
-        hmod[r] = {t.q, lo, hi, 0};
+        u64 fp_bits = 0;
+        if (!int_only && t.q < (1ull << 50)) {
+            const double qinv = 1.0 / (double)t.q;
+            std::memcpy(&fp_bits, &qinv, 8);
+        }
+        hmod[r] = {t.q, lo, hi, fp_bits};
         const u64 s1 = hm::mulmod(t.ninv, t.iw[1], t.q);
         hic[4 * r + 0] = t.ninv;
         hic[4 * r + 1] = hm::shoup(t.ninv, t.q);
@@ -122,8 +169,10 @@ Context::Context(const Params &p) : params_(p) {
     uint8_t *db = static_cast<uint8_t *>(tw_buf_);
     tables_.tw = reinterpret_cast<const TwPair *>(db);
     tables_.itw = reinterpret_cast<const TwPair *>(db + twb);
-    tables_.mod = reinterpret_cast<const DevModulus *>(db + 2 * twb);
-    tables_.inv_consts = reinterpret_cast<const u64 *>(db + 2 * twb + modb);
+    tables_.twd = reinterpret_cast<const double *>(db + 2 * twb);
+    tables_.itwd = reinterpret_cast<const double *>(db + 2 * twb + twdb);
+    tables_.mod = reinterpret_cast<const DevModulus *>(db + 2 * twb + 2 * twdb);
+    tables_.inv_consts = reinterpret_cast<const u64 *>(db + 2 * twb + 2 * twdb + modb);
     tables_.logN = logN_;
     tables_.N = (u32)N_;
 
@@ -139,6 +188,23 @@ Context::Context(const Params &p) : params_(p) {
     std::vector<Offs> offs(size_Q_ + 1);
     levels_.resize(size_Q_ + 1);
     auto row_of = [&](u32 Ql, u32 pos) -> u32 { return pos < Ql ? pos : size_Q_ + (pos - Ql); };
+    // MFMA operands of a FastBConv matrix (only when K = 9 * ni fits one 128-wide slab)
+    auto add_mfma = [&](BconvDesc &d, const std::vector<u64> &mat, const std::vector<u64> &om) {
+        d.rt = 0;
+        if (d.ni > 14) return;
+        u32 rt = 0;
+        std::vector<uint8_t> blob = bconv_mfma_operands(mat, d.ni, d.no, om, rt);
+        std::vector<u64> words(blob.size() / 8);
+        std::memcpy(words.data(), blob.data(), blob.size());
+        d.a_off = cb.push64(words) / 8;
+        std::vector<u64> pinv(d.no);
+        for (u32 o = 0; o < d.no; ++o) {
+            const double v = 1.0 / (double)om[o];
+            std::memcpy(&pinv[o], &v, 8);
+        }
+        d.pinv_off = cb.push64(pinv) / 8;
+        d.rt = rt;
+    };
     auto scale_consts = [&](u32 row, u64 scale) {
         const auto &t = host_tables_[row];
         const u64 s0 = hm::mulmod(t.ninv, scale, t.q);
@@ -182,6 +248,9 @@ Context::Context(const Params &p) : params_(p) {
             d.mat_off = cb.push64(mat) / 8;
             d.pos_off = cb.push32(pos) / 4;
             d.row_off = cb.push32(rows) / 4;
+            std::vector<u64> om(no);
+            for (u32 o = 0; o < no; ++o) om[o] = qlp[pos[o]];
+            add_mfma(d, mat, om);
             descs.push_back(d);
         }
         offs[c].modup_scale = cb.push64(msc);
@@ -212,6 +281,7 @@ Context::Context(const Params &p) : params_(p) {
             d.mat_off = cb.push64(mat) / 8;
             d.pos_off = cb.push32(pos) / 4;
             d.row_off = cb.push32(rows) / 4;
+            add_mfma(d, mat, std::vector<u64>(qp, qp + Ql));
             offs[c].moddown_bconv = cb.push_struct(std::vector<BconvDesc>{d});
             L.h_moddown_bconv = d;
         }

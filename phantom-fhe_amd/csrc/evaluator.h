@@ -65,6 +65,9 @@ void inner_product(const Context &ctx, u32 chain, const u64 *t_mod_up, const KSw
 void moddown(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, const u64 *base1, u32 npolys,
              hipStream_t s);
 
+// streaming copy (bandwidth calibration)
+void copy_u64(u64 *dst, const u64 *src, size_t n, hipStream_t s);
+
 // ---- keys and (test-side) encryption ----
 void secret_key_generate(const Context &ctx, SecretKey &sk, u64 seed, hipStream_t s);
 void secret_key_upload(const Context &ctx, SecretKey &sk, const int8_t *coeffs, hipStream_t s);

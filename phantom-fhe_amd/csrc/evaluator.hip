@@ -3,9 +3,11 @@
 // relinearisation, rotation and rescale; key generation and symmetric
 // encryption for tests.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "evaluator.h"
+#include "tune.h"
 
 namespace phx {
 
@@ -55,6 +57,170 @@ __global__ __launch_bounds__(kThreads) void k_bconv(const u64 *__restrict__ cbuf
         for (u32 i = 0; i < d.ni; ++i) outj[(u64)(d.own_first + i) * N] = cs[(u64)i * N];
     }
 }
+
+// ---------------------------------------------------------------------------
+// FastBConv phase 2 on the matrix cores, exact (same outputs as k_bconv).
+//
+// With x_i = sum_a x_ia 2^(7a) (7-bit slices, a < 9) and
+// M'_oia = 2^(7a) M[o][i] mod p_o = sum_b m_oiab 2^(7b):
+//   sum_i x_i M[o][i] = sum_b 2^(7b) Z[o,b]  (mod p_o),
+//   Z[o,b] = sum_(i,a) m_oiab x_ia                 (an int8 GEMM, int32 sums)
+// Z < 9*14*127*127 < 2^21, so the int32 accumulation is exact. Rows (o,b) =
+// o*9+b, K = (i,a) = i*9+a <= 126 -> two v_mfma_i32_16x16x64_i8 per 16x16 tile.
+// The 9 partial sums of an output are recombined into V < 2^78 and reduced
+// with an FP64 quotient estimate (off by at most one) and one correction.
+// ---------------------------------------------------------------------------
+typedef int v4i __attribute__((ext_vector_type(4)));
+constexpr u32 kBconvTileCols = 16;   // coefficients per MFMA column tile
+constexpr u32 kBconvThreads = 512;   // 8 waves share one staged copy of the operands
+
+__global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(const u64 *__restrict__ cbuf,
+                                                         const BconvDesc *__restrict__ descs, u32 desc_stride,
+                                                         const u64 *__restrict__ in, u64 in_job_stride,
+                                                         u64 *__restrict__ out, u64 out_job_stride,
+                                                         const u64 *__restrict__ copy_src, u64 copy_job_stride,
+                                                         const DevModulus *__restrict__ mod, u32 N) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const u32 job = blockIdx.y;
+    const BconvDesc d = descs[job * desc_stride];
+    const u32 rt = d.rt, ni = d.ni, no = d.no;
+    const u32 tid = threadIdx.x;
+    // A operand tiles staged once per workgroup: [rt][2][64 lanes] x 16 B
+    uint4 *As = reinterpret_cast<uint4 *>(smem);
+    const uint4 *Ag = reinterpret_cast<const uint4 *>(cbuf + d.a_off);
+    for (u32 i = tid; i < rt * 128; i += kBconvThreads) As[i] = Ag[i];
+    __syncthreads();
+
+    const u64 *inj = in + job * in_job_stride + (u64)d.in_first * N;
+    u64 *outj = out + job * out_job_stride;
+    const u32 *pos = reinterpret_cast<const u32 *>(cbuf) + d.pos_off;
+    const u32 *rows = reinterpret_cast<const u32 *>(cbuf) + d.row_off;
+    const double *pinv = reinterpret_cast<const double *>(cbuf + d.pinv_off);
+    const u32 wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
+    const u32 ntiles = N / kBconvTileCols;
+    const u32 groups = rt / 3;
+
+    // each wave walks its own column tiles (no workgroup barriers below); the
+    // inputs of the next tile are loaded before the current tile is computed
+    const u32 nwaves = kBconvThreads / 64;
+    u32 ct = blockIdx.x * nwaves + wave;
+    const u32 ct_step = gridDim.x * nwaves;
+    u32 iA[2], aA[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const u32 K0 = 64 * h + 16 * lg;
+        iA[h] = K0 / 9;
+        aA[h] = K0 - 9 * iA[h];
+    }
+    u64 xn[2][3];
+    auto load_x = [&](u32 tile) {
+        const u32 kk = tile * kBconvTileCols + ln;
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) xn[h][r] = iA[h] + r < ni ? inj[(u64)(iA[h] + r) * N + kk] : 0;
+    };
+    if (ct < ntiles) load_x(ct);
+    for (; ct < ntiles; ct += ct_step) {
+        const u32 k = ct * kBconvTileCols + ln;   // this lane's coefficient
+        // B operand: 7-bit slices of x_i[k] for K = 64h + 16*lg + j; K0 = 64h + 16 lg
+        // spans inputs i0 .. i0+2 (16 slices < 2 * 9)
+        v4i b[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const u64 x0 = xn[h][0], x1 = xn[h][1], x2 = xn[h][2];
+            u32 w[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (u32 j = 0; j < 16; ++j) {
+                const u32 a = aA[h] + j;   // slice index relative to input i0, < 24
+                const u64 x = a < 9 ? x0 : (a < 18 ? x1 : x2);
+                const u32 sh = 7 * (a < 9 ? a : (a < 18 ? a - 9 : a - 18));
+                w[j >> 2] |= (u32)((x >> sh) & 127ull) << (8 * (j & 3));
+            }
+            b[h] = v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+        }
+        if (ct + ct_step < ntiles) load_x(ct + ct_step);
+        // 3 tiles per group of 4 outputs; lane (lg, ln) owns output 4T + lg
+        for (u32 T = 0; T < groups; ++T) {
+            v4i acc[3];
+#pragma unroll
+            for (int sgrp = 0; sgrp < 3; ++sgrp) {
+                const u32 t = 3 * T + sgrp;
+                const uint4 ua0 = As[(t * 2 + 0) * 64 + lane];
+                const uint4 ua1 = As[(t * 2 + 1) * 64 + lane];
+                const v4i a0 = {(int)ua0.x, (int)ua0.y, (int)ua0.z, (int)ua0.w};
+                const v4i a1 = {(int)ua1.x, (int)ua1.y, (int)ua1.z, (int)ua1.w};
+                v4i c = {0, 0, 0, 0};
+                c = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b[0], c, 0, 0, 0);
+                acc[sgrp] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b[1], c, 0, 0, 0);
+            }
+            const u32 o = 4 * T + lg;
+            if (o < no) {
+                // Z_b = acc[b / 4][b % 4], b < 9
+                const u64 s = (u64)(u32)acc[0][0] + ((u64)(u32)acc[0][1] << 7) + ((u64)(u32)acc[0][2] << 14) +
+                              ((u64)(u32)acc[0][3] << 21) + ((u64)(u32)acc[1][0] << 28) +
+                              ((u64)(u32)acc[1][1] << 35);
+                const u64 Tt = (u64)(u32)acc[1][2] + ((u64)(u32)acc[1][3] << 7) + ((u64)(u32)acc[2][0] << 14);
+                const u64 lo = s + (Tt << 42);
+                const u64 hi = (Tt >> 22) + (lo < s ? 1 : 0);
+                const u64 p = mod[rows[o]].value;
+                const double vd = __builtin_fma((double)hi, 0x1p64, (double)lo);
+                const u64 kq = (u64)__builtin_floor(vd * pinv[o]);
+                u64 r = lo - kq * p;
+                if ((long long)r < 0) r += p;
+                else if (r >= p) r -= p;
+                outj[(u64)pos[o] * N + k] = r;
+            }
+        }
+    }
+    if (copy_src) {
+        const u64 *cs = copy_src + job * copy_job_stride + (u64)d.in_first * N;
+        const u64 total = (u64)ni * N;
+        for (u64 idx = (u64)blockIdx.x * kBconvThreads + tid; idx < total; idx += (u64)gridDim.x * kBconvThreads) {
+            const u64 i = idx / N, kk = idx % N;
+            outj[(d.own_first + i) * N + kk] = cs[i * N + kk];
+        }
+    }
+}
+
+namespace {
+// FastBConv dispatch: matrix cores when every job has an MFMA form, VALU otherwise
+// (the bconv_mfma switch set to 0 forces the VALU kernel).
+bool bconv_force_valu() { return tune_value("bconv_mfma") == 0; }
+
+void bconv_launch(const Context &ctx, const BconvDesc *descs_dev, const std::vector<BconvDesc> &descs_host,
+                  u32 desc_stride, u32 jobs, const u64 *in, u64 in_job_stride, u64 *out, u64 out_job_stride,
+                  const u64 *copy_src, u64 copy_job_stride, hipStream_t s) {
+    const u64 N = ctx.N();
+    u32 rt_max = 0;
+    bool mfma = !bconv_force_valu() && N % kBconvTileCols == 0;
+    for (const auto &d : descs_host) {
+        if (d.rt == 0) mfma = false;
+        rt_max = std::max(rt_max, d.rt);
+    }
+    const size_t lds = (size_t)rt_max * 2048;
+    static const size_t lds_limit = [] {
+        int dev = 0, v = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        return (size_t)v;
+    }();
+    if (lds > lds_limit) mfma = false;
+    if (mfma) {
+        // enough workgroups to fill the chip twice over; each stages the operands once
+        const u32 tiles = (u32)(N / kBconvTileCols);
+        const u32 waves = kBconvThreads / 64;
+        const u32 gx = std::max<u32>(1, std::min<u32>((tiles + waves - 1) / waves, (512 + jobs - 1) / jobs));
+        hipLaunchKernelGGL(k_bconv_mfma, dim3(gx, jobs), dim3(kBconvThreads), lds, s, ctx.const_buf(),
+                           descs_dev, desc_stride, in, in_job_stride, out, out_job_stride, copy_src, copy_job_stride,
+                           ctx.dev_mod(), (u32)N);
+    } else {
+        hipLaunchKernelGGL(k_bconv, grid_for(N, jobs), dim3(kThreads), 0, s, ctx.const_buf(), descs_dev, desc_stride,
+                           in, in_job_stride, out, out_job_stride, copy_src, copy_job_stride, ctx.dev_mod(), (u32)N);
+    }
+    PHX_HIP_CHECK(hipGetLastError());
+}
+}  // namespace
 
 // key_switch_inner_prod_c2_and_evk (src/eval_key_switch.cu:14-69):
 // cx0[pos] = sum_j t[j][pos] * key_j.b[row], cx1[pos] = sum_j t[j][pos] * key_j.a[row]
@@ -208,6 +374,22 @@ __global__ void k_dot_secret(const u64 *ct, const u64 *s, const u64 *s2, u64 *ou
     out[o] = v;
 }
 
+// 16-byte streaming copy, grid-stride
+__global__ __launch_bounds__(kThreads) void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, u64 n16) {
+    for (u64 i = (u64)blockIdx.x * kThreads + threadIdx.x; i < n16; i += (u64)gridDim.x * kThreads) dst[i] = src[i];
+}
+
+void copy_u64(u64 *dst, const u64 *src, size_t n, hipStream_t s) {
+    require(n % 2 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0,
+            "copy_u64 needs an even count and 16-byte aligned buffers");
+    const u64 n16 = n / 2;
+    const u32 blocks = (u32)std::min<u64>((n16 + kThreads - 1) / kThreads, 256 * 64);
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kThreads), 0, s, reinterpret_cast<const uint4 *>(src),
+                       reinterpret_cast<uint4 *>(dst), n16);
+    PHX_HIP_CHECK(hipGetLastError());
+}
+
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
@@ -254,9 +436,7 @@ void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStrea
     mi.scale = L.modup_scale;
     ntt_inverse(ctx.tables(), mi, Ql, 1, s);
     // FastBConv of every digit to its complement, own limbs copied
-    hipLaunchKernelGGL(k_bconv, grid_for(N, beta), dim3(kThreads), 0, s, ctx.const_buf(), L.modup_bconv, 1u,
-                       (const u64 *)t_cks, (u64)0, t_mod_up, (u64)QlP * N, c2, (u64)0, ctx.dev_mod(), (u32)N);
-    PHX_HIP_CHECK(hipGetLastError());
+    bconv_launch(ctx, L.modup_bconv, L.h_modup_bconv, 1u, beta, t_cks, 0, t_mod_up, (u64)QlP * N, c2, 0, s);
     // forward NTT of the complement limbs of every digit
     LimbMap mf = limb_map(t_mod_up, 0, QlP, (u64)QlP * N);
     mf.split = Ql;
@@ -292,10 +472,8 @@ void moddown(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, 
     mi.scale = L.moddown_scale;
     ntt_inverse(ctx.tables(), mi, P, npolys, s);
     u64 *delta = static_cast<u64 *>(dev_alloc((size_t)npolys * Ql * N * 8, s));
-    hipLaunchKernelGGL(k_bconv, grid_for(N, npolys), dim3(kThreads), 0, s, ctx.const_buf(), L.moddown_bconv, 0u,
-                       (const u64 *)cx, (u64)QlP * N, delta, (u64)Ql * N, (const u64 *)nullptr, (u64)0,
-                       ctx.dev_mod(), (u32)N);
-    PHX_HIP_CHECK(hipGetLastError());
+    bconv_launch(ctx, L.moddown_bconv, {L.h_moddown_bconv}, 0u, npolys, cx, (u64)QlP * N, delta, (u64)Ql * N,
+                 nullptr, 0, s);
     StoreModDown st{};
     st.ct = ct;
     st.base0 = base0;

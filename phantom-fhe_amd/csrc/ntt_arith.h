@@ -1,0 +1,106 @@
+// ntt_arith.h -- the two butterfly arithmetics of the NTT kernels.
+//
+// IntArith: 64-bit Shoup/Harvey lazy butterflies on u64 values (any q < 2^61),
+//   values in [0, 4q) (forward) / [0, 2q) (inverse).
+//
+// FpArith: exact modular arithmetic on the FP64 pipe for q < 2^50 (the
+//   MI355X VALU runs FP64 FMA at full rate, 32-bit integer multiplies at a
+//   quarter of it). Values are integers held exactly in doubles, signed,
+//   |v| < 2^53. The product y*w is split exactly as h + l with
+//   h = fl(y*w), l = fma(y, w, -h); the quotient k = rint(h / q) comes from
+//   the precomputed reciprocal (a Barrett reduction) and r = fma(-k, q, h) + l
+//   is exact because |h - k q| and |r| are integers below 2^53.
+//   Bounds (q < 2^50): reduced values |v| <= q/2 (+ eps); mulmod outputs
+//   |r| < 2q for |y| < 2^52; butterfly outputs |v| < 2.5q < 2^52.
+#pragma once
+
+#include "common.h"
+
+namespace phx {
+
+struct IntArith {
+    using T = u64;
+    const TwPair *tw;
+    u64 q, q2;
+
+    __device__ __forceinline__ static T from_canon(u64 v) { return v; }
+    __device__ __forceinline__ static T from_bits(u64 b) { return b; }
+    __device__ __forceinline__ static u64 to_bits(T v) { return v; }
+    // forward output [0, 4q) -> [0, q)
+    __device__ __forceinline__ u64 fwd_canon(T v) const { return csub(csub(v, q2), q); }
+
+    __device__ __forceinline__ void ct(T &x, T &y, u32 idx) const {
+        const TwPair w = tw[idx];
+        ct_bfly(x, y, w.w, w.ws, q, q2);
+    }
+    __device__ __forceinline__ void gs(T &x, T &y, u32 idx) const {
+        const TwPair w = tw[idx];
+        gs_bfly(x, y, w.w, w.ws, q, q2);
+    }
+    // top inverse stage with both outputs scaled to canonical form
+    __device__ __forceinline__ void gs_last(T &x, T &y, u64 s0, u64 s0s, u64 s1, u64 s1s) const {
+        const u64 sum = x + y, dif = x + q2 - y;
+        x = mul_shoup(sum, s0, s0s, q);
+        y = mul_shoup(dif, s1, s1s, q);
+    }
+    __device__ __forceinline__ static u64 inv_canon(T v) { return v; }
+};
+
+struct FpArith {
+    using T = double;
+    const double *tw;
+    double q, qinv;
+
+    __device__ __forceinline__ static T from_canon(u64 v) {
+        // exact for v < 2^52: set the exponent of 2^52 and subtract it
+        return __longlong_as_double((long long)(v | 0x4330000000000000ull)) - 0x1p52;
+    }
+    __device__ __forceinline__ static T from_bits(u64 b) { return __longlong_as_double((long long)b); }
+    __device__ __forceinline__ static u64 to_bits(T v) { return (u64)__double_as_longlong(v); }
+
+    __device__ __forceinline__ T red(T x) const {
+#pragma clang fp contract(off)
+        const T k = __builtin_rint(x * qinv);
+        return __builtin_fma(-k, q, x);
+    }
+    __device__ __forceinline__ T mulmod(T y, T w) const {
+#pragma clang fp contract(off)
+        const T h = y * w;
+        const T l = __builtin_fma(y, w, -h);
+        const T k = __builtin_rint(h * qinv);
+        return __builtin_fma(-k, q, h) + l;
+    }
+    // canonical residue in [0, q) of an exact integer |v| < 2^53, as u64
+    __device__ __forceinline__ u64 canon(T v) const {
+#pragma clang fp contract(off)
+        const T k = __builtin_floor(v * qinv);
+        T r = __builtin_fma(-k, q, v);
+        r = r < 0.0 ? r + q : r;
+        r = r >= q ? r - q : r;
+        return (u64)__double_as_longlong(r + 0x1p52) & 0x000fffffffffffffull;
+    }
+    __device__ __forceinline__ u64 fwd_canon(T v) const { return canon(v); }
+
+    __device__ __forceinline__ void ct(T &x, T &y, u32 idx) const {
+#pragma clang fp contract(off)
+        const T t = mulmod(y, tw[idx]);
+        const T xr = red(x);
+        x = xr + t;
+        y = xr - t;
+    }
+    __device__ __forceinline__ void gs(T &x, T &y, u32 idx) const {
+#pragma clang fp contract(off)
+        const T s = x + y, d = x - y;
+        x = red(s);
+        y = mulmod(d, tw[idx]);
+    }
+    __device__ __forceinline__ void gs_last(T &x, T &y, u64 s0, u64, u64 s1, u64) const {
+#pragma clang fp contract(off)
+        const T s = x + y, d = x - y;
+        x = mulmod(s, from_canon(s0));
+        y = mulmod(d, from_canon(s1));
+    }
+    __device__ __forceinline__ u64 inv_canon(T v) const { return canon(v); }
+};
+
+}  // namespace phx

@@ -18,6 +18,7 @@
 #pragma once
 
 #include "common.h"
+#include "ntt_arith.h"
 
 namespace phx {
 
@@ -25,7 +26,7 @@ constexpr int kNttThreads = 256;
 constexpr int kNttRegs = 16;
 constexpr int kNttTile = kNttThreads * kNttRegs;  // 4096
 
-// Per-launch limb mapping: blockIdx.y selects a limb of the call, blockIdx.z
+// Per-launch limb mapping: blockIdx.z selects a limb of the call, blockIdx.y
 // a polynomial (stride poly_stride elements). Limbs [skip_lo, skip_hi) are
 // not launched (the digit's own limbs in modup, already in NTT form).
 struct LimbMap {
@@ -55,9 +56,13 @@ struct LimbMap {
     }
 };
 
+// DevModulus.pad of a row holds the bits of fl(1/q) when the row takes the
+// FP64 arithmetic (q < 2^50), 0 otherwise.
 struct NttTables {
     const TwPair *tw;        // [rows][N]
     const TwPair *itw;       // [rows][N]
+    const double *twd;       // [rows][N] forward twiddles as doubles (FP64 rows)
+    const double *itwd;      // [rows][N]
     const DevModulus *mod;   // [rows]
     const u64 *inv_consts;   // [rows][4]: n^-1, shoup, n^-1 * itw[1], shoup
     u32 logN;
@@ -68,11 +73,11 @@ struct NttTables {
 // v holds (16 >> NB) blocks of 2^NB values; value e of block b has global
 // index base[b] + (e << p). A round applies the NB stages whose pair distance
 // is 2^(p+NB-1) ... 2^p. Stage with distance t = 2^(p+jj) has
-// m = N / (2t) groups; the group of index i is i >> (p+jj+1).
-template <int NB>
-__device__ __forceinline__ void fwd_round(u64 (&v)[16], const u32 (&base)[16 >> NB], int p,
-                                          u32 logN, const TwPair *tw, u64 q) {
-    const u64 q2 = q << 1;
+// m = N / (2t) groups; the group of index i is i >> (p+jj+1), and its twiddle
+// is entry m + group of the bit-reversed table.
+template <int NB, class A>
+__device__ __forceinline__ void fwd_round(typename A::T (&v)[16], const u32 (&base)[16 >> NB], int p,
+                                          u32 logN, const A &a) {
 #pragma unroll
     for (int jj = NB - 1; jj >= 0; --jj) {
         const int sh = p + jj + 1;
@@ -83,17 +88,15 @@ __device__ __forceinline__ void fwd_round(u64 (&v)[16], const u32 (&base)[16 >> 
 #pragma unroll
             for (int e = 0; e < (1 << NB); ++e) {
                 if (e & (1 << jj)) continue;
-                const TwPair w = tw[g0 + (e >> (jj + 1))];
-                ct_bfly(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], w.w, w.ws, q, q2);
+                a.ct(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], g0 + (e >> (jj + 1)));
             }
         }
     }
 }
 
-template <int NB>
-__device__ __forceinline__ void inv_round(u64 (&v)[16], const u32 (&base)[16 >> NB], int p,
-                                          u32 logN, const TwPair *itw, u64 q) {
-    const u64 q2 = q << 1;
+template <int NB, class A>
+__device__ __forceinline__ void inv_round(typename A::T (&v)[16], const u32 (&base)[16 >> NB], int p,
+                                          u32 logN, const A &a) {
 #pragma unroll
     for (int jj = 0; jj < NB; ++jj) {
         const int sh = p + jj + 1;
@@ -104,22 +107,18 @@ __device__ __forceinline__ void inv_round(u64 (&v)[16], const u32 (&base)[16 >> 
 #pragma unroll
             for (int e = 0; e < (1 << NB); ++e) {
                 if (e & (1 << jj)) continue;
-                const TwPair w = itw[g0 + (e >> (jj + 1))];
-                gs_bfly(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], w.w, w.ws, q, q2);
+                a.gs(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], g0 + (e >> (jj + 1)));
             }
         }
     }
 }
 
-// Last inverse round: same as inv_round but the final stage (distance N/2,
-// m = 1) multiplies both outputs into canonical form by the per-limb
-// constants s0 (x output: n^-1 * scale) and s1 (y output: n^-1 * psi^-brv(1)
-// * scale). Only valid when the round contains the top stage.
-template <int NB>
-__device__ __forceinline__ void inv_round_last(u64 (&v)[16], const u32 (&base)[16 >> NB], int p,
-                                               u32 logN, const TwPair *itw, u64 q, u64 s0,
-                                               u64 s0s, u64 s1, u64 s1s) {
-    const u64 q2 = q << 1;
+// Last inverse round: as inv_round, but the top stage (distance N/2, m = 1)
+// multiplies its outputs by s0 (x output: n^-1 * scale) and s1 (y output:
+// n^-1 * psi^-brv(1) * scale), leaving them ready for canonical form.
+template <int NB, class A>
+__device__ __forceinline__ void inv_round_last(typename A::T (&v)[16], const u32 (&base)[16 >> NB], int p,
+                                               u32 logN, const A &a, u64 s0, u64 s0s, u64 s1, u64 s1s) {
 #pragma unroll
     for (int jj = 0; jj < NB - 1; ++jj) {
         const int sh = p + jj + 1;
@@ -130,8 +129,7 @@ __device__ __forceinline__ void inv_round_last(u64 (&v)[16], const u32 (&base)[1
 #pragma unroll
             for (int e = 0; e < (1 << NB); ++e) {
                 if (e & (1 << jj)) continue;
-                const TwPair w = itw[g0 + (e >> (jj + 1))];
-                gs_bfly(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], w.w, w.ws, q, q2);
+                a.gs(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], g0 + (e >> (jj + 1)));
             }
         }
     }
@@ -141,12 +139,7 @@ __device__ __forceinline__ void inv_round_last(u64 (&v)[16], const u32 (&base)[1
 #pragma unroll
         for (int e = 0; e < (1 << NB); ++e) {
             if (e & (1 << jj)) continue;
-            u64 &x = v[b * (1 << NB) + e];
-            u64 &y = v[b * (1 << NB) + e + (1 << jj)];
-            const u64 sum = x + y;          // < 4q
-            const u64 dif = x + q2 - y;     // < 4q
-            x = mul_shoup(sum, s0, s0s, q);
-            y = mul_shoup(dif, s1, s1s, q);
+            a.gs_last(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], s0, s0s, s1, s1s);
         }
     }
 }

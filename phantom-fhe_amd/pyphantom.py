@@ -102,6 +102,9 @@ _SIGS = {
     "phx_event_elapsed_ms": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
     "phx_event_destroy": (ctypes.c_int, [_vp]),
     "phx_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "phx_copy_u64": (ctypes.c_int, [_u64p, _u64p, _sz, _vp]),
+    "phx_tune_set": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    "phx_tune_get": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
 }
 
 
@@ -501,6 +504,20 @@ class Event:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.phx_event_destroy(self._h)
             self._h = None
+
+
+def tune_set(key, value):
+    _check(lib().phx_tune_set(key.encode(), int(value)))
+
+
+def tune_get(key):
+    v = ctypes.c_int()
+    _check(lib().phx_tune_get(key.encode(), ctypes.byref(v)))
+    return v.value
+
+
+def copy_u64(dst, src, n, stream=None):
+    _check(lib().phx_copy_u64(dst.ptr(), src.ptr(), n, _stream(stream)))
 
 
 def device_count():

@@ -52,3 +52,13 @@ def test_invalid_arguments_raise_value_error():
     with pytest.raises(ValueError):
         ph.galois_elt_from_step(1 << 20, 4096)
     assert "step count too large" in ph.lib().phx_last_error().decode()
+
+
+def test_tune_switches_roundtrip():
+    assert ph.tune_get("bconv_mfma") in (0, 1)
+    old = ph.tune_get("ntt_fp64")
+    ph.tune_set("ntt_fp64", 0)
+    assert ph.tune_get("ntt_fp64") == 0
+    ph.tune_set("ntt_fp64", old)
+    with pytest.raises(ValueError):
+        ph.tune_set("no_such_knob", 1)

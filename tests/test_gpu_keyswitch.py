@@ -197,3 +197,53 @@ def test_reference_valid_shape_36_12():
     c1, c2 = upload_ct(ctx, a, 1), upload_ct(ctx, b, 1)
     ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
     assert np.array_equal(c1.download(N), octx.multiply_and_relin(36, a, b, key))
+
+
+# ---------------------------------------------------------------- kernel variants
+@pytest.mark.parametrize("bconv_mfma,ntt_fp64", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_variants_bit_exact(bconv_mfma, ntt_fp64):
+    """Both FastBConv kernels (VALU / matrix cores) and both NTT arithmetics
+    (64-bit integer / exact FP64) give the oracle's outputs."""
+    say(f"case: variants bconv_mfma={bconv_mfma} ntt_fp64={ntt_fp64}")
+    old = (ph.tune_get("bconv_mfma"), ph.tune_get("ntt_fp64"))
+    try:
+        ph.tune_set("bconv_mfma", bconv_mfma)
+        ph.tune_set("ntt_fp64", ntt_fp64)
+        N = 1 << 14
+        ctx, octx, mods = make_ctx(N, [60] + [50] * 9 + [60] * 4, 4)
+        rng = np.random.default_rng(50 + 2 * bconv_mfma + ntt_fp64)
+        key = rand_key(rng, mods, N, ctx.dnum)
+        rk = ph.relin_key(ctx, host=key)
+        for chain in (1, 3):
+            Ql = ctx.limbs_at(chain)
+            a, b = rand_ct(rng, mods[:Ql], N), rand_ct(rng, mods[:Ql], N)
+            c1, c2 = upload_ct(ctx, a, chain), upload_ct(ctx, b, chain)
+            ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
+            assert np.array_equal(c1.download(N), octx.multiply_and_relin(Ql, a, b, key))
+    finally:
+        ph.tune_set("bconv_mfma", old[0])
+        ph.tune_set("ntt_fp64", old[1])
+
+
+def test_bconv_extreme_inputs_mfma():
+    """All inputs q-1 (largest slices everywhere) through modup and moddown."""
+    say("case: bconv extreme inputs")
+    N = 1 << 13
+    ctx, octx, mods = make_ctx(N, [60] + [50] * 13 + [60] * 14, 14)
+    Ql = 14
+    qs = np.array(mods[:Ql], np.uint64)[:, None]
+    c2 = np.broadcast_to(qs - 1, (Ql, N)).copy()
+    beta = octx.beta(Ql)
+    QlP = Ql + 14
+    c2b = ph.DeviceBuffer.from_array(c2)
+    tb = ph.DeviceBuffer(beta * QlP * N)
+    ph.modup(ctx, 1, c2b, tb)
+    ph.device_synchronize()
+    assert np.array_equal(tb.to_array((beta, QlP, N)), octx.modup(Ql, c2))
+    qlp = np.array(mods, np.uint64)[:, None]
+    cx = np.broadcast_to(qlp - 1, (QlP, N)).copy()
+    cxb = ph.DeviceBuffer.from_array(cx)
+    ob = ph.DeviceBuffer(Ql * N)
+    ph.moddown_from_ntt(ctx, 1, cxb, ob, 1)
+    ph.device_synchronize()
+    assert np.array_equal(ob.to_array((Ql, N)), octx.moddown(Ql, cx))

@@ -52,9 +52,14 @@ def test_ntt_edge_values():
     """all-zero, all q-1 and the reference's constant input (test/ntt_test.cu:102-121)"""
     say("case: ntt edge values N=2^16")
     N = 1 << 16
-    ctx, octx, mods = make_ctx(N, [60, 50, 60], 1)
+    ctx, octx, mods = make_ctx(N, [60, 50, 49, 60], 1)
     qs = np.array(mods, np.uint64)[:, None]
-    for a in (np.zeros((3, N), np.uint64), np.broadcast_to(qs - 1, (3, N)).copy(), np.full((3, N), 2, np.uint64)):
+    L = len(mods)
+    alt = np.where(np.arange(N) % 2 == 0, qs - 1, 0).astype(np.uint64)
+    half = np.broadcast_to(qs // 2, (L, N)).copy()
+    for a in (np.zeros((L, N), np.uint64), np.broadcast_to(qs - 1, (L, N)).copy(), np.full((L, N), 2, np.uint64),
+              alt, half):
         f = run_ntt(ctx, a, 0)
         assert np.array_equal(f, octx.ntt_forward(a, 0))
         assert np.array_equal(run_ntt(ctx, f, 0, inverse=True), a)
+        assert np.array_equal(run_ntt(ctx, a, 0, inverse=True), octx.ntt_inverse(a, 0))
