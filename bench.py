@@ -77,6 +77,33 @@ def bench_hmult(ph, ctx, mods, N, batch, steps, warmup, stream):
     return ms, wall
 
 
+def bench_hmult_batched(ph, ctx, mods, N, batch, steps, warmup, stream):
+    """The same work as bench_hmult, as one batch plan replayed from a hipGraph."""
+    rng = np.random.default_rng(4321)
+    Ql = ctx.size_Q
+    key = np.stack([np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in mods])
+                              for _ in range(2)]) for _ in range(ctx.dnum)])
+    rlk = ph.relin_key(ctx, host=key)
+    del key
+    a = [ph.ciphertext(ctx, 1, 2) for _ in range(batch)]
+    b = [ph.ciphertext(ctx, 1, 2) for _ in range(batch)]
+    for i in range(batch):
+        a[i].upload(rand_ct(rng, mods[:Ql], N))
+        b[i].upload(rand_ct(rng, mods[:Ql], N))
+    bt = ph.batch.multiply_and_relin(ctx, a, b, rlk)
+    bt.capture(stream)
+    for _ in range(warmup):
+        bt.replay(stream)
+    ph.synchronize(stream)
+    ev0, ev1 = ph.Event(), ph.Event()
+    ev0.record(stream)
+    for _ in range(steps):
+        bt.replay(stream)
+    ev1.record(stream)
+    ph.synchronize(stream)
+    return ev0.elapsed_ms(ev1), bt.scratch_bytes()
+
+
 def bench_ntt(ph, ctx, N, L, npolys, iters, stream):
     """Forward NTT over npolys x L limbs in one launch pair (pass 1 + pass 2)."""
     rng = np.random.default_rng(7)
@@ -146,7 +173,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--ntt-polys", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -161,10 +188,14 @@ def main():
     ctx, mods = make_context(ph, N, CONFIG2_BITS, 12)
     stream = ph.stream_create()
     log(f"bench: HMult+relin N=2^16 L=35 P=12 dnum={ctx.dnum} batch={args.batch} steps={args.steps}")
-    ms, wall = bench_hmult(ph, ctx, mods, N, args.batch, args.steps, args.warmup, stream)
+    ms_plain, wall = bench_hmult(ph, ctx, mods, N, args.batch, args.steps, args.warmup, stream)
     ops = args.batch * args.steps
+    plain = ops / (ms_plain / 1e3)
+    log(f"bench: plain   {ops} ops in {ms_plain:.2f} ms (events), {wall * 1e3:.2f} ms wall -> {plain:.1f} op/s")
+    ms, scratch = bench_hmult_batched(ph, ctx, mods, N, args.batch, args.steps, args.warmup, stream)
     value = ops / (ms / 1e3)
-    log(f"bench: {ops} ops in {ms:.2f} ms (events), {wall * 1e3:.2f} ms wall -> {value:.1f} op/s")
+    log(f"bench: batched {ops} ops in {ms:.2f} ms (graph replays) -> {value:.1f} op/s "
+        f"({value / plain:.2f}x plain; scratch {scratch / 2**30:.2f} GiB)")
 
     # dominant kernel family: the forward NTT (two passes) over 47 x P limbs
     L = ctx.size_Q + ctx.size_P
@@ -201,8 +232,12 @@ def main():
         "dtype": "u64",
         "data": "synthetic (seeded uniform residues, resident in HBM)",
         "config": {"workload": "CKKS HMult+relin, N=65536, 35 data limbs (60+34x50 bit) + 12 special (60 bit), "
-                               "dnum=3, plain path (one ciphertext pair per call)",
-                   "batch_per_step": args.batch, "mode": "plain"},
+                               "dnum=3; batched path: one batch plan over all pairs of a step, replayed as a "
+                               "captured hipGraph",
+                   "batch_per_step": args.batch, "mode": "batched"},
+        "plain_value": round(plain, 2),
+        "plain_ms_per_step": round(ms_plain / args.steps, 4),
+        "batched_over_plain": round(value / plain, 3),
         "op_algorithmic_GBps": round(op_bytes * value / 1e9, 1),
         "ntt": {"limbs": L * args.ntt_polys, "N": N, "ms": round(ntt_ms, 4), "ms_min": round(ntt_min, 4),
                 "GBps_algorithmic": round(achieved, 1)},

@@ -142,6 +142,28 @@ int phx_rotate_inplace(const phx_context *ctx, phx_ciphertext *ct, int step, con
 int phx_keyswitch_inplace(const phx_context *ctx, phx_ciphertext *ct, const uint64_t *c2_dev,
                           const phx_relin_key *key, int is_relin, phx_stream stream);
 
+/* ---- batched hot path (no single reference counterpart: it runs the reference's
+ * multiply_and_relin_inplace / rotate_inplace over many ciphertexts at once) -----------
+ * A batch binds a fixed list of ciphertexts (their device buffers must not be resized
+ * while the batch lives) to preallocated scratch; ciphertexts are grouped by level and
+ * every stage of the pipeline is one launch per group. run() launches eagerly,
+ * capture() records the launches into a hipGraph on an explicit (non-NULL) stream and
+ * replay() launches that graph. Results are bit-identical to the one-at-a-time calls. */
+typedef struct phx_batch phx_batch;
+/* ct1[i] = relin(ct1[i] * ct2[i]); ct1[i] may equal ct2[i] (squaring) */
+int phx_batch_multiply_and_relin_create(const phx_context *ctx, phx_ciphertext *const *ct1,
+                                        const phx_ciphertext *const *ct2, size_t n, const phx_relin_key *rlk,
+                                        phx_batch **out);
+/* cts[i] = rotate(cts[i], steps[i]); each step's Galois element must have a key in gk */
+int phx_batch_rotate_create(const phx_context *ctx, phx_ciphertext *const *cts, const int *steps, size_t n,
+                            const phx_galois_key *gk, phx_batch **out);
+int phx_batch_run(phx_batch *batch, phx_stream stream);
+int phx_batch_capture(phx_batch *batch, phx_stream stream);
+int phx_batch_replay(phx_batch *batch, phx_stream stream);
+/* bytes of device scratch the batch holds */
+int phx_batch_scratch_bytes(const phx_batch *batch, size_t *bytes);
+int phx_batch_destroy(phx_batch *batch);
+
 /* ---- low-level RNS kernels (device pointers) ------------------------------- */
 /* nwt_2d_radix8_forward_inplace (include/ntt.cuh:173-226, src/ntt/fntt_2d.cu:620-653):
  * limbs [0, L) of dev use QP table rows [start, start + L) */

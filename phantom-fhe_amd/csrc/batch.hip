@@ -1,0 +1,339 @@
+// batch.hip -- batched multiply-and-relinearise and rotation (see batch.h).
+#include <algorithm>
+#include <map>
+
+#include "batch.h"
+
+namespace phx {
+
+namespace {
+constexpr int kThreads = 256;
+inline dim3 grid3(u64 N, u32 y, u32 z) { return dim3((u32)((N + kThreads - 1) / kThreads), y, z); }
+}  // namespace
+
+// tensor of item z: (d0, d1) into out[z], d2 into c2[z]
+// (tensor_prod_2x2_rns_poly, src/polymath.cu:463-498)
+__global__ __launch_bounds__(kThreads) void k_tensor_batch(const u64 *const *a, const u64 *const *b, u64 *const *out,
+                                                           u64 *__restrict__ c2, const DevModulus *__restrict__ mod,
+                                                           u32 N, u32 Ql) {
+    const u32 limb = blockIdx.y, item = blockIdx.z;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 LN = (u64)Ql * N, o = (u64)limb * N + k;
+    const u64 *pa = a[item], *pb = b[item];
+    const DevModulus m = mod[limb];
+    const u64 q = m.value;
+    const u64 a0 = pa[o], a1 = pa[LN + o], b0 = pb[o], b1 = pb[LN + o];
+    const u64 r0 = mul_mod(a0, b0, m);
+    const u64 r2 = mul_mod(a1, b1, m);
+    u64 r1 = mul_mod(add_mod(a0, a1, q), add_mod(b0, b1, q), m);
+    r1 = sub_mod(sub_mod(r1, r0, q), r2, q);
+    u64 *po = out[item];
+    po[o] = r0;
+    po[LN + o] = r1;
+    c2[(u64)item * LN + o] = r2;
+}
+
+// Galois permutation of item z (apply_galois_ntt_permutation, src/galois.cu:11-18):
+// base0[z] = pi(c0), c2[z] = pi(c1)
+__global__ __launch_bounds__(kThreads) void k_permute_batch(const u64 *const *ct, const u32 *const *perm,
+                                                            u64 *__restrict__ base0, u64 *__restrict__ c2, u32 N,
+                                                            u32 Ql) {
+    const u32 limb = blockIdx.y, item = blockIdx.z;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 LN = (u64)Ql * N, o = (u64)limb * N;
+    const u64 *src = ct[item];
+    const u32 pk = perm[item][k];
+    base0[(u64)item * LN + o + k] = src[o + pk];
+    c2[(u64)item * LN + o + k] = src[LN + o + pk];
+}
+
+// key inner product over a group (key_switch_inner_prod_c2_and_evk,
+// src/eval_key_switch.cu:14-69): items sharing a key read it once per coefficient.
+__global__ __launch_bounds__(kThreads) void k_inner_product_batch(const u64 *__restrict__ t,
+                                                                  const u64 *const *keys, u64 *__restrict__ cx,
+                                                                  const DevModulus *__restrict__ mod, u32 N, u32 Ql,
+                                                                  u32 QlP, u32 size_Q, u32 QP, u32 beta, u32 B) {
+    const u32 pos = blockIdx.y;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u32 row = pos < Ql ? pos : size_Q + (pos - Ql);
+    const u64 QPN = (u64)QP * N;
+    const DevModulus m = mod[row];
+    constexpr u32 kMaxBeta = 8;
+    u64 kb[kMaxBeta], ka[kMaxBeta];
+    const u64 *cur = nullptr;
+    for (u32 item = 0; item < B; ++item) {
+        const u64 *key = keys[item];
+        if (key != cur) {
+            cur = key;
+#pragma unroll
+            for (u32 j = 0; j < kMaxBeta; ++j)
+                if (j < beta) {
+                    const u64 *p = key + (u64)j * 2 * QPN + (u64)row * N + k;
+                    kb[j] = p[0];
+                    ka[j] = p[QPN];
+                }
+        }
+        const u64 *ti = t + (u64)item * beta * QlP * N + (u64)pos * N + k;
+        u128 a0 = 0, a1 = 0;
+#pragma unroll
+        for (u32 j = 0; j < kMaxBeta; ++j)
+            if (j < beta) {
+                const u64 x = ti[(u64)j * QlP * N];
+                a0 += (u128)x * kb[j];
+                a1 += (u128)x * ka[j];
+            }
+        u64 *co = cx + (u64)item * 2 * QlP * N + (u64)pos * N + k;
+        co[0] = barrett128(a0, m.value, m.ratio_lo, m.ratio_hi);
+        co[(u64)QlP * N] = barrett128(a1, m.value, m.ratio_lo, m.ratio_hi);
+    }
+}
+
+// ---------------------------------------------------------------------------
+BatchPlan::~BatchPlan() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    (void)hipDeviceSynchronize();
+    for (void *p : allocations_) (void)hipFree(p);
+}
+
+void BatchPlan::build_groups(const std::vector<u32> &chains) {
+    std::map<u32, std::vector<u32>> by;
+    for (u32 i = 0; i < chains.size(); ++i) by[chains[i]].push_back(i);
+    for (auto &kv : by) {
+        Group g;
+        g.chain = kv.first;
+        const LevelConsts &L = ctx_.level(kv.first);
+        g.Ql = L.size_Ql;
+        g.QlP = L.size_QlP;
+        g.beta = L.beta;
+        g.items = kv.second;
+        g.B = (u32)g.items.size();
+        groups_.push_back(g);
+    }
+}
+
+void BatchPlan::alloc_group(Group &g) {
+    const u64 N = ctx_.N(), LN = (u64)g.Ql * N, LPN = (u64)g.QlP * N;
+    auto alloc = [&](size_t bytes) {
+        void *p = nullptr;
+        PHX_HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+        allocations_.push_back(p);
+        scratch_bytes_ += bytes;
+        return p;
+    };
+    g.c2 = static_cast<u64 *>(alloc(g.B * LN * 8));
+    g.t_cks = static_cast<u64 *>(alloc(g.B * LN * 8));
+    g.t_mod_up = static_cast<u64 *>(alloc((size_t)g.B * g.beta * LPN * 8));
+    g.cx = static_cast<u64 *>(alloc((size_t)g.B * 2 * LPN * 8));
+    g.delta = static_cast<u64 *>(alloc((size_t)g.B * 2 * LN * 8));
+    if (kind_ == kRotate) g.base0 = static_cast<u64 *>(alloc(g.B * LN * 8));
+    g.a = static_cast<const u64 **>(alloc(g.B * sizeof(void *)));
+    g.b = static_cast<const u64 **>(alloc(g.B * sizeof(void *)));
+    g.out = static_cast<u64 **>(alloc(g.B * sizeof(void *)));
+    g.keys = static_cast<const u64 **>(alloc(g.B * sizeof(void *)));
+    g.perms = static_cast<const u32 **>(alloc(g.B * sizeof(void *)));
+}
+
+BatchPlan *BatchPlan::mul_relin(const Context &ctx, const std::vector<Ciphertext *> &ct1,
+                                const std::vector<const Ciphertext *> &ct2, const KSwitchKey &rlk) {
+    require(ctx.params().scheme == kSchemeCKKS, "batched multiply_and_relin: only CKKS is on this path");
+    require(ct1.size() == ct2.size() && !ct1.empty(), "batch operands must be non-empty and of equal count");
+    require(rlk.data != nullptr, "relinearisation key is empty");
+    auto *p = new BatchPlan(ctx, kMulRelin);
+    std::vector<u32> chains;
+    for (size_t i = 0; i < ct1.size(); ++i) {
+        require(ct1[i] && ct2[i] && ct1[i]->data && ct2[i]->data, "batch ciphertext is empty");
+        require(ct1[i]->chain_index == ct2[i]->chain_index, "encrypted1 and encrypted2 parameter mismatch");
+        require(ct1[i]->is_ntt_form && ct2[i]->is_ntt_form, "encrypted1 and encrypted2 must be in NTT form");
+        require(ct1[i]->size == 2 && ct2[i]->size == 2, "poly number mismatch");
+        require(rlk.dnum >= ctx.level(ct1[i]->chain_index).beta, "key-switching key has too few digits");
+        for (size_t j = 0; j < ct1.size(); ++j) {
+            if (j < i) require(ct1[j] != ct1[i], "a ciphertext appears twice as a batch output");
+            // an output may only alias its own second operand (squaring)
+            if (j != i) require(ct2[j] != ct1[i], "a batch output is another item's operand");
+        }
+        chains.push_back(ct1[i]->chain_index);
+        p->cts_.push_back(ct1[i]);
+        p->scale_mul_.push_back(ct2[i]->scale);
+    }
+    p->items_ = ct1.size();
+    p->build_groups(chains);
+    for (auto &g : p->groups_) {
+        p->alloc_group(g);
+        std::vector<const u64 *> a, b, keys;
+        std::vector<u64 *> out;
+        for (u32 i : g.items) {
+            a.push_back(ct1[i]->data);
+            b.push_back(ct2[i]->data);
+            out.push_back(ct1[i]->data);
+            keys.push_back(rlk.data);
+        }
+        PHX_HIP_CHECK(hipMemcpy(g.a, a.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
+        PHX_HIP_CHECK(hipMemcpy(g.b, b.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
+        PHX_HIP_CHECK(hipMemcpy(g.out, out.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
+        PHX_HIP_CHECK(hipMemcpy(g.keys, keys.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
+    }
+    return p;
+}
+
+BatchPlan *BatchPlan::rotate(const Context &ctx, const std::vector<Ciphertext *> &cts, const std::vector<int> &steps,
+                             const GaloisKeys &gk) {
+    require(cts.size() == steps.size() && !cts.empty(), "batch operands must be non-empty and of equal count");
+    auto *p = new BatchPlan(ctx, kRotate);
+    std::vector<u32> chains;
+    std::vector<const u64 *> item_key(cts.size());
+    std::vector<const u32 *> item_perm(cts.size());
+    for (size_t i = 0; i < cts.size(); ++i) {
+        require(cts[i] && cts[i]->data, "batch ciphertext is empty");
+        require(cts[i]->size == 2 && cts[i]->is_ntt_form, "encrypted size must be 2 (NTT form)");
+        for (size_t j = 0; j < i; ++j) require(cts[j] != cts[i], "a ciphertext appears twice in a batch");
+        const u32 elt = galois_elt_from_step(steps[i], ctx.N());
+        auto it = std::find(gk.elts.begin(), gk.elts.end(), elt);
+        require(it != gk.elts.end(), "Galois elt not present");
+        const KSwitchKey &key = gk.keys[it - gk.elts.begin()];
+        require(key.dnum >= ctx.level(cts[i]->chain_index).beta, "key-switching key has too few digits");
+        item_key[i] = key.data;
+        item_perm[i] = ctx.galois_perm(elt);
+        chains.push_back(cts[i]->chain_index);
+        p->cts_.push_back(cts[i]);
+    }
+    p->items_ = cts.size();
+    p->build_groups(chains);
+    for (auto &g : p->groups_) {
+        p->alloc_group(g);
+        // items sharing a key are placed next to each other so the inner product reuses it
+        std::stable_sort(g.items.begin(), g.items.end(),
+                         [&](u32 x, u32 y) { return item_key[x] < item_key[y]; });
+        std::vector<const u64 *> a, keys;
+        std::vector<u64 *> out;
+        std::vector<const u32 *> perms;
+        for (u32 i : g.items) {
+            a.push_back(cts[i]->data);
+            out.push_back(cts[i]->data);
+            keys.push_back(item_key[i]);
+            perms.push_back(item_perm[i]);
+        }
+        PHX_HIP_CHECK(hipMemcpy(g.a, a.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
+        PHX_HIP_CHECK(hipMemcpy(g.out, out.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
+        PHX_HIP_CHECK(hipMemcpy(g.keys, keys.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
+        PHX_HIP_CHECK(hipMemcpy(g.perms, perms.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
+    }
+    return p;
+}
+
+// The same pipeline as keyswitch() in evaluator.hip, each stage over the group.
+void BatchPlan::run_group(const Group &g, hipStream_t s) {
+    const Context &ctx = ctx_;
+    const LevelConsts &L = ctx.level(g.chain);
+    const u64 N = ctx.N();
+    const u32 Ql = g.Ql, QlP = g.QlP, beta = g.beta, B = g.B, P = ctx.size_P(), alpha = ctx.size_P();
+    const u64 LN = (u64)Ql * N, LPN = (u64)QlP * N;
+    if (kind_ == kMulRelin) {
+        hipLaunchKernelGGL(k_tensor_batch, grid3(N, Ql, B), dim3(kThreads), 0, s, g.a, g.b, g.out, g.c2,
+                           ctx.dev_mod(), (u32)N, Ql);
+    } else {
+        hipLaunchKernelGGL(k_permute_batch, grid3(N, Ql, B), dim3(kThreads), 0, s, g.a, g.perms, g.base0, g.c2,
+                           (u32)N, Ql);
+    }
+    PHX_HIP_CHECK(hipGetLastError());
+    // modup: INTT(c2) * partQlHatInv for every item
+    LimbMap mi = limb_map(g.t_cks, 0, Ql, LN);
+    mi.src = g.c2;
+    mi.src_poly_stride = LN;
+    mi.scale = L.modup_scale;
+    ntt_inverse(ctx.tables(), mi, Ql, B, s);
+    // FastBConv of every digit of every item
+    BconvArgs ba{};
+    ba.cbuf = ctx.const_buf();
+    ba.descs = L.modup_bconv;
+    ba.desc_stride = 1;
+    ba.jobs_per_batch = beta;
+    ba.in = g.t_cks;
+    ba.in_job_stride = 0;
+    ba.in_batch_stride = LN;
+    ba.out = g.t_mod_up;
+    ba.out_job_stride = LPN;
+    ba.out_batch_stride = (u64)beta * LPN;
+    ba.copy_src = g.c2;
+    ba.copy_job_stride = 0;
+    ba.copy_batch_stride = LN;
+    ba.mod = ctx.dev_mod();
+    ba.N = (u32)N;
+    bconv_run(ctx, L.h_modup_bconv, ba, B * beta, s);
+    // NTT of the complement limbs of every (item, digit)
+    LimbMap mf = limb_map(g.t_mod_up, 0, QlP, LPN);
+    mf.split = Ql;
+    mf.special_row = ctx.size_Q();
+    mf.skip_alpha = alpha;
+    mf.skip_cap = Ql;
+    mf.skip_mod = beta;
+    mf.nlimbs = QlP;
+    const u32 last_sz = Ql - alpha * (beta - 1);
+    ntt_forward(ctx.tables(), mf, QlP - last_sz, B * beta, s);
+    // inner product (a shared key is read once per coefficient for the group)
+    require(beta <= 8, "batched key switching supports at most 8 digits");
+    hipLaunchKernelGGL(k_inner_product_batch, dim3((u32)(N / kThreads), QlP), dim3(kThreads), 0, s, g.t_mod_up,
+                       g.keys, g.cx, ctx.dev_mod(), (u32)N, Ql, QlP, ctx.size_Q(), ctx.size_QP(), beta, B);
+    PHX_HIP_CHECK(hipGetLastError());
+    // moddown of both halves of every item
+    LimbMap md = limb_map(g.cx + LN, ctx.size_Q(), P, LPN);
+    md.scale = L.moddown_scale;
+    ntt_inverse(ctx.tables(), md, P, 2 * B, s);
+    BconvArgs bd{};
+    bd.cbuf = ctx.const_buf();
+    bd.descs = L.moddown_bconv;
+    bd.desc_stride = 0;
+    bd.jobs_per_batch = 1;
+    bd.in = g.cx;
+    bd.in_batch_stride = LPN;
+    bd.out = g.delta;
+    bd.out_batch_stride = LN;
+    bd.mod = ctx.dev_mod();
+    bd.N = (u32)N;
+    bconv_run(ctx, {L.h_moddown_bconv}, bd, 2 * B, s);
+    StoreModDownBatch st{};
+    st.out = g.out;
+    st.base0 = kind_ == kRotate ? g.base0 : nullptr;
+    st.cx = g.cx;
+    st.pinv = L.pinv;
+    st.cx_stride = LPN;
+    st.N = (u32)N;
+    st.Ql = Ql;
+    ntt_forward_store_batch(ctx.tables(), limb_map(g.delta, 0, Ql, LN), Ql, 2 * B, st, s);
+}
+
+void BatchPlan::run(hipStream_t s) {
+    for (const auto &g : groups_) run_group(g, s);
+    if (kind_ == kMulRelin)
+        for (size_t i = 0; i < cts_.size(); ++i) cts_[i]->scale *= scale_mul_[i];
+}
+
+void BatchPlan::capture(hipStream_t s) {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    exec_ = nullptr;
+    graph_ = nullptr;
+    PHX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        for (const auto &g : groups_) run_group(g, s);
+    } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(s, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+        throw;
+    }
+    PHX_HIP_CHECK(hipStreamEndCapture(s, &graph_));
+    PHX_HIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+}
+
+void BatchPlan::replay(hipStream_t s) {
+    require(exec_ != nullptr, "batch has not been captured");
+    PHX_HIP_CHECK(hipGraphLaunch(exec_, s));
+    if (kind_ == kMulRelin)
+        for (size_t i = 0; i < cts_.size(); ++i) cts_[i]->scale *= scale_mul_[i];
+}
+
+}  // namespace phx

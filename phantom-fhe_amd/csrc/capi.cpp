@@ -4,6 +4,7 @@
 #include <cstring>
 #include <string>
 
+#include "batch.h"
 #include "evaluator.h"
 #include "tune.h"
 
@@ -23,6 +24,9 @@ struct phx_relin_key {
 };
 struct phx_galois_key {
     GaloisKeys gk;
+};
+struct phx_batch {
+    BatchPlan *plan;
 };
 
 namespace {
@@ -471,6 +475,84 @@ int phx_keyswitch_inplace(const phx_context *ctx, phx_ciphertext *ct, const uint
         require(ct->ct.size >= 2, "ciphertext must have two polynomials");
         const u64 LN = (u64)ct->ct.coeff_modulus_size * ctx->ctx->N();
         keyswitch(*ctx->ctx, ct->ct.chain_index, c2, key->key, ct->ct.data, ct->ct.data, ct->ct.data + LN, S(s));
+    });
+}
+
+int phx_batch_multiply_and_relin_create(const phx_context *ctx, phx_ciphertext *const *ct1,
+                                        const phx_ciphertext *const *ct2, size_t n, const phx_relin_key *rlk,
+                                        phx_batch **out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct1, "ct1");
+        nn(ct2, "ct2");
+        nn(rlk, "relin_keys");
+        nn(out, "out");
+        std::vector<Ciphertext *> a;
+        std::vector<const Ciphertext *> b;
+        for (size_t i = 0; i < n; ++i) {
+            nn(ct1[i], "ct1[i]");
+            nn(ct2[i], "ct2[i]");
+            a.push_back(&ct1[i]->ct);
+            b.push_back(&ct2[i]->ct);
+        }
+        PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+        *out = new phx_batch{BatchPlan::mul_relin(*ctx->ctx, a, b, rlk->key)};
+    });
+}
+
+int phx_batch_rotate_create(const phx_context *ctx, phx_ciphertext *const *cts, const int *steps, size_t n,
+                            const phx_galois_key *gk, phx_batch **out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(cts, "cts");
+        nn(steps, "steps");
+        nn(gk, "galois_keys");
+        nn(out, "out");
+        std::vector<Ciphertext *> a;
+        for (size_t i = 0; i < n; ++i) {
+            nn(cts[i], "cts[i]");
+            a.push_back(&cts[i]->ct);
+        }
+        PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+        *out = new phx_batch{BatchPlan::rotate(*ctx->ctx, a, std::vector<int>(steps, steps + n), gk->gk)};
+    });
+}
+
+int phx_batch_run(phx_batch *b, phx_stream s) {
+    return guard([&] {
+        nn(b, "batch");
+        b->plan->run(S(s));
+    });
+}
+
+int phx_batch_capture(phx_batch *b, phx_stream s) {
+    return guard([&] {
+        nn(b, "batch");
+        require(s != nullptr, "graph capture needs an explicit stream");
+        b->plan->capture(S(s));
+    });
+}
+
+int phx_batch_replay(phx_batch *b, phx_stream s) {
+    return guard([&] {
+        nn(b, "batch");
+        b->plan->replay(S(s));
+    });
+}
+
+int phx_batch_scratch_bytes(const phx_batch *b, size_t *bytes) {
+    return guard([&] {
+        nn(b, "batch");
+        nn(bytes, "bytes");
+        *bytes = b->plan->scratch_bytes();
+    });
+}
+
+int phx_batch_destroy(phx_batch *b) {
+    return guard([&] {
+        if (!b) return;
+        delete b->plan;
+        delete b;
     });
 }
 

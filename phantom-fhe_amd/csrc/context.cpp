@@ -47,31 +47,49 @@ std::vector<uint8_t> bconv_mfma_operands(const std::vector<u64> &mat, u32 ni, u3
                                          u32 &rt) {
     // 4 outputs per group of 3 tiles; tile 3T+s, row 4g+i holds output 4T+g, slice b = 4s+i
     rt = 3 * ((no + 3) / 4);
-    std::vector<uint8_t> blob((size_t)rt * 2 * 64 * 16, 0);
-    // mp[o][i][a] = 2^(7a) * mat[o][i] mod p_o
-    std::vector<u64> mp((size_t)no * ni * 9);
+    const size_t a_bytes = (size_t)rt * 2 * 64 * 16;
+    std::vector<uint8_t> blob(a_bytes + (size_t)no * 16, 0);
+    // mp[o][i][a] = 2^(8a) * mat[o][i] mod p_o (a indexes the input's bytes)
+    std::vector<u64> mp((size_t)no * ni * 8);
     for (u32 o = 0; o < no; ++o) {
         const u64 p = omod[o];
         for (u32 i = 0; i < ni; ++i) {
             u64 v = mat[(size_t)o * ni + i] % p;
-            for (u32 a = 0; a < 9; ++a) {
-                mp[((size_t)o * ni + i) * 9 + a] = v;
-                v = hm::mulmod(v, 128, p);
+            for (u32 a = 0; a < 8; ++a) {
+                mp[((size_t)o * ni + i) * 8 + a] = v;
+                v = hm::mulmod(v, 256, p);
             }
         }
     }
-    // A[row][K]: row -> (output, slice) as above, K = i*9 + a
+    auto slice = [&](u32 o, u32 b, u32 i, u32 a) -> uint8_t {
+        return (uint8_t)((mp[((size_t)o * ni + i) * 8 + a] >> (7 * b)) & 127);
+    };
+    // A[row][K]: row -> (output, 7-bit slice) as above, K = i*8 + a
     auto A = [&](u32 row, u32 K) -> uint8_t {
         const u32 t = row / 16, rr = row % 16;
-        const u32 o = 4 * (t / 3) + rr / 4, b = 4 * (t % 3) + rr % 4, i = K / 9, a = K % 9;
+        const u32 o = 4 * (t / 3) + rr / 4, b = 4 * (t % 3) + rr % 4, i = K / 8, a = K % 8;
         if (o >= no || b >= 9 || i >= ni) return 0;
-        return (uint8_t)((mp[((size_t)o * ni + i) * 9 + a] >> (7 * b)) & 127);
+        return slice(o, b, i, a);
     };
     for (u32 t = 0; t < rt; ++t)
         for (u32 h = 0; h < 2; ++h)
             for (u32 l = 0; l < 64; ++l)
                 for (u32 j = 0; j < 16; ++j)
                     blob[(((size_t)t * 2 + h) * 64 + l) * 16 + j] = A(16 * t + l % 16, 64 * h + 16 * (l / 16) + j);
+    // bias of the signed input bytes: the kernel feeds s = byte - 128, so output o
+    // adds back C_o = 128 * sum_b 2^(7b) sum_(i,a) slice_b(mp[o][i][a])  (< 2^80)
+    for (u32 o = 0; o < no; ++o) {
+        hm::u128 c = 0;
+        for (u32 b = 0; b < 9; ++b) {
+            u64 r = 0;
+            for (u32 i = 0; i < ni; ++i)
+                for (u32 a = 0; a < 8; ++a) r += slice(o, b, i, a);
+            c += ((hm::u128)r * 128) << (7 * b);
+        }
+        const u64 lo = (u64)c, hi = (u64)(c >> 64);
+        std::memcpy(blob.data() + a_bytes + (size_t)o * 16, &lo, 8);
+        std::memcpy(blob.data() + a_bytes + (size_t)o * 16 + 8, &hi, 8);
+    }
     return blob;
 }
 
@@ -191,7 +209,7 @@ Context::Context(const Params &p) : params_(p) {
     // MFMA operands of a FastBConv matrix (only when K = 9 * ni fits one 128-wide slab)
     auto add_mfma = [&](BconvDesc &d, const std::vector<u64> &mat, const std::vector<u64> &om) {
         d.rt = 0;
-        if (d.ni > 14) return;
+        if (d.ni > 16) return;
         u32 rt = 0;
         std::vector<uint8_t> blob = bconv_mfma_operands(mat, d.ni, d.no, om, rt);
         std::vector<u64> words(blob.size() / 8);

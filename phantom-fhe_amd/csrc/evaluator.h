@@ -45,6 +45,27 @@ struct SecretKey {
     u64 *s2 = nullptr;   // s^2, [QP][N] NTT form (lazily computed)
 };
 
+// FastBConv launch arguments. Grid row y = batch * jobs_per_batch + job; job j
+// uses descs[j * desc_stride]; pointers advance by job and batch strides.
+struct BconvArgs {
+    const u64 *cbuf;
+    const BconvDesc *descs;
+    u32 desc_stride;
+    u32 jobs_per_batch;
+    const u64 *in;
+    u64 in_job_stride, in_batch_stride;
+    u64 *out;
+    u64 out_job_stride, out_batch_stride;
+    const u64 *copy_src;   // modup: the digit's own limbs are copied from here (or null)
+    u64 copy_job_stride, copy_batch_stride;
+    const DevModulus *mod;
+    u32 N;
+};
+// runs FastBConv over `jobs` grid rows: matrix cores when every descriptor has
+// an MFMA form (and the bconv_mfma switch is on), the VALU kernel otherwise
+void bconv_run(const Context &ctx, const std::vector<BconvDesc> &descs_host, const BconvArgs &args, u32 jobs,
+               hipStream_t s);
+
 // ---- hot path ----
 void tensor_2x2(const Context &ctx, u32 Ql, const u64 *a, const u64 *b, u64 *d0, u64 *d1, u64 *d2, hipStream_t s);
 // ct[0..1] (Ql limbs each) = base0/base1 + keyswitch(c2); base may be null (zero)

@@ -26,20 +26,18 @@ inline dim3 grid_for(u64 N, u32 y, u32 z = 1) { return dim3((u32)((N + kThreads 
 // reference: bconv_matmul_padded_unroll2_kernel src/rns_bconv.cu:455-485,
 //            bconv_matmul_unroll2_kernel 143-168, modup_copy_partQl 522-528
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_bconv(const u64 *__restrict__ cbuf, const BconvDesc *__restrict__ descs,
-                                                    u32 desc_stride, const u64 *__restrict__ in, u64 in_job_stride,
-                                                    u64 *__restrict__ out, u64 out_job_stride,
-                                                    const u64 *__restrict__ copy_src, u64 copy_job_stride,
-                                                    const DevModulus *__restrict__ mod, u32 N) {
-    const u32 job = blockIdx.y;
-    const BconvDesc d = descs[job * desc_stride];
+__global__ __launch_bounds__(kThreads) void k_bconv(BconvArgs A) {
+    const u32 j = blockIdx.y % A.jobs_per_batch, bt = blockIdx.y / A.jobs_per_batch;
+    const BconvDesc d = A.descs[j * A.desc_stride];
+    const u32 N = A.N;
     const u32 k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= N) return;
-    const u64 *inj = in + job * in_job_stride + (u64)d.in_first * N + k;
-    u64 *outj = out + job * out_job_stride + k;
-    const u64 *mat = cbuf + d.mat_off;
-    const u32 *pos = reinterpret_cast<const u32 *>(cbuf) + d.pos_off;
-    const u32 *rows = reinterpret_cast<const u32 *>(cbuf) + d.row_off;
+    const u64 *inj = A.in + bt * A.in_batch_stride + j * A.in_job_stride + (u64)d.in_first * N + k;
+    u64 *outj = A.out + bt * A.out_batch_stride + j * A.out_job_stride + k;
+    const u64 *mat = A.cbuf + d.mat_off;
+    const u32 *pos = reinterpret_cast<const u32 *>(A.cbuf) + d.pos_off;
+    const u32 *rows = reinterpret_cast<const u32 *>(A.cbuf) + d.row_off;
+    const DevModulus *mod = A.mod;
     u64 x[kMaxDigitLimbs];
 #pragma unroll
     for (int i = 0; i < kMaxDigitLimbs; ++i) x[i] = (u32)i < d.ni ? inj[(u64)i * N] : 0;
@@ -52,8 +50,8 @@ __global__ __launch_bounds__(kThreads) void k_bconv(const u64 *__restrict__ cbuf
         const DevModulus m = mod[rows[o]];
         outj[(u64)pos[o] * N] = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
     }
-    if (copy_src) {
-        const u64 *cs = copy_src + job * copy_job_stride + (u64)d.in_first * N + k;
+    if (A.copy_src) {
+        const u64 *cs = A.copy_src + bt * A.copy_batch_stride + j * A.copy_job_stride + (u64)d.in_first * N + k;
         for (u32 i = 0; i < d.ni; ++i) outj[(u64)(d.own_first + i) * N] = cs[(u64)i * N];
     }
 }
@@ -61,41 +59,57 @@ __global__ __launch_bounds__(kThreads) void k_bconv(const u64 *__restrict__ cbuf
 // ---------------------------------------------------------------------------
 // FastBConv phase 2 on the matrix cores, exact (same outputs as k_bconv).
 //
-// With x_i = sum_a x_ia 2^(7a) (7-bit slices, a < 9) and
-// M'_oia = 2^(7a) M[o][i] mod p_o = sum_b m_oiab 2^(7b):
-//   sum_i x_i M[o][i] = sum_b 2^(7b) Z[o,b]  (mod p_o),
-//   Z[o,b] = sum_(i,a) m_oiab x_ia                 (an int8 GEMM, int32 sums)
-// Z < 9*14*127*127 < 2^21, so the int32 accumulation is exact. Rows (o,b) =
-// o*9+b, K = (i,a) = i*9+a <= 126 -> two v_mfma_i32_16x16x64_i8 per 16x16 tile.
-// The 9 partial sums of an output are recombined into V < 2^78 and reduced
-// with an FP64 quotient estimate (off by at most one) and one correction.
+// With x_i = sum_a x_ia 2^(8a) (its bytes, a < 8) and
+// M'_oia = 2^(8a) M[o][i] mod p_o = sum_b m_oiab 2^(7b) (7-bit slices, b < 9):
+//   sum_i x_i M[o][i] = sum_b 2^(7b) W[o,b]  (mod p_o),
+//   W[o,b] = sum_(i,a) m_oiab x_ia.
+// The matrix cores multiply signed bytes, so the inputs enter as
+// s_ia = x_ia - 128 (bit 7 flipped) and Z[o,b] = sum m_oiab s_ia is an int8
+// GEMM with int32 sums (|Z| < 128*127*128 < 2^21, exact); the constant
+// 128 * sum_b 2^(7b) sum_(i,a) m_oiab is precomputed per output and added
+// back. K = (i,a) = 8i + a <= 128 -> two v_mfma_i32_16x16x64_i8 per 16x16
+// tile; rows are permuted so the 9 sums of an output sit in one lane's
+// accumulators. V = sum_b 2^(7b) Z_b + C_o < 2^79 is reduced with an FP64
+// quotient estimate (off by at most one) and one signed correction.
 // ---------------------------------------------------------------------------
 typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr u32 kBconvTileCols = 16;   // coefficients per MFMA column tile
 constexpr u32 kBconvThreads = 512;   // 8 waves share one staged copy of the operands
 
-__global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(const u64 *__restrict__ cbuf,
-                                                         const BconvDesc *__restrict__ descs, u32 desc_stride,
-                                                         const u64 *__restrict__ in, u64 in_job_stride,
-                                                         u64 *__restrict__ out, u64 out_job_stride,
-                                                         const u64 *__restrict__ copy_src, u64 copy_job_stride,
-                                                         const DevModulus *__restrict__ mod, u32 N) {
+__global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
     extern __shared__ __align__(16) uint8_t smem[];
-    const u32 job = blockIdx.y;
-    const BconvDesc d = descs[job * desc_stride];
+    const u32 j = blockIdx.y % A.jobs_per_batch, bt = blockIdx.y / A.jobs_per_batch;
+    const BconvDesc d = A.descs[j * A.desc_stride];
+    const u64 *cbuf = A.cbuf;
+    const DevModulus *mod = A.mod;
+    const u32 N = A.N;
     const u32 rt = d.rt, ni = d.ni, no = d.no;
     const u32 tid = threadIdx.x;
     // A operand tiles staged once per workgroup: [rt][2][64 lanes] x 16 B
     uint4 *As = reinterpret_cast<uint4 *>(smem);
     const uint4 *Ag = reinterpret_cast<const uint4 *>(cbuf + d.a_off);
     for (u32 i = tid; i < rt * 128; i += kBconvThreads) As[i] = Ag[i];
+    // per-output constants next to the tiles: modulus, fl(1/p), bias C_o, output offset
+    struct OutConst {
+        u64 p;
+        double pinv;
+        u64 bias_lo, bias_hi;
+        u64 off;
+        u64 pad;
+    };
+    OutConst *oc = reinterpret_cast<OutConst *>(smem + (size_t)rt * 2048);
+    {
+        const u32 *pos = reinterpret_cast<const u32 *>(cbuf) + d.pos_off;
+        const u32 *rows = reinterpret_cast<const u32 *>(cbuf) + d.row_off;
+        const double *pinv = reinterpret_cast<const double *>(cbuf + d.pinv_off);
+        const u64 *bias = cbuf + d.a_off + (u64)rt * 256;   // after rt x 2 KiB of operand tiles
+        for (u32 o = tid; o < no; o += kBconvThreads)
+            oc[o] = OutConst{mod[rows[o]].value, pinv[o], bias[2 * o], bias[2 * o + 1], (u64)pos[o] * N, 0};
+    }
     __syncthreads();
 
-    const u64 *inj = in + job * in_job_stride + (u64)d.in_first * N;
-    u64 *outj = out + job * out_job_stride;
-    const u32 *pos = reinterpret_cast<const u32 *>(cbuf) + d.pos_off;
-    const u32 *rows = reinterpret_cast<const u32 *>(cbuf) + d.row_off;
-    const double *pinv = reinterpret_cast<const double *>(cbuf + d.pinv_off);
+    const u64 *inj = A.in + bt * A.in_batch_stride + j * A.in_job_stride + (u64)d.in_first * N;
+    u64 *outj = A.out + bt * A.out_batch_stride + j * A.out_job_stride;
     const u32 wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
     const u32 ntiles = N / kBconvTileCols;
     const u32 groups = rt / 3;
@@ -105,40 +119,27 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(const u64 *__restr
     const u32 nwaves = kBconvThreads / 64;
     u32 ct = blockIdx.x * nwaves + wave;
     const u32 ct_step = gridDim.x * nwaves;
-    u32 iA[2], aA[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const u32 K0 = 64 * h + 16 * lg;
-        iA[h] = K0 / 9;
-        aA[h] = K0 - 9 * iA[h];
-    }
-    u64 xn[2][3];
+    // lane (lg, ln) feeds K = 64h + 16 lg + j: the 8 bytes of inputs 8h + 2 lg and 8h + 2 lg + 1
+    u64 xn[2][2];
     auto load_x = [&](u32 tile) {
         const u32 kk = tile * kBconvTileCols + ln;
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int r = 0; r < 3; ++r) xn[h][r] = iA[h] + r < ni ? inj[(u64)(iA[h] + r) * N + kk] : 0;
+            for (int r = 0; r < 2; ++r) {
+                const u32 i = 8 * h + 2 * lg + r;
+                xn[h][r] = i < ni ? inj[(u64)i * N + kk] : 0;
+            }
     };
     if (ct < ntiles) load_x(ct);
     for (; ct < ntiles; ct += ct_step) {
         const u32 k = ct * kBconvTileCols + ln;   // this lane's coefficient
-        // B operand: 7-bit slices of x_i[k] for K = 64h + 16*lg + j; K0 = 64h + 16 lg
-        // spans inputs i0 .. i0+2 (16 slices < 2 * 9)
+        // B operand: the input bytes as signed int8, s = byte - 128 (flip bit 7)
         v4i b[2];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const u64 x0 = xn[h][0], x1 = xn[h][1], x2 = xn[h][2];
-            u32 w[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (u32 j = 0; j < 16; ++j) {
-                const u32 a = aA[h] + j;   // slice index relative to input i0, < 24
-                const u64 x = a < 9 ? x0 : (a < 18 ? x1 : x2);
-                const u32 sh = 7 * (a < 9 ? a : (a < 18 ? a - 9 : a - 18));
-                w[j >> 2] |= (u32)((x >> sh) & 127ull) << (8 * (j & 3));
-            }
-            b[h] = v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
-        }
+        for (int h = 0; h < 2; ++h)
+            b[h] = v4i{(int)((u32)xn[h][0] ^ 0x80808080u), (int)((u32)(xn[h][0] >> 32) ^ 0x80808080u),
+                       (int)((u32)xn[h][1] ^ 0x80808080u), (int)((u32)(xn[h][1] >> 32) ^ 0x80808080u)};
         if (ct + ct_step < ntiles) load_x(ct + ct_step);
         // 3 tiles per group of 4 outputs; lane (lg, ln) owns output 4T + lg
         for (u32 T = 0; T < groups; ++T) {
@@ -156,25 +157,27 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(const u64 *__restr
             }
             const u32 o = 4 * T + lg;
             if (o < no) {
-                // Z_b = acc[b / 4][b % 4], b < 9
-                const u64 s = (u64)(u32)acc[0][0] + ((u64)(u32)acc[0][1] << 7) + ((u64)(u32)acc[0][2] << 14) +
-                              ((u64)(u32)acc[0][3] << 21) + ((u64)(u32)acc[1][0] << 28) +
-                              ((u64)(u32)acc[1][1] << 35);
-                const u64 Tt = (u64)(u32)acc[1][2] + ((u64)(u32)acc[1][3] << 7) + ((u64)(u32)acc[2][0] << 14);
-                const u64 lo = s + (Tt << 42);
-                const u64 hi = (Tt >> 22) + (lo < s ? 1 : 0);
-                const u64 p = mod[rows[o]].value;
+                // Z_b = acc[b / 4][b % 4] (signed), b < 9; V = sum_b 2^(7b) Z_b + C_o < 2^79
+                const long long sZ = (long long)acc[0][0] + (long long)acc[0][1] * (1ll << 7) +
+                                     (long long)acc[0][2] * (1ll << 14) + (long long)acc[0][3] * (1ll << 21) +
+                                     (long long)acc[1][0] * (1ll << 28) + (long long)acc[1][1] * (1ll << 35);
+                const long long tZ =
+                    (long long)acc[1][2] + (long long)acc[1][3] * (1ll << 7) + (long long)acc[2][0] * (1ll << 14);
+                const OutConst c = oc[o];
+                const __int128 V = (__int128)sZ + (__int128)tZ * ((__int128)1 << 42) +
+                                   (__int128)(((u128)c.bias_hi << 64) | c.bias_lo);
+                const u64 lo = (u64)V, hi = (u64)((u128)V >> 64);
                 const double vd = __builtin_fma((double)hi, 0x1p64, (double)lo);
-                const u64 kq = (u64)__builtin_floor(vd * pinv[o]);
-                u64 r = lo - kq * p;
-                if ((long long)r < 0) r += p;
-                else if (r >= p) r -= p;
-                outj[(u64)pos[o] * N + k] = r;
+                const u64 kq = (u32)__builtin_floor(vd * c.pinv);   // V / p < 2^31
+                u64 r = lo - kq * c.p;
+                if ((long long)r < 0) r += c.p;
+                else if (r >= c.p) r -= c.p;
+                outj[c.off + k] = r;
             }
         }
     }
-    if (copy_src) {
-        const u64 *cs = copy_src + job * copy_job_stride + (u64)d.in_first * N;
+    if (A.copy_src) {
+        const u64 *cs = A.copy_src + bt * A.copy_batch_stride + j * A.copy_job_stride + (u64)d.in_first * N;
         const u64 total = (u64)ni * N;
         for (u64 idx = (u64)blockIdx.x * kBconvThreads + tid; idx < total; idx += (u64)gridDim.x * kBconvThreads) {
             const u64 i = idx / N, kk = idx % N;
@@ -188,17 +191,19 @@ namespace {
 // (the bconv_mfma switch set to 0 forces the VALU kernel).
 bool bconv_force_valu() { return tune_value("bconv_mfma") == 0; }
 
-void bconv_launch(const Context &ctx, const BconvDesc *descs_dev, const std::vector<BconvDesc> &descs_host,
-                  u32 desc_stride, u32 jobs, const u64 *in, u64 in_job_stride, u64 *out, u64 out_job_stride,
-                  const u64 *copy_src, u64 copy_job_stride, hipStream_t s) {
+}  // namespace
+
+void bconv_run(const Context &ctx, const std::vector<BconvDesc> &descs_host, const BconvArgs &args, u32 jobs,
+               hipStream_t s) {
     const u64 N = ctx.N();
-    u32 rt_max = 0;
+    u32 rt_max = 0, no_max = 0;
     bool mfma = !bconv_force_valu() && N % kBconvTileCols == 0;
     for (const auto &d : descs_host) {
         if (d.rt == 0) mfma = false;
         rt_max = std::max(rt_max, d.rt);
+        no_max = std::max(no_max, d.no);
     }
-    const size_t lds = (size_t)rt_max * 2048;
+    const size_t lds = (size_t)rt_max * 2048 + (size_t)no_max * 48;
     static const size_t lds_limit = [] {
         int dev = 0, v = 0;
         (void)hipGetDevice(&dev);
@@ -211,14 +216,31 @@ void bconv_launch(const Context &ctx, const BconvDesc *descs_dev, const std::vec
         const u32 tiles = (u32)(N / kBconvTileCols);
         const u32 waves = kBconvThreads / 64;
         const u32 gx = std::max<u32>(1, std::min<u32>((tiles + waves - 1) / waves, (512 + jobs - 1) / jobs));
-        hipLaunchKernelGGL(k_bconv_mfma, dim3(gx, jobs), dim3(kBconvThreads), lds, s, ctx.const_buf(),
-                           descs_dev, desc_stride, in, in_job_stride, out, out_job_stride, copy_src, copy_job_stride,
-                           ctx.dev_mod(), (u32)N);
+        hipLaunchKernelGGL(k_bconv_mfma, dim3(gx, jobs), dim3(kBconvThreads), lds, s, args);
     } else {
-        hipLaunchKernelGGL(k_bconv, grid_for(N, jobs), dim3(kThreads), 0, s, ctx.const_buf(), descs_dev, desc_stride,
-                           in, in_job_stride, out, out_job_stride, copy_src, copy_job_stride, ctx.dev_mod(), (u32)N);
+        hipLaunchKernelGGL(k_bconv, grid_for(N, jobs), dim3(kThreads), 0, s, args);
     }
     PHX_HIP_CHECK(hipGetLastError());
+}
+
+namespace {
+void bconv_launch(const Context &ctx, const BconvDesc *descs_dev, const std::vector<BconvDesc> &descs_host,
+                  u32 desc_stride, u32 jobs, const u64 *in, u64 in_job_stride, u64 *out, u64 out_job_stride,
+                  const u64 *copy_src, u64 copy_job_stride, hipStream_t s) {
+    BconvArgs a{};
+    a.cbuf = ctx.const_buf();
+    a.descs = descs_dev;
+    a.desc_stride = desc_stride;
+    a.jobs_per_batch = jobs;
+    a.in = in;
+    a.in_job_stride = in_job_stride;
+    a.out = out;
+    a.out_job_stride = out_job_stride;
+    a.copy_src = copy_src;
+    a.copy_job_stride = copy_job_stride;
+    a.mod = ctx.dev_mod();
+    a.N = (u32)ctx.N();
+    bconv_run(ctx, descs_host, a, jobs, s);
 }
 }  // namespace
 

@@ -354,6 +354,11 @@ void ntt_forward_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npo
     launch_forward(T, M, nlimbs, npolys, LoadPlain{}, st, s);
 }
 
+void ntt_forward_store_batch(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys,
+                             const StoreModDownBatch &st, hipStream_t s) {
+    launch_forward(T, M, nlimbs, npolys, LoadPlain{}, st, s);
+}
+
 void ntt_forward_rescale(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const LoadRescale &ld,
                          const StoreRescale &st, hipStream_t s) {
     launch_forward(T, M, nlimbs, npolys, ld, st, s);

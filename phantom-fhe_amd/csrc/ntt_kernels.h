@@ -39,14 +39,15 @@ struct LimbMap {
     u32 split;          // limbs >= split use special rows
     u32 special_row;    // table row of limb `split`
     u32 skip_lo, skip_hi;
-    u32 skip_alpha;     // nonzero: poly z skips limbs [alpha*z, min(alpha*(z+1), skip_cap))
-    u32 skip_cap;
+    u32 skip_alpha;     // nonzero: poly z skips limbs [alpha*d, min(alpha*(d+1), skip_cap)),
+    u32 skip_cap;       //   digit d = z (skip_mod == 0) or z % skip_mod
+    u32 skip_mod;
     u32 nlimbs;         // limbs >= nlimbs are not processed (ragged launches)
 
     __device__ __forceinline__ u32 limb(u32 y, u32 z) const {
         u32 lo = skip_lo, hi = skip_hi;
         if (skip_alpha) {
-            lo = skip_alpha * z;
+            lo = skip_alpha * (skip_mod ? z % skip_mod : z);
             hi = lo + skip_alpha < skip_cap ? lo + skip_alpha : skip_cap;
         }
         return y < lo ? y : y + (hi - lo);

@@ -103,6 +103,15 @@ _SIGS = {
     "phx_event_destroy": (ctypes.c_int, [_vp]),
     "phx_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
     "phx_copy_u64": (ctypes.c_int, [_u64p, _u64p, _sz, _vp]),
+    "phx_batch_multiply_and_relin_create": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _sz, _vp,
+                                                           ctypes.POINTER(_vp)]),
+    "phx_batch_rotate_create": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int), _sz, _vp,
+                                               ctypes.POINTER(_vp)]),
+    "phx_batch_run": (ctypes.c_int, [_vp, _vp]),
+    "phx_batch_capture": (ctypes.c_int, [_vp, _vp]),
+    "phx_batch_replay": (ctypes.c_int, [_vp, _vp]),
+    "phx_batch_scratch_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(_sz)]),
+    "phx_batch_destroy": (ctypes.c_int, [_vp]),
     "phx_tune_set": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "phx_tune_get": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
 }
@@ -410,6 +419,50 @@ def rotate_inplace(ctx, ct, step, gk, stream=None):
 
 def keyswitch_inplace(ctx, ct, c2_dev, key, is_relin=True, stream=None):
     _check(lib().phx_keyswitch_inplace(ctx._h, ct._h, c2_dev, key._h, int(is_relin), _stream(stream)))
+
+
+# ------------------------------------------------------------------ batches
+class batch:
+    """Batched hot path over a fixed list of ciphertexts (phx_batch_*)."""
+
+    def __init__(self, h, keep):
+        self._h = h
+        self._keep = keep   # keep the ciphertexts/keys alive while the batch lives
+
+    @classmethod
+    def multiply_and_relin(cls, ctx, ct1s, ct2s, rlk):
+        a = (_vp * len(ct1s))(*[c._h for c in ct1s])
+        b = (_vp * len(ct2s))(*[c._h for c in ct2s])
+        h = _vp()
+        _check(lib().phx_batch_multiply_and_relin_create(ctx._h, a, b, len(ct1s), rlk._h, ctypes.byref(h)))
+        return cls(h, (ctx, list(ct1s), list(ct2s), rlk))
+
+    @classmethod
+    def rotate(cls, ctx, cts, steps, gk):
+        a = (_vp * len(cts))(*[c._h for c in cts])
+        st = (ctypes.c_int * len(steps))(*steps)
+        h = _vp()
+        _check(lib().phx_batch_rotate_create(ctx._h, a, st, len(cts), gk._h, ctypes.byref(h)))
+        return cls(h, (ctx, list(cts), gk))
+
+    def run(self, stream=None):
+        _check(lib().phx_batch_run(self._h, _stream(stream)))
+
+    def capture(self, stream):
+        _check(lib().phx_batch_capture(self._h, stream))
+
+    def replay(self, stream=None):
+        _check(lib().phx_batch_replay(self._h, _stream(stream)))
+
+    def scratch_bytes(self):
+        v = _sz()
+        _check(lib().phx_batch_scratch_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.phx_batch_destroy(self._h)
+            self._h = None
 
 
 # ------------------------------------------------------------------ device buffers

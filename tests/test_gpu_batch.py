@@ -1,0 +1,127 @@
+"""Batched hot path (phx_batch_*): every item bit-exact against the oracle,
+for mixed levels, squaring, mixed rotation steps, and hipGraph replays."""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import pyphantom as ph
+from gpu_helpers import make_ctx, rand_ct, rand_key, say, upload_ct
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small():
+    ob.set_threads(8)
+    ctx, octx, mods = make_ctx(1 << 13, [60, 50, 50, 50, 50, 50, 50, 60, 60, 60], 3)
+    rng = np.random.default_rng(7)
+    key = rand_key(rng, mods, ctx.N, ctx.dnum)
+    return ctx, octx, mods, key
+
+
+def test_batch_multiply_and_relin_mixed_levels(small):
+    ctx, octx, mods, key = small
+    N = ctx.N
+    say("case: batch mulrelin mixed levels")
+    rng = np.random.default_rng(1)
+    chains = [1, 1, 3, 7, 2, 1]
+    data = []
+    for c in chains:
+        Ql = ctx.limbs_at(c)
+        data.append((c, Ql, rand_ct(rng, mods[:Ql], N), rand_ct(rng, mods[:Ql], N)))
+    rk = ph.relin_key(ctx, host=key)
+    c1 = [upload_ct(ctx, a, c) for c, _, a, _ in data]
+    c2 = [upload_ct(ctx, b, c) for c, _, _, b in data]
+    c2[4] = c1[4]   # squaring item: operand aliases the output
+    bt = ph.batch.multiply_and_relin(ctx, c1, c2, rk)
+    bt.run()
+    ph.device_synchronize()
+    for i, (c, Ql, a, b) in enumerate(data):
+        other = a if i == 4 else b
+        assert np.array_equal(c1[i].download(N), octx.multiply_and_relin(Ql, a, other, key)), i
+
+
+def test_batch_rotate_mixed_steps(small):
+    ctx, octx, mods, key = small
+    N = ctx.N
+    say("case: batch rotate")
+    steps_set = [1, 2, -1, 5]
+    elts = [ph.galois_elt_from_step(s, N) for s in steps_set]
+    rng = np.random.default_rng(2)
+    keys = [rand_key(rng, mods, N, ctx.dnum) for _ in elts]
+    gk = ph.galois_key(ctx, elts, host=np.stack(keys))
+    chains = [1, 2, 1, 5, 3, 1, 1]
+    steps = [1, 2, -1, 5, 1, 5, 2]
+    data = [(c, ctx.limbs_at(c), rand_ct(rng, mods[:ctx.limbs_at(c)], N)) for c in chains]
+    cts = [upload_ct(ctx, a, c) for c, _, a in data]
+    bt = ph.batch.rotate(ctx, cts, steps, gk)
+    bt.run()
+    ph.device_synchronize()
+    for i, ((c, Ql, a), st) in enumerate(zip(data, steps)):
+        k = keys[steps_set.index(st)]
+        want = octx.apply_galois(Ql, a, ph.galois_elt_from_step(st, N), k)
+        assert np.array_equal(cts[i].download(N), want), i
+
+
+def test_batch_graph_replays(small):
+    ctx, octx, mods, key = small
+    N = ctx.N
+    say("case: batch graph replays")
+    rng = np.random.default_rng(3)
+    B, chain = 4, 1
+    Ql = ctx.limbs_at(chain)
+    rk = ph.relin_key(ctx, host=key)
+    a = [rand_ct(rng, mods[:Ql], N) for _ in range(B)]
+    b = [rand_ct(rng, mods[:Ql], N) for _ in range(B)]
+    c1 = [upload_ct(ctx, x, chain) for x in a]
+    c2 = [upload_ct(ctx, x, chain) for x in b]
+    want = [octx.multiply_and_relin(Ql, a[i], b[i], key) for i in range(B)]
+    stream = ph.stream_create()
+    bt = ph.batch.multiply_and_relin(ctx, c1, c2, rk)
+    bt.capture(stream)
+    for rep in range(3):
+        for i in range(B):     # eager work between replays: reset the inputs
+            c1[i].upload(a[i])
+        ph.device_synchronize()
+        bt.replay(stream)
+        ph.synchronize(stream)
+        for i in range(B):
+            assert np.array_equal(c1[i].download(N), want[i]), (rep, i)
+
+
+def test_batch_rejects_missing_key_and_aliasing(small):
+    ctx, octx, mods, key = small
+    N = ctx.N
+    rng = np.random.default_rng(4)
+    gk = ph.galois_key(ctx, [ph.galois_elt_from_step(1, N)], host=rand_key(rng, mods, N, ctx.dnum)[None])
+    c = upload_ct(ctx, rand_ct(rng, mods[:7], N), 1)
+    with pytest.raises(ValueError):
+        ph.batch.rotate(ctx, [c], [3], gk)
+    rk = ph.relin_key(ctx, host=key)
+    x = upload_ct(ctx, rand_ct(rng, mods[:7], N), 1)
+    y = upload_ct(ctx, rand_ct(rng, mods[:7], N), 1)
+    with pytest.raises(ValueError):
+        ph.batch.multiply_and_relin(ctx, [x, y], [y, x], rk)
+
+
+def test_batch_config2_bit_exact():
+    """N=2^16, 35 + 12 limbs, dnum 3: four pairs in one batch, graph replay."""
+    ob.set_threads(16)
+    N = 1 << 16
+    say("case: batch config2")
+    ctx, octx, mods = make_ctx(N, [60] + [50] * 34 + [60] * 12, 12)
+    rng = np.random.default_rng(11)
+    key = rand_key(rng, mods, N, 3)
+    rk = ph.relin_key(ctx, host=key)
+    B = 4
+    a = [rand_ct(rng, mods[:35], N) for _ in range(B)]
+    b = [rand_ct(rng, mods[:35], N) for _ in range(B)]
+    c1 = [upload_ct(ctx, x, 1) for x in a]
+    c2 = [upload_ct(ctx, x, 1) for x in b]
+    stream = ph.stream_create()
+    bt = ph.batch.multiply_and_relin(ctx, c1, c2, rk)
+    bt.capture(stream)
+    bt.replay(stream)
+    ph.synchronize(stream)
+    for i in (0, 3):
+        assert np.array_equal(c1[i].download(N), octx.multiply_and_relin(35, a[i], b[i], key)), i
