@@ -177,6 +177,8 @@ def main():
     ap.add_argument("--ntt-polys", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="kernel-variant switch (bconv_mfma, ntt_fp64, ntt_fused, ntt_chunk_limbs, ...)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -184,6 +186,11 @@ def main():
 
     import pyphantom as ph
 
+    tune = {}
+    for kv in args.tune:
+        k, v = kv.split("=", 1)
+        ph.tune_set(k, int(v))
+        tune[k] = int(v)
     N = 1 << 16
     ctx, mods = make_context(ph, N, CONFIG2_BITS, 12)
     stream = ph.stream_create()
@@ -234,7 +241,7 @@ def main():
         "config": {"workload": "CKKS HMult+relin, N=65536, 35 data limbs (60+34x50 bit) + 12 special (60 bit), "
                                "dnum=3; batched path: one batch plan over all pairs of a step, replayed as a "
                                "captured hipGraph",
-                   "batch_per_step": args.batch, "mode": "batched"},
+                   "batch_per_step": args.batch, "mode": "batched", "tune": tune},
         "plain_value": round(plain, 2),
         "plain_ms_per_step": round(ms_plain / args.steps, 4),
         "batched_over_plain": round(value / plain, 3),

@@ -3,6 +3,7 @@
 #include <map>
 
 #include "batch.h"
+#include "tune.h"
 
 namespace phx {
 
@@ -50,12 +51,15 @@ __global__ __launch_bounds__(kThreads) void k_permute_batch(const u64 *const *ct
 }
 
 // key inner product over a group (key_switch_inner_prod_c2_and_evk,
-// src/eval_key_switch.cu:14-69): items sharing a key read it once per coefficient.
-__global__ __launch_bounds__(kThreads) void k_inner_product_batch(const u64 *__restrict__ t,
+// src/eval_key_switch.cu:14-69): items sharing a key read it once per coefficient;
+// digit j's own limbs (pos / alpha == j) are read from c2 (NTT form) directly.
+__global__ __launch_bounds__(kThreads) void k_inner_product_batch(const u64 *__restrict__ t, const u64 *__restrict__ c2,
                                                                   const u64 *const *keys, u64 *__restrict__ cx,
                                                                   const DevModulus *__restrict__ mod, u32 N, u32 Ql,
-                                                                  u32 QlP, u32 size_Q, u32 QP, u32 beta, u32 B) {
+                                                                  u32 QlP, u32 size_Q, u32 QP, u32 beta, u32 alpha,
+                                                                  u32 B) {
     const u32 pos = blockIdx.y;
+    const u32 own = pos < Ql ? pos / alpha : ~0u;
     const u32 k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= N) return;
     const u32 row = pos < Ql ? pos : size_Q + (pos - Ql);
@@ -77,11 +81,12 @@ __global__ __launch_bounds__(kThreads) void k_inner_product_batch(const u64 *__r
                 }
         }
         const u64 *ti = t + (u64)item * beta * QlP * N + (u64)pos * N + k;
+        const u64 *ci = c2 + (u64)item * Ql * N + (u64)pos * N + k;
         u128 a0 = 0, a1 = 0;
 #pragma unroll
         for (u32 j = 0; j < kMaxBeta; ++j)
             if (j < beta) {
-                const u64 x = ti[(u64)j * QlP * N];
+                const u64 x = j == own ? *ci : ti[(u64)j * QlP * N];
                 a0 += (u128)x * kb[j];
                 a1 += (u128)x * ka[j];
             }
@@ -257,7 +262,9 @@ void BatchPlan::run_group(const Group &g, hipStream_t s) {
     ba.out = g.t_mod_up;
     ba.out_job_stride = LPN;
     ba.out_batch_stride = (u64)beta * LPN;
-    ba.copy_src = g.c2;
+    // the inner product reads the digits' own limbs from c2 directly: no copy
+    const bool fused_ip = tune_value("ks_fused_ip") != 0;
+    ba.copy_src = nullptr;
     ba.copy_job_stride = 0;
     ba.copy_batch_stride = LN;
     ba.mod = ctx.dev_mod();
@@ -272,12 +279,31 @@ void BatchPlan::run_group(const Group &g, hipStream_t s) {
     mf.skip_mod = beta;
     mf.nlimbs = QlP;
     const u32 last_sz = Ql - alpha * (beta - 1);
-    ntt_forward(ctx.tables(), mf, QlP - last_sz, B * beta, s);
-    // inner product (a shared key is read once per coefficient for the group)
     require(beta <= 8, "batched key switching supports at most 8 digits");
-    hipLaunchKernelGGL(k_inner_product_batch, dim3((u32)(N / kThreads), QlP), dim3(kThreads), 0, s, g.t_mod_up,
-                       g.keys, g.cx, ctx.dev_mod(), (u32)N, Ql, QlP, ctx.size_Q(), ctx.size_QP(), beta, B);
-    PHX_HIP_CHECK(hipGetLastError());
+    if (fused_ip) {
+        // pass 1 here; pass 2 fused with the inner product
+        ntt_forward_cols(ctx.tables(), mf, QlP - last_sz, B * beta, s);
+        RowsIpArgs ip{};
+        ip.t = g.t_mod_up;
+        ip.c2 = g.c2;
+        ip.c2_stride = LN;
+        ip.keys = g.keys;
+        ip.cx = g.cx;
+        ip.Ql = Ql;
+        ip.QlP = QlP;
+        ip.size_Q = ctx.size_Q();
+        ip.QP = ctx.size_QP();
+        ip.beta = beta;
+        ip.alpha = alpha;
+        ntt_forward_rows_ip(ctx.tables(), ip, B, s);
+    } else {
+        ntt_forward(ctx.tables(), mf, QlP - last_sz, B * beta, s);
+        // inner product (a shared key is read once per coefficient for the group)
+        hipLaunchKernelGGL(k_inner_product_batch, dim3((u32)(N / kThreads), QlP), dim3(kThreads), 0, s, g.t_mod_up,
+                           g.c2, g.keys, g.cx, ctx.dev_mod(), (u32)N, Ql, QlP, ctx.size_Q(), ctx.size_QP(), beta,
+                           alpha, B);
+        PHX_HIP_CHECK(hipGetLastError());
+    }
     // moddown of both halves of every item
     LimbMap md = limb_map(g.cx + LN, ctx.size_Q(), P, LPN);
     md.scale = L.moddown_scale;

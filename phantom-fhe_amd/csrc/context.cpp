@@ -77,7 +77,8 @@ std::vector<uint8_t> bconv_mfma_operands(const std::vector<u64> &mat, u32 ni, u3
                 for (u32 j = 0; j < 16; ++j)
                     blob[(((size_t)t * 2 + h) * 64 + l) * 16 + j] = A(16 * t + l % 16, 64 * h + 16 * (l / 16) + j);
     // bias of the signed input bytes: the kernel feeds s = byte - 128, so output o
-    // adds back C_o = 128 * sum_b 2^(7b) sum_(i,a) slice_b(mp[o][i][a])  (< 2^80)
+    // adds back C_o = 128 * sum_b 2^(7b) sum_(i,a) slice_b(mp[o][i][a])  (< 2^76);
+    // stored as (C_o mod 2^64, fl(C_o) as double bits)
     for (u32 o = 0; o < no; ++o) {
         hm::u128 c = 0;
         for (u32 b = 0; b < 9; ++b) {
@@ -86,9 +87,10 @@ std::vector<uint8_t> bconv_mfma_operands(const std::vector<u64> &mat, u32 ni, u3
                 for (u32 a = 0; a < 8; ++a) r += slice(o, b, i, a);
             c += ((hm::u128)r * 128) << (7 * b);
         }
-        const u64 lo = (u64)c, hi = (u64)(c >> 64);
+        const u64 lo = (u64)c;
+        const double cd = (double)(u64)(c >> 64) * 0x1p64 + (double)lo;
         std::memcpy(blob.data() + a_bytes + (size_t)o * 16, &lo, 8);
-        std::memcpy(blob.data() + a_bytes + (size_t)o * 16 + 8, &hi, 8);
+        std::memcpy(blob.data() + a_bytes + (size_t)o * 16 + 8, &cd, 8);
     }
     return blob;
 }

@@ -38,7 +38,7 @@ struct BconvDesc {
     u64 pos_off;         // offset (u32 units) of out_pos[no]
     u64 row_off;         // offset (u32 units) of the table rows of the outputs
     // MFMA form (ni <= 16): int8 operand tiles of the sliced matrix, see bconv_mfma
-    u64 a_off;           // offset (u64 units) of rt x 2 x 64 lanes x 16 bytes, then no x (lo, hi) bias
+    u64 a_off;           // offset (u64 units) of rt x 2 x 64 lanes x 16 bytes, then no x (lo, fl) bias
     u64 pinv_off;        // offset (u64 units) of fl(1/p_o) as double bits, [no]
     u32 rt;              // 16-row operand tiles: 3 * ceil(no / 4); 0 = no MFMA form
     u32 pad;
@@ -50,7 +50,7 @@ struct BconvDesc {
 // (i, a) = i*8 + a indexes byte a of input i; A[row][K] = bits [7b, 7b+7) of
 // (2^(8a) * mat[o][i] mod p_o). Lane l of tile t, half h holds bytes
 // A[16t + l%16][64h + 16(l/16) + j], j < 16. After the tiles: per output the
-// 128-bit bias C_o (lo, hi) that restores the unsigned input bytes.
+// bias C_o that restores the unsigned input bytes, as (C_o mod 2^64, fl(C_o)).
 std::vector<uint8_t> bconv_mfma_operands(const std::vector<u64> &mat, u32 ni, u32 no,
                                          const std::vector<u64> &omod, u32 &rt);
 

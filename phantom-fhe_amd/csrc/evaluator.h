@@ -80,9 +80,11 @@ void rotate(const Context &ctx, Ciphertext &ct, int step, const GaloisKeys &gk, 
 u32 galois_elt_from_step(int step, u64 N);
 
 // ---- pieces exposed for tests / the low-level API ----
-void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStream_t s);
+// copy_own = false: the digits' own limbs of t_mod_up are left unwritten
+void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStream_t s, bool copy_own = true);
+// c2 given: the digits' own limbs are read from c2 (NTT form) instead of t_mod_up
 void inner_product(const Context &ctx, u32 chain, const u64 *t_mod_up, const KSwitchKey &key, u64 *cx,
-                   hipStream_t s);
+                   hipStream_t s, const u64 *c2 = nullptr);
 void moddown(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, const u64 *base1, u32 npolys,
              hipStream_t s);
 

@@ -92,9 +92,10 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
     // per-output constants next to the tiles: modulus, fl(1/p), bias C_o, output offset
     struct OutConst {
         u64 p;
-        double pinv;
-        u64 bias_lo, bias_hi;
-        u64 off;
+        double pinv;    // fl(1/p)
+        u64 bias_lo;    // C_o mod 2^64
+        double bias;    // fl(C_o)
+        u64 off;        // output limb offset pos[o] * N
         u64 pad;
     };
     OutConst *oc = reinterpret_cast<OutConst *>(smem + (size_t)rt * 2048);
@@ -104,7 +105,8 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
         const double *pinv = reinterpret_cast<const double *>(cbuf + d.pinv_off);
         const u64 *bias = cbuf + d.a_off + (u64)rt * 256;   // after rt x 2 KiB of operand tiles
         for (u32 o = tid; o < no; o += kBconvThreads)
-            oc[o] = OutConst{mod[rows[o]].value, pinv[o], bias[2 * o], bias[2 * o + 1], (u64)pos[o] * N, 0};
+            oc[o] = OutConst{mod[rows[o]].value, pinv[o], bias[2 * o], __longlong_as_double((long long)bias[2 * o + 1]),
+                             (u64)pos[o] * N, 0};
     }
     __syncthreads();
 
@@ -142,8 +144,7 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
                        (int)((u32)xn[h][1] ^ 0x80808080u), (int)((u32)(xn[h][1] >> 32) ^ 0x80808080u)};
         if (ct + ct_step < ntiles) load_x(ct + ct_step);
         // 3 tiles per group of 4 outputs; lane (lg, ln) owns output 4T + lg
-        for (u32 T = 0; T < groups; ++T) {
-            v4i acc[3];
+        auto products = [&](u32 T, v4i(&acc)[3]) {
 #pragma unroll
             for (int sgrp = 0; sgrp < 3; ++sgrp) {
                 const u32 t = 3 * T + sgrp;
@@ -155,25 +156,41 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
                 c = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b[0], c, 0, 0, 0);
                 acc[sgrp] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b[1], c, 0, 0, 0);
             }
+        };
+        // Z_b = acc[b / 4][b % 4] (signed, |Z_b| < 2^21), b < 9;
+        // V = sum_b 2^(7b) Z_b + C_o is in [0, 2^15 p): its low 64 bits exactly,
+        // its value in FP64 to within 2^-33 p, then r = V - floor(V/p - 2^-30) p
+        // lies in [0, 2p)
+        auto reduce = [&](u32 T, const v4i(&acc)[3]) {
             const u32 o = 4 * T + lg;
-            if (o < no) {
-                // Z_b = acc[b / 4][b % 4] (signed), b < 9; V = sum_b 2^(7b) Z_b + C_o < 2^79
-                const long long sZ = (long long)acc[0][0] + (long long)acc[0][1] * (1ll << 7) +
-                                     (long long)acc[0][2] * (1ll << 14) + (long long)acc[0][3] * (1ll << 21) +
-                                     (long long)acc[1][0] * (1ll << 28) + (long long)acc[1][1] * (1ll << 35);
-                const long long tZ =
-                    (long long)acc[1][2] + (long long)acc[1][3] * (1ll << 7) + (long long)acc[2][0] * (1ll << 14);
-                const OutConst c = oc[o];
-                const __int128 V = (__int128)sZ + (__int128)tZ * ((__int128)1 << 42) +
-                                   (__int128)(((u128)c.bias_hi << 64) | c.bias_lo);
-                const u64 lo = (u64)V, hi = (u64)((u128)V >> 64);
-                const double vd = __builtin_fma((double)hi, 0x1p64, (double)lo);
-                const u64 kq = (u32)__builtin_floor(vd * c.pinv);   // V / p < 2^31
-                u64 r = lo - kq * c.p;
-                if ((long long)r < 0) r += c.p;
-                else if (r >= c.p) r -= c.p;
-                outj[c.off + k] = r;
-            }
+            if (o >= no) return;
+            const OutConst c = oc[o];
+            const int y0 = acc[0][0] + acc[0][1] * 128, y1 = acc[0][2] + acc[0][3] * 128;
+            const int y2 = acc[1][0] + acc[1][1] * 128, y3 = acc[1][2] + acc[1][3] * 128;
+            const int y4 = acc[2][0];
+            const u64 lo = c.bias_lo + (u64)((long long)y0 + (long long)y1 * (1ll << 14) +
+                                             (long long)y2 * (1ll << 28)) +
+                           (u64)(long long)y3 * (1ull << 42) + (u64)(long long)y4 * (1ull << 56);
+            double v = __builtin_fma((double)y4, 0x1p56, c.bias);
+            v = __builtin_fma((double)y3, 0x1p42, v);
+            v = __builtin_fma((double)y2, 0x1p28, v);
+            v = __builtin_fma((double)y1, 0x1p14, v);
+            v += (double)y0;
+            const u32 kq = (u32)__builtin_fmax(__builtin_floor(__builtin_fma(v, c.pinv, -0x1p-30)), 0.0);
+            u64 r = lo - (u64)kq * c.p;
+            r = r >= c.p ? r - c.p : r;
+            outj[c.off + k] = r;
+        };
+        // the next group's products are issued before this group's reduction
+        v4i acc0[3], acc1[3];
+        products(0, acc0);
+        for (u32 T = 0; T < groups; T += 2) {
+            const bool odd = T + 1 < groups;
+            if (odd) products(T + 1, acc1);
+            reduce(T, acc0);
+            if (!odd) break;
+            if (T + 2 < groups) products(T + 2, acc0);
+            reduce(T + 1, acc1);
         }
     }
     if (A.copy_src) {
@@ -212,10 +229,24 @@ void bconv_run(const Context &ctx, const std::vector<BconvDesc> &descs_host, con
     }();
     if (lds > lds_limit) mfma = false;
     if (mfma) {
-        // enough workgroups to fill the chip twice over; each stages the operands once
+        // one round of co-resident workgroups (no tail round); each stages the
+        // operands once and walks its job's column tiles
+        static const u32 cus = [] {
+            int dev = 0, v = 0;
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev);
+            return (u32)std::max(1, v);
+        }();
+        static const size_t lds_cu = [] {
+            int dev = 0, v = 0;
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+            return (size_t)std::max(v, 65536);
+        }();
+        const u32 per_cu = (u32)std::max<size_t>(1, std::min<size_t>(lds_cu / lds, 2048 / kBconvThreads));
         const u32 tiles = (u32)(N / kBconvTileCols);
         const u32 waves = kBconvThreads / 64;
-        const u32 gx = std::max<u32>(1, std::min<u32>((tiles + waves - 1) / waves, (512 + jobs - 1) / jobs));
+        const u32 gx = std::max<u32>(1, std::min<u32>((tiles + waves - 1) / waves, cus * per_cu / jobs));
         hipLaunchKernelGGL(k_bconv_mfma, dim3(gx, jobs), dim3(kBconvThreads), lds, s, args);
     } else {
         hipLaunchKernelGGL(k_bconv, grid_for(N, jobs), dim3(kThreads), 0, s, args);
@@ -246,17 +277,21 @@ void bconv_launch(const Context &ctx, const BconvDesc *descs_dev, const std::vec
 
 // key_switch_inner_prod_c2_and_evk (src/eval_key_switch.cu:14-69):
 // cx0[pos] = sum_j t[j][pos] * key_j.b[row], cx1[pos] = sum_j t[j][pos] * key_j.a[row]
-__global__ __launch_bounds__(kThreads) void k_inner_product(const u64 *__restrict__ t, const u64 *__restrict__ key,
-                                                            u64 *__restrict__ cx, const DevModulus *__restrict__ mod,
-                                                            u32 N, u32 Ql, u32 QlP, u32 size_Q, u32 QP, u32 beta) {
+// With c2 given, digit j's own limbs (pos / alpha == j) are read from c2 (NTT
+// form) instead of from t, which then need not hold them.
+__global__ __launch_bounds__(kThreads) void k_inner_product(const u64 *__restrict__ t, const u64 *__restrict__ c2,
+                                                            const u64 *__restrict__ key, u64 *__restrict__ cx,
+                                                            const DevModulus *__restrict__ mod, u32 N, u32 Ql,
+                                                            u32 QlP, u32 size_Q, u32 QP, u32 beta, u32 alpha) {
     const u32 pos = blockIdx.y;
     const u32 k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= N) return;
     const u32 row = pos < Ql ? pos : size_Q + (pos - Ql);
+    const u32 own = c2 && pos < Ql ? pos / alpha : ~0u;
     const u64 QPN = (u64)QP * N;
     u128 a0 = 0, a1 = 0;
     for (u32 j = 0; j < beta; ++j) {
-        const u64 x = t[((u64)j * QlP + pos) * N + k];
+        const u64 x = j == own ? c2[(u64)pos * N + k] : t[((u64)j * QlP + pos) * N + k];
         const u64 *kb = key + (u64)j * 2 * QPN + (u64)row * N + k;
         a0 += (u128)x * kb[0];
         a1 += (u128)x * kb[QPN];
@@ -445,8 +480,9 @@ void tensor_2x2(const Context &ctx, u32 Ql, const u64 *a, const u64 *b, u64 *d0,
                        (u64)Ql * N);
 }
 
-// DRNSTool::modup for CKKS (src/rns_bconv.cu:530-628)
-void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStream_t s) {
+// DRNSTool::modup for CKKS (src/rns_bconv.cu:530-628); copy_own = false leaves
+// the digits' own limbs of t_mod_up unwritten (the inner product then reads c2)
+void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStream_t s, bool copy_own) {
     const LevelConsts &L = ctx.level(chain);
     const u64 N = ctx.N();
     const u32 Ql = L.size_Ql, QlP = L.size_QlP, beta = L.beta, alpha = ctx.size_P();
@@ -458,7 +494,8 @@ void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStrea
     mi.scale = L.modup_scale;
     ntt_inverse(ctx.tables(), mi, Ql, 1, s);
     // FastBConv of every digit to its complement, own limbs copied
-    bconv_launch(ctx, L.modup_bconv, L.h_modup_bconv, 1u, beta, t_cks, 0, t_mod_up, (u64)QlP * N, c2, 0, s);
+    bconv_launch(ctx, L.modup_bconv, L.h_modup_bconv, 1u, beta, t_cks, 0, t_mod_up, (u64)QlP * N,
+                 copy_own ? c2 : nullptr, 0, s);
     // forward NTT of the complement limbs of every digit
     LimbMap mf = limb_map(t_mod_up, 0, QlP, (u64)QlP * N);
     mf.split = Ql;
@@ -472,13 +509,13 @@ void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStrea
 }
 
 void inner_product(const Context &ctx, u32 chain, const u64 *t_mod_up, const KSwitchKey &key, u64 *cx,
-                   hipStream_t s) {
+                   hipStream_t s, const u64 *c2) {
     const LevelConsts &L = ctx.level(chain);
     require(key.dnum >= L.beta, "key-switching key has too few digits for this level");
     const u64 N = ctx.N();
-    hipLaunchKernelGGL(k_inner_product, grid_for(N, L.size_QlP), dim3(kThreads), 0, s, t_mod_up,
+    hipLaunchKernelGGL(k_inner_product, grid_for(N, L.size_QlP), dim3(kThreads), 0, s, t_mod_up, c2,
                        (const u64 *)key.data, cx, ctx.dev_mod(), (u32)N, L.size_Ql, L.size_QlP, ctx.size_Q(),
-                       ctx.size_QP(), L.beta);
+                       ctx.size_QP(), L.beta, ctx.size_P());
     PHX_HIP_CHECK(hipGetLastError());
 }
 
@@ -518,8 +555,8 @@ void keyswitch(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey &k
     const u64 N = ctx.N();
     u64 *t_mod_up = static_cast<u64 *>(dev_alloc((size_t)L.beta * L.size_QlP * N * 8, s));
     u64 *cx = static_cast<u64 *>(dev_alloc((size_t)2 * L.size_QlP * N * 8, s));
-    modup(ctx, chain, c2, t_mod_up, s);
-    inner_product(ctx, chain, t_mod_up, key, cx, s);
+    modup(ctx, chain, c2, t_mod_up, s, false);
+    inner_product(ctx, chain, t_mod_up, key, cx, s, c2);
     moddown(ctx, chain, cx, ct, base0, base1, 2, s);
     dev_free(cx, s);
     dev_free(t_mod_up, s);

@@ -104,6 +104,23 @@ void ntt_forward_store_batch(const NttTables &T, const LimbMap &M, u32 nlimbs, u
 void ntt_forward_rescale(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const LoadRescale &ld,
                          const StoreRescale &st, hipStream_t s);
 
+// Pass 2 of the modup NTT fused with the key inner product, over a batch:
+//   cx[i][h][pos] = sum_j NTT(t[i][j][pos]) * key_i[j][h][row(pos)]  (mod q_row)
+// where t[i][j][pos] holds pass-1 output, except for the digit's own limbs
+// (pos / alpha == j, pos < Ql), which are read from c2[i][pos] (NTT form).
+struct RowsIpArgs {
+    const u64 *t;            // [item][beta][QlP][N], pass-1 output
+    const u64 *c2;           // [item] at stride c2_stride: [Ql][N]
+    u64 c2_stride;
+    const u64 *const *keys;  // [item] -> key [dnum][2][QP][N]
+    u64 *cx;                 // [item][2][QlP][N]
+    u32 Ql, QlP, size_Q, QP, beta, alpha;
+};
+
+// pass 1 of the forward NTT only (in place)
+void ntt_forward_cols(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s);
+void ntt_forward_rows_ip(const NttTables &T, const RowsIpArgs &P, u32 nitems, hipStream_t s);
+
 // contiguous limbs [0, L) of one polynomial with rows first_row + i
 inline LimbMap limb_map(u64 *data, u32 first_row, u32 L, u64 poly_stride = 0) {
     LimbMap m{};

@@ -5,6 +5,7 @@
 // two passes the tile is stored in the arithmetic's own form (lazy u64 or
 // double bits); the first load and the last store are canonical u64.
 #include "ntt.h"
+#include "tune.h"
 
 namespace phx {
 
@@ -37,14 +38,13 @@ __device__ __forceinline__ FpArith fp_arith(const double *tw, const DevModulus &
 // ---------------------------------------------------------------------------
 template <int SR, class A, class Load>
 __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u32 logN, u32 c0, u32 limb, u32 z,
-                                              const DevModulus &md, const Load &ld) {
+                                              const DevModulus &md, const Load &ld, const u32 t) {
     using T = typename A::T;
     constexpr int R = 1 << SR;
     constexpr int TC = kNttTile / R;
     constexpr int NA = SR < 4 ? SR : 4;
     constexpr int NB = SR - NA;
     constexpr int BA = 16 >> NA;
-    const u32 t = threadIdx.x;
     T v[16];
     u32 baseA[BA], uA[BA], lowA[BA];
 #pragma unroll
@@ -97,12 +97,13 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
     }
 }
 
-// Forward pass 2 body: the last 8 stages (distances 128 .. 1) on 16 rows.
-template <class A, class Store>
-__device__ __forceinline__ void fwd_rows_body(const A &a, u64 *lds, u64 *data, u32 logN, u32 r0, u32 limb, u32 z,
-                                              const DevModulus &md, const Store &st) {
+// Forward pass 2 core: the last 8 stages (distances 128 .. 1) on 16 rows;
+// returns the canonical outputs at tile index t + 256 k in out[k]. The LDS
+// region is free again after the caller's next barrier.
+template <class A>
+__device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *data, u32 logN, u32 r0, const u32 t,
+                                              u64 (&out)[16]) {
     using T = typename A::T;
-    const u32 t = threadIdx.x;
     const u32 l = t & 15, u = t >> 4;
     T v[16];
     u32 baseA[1] = {(r0 + u) * 256 + l};
@@ -124,16 +125,25 @@ __device__ __forceinline__ void fwd_rows_body(const A &a, u64 *lds, u64 *data, u
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const u32 idx = t + kNttThreads * k;
-        st(data, r0 * 256 + idx, lds[row_addr(idx >> 8, idx & 255)], limb, z, md);
+        out[k] = lds[row_addr(idx >> 8, idx & 255)];
     }
+}
+
+// Forward pass 2 body: the core, then the store policy per output.
+template <class A, class Store>
+__device__ __forceinline__ void fwd_rows_body(const A &a, u64 *lds, u64 *data, u32 logN, u32 r0, u32 limb, u32 z,
+                                              const DevModulus &md, const Store &st, const u32 t) {
+    u64 out[16];
+    fwd_rows_core(a, lds, data, logN, r0, t, out);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) st(data, r0 * 256 + t + kNttThreads * k, out[k], limb, z, md);
 }
 
 // Inverse pass 1 body: the first 8 stages (distances 1 .. 128) on 16 rows.
 template <class A, class Load>
 __device__ __forceinline__ void inv_rows_body(const A &a, u64 *lds, u64 *data, const u64 *src, u32 logN, u32 r0,
-                                              u32 limb, u32 z, const DevModulus &md, const Load &ld) {
+                                              u32 limb, u32 z, const DevModulus &md, const Load &ld, const u32 t) {
     using T = typename A::T;
-    const u32 t = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const u32 idx = t + kNttThreads * k;
@@ -163,14 +173,13 @@ __device__ __forceinline__ void inv_rows_body(const A &a, u64 *lds, u64 *data, c
 template <int SR, class A, class Store>
 __device__ __forceinline__ void inv_cols_body(const A &a, u64 *lds, u64 *data, u32 logN, u32 c0, u32 limb, u32 z,
                                               const DevModulus &md, const Store &st, u64 s0, u64 s0s, u64 s1,
-                                              u64 s1s) {
+                                              u64 s1s, const u32 t) {
     using T = typename A::T;
     constexpr int R = 1 << SR;
     constexpr int TC = kNttTile / R;
     constexpr int NA = SR < 4 ? SR : 4;
     constexpr int NB = SR - NA;
     constexpr int BA = 16 >> NA;
-    const u32 t = threadIdx.x;
     T v[16];
     u32 baseA[BA], uA[BA], lowA[BA];
 #pragma unroll
@@ -232,49 +241,56 @@ __global__ __launch_bounds__(kNttThreads) void k_fwd_cols(NttTables T, LimbMap M
     constexpr int R = 1 << SR;
     constexpr int TC = kNttTile / R;
     __shared__ u64 lds[R * (TC + 1)];
-    const u32 limb = M.limb(blockIdx.z, blockIdx.y);
+    const u32 z = blockIdx.y + M.poly0;
+    const u32 limb = M.limb(blockIdx.z, z);
     if (limb >= M.nlimbs) return;
     const u32 row = M.row(limb);
-    u64 *data = M.data + blockIdx.y * M.poly_stride + (u64)limb * T.N;
+    u64 *data = M.data + z * M.poly_stride + (u64)limb * T.N;
     const DevModulus md = T.mod[row];
     const u32 c0 = blockIdx.x * TC;
     if (md.pad)
-        fwd_cols_body<SR>(fp_arith(T.twd + (u64)row * T.N, md), lds, data, T.logN, c0, limb, blockIdx.y, md, ld);
+        fwd_cols_body<SR>(fp_arith(T.twd + (u64)row * T.N, md), lds, data, T.logN, c0, limb, z, md, ld,
+                          threadIdx.x);
     else
-        fwd_cols_body<SR>(int_arith(T.tw + (u64)row * T.N, md.value), lds, data, T.logN, c0, limb, blockIdx.y, md,
-                          ld);
+        fwd_cols_body<SR>(int_arith(T.tw + (u64)row * T.N, md.value), lds, data, T.logN, c0, limb, z, md,
+                          ld, threadIdx.x);
 }
 
 template <class Store>
 __global__ __launch_bounds__(kNttThreads) void k_fwd_rows(NttTables T, LimbMap M, Store st) {
     __shared__ u64 lds[16 * 272];
-    const u32 limb = M.limb(blockIdx.z, blockIdx.y);
+    const u32 z = blockIdx.y + M.poly0;
+    const u32 limb = M.limb(blockIdx.z, z);
     if (limb >= M.nlimbs) return;
     const u32 row = M.row(limb);
-    u64 *data = M.data + blockIdx.y * M.poly_stride + (u64)limb * T.N;
+    u64 *data = M.data + z * M.poly_stride + (u64)limb * T.N;
     const DevModulus md = T.mod[row];
     const u32 r0 = blockIdx.x * 16;
     if (md.pad)
-        fwd_rows_body(fp_arith(T.twd + (u64)row * T.N, md), lds, data, T.logN, r0, limb, blockIdx.y, md, st);
+        fwd_rows_body(fp_arith(T.twd + (u64)row * T.N, md), lds, data, T.logN, r0, limb, z, md, st,
+                      threadIdx.x);
     else
-        fwd_rows_body(int_arith(T.tw + (u64)row * T.N, md.value), lds, data, T.logN, r0, limb, blockIdx.y, md, st);
+        fwd_rows_body(int_arith(T.tw + (u64)row * T.N, md.value), lds, data, T.logN, r0, limb, z, md, st,
+                      threadIdx.x);
 }
 
 template <class Load>
 __global__ __launch_bounds__(kNttThreads) void k_inv_rows(NttTables T, LimbMap M, Load ld) {
     __shared__ u64 lds[16 * 272];
-    const u32 limb = M.limb(blockIdx.z, blockIdx.y);
+    const u32 z = blockIdx.y + M.poly0;
+    const u32 limb = M.limb(blockIdx.z, z);
     if (limb >= M.nlimbs) return;
     const u32 row = M.row(limb);
-    u64 *data = M.data + blockIdx.y * M.poly_stride + (u64)limb * T.N;
-    const u64 *src = M.src ? M.src + blockIdx.y * M.src_poly_stride + (u64)limb * T.N : data;
+    u64 *data = M.data + z * M.poly_stride + (u64)limb * T.N;
+    const u64 *src = M.src ? M.src + z * M.src_poly_stride + (u64)limb * T.N : data;
     const DevModulus md = T.mod[row];
     const u32 r0 = blockIdx.x * 16;
     if (md.pad)
-        inv_rows_body(fp_arith(T.itwd + (u64)row * T.N, md), lds, data, src, T.logN, r0, limb, blockIdx.y, md, ld);
+        inv_rows_body(fp_arith(T.itwd + (u64)row * T.N, md), lds, data, src, T.logN, r0, limb, z, md, ld,
+                      threadIdx.x);
     else
-        inv_rows_body(int_arith(T.itw + (u64)row * T.N, md.value), lds, data, src, T.logN, r0, limb, blockIdx.y, md,
-                      ld);
+        inv_rows_body(int_arith(T.itw + (u64)row * T.N, md.value), lds, data, src, T.logN, r0, limb, z, md,
+                      ld, threadIdx.x);
 }
 
 template <int SR, class Store>
@@ -282,26 +298,185 @@ __global__ __launch_bounds__(kNttThreads) void k_inv_cols(NttTables T, LimbMap M
     constexpr int R = 1 << SR;
     constexpr int TC = kNttTile / R;
     __shared__ u64 lds[R * (TC + 1)];
-    const u32 limb = M.limb(blockIdx.z, blockIdx.y);
+    const u32 z = blockIdx.y + M.poly0;
+    const u32 limb = M.limb(blockIdx.z, z);
     if (limb >= M.nlimbs) return;
     const u32 row = M.row(limb);
-    u64 *data = M.data + blockIdx.y * M.poly_stride + (u64)limb * T.N;
+    u64 *data = M.data + z * M.poly_stride + (u64)limb * T.N;
     const DevModulus md = T.mod[row];
     const u32 c0 = blockIdx.x * TC;
     u64 s0, s0s, s1, s1s;
     limb_scale(T, M, limb, row, s0, s0s, s1, s1s);
     if (md.pad)
-        inv_cols_body<SR>(fp_arith(T.itwd + (u64)row * T.N, md), lds, data, T.logN, c0, limb, blockIdx.y, md, st,
-                          s0, s0s, s1, s1s);
+        inv_cols_body<SR>(fp_arith(T.itwd + (u64)row * T.N, md), lds, data, T.logN, c0, limb, z, md, st,
+                          s0, s0s, s1, s1s, threadIdx.x);
     else
-        inv_cols_body<SR>(int_arith(T.itw + (u64)row * T.N, md.value), lds, data, T.logN, c0, limb, blockIdx.y, md,
-                          st, s0, s0s, s1, s1s);
+        inv_cols_body<SR>(int_arith(T.itw + (u64)row * T.N, md.value), lds, data, T.logN, c0, limb, z, md,
+                          st, s0, s0s, s1, s1s, threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Pass 2 of the modup NTT fused with the key inner product
+// (key_switch_inner_prod_c2_and_evk, src/eval_key_switch.cu:14-69). Workgroup
+// (tile, item, limb pos) runs, for every digit j, the row pass on the pass-1
+// output of t[item][j][pos] -- or, for the digit's own limbs, takes c2 (already
+// in NTT form) -- multiplies by key_j at pos and accumulates, then writes the
+// two outputs cx[item][0 / 1][pos]. The transformed digits never reach HBM.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kNttThreads) void k_fwd_rows_ip(NttTables T, RowsIpArgs P) {
+    __shared__ u64 lds[16 * 272];
+    const u32 pos = blockIdx.z, item = blockIdx.y, r0 = blockIdx.x * 16, t = threadIdx.x;
+    const u32 row = pos < P.Ql ? pos : P.size_Q + (pos - P.Ql);
+    const DevModulus md = T.mod[row];
+    const u64 N = T.N, QPN = (u64)P.QP * N;
+    const u64 *key = P.keys[item] + (u64)row * N + r0 * 256 + t;
+    const u64 *t_item = P.t + (u64)item * P.beta * P.QlP * N + (u64)pos * N;
+    u64 acc0[16], acc1[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc0[k] = acc1[k] = 0;
+    for (u32 j = 0; j < P.beta; ++j) {
+        u64 x[16];
+        if (pos < P.Ql && pos / P.alpha == j) {
+            const u64 *c2 = P.c2 + (u64)item * P.c2_stride + (u64)pos * N + r0 * 256 + t;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) x[k] = c2[kNttThreads * k];
+        } else {
+            const u64 *src = t_item + (u64)j * P.QlP * N;
+            if (md.pad)
+                fwd_rows_core(fp_arith(T.twd + (u64)row * N, md), lds, src, T.logN, r0, t, x);
+            else
+                fwd_rows_core(int_arith(T.tw + (u64)row * N, md.value), lds, src, T.logN, r0, t, x);
+            __syncthreads();
+        }
+        const u64 *kj = key + (u64)j * 2 * QPN;
+        // each product is reduced, so 8 digits of partial sums stay below 2^63
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            acc0[k] += mul_mod(x[k], kj[kNttThreads * k], md);
+            acc1[k] += mul_mod(x[k], kj[QPN + kNttThreads * k], md);
+        }
+    }
+    u64 *out = P.cx + (u64)item * 2 * P.QlP * N + (u64)pos * N + r0 * 256 + t;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        out[kNttThreads * k] = barrett64(acc0[k], md.value, md.ratio_hi);
+        out[(u64)P.QlP * N + kNttThreads * k] = barrett64(acc1[k], md.value, md.ratio_hi);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused kernels: one workgroup of G groups x 256 threads runs both passes of
+// one limb. The pass-1 output is written in place and read back by the same
+// workgroup after a barrier, so while a few hundred limbs are in flight it is
+// served from L2 / the memory-attached last-level cache instead of HBM.
+// Every group runs the same number of tiles, so the bodies' barriers match.
+// ---------------------------------------------------------------------------
+constexpr u32 kNttSlot = 16 * 272;   // LDS words per group: >= R * (TC + 1) for R <= 256
+
+template <int SR, int G, class Load, class Store>
+__global__ __launch_bounds__(kNttThreads *G) void k_fwd_fused(NttTables T, LimbMap M, Load ld, Store st) {
+    constexpr int TC = kNttTile >> SR;
+    __shared__ u64 lds_all[G * kNttSlot];
+    const u32 z = blockIdx.y + M.poly0;
+    const u32 limb = M.limb(blockIdx.z, z);
+    if (limb >= M.nlimbs) return;
+    const u32 row = M.row(limb);
+    u64 *data = M.data + z * M.poly_stride + (u64)limb * T.N;
+    const DevModulus md = T.mod[row];
+    const u32 g = threadIdx.x / kNttThreads, t = threadIdx.x % kNttThreads;
+    u64 *lds = lds_all + g * kNttSlot;
+    const u32 tiles = T.N / kNttTile;
+    auto run = [&](const auto &a) {
+        for (u32 tile = g; tile < tiles; tile += G) {
+            fwd_cols_body<SR>(a, lds, data, T.logN, tile * TC, limb, z, md, ld, t);
+            __syncthreads();
+        }
+        for (u32 tile = g; tile < tiles; tile += G) {
+            fwd_rows_body(a, lds, data, T.logN, tile * 16, limb, z, md, st, t);
+            __syncthreads();
+        }
+    };
+    if (md.pad)
+        run(fp_arith(T.twd + (u64)row * T.N, md));
+    else
+        run(int_arith(T.tw + (u64)row * T.N, md.value));
+}
+
+template <int SR, int G, class Load, class Store>
+__global__ __launch_bounds__(kNttThreads *G) void k_inv_fused(NttTables T, LimbMap M, Load ld, Store st) {
+    constexpr int TC = kNttTile >> SR;
+    __shared__ u64 lds_all[G * kNttSlot];
+    const u32 z = blockIdx.y + M.poly0;
+    const u32 limb = M.limb(blockIdx.z, z);
+    if (limb >= M.nlimbs) return;
+    const u32 row = M.row(limb);
+    u64 *data = M.data + z * M.poly_stride + (u64)limb * T.N;
+    const u64 *src = M.src ? M.src + z * M.src_poly_stride + (u64)limb * T.N : data;
+    const DevModulus md = T.mod[row];
+    u64 s0, s0s, s1, s1s;
+    limb_scale(T, M, limb, row, s0, s0s, s1, s1s);
+    const u32 g = threadIdx.x / kNttThreads, t = threadIdx.x % kNttThreads;
+    u64 *lds = lds_all + g * kNttSlot;
+    const u32 tiles = T.N / kNttTile;
+    auto run = [&](const auto &a) {
+        for (u32 tile = g; tile < tiles; tile += G) {
+            inv_rows_body(a, lds, data, src, T.logN, tile * 16, limb, z, md, ld, t);
+            __syncthreads();
+        }
+        for (u32 tile = g; tile < tiles; tile += G) {
+            inv_cols_body<SR>(a, lds, data, T.logN, tile * TC, limb, z, md, st, s0, s0s, s1, s1s, t);
+            __syncthreads();
+        }
+    };
+    if (md.pad)
+        run(fp_arith(T.itwd + (u64)row * T.N, md));
+    else
+        run(int_arith(T.itw + (u64)row * T.N, md.value));
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 namespace {
+// polynomials per launch pair: slices of about ntt_chunk_limbs limbs (0 = one launch)
+u32 chunk_polys(u32 nlimbs, u32 npolys) {
+    const int c = tune_value("ntt_chunk_limbs");
+    if (c <= 0) return npolys;
+    return std::max<u32>(1, (u32)c / std::max<u32>(1, nlimbs));
+}
+
+// workgroup groups of the fused form for this launch, 0 = two passes
+u32 fused_groups(const NttTables &T, u32 nlimbs, u32 npolys) {
+    const int g = tune_value("ntt_fused");
+    if (g != 1 && g != 2 && g != 4) return 0;
+    const u32 tiles = T.N / kNttTile;
+    if (tiles % (u32)g) return 0;
+    if ((u64)nlimbs * npolys < (u64)std::max(0, tune_value("ntt_fused_min_limbs"))) return 0;
+    return (u32)g;
+}
+
+template <int SR, class Load, class Store>
+void launch_fused_fwd(u32 G, dim3 grid, hipStream_t s, const NttTables &T, const LimbMap &M, const Load &ld,
+                      const Store &st) {
+    if (G == 4)
+        hipLaunchKernelGGL((k_fwd_fused<SR, 4, Load, Store>), grid, dim3(kNttThreads * 4), 0, s, T, M, ld, st);
+    else if (G == 2)
+        hipLaunchKernelGGL((k_fwd_fused<SR, 2, Load, Store>), grid, dim3(kNttThreads * 2), 0, s, T, M, ld, st);
+    else
+        hipLaunchKernelGGL((k_fwd_fused<SR, 1, Load, Store>), grid, dim3(kNttThreads), 0, s, T, M, ld, st);
+}
+
+template <int SR, class Load, class Store>
+void launch_fused_inv(u32 G, dim3 grid, hipStream_t s, const NttTables &T, const LimbMap &M, const Load &ld,
+                      const Store &st) {
+    if (G == 4)
+        hipLaunchKernelGGL((k_inv_fused<SR, 4, Load, Store>), grid, dim3(kNttThreads * 4), 0, s, T, M, ld, st);
+    else if (G == 2)
+        hipLaunchKernelGGL((k_inv_fused<SR, 2, Load, Store>), grid, dim3(kNttThreads * 2), 0, s, T, M, ld, st);
+    else
+        hipLaunchKernelGGL((k_inv_fused<SR, 1, Load, Store>), grid, dim3(kNttThreads), 0, s, T, M, ld, st);
+}
+
 template <class F>
 void dispatch_sr(u32 logN, F &&f) {
     switch (logN) {
@@ -319,14 +494,28 @@ void launch_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys
                     hipStream_t s) {
     if (nlimbs == 0 || npolys == 0) return;
     const u32 tiles = T.N / kNttTile;
+    if (const u32 G = fused_groups(T, nlimbs, npolys)) {
+        dispatch_sr(T.logN, [&](auto sr) {
+            launch_fused_fwd<decltype(sr)::value>(G, dim3(1, npolys, nlimbs), s, T, M, ld, st);
+        });
+        PHX_HIP_CHECK(hipGetLastError());
+        return;
+    }
     // polynomials sharing a prime run next to each other (and, with 8 or 16
-    // tiles per limb, on the same XCD), so the prime's twiddles stay in L2
-    dim3 grid(tiles, npolys, nlimbs);
-    dispatch_sr(T.logN, [&](auto sr) {
-        constexpr int SR = decltype(sr)::value;
-        hipLaunchKernelGGL((k_fwd_cols<SR, Load>), grid, dim3(kNttThreads), 0, s, T, M, ld);
-    });
-    hipLaunchKernelGGL((k_fwd_rows<Store>), grid, dim3(kNttThreads), 0, s, T, M, st);
+    // tiles per limb, on the same XCD), so the prime's twiddles stay in L2;
+    // large launches go in slices of whole polynomials so that the pass-1
+    // output of a slice is still in the last-level cache when pass 2 reads it
+    const u32 step = chunk_polys(nlimbs, npolys);
+    for (u32 p0 = 0; p0 < npolys; p0 += step) {
+        LimbMap Mc = M;
+        Mc.poly0 = M.poly0 + p0;
+        dim3 grid(tiles, std::min(step, npolys - p0), nlimbs);
+        dispatch_sr(T.logN, [&](auto sr) {
+            constexpr int SR = decltype(sr)::value;
+            hipLaunchKernelGGL((k_fwd_cols<SR, Load>), grid, dim3(kNttThreads), 0, s, T, Mc, ld);
+        });
+        hipLaunchKernelGGL((k_fwd_rows<Store>), grid, dim3(kNttThreads), 0, s, T, Mc, st);
+    }
     PHX_HIP_CHECK(hipGetLastError());
 }
 }  // namespace
@@ -338,14 +527,42 @@ void ntt_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, h
 void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s) {
     if (nlimbs == 0 || npolys == 0) return;
     const u32 tiles = T.N / kNttTile;
-    // polynomials sharing a prime run next to each other (and, with 8 or 16
-    // tiles per limb, on the same XCD), so the prime's twiddles stay in L2
-    dim3 grid(tiles, npolys, nlimbs);
-    hipLaunchKernelGGL((k_inv_rows<LoadPlain>), grid, dim3(kNttThreads), 0, s, T, M, LoadPlain{});
+    if (const u32 G = fused_groups(T, nlimbs, npolys)) {
+        dispatch_sr(T.logN, [&](auto sr) {
+            launch_fused_inv<decltype(sr)::value>(G, dim3(1, npolys, nlimbs), s, T, M, LoadPlain{}, StorePlain{});
+        });
+        PHX_HIP_CHECK(hipGetLastError());
+        return;
+    }
+    // launch order and slicing as in launch_forward
+    const u32 step = chunk_polys(nlimbs, npolys);
+    for (u32 p0 = 0; p0 < npolys; p0 += step) {
+        LimbMap Mc = M;
+        Mc.poly0 = M.poly0 + p0;
+        dim3 grid(tiles, std::min(step, npolys - p0), nlimbs);
+        hipLaunchKernelGGL((k_inv_rows<LoadPlain>), grid, dim3(kNttThreads), 0, s, T, Mc, LoadPlain{});
+        dispatch_sr(T.logN, [&](auto sr) {
+            constexpr int SR = decltype(sr)::value;
+            hipLaunchKernelGGL((k_inv_cols<SR, StorePlain>), grid, dim3(kNttThreads), 0, s, T, Mc, StorePlain{});
+        });
+    }
+    PHX_HIP_CHECK(hipGetLastError());
+}
+
+void ntt_forward_cols(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s) {
+    if (nlimbs == 0 || npolys == 0) return;
+    dim3 grid(T.N / kNttTile, npolys, nlimbs);
     dispatch_sr(T.logN, [&](auto sr) {
         constexpr int SR = decltype(sr)::value;
-        hipLaunchKernelGGL((k_inv_cols<SR, StorePlain>), grid, dim3(kNttThreads), 0, s, T, M, StorePlain{});
+        hipLaunchKernelGGL((k_fwd_cols<SR, LoadPlain>), grid, dim3(kNttThreads), 0, s, T, M, LoadPlain{});
     });
+    PHX_HIP_CHECK(hipGetLastError());
+}
+
+void ntt_forward_rows_ip(const NttTables &T, const RowsIpArgs &P, u32 nitems, hipStream_t s) {
+    if (nitems == 0) return;
+    require(P.beta >= 1 && P.beta <= 8, "fused key inner product supports 1 .. 8 digits");
+    hipLaunchKernelGGL(k_fwd_rows_ip, dim3(T.N / kNttTile, nitems, P.QlP), dim3(kNttThreads), 0, s, T, P);
     PHX_HIP_CHECK(hipGetLastError());
 }
 

@@ -27,8 +27,9 @@ constexpr int kNttRegs = 16;
 constexpr int kNttTile = kNttThreads * kNttRegs;  // 4096
 
 // Per-launch limb mapping: blockIdx.z selects a limb of the call, blockIdx.y
-// a polynomial (stride poly_stride elements). Limbs [skip_lo, skip_hi) are
-// not launched (the digit's own limbs in modup, already in NTT form).
+// + poly0 a polynomial (stride poly_stride elements). Limbs [skip_lo,
+// skip_hi) are not launched (the digit's own limbs in modup, already in NTT
+// form).
 struct LimbMap {
     u64 *data;          // limb 0 of poly 0
     const u64 *src;     // out-of-place source for the first inverse pass (or null)
@@ -43,6 +44,8 @@ struct LimbMap {
     u32 skip_cap;       //   digit d = z (skip_mod == 0) or z % skip_mod
     u32 skip_mod;
     u32 nlimbs;         // limbs >= nlimbs are not processed (ragged launches)
+    u32 poly0;          // polynomial of blockIdx.y == 0 (launches split over polynomials)
+    u32 pad_;
 
     __device__ __forceinline__ u32 limb(u32 y, u32 z) const {
         u32 lo = skip_lo, hi = skip_hi;

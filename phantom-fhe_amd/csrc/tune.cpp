@@ -3,6 +3,15 @@
 //   bconv_mfma   1: FastBConv on the matrix cores (default), 0: VALU kernel
 //   ntt_fp64     1: FP64 butterflies for primes < 2^50 (default), 0: integer only;
 //                read when a context is created
+//   ntt_fused    G in {1, 2, 4}: one workgroup of G x 256 threads runs both passes
+//                of a limb (intermediate kept on chip / in the last-level cache);
+//                0: two launches per transform
+//   ntt_fused_min_limbs  launches over fewer limbs than this use two passes
+//   ntt_chunk_limbs  two-pass launches over more limbs than this are split into
+//                slices of whole polynomials of about this many limbs (0 = never)
+//   ks_fused_ip  1: batched key switching runs the modup NTT's second pass fused
+//                with the key inner product; 0: separate kernels (default, faster
+//                on MI355X: the fused kernel needs 242 VGPRs)
 // Environment defaults: PHX_BCONV_VALU=1 -> bconv_mfma=0, PHX_NTT_INT_ONLY=1 -> ntt_fp64=0.
 #include <atomic>
 #include <cstdlib>
@@ -19,11 +28,19 @@ bool env_is_one(const char *name) {
 }
 std::atomic<int> g_bconv_mfma{env_is_one("PHX_BCONV_VALU") ? 0 : 1};
 std::atomic<int> g_ntt_fp64{env_is_one("PHX_NTT_INT_ONLY") ? 0 : 1};
+std::atomic<int> g_ntt_fused{0};
+std::atomic<int> g_ntt_fused_min{512};
+std::atomic<int> g_ntt_chunk{0};
+std::atomic<int> g_ks_fused_ip{0};
 
 std::atomic<int> *slot(const char *key) {
     if (!key) return nullptr;
     if (!std::strcmp(key, "bconv_mfma")) return &g_bconv_mfma;
     if (!std::strcmp(key, "ntt_fp64")) return &g_ntt_fp64;
+    if (!std::strcmp(key, "ntt_fused")) return &g_ntt_fused;
+    if (!std::strcmp(key, "ntt_fused_min_limbs")) return &g_ntt_fused_min;
+    if (!std::strcmp(key, "ntt_chunk_limbs")) return &g_ntt_chunk;
+    if (!std::strcmp(key, "ks_fused_ip")) return &g_ks_fused_ip;
     return nullptr;
 }
 }  // namespace

@@ -125,3 +125,38 @@ def test_batch_config2_bit_exact():
     ph.synchronize(stream)
     for i in (0, 3):
         assert np.array_equal(c1[i].download(N), octx.multiply_and_relin(35, a[i], b[i], key)), i
+
+
+def test_batch_fused_inner_product_variant(small):
+    """ks_fused_ip=1: the modup NTT's second pass fused with the key inner
+    product gives the same outputs (mixed levels, squaring, rotation)."""
+    ctx, octx, mods, key = small
+    N = ctx.N
+    say("case: batch fused inner product")
+    old = ph.tune_get("ks_fused_ip")
+    try:
+        ph.tune_set("ks_fused_ip", 1)
+        rng = np.random.default_rng(21)
+        chains = [1, 3, 1, 7]
+        data = [(c, ctx.limbs_at(c), rand_ct(rng, mods[:ctx.limbs_at(c)], N),
+                 rand_ct(rng, mods[:ctx.limbs_at(c)], N)) for c in chains]
+        rk = ph.relin_key(ctx, host=key)
+        c1 = [upload_ct(ctx, a, c) for c, _, a, _ in data]
+        c2 = [upload_ct(ctx, b, c) for c, _, _, b in data]
+        c2[2] = c1[2]
+        bt = ph.batch.multiply_and_relin(ctx, c1, c2, rk)
+        bt.run()
+        ph.device_synchronize()
+        for i, (c, Ql, a, b) in enumerate(data):
+            other = a if i == 2 else b
+            assert np.array_equal(c1[i].download(N), octx.multiply_and_relin(Ql, a, other, key)), i
+        elt = ph.galois_elt_from_step(2, N)
+        gk = ph.galois_key(ctx, [elt], host=key[None])
+        x = rand_ct(rng, mods[:5], N)
+        cx = [upload_ct(ctx, x, 3)]
+        bt = ph.batch.rotate(ctx, cx, [2], gk)
+        bt.run()
+        ph.device_synchronize()
+        assert np.array_equal(cx[0].download(N), octx.apply_galois(5, x, elt, key))
+    finally:
+        ph.tune_set("ks_fused_ip", old)

@@ -247,3 +247,48 @@ def test_bconv_extreme_inputs_mfma():
     ph.moddown_from_ntt(ctx, 1, cxb, ob, 1)
     ph.device_synchronize()
     assert np.array_equal(ob.to_array((Ql, N)), octx.moddown(Ql, cx))
+
+
+@pytest.mark.parametrize("G", [2, 4])
+def test_fused_ntt_hot_path(G):
+    """The one-launch NTT form (one workgroup per limb, both passes) under
+    every fused load/store policy: modup, moddown's subtract-and-scale store,
+    rescale's load/store pair, rotation, and the batch plan."""
+    say(f"case: fused ntt hot path G={G}")
+    old = (ph.tune_get("ntt_fused"), ph.tune_get("ntt_fused_min_limbs"))
+    try:
+        ph.tune_set("ntt_fused", G)
+        ph.tune_set("ntt_fused_min_limbs", 0)
+        N = 1 << 14
+        ctx, octx, mods = make_ctx(N, [60] + [50] * 9 + [60] * 4, 4)
+        rng = np.random.default_rng(60 + G)
+        key = rand_key(rng, mods, N, ctx.dnum)
+        rk = ph.relin_key(ctx, host=key)
+        for chain in (1, 4):
+            Ql = ctx.limbs_at(chain)
+            a, b = rand_ct(rng, mods[:Ql], N), rand_ct(rng, mods[:Ql], N)
+            c1, c2 = upload_ct(ctx, a, chain), upload_ct(ctx, b, chain)
+            ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
+            want = octx.multiply_and_relin(Ql, a, b, key)
+            assert np.array_equal(c1.download(N), want)
+            c1.set_scale(2.0 ** 50)
+            r = ph.rescale_to_next(ctx, c1)
+            assert np.array_equal(r.download(N), octx.rescale(Ql, want))
+        elt = ph.galois_elt_from_step(3, N)
+        gk = ph.galois_key(ctx, [elt], host=key[None])
+        a = rand_ct(rng, mods[:10], N)
+        c = upload_ct(ctx, a, 1)
+        ph.rotate_inplace(ctx, c, 3, gk)
+        assert np.array_equal(c.download(N), octx.apply_galois(10, a, elt, key))
+        xs = [rand_ct(rng, mods[:10], N) for _ in range(3)]
+        ys = [rand_ct(rng, mods[:10], N) for _ in range(3)]
+        cx = [upload_ct(ctx, x, 1) for x in xs]
+        cy = [upload_ct(ctx, y, 1) for y in ys]
+        bt = ph.batch.multiply_and_relin(ctx, cx, cy, rk)
+        bt.run()
+        ph.device_synchronize()
+        for i in range(3):
+            assert np.array_equal(cx[i].download(N), octx.multiply_and_relin(10, xs[i], ys[i], key)), i
+    finally:
+        ph.tune_set("ntt_fused", old[0])
+        ph.tune_set("ntt_fused_min_limbs", old[1])

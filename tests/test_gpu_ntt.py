@@ -63,3 +63,30 @@ def test_ntt_edge_values():
         assert np.array_equal(f, octx.ntt_forward(a, 0))
         assert np.array_equal(run_ntt(ctx, f, 0, inverse=True), a)
         assert np.array_equal(run_ntt(ctx, a, 0, inverse=True), octx.ntt_inverse(a, 0))
+
+
+@pytest.fixture
+def fused():
+    """Switch the launchers to the one-launch form for every limb count."""
+    old = ph.tune_get("ntt_fused"), ph.tune_get("ntt_fused_min_limbs")
+    yield lambda g: (ph.tune_set("ntt_fused", g), ph.tune_set("ntt_fused_min_limbs", 0))
+    ph.tune_set("ntt_fused", old[0])
+    ph.tune_set("ntt_fused_min_limbs", old[1])
+
+
+@pytest.mark.parametrize("G", [1, 2, 4])
+@pytest.mark.parametrize("logN", [13, 15, 16])
+def test_ntt_fused_parity(fused, G, logN):
+    """One workgroup per limb runs both passes: same outputs as the oracle."""
+    N = 1 << logN
+    say(f"case: fused ntt G={G} N=2^{logN}")
+    fused(G)
+    ctx, octx, mods = make_ctx(N, [60, 50, 50, 49, 40, 60, 60], 2)
+    rng = np.random.default_rng(100 + logN)
+    a = rand_limbs(rng, mods, N)
+    got = run_ntt(ctx, a, 0)
+    assert np.array_equal(got, octx.ntt_forward(a, 0))
+    b = rand_limbs(rng, mods[2:6], N)
+    got_i = run_ntt(ctx, b, 2, inverse=True)
+    assert np.array_equal(got_i, octx.ntt_inverse(b, 2))
+    assert np.array_equal(run_ntt(ctx, got_i, 2), b)
