@@ -104,6 +104,36 @@ def bench_hmult_batched(ph, ctx, mods, N, batch, steps, warmup, stream):
     return ev0.elapsed_ms(ev1), bt.scratch_bytes()
 
 
+def ckks_error_report(ph, ctx, scale=2.0 ** 50, step=5, seed=20):
+    """Real CKKS encryptions of seeded messages (|x|, |y| <= sqrt(2)): max slot
+    error against FP64 after each stage of x*y -> relin -> rescale -> rotate."""
+    N = ctx.N
+    enc = ph.ckks_encoder(ctx)
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+    y = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+    sk = ph.secret_key(ctx, seed=seed)
+    rk = sk.gen_relinkey(ctx, seed=seed + 1)
+    gk = sk.create_galois_keys(ctx, [ph.galois_elt_from_step(step, N)], seed=seed + 2)
+    cx = sk.encrypt_symmetric(ctx, enc.encode(x, scale, 1), 1, seed=seed + 3)
+    cy = sk.encrypt_symmetric(ctx, enc.encode(y, scale, 1), 1, seed=seed + 4)
+    cx.set_scale(scale)
+    cy.set_scale(scale)
+
+    def err(ct, want):
+        got = enc.decode(sk.decrypt(ctx, ct), ct.chain_index(), ct.scale())
+        return float(np.max(np.abs(got - want)))
+
+    out = {"scale_log2": 50, "fresh": err(cx, x)}
+    ph.multiply_and_relin_inplace(ctx, cx, cy, rk)
+    out["mul_relin"] = err(cx, x * y)
+    r = ph.rescale_to_next(ctx, cx)
+    out["rescale"] = err(r, x * y)
+    ph.rotate_inplace(ctx, r, step, gk)
+    out["rotate"] = err(r, np.roll(x * y, -step))
+    return {k: (float(f"{v:.3e}") if isinstance(v, float) else v) for k, v in out.items()}
+
+
 def bench_ntt(ph, ctx, N, L, npolys, iters, stream):
     """Forward NTT over npolys x L limbs in one launch pair (pass 1 + pass 2)."""
     rng = np.random.default_rng(7)
@@ -217,6 +247,9 @@ def main():
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
 
+    ckks_err = ckks_error_report(ph, ctx)
+    log(f"bench: CKKS max slot error vs FP64 (scale 2^50): {ckks_err}")
+
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (oracle) ...")
@@ -253,6 +286,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "forward NTT (k_fwd_cols + k_fwd_rows), algorithmic bytes 2*N*8 per limb"},
         "cpu_baseline": cpu,
+        "ckks_error_vs_fp64": ckks_err,
     }
     print(json.dumps(out), flush=True)
 

@@ -121,6 +121,17 @@ int phx_encrypt_symmetric(const phx_context *ctx, const phx_secret_key *sk, cons
 int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext *ct, uint64_t *out_host,
                 phx_stream stream);
 
+/* ---- CKKS encoder (include/ckks.h), host side ------------------------------ */
+/* PhantomCKKSEncoder::encode (src/ckks.cu:65-135): count <= N/2 complex values
+ * (re[i] + i*im[i], im may be NULL) -> round(scale * m) as coefficient residues
+ * [Ql][N] at chain_index, ready for phx_encrypt_symmetric */
+int phx_ckks_encode(const phx_context *ctx, const double *re, const double *im, size_t count, double scale,
+                    size_t chain_index, uint64_t *out_host);
+/* PhantomCKKSEncoder::decode (src/ckks.cu:347-400): coefficient residues [Ql][N]
+ * (as phx_decrypt returns them) -> N/2 slot values re[i] + i*im[i] */
+int phx_ckks_decode(const phx_context *ctx, const uint64_t *in_host, size_t chain_index, double scale, double *re,
+                    double *im);
+
 /* ---- evaluator: the hot path (include/evaluate.cuh) ------------------------ */
 /* multiply_and_relin_inplace (include/evaluate.cuh:115-116, src/evaluate.cu:1061-1104) */
 int phx_multiply_and_relin_inplace(const phx_context *ctx, phx_ciphertext *ct1, const phx_ciphertext *ct2,

@@ -7,6 +7,7 @@
 #include "batch.h"
 #include "evaluator.h"
 #include "tune.h"
+#include "ckks.h"
 
 using namespace phx;
 
@@ -385,6 +386,34 @@ int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext
         PHX_HIP_CHECK(hipMemcpyAsync(out, d, n * 8, hipMemcpyDeviceToHost, S(s)));
         dev_free(d, S(s));
         PHX_HIP_CHECK(hipStreamSynchronize(S(s)));
+    });
+}
+
+int phx_ckks_encode(const phx_context *ctx, const double *re, const double *im, size_t count, double scale,
+                    size_t chain_index, uint64_t *out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(re, "values");
+        nn(out, "out");
+        std::vector<std::complex<double>> v(count);
+        for (size_t i = 0; i < count; ++i) v[i] = {re[i], im ? im[i] : 0.0};
+        ckks_encode(*ctx->ctx, v.data(), count, scale, (u32)chain_index, out);
+    });
+}
+
+int phx_ckks_decode(const phx_context *ctx, const uint64_t *in, size_t chain_index, double scale, double *re,
+                    double *im) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(in, "input");
+        nn(re, "re");
+        const Context &c = *ctx->ctx;
+        std::vector<std::complex<double>> v(c.N() / 2);
+        ckks_decode(c, in, (u32)chain_index, scale, v.data());
+        for (size_t i = 0; i < v.size(); ++i) {
+            re[i] = v[i].real();
+            if (im) im[i] = v[i].imag();
+        }
     });
 }
 

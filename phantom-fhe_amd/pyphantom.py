@@ -23,6 +23,7 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
+_dp = ctypes.POINTER(ctypes.c_double)
 _lib = None
 
 
@@ -75,6 +76,8 @@ _SIGS = {
     "phx_galois_key_destroy": (ctypes.c_int, [_vp]),
     "phx_encrypt_symmetric": (ctypes.c_int, [_vp, _vp, _u64p, _sz, ctypes.c_uint64, _vp, _vp]),
     "phx_decrypt": (ctypes.c_int, [_vp, _vp, _vp, _u64p, _vp]),
+    "phx_ckks_encode": (ctypes.c_int, [_vp, _dp, _dp, _sz, ctypes.c_double, _sz, _u64p]),
+    "phx_ckks_decode": (ctypes.c_int, [_vp, _u64p, _sz, ctypes.c_double, _dp, _dp]),
     "phx_multiply_and_relin_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "phx_multiply_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "phx_relinearize_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
@@ -286,6 +289,36 @@ class ciphertext:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.phx_ciphertext_destroy(self._h)
             self._h = None
+
+
+class ckks_encoder:
+    """PhantomCKKSEncoder (include/ckks.h): slot j <-> m(zeta^(5^j)), host side."""
+
+    def __init__(self, ctx):
+        self._ctx = ctx
+
+    def slot_count(self):
+        return self._ctx.N // 2
+
+    def encode(self, values, scale, chain_index=1):
+        """values (complex or real, at most N/2) -> coefficient residues [Ql][N]."""
+        v = np.asarray(values)
+        re = np.ascontiguousarray(v.real, np.float64)
+        im = np.ascontiguousarray(v.imag if np.iscomplexobj(v) else np.zeros(len(v)), np.float64)
+        L = self._ctx.limbs_at(chain_index)
+        out = np.zeros((L, self._ctx.N), np.uint64)
+        _check(lib().phx_ckks_encode(self._ctx._h, re.ctypes.data_as(_dp), im.ctypes.data_as(_dp), len(v),
+                                     float(scale), chain_index, _p(out)))
+        return out
+
+    def decode(self, coeffs, chain_index, scale):
+        """coefficient residues [Ql][N] -> N/2 complex slot values."""
+        a = np.ascontiguousarray(coeffs, np.uint64)
+        n = self._ctx.N // 2
+        re, im = np.zeros(n), np.zeros(n)
+        _check(lib().phx_ckks_decode(self._ctx._h, _p(a), chain_index, float(scale), re.ctypes.data_as(_dp),
+                                     im.ctypes.data_as(_dp)))
+        return re + 1j * im
 
 
 class secret_key:
