@@ -104,6 +104,97 @@ def bench_hmult_batched(ph, ctx, mods, N, batch, steps, warmup, stream):
     return ev0.elapsed_ms(ev1), bt.scratch_bytes()
 
 
+ROT_STEPS = [1, 2, 3, 4, 5, 7, 8, 16, 32, 64, 128, 256, 512, 1024, -1, -5]
+
+
+def bench_rotations(ph, ctx, mods, N, stream, reps=3, n_ct=128):
+    """Batched CKKS rotations: n_ct ciphertexts with 5..35 limbs, each rotated
+    by a step from 16 resident rotation keys; one captured batch plan replayed,
+    against one rotate_inplace call per ciphertext."""
+    rng = np.random.default_rng(77)
+    sk = ph.secret_key(ctx, seed=77)
+    gk = sk.create_galois_keys(ctx, [ph.galois_elt_from_step(st, N) for st in ROT_STEPS], seed=78)
+    limbs = rng.integers(5, 36, n_ct)
+    steps = [ROT_STEPS[k] for k in rng.integers(0, len(ROT_STEPS), n_ct)]
+    cts = []
+    for l in limbs:
+        c = ph.ciphertext(ctx, 36 - int(l), 2)
+        c.upload(rand_ct(rng, mods[:int(l)], N))
+        cts.append(c)
+    ev0, ev1 = ph.Event(), ph.Event()
+    for c, st in zip(cts, steps):     # warm-up
+        ph.rotate_inplace(ctx, c, st, gk, stream)
+    ph.synchronize(stream)
+    ev0.record(stream)
+    for _ in range(reps):
+        for c, st in zip(cts, steps):
+            ph.rotate_inplace(ctx, c, st, gk, stream)
+    ev1.record(stream)
+    ph.synchronize(stream)
+    plain = reps * n_ct / (ev0.elapsed_ms(ev1) / 1e3)
+    bt = ph.batch.rotate(ctx, cts, steps, gk)
+    bt.capture(stream)
+    bt.replay(stream)
+    ph.synchronize(stream)
+    ev0.record(stream)
+    for _ in range(reps):
+        bt.replay(stream)
+    ev1.record(stream)
+    ph.synchronize(stream)
+    batched = reps * n_ct / (ev0.elapsed_ms(ev1) / 1e3)
+    return {"workload": f"{n_ct} ciphertexts, N=2^16, 5..35 limbs, 16 resident rotation keys",
+            "plain_rot_per_s": round(plain, 1), "batched_rot_per_s": round(batched, 1),
+            "batched_over_plain": round(batched / plain, 3), "mean_limbs": float(np.mean(limbs)),
+            "scratch_GiB": round(bt.scratch_bytes() / 2**30, 2)}
+
+
+def bench_serving(ph, lanes, qps, n_req, seed=55):
+    """Serving loop: Poisson arrivals at `qps` of chains of k in 1..8 links
+    (HMult+relin, rescale, rotate by a random step), N in {2^15, 2^16}, start
+    level 10..35 limbs; keys resident; p50/p99 latency beside throughput."""
+    rng = np.random.default_rng(seed)
+    pool = []        # per lane: ciphertexts at every start level, uploaded once
+    for ctx, mods, _, _ in lanes:
+        full = rand_ct(rng, mods[:35], ctx.N)
+        per = {}
+        for limbs in range(10, 36):
+            c = ph.ciphertext(ctx, 36 - limbs, 2)
+            c.upload(np.ascontiguousarray(full[:, :limbs]))
+            c.set_scale(2.0 ** 50)
+            per[limbs] = c
+        pool.append(per)
+    srv = ph.server([(ctx, rk, gk) for ctx, _, rk, gk in lanes], mem_budget=64 << 30, max_active=512, depth=2)
+    reqs = []
+    for _ in range(n_req):
+        k = int(rng.integers(1, 9))
+        limbs = int(rng.integers(max(10, k + 1), 36))
+        reqs.append((int(rng.integers(0, len(lanes))), limbs, [ROT_STEPS[j] for j in rng.integers(0, 16, k)]))
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, n_req))
+    t0 = srv.now()
+    ids, i, done = [], 0, 0
+    while done < n_req:
+        now = srv.now() - t0
+        while i < n_req and arrivals[i] <= now:
+            lane, limbs, steps = reqs[i]
+            ids.append(srv.submit(lane, pool[lane][limbs], steps, t0 + arrivals[i]))
+            i += 1
+        issued = srv.step()
+        for rid in ids[done:]:
+            if srv.status(rid) != 2:
+                break
+            srv.take(rid)
+            done += 1
+        if issued == 0 and i < n_req:
+            time.sleep(max(0.0, min(0.001, arrivals[i] - (srv.now() - t0))))
+    st = srv.stats()
+    return {"workload": f"{n_req} requests, Poisson {qps}/s, k in 1..8 links of (HMult+relin, rescale, rotate), "
+                        f"N in {{2^15, 2^16}}, start 10..35 limbs, 16 resident rotation keys per N",
+            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "mean_ms": round(st["mean_ms"], 3),
+            "req_per_s": round(st["req_per_s"], 1), "ops_per_s": round(st["ops_per_s"], 1),
+            "peak_active": st["peak_active"], "ticks": st["ticks"],
+            "peak_reserved_GiB": round(st["peak_reserved_bytes"] / 2**30, 2)}
+
+
 def ckks_error_report(ph, ctx, scale=2.0 ** 50, step=5, seed=20):
     """Real CKKS encryptions of seeded messages (|x|, |y| <= sqrt(2)): max slot
     error against FP64 after each stage of x*y -> relin -> rescale -> rotate."""
@@ -207,6 +298,8 @@ def main():
     ap.add_argument("--ntt-polys", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-qps", type=float, default=400.0)
+    ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="kernel-variant switch (bconv_mfma, ntt_fp64, ntt_fused, ntt_chunk_limbs, ...)")
     args = ap.parse_args()
@@ -247,6 +340,18 @@ def main():
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
 
+    rot = bench_rotations(ph, ctx, mods, N, stream)
+    log(f"bench: rotations {rot}")
+    serving = None
+    if args.serve_requests > 0:
+        lanes = []
+        for n_ in (1 << 15, 1 << 16):
+            c_, m_ = (ctx, mods) if n_ == N else make_context(ph, n_, CONFIG2_BITS, 12)
+            sk_ = ph.secret_key(c_, seed=90)
+            lanes.append((c_, m_, sk_.gen_relinkey(c_, seed=91),
+                          sk_.create_galois_keys(c_, [ph.galois_elt_from_step(st, n_) for st in ROT_STEPS], seed=92)))
+        serving = bench_serving(ph, lanes, args.serve_qps, args.serve_requests)
+        log(f"bench: serving {serving}")
     ckks_err = ckks_error_report(ph, ctx)
     log(f"bench: CKKS max slot error vs FP64 (scale 2^50): {ckks_err}")
 
@@ -287,6 +392,8 @@ def main():
                      "kernel": "forward NTT (k_fwd_cols + k_fwd_rows), algorithmic bytes 2*N*8 per limb"},
         "cpu_baseline": cpu,
         "ckks_error_vs_fp64": ckks_err,
+        "rotations": rot,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -175,6 +175,42 @@ int phx_batch_replay(phx_batch *batch, phx_stream stream);
 int phx_batch_scratch_bytes(const phx_batch *batch, size_t *bytes);
 int phx_batch_destroy(phx_batch *batch);
 
+/* ---- serving loop (no single reference counterpart: a request loop around the
+ * reference's evaluator calls, multiply_and_relin_inplace / rescale_to_next /
+ * rotate_inplace of include/evaluate.cuh:115-116, 137, 225-226).
+ * A request is a CKKS ciphertext and a chain of k links, link i being
+ *   ct <- rotate(rescale(ct * ct relinearised), steps[i]).
+ * Evaluation keys stay resident. Each phx_server_step is one scheduling tick:
+ * it admits pending requests in arrival order while their reservation (their
+ * ciphertext + per-item scratch at their level) fits mem_budget_bytes and fewer
+ * than max_active are active, then issues the next operation of every active
+ * request as batched launches (continuous batching). The host runs ahead of the
+ * device by at most `depth` ticks. Latency = completion observed - arrival. */
+typedef struct phx_server phx_server;
+typedef struct {
+    uint64_t submitted, admitted, completed, ticks, ops;
+    double p50_ms, p99_ms, mean_ms, max_ms;
+    double span_s, req_per_s, ops_per_s;
+    uint64_t peak_reserved_bytes, budget_bytes, peak_active;
+} phx_server_stats;
+/* lane i serves ciphertexts of ctxs[i] with rlks[i] and gks[i] */
+int phx_server_create(const phx_context *const *ctxs, const phx_relin_key *const *rlks,
+                      const phx_galois_key *const *gks, size_t nlanes, size_t mem_budget_bytes, size_t max_active,
+                      size_t depth, phx_stream stream, phx_server **out);
+/* seconds since the server was created (the clock of arrival times) */
+int phx_server_now(const phx_server *srv, double *t);
+/* copies ct (size 2, NTT form); arrival < 0 means now */
+int phx_server_submit(phx_server *srv, size_t lane, const phx_ciphertext *ct, const int *steps, size_t k,
+                      double arrival, uint64_t *id);
+int phx_server_step(phx_server *srv, size_t *issued_ops);
+int phx_server_drain(phx_server *srv);
+/* 0 pending, 1 active, 2 complete (result waiting), 3 taken or unknown */
+int phx_server_status(const phx_server *srv, uint64_t id, int *state);
+/* moves a complete request's result into out and frees its reservation */
+int phx_server_take(phx_server *srv, uint64_t id, phx_ciphertext *out, double *latency_s);
+int phx_server_stats_get(const phx_server *srv, phx_server_stats *out);
+int phx_server_destroy(phx_server *srv);
+
 /* ---- low-level RNS kernels (device pointers) ------------------------------- */
 /* nwt_2d_radix8_forward_inplace (include/ntt.cuh:173-226, src/ntt/fntt_2d.cu:620-653):
  * limbs [0, L) of dev use QP table rows [start, start + L) */

@@ -1,6 +1,7 @@
 // batch.hip -- batched multiply-and-relinearise and rotation (see batch.h).
 #include <algorithm>
 #include <map>
+#include <memory>
 
 #include "batch.h"
 #include "tune.h"
@@ -100,8 +101,11 @@ __global__ __launch_bounds__(kThreads) void k_inner_product_batch(const u64 *__r
 BatchPlan::~BatchPlan() {
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
-    (void)hipDeviceSynchronize();
-    for (void *p : allocations_) (void)hipFree(p);
+    if (arena_ || tables_) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(arena_);
+        (void)hipFree(const_cast<void **>(tables_));
+    }
 }
 
 void BatchPlan::build_groups(const std::vector<u32> &chains) {
@@ -120,34 +124,76 @@ void BatchPlan::build_groups(const std::vector<u32> &chains) {
     }
 }
 
-void BatchPlan::alloc_group(Group &g) {
-    const u64 N = ctx_.N(), LN = (u64)g.Ql * N, LPN = (u64)g.QlP * N;
-    auto alloc = [&](size_t bytes) {
-        void *p = nullptr;
-        PHX_HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
-        allocations_.push_back(p);
-        scratch_bytes_ += bytes;
-        return p;
-    };
-    g.c2 = static_cast<u64 *>(alloc(g.B * LN * 8));
-    g.t_cks = static_cast<u64 *>(alloc(g.B * LN * 8));
-    g.t_mod_up = static_cast<u64 *>(alloc((size_t)g.B * g.beta * LPN * 8));
-    g.cx = static_cast<u64 *>(alloc((size_t)g.B * 2 * LPN * 8));
-    g.delta = static_cast<u64 *>(alloc((size_t)g.B * 2 * LN * 8));
-    if (kind_ == kRotate) g.base0 = static_cast<u64 *>(alloc(g.B * LN * 8));
-    g.a = static_cast<const u64 **>(alloc(g.B * sizeof(void *)));
-    g.b = static_cast<const u64 **>(alloc(g.B * sizeof(void *)));
-    g.out = static_cast<u64 **>(alloc(g.B * sizeof(void *)));
-    g.keys = static_cast<const u64 **>(alloc(g.B * sizeof(void *)));
-    g.perms = static_cast<const u32 **>(alloc(g.B * sizeof(void *)));
+namespace {
+// scratch words of one item: c2, t_cks, t_mod_up, cx, delta (+ base0 for rotations)
+size_t item_words(u64 N, u32 Ql, u32 QlP, u32 beta, bool rotate) {
+    return (u64)N * ((u64)Ql * (2 + (rotate ? 1 : 0)) + (u64)beta * QlP + 2ull * QlP + 2ull * Ql);
+}
+}  // namespace
+
+size_t BatchPlan::item_scratch_bytes(const Context &ctx, u32 chain, Kind k) {
+    const LevelConsts &L = ctx.level(chain);
+    return 8 * item_words(ctx.N(), L.size_Ql, L.size_QlP, L.beta, k == kRotate);
+}
+
+size_t BatchPlan::group_scratch_words(const Group &g) const {
+    return (size_t)g.B * item_words(ctx_.N(), g.Ql, g.QlP, g.beta, kind_ == kRotate);
+}
+
+void BatchPlan::set_tables(Group &g, const std::vector<const void *> &a, const std::vector<const void *> &b,
+                           const std::vector<const void *> &out, const std::vector<const void *> &keys,
+                           const std::vector<const void *> &perms) {
+    g.table0 = host_tables_.size();
+    for (const auto *v : {&a, &b, &out, &keys, &perms})
+        for (u32 i = 0; i < g.B; ++i) host_tables_.push_back(v->empty() ? nullptr : (*v)[i]);
+}
+
+void BatchPlan::layout() {
+    size_t words = 0;
+    for (const auto &g : groups_) words = std::max(words, group_scratch_words(g));
+    scratch_bytes_ = words * 8;
+    if (!transient_) {
+        PHX_HIP_CHECK(hipMalloc(&arena_, std::max<size_t>(scratch_bytes_, 256)));
+        void *t = nullptr;
+        PHX_HIP_CHECK(hipMalloc(&t, std::max<size_t>(host_tables_.size() * sizeof(void *), 256)));
+        tables_ = static_cast<const void **>(t);
+        PHX_HIP_CHECK(hipMemcpy(t, host_tables_.data(), host_tables_.size() * sizeof(void *),
+                                hipMemcpyHostToDevice));
+        bind(arena_, tables_);
+    }
+}
+
+void BatchPlan::bind(u64 *arena, const void **tables) {
+    const u64 N = ctx_.N();
+    for (auto &g : groups_) {
+        const u64 LN = (u64)g.Ql * N, LPN = (u64)g.QlP * N;
+        u64 *p = arena;
+        auto take = [&](u64 words) {
+            u64 *r = p;
+            p += words;
+            return r;
+        };
+        g.c2 = take(g.B * LN);
+        g.t_cks = take(g.B * LN);
+        g.t_mod_up = take((u64)g.B * g.beta * LPN);
+        g.cx = take((u64)g.B * 2 * LPN);
+        g.delta = take((u64)g.B * 2 * LN);
+        g.base0 = kind_ == kRotate ? take(g.B * LN) : nullptr;
+        const void **t = tables + g.table0;
+        g.a = reinterpret_cast<const u64 **>(t);
+        g.b = reinterpret_cast<const u64 **>(t + g.B);
+        g.out = reinterpret_cast<u64 **>(const_cast<void **>(t + 2 * g.B));
+        g.keys = reinterpret_cast<const u64 **>(t + 3 * g.B);
+        g.perms = reinterpret_cast<const u32 **>(t + 4 * g.B);
+    }
 }
 
 BatchPlan *BatchPlan::mul_relin(const Context &ctx, const std::vector<Ciphertext *> &ct1,
-                                const std::vector<const Ciphertext *> &ct2, const KSwitchKey &rlk) {
+                                const std::vector<const Ciphertext *> &ct2, const KSwitchKey &rlk, bool transient) {
     require(ctx.params().scheme == kSchemeCKKS, "batched multiply_and_relin: only CKKS is on this path");
     require(ct1.size() == ct2.size() && !ct1.empty(), "batch operands must be non-empty and of equal count");
     require(rlk.data != nullptr, "relinearisation key is empty");
-    auto *p = new BatchPlan(ctx, kMulRelin);
+    std::unique_ptr<BatchPlan> p(new BatchPlan(ctx, kMulRelin, transient));
     std::vector<u32> chains;
     for (size_t i = 0; i < ct1.size(); ++i) {
         require(ct1[i] && ct2[i] && ct1[i]->data && ct2[i]->data, "batch ciphertext is empty");
@@ -167,27 +213,23 @@ BatchPlan *BatchPlan::mul_relin(const Context &ctx, const std::vector<Ciphertext
     p->items_ = ct1.size();
     p->build_groups(chains);
     for (auto &g : p->groups_) {
-        p->alloc_group(g);
-        std::vector<const u64 *> a, b, keys;
-        std::vector<u64 *> out;
+        std::vector<const void *> a, b, out, keys;
         for (u32 i : g.items) {
             a.push_back(ct1[i]->data);
             b.push_back(ct2[i]->data);
             out.push_back(ct1[i]->data);
             keys.push_back(rlk.data);
         }
-        PHX_HIP_CHECK(hipMemcpy(g.a, a.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
-        PHX_HIP_CHECK(hipMemcpy(g.b, b.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
-        PHX_HIP_CHECK(hipMemcpy(g.out, out.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
-        PHX_HIP_CHECK(hipMemcpy(g.keys, keys.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
+        p->set_tables(g, a, b, out, keys, {});
     }
-    return p;
+    p->layout();
+    return p.release();
 }
 
 BatchPlan *BatchPlan::rotate(const Context &ctx, const std::vector<Ciphertext *> &cts, const std::vector<int> &steps,
-                             const GaloisKeys &gk) {
+                             const GaloisKeys &gk, bool transient) {
     require(cts.size() == steps.size() && !cts.empty(), "batch operands must be non-empty and of equal count");
-    auto *p = new BatchPlan(ctx, kRotate);
+    std::unique_ptr<BatchPlan> p(new BatchPlan(ctx, kRotate, transient));
     std::vector<u32> chains;
     std::vector<const u64 *> item_key(cts.size());
     std::vector<const u32 *> item_perm(cts.size());
@@ -208,25 +250,20 @@ BatchPlan *BatchPlan::rotate(const Context &ctx, const std::vector<Ciphertext *>
     p->items_ = cts.size();
     p->build_groups(chains);
     for (auto &g : p->groups_) {
-        p->alloc_group(g);
         // items sharing a key are placed next to each other so the inner product reuses it
         std::stable_sort(g.items.begin(), g.items.end(),
                          [&](u32 x, u32 y) { return item_key[x] < item_key[y]; });
-        std::vector<const u64 *> a, keys;
-        std::vector<u64 *> out;
-        std::vector<const u32 *> perms;
+        std::vector<const void *> a, out, keys, perms;
         for (u32 i : g.items) {
             a.push_back(cts[i]->data);
             out.push_back(cts[i]->data);
             keys.push_back(item_key[i]);
             perms.push_back(item_perm[i]);
         }
-        PHX_HIP_CHECK(hipMemcpy(g.a, a.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
-        PHX_HIP_CHECK(hipMemcpy(g.out, out.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
-        PHX_HIP_CHECK(hipMemcpy(g.keys, keys.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
-        PHX_HIP_CHECK(hipMemcpy(g.perms, perms.data(), g.B * sizeof(void *), hipMemcpyHostToDevice));
+        p->set_tables(g, a, {}, out, keys, perms);
     }
-    return p;
+    p->layout();
+    return p.release();
 }
 
 // The same pipeline as keyswitch() in evaluator.hip, each stage over the group.
@@ -332,12 +369,24 @@ void BatchPlan::run_group(const Group &g, hipStream_t s) {
 }
 
 void BatchPlan::run(hipStream_t s) {
-    for (const auto &g : groups_) run_group(g, s);
+    if (transient_) {
+        u64 *arena = static_cast<u64 *>(dev_alloc(std::max<size_t>(scratch_bytes_, 256), s));
+        void *t = dev_alloc(host_tables_.size() * sizeof(void *), s);
+        PHX_HIP_CHECK(hipMemcpyAsync(t, host_tables_.data(), host_tables_.size() * sizeof(void *),
+                                     hipMemcpyHostToDevice, s));
+        bind(arena, static_cast<const void **>(t));
+        for (const auto &g : groups_) run_group(g, s);
+        dev_free(t, s);
+        dev_free(arena, s);
+    } else {
+        for (const auto &g : groups_) run_group(g, s);
+    }
     if (kind_ == kMulRelin)
         for (size_t i = 0; i < cts_.size(); ++i) cts_[i]->scale *= scale_mul_[i];
 }
 
 void BatchPlan::capture(hipStream_t s) {
+    require(!transient_, "a transient batch cannot be captured");
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     exec_ = nullptr;

@@ -8,6 +8,7 @@
 #include "evaluator.h"
 #include "tune.h"
 #include "ckks.h"
+#include "serve.h"
 
 using namespace phx;
 
@@ -28,6 +29,9 @@ struct phx_galois_key {
 };
 struct phx_batch {
     BatchPlan *plan;
+};
+struct phx_server {
+    Server *srv;
 };
 
 namespace {
@@ -582,6 +586,95 @@ int phx_batch_destroy(phx_batch *b) {
         if (!b) return;
         delete b->plan;
         delete b;
+    });
+}
+
+int phx_server_create(const phx_context *const *ctxs, const phx_relin_key *const *rlks,
+                      const phx_galois_key *const *gks, size_t nlanes, size_t mem_budget, size_t max_active,
+                      size_t depth, phx_stream s, phx_server **out) {
+    return guard([&] {
+        nn(ctxs, "contexts");
+        nn(rlks, "relinearisation keys");
+        nn(gks, "Galois keys");
+        nn(out, "out");
+        std::vector<Server::Lane> lanes;
+        for (size_t i = 0; i < nlanes; ++i) {
+            nn(ctxs[i], "context");
+            nn(rlks[i], "relinearisation key");
+            nn(gks[i], "Galois keys");
+            lanes.push_back({ctxs[i]->ctx, &rlks[i]->key, &gks[i]->gk});
+        }
+        require(s != nullptr, "the server needs an explicit stream");
+        *out = new phx_server{new Server(lanes, mem_budget, (u32)max_active, (u32)depth, S(s))};
+    });
+}
+
+int phx_server_now(const phx_server *srv, double *t) {
+    return guard([&] {
+        nn(srv, "server");
+        nn(t, "t");
+        *t = srv->srv->now();
+    });
+}
+
+int phx_server_submit(phx_server *srv, size_t lane, const phx_ciphertext *ct, const int *steps, size_t k,
+                      double arrival, uint64_t *id) {
+    return guard([&] {
+        nn(srv, "server");
+        nn(ct, "ciphertext");
+        nn(steps, "steps");
+        nn(id, "id");
+        *id = srv->srv->submit((u32)lane, ct->ct, std::vector<int>(steps, steps + k), arrival);
+    });
+}
+
+int phx_server_step(phx_server *srv, size_t *issued) {
+    return guard([&] {
+        nn(srv, "server");
+        const u32 n = srv->srv->step();
+        if (issued) *issued = n;
+    });
+}
+
+int phx_server_drain(phx_server *srv) {
+    return guard([&] {
+        nn(srv, "server");
+        srv->srv->drain();
+    });
+}
+
+int phx_server_status(const phx_server *srv, uint64_t id, int *state) {
+    return guard([&] {
+        nn(srv, "server");
+        nn(state, "state");
+        *state = srv->srv->status(id);
+    });
+}
+
+int phx_server_take(phx_server *srv, uint64_t id, phx_ciphertext *out, double *latency_s) {
+    return guard([&] {
+        nn(srv, "server");
+        nn(out, "out");
+        srv->srv->take(id, out->ct, latency_s);
+    });
+}
+
+int phx_server_stats_get(const phx_server *srv, phx_server_stats *out) {
+    return guard([&] {
+        nn(srv, "server");
+        nn(out, "out");
+        const ServerStats st = srv->srv->stats();
+        *out = phx_server_stats{st.submitted, st.admitted,   st.completed, st.ticks,        st.ops,
+                                st.p50_ms,    st.p99_ms,     st.mean_ms,   st.max_ms,       st.span_s,
+                                st.req_per_s, st.ops_per_s,  st.peak_reserved, st.budget, st.peak_active};
+    });
+}
+
+int phx_server_destroy(phx_server *srv) {
+    return guard([&] {
+        if (!srv) return;
+        delete srv->srv;
+        delete srv;
     });
 }
 

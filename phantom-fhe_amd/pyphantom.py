@@ -76,6 +76,17 @@ _SIGS = {
     "phx_galois_key_destroy": (ctypes.c_int, [_vp]),
     "phx_encrypt_symmetric": (ctypes.c_int, [_vp, _vp, _u64p, _sz, ctypes.c_uint64, _vp, _vp]),
     "phx_decrypt": (ctypes.c_int, [_vp, _vp, _vp, _u64p, _vp]),
+    "phx_server_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _sz, _sz,
+                                         _sz, _sz, _vp, ctypes.POINTER(_vp)]),
+    "phx_server_now": (ctypes.c_int, [_vp, _dp]),
+    "phx_server_submit": (ctypes.c_int, [_vp, _sz, _vp, ctypes.POINTER(ctypes.c_int), _sz, ctypes.c_double,
+                                         _u64p]),
+    "phx_server_step": (ctypes.c_int, [_vp, ctypes.POINTER(_sz)]),
+    "phx_server_drain": (ctypes.c_int, [_vp]),
+    "phx_server_status": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    "phx_server_take": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp, _dp]),
+    "phx_server_stats_get": (ctypes.c_int, [_vp, _vp]),
+    "phx_server_destroy": (ctypes.c_int, [_vp]),
     "phx_ckks_encode": (ctypes.c_int, [_vp, _dp, _dp, _sz, ctypes.c_double, _sz, _u64p]),
     "phx_ckks_decode": (ctypes.c_int, [_vp, _u64p, _sz, ctypes.c_double, _dp, _dp]),
     "phx_multiply_and_relin_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
@@ -288,6 +299,70 @@ class ciphertext:
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.phx_ciphertext_destroy(self._h)
+            self._h = None
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "admitted", "completed", "ticks", "ops")] + \
+               [(n, ctypes.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "span_s", "req_per_s",
+                                               "ops_per_s")] + \
+               [(n, ctypes.c_uint64) for n in ("peak_reserved_bytes", "budget_bytes", "peak_active")]
+
+
+class server:
+    """Continuous-batching request loop (phx_server_*): request = ciphertext and
+    rotation steps; link i is rotate(rescale(ct * ct relinearised), steps[i])."""
+
+    def __init__(self, lanes, mem_budget, max_active=256, depth=2, stream=None):
+        n = len(lanes)
+        ctxs = (_vp * n)(*[c._h for c, _, _ in lanes])
+        rks = (_vp * n)(*[r._h for _, r, _ in lanes])
+        gks = (_vp * n)(*[g._h for _, _, g in lanes])
+        self._keep = lanes
+        self.stream = stream if stream is not None else stream_create()
+        h = _vp()
+        _check(lib().phx_server_create(ctxs, rks, gks, n, int(mem_budget), int(max_active), int(depth),
+                                       self.stream, ctypes.byref(h)))
+        self._h = h
+
+    def now(self):
+        t = ctypes.c_double()
+        _check(lib().phx_server_now(self._h, ctypes.byref(t)))
+        return t.value
+
+    def submit(self, lane, ct, steps, arrival=-1.0):
+        st = (ctypes.c_int * len(steps))(*steps)
+        rid = ctypes.c_uint64()
+        _check(lib().phx_server_submit(self._h, lane, ct._h, st, len(steps), float(arrival), ctypes.byref(rid)))
+        return rid.value
+
+    def step(self):
+        n = _sz()
+        _check(lib().phx_server_step(self._h, ctypes.byref(n)))
+        return n.value
+
+    def drain(self):
+        _check(lib().phx_server_drain(self._h))
+
+    def status(self, rid):
+        v = ctypes.c_int()
+        _check(lib().phx_server_status(self._h, rid, ctypes.byref(v)))
+        return v.value
+
+    def take(self, rid):
+        out = ciphertext()
+        lat = ctypes.c_double()
+        _check(lib().phx_server_take(self._h, rid, out._h, ctypes.byref(lat)))
+        return out, lat.value
+
+    def stats(self):
+        st = _ServerStats()
+        _check(lib().phx_server_stats_get(self._h, ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in st._fields_}
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.phx_server_destroy(self._h)
             self._h = None
 
 

@@ -160,3 +160,44 @@ def test_batch_fused_inner_product_variant(small):
         assert np.array_equal(cx[0].download(N), octx.apply_galois(5, x, elt, key))
     finally:
         ph.tune_set("ks_fused_ip", old)
+
+
+CONFIG3_STEPS = [1, 2, 3, 4, 5, 7, 8, 16, 32, 64, 128, 256, 512, 1024, -1, -5]
+
+
+def test_config3_batched_rotations_bit_exact():
+    """128 ciphertexts at N=2^16 with 5..35 limbs, each rotated by a step drawn
+    from 16 resident rotation keys, in one captured batch plan; sampled items
+    (smallest and largest level among them) bit-exact against the oracle."""
+    ob.set_threads(16)
+    N = 1 << 16
+    say("case: config3 batched rotations (128 cts, 16 keys)")
+    ctx, octx, mods = make_ctx(N, [60] + [50] * 34 + [60] * 12, 12)
+    elts = [ph.galois_elt_from_step(s, N) for s in CONFIG3_STEPS]
+    rng = np.random.default_rng(33)
+    key_seeds = [1000 + i for i in range(len(elts))]
+
+    def key_of(i):
+        return rand_key(np.random.default_rng(key_seeds[i]), mods, N, 3)
+
+    gk = ph.galois_key(ctx, elts, host=np.stack([key_of(i) for i in range(len(elts))]))
+    limbs = rng.integers(5, 36, 128)
+    steps_idx = rng.integers(0, len(elts), 128)
+    chains = [36 - int(l) for l in limbs]           # chain c holds 35 - (c - 1) limbs
+    data_seeds = [5000 + i for i in range(128)]
+    cts = []
+    for i in range(128):
+        a = rand_ct(np.random.default_rng(data_seeds[i]), mods[:int(limbs[i])], N)
+        cts.append(upload_ct(ctx, a, chains[i]))
+    stream = ph.stream_create()
+    bt = ph.batch.rotate(ctx, cts, [CONFIG3_STEPS[k] for k in steps_idx], gk)
+    bt.capture(stream)
+    bt.replay(stream)
+    ph.synchronize(stream)
+    order = np.argsort(limbs)
+    for i in [int(order[0]), int(order[-1]), 0, 17, 64, 127]:
+        Ql = int(limbs[i])
+        a = rand_ct(np.random.default_rng(data_seeds[i]), mods[:Ql], N)
+        k = int(steps_idx[i])
+        want = octx.apply_galois(Ql, a, elts[k], key_of(k))
+        assert np.array_equal(cts[i].download(N), want), (i, Ql, CONFIG3_STEPS[k])
