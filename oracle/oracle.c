@@ -748,3 +748,266 @@ int orc_make_kswitch_key(const orc_ctx *c, const u64 *s_ntt, const u64 *s2_ntt, 
     free(en);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* BFV multiply with BEHZ base extension, BFV relinearisation          */
+/* ------------------------------------------------------------------ */
+
+/* BEHZ constants of the top data level (src/rns.cu:392-571): base B of
+ * 61-bit primes (size_Q, one more when 32 + bits(t) + bits(Q) >= 61 (size_Q + 1)),
+ * m_sk (the first prime of the same search), m_tilde = 2^32, Bsk = B u {m_sk}. */
+struct orc_behz {
+    size_t size_Q, size_B, size_Bsk;
+    u64 t, m_tilde;
+    u64 *bsk;                        /* [Bsk]: B then m_sk */
+    u64 *rp, *rps, *irp, *irps, *ninv;   /* NTT tables of the Bsk primes */
+};
+
+static int bits_of_product(const u64 *q, size_t n) {
+    /* significant bits of prod q_i, exact (multi-word) */
+    u64 w[64] = {0};
+    size_t len = 1;
+    w[0] = 1;
+    for (size_t i = 0; i < n; i++) {
+        u128 carry = 0;
+        for (size_t k = 0; k < len; k++) {
+            u128 v = (u128)w[k] * q[i] + carry;
+            w[k] = (u64)v;
+            carry = v >> 64;
+        }
+        if (carry) w[len++] = (u64)carry;
+    }
+    int b = 64 * (int)(len - 1);
+    u64 top = w[len - 1];
+    while (top) { b++; top >>= 1; }
+    return b;
+}
+
+static int bit_count(u64 v) { int b = 0; while (v) { b++; v >>= 1; } return b; }
+
+orc_behz *orc_behz_create(const orc_ctx *c, u64 t) {
+    orc_behz *b = (orc_behz *)calloc(1, sizeof(orc_behz));
+    b->size_Q = c->size_Q;
+    b->t = t;
+    b->m_tilde = (u64)1 << 32;
+    int total = bits_of_product(c->q, c->size_Q);
+    b->size_B = c->size_Q;
+    if (32 + bit_count(t) + total >= 61 * (int)c->size_Q + 61) b->size_B++;
+    b->size_Bsk = b->size_B + 1;
+    u64 *primes = (u64 *)malloc((b->size_B + 1) * 8);
+    orc_get_primes(c->N, 61, b->size_B + 1, primes);
+    b->bsk = (u64 *)malloc(b->size_Bsk * 8);
+    memcpy(b->bsk, primes + 1, b->size_B * 8);   /* B */
+    b->bsk[b->size_B] = primes[0];                /* m_sk */
+    free(primes);
+    size_t N = c->N, tot = b->size_Bsk * N;
+    b->rp = (u64 *)malloc(tot * 8); b->rps = (u64 *)malloc(tot * 8);
+    b->irp = (u64 *)malloc(tot * 8); b->irps = (u64 *)malloc(tot * 8);
+    b->ninv = (u64 *)malloc(b->size_Bsk * 8);
+    for (size_t i = 0; i < b->size_Bsk; i++)
+        orc_ntt_tables(b->bsk[i], c->logN, b->rp + i * N, b->rps + i * N, b->irp + i * N, b->irps + i * N,
+                       b->ninv + i);
+    return b;
+}
+
+void orc_behz_destroy(orc_behz *b) {
+    if (!b) return;
+    free(b->bsk); free(b->rp); free(b->rps); free(b->irp); free(b->irps); free(b->ninv);
+    free(b);
+}
+
+size_t orc_behz_bsk_size(const orc_behz *b) { return b->size_Bsk; }
+void orc_behz_bsk(const orc_behz *b, u64 *out) { memcpy(out, b->bsk, b->size_Bsk * 8); }
+
+static void bsk_ntt(const orc_behz *b, size_t N, u64 *data, int inverse) {
+    for (size_t i = 0; i < b->size_Bsk; i++) {
+        if (inverse)
+            ntt_inv_limb(data + i * N, N, b->bsk[i], b->irp + i * N, b->irps + i * N, b->ninv[i]);
+        else
+            ntt_fwd_limb(data + i * N, N, b->bsk[i], b->rp + i * N, b->rps + i * N);
+    }
+}
+
+/* mod of a value below 2^64 by m (m may be 2^32) */
+static inline u64 mod64(u64 x, u64 m) { return x % m; }
+
+/* BEHZ steps (1)-(3) for one polynomial (BEHZ_mul_1, src/evaluate.cu:404-438):
+ * fastbconv_m_tilde (src/rns.cu:1249-1277) then sm_mrq (1290-1333); out in
+ * coefficient form [Bsk][N] */
+static void behz_lift(const orc_ctx *c, const orc_behz *b, const u64 *x, u64 *out) {
+    size_t N = c->N, k = b->size_Q, nb = b->size_Bsk;
+    u64 mt = b->m_tilde;
+    u64 *y = (u64 *)malloc(k * N * 8);
+    for (size_t i = 0; i < k; i++) {
+        u64 q = c->q[i];
+        /* m_tilde * QHatInv mod q_i (src/rns.cu:452-459) */
+        u64 f = mulmod(mt % q, invmod(punctured_mod(c->q, k, i, q), q), q);
+        for (size_t n = 0; n < N; n++) y[i * N + n] = mulmod(x[i * N + n], f, q);
+    }
+    /* -Q^-1 mod m_tilde (src/rns.cu:546-553); inverse of an odd value mod 2^32 */
+    u64 qmod = prod_mod(c->q, k, mt), inv = 1;
+    for (int it = 0; it < 6; it++) inv = (inv * (2 - qmod * inv)) & (mt - 1);
+    u64 neg_inv_q = (mt - inv) & (mt - 1);
+    for (size_t p = 0; p < nb; p++) {
+        u64 pm = b->bsk[p];
+        u64 qmodp = prod_mod(c->q, k, pm);
+        u64 mtinv = invmod(mt % pm, pm);
+        for (size_t n = 0; n < N; n++) {
+            u128 acc = 0, accm = 0;
+            for (size_t i = 0; i < k; i++) {
+                acc += (u128)y[i * N + n] * punctured_mod(c->q, k, i, pm);
+                accm += (u128)y[i * N + n] * punctured_mod(c->q, k, i, mt);
+            }
+            u64 yp = (u64)(acc % pm), ym = (u64)(accm % mt);
+            u64 r = mod64(ym * neg_inv_q, mt);
+            if (r >= mt >> 1) r += pm - mt;
+            u128 v = (u128)r * qmodp + yp;
+            out[p * N + n] = mulmod((u64)(v % pm), mtinv, pm);
+        }
+    }
+    free(y);
+}
+
+/* bfv_multiply_behz (src/evaluate.cu:440-545): ct1, ct2 [2][size_Q][N]
+ * coefficient form -> out3 [3][size_Q][N] */
+int orc_bfv_multiply(const orc_ctx *c, const orc_behz *b, const u64 *ct1, const u64 *ct2, u64 *out3) {
+    size_t N = c->N, k = b->size_Q, nb = b->size_Bsk, QN = k * N, BN = nb * N;
+    u64 *aq = (u64 *)malloc(3 * QN * 8), *bq = (u64 *)malloc(2 * QN * 8);
+    u64 *ab = (u64 *)malloc(3 * BN * 8), *bb = (u64 *)malloc(2 * BN * 8);
+    const u64 *src[2] = {ct1, ct2};
+    u64 *dq[2] = {aq, bq}, *db[2] = {ab, bb};
+    for (int s = 0; s < 2; s++)
+        for (int poly = 0; poly < 2; poly++) {
+            memcpy(dq[s] + poly * QN, src[s] + poly * QN, QN * 8);
+            orc_ntt_forward(c, dq[s] + poly * QN, k, 0);
+            behz_lift(c, b, src[s] + poly * QN, db[s] + poly * BN);
+            bsk_ntt(b, N, db[s] + poly * BN, 0);
+        }
+    /* (4) tensor in both bases (tensor_prod_2x2_rns_poly, src/polymath.cu:463-498) */
+    for (int base = 0; base < 2; base++) {
+        size_t L = base ? nb : k;
+        u64 *A = base ? ab : aq, *B = base ? bb : bq;
+        for (size_t i = 0; i < L; i++) {
+            u64 q = base ? b->bsk[i] : c->q[i];
+            for (size_t n = 0; n < N; n++) {
+                size_t o = i * N + n, LN = L * N;
+                u64 a0 = A[o], a1 = A[LN + o], b0 = B[o], b1 = B[LN + o];
+                u64 r0 = mulmod(a0, b0, q), r2 = mulmod(a1, b1, q);
+                u64 r1 = mulmod(addmod(a0, a1, q), addmod(b0, b1, q), q);
+                r1 = submod(submod(r1, r0, q), r2, q);
+                A[o] = r0; A[LN + o] = r1; A[2 * LN + o] = r2;
+            }
+        }
+    }
+    /* (5)-(6) INTT, times t (nwt_2d_radix8_backward_inplace_scale) */
+    for (int poly = 0; poly < 3; poly++) {
+        orc_ntt_inverse(c, aq + poly * QN, k, 0);
+        bsk_ntt(b, N, ab + poly * BN, 1);
+        for (size_t i = 0; i < k; i++)
+            for (size_t n = 0; n < N; n++) aq[poly * QN + i * N + n] = mulmod(aq[poly * QN + i * N + n], b->t, c->q[i]);
+        for (size_t i = 0; i < nb; i++)
+            for (size_t n = 0; n < N; n++)
+                ab[poly * BN + i * N + n] = mulmod(ab[poly * BN + i * N + n], b->t % b->bsk[i], b->bsk[i]);
+    }
+    /* (7) fast_floor (src/rns.cu:1379-1407) and (8) fastbconv_sk (1455-1517,
+     * multiply_and_negated_add_rns_poly src/polymath.cu:606-634) */
+    size_t nB = b->size_B;
+    u64 msk = b->bsk[nB];
+    u64 *w = (u64 *)malloc(BN * 8);
+    u64 *yq = (u64 *)malloc(QN * 8), *u = (u64 *)malloc(nB * N * 8);
+    for (int poly = 0; poly < 3; poly++) {
+        const u64 *zq = aq + poly * QN, *zb = ab + poly * BN;
+        for (size_t i = 0; i < k; i++) {
+            u64 q = c->q[i], hinv = invmod(punctured_mod(c->q, k, i, q), q);
+            for (size_t n = 0; n < N; n++) yq[i * N + n] = mulmod(zq[i * N + n], hinv, q);
+        }
+        for (size_t p = 0; p < nb; p++) {
+            u64 pm = b->bsk[p], qinv = invmod(prod_mod(c->q, k, pm), pm);
+            for (size_t n = 0; n < N; n++) {
+                u128 acc = 0;
+                for (size_t i = 0; i < k; i++) acc += (u128)yq[i * N + n] * punctured_mod(c->q, k, i, pm);
+                u64 cv = (u64)(acc % pm);
+                w[p * N + n] = mulmod(addmod(zb[p * N + n], pm - cv == pm ? 0 : pm - cv, pm), qinv, pm);
+            }
+        }
+        for (size_t j = 0; j < nB; j++) {
+            u64 bj = b->bsk[j], hinv = invmod(punctured_mod(b->bsk, nB, j, bj), bj);
+            for (size_t n = 0; n < N; n++) u[j * N + n] = mulmod(w[j * N + n], hinv, bj);
+        }
+        u64 binv_msk = invmod(prod_mod(b->bsk, nB, msk), msk);
+        u64 *out = out3 + poly * QN;
+        for (size_t n = 0; n < N; n++) {
+            u128 acc = 0;
+            for (size_t j = 0; j < nB; j++) acc += (u128)u[j * N + n] * punctured_mod(b->bsk, nB, j, msk);
+            u64 bc = (u64)(acc % msk);
+            u64 alpha = mulmod(addmod(bc, msk - w[nB * N + n], msk), binv_msk, msk);
+            for (size_t i = 0; i < k; i++) {
+                u64 q = c->q[i];
+                u128 a2 = 0;
+                for (size_t j = 0; j < nB; j++) a2 += (u128)u[j * N + n] * punctured_mod(b->bsk, nB, j, q);
+                u64 v = (u64)(a2 % q);
+                u64 bq_ = prod_mod(b->bsk, nB, q);
+                u64 corr;
+                if (alpha > msk >> 1)
+                    corr = mulmod((msk - alpha) % q, bq_, q);          /* -alpha*B, alpha negative */
+                else
+                    corr = mulmod(alpha % q, bq_ ? q - bq_ : 0, q);   /* alpha * (-B mod q) */
+                out[i * N + n] = addmod(v, corr, q);
+            }
+        }
+    }
+    free(aq); free(bq); free(ab); free(bb); free(w); free(yq); free(u);
+    return 0;
+}
+
+/* relinearize_inplace for BFV (src/evaluate.cu:1342-1374) through
+ * keyswitch_inplace (src/eval_key_switch.cu:95-182) with the BFV branches of
+ * DRNSTool::modup (src/rns_bconv.cu:530-628: own limbs copied in coefficient
+ * form, complement limbs FastBConv of the digit, then NTT of all QlP limbs)
+ * and moddown_from_NTT (776-828: INTT of all QlP limbs, delta = FastBConv
+ * P -> Ql, (cx - delta) * P^-1 in coefficient form), added to ct[0], ct[1]. */
+int orc_bfv_relinearize(const orc_ctx *c, const u64 *ct3, const u64 *key, size_t dnum, u64 *out2) {
+    size_t N = c->N, size_Ql = c->size_Q, alpha = c->size_P, size_QlP = size_Ql + c->size_P;
+    size_t beta = orc_beta(c, size_Ql), QN = size_Ql * N;
+    if (beta > dnum) return -1;
+    const u64 *c2 = ct3 + 2 * QN;
+    u64 *qlp = (u64 *)malloc(size_QlP * 8);
+    size_t *rows = (size_t *)malloc(size_QlP * sizeof(size_t));
+    for (size_t p = 0; p < size_QlP; p++) {
+        rows[p] = qlp_row(c, size_Ql, p);
+        qlp[p] = c->q[rows[p]];
+    }
+    u64 *t = (u64 *)malloc(beta * size_QlP * N * 8);
+    for (size_t j = 0; j < beta; j++) {
+        size_t st = alpha * j, sz = (j == beta - 1) ? size_Ql - alpha * (beta - 1) : alpha;
+        u64 *tj = t + j * size_QlP * N;
+        size_t no = size_QlP - sz;
+        u64 *obase = (u64 *)malloc(no * 8), *conv = (u64 *)malloc(no * N * 8);
+        size_t *opos = (size_t *)malloc(no * sizeof(size_t));
+        size_t m = 0;
+        for (size_t p = 0; p < size_QlP; p++)
+            if (p < st || p >= st + sz) { obase[m] = qlp[p]; opos[m] = p; m++; }
+        orc_fastbconv(N, qlp + st, sz, obase, no, c2 + st * N, conv);
+        memcpy(tj + st * N, c2 + st * N, sz * N * 8);
+        for (size_t o = 0; o < no; o++) memcpy(tj + opos[o] * N, conv + o * N, N * 8);
+        ntt_rows(c, tj, rows, size_QlP, 0);
+        free(obase); free(conv); free(opos);
+    }
+    u64 *cx = (u64 *)malloc(2 * size_QlP * N * 8);
+    orc_inner_product(c, size_Ql, t, key, dnum, cx);
+    u64 *delta = (u64 *)malloc(QN * 8);
+    for (int h = 0; h < 2; h++) {
+        u64 *cxi = cx + h * size_QlP * N;
+        ntt_rows(c, cxi, rows, size_QlP, 1);
+        orc_fastbconv(N, c->q + c->size_Q, c->size_P, c->q, size_Ql, cxi + QN, delta);
+        for (size_t i = 0; i < size_Ql; i++) {
+            u64 q = c->q[i], pinv = invmod(prod_mod(c->q + c->size_Q, c->size_P, q), q);
+            for (size_t n = 0; n < N; n++) {
+                u64 v = mulmod(submod(cxi[i * N + n], delta[i * N + n], q), pinv, q);
+                out2[h * QN + i * N + n] = addmod(ct3[h * QN + i * N + n], v, q);
+            }
+        }
+    }
+    free(qlp); free(rows); free(t); free(cx); free(delta);
+    return 0;
+}

@@ -80,6 +80,19 @@ int orc_apply_galois(const orc_ctx *c, size_t size_Ql, const uint64_t *ct, uint3
 int orc_make_kswitch_key(const orc_ctx *c, const uint64_t *s_ntt, const uint64_t *s2_ntt,
                          const uint64_t *a, const uint64_t *e, size_t dnum, uint64_t *key);
 
+/* ---- BFV (coefficient form): BEHZ multiply at the top data level, relinearisation ---- */
+typedef struct orc_behz orc_behz;
+orc_behz *orc_behz_create(const orc_ctx *c, uint64_t t);
+void orc_behz_destroy(orc_behz *b);
+size_t orc_behz_bsk_size(const orc_behz *b);
+void orc_behz_bsk(const orc_behz *b, uint64_t *out);   /* base B, then m_sk */
+/* ct1, ct2 [2][size_Q][N] -> out3 [3][size_Q][N] */
+int orc_bfv_multiply(const orc_ctx *c, const orc_behz *b, const uint64_t *ct1, const uint64_t *ct2,
+                     uint64_t *out3);
+/* ct3 [3][size_Q][N] -> out2 [2][size_Q][N] */
+int orc_bfv_relinearize(const orc_ctx *c, const uint64_t *ct3, const uint64_t *key, size_t dnum,
+                        uint64_t *out2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,14 @@ def lib():
         L.orc_apply_galois.argtypes = [ctypes.c_void_p, _sz, _u64p, ctypes.c_uint32, _u64p, _sz, _u64p]
         L.orc_make_kswitch_key.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u64p, _u64p, _sz, _u64p]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_behz_create.restype = ctypes.c_void_p
+        L.orc_behz_create.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+        L.orc_behz_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_behz_bsk_size.restype = _sz
+        L.orc_behz_bsk_size.argtypes = [ctypes.c_void_p]
+        L.orc_behz_bsk.argtypes = [ctypes.c_void_p, _u64p]
+        L.orc_bfv_multiply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p, _u64p, _u64p]
+        L.orc_bfv_relinearize.argtypes = [ctypes.c_void_p, _u64p, _u64p, _sz, _u64p]
         _lib = L
     return _lib
 
@@ -214,3 +222,40 @@ class Ctx:
         lib().orc_make_kswitch_key(self._h, _p(np.ascontiguousarray(s_ntt)), _p(np.ascontiguousarray(s2_ntt)),
                                    _p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(e)), dnum, _p(key))
         return key
+
+
+class Behz:
+    """BEHZ constants of an oracle context's top data level, BFV multiply."""
+
+    def __init__(self, octx, t):
+        self.octx = octx
+        self.t = t
+        self._h = lib().orc_behz_create(octx._h, t)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.orc_behz_destroy(self._h)
+            self._h = None
+
+    def bsk(self):
+        n = lib().orc_behz_bsk_size(self._h)
+        out = np.zeros(n, np.uint64)
+        lib().orc_behz_bsk(self._h, _p(out))
+        return [int(x) for x in out]
+
+    def multiply(self, ct1, ct2):
+        k = self.octx.size_QP - self.octx.size_P
+        out = np.zeros((3, k, self.octx.N), np.uint64)
+        rc = lib().orc_bfv_multiply(self.octx._h, self._h, _p(np.ascontiguousarray(ct1, np.uint64)),
+                                    _p(np.ascontiguousarray(ct2, np.uint64)), _p(out))
+        assert rc == 0
+        return out
+
+
+def bfv_relinearize(octx, ct3, key):
+    k = octx.size_QP - octx.size_P
+    out = np.zeros((2, k, octx.N), np.uint64)
+    rc = lib().orc_bfv_relinearize(octx._h, _p(np.ascontiguousarray(ct3, np.uint64)), _p(key), key.shape[0],
+                                   _p(out))
+    assert rc == 0
+    return out

@@ -1,0 +1,159 @@
+"""Pins of the oracle's BFV path (BEHZ multiply, BFV relinearisation) by
+closed forms in Python integers, at small ring degree.
+
+BEHZ multiply (src/evaluate.cu:440-545): the lift (fastbconv_m_tilde +
+Montgomery sm_mrq) turns the CRT integer X in [0, Q) of each input coefficient
+into X' = (y + r Q) / m_tilde with y = sum_i [m_tilde x_i Qhat_i^-1]_{q_i} Qhat_i
+and r = -y Q^-1 mod m_tilde taken in [-m_tilde/2, m_tilde/2): X' = X or X - Q,
+the centred representative (but for a window of width ~ size_Q Q / m_tilde
+around Q/2); the tensor is the integer negacyclic product z of the X'; after the
+scale by t, fast_floor returns w = (t z - S) / Q where S = sum_i
+[t z * Qhat_i^-1]_{q_i} Qhat_i is the FastBConv value of [t z]_Q (so w =
+floor(t z / Q) - e, 0 <= e < size_Q); Shenoy-Kumaresan then returns w mod q_i
+exactly. Every output residue is checked against that w.
+
+BFV relinearisation: with a key built from a known ternary s, the output
+satisfies c0' + c1' s = c0 + c1 s + c2 s^2 + small (mod Q).
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from bigint_ref import crt, primes_below
+
+T = 65537
+
+
+def negacyclic_int(a, b):
+    N = len(a)
+    c = [0] * N
+    for i in range(N):
+        if a[i] == 0:
+            continue
+        for j in range(N):
+            k = i + j
+            if k < N:
+                c[k] += a[i] * b[j]
+            else:
+                c[k - N] -= a[i] * b[j]
+    return c
+
+
+def setup(logN, bits, P):
+    N = 1 << logN
+    mods = ob.coeff_modulus_create(N, bits)
+    return ob.Ctx(logN, mods, P), mods, N
+
+
+def test_behz_bases_closed_form():
+    octx, mods, N = setup(5, [30, 30, 30, 31], 1)
+    b = ob.Behz(octx, T)
+    k = 3
+    Q = 1
+    for q in mods[:k]:
+        Q *= q
+    size_B = k + (1 if 32 + T.bit_length() + Q.bit_length() >= 61 * k + 61 else 0)
+    p = primes_below(61, N, size_B + 1)
+    assert b.bsk() == p[1:] + p[:1]          # B (descending), then m_sk = the largest
+
+
+@pytest.mark.parametrize("logN,bits", [(5, [30, 30, 30, 31]), (4, [50, 50, 61]), (5, [54, 54, 54, 54, 60])])
+def test_behz_multiply_matches_integer_floor(logN, bits):
+    octx, mods, N = setup(logN, bits, 1)
+    k = len(mods) - 1
+    qs = mods[:k]
+    Q = 1
+    for q in qs:
+        Q *= q
+    beh = ob.Behz(octx, T)
+    rng = np.random.default_rng(logN * 7 + k)
+    ct1 = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs]) for _ in range(2)])
+    ct2 = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs]) for _ in range(2)])
+    out = beh.multiply(ct1, ct2)
+    qhat = [Q // q for q in qs]
+    qhinv = [pow(h % q, -1, q) for h, q in zip(qhat, qs)]
+    mt = 1 << 32
+    qinv_mt = pow(Q % mt, -1, mt)
+
+    def lift(res):
+        y = sum(((mt * int(x)) * hi % q) * h for x, q, hi, h in zip(res, qs, qhinv, qhat))
+        r = (-y * qinv_mt) % mt
+        if r >= mt // 2:
+            r -= mt
+        assert (y + r * Q) % mt == 0
+        v = (y + r * Q) // mt
+        X = crt([int(x) for x in res], qs)[0]
+        assert v in (X, X - Q)
+        return v
+
+    X = [[lift(ct1[p, :, n]) for n in range(N)] for p in range(2)]
+    Y = [[lift(ct2[p, :, n]) for n in range(N)] for p in range(2)]
+    z0 = negacyclic_int(X[0], Y[0])
+    z2 = negacyclic_int(X[1], Y[1])
+    z1 = [a + b for a, b in zip(negacyclic_int(X[0], Y[1]), negacyclic_int(X[1], Y[0]))]
+    for d, z in enumerate((z0, z1, z2)):
+        for n in range(N):
+            tz = T * z[n]
+            S = sum(((tz % q) * hi % q) * h for q, hi, h in zip(qs, qhinv, qhat))
+            assert (tz - S) % Q == 0
+            w = (tz - S) // Q
+            e = (S - tz % Q) // Q
+            assert 0 <= e < k
+            for i, q in enumerate(qs):
+                assert int(out[d, i, n]) == w % q, (d, n, i)
+
+
+def test_behz_multiply_planted_error_is_caught():
+    """A wrong t (a dropped scale) must fail the closed form above."""
+    octx, mods, N = setup(5, [30, 30, 30, 31], 1)
+    qs = mods[:3]
+    rng = np.random.default_rng(3)
+    ct = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs]) for _ in range(2)])
+    good = ob.Behz(octx, T).multiply(ct, ct)
+    bad = ob.Behz(octx, T + 2).multiply(ct, ct)
+    assert not np.array_equal(good, bad)
+
+
+def ternary(rng, N):
+    return [int(v) for v in rng.integers(-1, 2, N)]
+
+
+def residues(vals, qs):
+    return np.stack([np.array([int(v) % q for v in vals], np.uint64) for q in qs])
+
+
+@pytest.mark.parametrize("logN,bits,P", [(5, [30, 30, 30, 31], 1), (5, [40, 40, 40, 40, 50, 50], 2)])
+def test_bfv_relinearize_decrypts_like_size3(logN, bits, P):
+    octx, mods, N = setup(logN, bits, P)
+    QP = len(mods)
+    k = QP - P
+    qs = mods[:k]
+    rng = np.random.default_rng(11 * logN + P)
+    s = ternary(rng, N)
+    s2 = negacyclic_int(s, s)
+    s_ntt = octx.ntt_forward(residues(s, mods), 0)
+    s2_ntt = octx.ntt_forward(residues(s2, mods), 0)
+    dnum = octx.beta(k)
+    a = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in mods]) for _ in range(dnum)])
+    e = np.stack([residues(rng.integers(-8, 9, N), mods) for _ in range(dnum)])
+    key = octx.make_kswitch_key(s_ntt, s2_ntt, a, e)
+    ct3 = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs]) for _ in range(3)])
+    out = ob.bfv_relinearize(octx, ct3, key)
+    Q = 1
+    for q in qs:
+        Q *= q
+
+    def decrypt_int(polys):
+        acc = [0] * N
+        sp = [1] + [0] * (N - 1)
+        for c in polys:
+            cint = [crt([int(c[i, n]) for i in range(k)], qs)[0] for n in range(N)]
+            term = negacyclic_int(cint, sp) if sp != [1] + [0] * (N - 1) else cint
+            acc = [x + y for x, y in zip(acc, term)]
+            sp = negacyclic_int(sp, s)
+        return acc
+
+    d3 = decrypt_int([ct3[0], ct3[1], ct3[2]])
+    d2 = decrypt_int([out[0], out[1]])
+    noise = [((x - y) % Q + Q // 2) % Q - Q // 2 for x, y in zip(d2, d3)]
+    assert max(abs(v) for v in noise) < 1 << 16, max(abs(v) for v in noise)
