@@ -90,6 +90,9 @@ int phx_ciphertext_copy(const phx_context *ctx, phx_ciphertext *dst, const phx_c
 int phx_ciphertext_info(const phx_ciphertext *ct, size_t *chain_index, size_t *size, size_t *coeff_modulus_size,
                         double *scale, int *is_ntt_form);
 int phx_ciphertext_set_scale(phx_ciphertext *ct, double scale);
+/* PhantomCiphertext::is_ntt_form (include/ciphertext.h); a new ciphertext is in NTT
+ * form for CKKS and in coefficient form for BFV */
+int phx_ciphertext_set_ntt_form(phx_ciphertext *ct, int is_ntt_form);
 int phx_ciphertext_data(const phx_ciphertext *ct, uint64_t **dev_ptr);
 
 /* ---- keys (include/secretkey.h) ------------------------------------------ */

@@ -1,0 +1,13 @@
+// bfv.h -- BFV multiplication (BEHZ) and relinearisation, coefficient form.
+#pragma once
+
+#include "evaluator.h"
+
+namespace phx {
+
+// bfv_multiply_behz (src/evaluate.cu:440-545): ct1 <- ct1 (x) ct2, size 3
+void bfv_multiply(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, hipStream_t s);
+// relinearize_inplace for BFV (src/evaluate.cu:1342-1374)
+void bfv_relinearize(const Context &ctx, Ciphertext &ct, const KSwitchKey &rlk, hipStream_t s);
+
+}  // namespace phx

@@ -1,0 +1,212 @@
+// bfv.hip -- BFV multiplication with BEHZ base extension and BFV
+// relinearisation (coefficient-form ciphertexts).
+//
+// Pipeline of bfv_multiply_behz (src/evaluate.cu:440-545), per operand
+// polynomial: NTT of the q part; lift q -> Bsk (fastbconv_m_tilde and the
+// Montgomery step sm_mrq, one kernel: the m_tilde residue of a coefficient
+// is the low 32 bits of its 64-bit sum); NTT of the Bsk part. Then the
+// tensor in both bases, the inverse NTTs with the scale by t folded into the
+// last stage, and one kernel for fast_floor and the Shenoy-Kumaresan
+// conversion back to q. Every per-coefficient kernel keeps all of the
+// coefficient's residues in registers (|q| <= 16, |Bsk| <= 18).
+//
+// Relinearisation: BFV key switching equals the CKKS one conjugated by the
+// NTT (modup of c2 = modup of NTT(c2) with the digit's own limbs kept;
+// moddown = INTT of the NTT-form result), so it runs as NTT(c2) -> key
+// switch -> INTT, added to c0 and c1.
+#include "bfv.h"
+
+namespace phx {
+
+namespace {
+constexpr int kThreads = 256;
+constexpr u32 kMaxQ = 16, kMaxBsk = kMaxQ + 2;
+inline dim3 grid2(u64 N, u32 y) { return dim3((u32)((N + kThreads - 1) / kThreads), y); }
+}  // namespace
+
+// BEHZ steps (1)-(2): y_i = [x_i m_tilde Qhat_i^-1]_{q_i};
+// y_p = sum_i y_i [Qhat_i]_p, y_mt = sum_i y_i [Qhat_i]_{2^32};
+// r = -y_mt Q^-1 mod 2^32 in [-2^31, 2^31); out_p = (y_p + r Q) m_tilde^-1 mod p
+// (src/rns.cu:1249-1333)
+__global__ __launch_bounds__(kThreads) void k_behz_lift(const u64 *__restrict__ in, u64 *__restrict__ out,
+                                                        BehzConsts B, const DevModulus *__restrict__ mod, u32 N,
+                                                        u64 in_poly_stride, u64 out_poly_stride) {
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 *x = in + blockIdx.y * in_poly_stride + k;
+    u64 *o = out + blockIdx.y * out_poly_stride + k;
+    u64 y[kMaxQ];
+    u64 ymt = 0;
+#pragma unroll
+    for (u32 i = 0; i < kMaxQ; ++i)
+        if (i < B.k) {
+            y[i] = mul_mod(x[(u64)i * N], B.lift_f[i], mod[i]);
+            ymt += y[i] * B.qhat_mt[i];
+        }
+    const u64 mask = 0xffffffffull;
+    const u64 r = ((ymt & mask) * B.neg_qinv_mt) & mask;
+    const bool neg = r >= (1ull << 31);
+    for (u32 p = 0; p < B.nb; ++p) {
+        const DevModulus m = mod[B.bsk_row0 + p];
+        u128 acc = 0;
+#pragma unroll
+        for (u32 i = 0; i < kMaxQ; ++i)
+            if (i < B.k) acc += (u128)y[i] * B.qhat_bsk[(u64)p * B.k + i];
+        const u64 yp = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
+        const u64 rp = neg ? r + m.value - (1ull << 32) : r;
+        const u64 v = barrett128((u128)rp * B.q_mod_bsk[p] + yp, m.value, m.ratio_lo, m.ratio_hi);
+        o[(u64)p * N] = mul_mod(v, B.mtinv_bsk[p], m);
+    }
+}
+
+// BEHZ steps (7)-(8): fast_floor (src/rns.cu:1379-1407) then fastbconv_sk
+// (1455-1517) with the centred alpha_sk correction of
+// multiply_and_negated_add_rns_poly (src/polymath.cu:606-634)
+__global__ __launch_bounds__(kThreads) void k_behz_floor_sk(const u64 *__restrict__ zq, const u64 *__restrict__ zb,
+                                                            u64 *__restrict__ out, BehzConsts B,
+                                                            const DevModulus *__restrict__ mod, u32 N, u64 zq_stride,
+                                                            u64 zb_stride, u64 out_stride) {
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 *xq = zq + blockIdx.y * zq_stride + k;
+    const u64 *xb = zb + blockIdx.y * zb_stride + k;
+    u64 *o = out + blockIdx.y * out_stride + k;
+    u64 y[kMaxQ];
+#pragma unroll
+    for (u32 i = 0; i < kMaxQ; ++i)
+        if (i < B.k) y[i] = mul_mod(xq[(u64)i * N], B.qhatinv_q[i], mod[i]);
+    // w_p = (z_p - FastBConv_q->p) Q^-1 mod p
+    u64 w[kMaxBsk];
+#pragma unroll
+    for (u32 p = 0; p < kMaxBsk; ++p)
+        if (p < B.nb) {
+            const DevModulus m = mod[B.bsk_row0 + p];
+            u128 acc = 0;
+#pragma unroll
+            for (u32 i = 0; i < kMaxQ; ++i)
+                if (i < B.k) acc += (u128)y[i] * B.qhat_bsk[(u64)p * B.k + i];
+            const u64 c = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
+            w[p] = mul_mod(add_mod(xb[(u64)p * N], c ? m.value - c : 0, m.value), B.qinv_bsk[p], m);
+        }
+    // Shenoy-Kumaresan from B (with m_sk) to q
+    const u32 nB = B.nb - 1;
+    u64 u[kMaxBsk];
+    u128 am = 0;
+    const DevModulus msk = mod[B.bsk_row0 + nB];
+#pragma unroll
+    for (u32 j = 0; j < kMaxBsk; ++j)
+        if (j < nB) {
+            u[j] = mul_mod(w[j], B.bhatinv_b[j], mod[B.bsk_row0 + j]);
+            am += (u128)u[j] * B.bhat_msk[j];
+        }
+    u64 wm = 0;
+#pragma unroll
+    for (u32 p = 0; p < kMaxBsk; ++p)
+        if (p == nB) wm = w[p];
+    const u64 bc = barrett128(am, msk.value, msk.ratio_lo, msk.ratio_hi);
+    const u64 alpha = mul_mod(add_mod(bc, wm ? msk.value - wm : 0, msk.value), B.binv_msk, msk);
+    const bool neg = alpha > (msk.value >> 1);
+    const u64 amag = neg ? msk.value - alpha : alpha;
+#pragma unroll
+    for (u32 i = 0; i < kMaxQ; ++i)
+        if (i < B.k) {
+            const DevModulus m = mod[i];
+            u128 acc = 0;
+#pragma unroll
+            for (u32 j = 0; j < kMaxBsk; ++j)
+                if (j < nB) acc += (u128)u[j] * B.bhat_q[(u64)i * nB + j];
+            const u64 v = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
+            const u64 bq = B.b_mod_q[i];
+            const u64 f = neg ? bq : (bq ? m.value - bq : 0);   // -alpha*B mod q, alpha centred
+            const u64 corr = mul_mod(barrett64(amag, m.value, m.ratio_hi), f, m);
+            o[(u64)i * N] = add_mod(v, corr, m.value);
+        }
+}
+
+__global__ __launch_bounds__(kThreads) void k_add_inplace(u64 *__restrict__ a, const u64 *__restrict__ b,
+                                                          const DevModulus *__restrict__ mod, u32 N, u32 L) {
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u32 limb = blockIdx.y % L;
+    const u64 o = (u64)blockIdx.y * N + k;
+    a[o] = add_mod(a[o], b[o], mod[limb].value);
+}
+
+namespace {
+const BehzConsts &behz_of(const Context &ctx) {
+    const BehzConsts *B = ctx.behz();
+    require(B != nullptr, "the context has no BEHZ tool: scheme must be BFV with mul_tech behz");
+    require(B->k <= kMaxQ && B->nb <= kMaxBsk, "BEHZ multiply supports at most 16 data primes");
+    return *B;
+}
+
+// NTT, lift and NTT of the two polynomials of ct into q and Bsk buffers
+void behz_mul_1(const Context &ctx, const BehzConsts &B, const Ciphertext &ct, u64 *q_out, u64 *b_out,
+                hipStream_t s) {
+    const u64 N = ctx.N(), QN = (u64)B.k * N, BN = (u64)B.nb * N;
+    PHX_HIP_CHECK(hipMemcpyAsync(q_out, ct.data, 2 * QN * 8, hipMemcpyDeviceToDevice, s));
+    ntt_forward(ctx.tables(), limb_map(q_out, 0, B.k, QN), B.k, 2, s);
+    hipLaunchKernelGGL(k_behz_lift, grid2(N, 2), dim3(kThreads), 0, s, (const u64 *)ct.data, b_out, B, ctx.dev_mod(),
+                       (u32)N, QN, BN);
+    PHX_HIP_CHECK(hipGetLastError());
+    ntt_forward(ctx.tables(), limb_map(b_out, B.bsk_row0, B.nb, BN), B.nb, 2, s);
+}
+}  // namespace
+
+void bfv_multiply(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, hipStream_t s) {
+    const BehzConsts &B = behz_of(ctx);
+    require(ct1.chain_index == ct2.chain_index, "encrypted1 and encrypted2 parameter mismatch");
+    require(!ct1.is_ntt_form && !ct2.is_ntt_form, "encrypted1 or encrypted2 cannot be in NTT form");
+    require(ct1.size == 2 && ct2.size == 2, "poly number mismatch");
+    require(ct1.chain_index == 1, "BEHZ multiplication runs at the top data level");
+    const u64 N = ctx.N(), QN = (u64)B.k * N, BN = (u64)B.nb * N;
+    const bool square = &ct1 == &ct2 || ct1.data == ct2.data;
+    u64 *q1 = static_cast<u64 *>(dev_alloc(3 * QN * 8, s));
+    u64 *b1 = static_cast<u64 *>(dev_alloc(3 * BN * 8, s));
+    u64 *q2 = square ? q1 : static_cast<u64 *>(dev_alloc(2 * QN * 8, s));
+    u64 *b2 = square ? b1 : static_cast<u64 *>(dev_alloc(2 * BN * 8, s));
+    behz_mul_1(ctx, B, ct1, q1, b1, s);
+    if (!square) behz_mul_1(ctx, B, ct2, q2, b2, s);
+    // (4) tensor in both bases (tensor_prod_2x2_rns_poly, src/polymath.cu:463-498)
+    tensor_2x2(ctx, B.k, q1, q2, q1, q1 + QN, q1 + 2 * QN, s);
+    tensor_2x2_mod(ctx.dev_mod() + B.bsk_row0, N, B.nb, b1, b2, b1, b1 + BN, b1 + 2 * BN, s);
+    // (5)-(6) inverse NTTs, times t in the last stage
+    LimbMap mq = limb_map(q1, 0, B.k, QN);
+    mq.scale = B.tscale_q;
+    ntt_inverse(ctx.tables(), mq, B.k, 3, s);
+    LimbMap mb = limb_map(b1, B.bsk_row0, B.nb, BN);
+    mb.scale = B.tscale_bsk;
+    ntt_inverse(ctx.tables(), mb, B.nb, 3, s);
+    // (7)-(8) into the resized ciphertext
+    ct1.resize(ctx, ct1.chain_index, 3, s);
+    hipLaunchKernelGGL(k_behz_floor_sk, grid2(N, 3), dim3(kThreads), 0, s, (const u64 *)q1, (const u64 *)b1, ct1.data,
+                       B, ctx.dev_mod(), (u32)N, QN, BN, QN);
+    PHX_HIP_CHECK(hipGetLastError());
+    if (!square) {
+        dev_free(q2, s);
+        dev_free(b2, s);
+    }
+    dev_free(q1, s);
+    dev_free(b1, s);
+}
+
+void bfv_relinearize(const Context &ctx, Ciphertext &ct, const KSwitchKey &rlk, hipStream_t s) {
+    require(ct.size == 3, "destination_size must be 3");
+    require(!ct.is_ntt_form, "BFV encrypted cannot be in NTT form");
+    const u32 L = ct.coeff_modulus_size;
+    const u64 N = ctx.N(), LN = (u64)L * N;
+    u64 *c2 = static_cast<u64 *>(dev_alloc(LN * 8, s));
+    u64 *ks = static_cast<u64 *>(dev_alloc(2 * LN * 8, s));
+    PHX_HIP_CHECK(hipMemcpyAsync(c2, ct.data + 2 * LN, LN * 8, hipMemcpyDeviceToDevice, s));
+    ntt_forward(ctx.tables(), limb_map(c2, 0, L), L, 1, s);
+    keyswitch(ctx, ct.chain_index, c2, rlk, ks, nullptr, nullptr, s);
+    ntt_inverse(ctx.tables(), limb_map(ks, 0, L, LN), L, 2, s);
+    hipLaunchKernelGGL(k_add_inplace, grid2(N, 2 * L), dim3(kThreads), 0, s, ct.data, (const u64 *)ks, ctx.dev_mod(),
+                       (u32)N, L);
+    PHX_HIP_CHECK(hipGetLastError());
+    dev_free(ks, s);
+    dev_free(c2, s);
+    ct.size = 2;
+}
+
+}  // namespace phx

@@ -9,6 +9,7 @@
 #include "tune.h"
 #include "ckks.h"
 #include "serve.h"
+#include "bfv.h"
 
 using namespace phx;
 
@@ -155,7 +156,16 @@ int phx_ciphertext_resize(const phx_context *ctx, phx_ciphertext *ct, size_t cha
         nn(ct, "ciphertext");
         require(chain <= ctx->ctx->size_Q(), "invalid chain index");
         require(size >= 1 && size <= 3, "ciphertext size must be 1..3");
+        // a new ciphertext takes its scheme's form: BFV coefficient, CKKS NTT
+        if (!ct->ct.data) ct->ct.is_ntt_form = ctx->ctx->params().scheme != kSchemeBFV;
         ct->ct.resize(*ctx->ctx, (u32)chain, (u32)size, S(s));
+    });
+}
+
+int phx_ciphertext_set_ntt_form(phx_ciphertext *ct, int is_ntt_form) {
+    return guard([&] {
+        nn(ct, "ciphertext");
+        ct->ct.is_ntt_form = is_ntt_form != 0;
     });
 }
 
@@ -439,6 +449,12 @@ int phx_multiply_inplace(const phx_context *ctx, phx_ciphertext *ct1, const phx_
         nn(ct2, "encrypted2");
         Ciphertext &a = ct1->ct;
         const Ciphertext &b = ct2->ct;
+        if (ctx->ctx->params().scheme == kSchemeBFV) {
+            // bfv_multiply (src/evaluate.cu:1046-1048), BEHZ
+            require(ctx->ctx->params().mul_tech == kMulBEHZ, "unsupported mul tech for BFV multiply");
+            bfv_multiply(*ctx->ctx, a, b, S(s));
+            return;
+        }
         require(a.chain_index == b.chain_index, "encrypted1 and encrypted2 parameter mismatch");
         require(a.is_ntt_form && b.is_ntt_form, "encrypted1 and encrypted2 must be in NTT form");
         require(a.size == 2 && b.size == 2, "only size-2 operands are supported");

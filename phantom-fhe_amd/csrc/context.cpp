@@ -137,24 +137,52 @@ Context::Context(const Params &p) : params_(p) {
         for (size_t j = i + 1; j < QP; ++j) require(p.coeff_modulus[i] != p.coeff_modulus[j], "moduli must be distinct");
     pool_init();
 
+    // ---- BEHZ bases (BFV multiplication at the top data level, src/rns.cu:392-423):
+    // B has size_Q 61-bit primes, one more when 32 + bits(t) + bits(Q) >= 61 (size_Q + 1);
+    // m_sk is the first prime of the same search, Bsk = B then m_sk
+    if (p.scheme == kSchemeBFV && p.mul_tech == kMulBEHZ) {
+        require(p.plain_modulus > 1, "plain_modulus is not set");
+        std::vector<u64> Qw{1};
+        for (u32 i = 0; i < size_Q_; ++i) {
+            u128 carry = 0;
+            for (auto &w : Qw) {
+                const u128 v = (u128)w * p.coeff_modulus[i] + carry;
+                w = (u64)v;
+                carry = v >> 64;
+            }
+            if (carry) Qw.push_back((u64)carry);
+        }
+        const int qbits = 64 * ((int)Qw.size() - 1) + hm::bit_count(Qw.back());
+        u32 size_B = size_Q_;
+        if (32 + hm::bit_count(p.plain_modulus) + qbits >= 61 * (int)size_Q_ + 61) ++size_B;
+        const std::vector<u64> pr = hm::get_primes(N_, 61, size_B + 1);
+        bsk_.assign(pr.begin() + 1, pr.end());
+        bsk_.push_back(pr[0]);
+        for (u64 b : bsk_)
+            for (u64 q : p.coeff_modulus) require(b != q, "coefficient modulus collides with the BEHZ base");
+    }
+    std::vector<u64> rows_p = p.coeff_modulus;
+    rows_p.insert(rows_p.end(), bsk_.begin(), bsk_.end());
+    const size_t R = rows_p.size();
+
     // ---- NTT tables (host, one thread per prime) ----
-    host_tables_.resize(QP);
+    host_tables_.resize(R);
     {
         unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 16));
         std::vector<std::thread> th;
         for (unsigned w = 0; w < nt; ++w)
             th.emplace_back([&, w] {
-                for (size_t i = w; i < QP; i += nt) host_tables_[i] = hm::make_ntt_table(p.coeff_modulus[i], (int)logN_);
+                for (size_t i = w; i < R; i += nt) host_tables_[i] = hm::make_ntt_table(rows_p[i], (int)logN_);
             });
         for (auto &t : th) t.join();
     }
     // Rows with q < 2^50 run the NTT on the FP64 pipe (exact, see ntt_arith.h)
     // unless the ntt_fp64 switch is off (then every row uses 64-bit integers).
     const bool int_only = tune_value("ntt_fp64") == 0;
-    const size_t twb = QP * N_ * sizeof(TwPair);
-    const size_t twdb = QP * N_ * sizeof(double);
-    const size_t modb = QP * sizeof(DevModulus);
-    const size_t icb = QP * 4 * sizeof(u64);
+    const size_t twb = R * N_ * sizeof(TwPair);
+    const size_t twdb = R * N_ * sizeof(double);
+    const size_t modb = R * sizeof(DevModulus);
+    const size_t icb = R * 4 * sizeof(u64);
     std::vector<uint8_t> h(2 * twb + 2 * twdb + modb + icb);
     TwPair *htw = reinterpret_cast<TwPair *>(h.data());
     TwPair *hitw = reinterpret_cast<TwPair *>(h.data() + twb);
@@ -162,7 +190,7 @@ Context::Context(const Params &p) : params_(p) {
     double *hitwd = reinterpret_cast<double *>(h.data() + 2 * twb + twdb);
     DevModulus *hmod = reinterpret_cast<DevModulus *>(h.data() + 2 * twb + 2 * twdb);
     u64 *hic = reinterpret_cast<u64 *>(h.data() + 2 * twb + 2 * twdb + modb);
-    for (size_t r = 0; r < QP; ++r) {
+    for (size_t r = 0; r < R; ++r) {
         const auto &t = host_tables_[r];
         for (u64 k = 0; k < N_; ++k) {
             htw[r * N_ + k] = {t.fw[k], t.fws[k]};
@@ -329,6 +357,68 @@ Context::Context(const Params &p) : params_(p) {
         bp.push_back(hm::shoup(v, qp[i]));
     }
     const size_t bp_off = cb.push64(bp);
+    // BEHZ constants (src/rns.cu:424-571)
+    struct BehzOffs {
+        size_t lift_f, qhat_bsk, qhat_mt, q_mod_bsk, mtinv_bsk, qhatinv_q, qinv_bsk, bhatinv_b, bhat_msk, bhat_q,
+            b_mod_q, tscale_q, tscale_bsk;
+    } bo{};
+    if (!bsk_.empty()) {
+        const u32 k = size_Q_, nb = (u32)bsk_.size(), nB = nb - 1;
+        const u64 mt = 1ull << 32, t = p.plain_modulus, msk = bsk_[nB];
+        auto qhat_mod = [&](u32 i, u64 m) { return hm::punctured_mod(qp, k, i, m); };
+        auto bhat_mod = [&](u32 j, u64 m) { return hm::punctured_mod(bsk_.data(), nB, j, m); };
+        std::vector<u64> lift_f(k), qhat_mt(k), qhatinv(k), b_mod_q(k), bhat_q((size_t)k * nB);
+        for (u32 i = 0; i < k; ++i) {
+            const u64 hinv = hm::invmod(qhat_mod(i, qp[i]), qp[i]);
+            lift_f[i] = hm::mulmod(mt % qp[i], hinv, qp[i]);
+            qhat_mt[i] = qhat_mod(i, mt);
+            qhatinv[i] = hinv;
+            b_mod_q[i] = hm::product_mod(bsk_.data(), nB, qp[i]);
+            for (u32 j = 0; j < nB; ++j) bhat_q[(size_t)i * nB + j] = bhat_mod(j, qp[i]);
+        }
+        std::vector<u64> qhat_bsk((size_t)nb * k), q_mod_bsk(nb), mtinv(nb), qinv(nb);
+        for (u32 o = 0; o < nb; ++o) {
+            const u64 pm = bsk_[o];
+            for (u32 i = 0; i < k; ++i) qhat_bsk[(size_t)o * k + i] = qhat_mod(i, pm);
+            q_mod_bsk[o] = hm::product_mod(qp, k, pm);
+            mtinv[o] = hm::invmod(mt % pm, pm);
+            qinv[o] = hm::invmod(q_mod_bsk[o], pm);
+        }
+        std::vector<u64> bhatinv(nB), bhat_msk(nB);
+        for (u32 j = 0; j < nB; ++j) {
+            bhatinv[j] = hm::invmod(bhat_mod(j, bsk_[j]), bsk_[j]);
+            bhat_msk[j] = bhat_mod(j, msk);
+        }
+        // -Q^-1 mod 2^32 (Q odd): Newton iteration for the inverse
+        const u64 qm = hm::product_mod(qp, k, mt);
+        u64 inv = 1;
+        for (int it = 0; it < 6; ++it) inv = (inv * (2 - qm * inv)) & (mt - 1);
+        behz_.neg_qinv_mt = (mt - inv) & (mt - 1);
+        behz_.binv_msk = hm::invmod(hm::product_mod(bsk_.data(), nB, msk), msk);
+        auto tscale = [&](size_t row) {
+            const auto &tb = host_tables_[row];
+            const u64 tt = t % tb.q;
+            const u64 s0 = hm::mulmod(tb.ninv, tt, tb.q);
+            const u64 s1 = hm::mulmod(hm::mulmod(tb.ninv, tb.iw[1], tb.q), tt, tb.q);
+            return std::vector<u64>{s0, hm::shoup(s0, tb.q), s1, hm::shoup(s1, tb.q)};
+        };
+        std::vector<u64> tsq, tsb;
+        for (u32 i = 0; i < k; ++i) {
+            auto v = tscale(i);
+            tsq.insert(tsq.end(), v.begin(), v.end());
+        }
+        for (u32 o = 0; o < nb; ++o) {
+            auto v = tscale(QP + o);
+            tsb.insert(tsb.end(), v.begin(), v.end());
+        }
+        bo = {cb.push64(lift_f), cb.push64(qhat_bsk), cb.push64(qhat_mt), cb.push64(q_mod_bsk), cb.push64(mtinv),
+              cb.push64(qhatinv), cb.push64(qinv), cb.push64(bhatinv), cb.push64(bhat_msk), cb.push64(bhat_q),
+              cb.push64(b_mod_q), cb.push64(tsq), cb.push64(tsb)};
+        behz_.k = k;
+        behz_.nb = nb;
+        behz_.bsk_row0 = (u32)QP;
+        behz_.t = t;
+    }
     cb.align(256);
     PHX_HIP_CHECK(hipMalloc(&cbuf_, cb.buf.size()));
     PHX_HIP_CHECK(hipMemcpy(cbuf_, cb.buf.data(), cb.buf.size(), hipMemcpyHostToDevice));
@@ -343,6 +433,22 @@ Context::Context(const Params &p) : params_(p) {
         L.rescale_qinv = reinterpret_cast<const u64 *>(cbb + offs[c].rescale);
     }
     bigP_mod_q_ = reinterpret_cast<u64 *>(cbb + bp_off);
+    if (behz_.k) {
+        auto at = [&](size_t off) { return reinterpret_cast<const u64 *>(cbb + off); };
+        behz_.lift_f = at(bo.lift_f);
+        behz_.qhat_bsk = at(bo.qhat_bsk);
+        behz_.qhat_mt = at(bo.qhat_mt);
+        behz_.q_mod_bsk = at(bo.q_mod_bsk);
+        behz_.mtinv_bsk = at(bo.mtinv_bsk);
+        behz_.qhatinv_q = at(bo.qhatinv_q);
+        behz_.qinv_bsk = at(bo.qinv_bsk);
+        behz_.bhatinv_b = at(bo.bhatinv_b);
+        behz_.bhat_msk = at(bo.bhat_msk);
+        behz_.bhat_q = at(bo.bhat_q);
+        behz_.b_mod_q = at(bo.b_mod_q);
+        behz_.tscale_q = at(bo.tscale_q);
+        behz_.tscale_bsk = at(bo.tscale_bsk);
+    }
     levels_[0].chain_index = 0;
     levels_[0].size_Ql = size_Q_;
     levels_[0].size_QlP = size_QP_;

@@ -68,6 +68,32 @@ struct LevelConsts {
     BconvDesc h_moddown_bconv{};
 };
 
+// BEHZ constants of the top data level (BFV multiply; reference DRNSTool ctor,
+// src/rns.cu:392-571): q = the size_Q data primes, Bsk = B (61-bit primes) then
+// m_sk, m_tilde = 2^32. Device arrays live in the context constant buffer; the
+// Bsk primes' NTT tables are table rows bsk_row0 .. bsk_row0 + nb - 1.
+struct BehzConsts {
+    u32 k = 0, nb = 0;       // |q|, |Bsk|
+    u32 bsk_row0 = 0;
+    u32 pad = 0;
+    u64 t = 0;
+    u64 neg_qinv_mt = 0;     // -Q^-1 mod 2^32
+    u64 binv_msk = 0;        // B^-1 mod m_sk
+    const u64 *lift_f;       // [k]      m_tilde * Qhat_i^-1 mod q_i
+    const u64 *qhat_bsk;     // [nb][k]  Qhat_i mod p
+    const u64 *qhat_mt;      // [k]      Qhat_i mod 2^32
+    const u64 *q_mod_bsk;    // [nb]     Q mod p
+    const u64 *mtinv_bsk;    // [nb]     m_tilde^-1 mod p
+    const u64 *qhatinv_q;    // [k]      Qhat_i^-1 mod q_i
+    const u64 *qinv_bsk;     // [nb]     Q^-1 mod p
+    const u64 *bhatinv_b;    // [nb-1]   Bhat_j^-1 mod b_j
+    const u64 *bhat_msk;     // [nb-1]   Bhat_j mod m_sk
+    const u64 *bhat_q;       // [k][nb-1] Bhat_j mod q_i
+    const u64 *b_mod_q;      // [k]      B mod q_i
+    const u64 *tscale_q;     // [k][4]   final INTT stage constants times t
+    const u64 *tscale_bsk;   // [nb][4]
+};
+
 class Context {
 public:
     explicit Context(const Params &p);
@@ -94,6 +120,10 @@ public:
     const u64 *bigP_mod_q() const { return bigP_mod_q_; }
     const hm::NttTable &host_table(u32 row) const { return host_tables_[row]; }
 
+    // BFV with BEHZ multiplication: its constants (else null)
+    const BehzConsts *behz() const { return behz_.k ? &behz_ : nullptr; }
+    const std::vector<u64> &bsk_primes() const { return bsk_; }
+
     // Galois permutation tables (NTT domain), created on first use
     const u32 *galois_perm(u32 elt) const;
     const std::vector<u32> &galois_elts() const { return params_.galois_elts; }
@@ -109,6 +139,8 @@ private:
     u64 *bigP_mod_q_ = nullptr;
     std::vector<LevelConsts> levels_;
     mutable std::vector<std::pair<u32, u32 *>> galois_tables_;
+    BehzConsts behz_{};
+    std::vector<u64> bsk_;
 };
 
 // stream-ordered device allocation (pooled, like the reference's cuda_auto_ptr)

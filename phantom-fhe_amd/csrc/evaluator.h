@@ -68,6 +68,9 @@ void bconv_run(const Context &ctx, const std::vector<BconvDesc> &descs_host, con
 
 // ---- hot path ----
 void tensor_2x2(const Context &ctx, u32 Ql, const u64 *a, const u64 *b, u64 *d0, u64 *d1, u64 *d2, hipStream_t s);
+// the same over L limbs whose moduli start at mod
+void tensor_2x2_mod(const DevModulus *mod, u64 N, u32 L, const u64 *a, const u64 *b, u64 *d0, u64 *d1, u64 *d2,
+                    hipStream_t s);
 // ct[0..1] (Ql limbs each) = base0/base1 + keyswitch(c2); base may be null (zero)
 void keyswitch(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey &key, u64 *ct,
                const u64 *base0, const u64 *base1, hipStream_t s);

@@ -8,6 +8,7 @@
 
 #include "evaluator.h"
 #include "tune.h"
+#include "bfv.h"
 
 namespace phx {
 
@@ -475,9 +476,13 @@ void Ciphertext::release(hipStream_t s) {
 }
 
 void tensor_2x2(const Context &ctx, u32 Ql, const u64 *a, const u64 *b, u64 *d0, u64 *d1, u64 *d2, hipStream_t s) {
-    const u64 N = ctx.N();
-    hipLaunchKernelGGL(k_tensor, grid_for(N, Ql), dim3(kThreads), 0, s, a, b, d0, d1, d2, ctx.dev_mod(), (u32)N,
-                       (u64)Ql * N);
+    tensor_2x2_mod(ctx.dev_mod(), ctx.N(), Ql, a, b, d0, d1, d2, s);
+}
+
+void tensor_2x2_mod(const DevModulus *mod, u64 N, u32 L, const u64 *a, const u64 *b, u64 *d0, u64 *d1, u64 *d2,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_tensor, grid_for(N, L), dim3(kThreads), 0, s, a, b, d0, d1, d2, mod, (u32)N, (u64)L * N);
+    PHX_HIP_CHECK(hipGetLastError());
 }
 
 // DRNSTool::modup for CKKS (src/rns_bconv.cu:530-628); copy_own = false leaves
@@ -565,7 +570,14 @@ void keyswitch(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey &k
 // multiply_and_relin_inplace for CKKS (src/evaluate.cu:1061-1104)
 void multiply_and_relin(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, const KSwitchKey &rlk,
                         hipStream_t s) {
-    require(ctx.params().scheme == kSchemeCKKS, "multiply_and_relin: only CKKS is on this path");
+    if (ctx.params().scheme == kSchemeBFV) {
+        // BEHZ multiply, then relinearize (src/evaluate.cu:1087-1093)
+        require(ctx.params().mul_tech == kMulBEHZ, "unsupported mul tech in BFV mul&relin");
+        bfv_multiply(ctx, ct1, ct2, s);
+        bfv_relinearize(ctx, ct1, rlk, s);
+        return;
+    }
+    require(ctx.params().scheme == kSchemeCKKS, "multiply_and_relin: unsupported scheme");
     require(ct1.chain_index == ct2.chain_index, "encrypted1 and encrypted2 parameter mismatch");
     require(ct1.is_ntt_form == ct2.is_ntt_form, "NTT form mismatch");
     require(ct1.is_ntt_form, "encrypted1 and encrypted2 must be in NTT form");
@@ -581,6 +593,7 @@ void multiply_and_relin(const Context &ctx, Ciphertext &ct1, const Ciphertext &c
 
 // relinearize_inplace (src/evaluate.cu:1342-1374)
 void relinearize(const Context &ctx, Ciphertext &ct, const KSwitchKey &rlk, hipStream_t s) {
+    if (ctx.params().scheme == kSchemeBFV) return bfv_relinearize(ctx, ct, rlk, s);
     require(ct.size == 3, "destination_size must be 3");
     require(ct.is_ntt_form, "CKKS encrypted must be in NTT form");
     const u64 LN = (u64)ct.coeff_modulus_size * ctx.N();

@@ -76,6 +76,7 @@ _SIGS = {
     "phx_galois_key_destroy": (ctypes.c_int, [_vp]),
     "phx_encrypt_symmetric": (ctypes.c_int, [_vp, _vp, _u64p, _sz, ctypes.c_uint64, _vp, _vp]),
     "phx_decrypt": (ctypes.c_int, [_vp, _vp, _vp, _u64p, _vp]),
+    "phx_ciphertext_set_ntt_form": (ctypes.c_int, [_vp, ctypes.c_int]),
     "phx_server_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _sz, _sz,
                                          _sz, _sz, _vp, ctypes.POINTER(_vp)]),
     "phx_server_now": (ctypes.c_int, [_vp, _dp]),
@@ -277,6 +278,12 @@ class ciphertext:
 
     def set_scale(self, v):
         _check(lib().phx_ciphertext_set_scale(self._h, v))
+
+    def set_ntt_form(self, v):
+        _check(lib().phx_ciphertext_set_ntt_form(self._h, int(bool(v))))
+
+    def is_ntt_form(self):
+        return bool(self.info()["is_ntt_form"])
 
     def upload(self, arr, stream=None):
         a = np.ascontiguousarray(arr, np.uint64)

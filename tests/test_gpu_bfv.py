@@ -1,0 +1,179 @@
+"""BFV on the GPU: BEHZ multiply and relinearisation bit-exact against the
+oracle (coefficient-form ciphertexts), the config of 64 ciphertexts over
+N in {2^13, 2^14, 2^15} with 4/8/16 limbs of 54-bit primes and t = 65537, and
+a semantic check: encrypt m1, m2 -> multiply+relin -> decrypt = m1 m2 mod
+(X^N + 1, t)."""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import pyphantom as ph
+from bigint_ref import crt
+from gpu_helpers import rand_ct, rand_key, say
+
+pytestmark = pytest.mark.gpu
+
+T = 65537
+LIMBS = {1 << 13: 4, 1 << 14: 8, 1 << 15: 16}
+
+
+def make_bfv(N, k):
+    mods = ph.coeff_modulus_create(N, [54] * k + [60])
+    parms = ph.EncryptionParameters(ph.SCHEME_BFV)
+    parms.set_poly_modulus_degree(N)
+    parms.set_coeff_modulus(mods)
+    parms.set_special_modulus_size(1)
+    parms.set_plain_modulus(T)
+    parms.set_mul_tech(ph.MUL_TECH_BEHZ)
+    ctx = ph.context(parms)
+    octx = ob.Ctx(N.bit_length() - 1, mods, 1)
+    return ctx, octx, ob.Behz(octx, T), mods
+
+
+def upload_bfv(ctx, arr):
+    ct = ph.ciphertext(ctx, 1, arr.shape[0])
+    assert not ct.is_ntt_form()
+    ct.upload(arr)
+    return ct
+
+
+@pytest.fixture(scope="module")
+def small():
+    ob.set_threads(8)
+    return make_bfv(1 << 13, 4)
+
+
+def test_bfv_multiply_bit_exact(small):
+    ctx, octx, beh, mods = small
+    N = ctx.N
+    say("case: bfv multiply N=2^13 k=4")
+    rng = np.random.default_rng(1)
+    a, b = rand_ct(rng, mods[:4], N), rand_ct(rng, mods[:4], N)
+    c1, c2 = upload_bfv(ctx, a), upload_bfv(ctx, b)
+    ph.multiply_inplace(ctx, c1, c2)
+    ph.device_synchronize()
+    assert c1.size() == 3
+    assert np.array_equal(c1.download(N), beh.multiply(a, b))
+    # squaring (both operands the same ciphertext)
+    c3 = upload_bfv(ctx, a)
+    ph.multiply_inplace(ctx, c3, c3)
+    assert np.array_equal(c3.download(N), beh.multiply(a, a))
+
+
+def test_bfv_relinearize_bit_exact(small):
+    ctx, octx, beh, mods = small
+    N = ctx.N
+    say("case: bfv relinearize N=2^13 k=4")
+    rng = np.random.default_rng(2)
+    key = rand_key(rng, mods, N, ctx.dnum)
+    rk = ph.relin_key(ctx, host=key)
+    x = rand_ct(rng, mods[:4], N, size=3)
+    c = upload_bfv(ctx, x)
+    ph.relinearize_inplace(ctx, c, rk)
+    assert c.size() == 2
+    assert np.array_equal(c.download(N), ob.bfv_relinearize(octx, x, key))
+
+
+def test_bfv_rejects_ntt_form_and_wrong_scheme(small):
+    ctx, octx, beh, mods = small
+    rng = np.random.default_rng(3)
+    c = upload_bfv(ctx, rand_ct(rng, mods[:4], ctx.N))
+    c.set_ntt_form(True)
+    with pytest.raises(ValueError):
+        ph.multiply_inplace(ctx, c, c)
+
+
+def test_config4_batch_bit_exact():
+    """64 ciphertexts, N uniform in {2^13, 2^14, 2^15} (4/8/16 limbs), multiply
+    + relinearise each; items of every ring degree checked against the oracle."""
+    ob.set_threads(16)
+    say("case: config4 64 bfv ciphertexts")
+    rng = np.random.default_rng(44)
+    ctxs = {N: make_bfv(N, k) for N, k in LIMBS.items()}
+    keys = {N: rand_key(np.random.default_rng(N), v[3], N, v[0].dnum) for N, v in ctxs.items()}
+    rks = {N: ph.relin_key(v[0], host=keys[N]) for N, v in ctxs.items()}
+    Ns = rng.choice(list(LIMBS), 64)
+    items = []
+    for i, N in enumerate(Ns):
+        N = int(N)
+        ctx, octx, beh, mods = ctxs[N]
+        k = LIMBS[N]
+        a = rand_ct(np.random.default_rng(1000 + i), mods[:k], N)
+        b = rand_ct(np.random.default_rng(2000 + i), mods[:k], N)
+        c1, c2 = upload_bfv(ctx, a), upload_bfv(ctx, b)
+        ph.multiply_and_relin_inplace(ctx, c1, c2, rks[N])
+        items.append((i, N, c1))
+    ph.device_synchronize()
+    checked = set()
+    for i, N, c in items:
+        if N in checked and i not in (0, 63):
+            continue
+        checked.add(N)
+        ctx, octx, beh, mods = ctxs[N]
+        k = LIMBS[N]
+        a = rand_ct(np.random.default_rng(1000 + i), mods[:k], N)
+        b = rand_ct(np.random.default_rng(2000 + i), mods[:k], N)
+        want = ob.bfv_relinearize(octx, beh.multiply(a, b), keys[N])
+        assert np.array_equal(c.download(N), want), (i, N)
+    assert checked == set(LIMBS)
+
+
+def negacyclic_mod_t(a, b, N, octx_t):
+    fa = octx_t.ntt_forward(np.array(a, np.uint64)[None], 0)
+    fb = octx_t.ntt_forward(np.array(b, np.uint64)[None], 0)
+    prod = (fa.astype(object) * fb.astype(object) % T).astype(np.uint64)
+    return octx_t.ntt_inverse(prod, 0)[0]
+
+
+def poly_mul_rns(octx, x, y, L):
+    """negacyclic products limb by limb through the oracle NTT"""
+    fx = octx.ntt_forward(x, 0)
+    fy = octx.ntt_forward(y, 0)
+    qs = np.array(octx.moduli[:L], dtype=object)[:, None]
+    return octx.ntt_inverse(((fx.astype(object) * fy.astype(object)) % qs).astype(np.uint64), 0)
+
+
+def test_bfv_encrypt_multiply_relin_decrypt():
+    N, k = 1 << 13, 4
+    say("case: bfv semantic multiply")
+    ctx, octx, beh, mods = make_bfv(N, k)
+    qs = mods[:k]
+    Q = 1
+    for q in qs:
+        Q *= q
+    delta = Q // T
+    rng = np.random.default_rng(7)
+    s = rng.integers(-1, 2, N)
+    res = lambda v, ms: np.stack([np.array([int(x) % q for x in v], np.uint64) for q in ms])
+    s_all = res(s, mods)
+
+    def encrypt(m):
+        a = np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs])
+        e = res(rng.integers(-4, 5, N), qs)
+        dm = np.stack([np.array([(delta % q) * int(v) % q for v in m], np.uint64) for q in qs])
+        as_ = poly_mul_rns(octx, a, s_all[:k], k)
+        c0 = np.stack([(np.array(dm[i], dtype=object) + e[i].astype(object) - as_[i].astype(object)) % qs[i]
+                       for i in range(k)]).astype(np.uint64)
+        return np.stack([c0, a])
+
+    m1, m2 = rng.integers(0, T, N), rng.integers(0, T, N)
+    # relinearisation key for s^2 from explicit samples (NTT form, QP limbs)
+    s_ntt = octx.ntt_forward(s_all, 0)
+    s2 = poly_mul_rns(octx, s_all, s_all, len(mods))
+    s2_ntt = octx.ntt_forward(s2, 0)
+    dnum = ctx.dnum
+    ka = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in mods]) for _ in range(dnum)])
+    ke = np.stack([res(rng.integers(-4, 5, N), mods) for _ in range(dnum)])
+    rk = ph.relin_key(ctx, host=octx.make_kswitch_key(s_ntt, s2_ntt, ka, ke))
+    c1, c2 = upload_bfv(ctx, encrypt(m1)), upload_bfv(ctx, encrypt(m2))
+    ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
+    out = c1.download(N)
+    v = (out[0].astype(object) + poly_mul_rns(octx, out[1], s_all[:k], k).astype(object))
+    v = v % np.array(qs, dtype=object)[:, None]
+    dec = []
+    for n in range(N):
+        x = crt([int(v[i, n]) for i in range(k)], qs)[0]
+        dec.append(((x * T + Q // 2) // Q) % T)
+    octx_t = ob.Ctx(N.bit_length() - 1, [T, mods[0]], 1)
+    want = negacyclic_mod_t([int(x) for x in m1], [int(x) for x in m2], N, octx_t)
+    assert [int(x) for x in want] == dec
