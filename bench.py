@@ -298,6 +298,8 @@ def main():
     ap.add_argument("--ntt-polys", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ntt-only", action="store_true",
+                    help="only the forward NTT over 47 x 16 limbs and the copy kernel (for counter runs)")
     ap.add_argument("--serve-qps", type=float, default=400.0)
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
@@ -317,6 +319,14 @@ def main():
     N = 1 << 16
     ctx, mods = make_context(ph, N, CONFIG2_BITS, 12)
     stream = ph.stream_create()
+    if args.ntt_only:
+        L = ctx.size_Q + ctx.size_P
+        ntt_ms, ntt_min = bench_ntt(ph, ctx, N, L, args.ntt_polys, 10, stream)
+        copy_gbs = bench_copy(ph, 1 << 27, 10, stream)
+        print(json.dumps({"ntt_limbs": L * args.ntt_polys, "ntt_ms": ntt_ms, "ntt_bytes_algorithmic":
+                          2 * N * 8 * L * args.ntt_polys, "copy_bytes_per_launch": 2 * (1 << 27) * 8,
+                          "copy_GBps": copy_gbs}), flush=True)
+        return
     log(f"bench: HMult+relin N=2^16 L=35 P=12 dnum={ctx.dnum} batch={args.batch} steps={args.steps}")
     ms_plain, wall = bench_hmult(ph, ctx, mods, N, args.batch, args.steps, args.warmup, stream)
     ops = args.batch * args.steps
