@@ -11,6 +11,7 @@
  */
 #include "oracle.h"
 
+#include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1009,5 +1010,190 @@ int orc_bfv_relinearize(const orc_ctx *c, const u64 *ct3, const u64 *key, size_t
         }
     }
     free(qlp); free(rows); free(t); free(cx); free(delta);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* BFV multiply with HPS (mul_tech hps)                                */
+/* ------------------------------------------------------------------ */
+
+/* little-endian multiword helpers for the HPS constants */
+#define ORC_BIG_MAX 40
+typedef struct { u64 w[ORC_BIG_MAX]; size_t n; } orc_big;
+
+static void big_set(orc_big *a, u64 v) { memset(a, 0, sizeof(*a)); a->w[0] = v; a->n = 1; }
+static void big_mul(orc_big *a, u64 m) {
+    u128 carry = 0;
+    for (size_t i = 0; i < a->n; i++) {
+        u128 v = (u128)a->w[i] * m + carry;
+        a->w[i] = (u64)v;
+        carry = v >> 64;
+    }
+    if (carry) a->w[a->n++] = (u64)carry;
+}
+static u64 big_mod(const orc_big *a, u64 m) {
+    u128 r = 0;
+    for (size_t i = a->n; i-- > 0;) r = ((r << 64) | a->w[i]) % m;
+    return (u64)r;
+}
+static void big_div(orc_big *a, u64 m) {   /* a = floor(a / m) */
+    u128 r = 0;
+    for (size_t i = a->n; i-- > 0;) {
+        u128 cur = (r << 64) | a->w[i];
+        a->w[i] = (u64)(cur / m);
+        r = cur % m;
+    }
+    while (a->n > 1 && a->w[a->n - 1] == 0) a->n--;
+}
+
+/* HPS constants of the top data level (src/rns.cu:686-783 and the
+ * BaseConverter tables, src/host/rns.cu:438-497): R = size_Q + 1 primes below
+ * the smallest q (get_primes_below, src/host/numth.cu:235-262). */
+struct orc_hps {
+    size_t k, nr;
+    u64 t;
+    u64 *r;                        /* [nr] */
+    u64 *rp, *rps, *irp, *irps, *ninv;
+    double *frac;                  /* [k] tRSHatInvModsDivsFrac */
+    u64 *divs_mod_r;               /* [nr][k + 1] tRSHatInvModsDivsModr */
+};
+
+orc_hps *orc_hps_create(const orc_ctx *c, u64 t) {
+    orc_hps *h = (orc_hps *)calloc(1, sizeof(orc_hps));
+    size_t k = c->size_Q, nr = k + 1, N = c->N;
+    h->k = k; h->nr = nr; h->t = t;
+    u64 qmin = c->q[0];
+    for (size_t i = 1; i < k; i++) if (c->q[i] < qmin) qmin = c->q[i];
+    h->r = (u64 *)malloc(nr * 8);
+    int bits = bit_count(qmin);
+    u64 lower = (u64)1 << (bits - 1), v = qmin - 2 * N;
+    size_t m = 0;
+    while (m < nr && v > lower) {
+        if (orc_is_prime(v)) h->r[m++] = v;
+        v -= 2 * N;
+    }
+    h->rp = (u64 *)malloc(nr * N * 8); h->rps = (u64 *)malloc(nr * N * 8);
+    h->irp = (u64 *)malloc(nr * N * 8); h->irps = (u64 *)malloc(nr * N * 8);
+    h->ninv = (u64 *)malloc(nr * 8);
+    for (size_t j = 0; j < nr; j++)
+        orc_ntt_tables(h->r[j], c->logN, h->rp + j * N, h->rps + j * N, h->irp + j * N, h->irps + j * N, h->ninv + j);
+    /* S = Q R; theta_i = t R [(S/s_i)^-1]_{s_i} */
+    size_t ns = k + nr;
+    u64 *s = (u64 *)malloc(ns * 8);
+    memcpy(s, c->q, k * 8);
+    memcpy(s + k, h->r, nr * 8);
+    h->frac = (double *)malloc(k * sizeof(double));
+    h->divs_mod_r = (u64 *)malloc(nr * (k + 1) * 8);
+    for (size_t i = 0; i < ns; i++) {
+        u64 shinv = invmod(punctured_mod(s, ns, i, s[i]), s[i]);
+        orc_big th;
+        big_set(&th, t);
+        for (size_t j = 0; j < nr; j++) big_mul(&th, h->r[j]);
+        big_mul(&th, shinv);
+        if (i < k) h->frac[i] = (double)big_mod(&th, s[i]) / (double)s[i];
+        big_div(&th, s[i]);
+        for (size_t j = 0; j < nr; j++) {
+            if (i < k) h->divs_mod_r[j * (k + 1) + i] = big_mod(&th, h->r[j]);
+            else if (i == k + j) h->divs_mod_r[j * (k + 1) + k] = big_mod(&th, h->r[j]);
+        }
+    }
+    free(s);
+    return h;
+}
+
+void orc_hps_destroy(orc_hps *h) {
+    if (!h) return;
+    free(h->r); free(h->rp); free(h->rps); free(h->irp); free(h->irps); free(h->ninv);
+    free(h->frac); free(h->divs_mod_r);
+    free(h);
+}
+
+size_t orc_hps_r_size(const orc_hps *h) { return h->nr; }
+void orc_hps_r(const orc_hps *h, u64 *out) { memcpy(out, h->r, h->nr * 8); }
+
+/* DBaseConverter::bConv_HPS (src/rns_bconv.cu:277-372): y_i = [x_i ihat_i^-1]_{i};
+ * out_p = sum_i y_i [ihat_i]_p - v [I]_p with v = llround(sum_i y_i / i_i), the
+ * FP64 sum accumulated in input order with fused multiply-adds */
+static void bconv_hps(size_t N, const u64 *ib, size_t ni, const u64 *ob, size_t no, const u64 *in, u64 *out) {
+    u64 *y = (u64 *)malloc(ni * N * 8);
+    for (size_t i = 0; i < ni; i++) {
+        u64 hinv = invmod(punctured_mod(ib, ni, i, ib[i]), ib[i]);
+        for (size_t n = 0; n < N; n++) y[i * N + n] = mulmod(in[i * N + n], hinv, ib[i]);
+    }
+    for (size_t o = 0; o < no; o++) {
+        u64 p = ob[o], imodp = prod_mod(ib, ni, p);
+        for (size_t n = 0; n < N; n++) {
+            u128 acc = 0;
+            double frac = 0.0;
+            for (size_t i = 0; i < ni; i++) {
+                acc += (u128)y[i * N + n] * punctured_mod(ib, ni, i, p);
+                frac = fma((double)y[i * N + n], 1.0 / (double)ib[i], frac);
+            }
+            long long v = llround(frac);
+            u64 sub = mulmod((u64)v % p, imodp, p);
+            out[o * N + n] = submod((u64)(acc % p), sub, p);
+        }
+    }
+    free(y);
+}
+
+static void r_ntt(const orc_hps *h, size_t N, u64 *data, int inverse) {
+    for (size_t j = 0; j < h->nr; j++) {
+        if (inverse)
+            ntt_inv_limb(data + j * N, N, h->r[j], h->irp + j * N, h->irps + j * N, h->ninv[j]);
+        else
+            ntt_fwd_limb(data + j * N, N, h->r[j], h->rp + j * N, h->rps + j * N);
+    }
+}
+
+/* bfv_multiply_hps (src/evaluate.cu:648-801) for mul_tech hps: extend both
+ * operands Q -> QR (bConv_HPS), NTT, tensor, INTT, scaleAndRound_HPS_QR_R
+ * (src/rns.cu:1700-1746: nu = 0.5 + sum x_i frac_i with fused multiply-adds,
+ * alpha = (u64) nu), then bConv_HPS R -> Q. ct1, ct2 [2][k][N] -> out3 [3][k][N] */
+int orc_bfv_multiply_hps(const orc_ctx *c, const orc_hps *h, const u64 *ct1, const u64 *ct2, u64 *out3) {
+    size_t N = c->N, k = h->k, nr = h->nr, QN = k * N, RN = nr * N, SN = QN + RN;
+    u64 *a = (u64 *)malloc(3 * SN * 8), *b = (u64 *)malloc(2 * SN * 8);
+    const u64 *src[2] = {ct1, ct2};
+    u64 *dst[2] = {a, b};
+    for (int s = 0; s < 2; s++)
+        for (int p = 0; p < 2; p++) {
+            u64 *d = dst[s] + p * SN;
+            memcpy(d, src[s] + p * QN, QN * 8);
+            bconv_hps(N, c->q, k, h->r, nr, d, d + QN);
+            orc_ntt_forward(c, d, k, 0);
+            r_ntt(h, N, d + QN, 0);
+        }
+    for (size_t l = 0; l < k + nr; l++) {
+        u64 q = l < k ? c->q[l] : h->r[l - k];
+        for (size_t n = 0; n < N; n++) {
+            size_t o = l * N + n;
+            u64 a0 = a[o], a1 = a[SN + o], b0 = b[o], b1 = b[SN + o];
+            u64 r0 = mulmod(a0, b0, q), r2 = mulmod(a1, b1, q);
+            u64 r1 = mulmod(addmod(a0, a1, q), addmod(b0, b1, q), q);
+            r1 = submod(submod(r1, r0, q), r2, q);
+            a[o] = r0; a[SN + o] = r1; a[2 * SN + o] = r2;
+        }
+    }
+    u64 *tr = (u64 *)malloc(RN * 8);
+    for (int p = 0; p < 3; p++) {
+        u64 *d = a + p * SN;
+        orc_ntt_inverse(c, d, k, 0);
+        r_ntt(h, N, d + QN, 1);
+        for (size_t n = 0; n < N; n++) {
+            double nu = 0.5;
+            for (size_t i = 0; i < k; i++) nu = fma((double)d[i * N + n], h->frac[i], nu);
+            u64 alpha = (u64)nu;
+            for (size_t j = 0; j < nr; j++) {
+                u64 rj = h->r[j];
+                const u64 *dm = h->divs_mod_r + j * (k + 1);
+                u128 acc = 0;
+                for (size_t i = 0; i < k; i++) acc += (u128)d[i * N + n] * dm[i];
+                acc += (u128)d[QN + j * N + n] * dm[k];
+                tr[j * N + n] = addmod((u64)(acc % rj), alpha % rj, rj);
+            }
+        }
+        bconv_hps(N, h->r, nr, c->q, k, tr, out3 + p * QN);
+    }
+    free(a); free(b); free(tr);
     return 0;
 }

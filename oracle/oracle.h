@@ -93,6 +93,15 @@ int orc_bfv_multiply(const orc_ctx *c, const orc_behz *b, const uint64_t *ct1, c
 int orc_bfv_relinearize(const orc_ctx *c, const uint64_t *ct3, const uint64_t *key, size_t dnum,
                         uint64_t *out2);
 
+/* ---- BFV HPS multiply (mul_tech hps) at the top data level ---- */
+typedef struct orc_hps orc_hps;
+orc_hps *orc_hps_create(const orc_ctx *c, uint64_t t);
+void orc_hps_destroy(orc_hps *h);
+size_t orc_hps_r_size(const orc_hps *h);
+void orc_hps_r(const orc_hps *h, uint64_t *out);
+int orc_bfv_multiply_hps(const orc_ctx *c, const orc_hps *h, const uint64_t *ct1, const uint64_t *ct2,
+                         uint64_t *out3);
+
 #ifdef __cplusplus
 }
 #endif

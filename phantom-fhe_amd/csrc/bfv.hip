@@ -10,6 +10,10 @@
 // conversion back to q. Every per-coefficient kernel keeps all of the
 // coefficient's residues in registers (|q| <= 16, |Bsk| <= 18).
 //
+// HPS (mul_tech hps, src/evaluate.cu:648-801): extend Q -> Q u R with the
+// FP64-corrected exact conversion, NTT over Q u R, tensor, INTT, then one
+// kernel for the scale-and-round to R and the conversion back to Q.
+//
 // Relinearisation: BFV key switching equals the CKKS one conjugated by the
 // NTT (modup of c2 = modup of NTT(c2) with the digit's own limbs kept;
 // moddown = INTT of the NTT-form result), so it runs as NTT(c2) -> key
@@ -123,6 +127,85 @@ __global__ __launch_bounds__(kThreads) void k_behz_floor_sk(const u64 *__restric
         }
 }
 
+// HPS (mul_tech hps). bConv_HPS (src/rns_bconv.cu:277-372): y_i = [x_i Ihat_i^-1]_{i},
+// out_p = sum_i y_i [Ihat_i]_p - v [I]_p with v = llround(sum_i y_i / i_i), the
+// FP64 sum taken with fused multiply-adds in input order.
+// k_hps_extend: Q part copied, R part = bConv_HPS(Q -> R), per polynomial.
+__global__ __launch_bounds__(kThreads) void k_hps_extend(const u64 *__restrict__ in, u64 *__restrict__ out,
+                                                         HpsConsts H, u32 N, u64 in_stride, u64 out_stride) {
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 *x = in + blockIdx.y * in_stride + k;
+    u64 *o = out + blockIdx.y * out_stride + k;
+    u64 y[kMaxQ];
+    double frac = 0.0;
+#pragma unroll
+    for (u32 i = 0; i < kMaxQ; ++i)
+        if (i < H.k) {
+            const u64 xi = x[(u64)i * N];
+            o[(u64)i * N] = xi;
+            y[i] = mul_mod(xi, H.qhatinv_q[i], H.mod_qr[i]);
+            frac = __builtin_fma((double)y[i], H.qinv[i], frac);
+        }
+    const u32 v = (u32)__builtin_llround(frac);
+    for (u32 j = 0; j < H.nr; ++j) {
+        const DevModulus m = H.mod_qr[H.k + j];
+        u128 acc = 0;
+#pragma unroll
+        for (u32 i = 0; i < kMaxQ; ++i)
+            if (i < H.k) acc += (u128)y[i] * H.qhat_r[(u64)j * H.k + i];
+        const u64 a = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
+        o[(u64)(H.k + j) * N] = sub_mod(a, H.alphaq_r[(u64)v * H.nr + j], m.value);
+    }
+}
+
+// scaleAndRound_HPS_QR_R (src/rns.cu:1700-1746) into registers, then
+// bConv_HPS R -> Q into the output ciphertext
+__global__ __launch_bounds__(kThreads) void k_hps_round_convert(const u64 *__restrict__ in, u64 *__restrict__ out,
+                                                                HpsConsts H, u32 N, u64 in_stride, u64 out_stride) {
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 *x = in + blockIdx.y * in_stride + k;
+    u64 *o = out + blockIdx.y * out_stride + k;
+    u64 xq[kMaxQ];
+    double nu = 0.5;
+#pragma unroll
+    for (u32 i = 0; i < kMaxQ; ++i)
+        if (i < H.k) {
+            xq[i] = x[(u64)i * N];
+            nu = __builtin_fma((double)xq[i], H.frac[i], nu);
+        }
+    const u64 alpha = (u64)nu;
+    u64 y[kMaxBsk];
+    double frac = 0.0;
+#pragma unroll
+    for (u32 j = 0; j < kMaxBsk; ++j)
+        if (j < H.nr) {
+            const DevModulus m = H.mod_qr[H.k + j];
+            const u64 *dm = H.divs_mod_r + (u64)j * (H.k + 1);
+            u128 acc = (u128)x[(u64)(H.k + j) * N] * dm[H.k];
+#pragma unroll
+            for (u32 i = 0; i < kMaxQ; ++i)
+                if (i < H.k) acc += (u128)xq[i] * dm[i];
+            const u64 r = add_mod(barrett128(acc, m.value, m.ratio_lo, m.ratio_hi),
+                                  barrett64(alpha, m.value, m.ratio_hi), m.value);
+            y[j] = mul_mod(r, H.rhatinv_r[j], m);
+            frac = __builtin_fma((double)y[j], H.rinv[j], frac);
+        }
+    const u32 v = (u32)__builtin_llround(frac);
+#pragma unroll
+    for (u32 i = 0; i < kMaxQ; ++i)
+        if (i < H.k) {
+            const DevModulus m = H.mod_qr[i];
+            u128 acc = 0;
+#pragma unroll
+            for (u32 j = 0; j < kMaxBsk; ++j)
+                if (j < H.nr) acc += (u128)y[j] * H.rhat_q[(u64)i * H.nr + j];
+            const u64 a = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
+            o[(u64)i * N] = sub_mod(a, H.alphar_q[(u64)v * H.k + i], m.value);
+        }
+}
+
 __global__ __launch_bounds__(kThreads) void k_add_inplace(u64 *__restrict__ a, const u64 *__restrict__ b,
                                                           const DevModulus *__restrict__ mod, u32 N, u32 L) {
     const u32 k = blockIdx.x * kThreads + threadIdx.x;
@@ -154,6 +237,7 @@ void behz_mul_1(const Context &ctx, const BehzConsts &B, const Ciphertext &ct, u
 }  // namespace
 
 void bfv_multiply(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, hipStream_t s) {
+    if (ctx.params().mul_tech == kMulHPS) return bfv_multiply_hps(ctx, ct1, ct2, s);
     const BehzConsts &B = behz_of(ctx);
     require(ct1.chain_index == ct2.chain_index, "encrypted1 and encrypted2 parameter mismatch");
     require(!ct1.is_ntt_form && !ct2.is_ntt_form, "encrypted1 or encrypted2 cannot be in NTT form");
@@ -188,6 +272,44 @@ void bfv_multiply(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, hi
     }
     dev_free(q1, s);
     dev_free(b1, s);
+}
+
+// bfv_multiply_hps (src/evaluate.cu:648-801), mul_tech hps
+void bfv_multiply_hps(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, hipStream_t s) {
+    const HpsConsts *Hp = ctx.hps();
+    require(Hp != nullptr, "the context has no HPS tool: scheme must be BFV with mul_tech hps");
+    const HpsConsts &H = *Hp;
+    require(H.k <= kMaxQ && H.nr <= kMaxBsk, "HPS multiply supports at most 16 data primes");
+    require(ct1.chain_index == ct2.chain_index, "encrypted1 and encrypted2 parameter mismatch");
+    require(!ct1.is_ntt_form && !ct2.is_ntt_form, "encrypted1 or encrypted2 cannot be in NTT form");
+    require(ct1.size == 2 && ct2.size == 2, "dest_size must be 3 when computing BFV multiplication using HPS");
+    require(ct1.chain_index == 1, "HPS multiplication runs at the top data level");
+    const u64 N = ctx.N(), QN = (u64)H.k * N, SN = (u64)(H.k + H.nr) * N;
+    const bool square = &ct1 == &ct2 || ct1.data == ct2.data;
+    u64 *a = static_cast<u64 *>(dev_alloc(3 * SN * 8, s));
+    u64 *b = square ? a : static_cast<u64 *>(dev_alloc(2 * SN * 8, s));
+    LimbMap m = limb_map(a, 0, H.k + H.nr, SN);
+    m.split = H.k;
+    m.special_row = H.r_row0;
+    hipLaunchKernelGGL(k_hps_extend, grid2(N, 2), dim3(kThreads), 0, s, (const u64 *)ct1.data, a, H, (u32)N, QN, SN);
+    PHX_HIP_CHECK(hipGetLastError());
+    ntt_forward(ctx.tables(), m, H.k + H.nr, 2, s);
+    if (!square) {
+        hipLaunchKernelGGL(k_hps_extend, grid2(N, 2), dim3(kThreads), 0, s, (const u64 *)ct2.data, b, H, (u32)N, QN,
+                           SN);
+        PHX_HIP_CHECK(hipGetLastError());
+        LimbMap mb = m;
+        mb.data = b;
+        ntt_forward(ctx.tables(), mb, H.k + H.nr, 2, s);
+    }
+    tensor_2x2_mod(H.mod_qr, N, H.k + H.nr, a, b, a, a + SN, a + 2 * SN, s);
+    ntt_inverse(ctx.tables(), m, H.k + H.nr, 3, s);
+    ct1.resize(ctx, ct1.chain_index, 3, s);
+    hipLaunchKernelGGL(k_hps_round_convert, grid2(N, 3), dim3(kThreads), 0, s, (const u64 *)a, ct1.data, H, (u32)N,
+                       SN, QN);
+    PHX_HIP_CHECK(hipGetLastError());
+    if (!square) dev_free(b, s);
+    dev_free(a, s);
 }
 
 void bfv_relinearize(const Context &ctx, Ciphertext &ct, const KSwitchKey &rlk, hipStream_t s) {

@@ -1,11 +1,13 @@
 // context.cpp -- context construction: tables, per-level constants, pool.
 #include "context.h"
-#include "tune.h"
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <thread>
+
+#include "tune.h"
 
 namespace phx {
 
@@ -161,8 +163,21 @@ Context::Context(const Params &p) : params_(p) {
         for (u64 b : bsk_)
             for (u64 q : p.coeff_modulus) require(b != q, "coefficient modulus collides with the BEHZ base");
     }
+    // ---- HPS base R (src/rns.cu:686-699): size_Q + 1 primes below the smallest q,
+    // stepping down by 2N within its bit size (get_primes_below, src/host/numth.cu:235-262)
+    if (p.scheme == kSchemeBFV && p.mul_tech == kMulHPS) {
+        require(p.plain_modulus > 1, "plain_modulus is not set");
+        const u64 qmin = *std::min_element(p.coeff_modulus.begin(), p.coeff_modulus.begin() + size_Q_);
+        const u64 lower = 1ull << (hm::bit_count(qmin) - 1);
+        for (u64 v = qmin - 2 * N_; hps_r_.size() < size_Q_ + 1 && v > lower; v -= 2 * N_)
+            if (hm::is_prime(v)) hps_r_.push_back(v);
+        require(hps_r_.size() == size_Q_ + 1, "failed to find enough qualifying primes");
+        for (u64 r : hps_r_)
+            for (u64 q : p.coeff_modulus) require(r != q, "coefficient modulus collides with the HPS base");
+    }
     std::vector<u64> rows_p = p.coeff_modulus;
     rows_p.insert(rows_p.end(), bsk_.begin(), bsk_.end());
+    rows_p.insert(rows_p.end(), hps_r_.begin(), hps_r_.end());
     const size_t R = rows_p.size();
 
     // ---- NTT tables (host, one thread per prime) ----
@@ -419,6 +434,86 @@ Context::Context(const Params &p) : params_(p) {
         behz_.bsk_row0 = (u32)QP;
         behz_.t = t;
     }
+    // HPS constants
+    struct HpsOffs {
+        size_t mod_qr, qhatinv_q, qhat_r, qinv, alphaq_r, frac, divs_mod_r, rhatinv_r, rhat_q, rinv, alphar_q;
+    } ho{};
+    if (!hps_r_.empty()) {
+        const u32 k = size_Q_, nr = (u32)hps_r_.size(), ns = k + nr;
+        const u64 t = p.plain_modulus;
+        std::vector<u64> s(qp, qp + k);
+        s.insert(s.end(), hps_r_.begin(), hps_r_.end());
+        const u64 *rp = hps_r_.data();
+        std::vector<u64> qhatinv(k), qhat_r((size_t)nr * k), alphaq_r((size_t)(k + 1) * nr), rhatinv(nr),
+            rhat_q((size_t)k * nr), alphar_q((size_t)(nr + 1) * k), divs((size_t)nr * (k + 1));
+        std::vector<double> qinv(k), rinv(nr), frac(k);
+        for (u32 i = 0; i < k; ++i) {
+            qhatinv[i] = hm::invmod(hm::punctured_mod(qp, k, i, qp[i]), qp[i]);
+            qinv[i] = 1.0 / (double)qp[i];
+            for (u32 j = 0; j < nr; ++j) qhat_r[(size_t)j * k + i] = hm::punctured_mod(qp, k, i, rp[j]);
+        }
+        for (u32 v = 0; v <= k; ++v)
+            for (u32 j = 0; j < nr; ++j)
+                alphaq_r[(size_t)v * nr + j] = hm::mulmod(v, hm::product_mod(qp, k, rp[j]), rp[j]);
+        for (u32 j = 0; j < nr; ++j) {
+            rhatinv[j] = hm::invmod(hm::punctured_mod(rp, nr, j, rp[j]), rp[j]);
+            rinv[j] = 1.0 / (double)rp[j];
+            for (u32 i = 0; i < k; ++i) rhat_q[(size_t)i * nr + j] = hm::punctured_mod(rp, nr, j, qp[i]);
+        }
+        for (u32 v = 0; v <= nr; ++v)
+            for (u32 i = 0; i < k; ++i)
+                alphar_q[(size_t)v * k + i] = hm::mulmod(v, hm::product_mod(rp, nr, qp[i]), qp[i]);
+        // theta_i = t R [(S/s_i)^-1]_{s_i} (S = Q R): frac_i = [theta_i]_{q_i} / q_i,
+        // divs = floor(theta_i / s_i) mod r_j (src/rns.cu:734-781)
+        for (u32 i = 0; i < ns; ++i) {
+            const u64 shinv = hm::invmod(hm::punctured_mod(s.data(), ns, i, s[i]), s[i]);
+            std::vector<u64> th{t};
+            auto mul = [&](u64 m) {
+                u128 c = 0;
+                for (auto &w : th) {
+                    const u128 v = (u128)w * m + c;
+                    w = (u64)v;
+                    c = v >> 64;
+                }
+                if (c) th.push_back((u64)c);
+            };
+            auto mod = [&](u64 m) {
+                u128 r = 0;
+                for (size_t w = th.size(); w-- > 0;) r = ((r << 64) | th[w]) % m;
+                return (u64)r;
+            };
+            for (u32 j = 0; j < nr; ++j) mul(rp[j]);
+            mul(shinv);
+            if (i < k) frac[i] = (double)mod(s[i]) / (double)s[i];
+            u128 r = 0;
+            for (size_t w = th.size(); w-- > 0;) {
+                const u128 cur = (r << 64) | th[w];
+                th[w] = (u64)(cur / s[i]);
+                r = cur % s[i];
+            }
+            for (u32 j = 0; j < nr; ++j) {
+                if (i < k) divs[(size_t)j * (k + 1) + i] = mod(rp[j]);
+                else if (i == k + j) divs[(size_t)j * (k + 1) + k] = mod(rp[j]);
+            }
+        }
+        std::vector<DevModulus> mqr(ns);
+        for (u32 i = 0; i < ns; ++i) {
+            u64 lo, hi;
+            hm::barrett_ratio(s[i], lo, hi);
+            mqr[i] = {s[i], lo, hi, 0};
+        }
+        auto pushd = [&](const std::vector<double> &v) {
+            std::vector<u64> w(v.size());
+            std::memcpy(w.data(), v.data(), v.size() * 8);
+            return cb.push64(w);
+        };
+        ho = {cb.push_struct(mqr), cb.push64(qhatinv), cb.push64(qhat_r), pushd(qinv), cb.push64(alphaq_r),
+              pushd(frac),         cb.push64(divs),    cb.push64(rhatinv), cb.push64(rhat_q), pushd(rinv),
+              cb.push64(alphar_q)};
+        hps_.k = k;
+        hps_.nr = nr;
+        hps_.r_row0 = (u32)(QP + bsk_.size());
+    }
     cb.align(256);
     PHX_HIP_CHECK(hipMalloc(&cbuf_, cb.buf.size()));
     PHX_HIP_CHECK(hipMemcpy(cbuf_, cb.buf.data(), cb.buf.size(), hipMemcpyHostToDevice));
@@ -433,6 +528,21 @@ Context::Context(const Params &p) : params_(p) {
         L.rescale_qinv = reinterpret_cast<const u64 *>(cbb + offs[c].rescale);
     }
     bigP_mod_q_ = reinterpret_cast<u64 *>(cbb + bp_off);
+    if (hps_.k) {
+        auto at = [&](size_t off) { return reinterpret_cast<const u64 *>(cbb + off); };
+        auto atd = [&](size_t off) { return reinterpret_cast<const double *>(cbb + off); };
+        hps_.mod_qr = reinterpret_cast<const DevModulus *>(cbb + ho.mod_qr);
+        hps_.qhatinv_q = at(ho.qhatinv_q);
+        hps_.qhat_r = at(ho.qhat_r);
+        hps_.qinv = atd(ho.qinv);
+        hps_.alphaq_r = at(ho.alphaq_r);
+        hps_.frac = atd(ho.frac);
+        hps_.divs_mod_r = at(ho.divs_mod_r);
+        hps_.rhatinv_r = at(ho.rhatinv_r);
+        hps_.rhat_q = at(ho.rhat_q);
+        hps_.rinv = atd(ho.rinv);
+        hps_.alphar_q = at(ho.alphar_q);
+    }
     if (behz_.k) {
         auto at = [&](size_t off) { return reinterpret_cast<const u64 *>(cbb + off); };
         behz_.lift_f = at(bo.lift_f);

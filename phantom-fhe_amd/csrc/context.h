@@ -94,6 +94,27 @@ struct BehzConsts {
     const u64 *tscale_bsk;   // [nb][4]
 };
 
+// HPS constants of the top data level (BFV multiply with mul_tech hps;
+// reference DRNSTool ctor, src/rns.cu:686-783, and the BaseConverter tables of
+// src/host/rns.cu:438-497): R = size_Q + 1 primes below the smallest q, as
+// table rows r_row0 ..; mod_qr is the moduli of Q then R, contiguous.
+struct HpsConsts {
+    u32 k = 0, nr = 0;
+    u32 r_row0 = 0;
+    u32 pad = 0;
+    const DevModulus *mod_qr;   // [k + nr]
+    const u64 *qhatinv_q;       // [k]        Qhat_i^-1 mod q_i
+    const u64 *qhat_r;          // [nr][k]    Qhat_i mod r_j
+    const double *qinv;         // [k]        1 / q_i
+    const u64 *alphaq_r;        // [k+1][nr]  v Q mod r_j
+    const double *frac;         // [k]        tRSHatInvModsDivsFrac
+    const u64 *divs_mod_r;      // [nr][k+1]  tRSHatInvModsDivsModr
+    const u64 *rhatinv_r;       // [nr]       Rhat_j^-1 mod r_j
+    const u64 *rhat_q;          // [k][nr]    Rhat_j mod q_i
+    const double *rinv;         // [nr]       1 / r_j
+    const u64 *alphar_q;        // [nr+1][k]  v R mod q_i
+};
+
 class Context {
 public:
     explicit Context(const Params &p);
@@ -123,6 +144,9 @@ public:
     // BFV with BEHZ multiplication: its constants (else null)
     const BehzConsts *behz() const { return behz_.k ? &behz_ : nullptr; }
     const std::vector<u64> &bsk_primes() const { return bsk_; }
+    // BFV with HPS multiplication: its constants (else null)
+    const HpsConsts *hps() const { return hps_.k ? &hps_ : nullptr; }
+    const std::vector<u64> &hps_r_primes() const { return hps_r_; }
 
     // Galois permutation tables (NTT domain), created on first use
     const u32 *galois_perm(u32 elt) const;
@@ -141,6 +165,8 @@ private:
     mutable std::vector<std::pair<u32, u32 *>> galois_tables_;
     BehzConsts behz_{};
     std::vector<u64> bsk_;
+    HpsConsts hps_{};
+    std::vector<u64> hps_r_;
 };
 
 // stream-ordered device allocation (pooled, like the reference's cuda_auto_ptr)

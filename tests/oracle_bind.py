@@ -68,6 +68,13 @@ def lib():
         L.orc_behz_bsk.argtypes = [ctypes.c_void_p, _u64p]
         L.orc_bfv_multiply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p, _u64p, _u64p]
         L.orc_bfv_relinearize.argtypes = [ctypes.c_void_p, _u64p, _u64p, _sz, _u64p]
+        L.orc_hps_create.restype = ctypes.c_void_p
+        L.orc_hps_create.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+        L.orc_hps_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_hps_r_size.restype = _sz
+        L.orc_hps_r_size.argtypes = [ctypes.c_void_p]
+        L.orc_hps_r.argtypes = [ctypes.c_void_p, _u64p]
+        L.orc_bfv_multiply_hps.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p, _u64p, _u64p]
         _lib = L
     return _lib
 
@@ -259,3 +266,31 @@ def bfv_relinearize(octx, ct3, key):
                                    _p(out))
     assert rc == 0
     return out
+
+
+class Hps:
+    """HPS constants of an oracle context's top data level, BFV multiply."""
+
+    def __init__(self, octx, t):
+        self.octx = octx
+        self.t = t
+        self._h = lib().orc_hps_create(octx._h, t)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.orc_hps_destroy(self._h)
+            self._h = None
+
+    def r(self):
+        n = lib().orc_hps_r_size(self._h)
+        out = np.zeros(n, np.uint64)
+        lib().orc_hps_r(self._h, _p(out))
+        return [int(x) for x in out]
+
+    def multiply(self, ct1, ct2):
+        k = self.octx.size_QP - self.octx.size_P
+        out = np.zeros((3, k, self.octx.N), np.uint64)
+        rc = lib().orc_bfv_multiply_hps(self.octx._h, self._h, _p(np.ascontiguousarray(ct1, np.uint64)),
+                                        _p(np.ascontiguousarray(ct2, np.uint64)), _p(out))
+        assert rc == 0
+        return out

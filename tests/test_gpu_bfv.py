@@ -17,17 +17,18 @@ T = 65537
 LIMBS = {1 << 13: 4, 1 << 14: 8, 1 << 15: 16}
 
 
-def make_bfv(N, k):
+def make_bfv(N, k, tech=ph.MUL_TECH_BEHZ):
     mods = ph.coeff_modulus_create(N, [54] * k + [60])
     parms = ph.EncryptionParameters(ph.SCHEME_BFV)
     parms.set_poly_modulus_degree(N)
     parms.set_coeff_modulus(mods)
     parms.set_special_modulus_size(1)
     parms.set_plain_modulus(T)
-    parms.set_mul_tech(ph.MUL_TECH_BEHZ)
+    parms.set_mul_tech(tech)
     ctx = ph.context(parms)
     octx = ob.Ctx(N.bit_length() - 1, mods, 1)
-    return ctx, octx, ob.Behz(octx, T), mods
+    tool = ob.Behz(octx, T) if tech == ph.MUL_TECH_BEHZ else ob.Hps(octx, T)
+    return ctx, octx, tool, mods
 
 
 def upload_bfv(ctx, arr):
@@ -133,10 +134,34 @@ def poly_mul_rns(octx, x, y, L):
     return octx.ntt_inverse(((fx.astype(object) * fy.astype(object)) % qs).astype(np.uint64), 0)
 
 
-def test_bfv_encrypt_multiply_relin_decrypt():
+@pytest.mark.parametrize("N,k", [(1 << 13, 4), (1 << 15, 16)])
+def test_hps_multiply_and_relin_bit_exact(N, k):
+    """mul_tech hps: FP64-corrected base extension and scale-and-round, then
+    relinearisation, bit-exact against the oracle (same fused multiply-adds)."""
+    ob.set_threads(16)
+    say(f"case: hps multiply N={N} k={k}")
+    ctx, octx, hps, mods = make_bfv(N, k, ph.MUL_TECH_HPS)
+    rng = np.random.default_rng(N + k)
+    a, b = rand_ct(rng, mods[:k], N), rand_ct(rng, mods[:k], N)
+    c1, c2 = upload_bfv(ctx, a), upload_bfv(ctx, b)
+    ph.multiply_inplace(ctx, c1, c2)
+    want3 = hps.multiply(a, b)
+    assert np.array_equal(c1.download(N), want3)
+    c3 = upload_bfv(ctx, a)
+    ph.multiply_inplace(ctx, c3, c3)
+    assert np.array_equal(c3.download(N), hps.multiply(a, a))
+    key = rand_key(rng, mods, N, ctx.dnum)
+    rk = ph.relin_key(ctx, host=key)
+    c4, c5 = upload_bfv(ctx, a), upload_bfv(ctx, b)
+    ph.multiply_and_relin_inplace(ctx, c4, c5, rk)
+    assert np.array_equal(c4.download(N), ob.bfv_relinearize(octx, want3, key))
+
+
+@pytest.mark.parametrize("tech", [ph.MUL_TECH_BEHZ, ph.MUL_TECH_HPS])
+def test_bfv_encrypt_multiply_relin_decrypt(tech):
     N, k = 1 << 13, 4
-    say("case: bfv semantic multiply")
-    ctx, octx, beh, mods = make_bfv(N, k)
+    say(f"case: bfv semantic multiply tech={tech}")
+    ctx, octx, beh, mods = make_bfv(N, k, tech)
     qs = mods[:k]
     Q = 1
     for q in qs:

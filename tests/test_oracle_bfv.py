@@ -157,3 +157,70 @@ def test_bfv_relinearize_decrypts_like_size3(logN, bits, P):
     d2 = decrypt_int([out[0], out[1]])
     noise = [((x - y) % Q + Q // 2) % Q - Q // 2 for x, y in zip(d2, d3)]
     assert max(abs(v) for v in noise) < 1 << 16, max(abs(v) for v in noise)
+
+
+# ---------------------------------------------------------------- HPS
+# bfv_multiply_hps (src/evaluate.cu:648-801): bConv_HPS lifts each input to its
+# centred representative X_c in (-Q/2, Q/2] (exact CRT with the FP64 overflow
+# count), the tensor z of the X_c is exact in base Q R, scaleAndRound_HPS_QR_R
+# gives round(t z / Q) mod R and bConv_HPS R -> Q returns it (centred, exact)
+# mod q_i. So out_i = round(t z / Q) + d mod q_i, where d comes only from the
+# FP64 sum nu = 0.5 + sum_i x_i frac_i of scaleAndRound_HPS_QR_R: its absolute
+# error is at most k^2 2^(qbits - 53) (x_i < 2^qbits), so d = 0 for small
+# primes (exact regime, checked exactly) and |d| <= that bound + 1 for 50- and
+# 54-bit primes. Which way a sum near a rounding boundary falls (the
+# reference's compiled FP64 accumulation order and contraction) is the one part
+# no closed form pins; the oracle and the GPU use the same fused
+# multiply-adds in input order, so they agree bit for bit.
+
+def test_hps_r_primes_closed_form():
+    octx, mods, N = setup(5, [30, 30, 30, 31], 1)
+    h = ob.Hps(octx, T)
+    qmin = min(mods[:3])
+    want, v = [], qmin - 2 * N
+    from bigint_ref import is_prime
+    while len(want) < 4:
+        if is_prime(v):
+            want.append(v)
+        v -= 2 * N
+    assert h.r() == want
+
+
+@pytest.mark.parametrize("logN,bits", [(5, [30, 30, 30, 31]), (4, [50, 50, 61]), (5, [54, 54, 54, 54, 60])])
+def test_hps_multiply_matches_integer_round(logN, bits):
+    octx, mods, N = setup(logN, bits, 1)
+    k = len(mods) - 1
+    qs = mods[:k]
+    Q = 1
+    for q in qs:
+        Q *= q
+    h = ob.Hps(octx, T)
+    rng = np.random.default_rng(logN * 13 + k)
+    ct1 = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs]) for _ in range(2)])
+    ct2 = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs]) for _ in range(2)])
+    out = h.multiply(ct1, ct2)
+
+    def centred(res):
+        x = crt([int(v) for v in res], qs)[0]
+        return x - Q if 2 * x > Q else x
+
+    X = [[centred(ct1[p, :, n]) for n in range(N)] for p in range(2)]
+    Y = [[centred(ct2[p, :, n]) for n in range(N)] for p in range(2)]
+    zs = (negacyclic_int(X[0], Y[0]),
+          [a + b for a, b in zip(negacyclic_int(X[0], Y[1]), negacyclic_int(X[1], Y[0]))],
+          negacyclic_int(X[1], Y[1]))
+    qbits = max(q.bit_length() for q in qs)
+    err = k * k * 2.0 ** (qbits - 53)
+    bound = 0 if err < 2.0 ** -10 else int(err) + 1
+    worst = 0
+    for d, z in enumerate(zs):
+        for n in range(N):
+            num = T * z[n]
+            w = (2 * num + Q) // (2 * Q)        # round half up
+            got = crt([int(out[d, i, n]) for i in range(k)], qs)[0]
+            diff = (got - w) % Q
+            diff = diff - Q if 2 * diff > Q else diff
+            worst = max(worst, abs(diff))
+            assert abs(diff) <= bound, (d, n, diff, bound)
+    if bound:
+        assert worst <= bound
