@@ -124,6 +124,25 @@ int phx_encrypt_symmetric(const phx_context *ctx, const phx_secret_key *sk, cons
 int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext *ct, uint64_t *out_host,
                 phx_stream stream);
 
+/* ---- serialization: the reference's stream layouts, into / out of caller buffers.
+ * Ciphertext (PhantomCiphertext::save / load, include/ciphertext.h:173-216): 58-byte
+ * header (chain_index, size, poly_modulus_degree, coeff_modulus_size as u64, scale as
+ * double, correction_factor u64, noiseScaleDeg u64, is_ntt_form and is_asymmetric one
+ * byte each), then [size][L][N] u64. Relin key (PhantomRelinKey::save / load,
+ * include/secretkey.h:129-162): dnum (u64), then dnum public-key ciphertexts. Galois
+ * keys (PhantomGaloisKey::save / load, include/secretkey.h:194-219): the count (u64),
+ * then that many relin keys, in the order of the elts list given to load. */
+int phx_ciphertext_save_size(const phx_ciphertext *ct, size_t *bytes);
+int phx_ciphertext_save(const phx_ciphertext *ct, void *buf, size_t bytes);
+int phx_ciphertext_load(const phx_context *ctx, phx_ciphertext *ct, const void *buf, size_t bytes);
+int phx_relin_key_save_size(const phx_context *ctx, const phx_relin_key *key, size_t *bytes);
+int phx_relin_key_save(const phx_context *ctx, const phx_relin_key *key, void *buf, size_t bytes);
+int phx_relin_key_load(const phx_context *ctx, const void *buf, size_t bytes, phx_relin_key **out);
+int phx_galois_key_save_size(const phx_context *ctx, const phx_galois_key *gk, size_t *bytes);
+int phx_galois_key_save(const phx_context *ctx, const phx_galois_key *gk, void *buf, size_t bytes);
+int phx_galois_key_load(const phx_context *ctx, const uint32_t *elts, size_t n, const void *buf, size_t bytes,
+                        phx_galois_key **out);
+
 /* ---- CKKS encoder (include/ckks.h), host side ------------------------------ */
 /* PhantomCKKSEncoder::encode (src/ckks.cu:65-135): count <= N/2 complex values
  * (re[i] + i*im[i], im may be NULL) -> round(scale * m) as coefficient residues
@@ -151,6 +170,10 @@ int phx_apply_galois_inplace(const phx_context *ctx, phx_ciphertext *ct, uint32_
 /* rotate_inplace (include/evaluate.cuh:225-226, src/evaluate.cu:1633-1668) */
 int phx_rotate_inplace(const phx_context *ctx, phx_ciphertext *ct, int step, const phx_galois_key *gk,
                        phx_stream stream);
+/* hoisting_inplace (include/evaluate.cuh:237-238, src/evaluate.cu:1670-1865): ct <- the sum
+ * of the rotations of ct by steps[0..n-1], one modup shared by all of them (CKKS) */
+int phx_hoisting_inplace(const phx_context *ctx, phx_ciphertext *ct, const phx_galois_key *gk, const int *steps,
+                         size_t n, phx_stream stream);
 /* keyswitch_inplace (include/evaluate.cuh:29-32, src/eval_key_switch.cu:95-182):
  * ct[0..1] += keyswitch(c2) with c2 = [Ql][N] (NTT form) in device memory */
 int phx_keyswitch_inplace(const phx_context *ctx, phx_ciphertext *ct, const uint64_t *c2_dev,

@@ -1197,3 +1197,42 @@ int orc_bfv_multiply_hps(const orc_ctx *c, const orc_hps *h, const u64 *ct1, con
     free(a); free(b); free(tr);
     return 0;
 }
+
+/* hoisting_inplace for CKKS (src/evaluate.cu:1670-1865): the sum of the
+ * rotations of ct by elts[0..n-1] with one modup of c1:
+ *   acc_c0 = sum_i pi_i(c0),  acc_cx = sum_i IP(pi_i(modup(c1)), key_i)
+ *   out = (acc_c0 + moddown(acc_cx0), moddown(acc_cx1))
+ * keys [n][dnum][2][QP][N]; pi_i permutes every limb of every digit. */
+int orc_hoisting(const orc_ctx *c, size_t size_Ql, const u64 *ct, const uint32_t *elts, size_t n,
+                 const u64 *keys, size_t dnum, u64 *out2) {
+    size_t N = c->N, LN = size_Ql * N, QlP = size_Ql + c->size_P, LPN = QlP * N;
+    size_t beta = orc_beta(c, size_Ql);
+    if (beta > dnum || n == 0) return -1;
+    size_t key_words = dnum * 2 * c->size_QP * N;
+    u64 *t = (u64 *)malloc(beta * LPN * 8), *tp = (u64 *)malloc(beta * LPN * 8);
+    u64 *acc0 = (u64 *)calloc(LN, 8), *acx = (u64 *)calloc(2 * LPN, 8), *cx = (u64 *)malloc(2 * LPN * 8);
+    uint32_t *perm = (uint32_t *)malloc(N * 4);
+    orc_modup(c, size_Ql, ct + LN, t);
+    for (size_t e = 0; e < n; e++) {
+        orc_galois_perm_table(c->logN, elts[e], perm);
+        for (size_t i = 0; i < size_Ql; i++) {
+            u64 q = c->q[i];
+            for (size_t k = 0; k < N; k++) acc0[i * N + k] = addmod(acc0[i * N + k], ct[i * N + perm[k]], q);
+        }
+        for (size_t r = 0; r < beta * QlP; r++)
+            for (size_t k = 0; k < N; k++) tp[r * N + k] = t[r * N + perm[k]];
+        orc_inner_product(c, size_Ql, tp, keys + e * key_words, dnum, cx);
+        for (size_t h = 0; h < 2; h++)
+            for (size_t p = 0; p < QlP; p++) {
+                u64 q = c->q[qlp_row(c, size_Ql, p)];
+                for (size_t k = 0; k < N; k++)
+                    acx[h * LPN + p * N + k] = addmod(acx[h * LPN + p * N + k], cx[h * LPN + p * N + k], q);
+            }
+    }
+    orc_moddown(c, size_Ql, acx, out2);
+    orc_moddown(c, size_Ql, acx + LPN, out2 + LN);
+    for (size_t i = 0; i < size_Ql; i++)
+        for (size_t k = 0; k < N; k++) out2[i * N + k] = addmod(out2[i * N + k], acc0[i * N + k], c->q[i]);
+    free(t); free(tp); free(acc0); free(acx); free(cx); free(perm);
+    return 0;
+}

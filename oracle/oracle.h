@@ -75,6 +75,11 @@ void orc_galois_perm_table(int logN, uint32_t elt, uint32_t *out);
 int orc_apply_galois(const orc_ctx *c, size_t size_Ql, const uint64_t *ct, uint32_t elt,
                      const uint64_t *key, size_t dnum, uint64_t *out2);
 
+/* hoisting_inplace (CKKS): sum of the rotations of ct by elts, one shared modup;
+ * keys [n][dnum][2][QP][N] in the order of elts */
+int orc_hoisting(const orc_ctx *c, size_t size_Ql, const uint64_t *ct, const uint32_t *elts, size_t n,
+                 const uint64_t *keys, size_t dnum, uint64_t *out2);
+
 /* key material from explicit samples: s_ntt, s2_ntt [QP][N] (NTT), a [dnum][QP][N] (NTT),
  * e [dnum][QP][N] (coefficient domain, canonical residues) */
 int orc_make_kswitch_key(const orc_ctx *c, const uint64_t *s_ntt, const uint64_t *s2_ntt,

@@ -10,6 +10,7 @@
 #include "ckks.h"
 #include "serve.h"
 #include "bfv.h"
+#include "serial.h"
 
 using namespace phx;
 
@@ -403,6 +404,125 @@ int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext
     });
 }
 
+int phx_ciphertext_save_size(const phx_ciphertext *ct, size_t *bytes) {
+    return guard([&] {
+        nn(ct, "ciphertext");
+        nn(bytes, "bytes");
+        *bytes = ciphertext_bytes(ct->ct);
+    });
+}
+
+int phx_ciphertext_save(const phx_ciphertext *ct, void *buf, size_t bytes) {
+    return guard([&] {
+        nn(ct, "ciphertext");
+        nn(buf, "buffer");
+        ciphertext_save(ct->ct, static_cast<uint8_t *>(buf), bytes);
+    });
+}
+
+int phx_ciphertext_load(const phx_context *ctx, phx_ciphertext *ct, const void *buf, size_t bytes) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct, "ciphertext");
+        nn(buf, "buffer");
+        ciphertext_load(*ctx->ctx, ct->ct, static_cast<const uint8_t *>(buf), bytes);
+    });
+}
+
+int phx_relin_key_save_size(const phx_context *ctx, const phx_relin_key *key, size_t *bytes) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(key, "key");
+        nn(bytes, "bytes");
+        *bytes = kswitch_key_bytes(*ctx->ctx, key->key);
+    });
+}
+
+int phx_relin_key_save(const phx_context *ctx, const phx_relin_key *key, void *buf, size_t bytes) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(key, "key");
+        nn(buf, "buffer");
+        kswitch_key_save(*ctx->ctx, key->key, static_cast<uint8_t *>(buf), bytes);
+    });
+}
+
+int phx_relin_key_load(const phx_context *ctx, const void *buf, size_t bytes, phx_relin_key **out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(buf, "buffer");
+        nn(out, "out");
+        auto *k = new phx_relin_key{};
+        try {
+            kswitch_key_load(*ctx->ctx, k->key, static_cast<const uint8_t *>(buf), bytes);
+        } catch (...) {
+            delete k;
+            throw;
+        }
+        *out = k;
+    });
+}
+
+int phx_galois_key_save_size(const phx_context *ctx, const phx_galois_key *gk, size_t *bytes) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(gk, "Galois keys");
+        nn(bytes, "bytes");
+        size_t n = 8;
+        for (const auto &k : gk->gk.keys) n += kswitch_key_bytes(*ctx->ctx, k);
+        *bytes = n;
+    });
+}
+
+int phx_galois_key_save(const phx_context *ctx, const phx_galois_key *gk, void *buf, size_t bytes) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(gk, "Galois keys");
+        nn(buf, "buffer");
+        auto *p = static_cast<uint8_t *>(buf);
+        const u64 cnt = gk->gk.keys.size();
+        require(bytes >= 8, "buffer too small for the Galois keys");
+        std::memcpy(p, &cnt, 8);
+        size_t off = 8;
+        for (const auto &k : gk->gk.keys) {
+            const size_t kb = kswitch_key_bytes(*ctx->ctx, k);
+            require(off + kb <= bytes, "buffer too small for the Galois keys");
+            kswitch_key_save(*ctx->ctx, k, p + off, kb);
+            off += kb;
+        }
+    });
+}
+
+int phx_galois_key_load(const phx_context *ctx, const uint32_t *elts, size_t n, const void *buf, size_t bytes,
+                        phx_galois_key **out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(elts, "elts");
+        nn(buf, "buffer");
+        nn(out, "out");
+        const auto *p = static_cast<const uint8_t *>(buf);
+        require(bytes >= 8, "stream too short for Galois keys");
+        u64 cnt = 0;
+        std::memcpy(&cnt, p, 8);
+        require(cnt == n, "the number of Galois keys in the stream differs from the number of elements");
+        auto *g = new phx_galois_key{};
+        try {
+            size_t off = 8;
+            for (size_t i = 0; i < n; ++i) {
+                require((elts[i] & 1) && elts[i] < 2 * ctx->ctx->N(), "Galois element is not valid");
+                g->gk.elts.push_back(elts[i]);
+                g->gk.keys.emplace_back();
+                off += kswitch_key_load(*ctx->ctx, g->gk.keys.back(), p + off, bytes - off);
+            }
+        } catch (...) {
+            for (auto &k : g->gk.keys) dev_free(k.data, hipStreamPerThread);
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+
 int phx_ckks_encode(const phx_context *ctx, const double *re, const double *im, size_t count, double scale,
                     size_t chain_index, uint64_t *out) {
     return guard([&] {
@@ -512,6 +632,17 @@ int phx_rotate_inplace(const phx_context *ctx, phx_ciphertext *ct, int step, con
         nn(ct, "encrypted");
         nn(gk, "galois_keys");
         rotate(*ctx->ctx, ct->ct, step, gk->gk, S(s));
+    });
+}
+
+int phx_hoisting_inplace(const phx_context *ctx, phx_ciphertext *ct, const phx_galois_key *gk, const int *steps,
+                         size_t n, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct, "encrypted");
+        nn(gk, "Galois keys");
+        nn(steps, "steps");
+        hoisting(*ctx->ctx, ct->ct, gk->gk, std::vector<int>(steps, steps + n), S(s));
     });
 }
 

@@ -302,6 +302,52 @@ __global__ __launch_bounds__(kThreads) void k_inner_product(const u64 *__restric
     cx[((u64)QlP + pos) * N + k] = barrett128(a1, m.value, m.ratio_lo, m.ratio_hi);
 }
 
+// hoisting: the inner product of the permuted digits, accumulated:
+// cx[h][pos][k] (+)= sum_j t[j][pos][perm[k]] * key_j.(b|a)[row(pos)][k]
+__global__ __launch_bounds__(kThreads) void k_inner_product_perm_acc(const u64 *__restrict__ t,
+                                                                     const u32 *__restrict__ perm,
+                                                                     const u64 *__restrict__ key,
+                                                                     u64 *__restrict__ cx,
+                                                                     const DevModulus *__restrict__ mod, u32 N,
+                                                                     u32 Ql, u32 QlP, u32 size_Q, u32 QP, u32 beta,
+                                                                     int accumulate) {
+    const u32 pos = blockIdx.y;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u32 row = pos < Ql ? pos : size_Q + (pos - Ql);
+    const u64 QPN = (u64)QP * N;
+    const u32 pk = perm[k];
+    u128 a0 = 0, a1 = 0;
+    for (u32 j = 0; j < beta; ++j) {
+        const u64 x = t[((u64)j * QlP + pos) * N + pk];
+        const u64 *kb = key + (u64)j * 2 * QPN + (u64)row * N + k;
+        a0 += (u128)x * kb[0];
+        a1 += (u128)x * kb[QPN];
+    }
+    const DevModulus m = mod[row];
+    u64 r0 = barrett128(a0, m.value, m.ratio_lo, m.ratio_hi);
+    u64 r1 = barrett128(a1, m.value, m.ratio_lo, m.ratio_hi);
+    u64 *c0 = cx + (u64)pos * N + k, *c1 = cx + ((u64)QlP + pos) * N + k;
+    if (accumulate) {
+        r0 = add_mod(*c0, r0, m.value);
+        r1 = add_mod(*c1, r1, m.value);
+    }
+    *c0 = r0;
+    *c1 = r1;
+}
+
+// hoisting: acc (+)= pi(src) on L limbs
+__global__ __launch_bounds__(kThreads) void k_permute_acc(const u64 *__restrict__ src, u64 *__restrict__ acc,
+                                                          const u32 *__restrict__ perm,
+                                                          const DevModulus *__restrict__ mod, u32 N,
+                                                          int accumulate) {
+    const u64 off = (u64)blockIdx.y * N;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 v = src[off + perm[k]];
+    acc[off + k] = accumulate ? add_mod(acc[off + k], v, mod[blockIdx.y].value) : v;
+}
+
 // tensor_prod_2x2_rns_poly (src/polymath.cu:463-498), Karatsuba middle term
 __global__ __launch_bounds__(kThreads) void k_tensor(const u64 *__restrict__ a, const u64 *__restrict__ b,
                                                      u64 *d0, u64 *d1, u64 *__restrict__ d2,
@@ -627,6 +673,47 @@ void rescale_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, 
     dst.correction_factor = in.correction_factor;
     if (&out != &in) out.release(s);
     out = dst;
+}
+
+// hoisting_inplace for CKKS (src/evaluate.cu:1670-1865): ct <- sum_i rot(ct, steps[i])
+// with one modup of c1; each step's permutation is applied to c0 and, inside
+// the inner product, to the modup'ed digits; one moddown of the accumulated cx.
+void hoisting(const Context &ctx, Ciphertext &ct, const GaloisKeys &gk, const std::vector<int> &steps,
+              hipStream_t s) {
+    require(ctx.params().scheme == kSchemeCKKS, "hoisting: only CKKS is on this path");
+    require(ct.size == 2, "ciphertext size must be 2");
+    require(ct.is_ntt_form, "CKKS encrypted must be in NTT form");
+    require(!steps.empty(), "hoisting needs at least one step");
+    const LevelConsts &L = ctx.level(ct.chain_index);
+    const u32 Ql = L.size_Ql, QlP = L.size_QlP, beta = L.beta;
+    const u64 N = ctx.N(), LN = (u64)Ql * N, LPN = (u64)QlP * N;
+    std::vector<std::pair<const u32 *, const KSwitchKey *>> items;
+    for (int st : steps) {
+        const u32 elt = galois_elt_from_step(st, N);
+        auto it = std::find(gk.elts.begin(), gk.elts.end(), elt);
+        require(it != gk.elts.end(), "Galois key not present in hoisting");
+        const KSwitchKey &key = gk.keys[it - gk.elts.begin()];
+        require(key.dnum >= beta, "key-switching key has too few digits for this level");
+        items.emplace_back(ctx.galois_perm(elt), &key);
+    }
+    u64 *t = static_cast<u64 *>(dev_alloc((size_t)beta * LPN * 8, s));
+    u64 *cx = static_cast<u64 *>(dev_alloc(2 * LPN * 8, s));
+    u64 *acc0 = static_cast<u64 *>(dev_alloc(LN * 8, s));
+    modup(ctx, ct.chain_index, ct.data + LN, t, s, true);
+    for (size_t i = 0; i < items.size(); ++i) {
+        const int acc = i > 0;
+        hipLaunchKernelGGL(k_permute_acc, grid_for(N, Ql), dim3(kThreads), 0, s, (const u64 *)ct.data, acc0,
+                           items[i].first, ctx.dev_mod(), (u32)N, acc);
+        hipLaunchKernelGGL(k_inner_product_perm_acc, grid_for(N, QlP), dim3(kThreads), 0, s, (const u64 *)t,
+                           items[i].first, (const u64 *)items[i].second->data, cx, ctx.dev_mod(), (u32)N, Ql, QlP,
+                           ctx.size_Q(), ctx.size_QP(), beta, acc);
+        PHX_HIP_CHECK(hipGetLastError());
+    }
+    // ct0 = acc0 + moddown(cx0), ct1 = moddown(cx1)
+    moddown(ctx, ct.chain_index, cx, ct.data, acc0, nullptr, 2, s);
+    dev_free(acc0, s);
+    dev_free(cx, s);
+    dev_free(t, s);
 }
 
 // apply_galois_inplace (src/evaluate.cu:1567-1630)

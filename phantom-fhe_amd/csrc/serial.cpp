@@ -1,0 +1,124 @@
+// serial.cpp -- the reference's ciphertext / key stream layouts (serial.h).
+#include "serial.h"
+
+#include <cstring>
+
+namespace phx {
+
+namespace {
+struct Header {
+    u64 chain_index, size, poly_degree, coeff_modulus_size;
+    double scale;
+    u64 correction_factor, noise_scale_deg;
+    uint8_t is_ntt_form, is_asymmetric;
+};
+
+void put_header(uint8_t *p, const Header &h) {
+    std::memcpy(p + 0, &h.chain_index, 8);
+    std::memcpy(p + 8, &h.size, 8);
+    std::memcpy(p + 16, &h.poly_degree, 8);
+    std::memcpy(p + 24, &h.coeff_modulus_size, 8);
+    std::memcpy(p + 32, &h.scale, 8);
+    std::memcpy(p + 40, &h.correction_factor, 8);
+    std::memcpy(p + 48, &h.noise_scale_deg, 8);
+    p[56] = h.is_ntt_form;
+    p[57] = h.is_asymmetric;
+}
+
+Header get_header(const uint8_t *p) {
+    Header h{};
+    std::memcpy(&h.chain_index, p + 0, 8);
+    std::memcpy(&h.size, p + 8, 8);
+    std::memcpy(&h.poly_degree, p + 16, 8);
+    std::memcpy(&h.coeff_modulus_size, p + 24, 8);
+    std::memcpy(&h.scale, p + 32, 8);
+    std::memcpy(&h.correction_factor, p + 40, 8);
+    std::memcpy(&h.noise_scale_deg, p + 48, 8);
+    h.is_ntt_form = p[56];
+    h.is_asymmetric = p[57];
+    return h;
+}
+}  // namespace
+
+size_t ciphertext_bytes(const Ciphertext &ct) {
+    return kCtHeaderBytes + (size_t)ct.size * ct.coeff_modulus_size * ct.poly_degree * 8;
+}
+
+void ciphertext_save(const Ciphertext &ct, uint8_t *buf, size_t bytes) {
+    require(ct.data != nullptr, "ciphertext is empty");
+    require(bytes >= ciphertext_bytes(ct), "buffer too small for the ciphertext");
+    put_header(buf, Header{ct.chain_index, ct.size, ct.poly_degree, ct.coeff_modulus_size, ct.scale,
+                           ct.correction_factor, ct.noise_scale_deg, (uint8_t)ct.is_ntt_form,
+                           (uint8_t)ct.is_asymmetric});
+    PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+    PHX_HIP_CHECK(hipMemcpy(buf + kCtHeaderBytes, ct.data, ciphertext_bytes(ct) - kCtHeaderBytes,
+                            hipMemcpyDeviceToHost));
+}
+
+size_t ciphertext_load(const Context &ctx, Ciphertext &ct, const uint8_t *buf, size_t bytes) {
+    require(bytes >= kCtHeaderBytes, "stream too short for a ciphertext header");
+    const Header h = get_header(buf);
+    require(h.poly_degree == ctx.N(), "ciphertext poly_modulus_degree does not match the context");
+    require(h.chain_index <= ctx.size_Q(), "ciphertext chain_index is invalid for the context");
+    require(h.coeff_modulus_size == ctx.limbs_at((u32)h.chain_index),
+            "ciphertext coeff_modulus_size does not match its chain_index");
+    require(h.size >= 1 && h.size <= 3, "ciphertext size must be 1..3");
+    const size_t n = (size_t)h.size * h.coeff_modulus_size * h.poly_degree;
+    require(bytes >= kCtHeaderBytes + n * 8, "stream too short for the ciphertext data");
+    ct.resize(ctx, (u32)h.chain_index, (u32)h.size, hipStreamPerThread);
+    ct.scale = h.scale;
+    ct.correction_factor = h.correction_factor;
+    ct.noise_scale_deg = (u32)h.noise_scale_deg;
+    ct.is_ntt_form = h.is_ntt_form != 0;
+    ct.is_asymmetric = h.is_asymmetric != 0;
+    PHX_HIP_CHECK(hipMemcpyAsync(ct.data, buf + kCtHeaderBytes, n * 8, hipMemcpyHostToDevice, hipStreamPerThread));
+    PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+    return kCtHeaderBytes + n * 8;
+}
+
+size_t kswitch_key_bytes(const Context &ctx, const KSwitchKey &key) {
+    return 8 + (size_t)key.dnum * (kCtHeaderBytes + (size_t)2 * ctx.size_QP() * ctx.N() * 8);
+}
+
+void kswitch_key_save(const Context &ctx, const KSwitchKey &key, uint8_t *buf, size_t bytes) {
+    require(key.data != nullptr, "key is empty");
+    require(bytes >= kswitch_key_bytes(ctx, key), "buffer too small for the key");
+    const u64 dnum = key.dnum, per = (u64)2 * ctx.size_QP() * ctx.N();
+    std::memcpy(buf, &dnum, 8);
+    uint8_t *p = buf + 8;
+    PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+    for (u64 j = 0; j < dnum; ++j) {
+        // each digit is a public key: chain 0 (the key level), size 2, all QP limbs
+        put_header(p, Header{0, 2, ctx.N(), ctx.size_QP(), 1.0, 1, 1, 1, 0});
+        PHX_HIP_CHECK(hipMemcpy(p + kCtHeaderBytes, key.data + j * per, per * 8, hipMemcpyDeviceToHost));
+        p += kCtHeaderBytes + per * 8;
+    }
+}
+
+size_t kswitch_key_load(const Context &ctx, KSwitchKey &key, const uint8_t *buf, size_t bytes) {
+    require(bytes >= 8, "stream too short for a key");
+    u64 dnum = 0;
+    std::memcpy(&dnum, buf, 8);
+    require(dnum >= 1 && dnum <= ctx.size_Q(), "key dnum is invalid for the context");
+    const u64 per = (u64)2 * ctx.size_QP() * ctx.N();
+    require(bytes >= 8 + dnum * (kCtHeaderBytes + per * 8), "stream too short for the key data");
+    u64 *d = static_cast<u64 *>(dev_alloc(dnum * per * 8, hipStreamPerThread));
+    const uint8_t *p = buf + 8;
+    for (u64 j = 0; j < dnum; ++j) {
+        const Header h = get_header(p);
+        if (h.poly_degree != ctx.N() || h.coeff_modulus_size != ctx.size_QP() || h.size != 2) {
+            dev_free(d, hipStreamPerThread);
+            fail(kInvalidArgument, "key digit does not match the context's key level");
+        }
+        PHX_HIP_CHECK(hipMemcpyAsync(d + j * per, p + kCtHeaderBytes, per * 8, hipMemcpyHostToDevice,
+                                     hipStreamPerThread));
+        p += kCtHeaderBytes + per * 8;
+    }
+    PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+    if (key.data) dev_free(key.data, hipStreamPerThread);
+    key.data = d;
+    key.dnum = (u32)dnum;
+    return (size_t)(p - buf);
+}
+
+}  // namespace phx

@@ -1,0 +1,29 @@
+// serial.h -- byte layouts of the reference's save/load streams.
+//
+// PhantomCiphertext::save (include/ciphertext.h:173-189): chain_index, size,
+// poly_modulus_degree, coeff_modulus_size (8-byte unsigned each), scale
+// (double), correction_factor (u64), noiseScaleDeg (8-byte unsigned),
+// is_ntt_form, is_asymmetric (one byte each), then the data [size][L][N] u64;
+// little-endian, no padding: 58 header bytes.
+// PhantomRelinKey::save (include/secretkey.h:129-147): dnum (8 bytes), then
+// dnum public keys, each a ciphertext (chain 0, size 2, QP limbs).
+// PhantomGaloisKey::save (include/secretkey.h:194-206): the number of keys
+// (8 bytes), then that many relinearisation keys.
+#pragma once
+
+#include "evaluator.h"
+
+namespace phx {
+
+constexpr size_t kCtHeaderBytes = 58;
+
+size_t ciphertext_bytes(const Ciphertext &ct);
+void ciphertext_save(const Ciphertext &ct, uint8_t *buf, size_t bytes);
+// consumes one ciphertext from buf; returns the bytes read
+size_t ciphertext_load(const Context &ctx, Ciphertext &ct, const uint8_t *buf, size_t bytes);
+
+size_t kswitch_key_bytes(const Context &ctx, const KSwitchKey &key);
+void kswitch_key_save(const Context &ctx, const KSwitchKey &key, uint8_t *buf, size_t bytes);
+size_t kswitch_key_load(const Context &ctx, KSwitchKey &key, const uint8_t *buf, size_t bytes);
+
+}  // namespace phx

@@ -77,6 +77,16 @@ _SIGS = {
     "phx_encrypt_symmetric": (ctypes.c_int, [_vp, _vp, _u64p, _sz, ctypes.c_uint64, _vp, _vp]),
     "phx_decrypt": (ctypes.c_int, [_vp, _vp, _vp, _u64p, _vp]),
     "phx_ciphertext_set_ntt_form": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "phx_hoisting_inplace": (ctypes.c_int, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _sz, _vp]),
+    "phx_ciphertext_save_size": (ctypes.c_int, [_vp, ctypes.POINTER(_sz)]),
+    "phx_ciphertext_save": (ctypes.c_int, [_vp, _vp, _sz]),
+    "phx_ciphertext_load": (ctypes.c_int, [_vp, _vp, _vp, _sz]),
+    "phx_relin_key_save_size": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_sz)]),
+    "phx_relin_key_save": (ctypes.c_int, [_vp, _vp, _vp, _sz]),
+    "phx_relin_key_load": (ctypes.c_int, [_vp, _vp, _sz, ctypes.POINTER(_vp)]),
+    "phx_galois_key_save_size": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_sz)]),
+    "phx_galois_key_save": (ctypes.c_int, [_vp, _vp, _vp, _sz]),
+    "phx_galois_key_load": (ctypes.c_int, [_vp, _u32p, _sz, _vp, _sz, ctypes.POINTER(_vp)]),
     "phx_server_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _sz, _sz,
                                          _sz, _sz, _vp, ctypes.POINTER(_vp)]),
     "phx_server_now": (ctypes.c_int, [_vp, _dp]),
@@ -279,6 +289,20 @@ class ciphertext:
     def set_scale(self, v):
         _check(lib().phx_ciphertext_set_scale(self._h, v))
 
+    def save(self):
+        """PhantomCiphertext::save byte stream"""
+        n = _sz()
+        _check(lib().phx_ciphertext_save_size(self._h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().phx_ciphertext_save(self._h, buf, n.value))
+        return buf.raw
+
+    @staticmethod
+    def load(ctx, data):
+        ct = ciphertext()
+        _check(lib().phx_ciphertext_load(ctx._h, ct._h, data, len(data)))
+        return ct
+
     def set_ntt_form(self, v):
         _check(lib().phx_ciphertext_set_ntt_form(self._h, int(bool(v))))
 
@@ -466,6 +490,23 @@ class relin_key:
         _check(lib().phx_relin_key_download(self._ctx._h, self._h, _p(out)))
         return out
 
+    def save(self):
+        """PhantomRelinKey::save byte stream"""
+        n = _sz()
+        _check(lib().phx_relin_key_save_size(self._ctx._h, self._h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().phx_relin_key_save(self._ctx._h, self._h, buf, n.value))
+        return buf.raw
+
+    @classmethod
+    def load(cls, ctx, data):
+        k = cls.__new__(cls)
+        h = _vp()
+        _check(lib().phx_relin_key_load(ctx._h, data, len(data), ctypes.byref(h)))
+        k._h, k._ctx = h, ctx
+        k.dnum = int.from_bytes(data[:8], "little")
+        return k
+
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.phx_relin_key_destroy(self._h)
@@ -486,7 +527,25 @@ class galois_key:
             _check(lib().phx_galois_key_create(ctx._h, sk._h, e.ctypes.data_as(_u32p), len(e), seed,
                                                ctypes.byref(h)))
         self._h = h
+        self._ctx = ctx
         self.elts = [int(x) for x in elts]
+
+    def save(self):
+        """PhantomGaloisKey::save byte stream"""
+        n = _sz()
+        _check(lib().phx_galois_key_save_size(self._ctx._h, self._h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().phx_galois_key_save(self._ctx._h, self._h, buf, n.value))
+        return buf.raw
+
+    @classmethod
+    def load(cls, ctx, elts, data):
+        g = cls.__new__(cls)
+        e = np.array(elts, np.uint32)
+        h = _vp()
+        _check(lib().phx_galois_key_load(ctx._h, e.ctypes.data_as(_u32p), len(e), data, len(data), ctypes.byref(h)))
+        g._h, g._ctx, g.elts = h, ctx, [int(x) for x in elts]
+        return g
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -530,6 +589,12 @@ def apply_galois_inplace(ctx, ct, elt, gk, stream=None):
 
 def rotate_inplace(ctx, ct, step, gk, stream=None):
     _check(lib().phx_rotate_inplace(ctx._h, ct._h, step, gk._h, _stream(stream)))
+
+
+def hoisting_inplace(ctx, ct, gk, steps, stream=None):
+    """hoisting_inplace: ct <- sum of the rotations of ct by steps (one shared modup)"""
+    st = (ctypes.c_int * len(steps))(*steps)
+    _check(lib().phx_hoisting_inplace(ctx._h, ct._h, gk._h, st, len(steps), _stream(stream)))
 
 
 def keyswitch_inplace(ctx, ct, c2_dev, key, is_relin=True, stream=None):

@@ -292,3 +292,24 @@ def test_fused_ntt_hot_path(G):
     finally:
         ph.tune_set("ntt_fused", old[0])
         ph.tune_set("ntt_fused_min_limbs", old[1])
+
+
+@pytest.mark.parametrize("N,bits,P,chain", [(1 << 12, [60, 50, 50, 50, 50, 50, 60, 60], 2, 2),
+                                            (1 << 16, [60] + [50] * 34 + [60] * 12, 12, 1)])
+def test_hoisting_bit_exact(N, bits, P, chain):
+    """hoisting_inplace: the sum of several rotations sharing one modup."""
+    ob.set_threads(16)
+    say(f"case: hoisting N={N} chain={chain}")
+    ctx, octx, mods = make_ctx(N, bits, P)
+    rng = np.random.default_rng(N + chain)
+    steps = [1, 2, -3, 7]
+    elts = [ph.galois_elt_from_step(s, N) for s in steps]
+    keys = np.stack([rand_key(rng, mods, N, ctx.dnum) for _ in elts])
+    gk = ph.galois_key(ctx, elts, host=keys)
+    Ql = ctx.limbs_at(chain)
+    a = rand_ct(rng, mods[:Ql], N)
+    c = upload_ct(ctx, a, chain)
+    ph.hoisting_inplace(ctx, c, gk, steps)
+    assert np.array_equal(c.download(N), octx.hoisting(Ql, a, elts, keys))
+    with pytest.raises(ValueError):
+        ph.hoisting_inplace(ctx, c, gk, [5])

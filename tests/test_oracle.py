@@ -389,3 +389,35 @@ def test_apply_galois_decrypts_rotated_message():
     for k in range(TOY_N):
         v, Q = br.crt([dec[i, k] for i in range(Ql)], mods[:Ql])
         assert abs(_centered(v, Q) - _centered(want[k], 1 << 200)) < (1 << 22)
+
+
+def test_hoisting_decrypts_sum_of_rotations():
+    """hoisting_inplace (src/evaluate.cu:1670-1865) returns the sum of the
+    rotations: decryption must give sum_i sigma_{g_i}(m) up to key-switching
+    noise, with one key per element built for sigma_{g_i}(s)."""
+    ctx, mods = toy_ctx()
+    rng = np.random.default_rng(43)
+    s = _toy_secret(rng)
+    steps = (1, 3, -2)
+    elts = [ob.galois_elt_from_step(st, TOY_N) for st in steps]
+    keys = []
+    for i, g in enumerate(elts):
+        sg = [_centered(v, 1 << 200) for v in br.galois_coeff([int(v) % (1 << 200) for v in s], g, 1 << 200)]
+        key, s_ntt = _toy_keys(ctx, mods, s, lambda _, sg=sg: sg, 50 + i)
+        keys.append(key)
+    Ql = 4
+    m = rng.integers(-1000, 1000, TOY_N)
+    a = rand_limbs(rng, mods[:Ql], TOY_N)
+    mm = ctx.ntt_forward(_rns(m, mods[:Ql]), 0)
+    c0 = np.array([[(-int(a[i, k]) * int(s_ntt[i, k]) + int(mm[i, k])) % q for k in range(TOY_N)]
+                   for i, q in enumerate(mods[:Ql])], np.uint64)
+    out = ctx.hoisting(Ql, np.stack([c0, a]), elts, np.stack(keys))
+    dec = ctx.ntt_inverse(np.array([[(int(out[0, i, k]) + int(out[1, i, k]) * int(s_ntt[i, k])) % q
+                                     for k in range(TOY_N)] for i, q in enumerate(mods[:Ql])], np.uint64), 0)
+    want = [0] * TOY_N
+    for g in elts:
+        rot = br.galois_coeff([int(v) % (1 << 200) for v in m], g, 1 << 200)
+        want = [w + _centered(r, 1 << 200) for w, r in zip(want, rot)]
+    for k in range(TOY_N):
+        v, Q = br.crt([dec[i, k] for i in range(Ql)], mods[:Ql])
+        assert abs(_centered(v, Q) - want[k]) < (1 << 22), k
