@@ -302,6 +302,8 @@ def main():
                     help="only the forward NTT over 47 x 16 limbs and the copy kernel (for counter runs)")
     ap.add_argument("--serve-qps", type=float, default=400.0)
     ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (kernel variants stay at the process-wide switches)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="kernel-variant switch (bconv_mfma, ntt_fp64, ntt_fused, ntt_chunk_limbs, ...)")
     args = ap.parse_args()
@@ -319,6 +321,10 @@ def main():
     N = 1 << 16
     ctx, mods = make_context(ph, N, CONFIG2_BITS, 12)
     stream = ph.stream_create()
+    autotune = None
+    if not args.no_autotune and not args.tune:
+        autotune = ctx.autotune()
+        log(f"bench: autotune {autotune}")
     if args.ntt_only:
         L = ctx.size_Q + ctx.size_P
         ntt_ms, ntt_min = bench_ntt(ph, ctx, N, L, args.ntt_polys, 10, stream)
@@ -347,7 +353,8 @@ def main():
     log(f"bench: copy kernel (1 GiB read + 1 GiB write): {copy_gbs:.1f} GB/s")
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "ntt_fwd_traffic.json")
-    if os.path.exists(tpath):
+    # the counter runs measured the two-launch form
+    if os.path.exists(tpath) and ctx.tuning()["ntt_fused"] in (-1, 0):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
 
     rot = bench_rotations(ph, ctx, mods, N, stream)
@@ -390,6 +397,7 @@ def main():
                                "dnum=3; batched path: one batch plan over all pairs of a step, replayed as a "
                                "captured hipGraph",
                    "batch_per_step": args.batch, "mode": "batched", "tune": tune},
+        "autotune": autotune,
         "plain_value": round(plain, 2),
         "plain_ms_per_step": round(ms_plain / args.steps, 4),
         "batched_over_plain": round(value / plain, 3),

@@ -1,0 +1,244 @@
+// autotune.hip -- picks the kernel variants of one context by timing them.
+//
+// Searched, in this order (coordinate descent from the process-wide switches):
+//   1. the NTT launch form: two launches per transform with pass-2 launches
+//      sliced to {all, 128, 256, 512} limbs, or one workgroup of G x 256
+//      threads per limb (G in {1, 2, 4} where N / 4096 is a multiple of G) for
+//      launches of at least {0, 64, 256} limbs;
+//   2. the FastBConv kernel: matrix cores or VALU.
+// The radix split of a transform is not searched: the 4096-coefficient tile
+// (16 registers x 256 lanes) fixes the second pass at 2^8 points per column
+// block and the first at N / 2^8.
+// The objective is one HMult+relin at the top level on synthetic residues
+// resident in HBM (two ciphertexts and a relinearisation key of uniform
+// residues), timed with HIP events: median of three runs of 8 repetitions
+// after a warm-up. The winner is applied to the context and, when a path is
+// given, stored in a JSON cache keyed by (scheme, N, size_Q, size_P), which
+// context creation reads when PHX_TUNE_CACHE names it.
+#include "autotune.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+
+#include "evaluator.h"
+#include "tune.h"
+
+namespace phx {
+
+namespace {
+constexpr int kFillThreads = 256;
+
+// d[p][l][n] = pseudo-random residue mod mod[l], for `polys` polynomials of L limbs
+__global__ void k_fill_residues(u64 *d, const DevModulus *mod, u32 N, u32 L, u32 polys, u64 seed) {
+    const u64 total = (u64)N * L * polys;
+    for (u64 i = (u64)blockIdx.x * kFillThreads + threadIdx.x; i < total; i += (u64)gridDim.x * kFillThreads) {
+        const u32 limb = (u32)((i / N) % L);
+        u64 x = (i ^ seed) * 0x9e3779b97f4a7c15ull;
+        x ^= x >> 29;
+        x *= 0xbf58476d1ce4e5b9ull;
+        x ^= x >> 32;
+        d[i] = x % mod[limb].value;
+    }
+}
+
+void fill(u64 *d, const Context &ctx, u32 L, u32 polys, u64 seed, hipStream_t s) {
+    const u64 total = ctx.N() * L * polys;
+    const u32 blocks = (u32)std::min<u64>((total + kFillThreads - 1) / kFillThreads, 4096);
+    hipLaunchKernelGGL(k_fill_residues, dim3(blocks), dim3(kFillThreads), 0, s, d, ctx.dev_mod(), (u32)ctx.N(), L,
+                       polys, seed);
+    PHX_HIP_CHECK(hipGetLastError());
+}
+
+class EventTimer {
+public:
+    EventTimer() {
+        PHX_HIP_CHECK(hipEventCreate(&a_));
+        PHX_HIP_CHECK(hipEventCreate(&b_));
+    }
+    ~EventTimer() {
+        (void)hipEventDestroy(a_);
+        (void)hipEventDestroy(b_);
+    }
+    // median over 3 runs of `reps` calls, after one warm-up call; microseconds per call
+    template <class F>
+    double median_us(hipStream_t s, int reps, F &&f) {
+        f();
+        double r[3];
+        for (double &x : r) {
+            PHX_HIP_CHECK(hipEventRecord(a_, s));
+            for (int i = 0; i < reps; ++i) f();
+            PHX_HIP_CHECK(hipEventRecord(b_, s));
+            PHX_HIP_CHECK(hipEventSynchronize(b_));
+            float ms = 0;
+            PHX_HIP_CHECK(hipEventElapsedTime(&ms, a_, b_));
+            x = 1e3 * ms / reps;
+        }
+        std::sort(r, r + 3);
+        return r[1];
+    }
+
+private:
+    hipEvent_t a_, b_;
+};
+
+std::string describe(const TuneChoice &c) {
+    std::ostringstream o;
+    o << "ntt_fused=" << c.ntt_fused << ",ntt_fused_min_limbs=" << c.ntt_fused_min_limbs
+      << ",ntt_chunk_limbs=" << c.ntt_chunk_limbs << ",bconv_mfma=" << c.bconv_mfma;
+    return o.str();
+}
+}  // namespace
+
+TuneResult autotune(Context &ctx, hipStream_t s) {
+    const u64 N = ctx.N();
+    const u32 Ql = ctx.size_Q(), QP = ctx.size_QP(), dnum = ctx.dnum();
+    const bool bfv = ctx.params().scheme == kSchemeBFV;
+    const TuneChoice saved = ctx.tuning();
+    TuneResult res;
+
+    Ciphertext c1, c2;
+    c1.resize(ctx, 1, 2, s);
+    c2.resize(ctx, 1, 2, s);
+    c1.is_ntt_form = c2.is_ntt_form = !bfv;
+    KSwitchKey rlk;
+    rlk.dnum = dnum;
+    rlk.data = static_cast<u64 *>(dev_alloc((size_t)dnum * 2 * QP * N * 8, s));
+    fill(c1.data, ctx, Ql, 2, 11, s);
+    fill(c2.data, ctx, Ql, 2, 12, s);
+    fill(rlk.data, ctx, QP, 2 * dnum, 13, s);
+
+    EventTimer timer;
+    auto time_choice = [&](const TuneChoice &c) {
+        ctx.set_tuning(c);
+        const double us = timer.median_us(s, 8, [&] { multiply_and_relin(ctx, c1, c2, rlk, s); });
+        res.trials.emplace_back(describe(c), us);
+        return us;
+    };
+
+    try {
+        res.default_us = time_choice(TuneChoice{});
+        // 1. NTT launch form, FastBConv left at the process-wide switch
+        TuneChoice best;
+        best.ntt_fused = 0;
+        best.ntt_fused_min_limbs = 0;
+        best.ntt_chunk_limbs = 0;
+        double best_us = time_choice(best);
+        std::vector<TuneChoice> cands;
+        for (int chunk : {128, 256, 512}) {
+            TuneChoice c = best;
+            c.ntt_chunk_limbs = chunk;
+            cands.push_back(c);
+        }
+        const u64 tiles = N / kNttTile;
+        for (int g : {1, 2, 4}) {
+            if (tiles % (u64)g) continue;
+            for (int min_limbs : {0, 64, 256}) {
+                TuneChoice c;
+                c.ntt_fused = g;
+                c.ntt_fused_min_limbs = min_limbs;
+                c.ntt_chunk_limbs = 0;
+                cands.push_back(c);
+            }
+        }
+        for (const TuneChoice &c : cands) {
+            const double us = time_choice(c);
+            if (us < best_us) best_us = us, best = c;
+        }
+        // 2. FastBConv kernel
+        for (int mf : {1, 0}) {
+            TuneChoice c = best;
+            c.bconv_mfma = mf;
+            const double us = time_choice(c);
+            if (best.bconv_mfma < 0 || us < best_us) best_us = us, best = c;
+        }
+        res.choice = best;
+        res.best_us = best_us;
+    } catch (...) {
+        ctx.set_tuning(saved);
+        c1.release(s);
+        c2.release(s);
+        dev_free(rlk.data, s);
+        throw;
+    }
+    ctx.set_tuning(res.choice);
+    c1.release(s);
+    c2.release(s);
+    dev_free(rlk.data, s);
+    PHX_HIP_CHECK(hipStreamSynchronize(s));
+    return res;
+}
+
+// ---- cache: {"entries": [ {...}, ... ]} with one entry per line ----
+
+std::string tune_cache_key(u32 scheme, u64 N, u32 size_Q, u32 size_P) {
+    std::ostringstream k;
+    k << "scheme=" << (scheme == kSchemeBFV ? "bfv" : "ckks") << ",N=" << N << ",Q=" << size_Q << ",P=" << size_P;
+    return k.str();
+}
+
+namespace {
+bool entry_field(const std::string &line, const char *name, int *v) {
+    const std::string f = std::string("\"") + name + "\": ";
+    const size_t p = line.find(f);
+    if (p == std::string::npos) return false;
+    char *end = nullptr;
+    const long x = std::strtol(line.c_str() + p + f.size(), &end, 10);
+    if (end == line.c_str() + p + f.size()) return false;
+    *v = (int)x;
+    return true;
+}
+}  // namespace
+
+void tune_cache_store(const std::string &path, const std::string &key, const TuneChoice &c, double us) {
+    require(!path.empty(), "empty tuning cache path");
+    const std::string tag = "\"key\": \"" + key + "\"";
+    std::vector<std::string> kept;
+    {
+        std::ifstream in(path);
+        std::string line;
+        while (std::getline(in, line)) {
+            if (line.find("\"key\": \"") == std::string::npos || line.find(tag) != std::string::npos) continue;
+            while (!line.empty() && (line.back() == ',' || line.back() == ' ')) line.pop_back();
+            kept.push_back(line);
+        }
+    }
+    std::ostringstream e;
+    e << "  {" << tag << ", \"ntt_fused\": " << c.ntt_fused << ", \"ntt_fused_min_limbs\": " << c.ntt_fused_min_limbs
+      << ", \"ntt_chunk_limbs\": " << c.ntt_chunk_limbs << ", \"bconv_mfma\": " << c.bconv_mfma
+      << ", \"hmult_relin_us\": " << us << "}";
+    kept.push_back(e.str());
+    std::ofstream out(path, std::ios::trunc);
+    require((bool)out, "cannot write the tuning cache " + path);
+    out << "{\"entries\": [\n";
+    for (size_t i = 0; i < kept.size(); ++i) out << kept[i] << (i + 1 < kept.size() ? ",\n" : "\n");
+    out << "]}\n";
+    require((bool)out, "cannot write the tuning cache " + path);
+}
+
+bool tune_cache_lookup(const std::string &path, const std::string &key, TuneChoice *out) {
+    std::ifstream in(path);
+    if (!in) return false;
+    const std::string tag = "\"key\": \"" + key + "\"";
+    std::string line;
+    while (std::getline(in, line)) {
+        if (line.find(tag) == std::string::npos) continue;
+        TuneChoice c;
+        if (!entry_field(line, "ntt_fused", &c.ntt_fused) ||
+            !entry_field(line, "ntt_fused_min_limbs", &c.ntt_fused_min_limbs) ||
+            !entry_field(line, "ntt_chunk_limbs", &c.ntt_chunk_limbs) ||
+            !entry_field(line, "bconv_mfma", &c.bconv_mfma))
+            fail(kInvalidArgument, "malformed tuning cache entry for " + key + " in " + path);
+        *out = c;
+        return true;
+    }
+    return false;
+}
+
+bool tune_cache_load(const std::string &path, const Context &ctx, TuneChoice *out) {
+    return tune_cache_lookup(path, tune_cache_key((u32)ctx.params().scheme, ctx.N(), ctx.size_Q(), ctx.size_P()),
+                             out);
+}
+
+}  // namespace phx

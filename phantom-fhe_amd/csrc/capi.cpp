@@ -6,6 +6,7 @@
 
 #include "batch.h"
 #include "evaluator.h"
+#include "autotune.h"
 #include "tune.h"
 #include "ckks.h"
 #include "serve.h"
@@ -968,6 +969,89 @@ int phx_tune_get(const char *key, int *value) {
     return guard([&] {
         nn(value, "value");
         require(tune_get(key, value), std::string("unknown tuning key: ") + (key ? key : "(null)"));
+    });
+}
+
+namespace {
+TuneChoice choice_of(const int c[4]) {
+    TuneChoice t;
+    t.ntt_fused = c[0];
+    t.ntt_fused_min_limbs = c[1];
+    t.ntt_chunk_limbs = c[2];
+    t.bconv_mfma = c[3];
+    return t;
+}
+void put_choice(const TuneChoice &t, int c[4]) {
+    c[0] = t.ntt_fused;
+    c[1] = t.ntt_fused_min_limbs;
+    c[2] = t.ntt_chunk_limbs;
+    c[3] = t.bconv_mfma;
+}
+}  // namespace
+
+int phx_context_get_tuning(const phx_context *ctx, int choice[4]) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(choice, "choice");
+        put_choice(ctx->ctx->tuning(), choice);
+    });
+}
+
+int phx_context_set_tuning(phx_context *ctx, const int choice[4]) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(choice, "choice");
+        ctx->ctx->set_tuning(choice_of(choice));
+    });
+}
+
+int phx_autotune(phx_context *ctx, const char *cache_path, phx_tune_report *report, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        const TuneResult r = autotune(*ctx->ctx, S(s));
+        const Context &c = *ctx->ctx;
+        if (cache_path && *cache_path)
+            tune_cache_store(cache_path, tune_cache_key((u32)c.params().scheme, c.N(), c.size_Q(), c.size_P()),
+                             r.choice, r.best_us);
+        if (report) {
+            put_choice(r.choice, report->choice);
+            report->default_us = r.default_us;
+            report->best_us = r.best_us;
+            report->trials = (uint32_t)r.trials.size();
+        }
+    });
+}
+
+int phx_tune_load(phx_context *ctx, const char *cache_path, int *found) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(cache_path, "cache_path");
+        TuneChoice t;
+        const bool ok = tune_cache_load(cache_path, *ctx->ctx, &t);
+        if (ok) ctx->ctx->set_tuning(t);
+        if (found) *found = ok ? 1 : 0;
+    });
+}
+
+int phx_tune_cache_store(const char *cache_path, int scheme, uint64_t N, size_t size_Q, size_t size_P,
+                         const int choice[4], double hmult_relin_us) {
+    return guard([&] {
+        nn(cache_path, "cache_path");
+        nn(choice, "choice");
+        tune_cache_store(cache_path, tune_cache_key((u32)scheme, N, (u32)size_Q, (u32)size_P), choice_of(choice),
+                         hmult_relin_us);
+    });
+}
+
+int phx_tune_cache_lookup(const char *cache_path, int scheme, uint64_t N, size_t size_Q, size_t size_P,
+                          int choice[4], int *found) {
+    return guard([&] {
+        nn(cache_path, "cache_path");
+        nn(choice, "choice");
+        TuneChoice t;
+        const bool ok = tune_cache_lookup(cache_path, tune_cache_key((u32)scheme, N, (u32)size_Q, (u32)size_P), &t);
+        if (ok) put_choice(t, choice);
+        if (found) *found = ok ? 1 : 0;
     });
 }
 

@@ -7,6 +7,7 @@
 #include <mutex>
 #include <thread>
 
+#include "autotune.h"
 #include "tune.h"
 
 namespace phx {
@@ -238,6 +239,7 @@ Context::Context(const Params &p) : params_(p) {
     tables_.inv_consts = reinterpret_cast<const u64 *>(db + 2 * twb + 2 * twdb + modb);
     tables_.logN = logN_;
     tables_.N = (u32)N_;
+    set_tuning(TuneChoice{});
 
     // ---- per-level key-switching constants ----
     // reference: DRNSTool ctor, hybrid key-switching part (src/rns.cu:97-195)
@@ -562,6 +564,11 @@ Context::Context(const Params &p) : params_(p) {
     levels_[0].chain_index = 0;
     levels_[0].size_Ql = size_Q_;
     levels_[0].size_QlP = size_QP_;
+    // kernel variants from a tuning cache written by the autotuner, if one is named
+    if (const char *path = std::getenv("PHX_TUNE_CACHE")) {
+        TuneChoice t;
+        if (*path && tune_cache_load(path, *this, &t)) set_tuning(t);
+    }
 }
 
 Context::~Context() {
@@ -569,6 +576,17 @@ Context::~Context() {
     for (auto &g : galois_tables_) (void)hipFree(g.second);
     if (cbuf_) (void)hipFree(cbuf_);
     if (tw_buf_) (void)hipFree(tw_buf_);
+}
+
+void Context::set_tuning(const TuneChoice &t) {
+    require(t.ntt_fused == -1 || t.ntt_fused == 0 || t.ntt_fused == 1 || t.ntt_fused == 2 || t.ntt_fused == 4,
+            "ntt_fused must be -1, 0, 1, 2 or 4");
+    require(t.bconv_mfma >= -1 && t.bconv_mfma <= 1, "bconv_mfma must be -1, 0 or 1");
+    require(t.ntt_chunk_limbs >= -1 && t.ntt_fused_min_limbs >= -1, "limb counts must be -1 or non-negative");
+    tune_ = t;
+    tables_.fused_g = t.ntt_fused;
+    tables_.fused_min = t.ntt_fused_min_limbs;
+    tables_.chunk_limbs = t.ntt_chunk_limbs;
 }
 
 const LevelConsts &Context::level(u32 chain_index) const {

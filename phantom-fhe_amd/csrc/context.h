@@ -115,6 +115,15 @@ struct HpsConsts {
     const u64 *alphar_q;        // [nr+1][k]  v R mod q_i
 };
 
+// Kernel variants chosen for one context (by the autotuner or a tuning cache);
+// -1 = follow the process-wide switch of the same name.
+struct TuneChoice {
+    int ntt_fused = -1;          // 0: two-pass NTT; G in {1, 2, 4}: one workgroup of G x 256 per limb
+    int ntt_fused_min_limbs = -1;
+    int ntt_chunk_limbs = -1;    // limbs per two-pass launch pair (0 = one launch)
+    int bconv_mfma = -1;         // 1: FastBConv on the matrix cores, 0: VALU
+};
+
 class Context {
 public:
     explicit Context(const Params &p);
@@ -152,7 +161,12 @@ public:
     const u32 *galois_perm(u32 elt) const;
     const std::vector<u32> &galois_elts() const { return params_.galois_elts; }
 
+    // kernel variants for this context; set_tuning applies them to later launches
+    const TuneChoice &tuning() const { return tune_; }
+    void set_tuning(const TuneChoice &t);
+
 private:
+    TuneChoice tune_{};
     Params params_;
     u64 N_ = 0;
     u32 logN_ = 0, size_Q_ = 0, size_P_ = 0, size_QP_ = 0;

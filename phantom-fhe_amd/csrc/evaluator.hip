@@ -207,7 +207,10 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
 namespace {
 // FastBConv dispatch: matrix cores when every job has an MFMA form, VALU otherwise
 // (the bconv_mfma switch set to 0 forces the VALU kernel).
-bool bconv_force_valu() { return tune_value("bconv_mfma") == 0; }
+bool bconv_force_valu(const Context &ctx) {
+    const int c = ctx.tuning().bconv_mfma;
+    return (c >= 0 ? c : tune_value("bconv_mfma")) == 0;
+}
 
 }  // namespace
 
@@ -215,7 +218,7 @@ void bconv_run(const Context &ctx, const std::vector<BconvDesc> &descs_host, con
                hipStream_t s) {
     const u64 N = ctx.N();
     u32 rt_max = 0, no_max = 0;
-    bool mfma = !bconv_force_valu() && N % kBconvTileCols == 0;
+    bool mfma = !bconv_force_valu(ctx) && N % kBconvTileCols == 0;
     for (const auto &d : descs_host) {
         if (d.rt == 0) mfma = false;
         rt_max = std::max(rt_max, d.rt);

@@ -439,19 +439,20 @@ __global__ __launch_bounds__(kNttThreads *G) void k_inv_fused(NttTables T, LimbM
 // ---------------------------------------------------------------------------
 namespace {
 // polynomials per launch pair: slices of about ntt_chunk_limbs limbs (0 = one launch)
-u32 chunk_polys(u32 nlimbs, u32 npolys) {
-    const int c = tune_value("ntt_chunk_limbs");
+u32 chunk_polys(const NttTables &T, u32 nlimbs, u32 npolys) {
+    const int c = T.chunk_limbs >= 0 ? T.chunk_limbs : tune_value("ntt_chunk_limbs");
     if (c <= 0) return npolys;
     return std::max<u32>(1, (u32)c / std::max<u32>(1, nlimbs));
 }
 
 // workgroup groups of the fused form for this launch, 0 = two passes
 u32 fused_groups(const NttTables &T, u32 nlimbs, u32 npolys) {
-    const int g = tune_value("ntt_fused");
+    const int g = T.fused_g >= 0 ? T.fused_g : tune_value("ntt_fused");
     if (g != 1 && g != 2 && g != 4) return 0;
     const u32 tiles = T.N / kNttTile;
     if (tiles % (u32)g) return 0;
-    if ((u64)nlimbs * npolys < (u64)std::max(0, tune_value("ntt_fused_min_limbs"))) return 0;
+    const int min_limbs = T.fused_min >= 0 ? T.fused_min : tune_value("ntt_fused_min_limbs");
+    if ((u64)nlimbs * npolys < (u64)std::max(0, min_limbs)) return 0;
     return (u32)g;
 }
 
@@ -505,7 +506,7 @@ void launch_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys
     // tiles per limb, on the same XCD), so the prime's twiddles stay in L2;
     // large launches go in slices of whole polynomials so that the pass-1
     // output of a slice is still in the last-level cache when pass 2 reads it
-    const u32 step = chunk_polys(nlimbs, npolys);
+    const u32 step = chunk_polys(T, nlimbs, npolys);
     for (u32 p0 = 0; p0 < npolys; p0 += step) {
         LimbMap Mc = M;
         Mc.poly0 = M.poly0 + p0;
@@ -535,7 +536,7 @@ void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, h
         return;
     }
     // launch order and slicing as in launch_forward
-    const u32 step = chunk_polys(nlimbs, npolys);
+    const u32 step = chunk_polys(T, nlimbs, npolys);
     for (u32 p0 = 0; p0 < npolys; p0 += step) {
         LimbMap Mc = M;
         Mc.poly0 = M.poly0 + p0;

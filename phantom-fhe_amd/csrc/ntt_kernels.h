@@ -71,6 +71,12 @@ struct NttTables {
     const u64 *inv_consts;   // [rows][4]: n^-1, shoup, n^-1 * itw[1], shoup
     u32 logN;
     u32 N;
+    // launch shape chosen for this context by the autotuner; -1 = follow the
+    // process-wide switch of the same name (ntt_fused, ntt_fused_min_limbs,
+    // ntt_chunk_limbs)
+    int fused_g;
+    int fused_min;
+    int chunk_limbs;
 };
 
 // ---- register rounds ------------------------------------------------------

@@ -137,6 +137,14 @@ _SIGS = {
     "phx_batch_replay": (ctypes.c_int, [_vp, _vp]),
     "phx_batch_scratch_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(_sz)]),
     "phx_batch_destroy": (ctypes.c_int, [_vp]),
+    "phx_context_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    "phx_context_set_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    "phx_autotune": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp, _vp]),
+    "phx_tune_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
+    "phx_tune_cache_store": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64, _sz, _sz,
+                                            ctypes.POINTER(ctypes.c_int), ctypes.c_double]),
+    "phx_tune_cache_lookup": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64, _sz, _sz,
+                                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "phx_tune_set": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "phx_tune_get": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
 }
@@ -245,6 +253,35 @@ class context:
         out = _sz()
         _check(lib().phx_context_limbs_at(self._h, chain_index, ctypes.byref(out)))
         return out.value
+
+    def tuning(self):
+        """kernel variants of this context (-1 = the process-wide switch)"""
+        c = (ctypes.c_int * 4)()
+        _check(lib().phx_context_get_tuning(self._h, c))
+        return dict(zip(TUNE_KEYS, list(c)))
+
+    def set_tuning(self, **choice):
+        cur = self.tuning()
+        for k, v in choice.items():
+            if k not in cur:
+                raise ValueError(f"unknown tuning key: {k}")
+            cur[k] = int(v)
+        c = (ctypes.c_int * 4)(*[cur[k] for k in TUNE_KEYS])
+        _check(lib().phx_context_set_tuning(self._h, c))
+
+    def autotune(self, cache_path=None, stream=None):
+        """times the kernel variants on HMult+relin at the top level, applies the
+        fastest and (with cache_path) stores it; returns the report"""
+        r = _TuneReport()
+        _check(lib().phx_autotune(self._h, cache_path.encode() if cache_path else None, ctypes.byref(r),
+                                  _stream(stream)))
+        return dict(choice=dict(zip(TUNE_KEYS, list(r.choice))), default_us=r.default_us, best_us=r.best_us,
+                    trials=r.trials)
+
+    def tune_load(self, cache_path):
+        found = ctypes.c_int()
+        _check(lib().phx_tune_load(self._h, cache_path.encode(), ctypes.byref(found)))
+        return bool(found.value)
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -737,6 +774,28 @@ class Event:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.phx_event_destroy(self._h)
             self._h = None
+
+
+TUNE_KEYS = ("ntt_fused", "ntt_fused_min_limbs", "ntt_chunk_limbs", "bconv_mfma")
+
+
+class _TuneReport(ctypes.Structure):
+    _fields_ = [("choice", ctypes.c_int * 4), ("default_us", ctypes.c_double), ("best_us", ctypes.c_double),
+                ("trials", ctypes.c_uint32)]
+
+
+def tune_cache_store(path, scheme, N, size_Q, size_P, choice, us=0.0):
+    """writes one autotuner cache entry (host only)"""
+    c = (ctypes.c_int * 4)(*[int(choice[k]) for k in TUNE_KEYS])
+    _check(lib().phx_tune_cache_store(path.encode(), scheme, N, size_Q, size_P, c, float(us)))
+
+
+def tune_cache_lookup(path, scheme, N, size_Q, size_P):
+    """the cached choice for a shape, or None (host only)"""
+    c = (ctypes.c_int * 4)()
+    found = ctypes.c_int()
+    _check(lib().phx_tune_cache_lookup(path.encode(), scheme, N, size_Q, size_P, c, ctypes.byref(found)))
+    return dict(zip(TUNE_KEYS, list(c))) if found.value else None
 
 
 def tune_set(key, value):

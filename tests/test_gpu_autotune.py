@@ -1,0 +1,92 @@
+"""The autotuner on the GPU: it times the kernel variants on HMult+relin,
+applies a valid choice to the context, stores it in the cache, a context
+created with PHX_TUNE_CACHE naming that cache applies it, and results under
+every variant it may pick stay bit-exact against the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import pyphantom as ph
+from gpu_helpers import make_ctx, rand_ct, rand_key, say
+
+pytestmark = pytest.mark.gpu
+
+BITS = [60] + [50] * 6 + [60, 60]
+
+
+def check_mul_relin(ctx, octx, mods, seed):
+    N = ctx.N
+    Ql = ctx.size_Q
+    rng = np.random.default_rng(seed)
+    a, b = rand_ct(rng, mods[:Ql], N), rand_ct(rng, mods[:Ql], N)
+    key = rand_key(rng, mods, N, ctx.dnum)
+    rk = ph.relin_key(ctx, host=key)
+    c1 = ph.ciphertext(ctx, 1, 2)
+    c1.upload(a)
+    c2 = ph.ciphertext(ctx, 1, 2)
+    c2.upload(b)
+    ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
+    assert np.array_equal(c1.download(N), octx.multiply_and_relin(Ql, a, b, key))
+
+
+def test_autotune_applies_and_caches(tmp_path):
+    ob.set_threads(8)
+    N = 1 << 14
+    say("case: autotune N=2^14 L=9")
+    ctx, octx, mods = make_ctx(N, BITS, 2)
+    assert ctx.tuning() == dict(ntt_fused=-1, ntt_fused_min_limbs=-1, ntt_chunk_limbs=-1, bconv_mfma=-1)
+    path = str(tmp_path / "tune.json")
+    rep = ctx.autotune(path)
+    say(f"autotune: {rep}")
+    ch = rep["choice"]
+    assert ch["ntt_fused"] in (0, 1, 2, 4) and ch["bconv_mfma"] in (0, 1)
+    assert ch["ntt_chunk_limbs"] >= 0 and ch["ntt_fused_min_limbs"] >= 0
+    assert rep["trials"] >= 8 and rep["best_us"] > 0 and rep["default_us"] > 0
+    # the winner is no slower than the default beyond timing noise
+    assert rep["best_us"] <= 1.25 * rep["default_us"]
+    assert ctx.tuning() == ch
+    assert ph.tune_cache_lookup(path, ph.SCHEME_CKKS, N, ctx.size_Q, ctx.size_P) == ch
+    check_mul_relin(ctx, octx, mods, 1)
+    # a new context of the same shape picks the entry up at creation
+    os.environ["PHX_TUNE_CACHE"] = path
+    try:
+        ctx2, _, _ = make_ctx(N, BITS, 2)
+    finally:
+        del os.environ["PHX_TUNE_CACHE"]
+    assert ctx2.tuning() == ch
+    ctx3, _, _ = make_ctx(N, BITS, 2)
+    assert ctx3.tuning()["ntt_fused"] == -1
+    assert ctx3.tune_load(path) and ctx3.tuning() == ch
+
+
+@pytest.mark.parametrize("choice", [
+    dict(ntt_fused=0, ntt_fused_min_limbs=0, ntt_chunk_limbs=128, bconv_mfma=1),
+    dict(ntt_fused=1, ntt_fused_min_limbs=0, ntt_chunk_limbs=0, bconv_mfma=0),
+    dict(ntt_fused=4, ntt_fused_min_limbs=0, ntt_chunk_limbs=0, bconv_mfma=1),
+])
+def test_per_context_choice_bit_exact(choice):
+    """a context's own choice drives its launches (process-wide switches untouched)"""
+    N = 1 << 14
+    say(f"case: per-context tuning {choice}")
+    ctx, octx, mods = make_ctx(N, BITS, 2)
+    ctx.set_tuning(**choice)
+    before = {k: ph.tune_get(k) for k in ("ntt_fused", "ntt_chunk_limbs", "bconv_mfma")}
+    check_mul_relin(ctx, octx, mods, 2)
+    x = np.stack([np.random.default_rng(3).integers(0, q, N, dtype=np.uint64) for q in mods])
+    buf = ph.DeviceBuffer.from_array(x)
+    ph.ntt_forward(ctx, buf, len(mods), 0)
+    ph.device_synchronize()
+    assert np.array_equal(buf.to_array(x.shape), octx.ntt_forward(x, 0))
+    assert {k: ph.tune_get(k) for k in before} == before
+
+
+def test_set_tuning_rejects():
+    ctx, _, _ = make_ctx(1 << 12, [60, 50, 60], 1)
+    with pytest.raises(ValueError):
+        ctx.set_tuning(ntt_fused=3)
+    with pytest.raises(ValueError):
+        ctx.set_tuning(bconv_mfma=2)
+    with pytest.raises(ValueError):
+        ctx.set_tuning(colour=1)
