@@ -104,6 +104,68 @@ def bench_hmult_batched(ph, ctx, mods, N, batch, steps, warmup, stream):
     return ev0.elapsed_ms(ev1), bt.scratch_bytes()
 
 
+def bench_bfv_config4(ph, stream, n_ct=64, reps=3):
+    """BFV multiply + relin (BEHZ) over 64 ciphertexts with N uniform in
+    {2^13, 2^14, 2^15} (4/8/16 limbs of 54-bit primes, t = 65537, P = 1): one
+    multiply_and_relin_inplace call per ciphertext against one plan per ring
+    degree joined into one captured graph."""
+    limbs = {1 << 13: 4, 1 << 14: 8, 1 << 15: 16}
+    rng = np.random.default_rng(404)
+    Ns = [int(x) for x in rng.choice(list(limbs), n_ct)]
+    ctxs, rks = {}, {}
+    for N, k in limbs.items():
+        mods = ph.coeff_modulus_create(N, [54] * k + [60])
+        parms = ph.EncryptionParameters(ph.SCHEME_BFV)
+        parms.set_poly_modulus_degree(N)
+        parms.set_coeff_modulus(mods)
+        parms.set_special_modulus_size(1)
+        parms.set_plain_modulus(65537)
+        parms.set_mul_tech(ph.MUL_TECH_BEHZ)
+        ctx = ph.context(parms)
+        key = np.stack([np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in mods])
+                                  for _ in range(2)]) for _ in range(ctx.dnum)])
+        ctxs[N] = (ctx, mods)
+        rks[N] = ph.relin_key(ctx, host=key)
+    a, b = [], []
+    for N in Ns:
+        ctx, mods = ctxs[N]
+        for lst in (a, b):
+            c = ph.ciphertext(ctx, 1, 2)
+            c.upload(np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in mods[:limbs[N]]])
+                               for _ in range(2)]))
+            lst.append(c)
+    ev0, ev1 = ph.Event(), ph.Event()
+    for i, N in enumerate(Ns):      # warm-up
+        ph.multiply_and_relin_inplace(ctxs[N][0], a[i], b[i], rks[N], stream)
+    ph.synchronize(stream)
+    ev0.record(stream)
+    for _ in range(reps):
+        for i, N in enumerate(Ns):
+            ph.multiply_and_relin_inplace(ctxs[N][0], a[i], b[i], rks[N], stream)
+    ev1.record(stream)
+    ph.synchronize(stream)
+    plain = reps * n_ct / (ev0.elapsed_ms(ev1) / 1e3)
+    parts = []
+    for N in limbs:
+        idx = [i for i, n in enumerate(Ns) if n == N]
+        if idx:
+            parts.append(ph.batch.multiply_and_relin(ctxs[N][0], [a[i] for i in idx], [b[i] for i in idx], rks[N]))
+    jb = ph.batch.join(parts)
+    jb.capture(stream)
+    jb.replay(stream)
+    ph.synchronize(stream)
+    ev0.record(stream)
+    for _ in range(reps):
+        jb.replay(stream)
+    ev1.record(stream)
+    ph.synchronize(stream)
+    batched = reps * n_ct / (ev0.elapsed_ms(ev1) / 1e3)
+    return {"workload": f"{n_ct} BFV ciphertexts (BEHZ), N uniform in 2^13..2^15 with 4/8/16 limbs, t=65537",
+            "per_N": {str(N): Ns.count(N) for N in limbs},
+            "plain_op_per_s": round(plain, 1), "batched_op_per_s": round(batched, 1),
+            "batched_over_plain": round(batched / plain, 3), "scratch_GiB": round(jb.scratch_bytes() / 2**30, 2)}
+
+
 ROT_STEPS = [1, 2, 3, 4, 5, 7, 8, 16, 32, 64, 128, 256, 512, 1024, -1, -5]
 
 
@@ -359,6 +421,8 @@ def main():
 
     rot = bench_rotations(ph, ctx, mods, N, stream)
     log(f"bench: rotations {rot}")
+    bfv4 = bench_bfv_config4(ph, stream)
+    log(f"bench: bfv config4 {bfv4}")
     serving = None
     if args.serve_requests > 0:
         lanes = []
@@ -411,6 +475,7 @@ def main():
         "cpu_baseline": cpu,
         "ckks_error_vs_fp64": ckks_err,
         "rotations": rot,
+        "bfv_config4": bfv4,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

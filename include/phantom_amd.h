@@ -187,13 +187,21 @@ int phx_keyswitch_inplace(const phx_context *ctx, phx_ciphertext *ct, const uint
  * capture() records the launches into a hipGraph on an explicit (non-NULL) stream and
  * replay() launches that graph. Results are bit-identical to the one-at-a-time calls. */
 typedef struct phx_batch phx_batch;
-/* ct1[i] = relin(ct1[i] * ct2[i]); ct1[i] may equal ct2[i] (squaring) */
+/* ct1[i] = relin(ct1[i] * ct2[i]); ct1[i] may equal ct2[i] (squaring).
+ * CKKS: NTT form, any level. BFV: coefficient form at the top level, multiplied with
+ * the context's technique (BEHZ or HPS) as in multiply_inplace + relinearize_inplace
+ * (src/evaluate.cu:440-801, 1342-1374). */
 int phx_batch_multiply_and_relin_create(const phx_context *ctx, phx_ciphertext *const *ct1,
                                         const phx_ciphertext *const *ct2, size_t n, const phx_relin_key *rlk,
                                         phx_batch **out);
 /* cts[i] = rotate(cts[i], steps[i]); each step's Galois element must have a key in gk */
 int phx_batch_rotate_create(const phx_context *ctx, phx_ciphertext *const *cts, const int *steps, size_t n,
                             const phx_galois_key *gk, phx_batch **out);
+/* one batch that runs `parts` in order (one graph when captured) with one scratch arena
+ * of the largest part's size; parts may belong to different contexts (ring degrees).
+ * Takes ownership: the parts' handles are released and must not be used again; on
+ * error they are left untouched. */
+int phx_batch_join(phx_batch *const *parts, size_t n, phx_batch **out);
 int phx_batch_run(phx_batch *batch, phx_stream stream);
 int phx_batch_capture(phx_batch *batch, phx_stream stream);
 int phx_batch_replay(phx_batch *batch, phx_stream stream);

@@ -4,6 +4,7 @@
 #include <memory>
 
 #include "batch.h"
+#include "bfv.h"
 #include "tune.h"
 
 namespace phx {
@@ -54,11 +55,13 @@ __global__ __launch_bounds__(kThreads) void k_permute_batch(const u64 *const *ct
 // key inner product over a group (key_switch_inner_prod_c2_and_evk,
 // src/eval_key_switch.cu:14-69): items sharing a key read it once per coefficient;
 // digit j's own limbs (pos / alpha == j) are read from c2 (NTT form) directly.
+// MB = the digit capacity (registers caching a shared key's digits)
+template <u32 MB>
 __global__ __launch_bounds__(kThreads) void k_inner_product_batch(const u64 *__restrict__ t, const u64 *__restrict__ c2,
                                                                   const u64 *const *keys, u64 *__restrict__ cx,
                                                                   const DevModulus *__restrict__ mod, u32 N, u32 Ql,
                                                                   u32 QlP, u32 size_Q, u32 QP, u32 beta, u32 alpha,
-                                                                  u32 B) {
+                                                                  u32 B, u64 c2_stride) {
     const u32 pos = blockIdx.y;
     const u32 own = pos < Ql ? pos / alpha : ~0u;
     const u32 k = blockIdx.x * kThreads + threadIdx.x;
@@ -66,7 +69,7 @@ __global__ __launch_bounds__(kThreads) void k_inner_product_batch(const u64 *__r
     const u32 row = pos < Ql ? pos : size_Q + (pos - Ql);
     const u64 QPN = (u64)QP * N;
     const DevModulus m = mod[row];
-    constexpr u32 kMaxBeta = 8;
+    constexpr u32 kMaxBeta = MB;
     u64 kb[kMaxBeta], ka[kMaxBeta];
     const u64 *cur = nullptr;
     for (u32 item = 0; item < B; ++item) {
@@ -82,7 +85,7 @@ __global__ __launch_bounds__(kThreads) void k_inner_product_batch(const u64 *__r
                 }
         }
         const u64 *ti = t + (u64)item * beta * QlP * N + (u64)pos * N + k;
-        const u64 *ci = c2 + (u64)item * Ql * N + (u64)pos * N + k;
+        const u64 *ci = c2 + (u64)item * c2_stride + (u64)pos * N + k;
         u128 a0 = 0, a1 = 0;
 #pragma unroll
         for (u32 j = 0; j < kMaxBeta; ++j)
@@ -96,6 +99,28 @@ __global__ __launch_bounds__(kThreads) void k_inner_product_batch(const u64 *__r
         co[(u64)QlP * N] = barrett128(a1, m.value, m.ratio_lo, m.ratio_hi);
     }
 }
+
+// dst[z][0 .. words) = src_z[0 .. words) for item z = blockIdx.y
+__global__ __launch_bounds__(kThreads) void k_gather_items(const u64 *const *src, u64 *__restrict__ dst, u64 words,
+                                                           u64 dst_stride) {
+    const u64 *a = src[blockIdx.y];
+    u64 *d = dst + blockIdx.y * dst_stride;
+    for (u64 i = (u64)blockIdx.x * kThreads + threadIdx.x; i < words; i += (u64)gridDim.x * kThreads) d[i] = a[i];
+}
+
+// dst_z[0 .. words) = src[z][0 .. words)
+__global__ __launch_bounds__(kThreads) void k_scatter_items(u64 *const *dst, const u64 *__restrict__ src, u64 words,
+                                                            u64 src_stride) {
+    u64 *d = dst[blockIdx.y];
+    const u64 *a = src + blockIdx.y * src_stride;
+    for (u64 i = (u64)blockIdx.x * kThreads + threadIdx.x; i < words; i += (u64)gridDim.x * kThreads) d[i] = a[i];
+}
+
+namespace {
+dim3 items_grid(u64 words, u32 items) {
+    return dim3((u32)std::min<u64>((words + kThreads - 1) / kThreads, 1024), items);
+}
+}  // namespace
 
 // ---------------------------------------------------------------------------
 BatchPlan::~BatchPlan() {
@@ -137,7 +162,10 @@ size_t BatchPlan::item_scratch_bytes(const Context &ctx, u32 chain, Kind k) {
 }
 
 size_t BatchPlan::group_scratch_words(const Group &g) const {
-    return (size_t)g.B * item_words(ctx_.N(), g.Ql, g.QlP, g.beta, kind_ == kRotate);
+    size_t w = (size_t)g.B * item_words(ctx_.N(), g.Ql, g.QlP, g.beta, kind_ == kRotate);
+    if (kind_ == kBfvMulRelin)   // staged operands, product, multiply work
+        w += (size_t)g.B * (7ull * g.Ql * ctx_.N() + bfv_batch_work_words(ctx_));
+    return w;
 }
 
 void BatchPlan::set_tables(Group &g, const std::vector<const void *> &a, const std::vector<const void *> &b,
@@ -146,6 +174,7 @@ void BatchPlan::set_tables(Group &g, const std::vector<const void *> &a, const s
     g.table0 = host_tables_.size();
     for (const auto *v : {&a, &b, &out, &keys, &perms})
         for (u32 i = 0; i < g.B; ++i) host_tables_.push_back(v->empty() ? nullptr : (*v)[i]);
+    for (u32 i = 0; i < g.B; ++i) host_tables_.push_back(nullptr);   // key-switch outputs, set by rebind
 }
 
 void BatchPlan::layout() {
@@ -157,10 +186,21 @@ void BatchPlan::layout() {
         void *t = nullptr;
         PHX_HIP_CHECK(hipMalloc(&t, std::max<size_t>(host_tables_.size() * sizeof(void *), 256)));
         tables_ = static_cast<const void **>(t);
-        PHX_HIP_CHECK(hipMemcpy(t, host_tables_.data(), host_tables_.size() * sizeof(void *),
-                                hipMemcpyHostToDevice));
-        bind(arena_, tables_);
+        rebind(arena_);
     }
+}
+
+void BatchPlan::rebind(u64 *arena) {
+    bind(arena, tables_);
+    if (kind_ == kBfvMulRelin) {
+        // outputs of the key switch: polynomials 0 and 1 of each item's product
+        for (const auto &g : groups_) {
+            const u64 LN = (u64)g.Ql * ctx_.N();
+            for (u32 i = 0; i < g.B; ++i) host_tables_[g.table0 + 5 * g.B + i] = g.c3 + (u64)i * 3 * LN;
+        }
+    }
+    PHX_HIP_CHECK(hipMemcpy(const_cast<void **>(tables_), host_tables_.data(), host_tables_.size() * sizeof(void *),
+                            hipMemcpyHostToDevice));
 }
 
 void BatchPlan::bind(u64 *arena, const void **tables) {
@@ -173,7 +213,19 @@ void BatchPlan::bind(u64 *arena, const void **tables) {
             p += words;
             return r;
         };
+        g.c2_stride = LN;
+        if (kind_ == kBfvMulRelin) {
+            g.in1 = take((u64)g.B * 2 * LN);
+            g.in2 = take((u64)g.B * 2 * LN);
+            g.c3 = take((u64)g.B * 3 * LN);
+            g.work = take((u64)g.B * bfv_batch_work_words(ctx_));
+        }
         g.c2 = take(g.B * LN);
+        if (kind_ == kBfvMulRelin) {
+            // the key switch reads c2 where the product holds it (NTT form)
+            g.c2 = g.c3 + 2 * LN;
+            g.c2_stride = 3 * LN;
+        }
         g.t_cks = take(g.B * LN);
         g.t_mod_up = take((u64)g.B * g.beta * LPN);
         g.cx = take((u64)g.B * 2 * LPN);
@@ -185,20 +237,29 @@ void BatchPlan::bind(u64 *arena, const void **tables) {
         g.out = reinterpret_cast<u64 **>(const_cast<void **>(t + 2 * g.B));
         g.keys = reinterpret_cast<const u64 **>(t + 3 * g.B);
         g.perms = reinterpret_cast<const u32 **>(t + 4 * g.B);
+        g.ks_out = reinterpret_cast<u64 **>(const_cast<void **>(t + 5 * g.B));
     }
 }
 
 BatchPlan *BatchPlan::mul_relin(const Context &ctx, const std::vector<Ciphertext *> &ct1,
                                 const std::vector<const Ciphertext *> &ct2, const KSwitchKey &rlk, bool transient) {
-    require(ctx.params().scheme == kSchemeCKKS, "batched multiply_and_relin: only CKKS is on this path");
+    const bool bfv = ctx.params().scheme == kSchemeBFV;
+    require(bfv || ctx.params().scheme == kSchemeCKKS, "batched multiply_and_relin: unsupported scheme");
+    require(!(bfv && transient), "BFV batches are persistent plans");
     require(ct1.size() == ct2.size() && !ct1.empty(), "batch operands must be non-empty and of equal count");
     require(rlk.data != nullptr, "relinearisation key is empty");
-    std::unique_ptr<BatchPlan> p(new BatchPlan(ctx, kMulRelin, transient));
+    if (bfv) bfv_batch_work_words(ctx);   // rejects a context without its multiply tool
+    std::unique_ptr<BatchPlan> p(new BatchPlan(ctx, bfv ? kBfvMulRelin : kMulRelin, transient));
     std::vector<u32> chains;
     for (size_t i = 0; i < ct1.size(); ++i) {
         require(ct1[i] && ct2[i] && ct1[i]->data && ct2[i]->data, "batch ciphertext is empty");
         require(ct1[i]->chain_index == ct2[i]->chain_index, "encrypted1 and encrypted2 parameter mismatch");
-        require(ct1[i]->is_ntt_form && ct2[i]->is_ntt_form, "encrypted1 and encrypted2 must be in NTT form");
+        if (bfv) {
+            require(!ct1[i]->is_ntt_form && !ct2[i]->is_ntt_form, "encrypted1 or encrypted2 cannot be in NTT form");
+            require(ct1[i]->chain_index == 1, "BFV multiplication runs at the top data level");
+        } else {
+            require(ct1[i]->is_ntt_form && ct2[i]->is_ntt_form, "encrypted1 and encrypted2 must be in NTT form");
+        }
         require(ct1[i]->size == 2 && ct2[i]->size == 2, "poly number mismatch");
         require(rlk.dnum >= ctx.level(ct1[i]->chain_index).beta, "key-switching key has too few digits");
         for (size_t j = 0; j < ct1.size(); ++j) {
@@ -208,7 +269,7 @@ BatchPlan *BatchPlan::mul_relin(const Context &ctx, const std::vector<Ciphertext
         }
         chains.push_back(ct1[i]->chain_index);
         p->cts_.push_back(ct1[i]);
-        p->scale_mul_.push_back(ct2[i]->scale);
+        p->scale_mul_.push_back(bfv ? 1.0 : ct2[i]->scale);
     }
     p->items_ = ct1.size();
     p->build_groups(chains);
@@ -223,6 +284,36 @@ BatchPlan *BatchPlan::mul_relin(const Context &ctx, const std::vector<Ciphertext
         p->set_tables(g, a, b, out, keys, {});
     }
     p->layout();
+    return p.release();
+}
+
+BatchPlan *BatchPlan::join(const std::vector<BatchPlan *> &parts) {
+    require(!parts.empty(), "nothing to join");
+    for (size_t i = 0; i < parts.size(); ++i) {
+        require(parts[i] != nullptr, "batch is null");
+        require(!parts[i]->transient_ && parts[i]->kind_ != kJoin, "only persistent, unjoined batches can be joined");
+        for (size_t j = 0; j < i; ++j) require(parts[j] != parts[i], "a batch appears twice in a join");
+    }
+    std::unique_ptr<BatchPlan> p(new BatchPlan(parts[0]->ctx_, kJoin, false));
+    size_t bytes = 0;
+    for (auto *q : parts) {
+        bytes = std::max(bytes, q->scratch_bytes_);
+        p->items_ += q->items_;
+    }
+    // the parts run one after another on one stream: one arena of the largest size
+    PHX_HIP_CHECK(hipDeviceSynchronize());
+    PHX_HIP_CHECK(hipMalloc(&p->arena_, std::max<size_t>(bytes, 256)));
+    p->scratch_bytes_ = bytes;
+    for (auto *q : parts) {
+        PHX_HIP_CHECK(hipFree(q->arena_));
+        q->arena_ = nullptr;
+        q->rebind(p->arena_);
+        if (q->exec_) (void)hipGraphExecDestroy(q->exec_);
+        if (q->graph_) (void)hipGraphDestroy(q->graph_);
+        q->exec_ = nullptr;
+        q->graph_ = nullptr;
+        p->parts_.emplace_back(q);
+    }
     return p.release();
 }
 
@@ -269,10 +360,27 @@ BatchPlan *BatchPlan::rotate(const Context &ctx, const std::vector<Ciphertext *>
 // The same pipeline as keyswitch() in evaluator.hip, each stage over the group.
 void BatchPlan::run_group(const Group &g, hipStream_t s) {
     const Context &ctx = ctx_;
-    const LevelConsts &L = ctx.level(g.chain);
     const u64 N = ctx.N();
-    const u32 Ql = g.Ql, QlP = g.QlP, beta = g.beta, B = g.B, P = ctx.size_P(), alpha = ctx.size_P();
-    const u64 LN = (u64)Ql * N, LPN = (u64)QlP * N;
+    const u32 Ql = g.Ql, B = g.B;
+    const u64 LN = (u64)Ql * N;
+    if (kind_ == kBfvMulRelin) {
+        // BFV (evaluate.cu: bfv_multiply_behz / bfv_multiply_hps, then relinearize_inplace):
+        // product of every item, its NTT, the key switch of c2 added to (c0, c1) in the
+        // NTT domain, and back: INTT(NTT(c) + ks) = c + INTT(ks) exactly
+        hipLaunchKernelGGL(k_gather_items, items_grid(2 * LN, B), dim3(kThreads), 0, s, g.a, g.in1, 2 * LN, 2 * LN);
+        PHX_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_gather_items, items_grid(2 * LN, B), dim3(kThreads), 0, s, g.b, g.in2, 2 * LN, 2 * LN);
+        PHX_HIP_CHECK(hipGetLastError());
+        bfv_multiply_batch(ctx, g.in1, g.in2, B, g.work, g.c3, s);
+        ntt_forward(ctx.tables(), limb_map(g.c3, 0, Ql, LN), Ql, 3 * B, s);
+        keyswitch_group(g, s);
+        ntt_inverse(ctx.tables(), limb_map(g.c3, 0, Ql, 3 * LN), Ql, B, s);
+        ntt_inverse(ctx.tables(), limb_map(g.c3 + LN, 0, Ql, 3 * LN), Ql, B, s);
+        hipLaunchKernelGGL(k_scatter_items, items_grid(2 * LN, B), dim3(kThreads), 0, s, g.out, (const u64 *)g.c3,
+                           2 * LN, 3 * LN);
+        PHX_HIP_CHECK(hipGetLastError());
+        return;
+    }
     if (kind_ == kMulRelin) {
         hipLaunchKernelGGL(k_tensor_batch, grid3(N, Ql, B), dim3(kThreads), 0, s, g.a, g.b, g.out, g.c2,
                            ctx.dev_mod(), (u32)N, Ql);
@@ -281,10 +389,21 @@ void BatchPlan::run_group(const Group &g, hipStream_t s) {
                            (u32)N, Ql);
     }
     PHX_HIP_CHECK(hipGetLastError());
+    keyswitch_group(g, s);
+}
+
+// out[item] (+)= keyswitch(c2[item]) in NTT form: the same pipeline as keyswitch()
+// in evaluator.hip, each stage over the group
+void BatchPlan::keyswitch_group(const Group &g, hipStream_t s) {
+    const Context &ctx = ctx_;
+    const LevelConsts &L = ctx.level(g.chain);
+    const u64 N = ctx.N();
+    const u32 Ql = g.Ql, QlP = g.QlP, beta = g.beta, B = g.B, P = ctx.size_P(), alpha = ctx.size_P();
+    const u64 LN = (u64)Ql * N, LPN = (u64)QlP * N;
     // modup: INTT(c2) * partQlHatInv for every item
     LimbMap mi = limb_map(g.t_cks, 0, Ql, LN);
     mi.src = g.c2;
-    mi.src_poly_stride = LN;
+    mi.src_poly_stride = g.c2_stride;
     mi.scale = L.modup_scale;
     ntt_inverse(ctx.tables(), mi, Ql, B, s);
     // FastBConv of every digit of every item
@@ -303,7 +422,7 @@ void BatchPlan::run_group(const Group &g, hipStream_t s) {
     const bool fused_ip = tune_value("ks_fused_ip") != 0;
     ba.copy_src = nullptr;
     ba.copy_job_stride = 0;
-    ba.copy_batch_stride = LN;
+    ba.copy_batch_stride = g.c2_stride;
     ba.mod = ctx.dev_mod();
     ba.N = (u32)N;
     bconv_run(ctx, L.h_modup_bconv, ba, B * beta, s);
@@ -316,14 +435,15 @@ void BatchPlan::run_group(const Group &g, hipStream_t s) {
     mf.skip_mod = beta;
     mf.nlimbs = QlP;
     const u32 last_sz = Ql - alpha * (beta - 1);
-    require(beta <= 8, "batched key switching supports at most 8 digits");
+    require(beta <= 32, "batched key switching supports at most 32 digits");
+    require(!fused_ip || beta <= 8, "the fused inner product supports at most 8 digits");
     if (fused_ip) {
         // pass 1 here; pass 2 fused with the inner product
         ntt_forward_cols(ctx.tables(), mf, QlP - last_sz, B * beta, s);
         RowsIpArgs ip{};
         ip.t = g.t_mod_up;
         ip.c2 = g.c2;
-        ip.c2_stride = LN;
+        ip.c2_stride = g.c2_stride;
         ip.keys = g.keys;
         ip.cx = g.cx;
         ip.Ql = Ql;
@@ -336,9 +456,10 @@ void BatchPlan::run_group(const Group &g, hipStream_t s) {
     } else {
         ntt_forward(ctx.tables(), mf, QlP - last_sz, B * beta, s);
         // inner product (a shared key is read once per coefficient for the group)
-        hipLaunchKernelGGL(k_inner_product_batch, dim3((u32)(N / kThreads), QlP), dim3(kThreads), 0, s, g.t_mod_up,
-                           g.c2, g.keys, g.cx, ctx.dev_mod(), (u32)N, Ql, QlP, ctx.size_Q(), ctx.size_QP(), beta,
-                           alpha, B);
+        auto ip = beta <= 8 ? k_inner_product_batch<8> : beta <= 16 ? k_inner_product_batch<16>
+                                                                    : k_inner_product_batch<32>;
+        hipLaunchKernelGGL(ip, dim3((u32)(N / kThreads), QlP), dim3(kThreads), 0, s, g.t_mod_up, g.c2, g.keys, g.cx,
+                           ctx.dev_mod(), (u32)N, Ql, QlP, ctx.size_Q(), ctx.size_QP(), beta, alpha, B, g.c2_stride);
         PHX_HIP_CHECK(hipGetLastError());
     }
     // moddown of both halves of every item
@@ -358,7 +479,7 @@ void BatchPlan::run_group(const Group &g, hipStream_t s) {
     bd.N = (u32)N;
     bconv_run(ctx, {L.h_moddown_bconv}, bd, 2 * B, s);
     StoreModDownBatch st{};
-    st.out = g.out;
+    st.out = kind_ == kBfvMulRelin ? g.ks_out : g.out;
     st.base0 = kind_ == kRotate ? g.base0 : nullptr;
     st.cx = g.cx;
     st.pinv = L.pinv;
@@ -366,6 +487,17 @@ void BatchPlan::run_group(const Group &g, hipStream_t s) {
     st.N = (u32)N;
     st.Ql = Ql;
     ntt_forward_store_batch(ctx.tables(), limb_map(g.delta, 0, Ql, LN), Ql, 2 * B, st, s);
+}
+
+void BatchPlan::record(hipStream_t s) {
+    for (const auto &q : parts_) q->record(s);
+    for (const auto &g : groups_) run_group(g, s);
+}
+
+void BatchPlan::after_launch() {
+    for (const auto &q : parts_) q->after_launch();
+    if (kind_ == kMulRelin)
+        for (size_t i = 0; i < cts_.size(); ++i) cts_[i]->scale *= scale_mul_[i];
 }
 
 void BatchPlan::run(hipStream_t s) {
@@ -379,10 +511,9 @@ void BatchPlan::run(hipStream_t s) {
         dev_free(t, s);
         dev_free(arena, s);
     } else {
-        for (const auto &g : groups_) run_group(g, s);
+        record(s);
     }
-    if (kind_ == kMulRelin)
-        for (size_t i = 0; i < cts_.size(); ++i) cts_[i]->scale *= scale_mul_[i];
+    after_launch();
 }
 
 void BatchPlan::capture(hipStream_t s) {
@@ -393,7 +524,7 @@ void BatchPlan::capture(hipStream_t s) {
     graph_ = nullptr;
     PHX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
-        for (const auto &g : groups_) run_group(g, s);
+        record(s);
     } catch (...) {
         hipGraph_t tmp = nullptr;
         (void)hipStreamEndCapture(s, &tmp);
@@ -407,8 +538,7 @@ void BatchPlan::capture(hipStream_t s) {
 void BatchPlan::replay(hipStream_t s) {
     require(exec_ != nullptr, "batch has not been captured");
     PHX_HIP_CHECK(hipGraphLaunch(exec_, s));
-    if (kind_ == kMulRelin)
-        for (size_t i = 0; i < cts_.size(); ++i) cts_[i]->scale *= scale_mul_[i];
+    after_launch();
 }
 
 }  // namespace phx

@@ -215,6 +215,29 @@ __global__ __launch_bounds__(kThreads) void k_add_inplace(u64 *__restrict__ a, c
     a[o] = add_mod(a[o], b[o], mod[limb].value);
 }
 
+// tensor of B items at fixed strides: out[z] = (a0 b0, a0 b1 + a1 b0, a1 b1)
+// (tensor_prod_2x2_rns_poly, src/polymath.cu:463-498)
+__global__ __launch_bounds__(kThreads) void k_tensor_items(const u64 *__restrict__ a, const u64 *__restrict__ b,
+                                                           u64 *__restrict__ out, const DevModulus *__restrict__ mod,
+                                                           u32 N, u32 L, u64 a_stride, u64 b_stride, u64 out_stride) {
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u32 limb = blockIdx.y, item = blockIdx.z;
+    const u64 LN = (u64)L * N, o = (u64)limb * N + k;
+    const u64 *pa = a + item * a_stride + o, *pb = b + item * b_stride + o;
+    const DevModulus m = mod[limb];
+    const u64 q = m.value;
+    const u64 a0 = pa[0], a1 = pa[LN], b0 = pb[0], b1 = pb[LN];
+    const u64 r0 = mul_mod(a0, b0, m);
+    const u64 r2 = mul_mod(a1, b1, m);
+    u64 r1 = mul_mod(add_mod(a0, a1, q), add_mod(b0, b1, q), m);
+    r1 = sub_mod(sub_mod(r1, r0, q), r2, q);
+    u64 *po = out + item * out_stride + o;
+    po[0] = r0;
+    po[LN] = r1;
+    po[2 * LN] = r2;
+}
+
 namespace {
 const BehzConsts &behz_of(const Context &ctx) {
     const BehzConsts *B = ctx.behz();
@@ -310,6 +333,87 @@ void bfv_multiply_hps(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2
     PHX_HIP_CHECK(hipGetLastError());
     if (!square) dev_free(b, s);
     dev_free(a, s);
+}
+
+// ---- batched form: B items staged contiguously ----
+
+size_t bfv_batch_work_words(const Context &ctx) {
+    const u64 N = ctx.N();
+    if (ctx.params().mul_tech == kMulHPS) {
+        const HpsConsts *H = ctx.hps();
+        require(H != nullptr, "the context has no HPS tool: scheme must be BFV with mul_tech hps");
+        return (size_t)7 * (H->k + H->nr) * N;   // extended operands 2 + 2, tensor 3
+    }
+    const BehzConsts &B = behz_of(ctx);
+    return (size_t)N * (7ull * B.nb + 3ull * B.k);   // Bsk operands 2 + 2, tensor 3 (q and Bsk)
+}
+
+void bfv_multiply_batch(const Context &ctx, u64 *in1, u64 *in2, u32 items, u64 *work, u64 *out, hipStream_t s) {
+    require(items >= 1, "empty batch");
+    const u64 N = ctx.N();
+    const u32 nz = items;
+    if (ctx.params().mul_tech == kMulHPS) {
+        const HpsConsts *Hp = ctx.hps();
+        require(Hp != nullptr, "the context has no HPS tool: scheme must be BFV with mul_tech hps");
+        const HpsConsts &H = *Hp;
+        require(H.k <= kMaxQ && H.nr <= kMaxBsk, "HPS multiply supports at most 16 data primes");
+        const u32 S = H.k + H.nr;
+        const u64 QN = (u64)H.k * N, SN = (u64)S * N;
+        u64 *e1 = work, *e2 = work + 2ull * nz * SN, *t = work + 4ull * nz * SN;
+        LimbMap m = limb_map(e1, 0, S, SN);
+        m.split = H.k;
+        m.special_row = H.r_row0;
+        for (int o = 0; o < 2; ++o) {
+            u64 *e = o ? e2 : e1;
+            hipLaunchKernelGGL(k_hps_extend, grid2(N, 2 * nz), dim3(kThreads), 0, s, (const u64 *)(o ? in2 : in1), e,
+                               H, (u32)N, QN, SN);
+            PHX_HIP_CHECK(hipGetLastError());
+            LimbMap me = m;
+            me.data = e;
+            ntt_forward(ctx.tables(), me, S, 2 * nz, s);
+        }
+        hipLaunchKernelGGL(k_tensor_items, dim3((u32)((N + kThreads - 1) / kThreads), S, nz), dim3(kThreads), 0, s,
+                           (const u64 *)e1, (const u64 *)e2, t, H.mod_qr, (u32)N, S, 2 * SN, 2 * SN, 3 * SN);
+        PHX_HIP_CHECK(hipGetLastError());
+        LimbMap mt = m;
+        mt.data = t;
+        ntt_inverse(ctx.tables(), mt, S, 3 * nz, s);
+        hipLaunchKernelGGL(k_hps_round_convert, grid2(N, 3 * nz), dim3(kThreads), 0, s, (const u64 *)t, out, H,
+                           (u32)N, SN, QN);
+        PHX_HIP_CHECK(hipGetLastError());
+        return;
+    }
+    const BehzConsts &B = behz_of(ctx);
+    const u64 QN = (u64)B.k * N, BN = (u64)B.nb * N;
+    u64 *b1 = work, *b2 = work + 2ull * nz * BN, *tb = work + 4ull * nz * BN, *tq = work + 7ull * nz * BN;
+    // (1)-(3) per operand: lift to Bsk, then NTT of both bases (the q part in place)
+    for (int o = 0; o < 2; ++o) {
+        u64 *x = o ? in2 : in1, *bb = o ? b2 : b1;
+        hipLaunchKernelGGL(k_behz_lift, grid2(N, 2 * nz), dim3(kThreads), 0, s, (const u64 *)x, bb, B,
+                           ctx.dev_mod(), (u32)N, QN, BN);
+        PHX_HIP_CHECK(hipGetLastError());
+        ntt_forward(ctx.tables(), limb_map(x, 0, B.k, QN), B.k, 2 * nz, s);
+        ntt_forward(ctx.tables(), limb_map(bb, B.bsk_row0, B.nb, BN), B.nb, 2 * nz, s);
+    }
+    // (4) tensor in both bases
+    const u32 gx = (u32)((N + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_tensor_items, dim3(gx, B.k, nz), dim3(kThreads), 0, s, (const u64 *)in1, (const u64 *)in2, tq,
+                       ctx.dev_mod(), (u32)N, B.k, 2 * QN, 2 * QN, 3 * QN);
+    PHX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_tensor_items, dim3(gx, B.nb, nz), dim3(kThreads), 0, s, (const u64 *)b1, (const u64 *)b2, tb,
+                       ctx.dev_mod() + B.bsk_row0, (u32)N, B.nb, 2 * BN, 2 * BN, 3 * BN);
+    PHX_HIP_CHECK(hipGetLastError());
+    // (5)-(6) inverse NTTs with t folded in
+    LimbMap mq = limb_map(tq, 0, B.k, QN);
+    mq.scale = B.tscale_q;
+    ntt_inverse(ctx.tables(), mq, B.k, 3 * nz, s);
+    LimbMap mb = limb_map(tb, B.bsk_row0, B.nb, BN);
+    mb.scale = B.tscale_bsk;
+    ntt_inverse(ctx.tables(), mb, B.nb, 3 * nz, s);
+    // (7)-(8)
+    hipLaunchKernelGGL(k_behz_floor_sk, grid2(N, 3 * nz), dim3(kThreads), 0, s, (const u64 *)tq, (const u64 *)tb, out,
+                       B, ctx.dev_mod(), (u32)N, QN, BN, QN);
+    PHX_HIP_CHECK(hipGetLastError());
 }
 
 void bfv_relinearize(const Context &ctx, Ciphertext &ct, const KSwitchKey &rlk, hipStream_t s) {

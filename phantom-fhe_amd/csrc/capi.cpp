@@ -69,6 +69,24 @@ inline void nn(const void *p, const char *what) {
 }
 }  // namespace
 
+namespace {
+// tuning choice <-> the C array {ntt_fused, ntt_fused_min_limbs, ntt_chunk_limbs, bconv_mfma}
+TuneChoice choice_of(const int c[4]) {
+    TuneChoice t;
+    t.ntt_fused = c[0];
+    t.ntt_fused_min_limbs = c[1];
+    t.ntt_chunk_limbs = c[2];
+    t.bconv_mfma = c[3];
+    return t;
+}
+void put_choice(const TuneChoice &t, int c[4]) {
+    c[0] = t.ntt_fused;
+    c[1] = t.ntt_fused_min_limbs;
+    c[2] = t.ntt_chunk_limbs;
+    c[3] = t.bconv_mfma;
+}
+}  // namespace
+
 extern "C" {
 
 const char *phx_last_error(void) { return g_last_error.c_str(); }
@@ -700,6 +718,21 @@ int phx_batch_rotate_create(const phx_context *ctx, phx_ciphertext *const *cts, 
     });
 }
 
+int phx_batch_join(phx_batch *const *parts, size_t n, phx_batch **out) {
+    return guard([&] {
+        nn(parts, "parts");
+        nn(out, "out");
+        require(n >= 1, "nothing to join");
+        std::vector<BatchPlan *> v;
+        for (size_t i = 0; i < n; ++i) {
+            nn(parts[i], "parts[i]");
+            v.push_back(parts[i]->plan);
+        }
+        *out = new phx_batch{BatchPlan::join(v)};
+        for (size_t i = 0; i < n; ++i) delete parts[i];   // the plans now belong to the joined batch
+    });
+}
+
 int phx_batch_run(phx_batch *b, phx_stream s) {
     return guard([&] {
         nn(b, "batch");
@@ -971,23 +1004,6 @@ int phx_tune_get(const char *key, int *value) {
         require(tune_get(key, value), std::string("unknown tuning key: ") + (key ? key : "(null)"));
     });
 }
-
-namespace {
-TuneChoice choice_of(const int c[4]) {
-    TuneChoice t;
-    t.ntt_fused = c[0];
-    t.ntt_fused_min_limbs = c[1];
-    t.ntt_chunk_limbs = c[2];
-    t.bconv_mfma = c[3];
-    return t;
-}
-void put_choice(const TuneChoice &t, int c[4]) {
-    c[0] = t.ntt_fused;
-    c[1] = t.ntt_fused_min_limbs;
-    c[2] = t.ntt_chunk_limbs;
-    c[3] = t.bconv_mfma;
-}
-}  // namespace
 
 int phx_context_get_tuning(const phx_context *ctx, int choice[4]) {
     return guard([&] {

@@ -132,6 +132,7 @@ _SIGS = {
                                                            ctypes.POINTER(_vp)]),
     "phx_batch_rotate_create": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_int), _sz, _vp,
                                                ctypes.POINTER(_vp)]),
+    "phx_batch_join": (ctypes.c_int, [ctypes.POINTER(_vp), _sz, ctypes.POINTER(_vp)]),
     "phx_batch_run": (ctypes.c_int, [_vp, _vp]),
     "phx_batch_capture": (ctypes.c_int, [_vp, _vp]),
     "phx_batch_replay": (ctypes.c_int, [_vp, _vp]),
@@ -661,6 +662,18 @@ class batch:
         h = _vp()
         _check(lib().phx_batch_rotate_create(ctx._h, a, st, len(cts), gk._h, ctypes.byref(h)))
         return cls(h, (ctx, list(cts), gk))
+
+    @classmethod
+    def join(cls, parts):
+        """one batch running `parts` (any contexts) in order; the parts are consumed"""
+        arr = (_vp * len(parts))(*[p._h for p in parts])
+        h = _vp()
+        _check(lib().phx_batch_join(arr, len(parts), ctypes.byref(h)))
+        keep = []
+        for p in parts:
+            keep.append(p._keep)
+            p._h = None
+        return cls(h, tuple(keep))
 
     def run(self, stream=None):
         _check(lib().phx_batch_run(self._h, _stream(stream)))

@@ -201,3 +201,63 @@ def test_config3_batched_rotations_bit_exact():
         k = int(steps_idx[i])
         want = octx.apply_galois(Ql, a, elts[k], key_of(k))
         assert np.array_equal(cts[i].download(N), want), (i, Ql, CONFIG3_STEPS[k])
+
+
+def test_joined_batch_two_ring_degrees():
+    """CKKS plans of N=2^13 and N=2^14 joined into one plan and one graph:
+    every item bit-exact against the oracle, on eager runs and on graph replays
+    with eager work between them."""
+    ob.set_threads(8)
+    say("case: joined ckks batch N=2^13 + 2^14")
+    parts, checks = [], []
+    for N, bits, P in [(1 << 13, [60, 50, 50, 50, 60, 60], 2), (1 << 14, [60] + [50] * 6 + [60, 60, 60], 3)]:
+        ctx, octx, mods = make_ctx(N, bits, P)
+        rng = np.random.default_rng(N)
+        key = rand_key(rng, mods, N, ctx.dnum)
+        rk = ph.relin_key(ctx, host=key)
+        chains = [1, 2, 1]
+        data = [(c, ctx.limbs_at(c), rand_ct(rng, mods[:ctx.limbs_at(c)], N),
+                 rand_ct(rng, mods[:ctx.limbs_at(c)], N)) for c in chains]
+        c1 = [upload_ct(ctx, a, c) for c, _, a, _ in data]
+        c2 = [upload_ct(ctx, b, c) for c, _, _, b in data]
+        parts.append(ph.batch.multiply_and_relin(ctx, c1, c2, rk))
+        checks.append((ctx, octx, key, data, c1))
+    single = max(p.scratch_bytes() for p in parts)
+    jb = ph.batch.join(parts)
+    assert all(p._h is None for p in parts)
+    assert jb.scratch_bytes() == single
+    stream = ph.stream_create()
+
+    def verify():
+        for ctx, octx, key, data, c1 in checks:
+            for i, (c, Ql, a, b) in enumerate(data):
+                assert np.array_equal(c1[i].download(ctx.N), octx.multiply_and_relin(Ql, a, b, key)), (ctx.N, i)
+
+    def reset():
+        for ctx, octx, key, data, c1 in checks:
+            for i, (c, Ql, a, b) in enumerate(data):
+                c1[i].upload(a)
+
+    jb.run(stream)
+    ph.synchronize(stream)
+    verify()
+    jb.capture(stream)
+    for _ in range(2):
+        reset()
+        ph.device_synchronize()
+        jb.replay(stream)
+        ph.synchronize(stream)
+        verify()
+
+
+def test_join_rejects_repeated_part(small):
+    ctx, octx, mods, key = small
+    rk = ph.relin_key(ctx, host=key)
+    rng = np.random.default_rng(9)
+    a = upload_ct(ctx, rand_ct(rng, mods[:ctx.size_Q], ctx.N), 1)
+    b = upload_ct(ctx, rand_ct(rng, mods[:ctx.size_Q], ctx.N), 1)
+    bt = ph.batch.multiply_and_relin(ctx, [a], [b], rk)
+    with pytest.raises(ValueError):
+        ph.batch.join([bt, bt])
+    bt.run()   # untouched by the failed join
+    ph.device_synchronize()

@@ -2,7 +2,8 @@
 oracle (coefficient-form ciphertexts), the config of 64 ciphertexts over
 N in {2^13, 2^14, 2^15} with 4/8/16 limbs of 54-bit primes and t = 65537, and
 a semantic check: encrypt m1, m2 -> multiply+relin -> decrypt = m1 m2 mod
-(X^N + 1, t)."""
+(X^N + 1, t). The batched form (one plan per ring degree, joined into one graph)
+is checked item by item against the plain path and the oracle."""
 import numpy as np
 import pytest
 
@@ -202,3 +203,98 @@ def test_bfv_encrypt_multiply_relin_decrypt(tech):
     octx_t = ob.Ctx(N.bit_length() - 1, [T, mods[0]], 1)
     want = negacyclic_mod_t([int(x) for x in m1], [int(x) for x in m2], N, octx_t)
     assert [int(x) for x in want] == dec
+
+
+def plain_mul_relin(ctx, a, b, rk, square=False):
+    c1 = upload_bfv(ctx, a)
+    c2 = c1 if square else upload_bfv(ctx, b)
+    ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
+    return c1.download(ctx.N)
+
+
+@pytest.mark.parametrize("tech", [ph.MUL_TECH_BEHZ, ph.MUL_TECH_HPS])
+def test_bfv_batch_bit_exact(tech):
+    """batched BFV multiply + relin == plain path == oracle, squaring item included"""
+    ob.set_threads(8)
+    N, k = 1 << 13, 4
+    say(f"case: bfv batch tech={tech}")
+    ctx, octx, tool, mods = make_bfv(N, k, tech)
+    rng = np.random.default_rng(50 + tech)
+    key = rand_key(rng, mods, N, ctx.dnum)
+    rk = ph.relin_key(ctx, host=key)
+    data = [(rand_ct(rng, mods[:k], N), rand_ct(rng, mods[:k], N)) for _ in range(5)]
+    c1 = [upload_bfv(ctx, a) for a, _ in data]
+    c2 = [upload_bfv(ctx, b) for _, b in data]
+    c2[3] = c1[3]   # squaring
+    bt = ph.batch.multiply_and_relin(ctx, c1, c2, rk)
+    bt.run()
+    ph.device_synchronize()
+    for i, (a, b) in enumerate(data):
+        got = c1[i].download(N)
+        assert c1[i].size() == 2 and not c1[i].is_ntt_form()
+        assert np.array_equal(got, plain_mul_relin(ctx, a, b, rk, square=(i == 3))), i
+        if i in (0, 3):
+            other = a if i == 3 else b
+            assert np.array_equal(got, ob.bfv_relinearize(octx, tool.multiply(a, other), key)), i
+
+
+def test_bfv_batch_rejects():
+    ctx, octx, tool, mods = make_bfv(1 << 13, 4)
+    rng = np.random.default_rng(5)
+    rk = ph.relin_key(ctx, host=rand_key(rng, mods, ctx.N, ctx.dnum))
+    a = upload_bfv(ctx, rand_ct(rng, mods[:4], ctx.N))
+    b = upload_bfv(ctx, rand_ct(rng, mods[:4], ctx.N))
+    b.set_ntt_form(True)
+    with pytest.raises(ValueError):
+        ph.batch.multiply_and_relin(ctx, [a], [b], rk)
+
+
+def test_config4_joined_batch_graph():
+    """config 4 in batched form: 64 BFV ciphertexts, N uniform in {2^13, 2^14, 2^15}
+    (4/8/16 limbs), one BEHZ plan per ring degree joined into one captured graph;
+    every item equals the plain path, a sample equals the oracle; replayed twice
+    with eager work (re-uploads) between the replays."""
+    ob.set_threads(16)
+    say("case: config4 joined bfv batch")
+    rng = np.random.default_rng(404)
+    ctxs = {N: make_bfv(N, k) for N, k in LIMBS.items()}
+    keys = {N: rand_key(np.random.default_rng(N + 1), v[3], N, v[0].dnum) for N, v in ctxs.items()}
+    rks = {N: ph.relin_key(v[0], host=keys[N]) for N, v in ctxs.items()}
+    Ns = [int(x) for x in rng.choice(list(LIMBS), 64)]
+    inputs = {}
+    for i, N in enumerate(Ns):
+        mods, k = ctxs[N][3], LIMBS[N]
+        inputs[i] = (rand_ct(np.random.default_rng(3000 + i), mods[:k], N),
+                     rand_ct(np.random.default_rng(4000 + i), mods[:k], N))
+    want = {i: plain_mul_relin(ctxs[N][0], *inputs[i], rks[N]) for i, N in enumerate(Ns)}
+    parts, c1s = [], {}
+    for N in LIMBS:
+        idx = [i for i, n in enumerate(Ns) if n == N]
+        if not idx:
+            continue
+        ctx = ctxs[N][0]
+        c1 = [upload_bfv(ctx, inputs[i][0]) for i in idx]
+        c2 = [upload_bfv(ctx, inputs[i][1]) for i in idx]
+        c1s.update(zip(idx, c1))
+        parts.append(ph.batch.multiply_and_relin(ctx, c1, c2, rks[N]))
+    jb = ph.batch.join(parts)
+    stream = ph.stream_create()
+    jb.capture(stream)
+    for rep in range(2):
+        if rep:
+            for i, c in c1s.items():
+                c.upload(inputs[i][0])
+            ph.device_synchronize()
+        jb.replay(stream)
+        ph.synchronize(stream)
+        for i, N in enumerate(Ns):
+            assert np.array_equal(c1s[i].download(N), want[i]), (rep, i, N)
+    seen = set()
+    for i, N in enumerate(Ns):
+        if N in seen:
+            continue
+        seen.add(N)
+        ctx, octx, beh, mods = ctxs[N]
+        a, b = inputs[i]
+        assert np.array_equal(want[i], ob.bfv_relinearize(octx, beh.multiply(a, b), keys[N])), (i, N)
+    assert seen == set(LIMBS)
