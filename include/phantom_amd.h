@@ -39,6 +39,7 @@ extern "C" {
 #define PHX_MUL_TECH_NONE 0
 #define PHX_MUL_TECH_BEHZ 1
 #define PHX_MUL_TECH_HPS 2
+#define PHX_MUL_TECH_HPS_OVERQ 3
 
 typedef struct phx_context phx_context;
 typedef struct phx_ciphertext phx_ciphertext;

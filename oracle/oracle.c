@@ -1048,20 +1048,62 @@ static void big_div(orc_big *a, u64 m) {   /* a = floor(a / m) */
 
 /* HPS constants of the top data level (src/rns.cu:686-783 and the
  * BaseConverter tables, src/host/rns.cu:438-497): R = size_Q + 1 primes below
- * the smallest q (get_primes_below, src/host/numth.cu:235-262). */
+ * the smallest q (get_primes_below, src/host/numth.cu:235-262). The over-Q
+ * variant (mul_tech hps_overq, src/rns.cu:792-890) takes R = size_Q primes and
+ * the t/R scale-and-round constants of QR -> Q. */
 struct orc_hps {
     size_t k, nr;
     u64 t;
+    int overq;
     u64 *r;                        /* [nr] */
     u64 *rp, *rps, *irp, *irps, *ninv;
     double *frac;                  /* [k] tRSHatInvModsDivsFrac */
     u64 *divs_mod_r;               /* [nr][k + 1] tRSHatInvModsDivsModr */
+    /* over Q */
+    u64 *negrq;                    /* [k] negPQHatInvModq of the Q -> R converter (P = R) */
+    u64 *qinv_r;                   /* [nr][k] QInvModp: q_i^-1 mod r_j */
+    double *frac_r;                /* [nr] tQlSlHatInvModsDivsFrac */
+    u64 *divs_mod_q;               /* [k][nr + 1] tQlSlHatInvModsDivsModq */
 };
 
-orc_hps *orc_hps_create(const orc_ctx *c, u64 t) {
+static void hps_overq_consts(const orc_ctx *c, orc_hps *h) {
+    size_t k = h->k, nr = h->nr, ns = k + nr;
+    h->negrq = (u64 *)malloc(k * 8);
+    h->qinv_r = (u64 *)malloc(nr * k * 8);
+    h->frac_r = (double *)malloc(nr * sizeof(double));
+    h->divs_mod_q = (u64 *)malloc(k * (nr + 1) * 8);
+    for (size_t i = 0; i < k; i++) {
+        u64 q = c->q[i];
+        u64 v = mulmod(prod_mod(h->r, nr, q), invmod(punctured_mod(c->q, k, i, q), q), q);
+        h->negrq[i] = q - v;
+        for (size_t j = 0; j < nr; j++) h->qinv_r[j * k + i] = invmod(q % h->r[j], h->r[j]);
+    }
+    /* theta_i = t Q [(S/s_i)^-1]_{s_i}, S = Q R (src/rns.cu:833-890) */
+    u64 *s = (u64 *)malloc(ns * 8);
+    memcpy(s, c->q, k * 8);
+    memcpy(s + k, h->r, nr * 8);
+    for (size_t i = 0; i < ns; i++) {
+        u64 shinv = invmod(punctured_mod(s, ns, i, s[i]), s[i]);
+        orc_big th;
+        big_set(&th, h->t);
+        for (size_t m = 0; m < k; m++) big_mul(&th, c->q[m]);
+        big_mul(&th, shinv);
+        if (i >= k) h->frac_r[i - k] = (double)big_mod(&th, s[i]) / (double)s[i];
+        big_div(&th, s[i]);
+        for (size_t m = 0; m < k; m++) {
+            if (i >= k) h->divs_mod_q[m * (nr + 1) + (i - k)] = big_mod(&th, c->q[m]);
+            else if (i == m) h->divs_mod_q[m * (nr + 1) + nr] = big_mod(&th, c->q[m]);
+        }
+    }
+    free(s);
+}
+
+orc_hps *orc_hps_create(const orc_ctx *c, u64 t) { return orc_hps_create2(c, t, 0); }
+
+orc_hps *orc_hps_create2(const orc_ctx *c, u64 t, int overq) {
     orc_hps *h = (orc_hps *)calloc(1, sizeof(orc_hps));
-    size_t k = c->size_Q, nr = k + 1, N = c->N;
-    h->k = k; h->nr = nr; h->t = t;
+    size_t k = c->size_Q, nr = overq ? k : k + 1, N = c->N;
+    h->k = k; h->nr = nr; h->t = t; h->overq = overq;
     u64 qmin = c->q[0];
     for (size_t i = 1; i < k; i++) if (c->q[i] < qmin) qmin = c->q[i];
     h->r = (u64 *)malloc(nr * 8);
@@ -1098,6 +1140,7 @@ orc_hps *orc_hps_create(const orc_ctx *c, u64 t) {
         }
     }
     free(s);
+    if (overq) hps_overq_consts(c, h);
     return h;
 }
 
@@ -1105,6 +1148,7 @@ void orc_hps_destroy(orc_hps *h) {
     if (!h) return;
     free(h->r); free(h->rp); free(h->rps); free(h->irp); free(h->irps); free(h->ninv);
     free(h->frac); free(h->divs_mod_r);
+    free(h->negrq); free(h->qinv_r); free(h->frac_r); free(h->divs_mod_q);
     free(h);
 }
 
@@ -1146,20 +1190,52 @@ static void r_ntt(const orc_hps *h, size_t N, u64 *data, int inverse) {
     }
 }
 
-/* bfv_multiply_hps (src/evaluate.cu:648-801) for mul_tech hps: extend both
+/* DBaseConverter::bConv_BEHZ_var1 Q -> R (src/rns_bconv.cu:231-245):
+ * y_i = [x_i negPQHatInvModq_i]_{q_i}, out_j = [sum_i y_i q_i^-1]_{r_j}.
+ * For X = CRT(x) in [0, Q): out = (R X + u) / Q with u = sum_i y_i Qhat_i in
+ * [0, k Q), i.e. the modulus switch Q -> R of X rounded up by at most k. */
+void orc_hps_modswitch(const orc_ctx *c, const orc_hps *h, const u64 *x, u64 *out) {
+    size_t N = c->N, k = h->k, nr = h->nr;
+    for (size_t n = 0; n < N; n++) {
+        u64 y[64];
+        for (size_t i = 0; i < k; i++) y[i] = mulmod(x[i * N + n], h->negrq[i], c->q[i]);
+        for (size_t j = 0; j < nr; j++) {
+            u128 acc = 0;
+            for (size_t i = 0; i < k; i++) acc += (u128)y[i] * h->qinv_r[j * k + i];
+            out[j * N + n] = (u64)(acc % h->r[j]);
+        }
+    }
+}
+
+int orc_bfv_multiply_hps(const orc_ctx *c, const orc_hps *h, const u64 *ct1, const u64 *ct2, u64 *out3) {
+    return orc_bfv_multiply_hps2(c, h, ct1, ct2, 0, out3);
+}
+
+/* bfv_multiply_hps (src/evaluate.cu:648-801). mul_tech hps: extend both
  * operands Q -> QR (bConv_HPS), NTT, tensor, INTT, scaleAndRound_HPS_QR_R
  * (src/rns.cu:1700-1746: nu = 0.5 + sum x_i frac_i with fused multiply-adds,
- * alpha = (u64) nu), then bConv_HPS R -> Q. ct1, ct2 [2][k][N] -> out3 [3][k][N] */
-int orc_bfv_multiply_hps(const orc_ctx *c, const orc_hps *h, const u64 *ct1, const u64 *ct2, u64 *out3) {
+ * alpha = (u64) nu), then bConv_HPS R -> Q. hps_overq: the second operand is
+ * switched Q -> R (bConv_BEHZ_var1) and converted back R -> Q (bConv_HPS); a
+ * squaring (square != 0) extends the one operand like the first (lines
+ * 717-731); the result is scaleAndRound_HPS_QlRl_Ql (src/rns.cu:1749-1796:
+ * nu = 0.5 + sum_j x_{r_j} frac_j, alpha = (u64) nu, out_i = [sum_j x_{r_j}
+ * D_ij + x_{q_i} D_ik + alpha]_{q_i}). ct1, ct2 [2][k][N] -> out3 [3][k][N] */
+int orc_bfv_multiply_hps2(const orc_ctx *c, const orc_hps *h, const u64 *ct1, const u64 *ct2, int square,
+                          u64 *out3) {
     size_t N = c->N, k = h->k, nr = h->nr, QN = k * N, RN = nr * N, SN = QN + RN;
     u64 *a = (u64 *)malloc(3 * SN * 8), *b = (u64 *)malloc(2 * SN * 8);
-    const u64 *src[2] = {ct1, ct2};
+    const u64 *src[2] = {ct1, square ? ct1 : ct2};
     u64 *dst[2] = {a, b};
     for (int s = 0; s < 2; s++)
         for (int p = 0; p < 2; p++) {
             u64 *d = dst[s] + p * SN;
-            memcpy(d, src[s] + p * QN, QN * 8);
-            bconv_hps(N, c->q, k, h->r, nr, d, d + QN);
+            if (h->overq && s == 1 && !square) {
+                orc_hps_modswitch(c, h, src[s] + p * QN, d + QN);
+                bconv_hps(N, h->r, nr, c->q, k, d + QN, d);
+            } else {
+                memcpy(d, src[s] + p * QN, QN * 8);
+                bconv_hps(N, c->q, k, h->r, nr, d, d + QN);
+            }
             orc_ntt_forward(c, d, k, 0);
             r_ntt(h, N, d + QN, 0);
         }
@@ -1179,6 +1255,22 @@ int orc_bfv_multiply_hps(const orc_ctx *c, const orc_hps *h, const u64 *ct1, con
         u64 *d = a + p * SN;
         orc_ntt_inverse(c, d, k, 0);
         r_ntt(h, N, d + QN, 1);
+        if (h->overq) {
+            for (size_t n = 0; n < N; n++) {
+                double nu = 0.5;
+                for (size_t j = 0; j < nr; j++) nu = fma((double)d[QN + j * N + n], h->frac_r[j], nu);
+                u64 alpha = (u64)nu;
+                for (size_t i = 0; i < k; i++) {
+                    u64 qi = c->q[i];
+                    const u64 *dm = h->divs_mod_q + i * (nr + 1);
+                    u128 acc = 0;
+                    for (size_t j = 0; j < nr; j++) acc += (u128)d[QN + j * N + n] * dm[j];
+                    acc += (u128)d[i * N + n] * dm[nr];
+                    out3[p * QN + i * N + n] = addmod((u64)(acc % qi), alpha % qi, qi);
+                }
+            }
+            continue;
+        }
         for (size_t n = 0; n < N; n++) {
             double nu = 0.5;
             for (size_t i = 0; i < k; i++) nu = fma((double)d[i * N + n], h->frac[i], nu);

@@ -101,9 +101,16 @@ int orc_bfv_relinearize(const orc_ctx *c, const uint64_t *ct3, const uint64_t *k
 /* ---- BFV HPS multiply (mul_tech hps) at the top data level ---- */
 typedef struct orc_hps orc_hps;
 orc_hps *orc_hps_create(const orc_ctx *c, uint64_t t);
+/* overq != 0: mul_tech hps_overq (R = size_Q primes, t/R scale-and-round QR -> Q) */
+orc_hps *orc_hps_create2(const orc_ctx *c, uint64_t t, int overq);
+/* bConv_BEHZ_var1 Q -> R of one polynomial [k][N] -> [nr][N] (over-Q tool) */
+void orc_hps_modswitch(const orc_ctx *c, const orc_hps *h, const uint64_t *x, uint64_t *out);
 void orc_hps_destroy(orc_hps *h);
 size_t orc_hps_r_size(const orc_hps *h);
 void orc_hps_r(const orc_hps *h, uint64_t *out);
+/* square != 0: the reference's squaring path (one operand, extended as the first) */
+int orc_bfv_multiply_hps2(const orc_ctx *c, const orc_hps *h, const uint64_t *ct1, const uint64_t *ct2, int square,
+                          uint64_t *out3);
 int orc_bfv_multiply_hps(const orc_ctx *c, const orc_hps *h, const uint64_t *ct1, const uint64_t *ct2,
                          uint64_t *out3);
 

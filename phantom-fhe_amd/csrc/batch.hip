@@ -274,14 +274,16 @@ BatchPlan *BatchPlan::mul_relin(const Context &ctx, const std::vector<Ciphertext
     p->items_ = ct1.size();
     p->build_groups(chains);
     for (auto &g : p->groups_) {
-        std::vector<const void *> a, b, out, keys;
+        std::vector<const void *> a, b, out, keys, square;
         for (u32 i : g.items) {
             a.push_back(ct1[i]->data);
             b.push_back(ct2[i]->data);
             out.push_back(ct1[i]->data);
             keys.push_back(rlk.data);
+            // BFV: squaring flag (the same ciphertext as both operands)
+            square.push_back(ct1[i]->data == ct2[i]->data ? reinterpret_cast<const void *>(uintptr_t{1}) : nullptr);
         }
-        p->set_tables(g, a, b, out, keys, {});
+        p->set_tables(g, a, b, out, keys, bfv ? square : std::vector<const void *>{});
     }
     p->layout();
     return p.release();
@@ -371,7 +373,8 @@ void BatchPlan::run_group(const Group &g, hipStream_t s) {
         PHX_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_gather_items, items_grid(2 * LN, B), dim3(kThreads), 0, s, g.b, g.in2, 2 * LN, 2 * LN);
         PHX_HIP_CHECK(hipGetLastError());
-        bfv_multiply_batch(ctx, g.in1, g.in2, B, g.work, g.c3, s);
+        // the perms table of a BFV plan holds each item's squaring flag
+        bfv_multiply_batch(ctx, g.in1, g.in2, B, reinterpret_cast<const u64 *>(g.perms), g.work, g.c3, s);
         ntt_forward(ctx.tables(), limb_map(g.c3, 0, Ql, LN), Ql, 3 * B, s);
         keyswitch_group(g, s);
         ntt_inverse(ctx.tables(), limb_map(g.c3, 0, Ql, 3 * LN), Ql, B, s);

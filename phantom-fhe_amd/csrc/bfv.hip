@@ -130,13 +130,9 @@ __global__ __launch_bounds__(kThreads) void k_behz_floor_sk(const u64 *__restric
 // HPS (mul_tech hps). bConv_HPS (src/rns_bconv.cu:277-372): y_i = [x_i Ihat_i^-1]_{i},
 // out_p = sum_i y_i [Ihat_i]_p - v [I]_p with v = llround(sum_i y_i / i_i), the
 // FP64 sum taken with fused multiply-adds in input order.
-// k_hps_extend: Q part copied, R part = bConv_HPS(Q -> R), per polynomial.
-__global__ __launch_bounds__(kThreads) void k_hps_extend(const u64 *__restrict__ in, u64 *__restrict__ out,
-                                                         HpsConsts H, u32 N, u64 in_stride, u64 out_stride) {
-    const u32 k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= N) return;
-    const u64 *x = in + blockIdx.y * in_stride + k;
-    u64 *o = out + blockIdx.y * out_stride + k;
+// hps_extend: Q part copied, R part = bConv_HPS(Q -> R), one coefficient.
+__device__ __forceinline__ void hps_extend(const u64 *__restrict__ x, u64 *__restrict__ o, const HpsConsts &H,
+                                           u32 N) {
     u64 y[kMaxQ];
     double frac = 0.0;
 #pragma unroll
@@ -157,6 +153,98 @@ __global__ __launch_bounds__(kThreads) void k_hps_extend(const u64 *__restrict__
         const u64 a = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
         o[(u64)(H.k + j) * N] = sub_mod(a, H.alphaq_r[(u64)v * H.nr + j], m.value);
     }
+}
+
+__global__ __launch_bounds__(kThreads) void k_hps_extend(const u64 *__restrict__ in, u64 *__restrict__ out,
+                                                         HpsConsts H, u32 N, u64 in_stride, u64 out_stride) {
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    hps_extend(in + blockIdx.y * in_stride + k, out + blockIdx.y * out_stride + k, H, N);
+}
+
+// hps_overq, second operand: R part = bConv_BEHZ_var1(Q -> R) (src/rns_bconv.cu:231-245:
+// y_i = [x_i negPQHatInvModq_i]_{q_i}, z_j = [sum_i y_i q_i^-1]_{r_j}), then the Q part =
+// bConv_HPS(R -> Q) of z (src/evaluate.cu:745-751). One coefficient.
+__device__ __forceinline__ void hps_overq_switch(const u64 *__restrict__ x, u64 *__restrict__ o, const HpsConsts &H,
+                                                 u32 N) {
+    u64 y[kMaxQ];
+#pragma unroll
+    for (u32 i = 0; i < kMaxQ; ++i)
+        if (i < H.k) y[i] = mul_mod(x[(u64)i * N], H.negrq[i], H.mod_qr[i]);
+    u64 w[kMaxBsk];
+    double frac = 0.0;
+#pragma unroll
+    for (u32 j = 0; j < kMaxBsk; ++j)
+        if (j < H.nr) {
+            const DevModulus m = H.mod_qr[H.k + j];
+            u128 acc = 0;
+#pragma unroll
+            for (u32 i = 0; i < kMaxQ; ++i)
+                if (i < H.k) acc += (u128)y[i] * H.qinv_r[(u64)j * H.k + i];
+            const u64 z = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
+            o[(u64)(H.k + j) * N] = z;
+            w[j] = mul_mod(z, H.rhatinv_r[j], m);
+            frac = __builtin_fma((double)w[j], H.rinv[j], frac);
+        }
+    const u32 v = (u32)__builtin_llround(frac);
+#pragma unroll
+    for (u32 i = 0; i < kMaxQ; ++i)
+        if (i < H.k) {
+            const DevModulus m = H.mod_qr[i];
+            u128 acc = 0;
+#pragma unroll
+            for (u32 j = 0; j < kMaxBsk; ++j)
+                if (j < H.nr) acc += (u128)w[j] * H.rhat_q[(u64)i * H.nr + j];
+            o[(u64)i * N] = sub_mod(barrett128(acc, m.value, m.ratio_lo, m.ratio_hi), H.alphar_q[(u64)v * H.k + i],
+                                    m.value);
+        }
+}
+
+// second operands of hps_overq, `per_item` polynomials per item; an item whose
+// square flag is set (the reference's squaring path) is extended like a first operand
+__global__ __launch_bounds__(kThreads) void k_hps_overq_switch(const u64 *__restrict__ in, u64 *__restrict__ out,
+                                                               HpsConsts H, u32 N, u64 in_stride, u64 out_stride,
+                                                               const u64 *__restrict__ square, u32 per_item) {
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 *x = in + blockIdx.y * in_stride + k;
+    u64 *o = out + blockIdx.y * out_stride + k;
+    if (square && square[blockIdx.y / per_item])
+        hps_extend(x, o, H, N);
+    else
+        hps_overq_switch(x, o, H, N);
+}
+
+// scaleAndRound_HPS_QlRl_Ql (src/rns.cu:1749-1796): nu = 0.5 + sum_j x_{r_j} frac_r_j
+// (fused multiply-adds in order), alpha = (u64) nu,
+// out_i = [sum_j x_{r_j} D_ij + x_{q_i} D_ik]_{q_i} + [alpha]_{q_i}
+__global__ __launch_bounds__(kThreads) void k_hps_overq_round(const u64 *__restrict__ in, u64 *__restrict__ out,
+                                                              HpsConsts H, u32 N, u64 in_stride, u64 out_stride) {
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 *x = in + blockIdx.y * in_stride + k;
+    u64 *o = out + blockIdx.y * out_stride + k;
+    u64 xr[kMaxBsk];
+    double nu = 0.5;
+#pragma unroll
+    for (u32 j = 0; j < kMaxBsk; ++j)
+        if (j < H.nr) {
+            xr[j] = x[(u64)(H.k + j) * N];
+            nu = __builtin_fma((double)xr[j], H.frac_r[j], nu);
+        }
+    const u64 alpha = (u64)nu;
+#pragma unroll
+    for (u32 i = 0; i < kMaxQ; ++i)
+        if (i < H.k) {
+            const DevModulus m = H.mod_qr[i];
+            const u64 *dm = H.divs_mod_q + (u64)i * (H.nr + 1);
+            u128 acc = (u128)x[(u64)i * N] * dm[H.nr];
+#pragma unroll
+            for (u32 j = 0; j < kMaxBsk; ++j)
+                if (j < H.nr) acc += (u128)xr[j] * dm[j];
+            o[(u64)i * N] = add_mod(barrett128(acc, m.value, m.ratio_lo, m.ratio_hi),
+                                    barrett64(alpha, m.value, m.ratio_hi), m.value);
+        }
 }
 
 // scaleAndRound_HPS_QR_R (src/rns.cu:1700-1746) into registers, then
@@ -260,7 +348,8 @@ void behz_mul_1(const Context &ctx, const BehzConsts &B, const Ciphertext &ct, u
 }  // namespace
 
 void bfv_multiply(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, hipStream_t s) {
-    if (ctx.params().mul_tech == kMulHPS) return bfv_multiply_hps(ctx, ct1, ct2, s);
+    if (ctx.params().mul_tech == kMulHPS || ctx.params().mul_tech == kMulHPSOverQ)
+        return bfv_multiply_hps(ctx, ct1, ct2, s);
     const BehzConsts &B = behz_of(ctx);
     require(ct1.chain_index == ct2.chain_index, "encrypted1 and encrypted2 parameter mismatch");
     require(!ct1.is_ntt_form && !ct2.is_ntt_form, "encrypted1 or encrypted2 cannot be in NTT form");
@@ -318,8 +407,12 @@ void bfv_multiply_hps(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2
     PHX_HIP_CHECK(hipGetLastError());
     ntt_forward(ctx.tables(), m, H.k + H.nr, 2, s);
     if (!square) {
-        hipLaunchKernelGGL(k_hps_extend, grid2(N, 2), dim3(kThreads), 0, s, (const u64 *)ct2.data, b, H, (u32)N, QN,
-                           SN);
+        if (H.overq)
+            hipLaunchKernelGGL(k_hps_overq_switch, grid2(N, 2), dim3(kThreads), 0, s, (const u64 *)ct2.data, b, H,
+                               (u32)N, QN, SN, (const u64 *)nullptr, 2u);
+        else
+            hipLaunchKernelGGL(k_hps_extend, grid2(N, 2), dim3(kThreads), 0, s, (const u64 *)ct2.data, b, H, (u32)N,
+                               QN, SN);
         PHX_HIP_CHECK(hipGetLastError());
         LimbMap mb = m;
         mb.data = b;
@@ -328,8 +421,12 @@ void bfv_multiply_hps(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2
     tensor_2x2_mod(H.mod_qr, N, H.k + H.nr, a, b, a, a + SN, a + 2 * SN, s);
     ntt_inverse(ctx.tables(), m, H.k + H.nr, 3, s);
     ct1.resize(ctx, ct1.chain_index, 3, s);
-    hipLaunchKernelGGL(k_hps_round_convert, grid2(N, 3), dim3(kThreads), 0, s, (const u64 *)a, ct1.data, H, (u32)N,
-                       SN, QN);
+    if (H.overq)
+        hipLaunchKernelGGL(k_hps_overq_round, grid2(N, 3), dim3(kThreads), 0, s, (const u64 *)a, ct1.data, H,
+                           (u32)N, SN, QN);
+    else
+        hipLaunchKernelGGL(k_hps_round_convert, grid2(N, 3), dim3(kThreads), 0, s, (const u64 *)a, ct1.data, H,
+                           (u32)N, SN, QN);
     PHX_HIP_CHECK(hipGetLastError());
     if (!square) dev_free(b, s);
     dev_free(a, s);
@@ -339,7 +436,7 @@ void bfv_multiply_hps(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2
 
 size_t bfv_batch_work_words(const Context &ctx) {
     const u64 N = ctx.N();
-    if (ctx.params().mul_tech == kMulHPS) {
+    if (ctx.params().mul_tech == kMulHPS || ctx.params().mul_tech == kMulHPSOverQ) {
         const HpsConsts *H = ctx.hps();
         require(H != nullptr, "the context has no HPS tool: scheme must be BFV with mul_tech hps");
         return (size_t)7 * (H->k + H->nr) * N;   // extended operands 2 + 2, tensor 3
@@ -348,11 +445,12 @@ size_t bfv_batch_work_words(const Context &ctx) {
     return (size_t)N * (7ull * B.nb + 3ull * B.k);   // Bsk operands 2 + 2, tensor 3 (q and Bsk)
 }
 
-void bfv_multiply_batch(const Context &ctx, u64 *in1, u64 *in2, u32 items, u64 *work, u64 *out, hipStream_t s) {
+void bfv_multiply_batch(const Context &ctx, u64 *in1, u64 *in2, u32 items, const u64 *square, u64 *work, u64 *out,
+                        hipStream_t s) {
     require(items >= 1, "empty batch");
     const u64 N = ctx.N();
     const u32 nz = items;
-    if (ctx.params().mul_tech == kMulHPS) {
+    if (ctx.params().mul_tech == kMulHPS || ctx.params().mul_tech == kMulHPSOverQ) {
         const HpsConsts *Hp = ctx.hps();
         require(Hp != nullptr, "the context has no HPS tool: scheme must be BFV with mul_tech hps");
         const HpsConsts &H = *Hp;
@@ -365,8 +463,12 @@ void bfv_multiply_batch(const Context &ctx, u64 *in1, u64 *in2, u32 items, u64 *
         m.special_row = H.r_row0;
         for (int o = 0; o < 2; ++o) {
             u64 *e = o ? e2 : e1;
-            hipLaunchKernelGGL(k_hps_extend, grid2(N, 2 * nz), dim3(kThreads), 0, s, (const u64 *)(o ? in2 : in1), e,
-                               H, (u32)N, QN, SN);
+            if (o && H.overq)
+                hipLaunchKernelGGL(k_hps_overq_switch, grid2(N, 2 * nz), dim3(kThreads), 0, s, (const u64 *)in2, e,
+                                   H, (u32)N, QN, SN, square, 2u);
+            else
+                hipLaunchKernelGGL(k_hps_extend, grid2(N, 2 * nz), dim3(kThreads), 0, s,
+                                   (const u64 *)(o ? in2 : in1), e, H, (u32)N, QN, SN);
             PHX_HIP_CHECK(hipGetLastError());
             LimbMap me = m;
             me.data = e;
@@ -378,8 +480,12 @@ void bfv_multiply_batch(const Context &ctx, u64 *in1, u64 *in2, u32 items, u64 *
         LimbMap mt = m;
         mt.data = t;
         ntt_inverse(ctx.tables(), mt, S, 3 * nz, s);
-        hipLaunchKernelGGL(k_hps_round_convert, grid2(N, 3 * nz), dim3(kThreads), 0, s, (const u64 *)t, out, H,
-                           (u32)N, SN, QN);
+        if (H.overq)
+            hipLaunchKernelGGL(k_hps_overq_round, grid2(N, 3 * nz), dim3(kThreads), 0, s, (const u64 *)t, out, H,
+                               (u32)N, SN, QN);
+        else
+            hipLaunchKernelGGL(k_hps_round_convert, grid2(N, 3 * nz), dim3(kThreads), 0, s, (const u64 *)t, out, H,
+                               (u32)N, SN, QN);
         PHX_HIP_CHECK(hipGetLastError());
         return;
     }

@@ -590,7 +590,8 @@ int phx_multiply_inplace(const phx_context *ctx, phx_ciphertext *ct1, const phx_
         const Ciphertext &b = ct2->ct;
         if (ctx->ctx->params().scheme == kSchemeBFV) {
             // bfv_multiply (src/evaluate.cu:1046-1048), BEHZ
-            require(ctx->ctx->params().mul_tech == kMulBEHZ || ctx->ctx->params().mul_tech == kMulHPS,
+            require(ctx->ctx->params().mul_tech == kMulBEHZ || ctx->ctx->params().mul_tech == kMulHPS ||
+                        ctx->ctx->params().mul_tech == kMulHPSOverQ,
                     "mul_tech not supported for bfv_multiply");
             bfv_multiply(*ctx->ctx, a, b, S(s));
             return;

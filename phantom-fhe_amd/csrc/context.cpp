@@ -165,14 +165,17 @@ Context::Context(const Params &p) : params_(p) {
             for (u64 q : p.coeff_modulus) require(b != q, "coefficient modulus collides with the BEHZ base");
     }
     // ---- HPS base R (src/rns.cu:686-699): size_Q + 1 primes below the smallest q,
-    // stepping down by 2N within its bit size (get_primes_below, src/host/numth.cu:235-262)
-    if (p.scheme == kSchemeBFV && p.mul_tech == kMulHPS) {
+    // stepping down by 2N within its bit size (get_primes_below, src/host/numth.cu:235-262);
+    // size_Q primes for hps_overq (src/rns.cu:792-801)
+    const bool overq = p.mul_tech == kMulHPSOverQ;
+    if (p.scheme == kSchemeBFV && (p.mul_tech == kMulHPS || overq)) {
         require(p.plain_modulus > 1, "plain_modulus is not set");
         const u64 qmin = *std::min_element(p.coeff_modulus.begin(), p.coeff_modulus.begin() + size_Q_);
         const u64 lower = 1ull << (hm::bit_count(qmin) - 1);
-        for (u64 v = qmin - 2 * N_; hps_r_.size() < size_Q_ + 1 && v > lower; v -= 2 * N_)
+        const size_t nr = overq ? size_Q_ : size_Q_ + 1;
+        for (u64 v = qmin - 2 * N_; hps_r_.size() < nr && v > lower; v -= 2 * N_)
             if (hm::is_prime(v)) hps_r_.push_back(v);
-        require(hps_r_.size() == size_Q_ + 1, "failed to find enough qualifying primes");
+        require(hps_r_.size() == nr, "failed to find enough qualifying primes");
         for (u64 r : hps_r_)
             for (u64 q : p.coeff_modulus) require(r != q, "coefficient modulus collides with the HPS base");
     }
@@ -439,6 +442,7 @@ Context::Context(const Params &p) : params_(p) {
     // HPS constants
     struct HpsOffs {
         size_t mod_qr, qhatinv_q, qhat_r, qinv, alphaq_r, frac, divs_mod_r, rhatinv_r, rhat_q, rinv, alphar_q;
+        size_t negrq, qinv_r, frac_r, divs_mod_q;
     } ho{};
     if (!hps_r_.empty()) {
         const u32 k = size_Q_, nr = (u32)hps_r_.size(), ns = k + nr;
@@ -509,9 +513,66 @@ Context::Context(const Params &p) : params_(p) {
             std::memcpy(w.data(), v.data(), v.size() * 8);
             return cb.push64(w);
         };
-        ho = {cb.push_struct(mqr), cb.push64(qhatinv), cb.push64(qhat_r), pushd(qinv), cb.push64(alphaq_r),
-              pushd(frac),         cb.push64(divs),    cb.push64(rhatinv), cb.push64(rhat_q), pushd(rinv),
-              cb.push64(alphar_q)};
+        ho.mod_qr = cb.push_struct(mqr);
+        ho.qhatinv_q = cb.push64(qhatinv);
+        ho.qhat_r = cb.push64(qhat_r);
+        ho.qinv = pushd(qinv);
+        ho.alphaq_r = cb.push64(alphaq_r);
+        ho.frac = pushd(frac);
+        ho.divs_mod_r = cb.push64(divs);
+        ho.rhatinv_r = cb.push64(rhatinv);
+        ho.rhat_q = cb.push64(rhat_q);
+        ho.rinv = pushd(rinv);
+        ho.alphar_q = cb.push64(alphar_q);
+        if (overq) {
+            // bConv_BEHZ_var1 Q -> R tables (src/host/rns.cu:466-497 with P = R) and the
+            // t/R scale-and-round QR -> Q: theta_i = t Q [(S/s_i)^-1]_{s_i},
+            // frac_r_j = [theta_{k+j}]_{r_j} / r_j, D_ij = floor(theta_{k+j} / r_j) mod q_i,
+            // D_ik = floor(theta_i / q_i) mod q_i (src/rns.cu:833-890)
+            std::vector<u64> negrq(k), qinv_r((size_t)nr * k), dq((size_t)k * (nr + 1));
+            std::vector<double> frac_r(nr);
+            for (u32 i = 0; i < k; ++i) {
+                const u64 rq = hm::mulmod(hm::product_mod(rp, nr, qp[i]), qhatinv[i], qp[i]);
+                negrq[i] = qp[i] - rq;
+                for (u32 j = 0; j < nr; ++j) qinv_r[(size_t)j * k + i] = hm::invmod(qp[i] % rp[j], rp[j]);
+            }
+            for (u32 i = 0; i < ns; ++i) {
+                const u64 shinv = hm::invmod(hm::punctured_mod(s.data(), ns, i, s[i]), s[i]);
+                std::vector<u64> th{t};
+                auto mul = [&](u64 m) {
+                    u128 c = 0;
+                    for (auto &w : th) {
+                        const u128 v = (u128)w * m + c;
+                        w = (u64)v;
+                        c = v >> 64;
+                    }
+                    if (c) th.push_back((u64)c);
+                };
+                auto mod = [&](u64 m) {
+                    u128 r = 0;
+                    for (size_t w = th.size(); w-- > 0;) r = ((r << 64) | th[w]) % m;
+                    return (u64)r;
+                };
+                for (u32 m = 0; m < k; ++m) mul(qp[m]);
+                mul(shinv);
+                if (i >= k) frac_r[i - k] = (double)mod(s[i]) / (double)s[i];
+                u128 r = 0;
+                for (size_t w = th.size(); w-- > 0;) {
+                    const u128 cur = (r << 64) | th[w];
+                    th[w] = (u64)(cur / s[i]);
+                    r = cur % s[i];
+                }
+                for (u32 m = 0; m < k; ++m) {
+                    if (i >= k) dq[(size_t)m * (nr + 1) + (i - k)] = mod(qp[m]);
+                    else if (i == m) dq[(size_t)m * (nr + 1) + nr] = mod(qp[m]);
+                }
+            }
+            ho.negrq = cb.push64(negrq);
+            ho.qinv_r = cb.push64(qinv_r);
+            ho.frac_r = pushd(frac_r);
+            ho.divs_mod_q = cb.push64(dq);
+            hps_.overq = 1;
+        }
         hps_.k = k;
         hps_.nr = nr;
         hps_.r_row0 = (u32)(QP + bsk_.size());
@@ -544,6 +605,12 @@ Context::Context(const Params &p) : params_(p) {
         hps_.rhat_q = at(ho.rhat_q);
         hps_.rinv = atd(ho.rinv);
         hps_.alphar_q = at(ho.alphar_q);
+        if (hps_.overq) {
+            hps_.negrq = at(ho.negrq);
+            hps_.qinv_r = at(ho.qinv_r);
+            hps_.frac_r = atd(ho.frac_r);
+            hps_.divs_mod_q = at(ho.divs_mod_q);
+        }
     }
     if (behz_.k) {
         auto at = [&](size_t off) { return reinterpret_cast<const u64 *>(cbb + off); };

@@ -16,7 +16,7 @@
 namespace phx {
 
 enum SchemeType : int { kSchemeNone = 0, kSchemeBGV = 1, kSchemeBFV = 2, kSchemeCKKS = 3 };
-enum MulTech : int { kMulNone = 0, kMulBEHZ = 1, kMulHPS = 2 };
+enum MulTech : int { kMulNone = 0, kMulBEHZ = 1, kMulHPS = 2, kMulHPSOverQ = 3 };
 
 struct Params {
     int scheme = kSchemeCKKS;
@@ -113,6 +113,13 @@ struct HpsConsts {
     const u64 *rhat_q;          // [k][nr]    Rhat_j mod q_i
     const double *rinv;         // [nr]       1 / r_j
     const u64 *alphar_q;        // [nr+1][k]  v R mod q_i
+    // over Q (mul_tech hps_overq; nr = k): switch Q -> R, t/R scale-and-round QR -> Q
+    u32 overq = 0;
+    u32 pad2 = 0;
+    const u64 *negrq;           // [k]        q_i - [R Qhat_i^-1]_{q_i}
+    const u64 *qinv_r;          // [nr][k]    q_i^-1 mod r_j
+    const double *frac_r;       // [nr]       tQlSlHatInvModsDivsFrac
+    const u64 *divs_mod_q;      // [k][nr+1]  tQlSlHatInvModsDivsModq
 };
 
 // Kernel variants chosen for one context (by the autotuner or a tuning cache);

@@ -620,9 +620,11 @@ void keyswitch(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey &k
 void multiply_and_relin(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, const KSwitchKey &rlk,
                         hipStream_t s) {
     if (ctx.params().scheme == kSchemeBFV) {
-        // BEHZ multiply then relinearize (src/evaluate.cu:1087-1093); for hps the
-        // reference's fused bfv_mul_relin_hps (818-1027) runs the same two steps
-        require(ctx.params().mul_tech == kMulBEHZ || ctx.params().mul_tech == kMulHPS,
+        // BEHZ multiply then relinearize (src/evaluate.cu:1087-1093); for hps and
+        // hps_overq the reference's fused bfv_mul_relin_hps (818-1027) runs the same
+        // two steps
+        require(ctx.params().mul_tech == kMulBEHZ || ctx.params().mul_tech == kMulHPS ||
+                    ctx.params().mul_tech == kMulHPSOverQ,
                 "unsupported mul tech in BFV mul&relin");
         bfv_multiply(ctx, ct1, ct2, s);
         bfv_relinearize(ctx, ct1, rlk, s);

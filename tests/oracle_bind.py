@@ -71,6 +71,10 @@ def lib():
         L.orc_hoisting.argtypes = [ctypes.c_void_p, _sz, _u64p, _u32p, _sz, _u64p, _sz, _u64p]
         L.orc_hps_create.restype = ctypes.c_void_p
         L.orc_hps_create.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+        L.orc_hps_create2.restype = ctypes.c_void_p
+        L.orc_hps_create2.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int]
+        L.orc_hps_modswitch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p, _u64p]
+        L.orc_bfv_multiply_hps2.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p, _u64p, ctypes.c_int, _u64p]
         L.orc_hps_destroy.argtypes = [ctypes.c_void_p]
         L.orc_hps_r_size.restype = _sz
         L.orc_hps_r_size.argtypes = [ctypes.c_void_p]
@@ -261,7 +265,8 @@ class Behz:
         lib().orc_behz_bsk(self._h, _p(out))
         return [int(x) for x in out]
 
-    def multiply(self, ct1, ct2):
+    def multiply(self, ct1, ct2, square=False):
+        """BEHZ squaring is the product of the operand with itself (square is ignored)"""
         k = self.octx.size_QP - self.octx.size_P
         out = np.zeros((3, k, self.octx.N), np.uint64)
         rc = lib().orc_bfv_multiply(self.octx._h, self._h, _p(np.ascontiguousarray(ct1, np.uint64)),
@@ -282,10 +287,11 @@ def bfv_relinearize(octx, ct3, key):
 class Hps:
     """HPS constants of an oracle context's top data level, BFV multiply."""
 
-    def __init__(self, octx, t):
+    def __init__(self, octx, t, overq=False):
         self.octx = octx
         self.t = t
-        self._h = lib().orc_hps_create(octx._h, t)
+        self.overq = overq
+        self._h = lib().orc_hps_create2(octx._h, t, 1 if overq else 0)
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -298,10 +304,16 @@ class Hps:
         lib().orc_hps_r(self._h, _p(out))
         return [int(x) for x in out]
 
-    def multiply(self, ct1, ct2):
+    def multiply(self, ct1, ct2, square=False):
         k = self.octx.size_QP - self.octx.size_P
         out = np.zeros((3, k, self.octx.N), np.uint64)
-        rc = lib().orc_bfv_multiply_hps(self.octx._h, self._h, _p(np.ascontiguousarray(ct1, np.uint64)),
-                                        _p(np.ascontiguousarray(ct2, np.uint64)), _p(out))
+        rc = lib().orc_bfv_multiply_hps2(self.octx._h, self._h, _p(np.ascontiguousarray(ct1, np.uint64)),
+                                         _p(np.ascontiguousarray(ct2, np.uint64)), 1 if square else 0, _p(out))
         assert rc == 0
+        return out
+
+    def modswitch(self, x):
+        """bConv_BEHZ_var1 Q -> R of x [k][N] (over-Q tool)"""
+        out = np.zeros((len(self.r()), self.octx.N), np.uint64)
+        lib().orc_hps_modswitch(self.octx._h, self._h, _p(np.ascontiguousarray(x, np.uint64)), _p(out))
         return out

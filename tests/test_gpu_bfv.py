@@ -28,7 +28,7 @@ def make_bfv(N, k, tech=ph.MUL_TECH_BEHZ):
     parms.set_mul_tech(tech)
     ctx = ph.context(parms)
     octx = ob.Ctx(N.bit_length() - 1, mods, 1)
-    tool = ob.Behz(octx, T) if tech == ph.MUL_TECH_BEHZ else ob.Hps(octx, T)
+    tool = ob.Behz(octx, T) if tech == ph.MUL_TECH_BEHZ else ob.Hps(octx, T, overq=tech == ph.MUL_TECH_HPS_OVERQ)
     return ctx, octx, tool, mods
 
 
@@ -135,13 +135,16 @@ def poly_mul_rns(octx, x, y, L):
     return octx.ntt_inverse(((fx.astype(object) * fy.astype(object)) % qs).astype(np.uint64), 0)
 
 
-@pytest.mark.parametrize("N,k", [(1 << 13, 4), (1 << 15, 16)])
-def test_hps_multiply_and_relin_bit_exact(N, k):
-    """mul_tech hps: FP64-corrected base extension and scale-and-round, then
-    relinearisation, bit-exact against the oracle (same fused multiply-adds)."""
+@pytest.mark.parametrize("N,k,tech", [(1 << 13, 4, ph.MUL_TECH_HPS), (1 << 15, 16, ph.MUL_TECH_HPS),
+                                      (1 << 13, 4, ph.MUL_TECH_HPS_OVERQ), (1 << 15, 16, ph.MUL_TECH_HPS_OVERQ)])
+def test_hps_multiply_and_relin_bit_exact(N, k, tech):
+    """mul_tech hps / hps_overq: FP64-corrected base extension and scale-and-round
+    (over Q: the second operand switched Q -> R, t/R rounding QR -> Q; a squaring
+    extends the one operand like the first), then relinearisation, bit-exact
+    against the oracle (same fused multiply-adds)."""
     ob.set_threads(16)
-    say(f"case: hps multiply N={N} k={k}")
-    ctx, octx, hps, mods = make_bfv(N, k, ph.MUL_TECH_HPS)
+    say(f"case: hps multiply N={N} k={k} tech={tech}")
+    ctx, octx, hps, mods = make_bfv(N, k, tech)
     rng = np.random.default_rng(N + k)
     a, b = rand_ct(rng, mods[:k], N), rand_ct(rng, mods[:k], N)
     c1, c2 = upload_bfv(ctx, a), upload_bfv(ctx, b)
@@ -150,7 +153,7 @@ def test_hps_multiply_and_relin_bit_exact(N, k):
     assert np.array_equal(c1.download(N), want3)
     c3 = upload_bfv(ctx, a)
     ph.multiply_inplace(ctx, c3, c3)
-    assert np.array_equal(c3.download(N), hps.multiply(a, a))
+    assert np.array_equal(c3.download(N), hps.multiply(a, a, square=True))
     key = rand_key(rng, mods, N, ctx.dnum)
     rk = ph.relin_key(ctx, host=key)
     c4, c5 = upload_bfv(ctx, a), upload_bfv(ctx, b)
@@ -158,7 +161,7 @@ def test_hps_multiply_and_relin_bit_exact(N, k):
     assert np.array_equal(c4.download(N), ob.bfv_relinearize(octx, want3, key))
 
 
-@pytest.mark.parametrize("tech", [ph.MUL_TECH_BEHZ, ph.MUL_TECH_HPS])
+@pytest.mark.parametrize("tech", [ph.MUL_TECH_BEHZ, ph.MUL_TECH_HPS, ph.MUL_TECH_HPS_OVERQ])
 def test_bfv_encrypt_multiply_relin_decrypt(tech):
     N, k = 1 << 13, 4
     say(f"case: bfv semantic multiply tech={tech}")
@@ -212,7 +215,7 @@ def plain_mul_relin(ctx, a, b, rk, square=False):
     return c1.download(ctx.N)
 
 
-@pytest.mark.parametrize("tech", [ph.MUL_TECH_BEHZ, ph.MUL_TECH_HPS])
+@pytest.mark.parametrize("tech", [ph.MUL_TECH_BEHZ, ph.MUL_TECH_HPS, ph.MUL_TECH_HPS_OVERQ])
 def test_bfv_batch_bit_exact(tech):
     """batched BFV multiply + relin == plain path == oracle, squaring item included"""
     ob.set_threads(8)
@@ -234,8 +237,8 @@ def test_bfv_batch_bit_exact(tech):
         assert c1[i].size() == 2 and not c1[i].is_ntt_form()
         assert np.array_equal(got, plain_mul_relin(ctx, a, b, rk, square=(i == 3))), i
         if i in (0, 3):
-            other = a if i == 3 else b
-            assert np.array_equal(got, ob.bfv_relinearize(octx, tool.multiply(a, other), key)), i
+            prod = tool.multiply(a, a, square=True) if i == 3 else tool.multiply(a, b)
+            assert np.array_equal(got, ob.bfv_relinearize(octx, prod, key)), i
 
 
 def test_bfv_batch_rejects():

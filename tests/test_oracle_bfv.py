@@ -224,3 +224,107 @@ def test_hps_multiply_matches_integer_round(logN, bits):
             assert abs(diff) <= bound, (d, n, diff, bound)
     if bound:
         assert worst <= bound
+
+
+# ---- HPS over Q (mul_tech hps_overq) ----
+
+def _centred(x, M):
+    return x - M if 2 * x > M else x
+
+
+def test_hps_overq_r_primes_closed_form():
+    """R has size_Q primes (not size_Q + 1), the largest below min q stepping by 2N"""
+    octx, mods, N = setup(5, [30, 30, 30, 31], 1)
+    h = ob.Hps(octx, T, overq=True)
+    qmin = min(mods[:3])
+    want, v = [], qmin - 2 * N
+    from bigint_ref import is_prime
+    while len(want) < 3:
+        if is_prime(v):
+            want.append(v)
+        v -= 2 * N
+    assert h.r() == want
+
+
+@pytest.mark.parametrize("logN,bits", [(5, [30, 30, 30, 31]), (4, [54, 54, 54, 54, 60])])
+def test_hps_overq_modswitch_closed_form(logN, bits):
+    """bConv_BEHZ_var1 Q -> R: Z = (R X + u) / Q with 0 <= u < k Q, for X = CRT(x) in [0, Q)"""
+    octx, mods, N = setup(logN, bits, 1)
+    k = len(mods) - 1
+    qs = mods[:k]
+    h = ob.Hps(octx, T, overq=True)
+    rs = h.r()
+    Q, R = 1, 1
+    for q in qs:
+        Q *= q
+    for r in rs:
+        R *= r
+    rng = np.random.default_rng(logN + k)
+    x = np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs])
+    x[:, 0] = 0                       # X = 0 -> Z = u / Q, u = 0
+    x[:, 1] = [q - 1 for q in qs]     # X = Q - 1
+    z = h.modswitch(x)
+    for n in range(N):
+        X = crt([int(v) for v in x[:, n]], qs)[0]
+        Z = crt([int(v) for v in z[:, n]], rs)[0]
+        d = Z * Q - R * X
+        if d < 0:
+            d += R * Q                # Z wrapped past R
+        assert 0 <= d < k * Q, (n, d)
+    assert crt([int(v) for v in z[:, 0]], rs)[0] == 0
+
+
+@pytest.mark.parametrize("logN,bits,square", [(5, [30, 30, 30, 31], False), (4, [54, 54, 54, 54, 60], False),
+                                              (5, [30, 30, 30, 31], True)])
+def test_hps_overq_multiply_matches_integer_round(logN, bits, square):
+    """hps_overq: the output is round(t z / R) mod Q (within the FP64 bound), where z
+    is the integer negacyclic tensor of the centred X (ct1 mod Q) and Y: the centred
+    modulus switch of ct2 (mod R) or, for a squaring, X again"""
+    octx, mods, N = setup(logN, bits, 1)
+    k = len(mods) - 1
+    qs = mods[:k]
+    h = ob.Hps(octx, T, overq=True)
+    rs = h.r()
+    Q, R = 1, 1
+    for q in qs:
+        Q *= q
+    for r in rs:
+        R *= r
+    rng = np.random.default_rng(logN * 7 + k + square)
+    ct1 = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs]) for _ in range(2)])
+    ct2 = ct1 if square else np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs])
+                                       for _ in range(2)])
+    out = h.multiply(ct1, ct2, square=square)
+    X = [[_centred(crt([int(v) for v in ct1[p, :, n]], qs)[0], Q) for n in range(N)] for p in range(2)]
+    if square:
+        Y = X
+    else:
+        Y = []
+        for p in range(2):
+            z = h.modswitch(ct2[p])
+            Y.append([_centred(crt([int(v) for v in z[:, n]], rs)[0], R) for n in range(N)])
+    zs = (negacyclic_int(X[0], Y[0]),
+          [a + b for a, b in zip(negacyclic_int(X[0], Y[1]), negacyclic_int(X[1], Y[0]))],
+          negacyclic_int(X[1], Y[1]))
+    rbits = max(r.bit_length() for r in rs)
+    err = k * k * 2.0 ** (rbits - 53)
+    bound = 0 if err < 2.0 ** -10 else int(err) + 1
+    for d, z in enumerate(zs):
+        for n in range(N):
+            num = T * z[n]
+            w = (2 * num + R) // (2 * R)        # round half up
+            got = crt([int(out[d, i, n]) for i in range(k)], qs)[0]
+            diff = _centred((got - w) % Q, Q)
+            assert abs(diff) <= bound, (d, n, diff, bound)
+
+
+def test_hps_overq_differs_from_hps():
+    """the two techniques give different (both correctly rounded) results"""
+    octx, mods, N = setup(4, [30, 30, 30, 31], 1)
+    rng = np.random.default_rng(3)
+    qs = mods[:3]
+    ct1 = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs]) for _ in range(2)])
+    ct2 = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs]) for _ in range(2)])
+    a = ob.Hps(octx, T).multiply(ct1, ct2)
+    b = ob.Hps(octx, T, overq=True).multiply(ct1, ct2)
+    assert not np.array_equal(a, b)
