@@ -172,7 +172,8 @@ int phx_apply_galois_inplace(const phx_context *ctx, phx_ciphertext *ct, uint32_
 int phx_rotate_inplace(const phx_context *ctx, phx_ciphertext *ct, int step, const phx_galois_key *gk,
                        phx_stream stream);
 /* hoisting_inplace (include/evaluate.cuh:237-238, src/evaluate.cu:1670-1865): ct <- the sum
- * of the rotations of ct by steps[0..n-1], one modup shared by all of them (CKKS) */
+ * of the rotations of ct by steps[0..n-1], one modup shared by all of them (CKKS in NTT
+ * form; BFV in coefficient form) */
 int phx_hoisting_inplace(const phx_context *ctx, phx_ciphertext *ct, const phx_galois_key *gk, const int *steps,
                          size_t n, phx_stream stream);
 /* keyswitch_inplace (include/evaluate.cuh:29-32, src/eval_key_switch.cu:95-182):

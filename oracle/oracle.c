@@ -1328,3 +1328,36 @@ int orc_hoisting(const orc_ctx *c, size_t size_Ql, const u64 *ct, const uint32_t
     free(t); free(tp); free(acc0); free(acx); free(cx); free(perm);
     return 0;
 }
+
+/* hoisting_inplace for BFV (src/evaluate.cu:1670-1865, coefficient form): c0 is
+ * permuted in the coefficient domain (apply_galois: X^j -> X^{j g mod 2N}, negated
+ * when j g mod 2N >= N, include/galois.cuh), c1 is key-switched as in the CKKS
+ * form on its NTT and brought back: the BFV modup / moddown_from_NTT of the
+ * reference are the NTT-domain ones conjugated by the (exact) transform. */
+int orc_hoisting_bfv(const orc_ctx *c, size_t size_Ql, const u64 *ct, const uint32_t *elts, size_t n,
+                     const u64 *keys, size_t dnum, u64 *out2) {
+    size_t N = c->N, LN = size_Ql * N;
+    u64 *x = (u64 *)calloc(2 * LN, 8), *acc0 = (u64 *)calloc(LN, 8);
+    memcpy(x + LN, ct + LN, LN * 8);
+    orc_ntt_forward(c, x + LN, size_Ql, 0);
+    int rc = orc_hoisting(c, size_Ql, x, elts, n, keys, dnum, out2);
+    if (rc == 0) {
+        orc_ntt_inverse(c, out2, size_Ql, 0);
+        orc_ntt_inverse(c, out2 + LN, size_Ql, 0);
+        for (size_t e = 0; e < n; e++) {
+            u64 g = elts[e];
+            for (size_t i = 0; i < size_Ql; i++) {
+                u64 q = c->q[i];
+                for (size_t j = 0; j < N; j++) {
+                    u64 d = (u64)((u128)j * g % (2 * N));
+                    u64 v = ct[i * N + j];
+                    if (d >= N) { d -= N; v = v ? q - v : 0; }
+                    acc0[i * N + d] = addmod(acc0[i * N + d], v, q);
+                }
+            }
+        }
+        for (size_t i = 0; i < LN; i++) out2[i] = addmod(out2[i], acc0[i], c->q[i / N]);
+    }
+    free(x); free(acc0);
+    return rc;
+}

@@ -79,6 +79,9 @@ int orc_apply_galois(const orc_ctx *c, size_t size_Ql, const uint64_t *ct, uint3
  * keys [n][dnum][2][QP][N] in the order of elts */
 int orc_hoisting(const orc_ctx *c, size_t size_Ql, const uint64_t *ct, const uint32_t *elts, size_t n,
                  const uint64_t *keys, size_t dnum, uint64_t *out2);
+/* the same for BFV (ct and out2 in coefficient form) */
+int orc_hoisting_bfv(const orc_ctx *c, size_t size_Ql, const uint64_t *ct, const uint32_t *elts, size_t n,
+                     const uint64_t *keys, size_t dnum, uint64_t *out2);
 
 /* key material from explicit samples: s_ntt, s2_ntt [QP][N] (NTT), a [dnum][QP][N] (NTT),
  * e [dnum][QP][N] (coefficient domain, canonical residues) */

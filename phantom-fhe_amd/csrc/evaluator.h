@@ -80,7 +80,8 @@ void relinearize(const Context &ctx, Ciphertext &ct, const KSwitchKey &rlk, hipS
 void rescale_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, hipStream_t s);
 void apply_galois(const Context &ctx, Ciphertext &ct, u32 elt, const GaloisKeys &gk, hipStream_t s);
 void rotate(const Context &ctx, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s);
-// hoisting_inplace: ct <- sum_i rot(ct, steps[i]) with one shared modup (CKKS)
+// hoisting_inplace: ct <- sum_i rot(ct, steps[i]) with one shared modup
+// (CKKS in NTT form, BFV in coefficient form)
 void hoisting(const Context &ctx, Ciphertext &ct, const GaloisKeys &gk, const std::vector<int> &steps,
               hipStream_t s);
 u32 galois_elt_from_step(int step, u64 N);

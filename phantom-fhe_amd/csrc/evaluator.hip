@@ -685,9 +685,13 @@ void rescale_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, 
 // the inner product, to the modup'ed digits; one moddown of the accumulated cx.
 void hoisting(const Context &ctx, Ciphertext &ct, const GaloisKeys &gk, const std::vector<int> &steps,
               hipStream_t s) {
-    require(ctx.params().scheme == kSchemeCKKS, "hoisting: only CKKS is on this path");
+    const bool bfv = ctx.params().scheme == kSchemeBFV;
+    require(bfv || ctx.params().scheme == kSchemeCKKS, "hoisting: unsupported scheme");
     require(ct.size == 2, "ciphertext size must be 2");
-    require(ct.is_ntt_form, "CKKS encrypted must be in NTT form");
+    if (bfv)
+        require(!ct.is_ntt_form, "BFV encrypted cannot be in NTT form");
+    else
+        require(ct.is_ntt_form, "CKKS encrypted must be in NTT form");
     require(!steps.empty(), "hoisting needs at least one step");
     const LevelConsts &L = ctx.level(ct.chain_index);
     const u32 Ql = L.size_Ql, QlP = L.size_QlP, beta = L.beta;
@@ -701,6 +705,10 @@ void hoisting(const Context &ctx, Ciphertext &ct, const GaloisKeys &gk, const st
         require(key.dnum >= beta, "key-switching key has too few digits for this level");
         items.emplace_back(ctx.galois_perm(elt), &key);
     }
+    // BFV (coefficient form): the reference permutes c0 in the coefficient domain and
+    // runs its BFV modup / moddown; both equal the NTT-domain form conjugated by the
+    // exact transform, so the ciphertext is transformed, hoisted and transformed back
+    if (bfv) ntt_forward(ctx.tables(), limb_map(ct.data, 0, Ql, LN), Ql, 2, s);
     u64 *t = static_cast<u64 *>(dev_alloc((size_t)beta * LPN * 8, s));
     u64 *cx = static_cast<u64 *>(dev_alloc(2 * LPN * 8, s));
     u64 *acc0 = static_cast<u64 *>(dev_alloc(LN * 8, s));
@@ -716,6 +724,7 @@ void hoisting(const Context &ctx, Ciphertext &ct, const GaloisKeys &gk, const st
     }
     // ct0 = acc0 + moddown(cx0), ct1 = moddown(cx1)
     moddown(ctx, ct.chain_index, cx, ct.data, acc0, nullptr, 2, s);
+    if (bfv) ntt_inverse(ctx.tables(), limb_map(ct.data, 0, Ql, LN), Ql, 2, s);
     dev_free(acc0, s);
     dev_free(cx, s);
     dev_free(t, s);

@@ -69,6 +69,7 @@ def lib():
         L.orc_bfv_multiply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p, _u64p, _u64p]
         L.orc_bfv_relinearize.argtypes = [ctypes.c_void_p, _u64p, _u64p, _sz, _u64p]
         L.orc_hoisting.argtypes = [ctypes.c_void_p, _sz, _u64p, _u32p, _sz, _u64p, _sz, _u64p]
+        L.orc_hoisting_bfv.argtypes = [ctypes.c_void_p, _sz, _u64p, _u32p, _sz, _u64p, _sz, _u64p]
         L.orc_hps_create.restype = ctypes.c_void_p
         L.orc_hps_create.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         L.orc_hps_create2.restype = ctypes.c_void_p
@@ -228,12 +229,13 @@ class Ctx:
         assert rc == 0
         return out
 
-    def hoisting(self, size_Ql, ct, elts, keys):
-        """sum of the rotations by elts; keys [n][dnum][2][QP][N]"""
+    def hoisting(self, size_Ql, ct, elts, keys, bfv=False):
+        """sum of the rotations by elts; keys [n][dnum][2][QP][N]; bfv: coefficient form"""
         out = np.zeros((2, size_Ql, self.N), np.uint64)
         e = np.array(elts, np.uint32)
         keys = np.ascontiguousarray(keys, np.uint64)
-        rc = lib().orc_hoisting(self._h, size_Ql, _p(np.ascontiguousarray(ct, np.uint64)), e.ctypes.data_as(_u32p),
+        fn = lib().orc_hoisting_bfv if bfv else lib().orc_hoisting
+        rc = fn(self._h, size_Ql, _p(np.ascontiguousarray(ct, np.uint64)), e.ctypes.data_as(_u32p),
                                 len(e), _p(keys), keys.shape[1], _p(out))
         assert rc == 0
         return out

@@ -313,3 +313,34 @@ def test_hoisting_bit_exact(N, bits, P, chain):
     assert np.array_equal(c.download(N), octx.hoisting(Ql, a, elts, keys))
     with pytest.raises(ValueError):
         ph.hoisting_inplace(ctx, c, gk, [5])
+
+
+@pytest.mark.parametrize("N,k", [(1 << 13, 4), (1 << 15, 16)])
+def test_hoisting_bfv_bit_exact(N, k):
+    """BFV hoisting_inplace on a coefficient-form ciphertext, bit-exact against
+    the oracle (coefficient-domain automorphism of c0, key switch of c1)."""
+    ob.set_threads(16)
+    say(f"case: bfv hoisting N={N} k={k}")
+    mods = ph.coeff_modulus_create(N, [54] * k + [60])
+    parms = ph.EncryptionParameters(ph.SCHEME_BFV)
+    parms.set_poly_modulus_degree(N)
+    parms.set_coeff_modulus(mods)
+    parms.set_special_modulus_size(1)
+    parms.set_plain_modulus(65537)
+    ctx = ph.context(parms)
+    octx = ob.Ctx(N.bit_length() - 1, mods, 1)
+    rng = np.random.default_rng(N + k)
+    steps = [1, -2, 0]
+    elts = [ph.galois_elt_from_step(s, N) for s in steps]
+    keys = np.stack([rand_key(rng, mods, N, ctx.dnum) for _ in elts])
+    gk = ph.galois_key(ctx, elts, host=keys)
+    a = rand_ct(rng, mods[:k], N)
+    c = ph.ciphertext(ctx, 1, 2)
+    assert not c.is_ntt_form()
+    c.upload(a)
+    ph.hoisting_inplace(ctx, c, gk, steps)
+    assert not c.is_ntt_form()
+    assert np.array_equal(c.download(N), octx.hoisting(k, a, elts, keys, bfv=True))
+    c.set_ntt_form(True)
+    with pytest.raises(ValueError):
+        ph.hoisting_inplace(ctx, c, gk, steps)

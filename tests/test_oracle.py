@@ -421,3 +421,49 @@ def test_hoisting_decrypts_sum_of_rotations():
     for k in range(TOY_N):
         v, Q = br.crt([dec[i, k] for i in range(Ql)], mods[:Ql])
         assert abs(_centered(v, Q) - want[k]) < (1 << 22), k
+
+
+def test_hoisting_bfv_decrypts_sum_of_rotations():
+    """BFV hoisting (coefficient-form ciphertext): decryption gives
+    sum_i sigma_{g_i}(m) up to key-switching noise, and the c0 part equals the
+    coefficient-domain automorphism sum (a key switch of c1 = 0 is zero)."""
+    ctx, mods = toy_ctx()
+    rng = np.random.default_rng(47)
+    s = _toy_secret(rng)
+    steps = (2, -1, 5)
+    elts = [ob.galois_elt_from_step(st, TOY_N) for st in steps]
+    keys = []
+    for i, g in enumerate(elts):
+        sg = [_centered(v, 1 << 200) for v in br.galois_coeff([int(v) % (1 << 200) for v in s], g, 1 << 200)]
+        key, s_ntt = _toy_keys(ctx, mods, s, lambda _, sg=sg: sg, 70 + i)
+        keys.append(key)
+    keys = np.stack(keys)
+    Ql = 4
+    m = rng.integers(-1000, 1000, TOY_N)
+    a = rand_limbs(rng, mods[:Ql], TOY_N)
+    a_ntt = ctx.ntt_forward(a, 0)
+    as_ = ctx.ntt_inverse(np.array([[int(a_ntt[i, k]) * int(s_ntt[i, k]) % q for k in range(TOY_N)]
+                                    for i, q in enumerate(mods[:Ql])], np.uint64), 0)
+    mr = _rns(m, mods[:Ql])
+    c0 = np.array([[(int(mr[i, k]) - int(as_[i, k])) % q for k in range(TOY_N)]
+                   for i, q in enumerate(mods[:Ql])], np.uint64)
+    out = ctx.hoisting(Ql, np.stack([c0, a]), elts, keys, bfv=True)
+    o1 = ctx.ntt_forward(out[1], 0)
+    o1s = ctx.ntt_inverse(np.array([[int(o1[i, k]) * int(s_ntt[i, k]) % q for k in range(TOY_N)]
+                                    for i, q in enumerate(mods[:Ql])], np.uint64), 0)
+    want = [0] * TOY_N
+    for g in elts:
+        rot = br.galois_coeff([int(v) % (1 << 200) for v in m], g, 1 << 200)
+        want = [w + _centered(r, 1 << 200) for w, r in zip(want, rot)]
+    for k in range(TOY_N):
+        v, Q = br.crt([(int(out[0, i, k]) + int(o1s[i, k])) % mods[i] for i in range(Ql)], mods[:Ql])
+        assert abs(_centered(v, Q) - want[k]) < (1 << 22), k
+    # with c1 = 0 the output is (sum_i sigma_i(c0), 0)
+    z = ctx.hoisting(Ql, np.stack([c0, np.zeros_like(a)]), elts, keys, bfv=True)
+    assert not z[1].any()
+    for i, q in enumerate(mods[:Ql]):
+        acc = [0] * TOY_N
+        for g in elts:
+            rot = br.galois_coeff([int(v) for v in c0[i]], g, q)
+            acc = [(x + y) % q for x, y in zip(acc, rot)]
+        assert [int(v) for v in z[0, i]] == acc, i
