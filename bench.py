@@ -362,6 +362,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ntt-only", action="store_true",
                     help="only the forward NTT over 47 x 16 limbs and the copy kernel (for counter runs)")
+    ap.add_argument("--hmult-only", action="store_true",
+                    help="only the plain and batched HMult+relin (for kernel-trace runs)")
     ap.add_argument("--serve-qps", type=float, default=400.0)
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--no-autotune", action="store_true",
@@ -404,6 +406,10 @@ def main():
     value = ops / (ms / 1e3)
     log(f"bench: batched {ops} ops in {ms:.2f} ms (graph replays) -> {value:.1f} op/s "
         f"({value / plain:.2f}x plain; scratch {scratch / 2**30:.2f} GiB)")
+    if args.hmult_only:
+        print(json.dumps({"plain_op_per_s": round(plain, 1), "batched_op_per_s": round(value, 1),
+                          "ops": ops, "batch": args.batch, "steps": args.steps, "warmup": args.warmup}), flush=True)
+        return
 
     # dominant kernel family: the forward NTT (two passes) over 47 x P limbs
     L = ctx.size_Q + ctx.size_P
