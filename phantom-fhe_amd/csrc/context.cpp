@@ -212,10 +212,11 @@ Context::Context(const Params &p) : params_(p) {
     for (size_t r = 0; r < R; ++r) {
         const auto &t = host_tables_[r];
         for (u64 k = 0; k < N_; ++k) {
-            htw[r * N_ + k] = {t.fw[k], t.fws[k]};
-            hitw[r * N_ + k] = {t.iw[k], t.iws[k]};
-            htwd[r * N_ + k] = (double)t.fw[k];
-            hitwd[r * N_ + k] = (double)t.iw[k];
+            const u64 o = r * N_ + tw_slot(k, N_);   // device storage order (ntt_kernels.h)
+            htw[o] = {t.fw[k], t.fws[k]};
+            hitw[o] = {t.iw[k], t.iws[k]};
+            htwd[o] = (double)t.fw[k];
+            hitwd[o] = (double)t.iw[k];
         }
         u64 lo, hi;
         hm::barrett_ratio(t.q, lo, hi);

@@ -60,7 +60,7 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
         for (int e = 0; e < (1 << NA); ++e)
             v[b * (1 << NA) + e] = A::from_canon(ld(data, baseA[b] + ((u32)e << (8 + NB)), limb, z, md));
 
-    fwd_round<NA>(v, baseA, 8 + NB, logN, a);
+    fwd_round<NA, 8 + NB>(v, baseA, logN, a);
 
     if constexpr (NB > 0) {
 #pragma unroll
@@ -83,7 +83,7 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
 #pragma unroll
             for (int e = 0; e < (1 << NB); ++e)
                 v[b * (1 << NB) + e] = A::from_bits(lds[col_addr<TC>((hpB[b] << NB) + e, uB[b])]);
-        fwd_round<NB>(v, baseB, 8, logN, a);
+        fwd_round<NB, 8>(v, baseB, logN, a);
 #pragma unroll
         for (int b = 0; b < BB; ++b)
 #pragma unroll
@@ -109,7 +109,7 @@ __device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *d
     u32 baseA[1] = {(r0 + u) * 256 + l};
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = A::from_bits(data[baseA[0] + ((u32)e << 4)]);
-    fwd_round<4>(v, baseA, 4, logN, a);
+    fwd_round<4, 4>(v, baseA, logN, a);
 #pragma unroll
     for (int e = 0; e < 16; ++e) lds[row_addr(u, ((u32)e << 4) + l)] = A::to_bits(v[e]);
     __syncthreads();
@@ -117,7 +117,7 @@ __device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *d
     u32 baseB[1] = {(r0 + u) * 256 + (hp << 4)};
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = A::from_bits(lds[row_addr(u, (hp << 4) + e)]);
-    fwd_round<4>(v, baseB, 0, logN, a);
+    fwd_round<4, 0>(v, baseB, logN, a);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; ++e) lds[row_addr(u, (hp << 4) + e)] = a.fwd_canon(v[e]);
@@ -155,7 +155,7 @@ __device__ __forceinline__ void inv_rows_body(const A &a, u64 *lds, u64 *data, c
     u32 baseB[1] = {(r0 + u) * 256 + (l << 4)};
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = A::from_canon(lds[row_addr(u, (l << 4) + e)]);
-    inv_round<4>(v, baseB, 0, logN, a);
+    inv_round<4, 0>(v, baseB, logN, a);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; ++e) lds[row_addr(u, (l << 4) + e)] = A::to_bits(v[e]);
@@ -163,7 +163,7 @@ __device__ __forceinline__ void inv_rows_body(const A &a, u64 *lds, u64 *data, c
     u32 baseA[1] = {(r0 + u) * 256 + l};
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = A::from_bits(lds[row_addr(u, ((u32)e << 4) + l)]);
-    inv_round<4>(v, baseA, 4, logN, a);
+    inv_round<4, 4>(v, baseA, logN, a);
 #pragma unroll
     for (int e = 0; e < 16; ++e) data[baseA[0] + ((u32)e << 4)] = A::to_bits(v[e]);
 }
@@ -204,7 +204,7 @@ __device__ __forceinline__ void inv_cols_body(const A &a, u64 *lds, u64 *data, u
 #pragma unroll
             for (int e = 0; e < (1 << NB); ++e)
                 v[b * (1 << NB) + e] = A::from_bits(data[baseB[b] + ((u32)e << 8)]);
-        inv_round<NB>(v, baseB, 8, logN, a);
+        inv_round<NB, 8>(v, baseB, logN, a);
 #pragma unroll
         for (int b = 0; b < BB; ++b)
 #pragma unroll

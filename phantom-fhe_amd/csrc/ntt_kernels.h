@@ -14,7 +14,11 @@
 //
 // Twiddles: forward table tw[row][k] = (psi^brv(k), Shoup), inverse
 // itw[row][k] = (psi^-brv(k), Shoup); stage with m groups and group g uses
-// entry m + g, as in the reference tables (src/host/ntt.cu:11-56).
+// entry m + g, as in the reference tables (src/host/ntt.cu:11-56), except
+// that entries [N/16, N) are stored transposed (tw_slot): only the last
+// register round of the row passes (distances 8 .. 1) reads them, a thread
+// there needs 8 >> jj consecutive entries per stage, and transposed the 16
+// threads of a row read 16 consecutive entries per load, as for the data.
 #pragma once
 
 #include "common.h"
@@ -79,45 +83,70 @@ struct NttTables {
     int chunk_limbs;
 };
 
+// Storage slot of table entry k (N >= 4096): entries below N/16 in place; the
+// block [m, 2m) of the stage at distance 2^jj (m = N >> (jj+1), jj < 4) holds
+// per row r of 256 coefficients a 16 x c matrix (c = 8 >> jj; entry
+// m + r*16c + h*c + i for thread column h and i < c) stored as c x 16.
+__host__ __device__ inline u64 tw_slot(u64 k, u64 N) {
+    if (k < N / 16) return k;
+    u64 m = N / 2;
+    int jj = 0;
+    while (k < m) {
+        m >>= 1;
+        ++jj;
+    }
+    const u64 c = 8 >> jj, off = k - m, r = off / (16 * c), w = off % (16 * c);
+    return m + r * 16 * c + (w % c) * 16 + w / c;
+}
+
 // ---- register rounds ------------------------------------------------------
 // v holds (16 >> NB) blocks of 2^NB values; value e of block b has global
-// index base[b] + (e << p). A round applies the NB stages whose pair distance
-// is 2^(p+NB-1) ... 2^p. Stage with distance t = 2^(p+jj) has
-// m = N / (2t) groups; the group of index i is i >> (p+jj+1), and its twiddle
-// is entry m + group of the bit-reversed table.
-template <int NB, class A>
-__device__ __forceinline__ void fwd_round(typename A::T (&v)[16], const u32 (&base)[16 >> NB], int p,
-                                          u32 logN, const A &a) {
+// index base[b] + (e << P). A round applies the NB stages whose pair distance
+// is 2^(P+NB-1) ... 2^P. Stage with distance t = 2^(P+jj) has
+// m = N / (2t) groups; the group of index i is i >> (P+jj+1), and its twiddle
+// is entry m + group of the bit-reversed table. P = 0 is the last round of a
+// row pass (base = row * 256 + 16 h): its entries are read from tw_slot's
+// transposed storage.
+template <int P, int NB>
+__device__ __forceinline__ u32 tw_index(u32 m, u32 base, int jj, int e) {
+    const u32 k = (u32)e >> (jj + 1);
+    if constexpr (P == 0) {
+        static_assert(NB == 4, "the transposed twiddle layout serves radix-16 rounds");
+        return m + ((base >> 8) << (7 - jj)) + (k << 4) + ((base >> 4) & 15);
+    } else {
+        return m + (base >> (P + jj + 1)) + k;
+    }
+}
+
+template <int NB, int P, class A>
+__device__ __forceinline__ void fwd_round(typename A::T (&v)[16], const u32 (&base)[16 >> NB], u32 logN,
+                                          const A &a) {
 #pragma unroll
     for (int jj = NB - 1; jj >= 0; --jj) {
-        const int sh = p + jj + 1;
-        const u32 m = 1u << (logN - sh);
+        const u32 m = 1u << (logN - (P + jj + 1));
 #pragma unroll
         for (int b = 0; b < (16 >> NB); ++b) {
-            const u32 g0 = m + (base[b] >> sh);
 #pragma unroll
             for (int e = 0; e < (1 << NB); ++e) {
                 if (e & (1 << jj)) continue;
-                a.ct(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], g0 + (e >> (jj + 1)));
+                a.ct(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], tw_index<P, NB>(m, base[b], jj, e));
             }
         }
     }
 }
 
-template <int NB, class A>
-__device__ __forceinline__ void inv_round(typename A::T (&v)[16], const u32 (&base)[16 >> NB], int p,
-                                          u32 logN, const A &a) {
+template <int NB, int P, class A>
+__device__ __forceinline__ void inv_round(typename A::T (&v)[16], const u32 (&base)[16 >> NB], u32 logN,
+                                          const A &a) {
 #pragma unroll
     for (int jj = 0; jj < NB; ++jj) {
-        const int sh = p + jj + 1;
-        const u32 m = 1u << (logN - sh);
+        const u32 m = 1u << (logN - (P + jj + 1));
 #pragma unroll
         for (int b = 0; b < (16 >> NB); ++b) {
-            const u32 g0 = m + (base[b] >> sh);
 #pragma unroll
             for (int e = 0; e < (1 << NB); ++e) {
                 if (e & (1 << jj)) continue;
-                a.gs(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], g0 + (e >> (jj + 1)));
+                a.gs(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], tw_index<P, NB>(m, base[b], jj, e));
             }
         }
     }
