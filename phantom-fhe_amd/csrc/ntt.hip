@@ -42,6 +42,47 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
     using T = typename A::T;
     constexpr int R = 1 << SR;
     constexpr int TC = kNttTile / R;
+    if constexpr (SR > 8) {
+        // N = 2^17: three rounds of 4 + 4 + 1 stages over 512 rows x 8 columns.
+        // Row bits (high to low): 4 for round A, 4 for round B, 1 for round C.
+        static_assert(SR == 9, "the three-round column pass covers 512 rows");
+        T v[16];
+        const u32 uc = t % TC, rest = t / TC;   // rest < 32
+        // round A: rows rest + 32 e, distances N/2 .. 2^13
+        u32 baseA[1] = {rest * 256 + c0 + uc};
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = A::from_canon(ld(data, baseA[0] + ((u32)e << 13), limb, z, md));
+        fwd_round<4, 13>(v, baseA, logN, a);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) lds[col_addr<TC>(((u32)e << 5) + rest, uc)] = A::to_bits(v[e]);
+        __syncthreads();
+        // round B: rows 32 hi + lo + 2 e (hi = rest >> 1, lo = rest & 1), distances 2^12 .. 2^9
+        const u32 rb = (rest >> 1) * 32 + (rest & 1);
+        u32 baseB[1] = {rb * 256 + c0 + uc};
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = A::from_bits(lds[col_addr<TC>(rb + ((u32)e << 1), uc)]);
+        fwd_round<4, 9>(v, baseB, logN, a);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 16; ++e) lds[col_addr<TC>(rb + ((u32)e << 1), uc)] = A::to_bits(v[e]);
+        __syncthreads();
+        // round C: row pairs (2p, 2p + 1) with p = 32 b + rest, distance 2^8
+        u32 baseC[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const u32 r2 = 2 * ((u32)b * 32 + rest);
+            baseC[b] = r2 * 256 + c0 + uc;
+            v[2 * b] = A::from_bits(lds[col_addr<TC>(r2, uc)]);
+            v[2 * b + 1] = A::from_bits(lds[col_addr<TC>(r2 + 1, uc)]);
+        }
+        fwd_round<1, 8>(v, baseC, logN, a);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            data[baseC[b]] = A::to_bits(v[2 * b]);
+            data[baseC[b] + 256] = A::to_bits(v[2 * b + 1]);
+        }
+        return;
+    } else {
     constexpr int NA = SR < 4 ? SR : 4;
     constexpr int NB = SR - NA;
     constexpr int BA = 16 >> NA;
@@ -94,6 +135,7 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
         for (int b = 0; b < BA; ++b)
 #pragma unroll
             for (int e = 0; e < (1 << NA); ++e) data[baseA[b] + ((u32)e << 8)] = A::to_bits(v[b * (1 << NA) + e]);
+    }
     }
 }
 
@@ -177,6 +219,45 @@ __device__ __forceinline__ void inv_cols_body(const A &a, u64 *lds, u64 *data, u
     using T = typename A::T;
     constexpr int R = 1 << SR;
     constexpr int TC = kNttTile / R;
+    if constexpr (SR > 8) {
+        // N = 2^17: the forward column pass's rounds in reverse (C, B, then A with
+        // the top stage scaled)
+        static_assert(SR == 9, "the three-round column pass covers 512 rows");
+        T v[16];
+        const u32 uc = t % TC, rest = t / TC;
+        u32 baseC[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const u32 r2 = 2 * ((u32)b * 32 + rest);
+            baseC[b] = r2 * 256 + c0 + uc;
+            v[2 * b] = A::from_bits(data[baseC[b]]);
+            v[2 * b + 1] = A::from_bits(data[baseC[b] + 256]);
+        }
+        inv_round<1, 8>(v, baseC, logN, a);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const u32 r2 = 2 * ((u32)b * 32 + rest);
+            lds[col_addr<TC>(r2, uc)] = A::to_bits(v[2 * b]);
+            lds[col_addr<TC>(r2 + 1, uc)] = A::to_bits(v[2 * b + 1]);
+        }
+        __syncthreads();
+        const u32 rb = (rest >> 1) * 32 + (rest & 1);
+        u32 baseB[1] = {rb * 256 + c0 + uc};
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = A::from_bits(lds[col_addr<TC>(rb + ((u32)e << 1), uc)]);
+        inv_round<4, 9>(v, baseB, logN, a);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 16; ++e) lds[col_addr<TC>(rb + ((u32)e << 1), uc)] = A::to_bits(v[e]);
+        __syncthreads();
+        u32 baseA[1] = {rest * 256 + c0 + uc};
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = A::from_bits(lds[col_addr<TC>(((u32)e << 5) + rest, uc)]);
+        inv_round_last<4>(v, baseA, 13, logN, a, s0, s0s, s1, s1s);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) st(data, baseA[0] + ((u32)e << 13), a.inv_canon(v[e]), limb, z, md);
+        return;
+    } else {
     constexpr int NA = SR < 4 ? SR : 4;
     constexpr int NB = SR - NA;
     constexpr int BA = 16 >> NA;
@@ -229,6 +310,7 @@ __device__ __forceinline__ void inv_cols_body(const A &a, u64 *lds, u64 *data, u
 #pragma unroll
         for (int e = 0; e < (1 << NA); ++e)
             st(data, baseA[b] + ((u32)e << (8 + NB)), a.inv_canon(v[b * (1 << NA) + e]), limb, z, md);
+    }
 }
 
 }  // namespace
@@ -371,12 +453,18 @@ __global__ __launch_bounds__(kNttThreads) void k_fwd_rows_ip(NttTables T, RowsIp
 // served from L2 / the memory-attached last-level cache instead of HBM.
 // Every group runs the same number of tiles, so the bodies' barriers match.
 // ---------------------------------------------------------------------------
-constexpr u32 kNttSlot = 16 * 272;   // LDS words per group: >= R * (TC + 1) for R <= 256
+// LDS words per group: the row tile (16 x 272) or the column tile (R x (TC + 1))
+template <int SR>
+constexpr u32 ntt_slot() {
+    constexpr u32 cols = (1u << SR) * ((u32)kNttTile / (1u << SR) + 1);
+    return cols > 16 * 272 ? cols : 16 * 272;
+}
 
 template <int SR, int G, class Load, class Store>
 __global__ __launch_bounds__(kNttThreads *G) void k_fwd_fused(NttTables T, LimbMap M, Load ld, Store st) {
     constexpr int TC = kNttTile >> SR;
-    __shared__ u64 lds_all[G * kNttSlot];
+    constexpr u32 kSlot = ntt_slot<SR>();
+    __shared__ u64 lds_all[G * kSlot];
     const u32 z = blockIdx.y + M.poly0;
     const u32 limb = M.limb(blockIdx.z, z);
     if (limb >= M.nlimbs) return;
@@ -384,7 +472,7 @@ __global__ __launch_bounds__(kNttThreads *G) void k_fwd_fused(NttTables T, LimbM
     u64 *data = M.data + z * M.poly_stride + (u64)limb * T.N;
     const DevModulus md = T.mod[row];
     const u32 g = threadIdx.x / kNttThreads, t = threadIdx.x % kNttThreads;
-    u64 *lds = lds_all + g * kNttSlot;
+    u64 *lds = lds_all + g * kSlot;
     const u32 tiles = T.N / kNttTile;
     auto run = [&](const auto &a) {
         for (u32 tile = g; tile < tiles; tile += G) {
@@ -405,7 +493,8 @@ __global__ __launch_bounds__(kNttThreads *G) void k_fwd_fused(NttTables T, LimbM
 template <int SR, int G, class Load, class Store>
 __global__ __launch_bounds__(kNttThreads *G) void k_inv_fused(NttTables T, LimbMap M, Load ld, Store st) {
     constexpr int TC = kNttTile >> SR;
-    __shared__ u64 lds_all[G * kNttSlot];
+    constexpr u32 kSlot = ntt_slot<SR>();
+    __shared__ u64 lds_all[G * kSlot];
     const u32 z = blockIdx.y + M.poly0;
     const u32 limb = M.limb(blockIdx.z, z);
     if (limb >= M.nlimbs) return;
@@ -416,7 +505,7 @@ __global__ __launch_bounds__(kNttThreads *G) void k_inv_fused(NttTables T, LimbM
     u64 s0, s0s, s1, s1s;
     limb_scale(T, M, limb, row, s0, s0s, s1, s1s);
     const u32 g = threadIdx.x / kNttThreads, t = threadIdx.x % kNttThreads;
-    u64 *lds = lds_all + g * kNttSlot;
+    u64 *lds = lds_all + g * kSlot;
     const u32 tiles = T.N / kNttTile;
     auto run = [&](const auto &a) {
         for (u32 tile = g; tile < tiles; tile += G) {
@@ -486,7 +575,8 @@ void dispatch_sr(u32 logN, F &&f) {
         case 14: f(std::integral_constant<int, 6>{}); break;
         case 15: f(std::integral_constant<int, 7>{}); break;
         case 16: f(std::integral_constant<int, 8>{}); break;
-        default: fail(kInvalidArgument, "unsupported polynomial degree for the NTT (2^12 .. 2^16)");
+        case 17: f(std::integral_constant<int, 9>{}); break;
+        default: fail(kInvalidArgument, "unsupported polynomial degree for the NTT (2^12 .. 2^17)");
     }
 }
 

@@ -344,3 +344,29 @@ def test_hoisting_bfv_bit_exact(N, k):
     c.set_ntt_form(True)
     with pytest.raises(ValueError):
         ph.hoisting_inplace(ctx, c, gk, steps)
+
+
+def test_max_degree_multiply_relin_rotate_rescale():
+    """N = 2^17 end to end (multiply_and_relin, rotate, rescale) against the oracle"""
+    ob.set_threads(16)
+    N = 1 << 17
+    say("case: N=2^17 mulrelin/rotate/rescale")
+    ctx, octx, mods = make_ctx(N, [60, 50, 50, 50, 60, 60], 2)
+    rng = np.random.default_rng(17)
+    key = rand_key(rng, mods, N, ctx.dnum)
+    rk = ph.relin_key(ctx, host=key)
+    Ql = ctx.size_Q
+    a, b = rand_ct(rng, mods[:Ql], N), rand_ct(rng, mods[:Ql], N)
+    c1, c2 = upload_ct(ctx, a, 1), upload_ct(ctx, b, 1)
+    ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
+    want = octx.multiply_and_relin(Ql, a, b, key)
+    assert np.array_equal(c1.download(N), want)
+    elt = ph.galois_elt_from_step(3, N)
+    gkey = rand_key(rng, mods, N, ctx.dnum)
+    gk = ph.galois_key(ctx, [elt], host=gkey[None])
+    ph.rotate_inplace(ctx, c1, 3, gk)
+    rot = octx.apply_galois(Ql, want, elt, gkey)
+    assert np.array_equal(c1.download(N), rot)
+    c1.set_scale(2.0 ** 40)
+    r = ph.rescale_to_next(ctx, c1)
+    assert np.array_equal(r.download(N), octx.rescale(Ql, rot))

@@ -32,7 +32,7 @@ def test_config1_ntt_then_intt_bit_exact():
     assert np.array_equal(inv, a)
 
 
-@pytest.mark.parametrize("logN", [12, 13, 14, 15, 16])
+@pytest.mark.parametrize("logN", [12, 13, 14, 15, 16, 17])
 def test_ntt_parity_all_degrees(logN):
     N = 1 << logN
     say(f"case: ntt parity N=2^{logN}")
@@ -75,7 +75,7 @@ def fused():
 
 
 @pytest.mark.parametrize("G", [1, 2, 4])
-@pytest.mark.parametrize("logN", [13, 15, 16])
+@pytest.mark.parametrize("logN", [13, 15, 16, 17])
 def test_ntt_fused_parity(fused, G, logN):
     """One workgroup per limb runs both passes: same outputs as the oracle."""
     N = 1 << logN
@@ -90,3 +90,17 @@ def test_ntt_fused_parity(fused, G, logN):
     got_i = run_ntt(ctx, b, 2, inverse=True)
     assert np.array_equal(got_i, octx.ntt_inverse(b, 2))
     assert np.array_equal(run_ntt(ctx, got_i, 2), b)
+
+
+def test_ntt_max_degree_edge_values():
+    """N = 2^17 (the reference's largest ring degree, include/host/defines.h:23):
+    edge inputs and a round trip on 60- and 50-bit primes"""
+    say("case: ntt edge values N=2^17")
+    N = 1 << 17
+    ctx, octx, mods = make_ctx(N, [60, 50, 60], 1)
+    qs = np.array(mods, np.uint64)[:, None]
+    L = len(mods)
+    for a in (np.zeros((L, N), np.uint64), np.broadcast_to(qs - 1, (L, N)).copy(), np.full((L, N), 2, np.uint64)):
+        f = run_ntt(ctx, a, 0)
+        assert np.array_equal(f, octx.ntt_forward(a, 0))
+        assert np.array_equal(run_ntt(ctx, f, 0, inverse=True), a)
