@@ -303,14 +303,16 @@ int phx_tune_get(const char *key, int *value);
  *   bconv_mfma          1 = FastBConv on the matrix cores, 0 = VALU kernel */
 typedef struct {
     int choice[4];       /* the variants applied */
-    double default_us;   /* HMult+relin at the top level with every switch at -1 */
+    double default_us;   /* objective with every switch at -1: one HMult+relin at the top level plus
+                            one op of an 8-pair batch plan, microseconds */
     double best_us;      /* the same with `choice` */
     uint32_t trials;     /* candidates timed */
 } phx_tune_report;
 int phx_context_get_tuning(const phx_context *ctx, int choice[4]);
 int phx_context_set_tuning(phx_context *ctx, const int choice[4]);
-/* Autotuner: times HMult+relin at the top level (synthetic residues in HBM) for
- * each candidate, applies the fastest to ctx and, when cache_path is non-empty,
+/* Autotuner: times HMult+relin at the top level, plain and batched (synthetic
+ * residues in HBM), for each candidate, applies the fastest (a candidate must beat
+ * the incumbent by 2%) to ctx and, when cache_path is non-empty,
  * stores it in that JSON cache under (scheme, N, size_Q, size_P). Contexts created
  * while the environment variable PHX_TUNE_CACHE names such a file apply its entry. */
 int phx_autotune(phx_context *ctx, const char *cache_path, phx_tune_report *report, phx_stream stream);

@@ -11,14 +11,15 @@ namespace phx {
 
 struct TuneResult {
     TuneChoice choice;
-    double default_us = 0;   // HMult+relin with every switch at its process-wide value
-    double best_us = 0;      // with `choice`
+    double default_us = 0;   // objective (HMult+relin + one op of an 8-pair batch) at the process-wide switches
+    double best_us = 0;      // the same with `choice`
     std::vector<std::pair<std::string, double>> trials;
 };
 
 // Times HMult+relin at the top level (CKKS: tensor + relinearise; BFV: the
-// context's multiply technique + relinearise) on synthetic residues resident
-// in HBM for each candidate, applies the fastest to ctx and returns it.
+// context's multiply technique + relinearise), plain and as an 8-pair batch
+// plan, on synthetic residues resident in HBM for each candidate, applies the
+// fastest (by at least 2%) to ctx and returns it.
 TuneResult autotune(Context &ctx, hipStream_t s);
 
 // cache key of a context shape: "scheme=..,N=..,Q=..,P=.."
