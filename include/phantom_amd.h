@@ -121,6 +121,17 @@ int phx_galois_key_destroy(phx_galois_key *key);
 /* PhantomSecretKey::encrypt_symmetric (src/secretkey.cu:463-531); plain = [Ql][N] coefficient residues */
 int phx_encrypt_symmetric(const phx_context *ctx, const phx_secret_key *sk, const uint64_t *plain_host,
                           size_t chain_index, uint64_t seed, phx_ciphertext *out, phx_stream stream);
+/* encrypt_symmetric with c1 drawn from the seeded uniform sampler (sample_uniform_poly,
+ * src/prng.cu:174-210) for the 64-byte seed_a, as the reference does
+ * (src/secretkey.cu:232-300, 463-500); the ciphertext keeps the seed so that
+ * phx_ciphertext_save_symmetric can store it instead of c1. e comes from seed_e. */
+int phx_encrypt_symmetric_seeded(const phx_context *ctx, const phx_secret_key *sk, const uint64_t *plain_host,
+                                 size_t chain_index, const uint8_t *seed_a, uint64_t seed_e, phx_ciphertext *out,
+                                 phx_stream stream);
+/* sample_uniform_poly_wrap (src/prng.cu:206-210): out (device, [coeff_mod_size][N]) <- the
+ * seeded uniform sampler's output for the 64-byte seed (host) */
+int phx_sample_uniform_poly(const phx_context *ctx, const uint8_t *seed, size_t coeff_mod_size, uint64_t *out,
+                            phx_stream stream);
 /* PhantomSecretKey::ckks_decrypt (src/secretkey.cu:533-569) followed by the INTT: out = [Ql][N] */
 int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext *ct, uint64_t *out_host,
                 phx_stream stream);
@@ -136,6 +147,12 @@ int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext
 int phx_ciphertext_save_size(const phx_ciphertext *ct, size_t *bytes);
 int phx_ciphertext_save(const phx_ciphertext *ct, void *buf, size_t bytes);
 int phx_ciphertext_load(const phx_context *ctx, phx_ciphertext *ct, const void *buf, size_t bytes);
+/* PhantomCiphertext::save_symmetric / load_symmetric (include/ciphertext.h:216-305): header, c0,
+ * then the 64-byte seed instead of c1 (regenerated on load). Saving requires a ciphertext from
+ * phx_encrypt_symmetric_seeded (or a symmetric load) whose c1 is still the seed's output. */
+int phx_ciphertext_save_symmetric_size(const phx_ciphertext *ct, size_t *bytes);
+int phx_ciphertext_save_symmetric(const phx_context *ctx, const phx_ciphertext *ct, void *buf, size_t bytes);
+int phx_ciphertext_load_symmetric(const phx_context *ctx, phx_ciphertext *ct, const void *buf, size_t bytes);
 int phx_relin_key_save_size(const phx_context *ctx, const phx_relin_key *key, size_t *bytes);
 int phx_relin_key_save(const phx_context *ctx, const phx_relin_key *key, void *buf, size_t bytes);
 int phx_relin_key_load(const phx_context *ctx, const void *buf, size_t bytes, phx_relin_key **out);

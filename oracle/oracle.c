@@ -1361,3 +1361,82 @@ int orc_hoisting_bfv(const orc_ctx *c, size_t size_Ql, const u64 *ct, const uint
     free(x); free(acc0);
     return rc;
 }
+
+/* ---- seeded uniform sampler (src/prng.cu:17-210) ----
+ * Block: the Salsa20 hash (Bernstein's specification: quarterround, rowround,
+ * columnround, doubleround x 10, then the input added word-wise) of the 16
+ * words seed[0..7], nonce low, nonce high, seed[8..13] (little-endian words of
+ * the 64-byte seed). Sampler: block b (< L N / 8) yields coefficients 8b..8b+7
+ * of limb b / (N/8); a word above (2^64-1) - ((2^64-1) mod q) - 1 is replaced,
+ * from that position on, by a fresh block for nonce b + r L N (r counting the
+ * blocks drawn for b); accepted words are reduced mod q. */
+static uint32_t salsa_rotl32(uint32_t v, int c) { return (v << c) | (v >> (32 - c)); }
+
+void orc_salsa_quarterround(uint32_t y[4]) {
+    y[1] ^= salsa_rotl32(y[0] + y[3], 7);
+    y[2] ^= salsa_rotl32(y[1] + y[0], 9);
+    y[3] ^= salsa_rotl32(y[2] + y[1], 13);
+    y[0] ^= salsa_rotl32(y[3] + y[2], 18);
+}
+
+static void salsa_qr_at(uint32_t *x, int a, int b, int c, int d) {
+    uint32_t y[4] = {x[a], x[b], x[c], x[d]};
+    orc_salsa_quarterround(y);
+    x[a] = y[0]; x[b] = y[1]; x[c] = y[2]; x[d] = y[3];
+}
+
+void orc_salsa_hash(const uint8_t in[64], uint8_t out[64]) {
+    uint32_t w[16], x[16];
+    for (int i = 0; i < 16; i++)
+        w[i] = (uint32_t)in[4 * i] | ((uint32_t)in[4 * i + 1] << 8) | ((uint32_t)in[4 * i + 2] << 16) |
+               ((uint32_t)in[4 * i + 3] << 24);
+    memcpy(x, w, sizeof(x));
+    for (int r = 0; r < 10; r++) {
+        /* columnround: columns (0,4,8,12), (5,9,13,1), (10,14,2,6), (15,3,7,11) */
+        salsa_qr_at(x, 0, 4, 8, 12);
+        salsa_qr_at(x, 5, 9, 13, 1);
+        salsa_qr_at(x, 10, 14, 2, 6);
+        salsa_qr_at(x, 15, 3, 7, 11);
+        /* rowround: rows (0,1,2,3), (5,6,7,4), (10,11,8,9), (15,12,13,14) */
+        salsa_qr_at(x, 0, 1, 2, 3);
+        salsa_qr_at(x, 5, 6, 7, 4);
+        salsa_qr_at(x, 10, 11, 8, 9);
+        salsa_qr_at(x, 15, 12, 13, 14);
+    }
+    for (int i = 0; i < 16; i++) {
+        uint32_t v = x[i] + w[i];
+        out[4 * i] = (uint8_t)v; out[4 * i + 1] = (uint8_t)(v >> 8);
+        out[4 * i + 2] = (uint8_t)(v >> 16); out[4 * i + 3] = (uint8_t)(v >> 24);
+    }
+}
+
+static void seeded_block64(const uint8_t *seed, u64 nonce, u64 out[8]) {
+    uint8_t in[64], h[64];
+    memcpy(in, seed, 32);
+    for (int i = 0; i < 8; i++) in[32 + i] = (uint8_t)(nonce >> (8 * i));
+    memcpy(in + 40, seed + 32, 24);
+    orc_salsa_hash(in, h);
+    for (int k = 0; k < 8; k++) {
+        u64 v = 0;
+        for (int i = 7; i >= 0; i--) v = (v << 8) | h[8 * k + i];
+        out[k] = v;
+    }
+}
+
+void orc_sample_uniform_seeded(const orc_ctx *c, const uint8_t seed[64], size_t L, uint64_t *out) {
+    size_t N = c->N, per = N / 8;
+    for (size_t b = 0; b < per * L; b++) {
+        u64 q = c->q[b / per];
+        u64 max_accept = ~0ull - (~0ull % q) - 1;
+        u64 r[8];
+        seeded_block64(seed, b, r);
+        u64 draws = 1;
+        for (int i = 0; i < 8; i++) {
+            while (r[i] > max_accept) {
+                seeded_block64(seed, b + draws * (u64)N * L, r);
+                draws++;
+            }
+            out[b * 8 + i] = r[i] % q;
+        }
+    }
+}

@@ -117,6 +117,12 @@ int orc_bfv_multiply_hps2(const orc_ctx *c, const orc_hps *h, const uint64_t *ct
 int orc_bfv_multiply_hps(const orc_ctx *c, const orc_hps *h, const uint64_t *ct1, const uint64_t *ct2,
                          uint64_t *out3);
 
+/* the reference's seeded uniform sampler (sample_uniform_poly, src/prng.cu:174-210)
+ * and its block function (the Salsa20 hash, src/prng.cu:17-131) */
+void orc_salsa_quarterround(uint32_t y[4]);
+void orc_salsa_hash(const uint8_t in[64], uint8_t out[64]);
+void orc_sample_uniform_seeded(const orc_ctx *c, const uint8_t seed[64], size_t L, uint64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

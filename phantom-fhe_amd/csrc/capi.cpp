@@ -7,6 +7,7 @@
 #include "batch.h"
 #include "evaluator.h"
 #include "autotune.h"
+#include "prng.h"
 #include "tune.h"
 #include "ckks.h"
 #include "serve.h"
@@ -407,6 +408,26 @@ int phx_encrypt_symmetric(const phx_context *ctx, const phx_secret_key *sk, cons
     });
 }
 
+int phx_encrypt_symmetric_seeded(const phx_context *ctx, const phx_secret_key *sk, const uint64_t *plain,
+                                 size_t chain, const uint8_t *seed_a, uint64_t seed_e, phx_ciphertext *out,
+                                 phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(sk, "secret key");
+        nn(plain, "plain");
+        nn(seed_a, "seed");
+        nn(out, "out");
+        const Context &c = *ctx->ctx;
+        require(chain >= 1 && chain <= c.size_Q(), "invalid chain index");
+        const size_t n = (size_t)c.limbs_at((u32)chain) * c.N();
+        u64 *d = static_cast<u64 *>(dev_alloc(n * 8, S(s)));
+        PHX_HIP_CHECK(hipMemcpyAsync(d, plain, n * 8, hipMemcpyHostToDevice, S(s)));
+        encrypt_symmetric_seeded(c, sk->sk, d, (u32)chain, out->ct, seed_a, seed_e, S(s));
+        dev_free(d, S(s));
+        PHX_HIP_CHECK(hipStreamSynchronize(S(s)));
+    });
+}
+
 int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext *ct, uint64_t *out, phx_stream s) {
     return guard([&] {
         nn(ctx, "context");
@@ -445,6 +466,42 @@ int phx_ciphertext_load(const phx_context *ctx, phx_ciphertext *ct, const void *
         nn(ct, "ciphertext");
         nn(buf, "buffer");
         ciphertext_load(*ctx->ctx, ct->ct, static_cast<const uint8_t *>(buf), bytes);
+    });
+}
+
+int phx_ciphertext_save_symmetric_size(const phx_ciphertext *ct, size_t *bytes) {
+    return guard([&] {
+        nn(ct, "ciphertext");
+        nn(bytes, "bytes");
+        *bytes = ciphertext_symmetric_bytes(ct->ct);
+    });
+}
+
+int phx_ciphertext_save_symmetric(const phx_context *ctx, const phx_ciphertext *ct, void *buf, size_t bytes) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct, "ciphertext");
+        nn(buf, "buffer");
+        ciphertext_save_symmetric(*ctx->ctx, ct->ct, static_cast<uint8_t *>(buf), bytes);
+    });
+}
+
+int phx_ciphertext_load_symmetric(const phx_context *ctx, phx_ciphertext *ct, const void *buf, size_t bytes) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct, "ciphertext");
+        nn(buf, "buffer");
+        ciphertext_load_symmetric(*ctx->ctx, ct->ct, static_cast<const uint8_t *>(buf), bytes);
+    });
+}
+
+int phx_sample_uniform_poly(const phx_context *ctx, const uint8_t *seed, size_t coeff_mod_size, uint64_t *out,
+                            phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(seed, "seed");
+        nn(out, "out");
+        sample_uniform_seeded(*ctx->ctx, seed, (u32)coeff_mod_size, out, S(s));
     });
 }
 

@@ -22,6 +22,10 @@ struct Ciphertext {
     u64 *data = nullptr;
     size_t capacity = 0;  // elements
     bool owns = true;
+    // symmetric encryption from a seed: c1 is the seeded sampler's output
+    // (PhantomCiphertext::seed_, include/ciphertext.h:25)
+    bool has_seed = false;
+    uint8_t seed[64] = {};
 
     void resize(const Context &ctx, u32 chain, u32 sz, hipStream_t s);
     void release(hipStream_t s);
@@ -109,6 +113,11 @@ void galois_key_generate(const Context &ctx, const SecretKey &sk, u32 elt, KSwit
 // plain: [Ql][N] coefficient-domain residues at chain index `chain`; result NTT form
 void encrypt_symmetric(const Context &ctx, const SecretKey &sk, const u64 *plain_dev, u32 chain, Ciphertext &ct,
                        u64 seed, hipStream_t s);
+// the same with c1 = the seeded uniform sampler's output for seed_a (64 bytes), as
+// encrypt_symmetric / encrypt_zero_symmetric do (src/secretkey.cu:232-300, 463-500);
+// the ciphertext keeps the seed (save_symmetric). e comes from the counter hash (seed_e).
+void encrypt_symmetric_seeded(const Context &ctx, const SecretKey &sk, const u64 *plain_dev, u32 chain,
+                              Ciphertext &ct, const uint8_t *seed_a, u64 seed_e, hipStream_t s);
 // out: [Ql][N] coefficient-domain residues of c0 + c1 s (+ c2 s^2)
 void decrypt(const Context &ctx, SecretKey &sk, const Ciphertext &ct, u64 *out_dev, hipStream_t s);
 

@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "evaluator.h"
+#include "prng.h"
 #include "tune.h"
 #include "bfv.h"
 
@@ -449,15 +450,17 @@ __global__ void k_sample_error(u64 *e, const DevModulus *mod, u32 N, u32 row0, u
 // digit's limbs). Rows are data rows 0..L-1 (keys: all QP rows).
 // reference: encrypt_zero_symmetric src/secretkey.cu:232-289,
 //            multiply_temp_mod_and_add_rns_poly src/polymath.cu:318-338
+// a_in given: a is read from it (c1 may alias it), else drawn from the counter hash
 __global__ void k_rlwe(u64 *c0, u64 *c1, const u64 *s, const u64 *e_ntt, const u64 *m_ntt, const u64 *gadget_key,
-                       const u64 *bigP, u32 g_lo, u32 g_hi, const DevModulus *mod, u32 N, u64 seed) {
+                       const u64 *bigP, u32 g_lo, u32 g_hi, const DevModulus *mod, u32 N, u64 seed,
+                       const u64 *a_in) {
     const u32 limb = blockIdx.y;
     const u32 k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= N) return;
     const u64 o = (u64)limb * N + k;
     const DevModulus md = mod[limb];
     const u64 q = md.value;
-    const u64 a = uniform_mod(seed, o, q);
+    const u64 a = a_in ? a_in[o] : uniform_mod(seed, o, q);
     u64 v = add_mod(mul_mod(a, s[o], md), e_ntt[o], q);
     v = v ? q - v : 0;
     if (m_ntt) v = add_mod(v, m_ntt[o], q);
@@ -830,7 +833,7 @@ void kswitch_key_generate(const Context &ctx, const SecretKey &sk, const u64 *ne
         const u32 g_lo = alpha * j, g_hi = std::min(alpha * (j + 1), ctx.size_Q());
         hipLaunchKernelGGL(k_rlwe, grid_for(N, QP), dim3(kThreads), 0, s, b, b + QPN, (const u64 *)sk.s,
                            (const u64 *)e, (const u64 *)nullptr, new_key, ctx.bigP_mod_q(), g_lo, g_hi,
-                           ctx.dev_mod(), (u32)N, sj);
+                           ctx.dev_mod(), (u32)N, sj, (const u64 *)nullptr);
     }
     PHX_HIP_CHECK(hipGetLastError());
     dev_free(e, s);
@@ -859,12 +862,18 @@ void galois_key_generate(const Context &ctx, const SecretKey &sk, u32 elt, KSwit
     dev_free(rs, s);
 }
 
-void encrypt_symmetric(const Context &ctx, const SecretKey &sk, const u64 *plain_dev, u32 chain, Ciphertext &ct,
-                       u64 seed, hipStream_t s) {
+namespace {
+void encrypt_rlwe(const Context &ctx, const SecretKey &sk, const u64 *plain_dev, u32 chain, Ciphertext &ct, u64 seed,
+                  const uint8_t *seed_a, hipStream_t s) {
     const u64 N = ctx.N();
     const u32 L = ctx.limbs_at(chain);
     ct.resize(ctx, chain, 2, s);
     ct.is_ntt_form = true;
+    ct.has_seed = seed_a != nullptr;
+    if (seed_a) {
+        std::memcpy(ct.seed, seed_a, sizeof(ct.seed));
+        sample_uniform_seeded(ctx, seed_a, L, ct.data + (u64)L * N, s);
+    }
     u64 *m = static_cast<u64 *>(dev_alloc((size_t)L * N * 8, s));
     u64 *e = static_cast<u64 *>(dev_alloc((size_t)L * N * 8, s));
     PHX_HIP_CHECK(hipMemcpyAsync(m, plain_dev, (size_t)L * N * 8, hipMemcpyDeviceToDevice, s));
@@ -874,10 +883,22 @@ void encrypt_symmetric(const Context &ctx, const SecretKey &sk, const u64 *plain
     ntt_forward(ctx.tables(), limb_map(e, 0, L), L, 1, s);
     hipLaunchKernelGGL(k_rlwe, grid_for(N, L), dim3(kThreads), 0, s, ct.data, ct.data + (u64)L * N,
                        (const u64 *)sk.s, (const u64 *)e, (const u64 *)m, (const u64 *)nullptr, (const u64 *)nullptr,
-                       0u, 0u, ctx.dev_mod(), (u32)N, seed);
+                       0u, 0u, ctx.dev_mod(), (u32)N, seed, seed_a ? (const u64 *)(ct.data + (u64)L * N) : nullptr);
     PHX_HIP_CHECK(hipGetLastError());
     dev_free(m, s);
     dev_free(e, s);
+}
+}  // namespace
+
+void encrypt_symmetric(const Context &ctx, const SecretKey &sk, const u64 *plain_dev, u32 chain, Ciphertext &ct,
+                       u64 seed, hipStream_t s) {
+    encrypt_rlwe(ctx, sk, plain_dev, chain, ct, seed, nullptr, s);
+}
+
+void encrypt_symmetric_seeded(const Context &ctx, const SecretKey &sk, const u64 *plain_dev, u32 chain,
+                              Ciphertext &ct, const uint8_t *seed_a, u64 seed_e, hipStream_t s) {
+    require(seed_a != nullptr, "seed is null");
+    encrypt_rlwe(ctx, sk, plain_dev, chain, ct, seed_e, seed_a, s);
 }
 
 void decrypt(const Context &ctx, SecretKey &sk, const Ciphertext &ct, u64 *out_dev, hipStream_t s) {

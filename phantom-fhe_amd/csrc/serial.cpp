@@ -2,6 +2,10 @@
 #include "serial.h"
 
 #include <cstring>
+#include <vector>
+
+#include "ntt.h"
+#include "prng.h"
 
 namespace phx {
 
@@ -74,6 +78,73 @@ size_t ciphertext_load(const Context &ctx, Ciphertext &ct, const uint8_t *buf, s
     PHX_HIP_CHECK(hipMemcpyAsync(ct.data, buf + kCtHeaderBytes, n * 8, hipMemcpyHostToDevice, hipStreamPerThread));
     PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
     return kCtHeaderBytes + n * 8;
+}
+
+size_t ciphertext_symmetric_bytes(const Ciphertext &ct) {
+    return kCtHeaderBytes + (size_t)ct.coeff_modulus_size * ct.poly_degree * 8 + kPrngSeedBytes;
+}
+
+namespace {
+// c1 of a seeded ciphertext: the sampler's output, inverse-transformed for coefficient form
+u64 *regenerate_c1(const Context &ctx, const uint8_t *seed, u32 L, bool ntt_form, hipStream_t s) {
+    u64 *c1 = static_cast<u64 *>(dev_alloc((size_t)L * ctx.N() * 8, s));
+    sample_uniform_seeded(ctx, seed, L, c1, s);
+    if (!ntt_form) ntt_inverse(ctx.tables(), limb_map(c1, 0, L), L, 1, s);
+    return c1;
+}
+}  // namespace
+
+void ciphertext_save_symmetric(const Context &ctx, const Ciphertext &ct, uint8_t *buf, size_t bytes) {
+    require(ct.data != nullptr, "ciphertext is empty");
+    if (ct.is_asymmetric) fail(kRuntimeError, "Asymmetric ciphertext does not have seed.");
+    if (ct.size != 2) fail(kRuntimeError, "This method is only for 2-polynomial ciphertext.");
+    require(ct.has_seed, "the ciphertext was not encrypted from a seed");
+    if (ct.coeff_modulus_size != ctx.size_Q()) fail(kRuntimeError, "Only support ciphertext without modulus switching.");
+    require(bytes >= ciphertext_symmetric_bytes(ct), "buffer too small for the ciphertext");
+    const size_t n = (size_t)ct.coeff_modulus_size * ct.poly_degree;
+    hipStream_t s = hipStreamPerThread;
+    // the seed stands for c1 only while c1 is still the sampler's output
+    u64 *c1 = regenerate_c1(ctx, ct.seed, ct.coeff_modulus_size, ct.is_ntt_form, s);
+    std::vector<u64> want(n), have(n);
+    PHX_HIP_CHECK(hipMemcpyAsync(want.data(), c1, n * 8, hipMemcpyDeviceToHost, s));
+    PHX_HIP_CHECK(hipMemcpyAsync(have.data(), ct.data + n, n * 8, hipMemcpyDeviceToHost, s));
+    PHX_HIP_CHECK(hipStreamSynchronize(s));
+    dev_free(c1, s);
+    require(want == have, "the ciphertext's c1 no longer matches its seed (it was modified after encryption)");
+    put_header(buf, Header{ct.chain_index, ct.size, ct.poly_degree, ct.coeff_modulus_size, ct.scale,
+                           ct.correction_factor, ct.noise_scale_deg, (uint8_t)ct.is_ntt_form,
+                           (uint8_t)ct.is_asymmetric});
+    PHX_HIP_CHECK(hipMemcpy(buf + kCtHeaderBytes, ct.data, n * 8, hipMemcpyDeviceToHost));
+    std::memcpy(buf + kCtHeaderBytes + n * 8, ct.seed, kPrngSeedBytes);
+}
+
+size_t ciphertext_load_symmetric(const Context &ctx, Ciphertext &ct, const uint8_t *buf, size_t bytes) {
+    require(bytes >= kCtHeaderBytes, "stream too short for a ciphertext header");
+    const Header h = get_header(buf);
+    if (h.is_asymmetric) fail(kRuntimeError, "Asymmetric ciphertext does not have seed.");
+    if (h.size != 2) fail(kRuntimeError, "This method is only for 2-polynomial ciphertext.");
+    require(h.poly_degree == ctx.N(), "ciphertext poly_modulus_degree does not match the context");
+    if (h.coeff_modulus_size != ctx.size_Q()) fail(kRuntimeError, "Only support ciphertext without modulus switching.");
+    require(h.chain_index >= 1 && h.chain_index <= ctx.size_Q() &&
+                ctx.limbs_at((u32)h.chain_index) == h.coeff_modulus_size,
+            "ciphertext chain_index does not match its coeff_modulus_size");
+    const size_t n = (size_t)h.coeff_modulus_size * h.poly_degree;
+    require(bytes >= kCtHeaderBytes + n * 8 + kPrngSeedBytes, "stream too short for the ciphertext data");
+    hipStream_t s = hipStreamPerThread;
+    ct.resize(ctx, (u32)h.chain_index, 2, s);
+    ct.scale = h.scale;
+    ct.correction_factor = h.correction_factor;
+    ct.noise_scale_deg = (u32)h.noise_scale_deg;
+    ct.is_ntt_form = h.is_ntt_form != 0;
+    ct.is_asymmetric = false;
+    ct.has_seed = true;
+    std::memcpy(ct.seed, buf + kCtHeaderBytes + n * 8, kPrngSeedBytes);
+    PHX_HIP_CHECK(hipMemcpyAsync(ct.data, buf + kCtHeaderBytes, n * 8, hipMemcpyHostToDevice, s));
+    u64 *c1 = regenerate_c1(ctx, ct.seed, (u32)h.coeff_modulus_size, ct.is_ntt_form, s);
+    PHX_HIP_CHECK(hipMemcpyAsync(ct.data + n, c1, n * 8, hipMemcpyDeviceToDevice, s));
+    dev_free(c1, s);
+    PHX_HIP_CHECK(hipStreamSynchronize(s));
+    return kCtHeaderBytes + n * 8 + kPrngSeedBytes;
 }
 
 size_t kswitch_key_bytes(const Context &ctx, const KSwitchKey &key) {

@@ -5,6 +5,11 @@
 // (double), correction_factor (u64), noiseScaleDeg (8-byte unsigned),
 // is_ntt_form, is_asymmetric (one byte each), then the data [size][L][N] u64;
 // little-endian, no padding: 58 header bytes.
+// PhantomCiphertext::save_symmetric / load_symmetric (include/ciphertext.h:216-305):
+// the same header, c0 only ([L][N] u64), then the 64-byte seed from which load
+// regenerates c1 (the seeded sampler, NTT form; inverse-transformed for a
+// coefficient-form ciphertext); only 2-polynomial symmetric ciphertexts at the
+// first data level.
 // PhantomRelinKey::save (include/secretkey.h:129-147): dnum (8 bytes), then
 // dnum public keys, each a ciphertext (chain 0, size 2, QP limbs).
 // PhantomGaloisKey::save (include/secretkey.h:194-206): the number of keys
@@ -21,6 +26,11 @@ size_t ciphertext_bytes(const Ciphertext &ct);
 void ciphertext_save(const Ciphertext &ct, uint8_t *buf, size_t bytes);
 // consumes one ciphertext from buf; returns the bytes read
 size_t ciphertext_load(const Context &ctx, Ciphertext &ct, const uint8_t *buf, size_t bytes);
+
+size_t ciphertext_symmetric_bytes(const Ciphertext &ct);
+// requires the ciphertext's c1 to still be its seed's sampler output
+void ciphertext_save_symmetric(const Context &ctx, const Ciphertext &ct, uint8_t *buf, size_t bytes);
+size_t ciphertext_load_symmetric(const Context &ctx, Ciphertext &ct, const uint8_t *buf, size_t bytes);
 
 size_t kswitch_key_bytes(const Context &ctx, const KSwitchKey &key);
 void kswitch_key_save(const Context &ctx, const KSwitchKey &key, uint8_t *buf, size_t bytes);

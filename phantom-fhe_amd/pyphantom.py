@@ -81,6 +81,12 @@ _SIGS = {
     "phx_ciphertext_save_size": (ctypes.c_int, [_vp, ctypes.POINTER(_sz)]),
     "phx_ciphertext_save": (ctypes.c_int, [_vp, _vp, _sz]),
     "phx_ciphertext_load": (ctypes.c_int, [_vp, _vp, _vp, _sz]),
+    "phx_ciphertext_save_symmetric_size": (ctypes.c_int, [_vp, ctypes.POINTER(_sz)]),
+    "phx_ciphertext_save_symmetric": (ctypes.c_int, [_vp, _vp, _vp, _sz]),
+    "phx_ciphertext_load_symmetric": (ctypes.c_int, [_vp, _vp, _vp, _sz]),
+    "phx_sample_uniform_poly": (ctypes.c_int, [_vp, ctypes.c_char_p, _sz, _vp, _vp]),
+    "phx_encrypt_symmetric_seeded": (ctypes.c_int, [_vp, _vp, _u64p, _sz, ctypes.c_char_p, ctypes.c_uint64, _vp,
+                                                    _vp]),
     "phx_relin_key_save_size": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_sz)]),
     "phx_relin_key_save": (ctypes.c_int, [_vp, _vp, _vp, _sz]),
     "phx_relin_key_load": (ctypes.c_int, [_vp, _vp, _sz, ctypes.POINTER(_vp)]),
@@ -341,6 +347,20 @@ class ciphertext:
         _check(lib().phx_ciphertext_load(ctx._h, ct._h, data, len(data)))
         return ct
 
+    def save_symmetric(self, ctx):
+        """PhantomCiphertext::save_symmetric byte stream (c0 + seed)"""
+        n = _sz()
+        _check(lib().phx_ciphertext_save_symmetric_size(self._h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().phx_ciphertext_save_symmetric(ctx._h, self._h, buf, n.value))
+        return buf.raw
+
+    @staticmethod
+    def load_symmetric(ctx, data):
+        ct = ciphertext()
+        _check(lib().phx_ciphertext_load_symmetric(ctx._h, ct._h, data, len(data)))
+        return ct
+
     def set_ntt_form(self, v):
         _check(lib().phx_ciphertext_set_ntt_form(self._h, int(bool(v))))
 
@@ -494,6 +514,16 @@ class secret_key:
         a = np.ascontiguousarray(plain_limbs, np.uint64)
         ct = ciphertext()
         _check(lib().phx_encrypt_symmetric(ctx._h, self._h, _p(a), chain_index, seed, ct._h, _stream(stream)))
+        return ct
+
+    def encrypt_symmetric_seeded(self, ctx, plain_limbs, chain_index, seed_a, seed_e=5, stream=None):
+        """c1 from the seeded uniform sampler (64-byte seed_a), kept for save_symmetric"""
+        if len(seed_a) != 64:
+            raise ValueError("the seed is 64 bytes")
+        a = np.ascontiguousarray(plain_limbs, np.uint64)
+        ct = ciphertext()
+        _check(lib().phx_encrypt_symmetric_seeded(ctx._h, self._h, _p(a), chain_index, bytes(seed_a), seed_e, ct._h,
+                                                  _stream(stream)))
         return ct
 
     def decrypt(self, ctx, ct, stream=None):
@@ -819,6 +849,13 @@ def tune_get(key):
     v = ctypes.c_int()
     _check(lib().phx_tune_get(key.encode(), ctypes.byref(v)))
     return v.value
+
+
+def sample_uniform_poly(ctx, seed, coeff_mod_size, out, stream=None):
+    """out (DeviceBuffer, [coeff_mod_size][N]) <- the seeded uniform sampler for a 64-byte seed"""
+    if len(seed) != 64:
+        raise ValueError("the seed is 64 bytes")
+    _check(lib().phx_sample_uniform_poly(ctx._h, bytes(seed), coeff_mod_size, out.ptr(), _stream(stream)))
 
 
 def copy_u64(dst, src, n, stream=None):

@@ -77,6 +77,9 @@ def lib():
         L.orc_hps_modswitch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p, _u64p]
         L.orc_bfv_multiply_hps2.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p, _u64p, ctypes.c_int, _u64p]
         L.orc_hps_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_salsa_quarterround.argtypes = [ctypes.POINTER(ctypes.c_uint32)]
+        L.orc_salsa_hash.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        L.orc_sample_uniform_seeded.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _sz, _u64p]
         L.orc_hps_r_size.restype = _sz
         L.orc_hps_r_size.argtypes = [ctypes.c_void_p]
         L.orc_hps_r.argtypes = [ctypes.c_void_p, _u64p]
@@ -319,3 +322,23 @@ class Hps:
         out = np.zeros((len(self.r()), self.octx.N), np.uint64)
         lib().orc_hps_modswitch(self.octx._h, self._h, _p(np.ascontiguousarray(x, np.uint64)), _p(out))
         return out
+
+
+def salsa_quarterround(y):
+    a = (ctypes.c_uint32 * 4)(*y)
+    lib().orc_salsa_quarterround(a)
+    return list(a)
+
+
+def salsa_hash(data):
+    assert len(data) == 64
+    out = ctypes.create_string_buffer(64)
+    lib().orc_salsa_hash(bytes(data), out)
+    return list(out.raw)
+
+
+def sample_uniform_seeded(octx, seed, L):
+    """the reference's seeded uniform sampler, [L][N]"""
+    out = np.zeros((L, octx.N), np.uint64)
+    lib().orc_sample_uniform_seeded(octx._h, bytes(seed), L, _p(out))
+    return out
