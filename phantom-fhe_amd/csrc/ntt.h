@@ -8,13 +8,13 @@ namespace phx {
 
 struct LoadPlain {
     __device__ __forceinline__ u64 operator()(const u64 *d, u32 i, u32, u32, const DevModulus &) const {
-        return d[i];
+        return ld_stream(d + i);
     }
 };
 
 struct StorePlain {
     __device__ __forceinline__ void operator()(u64 *d, u32 i, u64 v, u32, u32, const DevModulus &) const {
-        d[i] = v;
+        st_stream(d + i, v);
     }
 };
 

@@ -26,6 +26,12 @@
 
 namespace phx {
 
+// Every NTT pass streams its tiles through HBM once (the working set of a
+// launch is far above the L2), so data and intermediate accesses are
+// nontemporal: they do not displace the twiddle tables from L2.
+__device__ __forceinline__ u64 ld_stream(const u64 *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_stream(u64 *p, u64 v) { __builtin_nontemporal_store(v, p); }
+
 constexpr int kNttThreads = 256;
 constexpr int kNttRegs = 16;
 constexpr int kNttTile = kNttThreads * kNttRegs;  // 4096
