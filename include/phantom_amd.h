@@ -153,6 +153,11 @@ int phx_ciphertext_load(const phx_context *ctx, phx_ciphertext *ct, const void *
 int phx_ciphertext_save_symmetric_size(const phx_ciphertext *ct, size_t *bytes);
 int phx_ciphertext_save_symmetric(const phx_context *ctx, const phx_ciphertext *ct, void *buf, size_t bytes);
 int phx_ciphertext_load_symmetric(const phx_context *ctx, phx_ciphertext *ct, const void *buf, size_t bytes);
+/* PhantomSecretKey::save / load (include/secretkey.h:347-389): max power, N, QP (8 bytes each),
+ * then s (and s^2 when computed) in NTT form over QP; load keeps the first two powers */
+int phx_secret_key_save_size(const phx_context *ctx, const phx_secret_key *sk, size_t *bytes);
+int phx_secret_key_save(const phx_context *ctx, const phx_secret_key *sk, void *buf, size_t bytes);
+int phx_secret_key_load(const phx_context *ctx, const void *buf, size_t bytes, phx_secret_key **out);
 int phx_relin_key_save_size(const phx_context *ctx, const phx_relin_key *key, size_t *bytes);
 int phx_relin_key_save(const phx_context *ctx, const phx_relin_key *key, void *buf, size_t bytes);
 int phx_relin_key_load(const phx_context *ctx, const void *buf, size_t bytes, phx_relin_key **out);

@@ -505,6 +505,40 @@ int phx_sample_uniform_poly(const phx_context *ctx, const uint8_t *seed, size_t 
     });
 }
 
+int phx_secret_key_save_size(const phx_context *ctx, const phx_secret_key *sk, size_t *bytes) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(sk, "secret key");
+        nn(bytes, "bytes");
+        *bytes = secret_key_bytes(*ctx->ctx, sk->sk);
+    });
+}
+
+int phx_secret_key_save(const phx_context *ctx, const phx_secret_key *sk, void *buf, size_t bytes) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(sk, "secret key");
+        nn(buf, "buffer");
+        secret_key_save(*ctx->ctx, sk->sk, static_cast<uint8_t *>(buf), bytes);
+    });
+}
+
+int phx_secret_key_load(const phx_context *ctx, const void *buf, size_t bytes, phx_secret_key **out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(buf, "buffer");
+        nn(out, "out");
+        auto *k = new phx_secret_key{};
+        try {
+            secret_key_load(*ctx->ctx, k->sk, static_cast<const uint8_t *>(buf), bytes);
+        } catch (...) {
+            delete k;
+            throw;
+        }
+        *out = k;
+    });
+}
+
 int phx_relin_key_save_size(const phx_context *ctx, const phx_relin_key *key, size_t *bytes) {
     return guard([&] {
         nn(ctx, "context");

@@ -192,4 +192,44 @@ size_t kswitch_key_load(const Context &ctx, KSwitchKey &key, const uint8_t *buf,
     return (size_t)(p - buf);
 }
 
+// ---- secret key: [max_power, N, QP] then the powers [max_power][QP][N] ----
+
+size_t secret_key_bytes(const Context &ctx, const SecretKey &sk) {
+    const u64 powers = sk.s2 ? 2 : 1;
+    return 24 + powers * ctx.size_QP() * ctx.N() * 8;
+}
+
+void secret_key_save(const Context &ctx, const SecretKey &sk, uint8_t *buf, size_t bytes) {
+    if (!sk.s) fail(kInvalidArgument, "PhantomSecretKey has not been generated");
+    require(bytes >= secret_key_bytes(ctx, sk), "buffer too small for the secret key");
+    const u64 hdr[3] = {sk.s2 ? 2ull : 1ull, ctx.N(), ctx.size_QP()};
+    std::memcpy(buf, hdr, 24);
+    const size_t per = (size_t)ctx.size_QP() * ctx.N() * 8;
+    PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+    PHX_HIP_CHECK(hipMemcpy(buf + 24, sk.s, per, hipMemcpyDeviceToHost));
+    if (sk.s2) PHX_HIP_CHECK(hipMemcpy(buf + 24 + per, sk.s2, per, hipMemcpyDeviceToHost));
+}
+
+size_t secret_key_load(const Context &ctx, SecretKey &sk, const uint8_t *buf, size_t bytes) {
+    require(bytes >= 24, "stream too short for a secret key header");
+    u64 hdr[3];
+    std::memcpy(hdr, buf, 24);
+    require(hdr[1] == ctx.N(), "secret key poly_modulus_degree does not match the context");
+    require(hdr[2] == ctx.size_QP(), "secret key coeff_modulus_size does not match the context's key level");
+    require(hdr[0] >= 1 && hdr[0] <= 64, "secret key max power is invalid");
+    const size_t per = (size_t)ctx.size_QP() * ctx.N() * 8;
+    require((bytes - 24) / per >= hdr[0], "stream too short for the secret key data");
+    hipStream_t s = hipStreamPerThread;
+    u64 *d = static_cast<u64 *>(dev_alloc(per, s));
+    u64 *d2 = hdr[0] >= 2 ? static_cast<u64 *>(dev_alloc(per, s)) : nullptr;
+    PHX_HIP_CHECK(hipMemcpyAsync(d, buf + 24, per, hipMemcpyHostToDevice, s));
+    if (d2) PHX_HIP_CHECK(hipMemcpyAsync(d2, buf + 24 + per, per, hipMemcpyHostToDevice, s));
+    PHX_HIP_CHECK(hipStreamSynchronize(s));
+    dev_free(sk.s, s);
+    dev_free(sk.s2, s);
+    sk.s = d;
+    sk.s2 = d2;
+    return 24 + hdr[0] * per;
+}
+
 }  // namespace phx

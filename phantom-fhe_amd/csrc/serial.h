@@ -14,6 +14,11 @@
 // dnum public keys, each a ciphertext (chain 0, size 2, QP limbs).
 // PhantomGaloisKey::save (include/secretkey.h:194-206): the number of keys
 // (8 bytes), then that many relinearisation keys.
+// PhantomSecretKey::save (include/secretkey.h:347-389): sk_max_power,
+// poly_modulus_degree, coeff_modulus_size (QP), 8 bytes each, then the powers
+// s, s^2, ... of the secret key, [max_power][QP][N] u64 in NTT form. This
+// build keeps s and s^2 (decryption of 2- and 3-polynomial ciphertexts);
+// a stream with higher powers loads its first two.
 #pragma once
 
 #include "evaluator.h"
@@ -35,5 +40,9 @@ size_t ciphertext_load_symmetric(const Context &ctx, Ciphertext &ct, const uint8
 size_t kswitch_key_bytes(const Context &ctx, const KSwitchKey &key);
 void kswitch_key_save(const Context &ctx, const KSwitchKey &key, uint8_t *buf, size_t bytes);
 size_t kswitch_key_load(const Context &ctx, KSwitchKey &key, const uint8_t *buf, size_t bytes);
+
+size_t secret_key_bytes(const Context &ctx, const SecretKey &sk);
+void secret_key_save(const Context &ctx, const SecretKey &sk, uint8_t *buf, size_t bytes);
+size_t secret_key_load(const Context &ctx, SecretKey &sk, const uint8_t *buf, size_t bytes);
 
 }  // namespace phx

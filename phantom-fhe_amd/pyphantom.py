@@ -87,6 +87,9 @@ _SIGS = {
     "phx_sample_uniform_poly": (ctypes.c_int, [_vp, ctypes.c_char_p, _sz, _vp, _vp]),
     "phx_encrypt_symmetric_seeded": (ctypes.c_int, [_vp, _vp, _u64p, _sz, ctypes.c_char_p, ctypes.c_uint64, _vp,
                                                     _vp]),
+    "phx_secret_key_save_size": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_sz)]),
+    "phx_secret_key_save": (ctypes.c_int, [_vp, _vp, _vp, _sz]),
+    "phx_secret_key_load": (ctypes.c_int, [_vp, _vp, _sz, ctypes.POINTER(_vp)]),
     "phx_relin_key_save_size": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_sz)]),
     "phx_relin_key_save": (ctypes.c_int, [_vp, _vp, _vp, _sz]),
     "phx_relin_key_load": (ctypes.c_int, [_vp, _vp, _sz, ctypes.POINTER(_vp)]),
@@ -525,6 +528,22 @@ class secret_key:
         _check(lib().phx_encrypt_symmetric_seeded(ctx._h, self._h, _p(a), chain_index, bytes(seed_a), seed_e, ct._h,
                                                   _stream(stream)))
         return ct
+
+    def save(self):
+        """PhantomSecretKey::save byte stream"""
+        n = _sz()
+        _check(lib().phx_secret_key_save_size(self._ctx._h, self._h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().phx_secret_key_save(self._ctx._h, self._h, buf, n.value))
+        return buf.raw
+
+    @classmethod
+    def load(cls, ctx, data):
+        k = cls.__new__(cls)
+        h = _vp()
+        _check(lib().phx_secret_key_load(ctx._h, data, len(data), ctypes.byref(h)))
+        k._h, k._ctx = h, ctx
+        return k
 
     def decrypt(self, ctx, ct, stream=None):
         L = ct.coeff_modulus_size()

@@ -194,3 +194,39 @@ def test_symmetric_format_rejects():
         ph.ciphertext.load_symmetric(ctx, bytes(fewer))
     with pytest.raises(ValueError):
         ph.ciphertext.load_symmetric(ctx, blob[:-1])
+
+
+def test_secret_key_roundtrip(small):
+    """PhantomSecretKey::save layout: max power, N, QP, then s (and s^2 once a
+    3-polynomial ciphertext was decrypted) in NTT form; a loaded key decrypts
+    like the original"""
+    ctx, octx, mods = small
+    N = ctx.N
+    say("case: secret key save/load")
+    sk = ph.secret_key(ctx, seed=31)
+    blob = sk.save()
+    assert len(blob) == 24 + ctx.size_QP * N * 8
+    assert struct.unpack("<QQQ", blob[:24]) == (1, N, ctx.size_QP)
+    assert np.array_equal(np.frombuffer(blob[24:], np.uint64).reshape(ctx.size_QP, N), sk.ntt_form())
+    sk2 = ph.secret_key.load(ctx, blob)
+    assert np.array_equal(sk2.ntt_form(), sk.ntt_form())
+    rng = np.random.default_rng(5)
+    a = rand_ct(rng, mods[:ctx.size_Q], N, size=3)
+    c = upload_ct(ctx, a, 1)
+    want = sk.decrypt(ctx, c)            # computes s^2
+    assert np.array_equal(sk2.decrypt(ctx, c), want)
+    blob2 = sk.save()
+    assert len(blob2) == 24 + 2 * ctx.size_QP * N * 8 and struct.unpack("<Q", blob2[:8])[0] == 2
+    s = sk.ntt_form().astype(object)
+    s2 = np.array([[(int(x) * int(x)) % mods[i] for x in s[i]] for i in range(ctx.size_QP)], dtype=object)
+    assert np.array_equal(np.frombuffer(blob2[24 + ctx.size_QP * N * 8:], np.uint64).reshape(ctx.size_QP, N),
+                          s2.astype(np.uint64))
+    sk3 = ph.secret_key.load(ctx, blob2)
+    assert np.array_equal(sk3.decrypt(ctx, c), want)
+    assert sk3.save() == blob2
+    with pytest.raises(ValueError):
+        ph.secret_key.load(ctx, blob[:-8])                      # truncated
+    bad = bytearray(blob)
+    bad[8:16] = struct.pack("<Q", 2 * N)
+    with pytest.raises(ValueError):
+        ph.secret_key.load(ctx, bytes(bad))                     # other ring degree
