@@ -744,9 +744,19 @@ void apply_galois(const Context &ctx, Ciphertext &ct, u32 elt, const GaloisKeys 
     auto it = std::find(gk.elts.begin(), gk.elts.end(), elt);
     require(it != gk.elts.end(), "Galois elt not present");
     const KSwitchKey &key = gk.keys[it - gk.elts.begin()];
+    const bool bfv = ctx.params().scheme == kSchemeBFV;
+    if (bfv)
+        require(!ct.is_ntt_form, "BFV encrypted cannot be in NTT form");
+    else
+        require(ct.is_ntt_form, "CKKS encrypted must be in NTT form");
     const u32 Ql = ct.coeff_modulus_size;
     const u64 N = ctx.N(), LN = (u64)Ql * N;
     const u32 *perm = ctx.galois_perm(elt);
+    // BFV (coefficient form): the reference's coefficient-domain automorphism and
+    // its BFV key switch equal the NTT-domain permutation and key switch
+    // conjugated by the exact transform, so the ciphertext is transformed,
+    // rotated and transformed back
+    if (bfv) ntt_forward(ctx.tables(), limb_map(ct.data, 0, Ql, LN), Ql, 2, s);
     u64 *tmp = static_cast<u64 *>(dev_alloc(2 * LN * 8, s));
     hipLaunchKernelGGL(k_permute, grid_for(N, 2 * Ql), dim3(kThreads), 0, s, (const u64 *)ct.data, tmp, perm,
                        (u32)N);
@@ -754,6 +764,7 @@ void apply_galois(const Context &ctx, Ciphertext &ct, u32 elt, const GaloisKeys 
     // (c0, c1) <- (pi(c0), 0) + keyswitch(pi(c1))
     keyswitch(ctx, ct.chain_index, tmp + LN, key, ct.data, tmp, nullptr, s);
     dev_free(tmp, s);
+    if (bfv) ntt_inverse(ctx.tables(), limb_map(ct.data, 0, Ql, LN), Ql, 2, s);
 }
 
 u32 galois_elt_from_step(int step, u64 N) {

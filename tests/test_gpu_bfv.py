@@ -301,3 +301,82 @@ def test_config4_joined_batch_graph():
         a, b = inputs[i]
         assert np.array_equal(want[i], ob.bfv_relinearize(octx, beh.multiply(a, b), keys[N])), (i, N)
     assert seen == set(LIMBS)
+
+
+def coeff_automorphism(x, g, mods):
+    """X^j -> X^(j g) on coefficient-form limbs [L][N] (negacyclic: a wrap past N negates)"""
+    L, N = x.shape
+    j = np.arange(N, dtype=np.int64)
+    d = (j * g) % (2 * N)
+    neg = d >= N
+    d = np.where(neg, d - N, d)
+    out = np.zeros_like(x)
+    for i in range(L):
+        q = np.uint64(mods[i])
+        out[i, d] = np.where(neg & (x[i] != 0), q - x[i], x[i])
+    return out
+
+
+def test_bfv_apply_galois_bit_exact(small):
+    """BFV apply_galois / rotate on coefficient-form ciphertexts: the reference's
+    coefficient-domain automorphism of c0 and c1, then the BFV key switch of
+    pi(c1), i.e. the oracle's BFV relinearisation of (pi(c0), 0, pi(c1))"""
+    ctx, octx, beh, mods = small
+    N, k = ctx.N, ctx.size_Q
+    say("case: bfv apply_galois N=2^13 k=4")
+    rng = np.random.default_rng(21)
+    elts = [ph.galois_elt_from_step(s, N) for s in (1, -3)] + [2 * N - 1]
+    keys = np.stack([rand_key(rng, mods, N, ctx.dnum) for _ in elts])
+    gk = ph.galois_key(ctx, elts, host=keys)
+    a = rand_ct(rng, mods[:k], N)
+    for elt, key in zip(elts, keys):
+        c = upload_bfv(ctx, a)
+        ph.apply_galois_inplace(ctx, c, elt, gk)
+        assert not c.is_ntt_form()
+        ct3 = np.stack([coeff_automorphism(a[0], elt, mods), np.zeros_like(a[0]), coeff_automorphism(a[1], elt, mods)])
+        assert np.array_equal(c.download(N), ob.bfv_relinearize(octx, ct3, key)), elt
+    c = upload_bfv(ctx, a)
+    ph.rotate_inplace(ctx, c, 1, gk)
+    ct3 = np.stack([coeff_automorphism(a[0], elts[0], mods), np.zeros_like(a[0]), coeff_automorphism(a[1], elts[0], mods)])
+    assert np.array_equal(c.download(N), ob.bfv_relinearize(octx, ct3, keys[0]))
+    c.set_ntt_form(True)
+    with pytest.raises(ValueError):
+        ph.apply_galois_inplace(ctx, c, elts[0], gk)
+
+
+def test_bfv_rotate_decrypts():
+    """encrypt m, rotate with a Galois key of pi(s), decrypt: the automorphism of m mod t"""
+    N, k = 1 << 13, 4
+    say("case: bfv rotate semantic")
+    ctx, octx, beh, mods = make_bfv(N, k, ph.MUL_TECH_HPS)
+    qs = mods[:k]
+    Q = 1
+    for q in qs:
+        Q *= q
+    delta = Q // T
+    rng = np.random.default_rng(8)
+    s = rng.integers(-1, 2, N)
+    res = lambda v, ms: np.stack([np.array([int(x) % q for x in v], np.uint64) for q in ms])
+    s_all = res(s, mods)
+    m = rng.integers(0, T, N)
+    a = np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs])
+    e = res(rng.integers(-4, 5, N), qs)
+    dm = np.stack([np.array([(delta % q) * int(v) % q for v in m], np.uint64) for q in qs])
+    as_ = poly_mul_rns(octx, a, s_all[:k], k)
+    c0 = np.stack([(dm[i].astype(object) + e[i].astype(object) - as_[i].astype(object)) % qs[i]
+                   for i in range(k)]).astype(np.uint64)
+    g = ph.galois_elt_from_step(3, N)
+    s_ntt = octx.ntt_forward(s_all, 0)
+    sg_ntt = octx.ntt_forward(coeff_automorphism(s_all, g, mods), 0)
+    dnum = ctx.dnum
+    ka = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in mods]) for _ in range(dnum)])
+    ke = np.stack([res(rng.integers(-4, 5, N), mods) for _ in range(dnum)])
+    gk = ph.galois_key(ctx, [g], host=octx.make_kswitch_key(s_ntt, sg_ntt, ka, ke)[None])
+    c = upload_bfv(ctx, np.stack([c0, a]))
+    ph.rotate_inplace(ctx, c, 3, gk)
+    out = c.download(N)
+    v = (out[0].astype(object) + poly_mul_rns(octx, out[1], s_all[:k], k).astype(object))
+    v = v % np.array(qs, dtype=object)[:, None]
+    dec = [((crt([int(v[i, n]) for i in range(k)], qs)[0] * T + Q // 2) // Q) % T for n in range(N)]
+    want = coeff_automorphism(np.array([m], np.uint64), g, [T])[0]
+    assert [int(x) for x in want] == dec
