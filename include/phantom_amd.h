@@ -40,6 +40,7 @@ extern "C" {
 #define PHX_MUL_TECH_BEHZ 1
 #define PHX_MUL_TECH_HPS 2
 #define PHX_MUL_TECH_HPS_OVERQ 3
+#define PHX_MUL_TECH_HPS_OVERQ_LEVELED 4
 
 typedef struct phx_context phx_context;
 typedef struct phx_ciphertext phx_ciphertext;

@@ -1060,23 +1060,24 @@ struct orc_hps {
     double *frac;                  /* [k] tRSHatInvModsDivsFrac */
     u64 *divs_mod_r;               /* [nr][k + 1] tRSHatInvModsDivsModr */
     /* over Q */
-    u64 *negrq;                    /* [k] negPQHatInvModq of the Q -> R converter (P = R) */
-    u64 *qinv_r;                   /* [nr][k] QInvModp: q_i^-1 mod r_j */
+    size_t ks;                     /* source moduli of the switch: q_0 .. q_{ks-1} */
+    u64 *negrq;                    /* [ks] negPQHatInvModq of the source -> R converter (P = R) */
+    u64 *qinv_r;                   /* [nr][ks] QInvModp: q_i^-1 mod r_j */
     double *frac_r;                /* [nr] tQlSlHatInvModsDivsFrac */
     u64 *divs_mod_q;               /* [k][nr + 1] tQlSlHatInvModsDivsModq */
 };
 
 static void hps_overq_consts(const orc_ctx *c, orc_hps *h) {
-    size_t k = h->k, nr = h->nr, ns = k + nr;
-    h->negrq = (u64 *)malloc(k * 8);
-    h->qinv_r = (u64 *)malloc(nr * k * 8);
+    size_t k = h->k, nr = h->nr, ns = k + nr, ks = h->ks;
+    h->negrq = (u64 *)malloc(ks * 8);
+    h->qinv_r = (u64 *)malloc(nr * ks * 8);
     h->frac_r = (double *)malloc(nr * sizeof(double));
     h->divs_mod_q = (u64 *)malloc(k * (nr + 1) * 8);
-    for (size_t i = 0; i < k; i++) {
+    for (size_t i = 0; i < ks; i++) {
         u64 q = c->q[i];
-        u64 v = mulmod(prod_mod(h->r, nr, q), invmod(punctured_mod(c->q, k, i, q), q), q);
+        u64 v = mulmod(prod_mod(h->r, nr, q), invmod(punctured_mod(c->q, ks, i, q), q), q);
         h->negrq[i] = q - v;
-        for (size_t j = 0; j < nr; j++) h->qinv_r[j * k + i] = invmod(q % h->r[j], h->r[j]);
+        for (size_t j = 0; j < nr; j++) h->qinv_r[j * ks + i] = invmod(q % h->r[j], h->r[j]);
     }
     /* theta_i = t Q [(S/s_i)^-1]_{s_i}, S = Q R (src/rns.cu:833-890) */
     u64 *s = (u64 *)malloc(ns * 8);
@@ -1100,12 +1101,14 @@ static void hps_overq_consts(const orc_ctx *c, orc_hps *h) {
 
 orc_hps *orc_hps_create(const orc_ctx *c, u64 t) { return orc_hps_create2(c, t, 0); }
 
-orc_hps *orc_hps_create2(const orc_ctx *c, u64 t, int overq) {
+/* HPS tool for Ql = q_0 .. q_{k-1}; R from the smallest prime of all of Q
+ * (src/rns.cu:686-699, 792-801: the R of a lower level is a prefix of the top one) */
+static orc_hps *hps_create(const orc_ctx *c, u64 t, int overq, size_t k, size_t ks) {
     orc_hps *h = (orc_hps *)calloc(1, sizeof(orc_hps));
-    size_t k = c->size_Q, nr = overq ? k : k + 1, N = c->N;
-    h->k = k; h->nr = nr; h->t = t; h->overq = overq;
+    size_t nr = overq ? k : k + 1, N = c->N;
+    h->k = k; h->nr = nr; h->t = t; h->overq = overq; h->ks = ks;
     u64 qmin = c->q[0];
-    for (size_t i = 1; i < k; i++) if (c->q[i] < qmin) qmin = c->q[i];
+    for (size_t i = 1; i < c->size_Q; i++) if (c->q[i] < qmin) qmin = c->q[i];
     h->r = (u64 *)malloc(nr * 8);
     int bits = bit_count(qmin);
     u64 lower = (u64)1 << (bits - 1), v = qmin - 2 * N;
@@ -1142,6 +1145,10 @@ orc_hps *orc_hps_create2(const orc_ctx *c, u64 t, int overq) {
     free(s);
     if (overq) hps_overq_consts(c, h);
     return h;
+}
+
+orc_hps *orc_hps_create2(const orc_ctx *c, u64 t, int overq) {
+    return hps_create(c, t, overq, c->size_Q, c->size_Q);
 }
 
 void orc_hps_destroy(orc_hps *h) {
@@ -1195,13 +1202,13 @@ static void r_ntt(const orc_hps *h, size_t N, u64 *data, int inverse) {
  * For X = CRT(x) in [0, Q): out = (R X + u) / Q with u = sum_i y_i Qhat_i in
  * [0, k Q), i.e. the modulus switch Q -> R of X rounded up by at most k. */
 void orc_hps_modswitch(const orc_ctx *c, const orc_hps *h, const u64 *x, u64 *out) {
-    size_t N = c->N, k = h->k, nr = h->nr;
+    size_t N = c->N, ks = h->ks, nr = h->nr;
     for (size_t n = 0; n < N; n++) {
         u64 y[64];
-        for (size_t i = 0; i < k; i++) y[i] = mulmod(x[i * N + n], h->negrq[i], c->q[i]);
+        for (size_t i = 0; i < ks; i++) y[i] = mulmod(x[i * N + n], h->negrq[i], c->q[i]);
         for (size_t j = 0; j < nr; j++) {
             u128 acc = 0;
-            for (size_t i = 0; i < k; i++) acc += (u128)y[i] * h->qinv_r[j * k + i];
+            for (size_t i = 0; i < ks; i++) acc += (u128)y[i] * h->qinv_r[j * ks + i];
             out[j * N + n] = (u64)(acc % h->r[j]);
         }
     }
@@ -1287,6 +1294,147 @@ int orc_bfv_multiply_hps2(const orc_ctx *c, const orc_hps *h, const u64 *ct1, co
         bconv_hps(N, h->r, nr, c->q, k, tr, out3 + p * QN);
     }
     free(a); free(b); free(tr);
+    return 0;
+}
+
+/* ---- hps_overq_leveled (src/evaluate.cu:549-646, 648-801, 819-1027; src/rns.cu:920-1001, 1799-1846) ---- */
+
+orc_hps *orc_hps_create_level(const orc_ctx *c, u64 t, size_t d) {
+    size_t k0 = c->size_Q;
+    if (d >= k0) return NULL;
+    return hps_create(c, t, 1, k0 - d, d ? k0 : k0);
+}
+
+/* FindLevelsToDrop (src/evaluate.cu:549-646): sigma = 3.2f, assurance 36,
+ * ternary key bound 1 (include/host/hestdparms.h:152-153); natural logs as
+ * the reference evaluates them */
+size_t orc_hps_levels_to_drop(const orc_ctx *c, u64 t, size_t depth, int key_switch, int asymmetric) {
+    double sigma = (double)3.2f, alpha = 36.0, p = (double)t, Bkey = 1.0;
+    uint32_t kP = (uint32_t)c->size_P, numPartQ = (uint32_t)orc_beta(c, c->size_Q), n = (uint32_t)c->N;
+    u64 qmax = 0;
+    for (size_t i = 0; i < c->size_Q; i++) if (c->q[i] > qmax) qmax = c->q[i];
+    double dcrtBits = (double)bit_count(qmax);
+    double Berr = sigma * sqrt(alpha);
+    double delta = 2. * sqrt(n);
+    double Vnorm = asymmetric ? (1. + delta * Bkey) / 2. : Berr * (1. + 2. * delta * Bkey);
+    double noiseKS = kP * (numPartQ * delta * Berr + delta * Bkey + 1.0) / 2;
+    double C1 = delta * delta * p * Bkey;
+    double C2 = delta * delta * Bkey * Bkey / 2.0 + noiseKS;
+    double logq = depth > 0 ? log(4 * p) + (double)(depth - 1) * log(C1) + log(C1 * Vnorm + (double)depth * C2)
+                            : log(p * (4 * Vnorm));
+    double loge = logq / log(2) - 2 - log2(p);
+    double logExtra = key_switch ? log2(noiseKS) : log2(delta);
+    int32_t levels = (int32_t)floor((loge - (double)(2 * depth) - 16 - logExtra) / dcrtBits);
+    if (levels < 0) levels = 0;
+    else if (levels > (int32_t)c->size_Q - 1) levels = (int32_t)c->size_Q - 1;
+    return (size_t)levels;
+}
+
+/* scaleAndRound_HPS_Q_Ql (src/rns.cu:1799-1806, tables 920-1001): theta_i = Ql
+ * [(Q/q_i)^-1]_{q_i}; nu = 0.5 + sum_j x_{Ql+j} frac_j (fused multiply-adds in
+ * order), alpha = (u64) nu; out_i = [sum_j x_{Ql+j} D_ij + x_i D_id + alpha]_{q_i},
+ * i.e. round(Ql X / Q) for X = CRT_Q(x). x [k0][N] -> out [Ql][N], d >= 1 */
+void orc_hps_scale_q_ql(const orc_ctx *c, size_t d, const u64 *x, u64 *out) {
+    size_t N = c->N, k0 = c->size_Q, Ql = k0 - d;
+    double *frac = (double *)malloc(d * sizeof(double));
+    u64 *D = (u64 *)malloc(Ql * (d + 1) * 8);
+    for (size_t i = 0; i < k0; i++) {
+        orc_big th;
+        big_set(&th, 1);
+        for (size_t m = 0; m < Ql; m++) big_mul(&th, c->q[m]);
+        big_mul(&th, invmod(punctured_mod(c->q, k0, i, c->q[i]), c->q[i]));
+        if (i >= Ql) frac[i - Ql] = (double)big_mod(&th, c->q[i]) / (double)c->q[i];
+        big_div(&th, c->q[i]);
+        for (size_t m = 0; m < Ql; m++) {
+            if (i >= Ql) D[m * (d + 1) + (i - Ql)] = big_mod(&th, c->q[m]);
+            else if (i == m) D[m * (d + 1) + d] = big_mod(&th, c->q[m]);
+        }
+    }
+    for (size_t n = 0; n < N; n++) {
+        double nu = 0.5;
+        for (size_t j = 0; j < d; j++) nu = fma((double)x[(Ql + j) * N + n], frac[j], nu);
+        u64 alpha = (u64)nu;
+        for (size_t i = 0; i < Ql; i++) {
+            u64 qi = c->q[i];
+            u128 acc = (u128)x[i * N + n] * D[i * (d + 1) + d];
+            for (size_t j = 0; j < d; j++) acc += (u128)x[(Ql + j) * N + n] * D[i * (d + 1) + j];
+            out[i * N + n] = addmod((u64)(acc % qi), alpha % qi, qi);
+        }
+    }
+    free(frac); free(D);
+}
+
+/* ExpandCRTBasis_Ql_Q (src/rns.cu:1808-1834): out_i = [x_i (Q/Ql)]_{q_i} for
+ * i < Ql, 0 for the dropped limbs. x [Ql][N] -> out [k0][N] (may alias) */
+void orc_hps_expand(const orc_ctx *c, size_t d, const u64 *x, u64 *out) {
+    size_t N = c->N, k0 = c->size_Q, Ql = k0 - d;
+    for (size_t i = 0; i < Ql; i++) {
+        u64 qi = c->q[i], f = prod_mod(c->q + Ql, d, qi);
+        for (size_t n = 0; n < N; n++) out[i * N + n] = mulmod(x[i * N + n], f, qi);
+    }
+    for (size_t i = Ql; i < k0; i++) memset(out + i * N, 0, N * 8);
+}
+
+/* bfv_multiply_hps / the multiply of bfv_mul_relin_hps with d dropped levels
+ * (src/evaluate.cu:648-801, 819-960), h = orc_hps_create_level(c, t, d):
+ * ct1 scaled Q -> Ql (scaleAndRound_HPS_Q_Ql) then extended Ql -> Rl (bConv_HPS);
+ * ct2 switched Q -> Rl (bConv_BEHZ_var1 from all of Q) and converted Rl -> Ql
+ * (bConv_HPS); a squaring extends ct1 only; NTT over QlRl, tensor, INTT,
+ * scaleAndRound_HPS_QlRl_Ql, then the first expand_polys results expanded Ql -> Q.
+ * ct1, ct2 [2][k0][N] -> out3 [3][k0][N]; limbs >= Ql of an unexpanded result are 0 */
+int orc_bfv_multiply_hps_level(const orc_ctx *c, const orc_hps *h, size_t d, const u64 *ct1, const u64 *ct2,
+                               int square, size_t expand_polys, u64 *out3) {
+    size_t N = c->N, k0 = c->size_Q, k = h->k, nr = h->nr, QN0 = k0 * N, QN = k * N, RN = nr * N, SN = QN + RN;
+    if (k + d != k0 || !h->overq) return -1;
+    u64 *a = (u64 *)malloc(3 * SN * 8), *b = (u64 *)malloc(2 * SN * 8);
+    const u64 *src[2] = {ct1, square ? ct1 : ct2};
+    u64 *dst[2] = {a, b};
+    for (int s2 = 0; s2 < 2; s2++)
+        for (int p = 0; p < 2; p++) {
+            u64 *dd = dst[s2] + p * SN;
+            const u64 *x = src[s2] + p * QN0;
+            if (s2 == 1 && !square) {
+                orc_hps_modswitch(c, h, x, dd + QN);
+                bconv_hps(N, h->r, nr, c->q, k, dd + QN, dd);
+            } else {
+                if (d) orc_hps_scale_q_ql(c, d, x, dd);
+                else memcpy(dd, x, QN * 8);
+                bconv_hps(N, c->q, k, h->r, nr, dd, dd + QN);
+            }
+            orc_ntt_forward(c, dd, k, 0);
+            r_ntt(h, N, dd + QN, 0);
+        }
+    for (size_t l = 0; l < k + nr; l++) {
+        u64 q = l < k ? c->q[l] : h->r[l - k];
+        for (size_t n = 0; n < N; n++) {
+            size_t o = l * N + n;
+            u64 a0 = a[o], a1 = a[SN + o], b0 = b[o], b1 = b[SN + o];
+            u64 r0 = mulmod(a0, b0, q), r2 = mulmod(a1, b1, q);
+            u64 r1 = mulmod(addmod(a0, a1, q), addmod(b0, b1, q), q);
+            r1 = submod(submod(r1, r0, q), r2, q);
+            a[o] = r0; a[SN + o] = r1; a[2 * SN + o] = r2;
+        }
+    }
+    memset(out3, 0, 3 * QN0 * 8);
+    for (int p = 0; p < 3; p++) {
+        u64 *dd = a + p * SN, *o = out3 + p * QN0;
+        orc_ntt_inverse(c, dd, k, 0);
+        r_ntt(h, N, dd + QN, 1);
+        for (size_t n = 0; n < N; n++) {
+            double nu = 0.5;
+            for (size_t j = 0; j < nr; j++) nu = fma((double)dd[QN + j * N + n], h->frac_r[j], nu);
+            u64 alpha = (u64)nu;
+            for (size_t i = 0; i < k; i++) {
+                u64 qi = c->q[i];
+                const u64 *dm = h->divs_mod_q + i * (nr + 1);
+                u128 acc = (u128)dd[i * N + n] * dm[nr];
+                for (size_t j = 0; j < nr; j++) acc += (u128)dd[QN + j * N + n] * dm[j];
+                o[i * N + n] = addmod((u64)(acc % qi), alpha % qi, qi);
+            }
+        }
+        if (d && (size_t)p < expand_polys) orc_hps_expand(c, d, o, o);
+    }
+    free(a); free(b);
     return 0;
 }
 

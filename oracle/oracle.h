@@ -117,6 +117,17 @@ int orc_bfv_multiply_hps2(const orc_ctx *c, const orc_hps *h, const uint64_t *ct
 int orc_bfv_multiply_hps(const orc_ctx *c, const orc_hps *h, const uint64_t *ct1, const uint64_t *ct2,
                          uint64_t *out3);
 
+/* hps_overq_leveled: the HPS tool of the level with d dropped primes (Ql = size_Q - d,
+ * Rl = the first Ql primes of R, the second operand switched from all of Q);
+ * FindLevelsToDrop; scaleAndRound_HPS_Q_Ql ([size_Q][N] -> [Ql][N]); ExpandCRTBasis_Ql_Q
+ * ([Ql][N] -> [size_Q][N]); the multiply with d dropped levels */
+orc_hps *orc_hps_create_level(const orc_ctx *c, uint64_t t, size_t d);
+size_t orc_hps_levels_to_drop(const orc_ctx *c, uint64_t t, size_t depth, int key_switch, int asymmetric);
+void orc_hps_scale_q_ql(const orc_ctx *c, size_t d, const uint64_t *x, uint64_t *out);
+void orc_hps_expand(const orc_ctx *c, size_t d, const uint64_t *x, uint64_t *out);
+int orc_bfv_multiply_hps_level(const orc_ctx *c, const orc_hps *h, size_t d, const uint64_t *ct1,
+                               const uint64_t *ct2, int square, size_t expand_polys, uint64_t *out3);
+
 /* the reference's seeded uniform sampler (sample_uniform_poly, src/prng.cu:174-210)
  * and its block function (the Salsa20 hash, src/prng.cu:17-131) */
 void orc_salsa_quarterround(uint32_t y[4]);

@@ -127,7 +127,7 @@ TuneResult autotune(Context &ctx, hipStream_t s) {
     auto time_choice = [&](const TuneChoice &ch) {
         ctx.set_tuning(ch);
         const double plain = timer.median_us(s, 8, [&] { multiply_and_relin(ctx, c[0], c[1], rlk, s); });
-        const double batch = timer.median_us(s, 2, [&] { plan->run(s); }) / kBatch;
+        const double batch = plan ? timer.median_us(s, 2, [&] { plan->run(s); }) / kBatch : 0.0;
         res.trials.emplace_back(describe(ch), plain + batch);
         return plain + batch;
     };
@@ -140,7 +140,8 @@ TuneResult autotune(Context &ctx, hipStream_t s) {
             b1.push_back(&c[2 + 2 * i]);
             b2.push_back(&c[3 + 2 * i]);
         }
-        plan.reset(BatchPlan::mul_relin(ctx, b1, b2, rlk));
+        // hps_overq_leveled has no batch plans (its level depends on each ciphertext)
+        if (!bfv || ctx.params().mul_tech != kMulHPSOverQLeveled) plan.reset(BatchPlan::mul_relin(ctx, b1, b2, rlk));
         res.default_us = time_choice(TuneChoice{});
         // 1. NTT launch form, FastBConv left at the process-wide switch
         TuneChoice best;

@@ -20,6 +20,9 @@
 // switch -> INTT, added to c0 and c1.
 #include "bfv.h"
 
+#include <algorithm>
+#include <cmath>
+
 namespace phx {
 
 namespace {
@@ -162,15 +165,16 @@ __global__ __launch_bounds__(kThreads) void k_hps_extend(const u64 *__restrict__
     hps_extend(in + blockIdx.y * in_stride + k, out + blockIdx.y * out_stride + k, H, N);
 }
 
-// hps_overq, second operand: R part = bConv_BEHZ_var1(Q -> R) (src/rns_bconv.cu:231-245:
-// y_i = [x_i negPQHatInvModq_i]_{q_i}, z_j = [sum_i y_i q_i^-1]_{r_j}), then the Q part =
+// hps_overq, second operand: R part = bConv_BEHZ_var1(S -> R) (src/rns_bconv.cu:231-245:
+// y_i = [x_i negPQHatInvModq_i]_{s_i}, z_j = [sum_i y_i s_i^-1]_{r_j}) from the ks source
+// moduli (Q at the top level, all of Q for hps_overq_leveled below it), then the Q part =
 // bConv_HPS(R -> Q) of z (src/evaluate.cu:745-751). One coefficient.
 __device__ __forceinline__ void hps_overq_switch(const u64 *__restrict__ x, u64 *__restrict__ o, const HpsConsts &H,
                                                  u32 N) {
     u64 y[kMaxQ];
 #pragma unroll
     for (u32 i = 0; i < kMaxQ; ++i)
-        if (i < H.k) y[i] = mul_mod(x[(u64)i * N], H.negrq[i], H.mod_qr[i]);
+        if (i < H.ks) y[i] = mul_mod(x[(u64)i * N], H.negrq[i], H.mod_src[i]);
     u64 w[kMaxBsk];
     double frac = 0.0;
 #pragma unroll
@@ -180,7 +184,7 @@ __device__ __forceinline__ void hps_overq_switch(const u64 *__restrict__ x, u64 
             u128 acc = 0;
 #pragma unroll
             for (u32 i = 0; i < kMaxQ; ++i)
-                if (i < H.k) acc += (u128)y[i] * H.qinv_r[(u64)j * H.k + i];
+                if (i < H.ks) acc += (u128)y[i] * H.qinv_r[(u64)j * H.ks + i];
             const u64 z = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
             o[(u64)(H.k + j) * N] = z;
             w[j] = mul_mod(z, H.rhatinv_r[j], m);
@@ -348,7 +352,8 @@ void behz_mul_1(const Context &ctx, const BehzConsts &B, const Ciphertext &ct, u
 }  // namespace
 
 void bfv_multiply(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, hipStream_t s) {
-    if (ctx.params().mul_tech == kMulHPS || ctx.params().mul_tech == kMulHPSOverQ)
+    if (ctx.params().mul_tech == kMulHPS || ctx.params().mul_tech == kMulHPSOverQ ||
+        ctx.params().mul_tech == kMulHPSOverQLeveled)
         return bfv_multiply_hps(ctx, ct1, ct2, s);
     const BehzConsts &B = behz_of(ctx);
     require(ct1.chain_index == ct2.chain_index, "encrypted1 and encrypted2 parameter mismatch");
@@ -386,9 +391,103 @@ void bfv_multiply(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, hi
     dev_free(b1, s);
 }
 
-// bfv_multiply_hps (src/evaluate.cu:648-801), mul_tech hps
-void bfv_multiply_hps(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, hipStream_t s) {
-    const HpsConsts *Hp = ctx.hps();
+// ---- hps_overq_leveled (src/evaluate.cu:549-646, 680-800, 845-1020;
+// src/eval_key_switch.cu:110-180; src/rns.cu:1799-1846) ----
+
+// FindLevelsToDrop (src/evaluate.cu:549-646): the noise estimate of a BFV
+// ciphertext of multiplicative depth `depth` (sigma = 3.2f, assurance 36,
+// ternary key bound 1, include/host/hestdparms.h:152-153) in natural logs, the
+// spare bits above it, divided by the bit size of the largest data prime.
+u32 hps_levels_to_drop(const Context &ctx, u64 depth, bool key_switch, bool asymmetric) {
+    require(ctx.params().mul_tech == kMulHPSOverQLeveled, "FindLevelsToDrop is only used in HPS over Q Leveled");
+    const double sigma = (double)3.2f, alpha = 36.0;
+    const double p = (double)ctx.params().plain_modulus;
+    const u32 k = ctx.size_P();
+    const u32 numPartQ = ctx.level(1).beta;
+    const double Bkey = 1.0;
+    const u32 n = (u32)ctx.N();
+    u64 qmax = 0;
+    for (u32 i = 0; i < ctx.size_Q(); ++i) qmax = std::max(qmax, ctx.moduli()[i]);
+    const double dcrtBits = (double)(64 - __builtin_clzll(qmax));
+    const double Berr = sigma * std::sqrt(alpha);
+    auto delta = [](u32 m) -> double { return 2. * std::sqrt(m); };
+    auto Vnorm = [&](u32 m) -> double {
+        if (asymmetric) return (1. + delta(m) * Bkey) / 2.;
+        return Berr * (1. + 2. * delta(m) * Bkey);
+    };
+    auto noiseKS = [&](u32 m) -> double { return k * (numPartQ * delta(m) * Berr + delta(m) * Bkey + 1.0) / 2; };
+    auto C1 = [&](u32 m) -> double { return delta(m) * delta(m) * p * Bkey; };
+    auto C2 = [&](u32 m) -> double { return delta(m) * delta(m) * Bkey * Bkey / 2.0 + noiseKS(m); };
+    auto logqBFV = [&](u32 m) -> double {
+        if (depth > 0) return std::log(4 * p) + (double)(depth - 1) * std::log(C1(m)) +
+                              std::log(C1(m) * Vnorm(m) + (double)depth * C2(m));
+        return std::log(p * (4 * (Vnorm(m))));
+    };
+    double logqPrev = 6. * std::log(10);
+    double logq = logqBFV(n);
+    while (std::fabs(logq - logqPrev) > std::log(1.001)) {
+        logqPrev = logq;
+        logq = logqBFV(n);
+    }
+    const double loge = logq / std::log(2) - 2 - std::log2(p);
+    const double logExtra = key_switch ? std::log2(noiseKS(n)) : std::log2(delta(n));
+    int32_t levels = (int32_t)std::floor((loge - (double)(2 * depth) - 16 - logExtra) / dcrtBits);
+    const int32_t sizeQ = (int32_t)ctx.size_Q();
+    if (levels < 0)
+        levels = 0;
+    else if (levels > sizeQ - 1)
+        levels = sizeQ - 1;
+    return (u32)levels;
+}
+
+u32 bfv_keyswitch_drop(const Context &ctx, const Ciphertext &ct, bool key_switch) {
+    if (ctx.params().scheme != kSchemeBFV || ctx.params().mul_tech != kMulHPSOverQLeveled) return 0;
+    return hps_levels_to_drop(ctx, (u64)ct.noise_scale_deg - 1, key_switch, ct.is_asymmetric);
+}
+
+// ExpandCRTBasis_Ql_Q (src/rns.cu:1808-1834): out_i = [x_i (Q/Ql)]_{q_i} for i < Ql and 0
+// for the dropped limbs; with add (ExpandCRTBasis_Ql_Q_add_to_ct, 1836-1846)
+// out_i += [x_i (Q/Ql)]_{q_i} for i < Ql, the dropped limbs unchanged
+__global__ __launch_bounds__(kThreads) void k_hps_expand(const u64 *__restrict__ in, u64 *__restrict__ out,
+                                                         const u64 *__restrict__ ex, const DevModulus *__restrict__ mod,
+                                                         u32 N, u32 Ql, u32 k0, u64 in_stride, u64 out_stride,
+                                                         int add) {
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u32 limb = blockIdx.y % k0, poly = blockIdx.y / k0;
+    u64 *o = out + poly * out_stride + (u64)limb * N + k;
+    if (limb < Ql) {
+        const u64 q = mod[limb].value;
+        const u64 v = mul_shoup(in[poly * in_stride + (u64)limb * N + k], ex[2 * limb], ex[2 * limb + 1], q);
+        *o = add ? add_mod(*o, v, q) : v;
+    } else if (!add) {
+        *o = 0;
+    }
+}
+
+void hps_scale_q_ql(const Context &ctx, u32 d, const u64 *in, u64 in_stride, u64 *out, u64 out_stride, u32 npolys,
+                    hipStream_t s) {
+    const HpsConsts &D = ctx.hps_drop(d);
+    hipLaunchKernelGGL(k_hps_overq_round, grid2(ctx.N(), npolys), dim3(kThreads), 0, s, in, out, D, (u32)ctx.N(),
+                       in_stride, out_stride);
+    PHX_HIP_CHECK(hipGetLastError());
+}
+
+void hps_expand_ql_q(const Context &ctx, u32 d, const u64 *in, u64 in_stride, u64 *out, u64 out_stride, u32 npolys,
+                     bool add, hipStream_t s) {
+    const u32 k0 = ctx.size_Q();
+    hipLaunchKernelGGL(k_hps_expand, grid2(ctx.N(), npolys * k0), dim3(kThreads), 0, s, in, out, ctx.hps_expand(d),
+                       ctx.dev_mod(), (u32)ctx.N(), k0 - d, k0, in_stride, out_stride, (int)add);
+    PHX_HIP_CHECK(hipGetLastError());
+}
+
+namespace {
+// HPS multiply with d dropped levels (d > 0 only for hps_overq_leveled): ct1 <- ct1 (x) ct2
+// over Ql = size_Q - d and Rl; the result's polynomials sit at Ql limbs in Q-sized slots,
+// the first `expand_polys` of them expanded back to Q
+void hps_multiply_core(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, u32 d, u32 expand_polys,
+                       hipStream_t s) {
+    const HpsConsts *Hp = ctx.params().mul_tech == kMulHPSOverQLeveled ? &ctx.hps_level(d) : ctx.hps();
     require(Hp != nullptr, "the context has no HPS tool: scheme must be BFV with mul_tech hps");
     const HpsConsts &H = *Hp;
     require(H.k <= kMaxQ && H.nr <= kMaxBsk, "HPS multiply supports at most 16 data primes");
@@ -396,46 +495,107 @@ void bfv_multiply_hps(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2
     require(!ct1.is_ntt_form && !ct2.is_ntt_form, "encrypted1 or encrypted2 cannot be in NTT form");
     require(ct1.size == 2 && ct2.size == 2, "dest_size must be 3 when computing BFV multiplication using HPS");
     require(ct1.chain_index == 1, "HPS multiplication runs at the top data level");
-    const u64 N = ctx.N(), QN = (u64)H.k * N, SN = (u64)(H.k + H.nr) * N;
+    const u32 k0 = ctx.size_Q(), Ql = H.k;
+    const u64 N = ctx.N(), QN0 = (u64)k0 * N, SN = (u64)(Ql + H.nr) * N;
     const bool square = &ct1 == &ct2 || ct1.data == ct2.data;
     u64 *a = static_cast<u64 *>(dev_alloc(3 * SN * 8, s));
     u64 *b = square ? a : static_cast<u64 *>(dev_alloc(2 * SN * 8, s));
-    LimbMap m = limb_map(a, 0, H.k + H.nr, SN);
-    m.split = H.k;
+    LimbMap m = limb_map(a, 0, Ql + H.nr, SN);
+    m.split = Ql;
     m.special_row = H.r_row0;
-    hipLaunchKernelGGL(k_hps_extend, grid2(N, 2), dim3(kThreads), 0, s, (const u64 *)ct1.data, a, H, (u32)N, QN, SN);
+    if (d) {
+        // scaleAndRound_HPS_Q_Ql, then bConv_HPS Ql -> Rl (src/evaluate.cu:709-715)
+        hps_scale_q_ql(ctx, d, ct1.data, QN0, a, SN, 2, s);
+        hipLaunchKernelGGL(k_hps_extend, grid2(N, 2), dim3(kThreads), 0, s, (const u64 *)a, a, H, (u32)N, SN, SN);
+    } else {
+        hipLaunchKernelGGL(k_hps_extend, grid2(N, 2), dim3(kThreads), 0, s, (const u64 *)ct1.data, a, H, (u32)N, QN0,
+                           SN);
+    }
     PHX_HIP_CHECK(hipGetLastError());
-    ntt_forward(ctx.tables(), m, H.k + H.nr, 2, s);
+    ntt_forward(ctx.tables(), m, Ql + H.nr, 2, s);
     if (!square) {
         if (H.overq)
             hipLaunchKernelGGL(k_hps_overq_switch, grid2(N, 2), dim3(kThreads), 0, s, (const u64 *)ct2.data, b, H,
-                               (u32)N, QN, SN, (const u64 *)nullptr, 2u);
+                               (u32)N, QN0, SN, (const u64 *)nullptr, 2u);
         else
             hipLaunchKernelGGL(k_hps_extend, grid2(N, 2), dim3(kThreads), 0, s, (const u64 *)ct2.data, b, H, (u32)N,
-                               QN, SN);
+                               QN0, SN);
         PHX_HIP_CHECK(hipGetLastError());
         LimbMap mb = m;
         mb.data = b;
-        ntt_forward(ctx.tables(), mb, H.k + H.nr, 2, s);
+        ntt_forward(ctx.tables(), mb, Ql + H.nr, 2, s);
     }
-    tensor_2x2_mod(H.mod_qr, N, H.k + H.nr, a, b, a, a + SN, a + 2 * SN, s);
-    ntt_inverse(ctx.tables(), m, H.k + H.nr, 3, s);
+    tensor_2x2_mod(H.mod_qr, N, Ql + H.nr, a, b, a, a + SN, a + 2 * SN, s);
+    ntt_inverse(ctx.tables(), m, Ql + H.nr, 3, s);
     ct1.resize(ctx, ct1.chain_index, 3, s);
     if (H.overq)
         hipLaunchKernelGGL(k_hps_overq_round, grid2(N, 3), dim3(kThreads), 0, s, (const u64 *)a, ct1.data, H,
-                           (u32)N, SN, QN);
+                           (u32)N, SN, QN0);
     else
         hipLaunchKernelGGL(k_hps_round_convert, grid2(N, 3), dim3(kThreads), 0, s, (const u64 *)a, ct1.data, H,
-                           (u32)N, SN, QN);
+                           (u32)N, SN, QN0);
     PHX_HIP_CHECK(hipGetLastError());
+    if (d && expand_polys) hps_expand_ql_q(ctx, d, ct1.data, QN0, ct1.data, QN0, expand_polys, false, s);
     if (!square) dev_free(b, s);
     dev_free(a, s);
+}
+
+// (c0, c1) of ct (Q limbs) += the BFV key switch of c2 (Ql = size_Q - d limbs,
+// coefficient form) at chain 1 + d, expanded back to Q when d > 0
+void bfv_keyswitch_add(const Context &ctx, Ciphertext &ct, const u64 *c2, u32 d, const KSwitchKey &key,
+                       hipStream_t s) {
+    const u32 k0 = ct.coeff_modulus_size, Ql = k0 - d;
+    const u64 N = ctx.N(), LN = (u64)Ql * N;
+    u64 *c2n = static_cast<u64 *>(dev_alloc(LN * 8, s));
+    u64 *ks = static_cast<u64 *>(dev_alloc(2 * LN * 8, s));
+    PHX_HIP_CHECK(hipMemcpyAsync(c2n, c2, LN * 8, hipMemcpyDeviceToDevice, s));
+    ntt_forward(ctx.tables(), limb_map(c2n, 0, Ql), Ql, 1, s);
+    keyswitch(ctx, ct.chain_index + d, c2n, key, ks, nullptr, nullptr, s);
+    ntt_inverse(ctx.tables(), limb_map(ks, 0, Ql, LN), Ql, 2, s);
+    if (d) {
+        hps_expand_ql_q(ctx, d, ks, LN, ct.data, (u64)k0 * N, 2, true, s);
+    } else {
+        hipLaunchKernelGGL(k_add_inplace, grid2(N, 2 * k0), dim3(kThreads), 0, s, ct.data, (const u64 *)ks,
+                           ctx.dev_mod(), (u32)N, k0);
+        PHX_HIP_CHECK(hipGetLastError());
+    }
+    dev_free(ks, s);
+    dev_free(c2n, s);
+}
+}  // namespace
+
+// bfv_multiply_hps (src/evaluate.cu:648-801): mul_tech hps, hps_overq and
+// hps_overq_leveled (levels dropped by the noise estimate of the operands'
+// degree, all three result polynomials expanded back to Q, degree + 1)
+void bfv_multiply_hps(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, hipStream_t s) {
+    const bool leveled = ctx.params().mul_tech == kMulHPSOverQLeveled;
+    const u32 deg = std::max(ct1.noise_scale_deg, ct2.noise_scale_deg);
+    const u32 d = leveled ? hps_levels_to_drop(ctx, (u64)deg - 1, false, ct1.is_asymmetric) : 0;
+    hps_multiply_core(ctx, ct1, ct2, d, 3, s);
+    if (leveled) ct1.noise_scale_deg = deg + 1;
+}
+
+// bfv_mul_relin_hps (src/evaluate.cu:819-1027) for hps_overq_leveled: c0, c1 are
+// expanded back to Q, c2 is key-switched at the dropped level and the result
+// expanded and added (for hps and hps_overq it equals multiply then relinearize)
+void bfv_mul_relin_leveled(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, const KSwitchKey &rlk,
+                           hipStream_t s) {
+    require(ctx.params().mul_tech == kMulHPSOverQLeveled, "bfv_mul_relin_leveled needs mul_tech hps_overq_leveled");
+    const u32 deg = std::max(ct1.noise_scale_deg, ct2.noise_scale_deg);
+    const u32 d = hps_levels_to_drop(ctx, (u64)deg - 1, false, ct1.is_asymmetric);
+    hps_multiply_core(ctx, ct1, ct2, d, 2, s);
+    ct1.noise_scale_deg = deg + 1;
+    bfv_keyswitch_add(ctx, ct1, ct1.data + 2 * (u64)ct1.coeff_modulus_size * ctx.N(), d, rlk, s);
+    ct1.size = 2;
 }
 
 // ---- batched form: B items staged contiguously ----
 
 size_t bfv_batch_work_words(const Context &ctx) {
     const u64 N = ctx.N();
+    if (ctx.params().mul_tech == kMulHPSOverQLeveled)
+        fail(kInvalidArgument, "hps_overq_leveled is not supported in batch plans: its level choice depends on each "
+                               "ciphertext's noise degree");
     if (ctx.params().mul_tech == kMulHPS || ctx.params().mul_tech == kMulHPSOverQ) {
         const HpsConsts *H = ctx.hps();
         require(H != nullptr, "the context has no HPS tool: scheme must be BFV with mul_tech hps");
@@ -527,17 +687,16 @@ void bfv_relinearize(const Context &ctx, Ciphertext &ct, const KSwitchKey &rlk, 
     require(!ct.is_ntt_form, "BFV encrypted cannot be in NTT form");
     const u32 L = ct.coeff_modulus_size;
     const u64 N = ctx.N(), LN = (u64)L * N;
-    u64 *c2 = static_cast<u64 *>(dev_alloc(LN * 8, s));
-    u64 *ks = static_cast<u64 *>(dev_alloc(2 * LN * 8, s));
-    PHX_HIP_CHECK(hipMemcpyAsync(c2, ct.data + 2 * LN, LN * 8, hipMemcpyDeviceToDevice, s));
-    ntt_forward(ctx.tables(), limb_map(c2, 0, L), L, 1, s);
-    keyswitch(ctx, ct.chain_index, c2, rlk, ks, nullptr, nullptr, s);
-    ntt_inverse(ctx.tables(), limb_map(ks, 0, L, LN), L, 2, s);
-    hipLaunchKernelGGL(k_add_inplace, grid2(N, 2 * L), dim3(kThreads), 0, s, ct.data, (const u64 *)ks, ctx.dev_mod(),
-                       (u32)N, L);
-    PHX_HIP_CHECK(hipGetLastError());
-    dev_free(ks, s);
-    dev_free(c2, s);
+    // hps_overq_leveled: c2 scaled down to the dropped level first (src/eval_key_switch.cu:110-150)
+    const u32 d = bfv_keyswitch_drop(ctx, ct, false);
+    if (d) {
+        u64 *c2 = static_cast<u64 *>(dev_alloc((u64)(L - d) * N * 8, s));
+        hps_scale_q_ql(ctx, d, ct.data + 2 * LN, LN, c2, (u64)(L - d) * N, 1, s);
+        bfv_keyswitch_add(ctx, ct, c2, d, rlk, s);
+        dev_free(c2, s);
+    } else {
+        bfv_keyswitch_add(ctx, ct, ct.data + 2 * LN, 0, rlk, s);
+    }
     ct.size = 2;
 }
 

@@ -682,7 +682,8 @@ int phx_multiply_inplace(const phx_context *ctx, phx_ciphertext *ct1, const phx_
         if (ctx->ctx->params().scheme == kSchemeBFV) {
             // bfv_multiply (src/evaluate.cu:1046-1048), BEHZ
             require(ctx->ctx->params().mul_tech == kMulBEHZ || ctx->ctx->params().mul_tech == kMulHPS ||
-                        ctx->ctx->params().mul_tech == kMulHPSOverQ,
+                        ctx->ctx->params().mul_tech == kMulHPSOverQ ||
+                        ctx->ctx->params().mul_tech == kMulHPSOverQLeveled,
                     "mul_tech not supported for bfv_multiply");
             bfv_multiply(*ctx->ctx, a, b, S(s));
             return;

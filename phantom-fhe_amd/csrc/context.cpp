@@ -44,6 +44,180 @@ struct ConstBuilder {
         return off;
     }
 };
+// offsets (bytes into the constant buffer) of one HpsConsts
+struct HpsOffs {
+    size_t mod_qr, qhatinv_q, qhat_r, qinv, alphaq_r, frac, divs_mod_r, rhatinv_r, rhat_q, rinv, alphar_q;
+    size_t mod_src, negrq, qinv_r, frac_r, divs_mod_q;
+};
+
+size_t push_doubles(ConstBuilder &cb, const std::vector<double> &v) {
+    std::vector<u64> w(v.size());
+    std::memcpy(w.data(), v.data(), v.size() * 8);
+    return cb.push64(w);
+}
+
+// unsigned big integer, little-endian 64-bit words
+struct BigU {
+    std::vector<u64> w{1};
+    void mul(u64 m) {
+        u128 c = 0;
+        for (auto &x : w) {
+            const u128 v = (u128)x * m + c;
+            x = (u64)v;
+            c = v >> 64;
+        }
+        if (c) w.push_back((u64)c);
+    }
+    u64 mod(u64 m) const {
+        u128 r = 0;
+        for (size_t i = w.size(); i-- > 0;) r = ((r << 64) | w[i]) % m;
+        return (u64)r;
+    }
+    void div(u64 m) {
+        u128 r = 0;
+        for (size_t i = w.size(); i-- > 0;) {
+            const u128 cur = (r << 64) | w[i];
+            w[i] = (u64)(cur / m);
+            r = cur % m;
+        }
+    }
+};
+
+// Tables of the HPS scale-and-round from S = s[0 .. ns) to its first k moduli
+// with factor F = prod(mult) (scaleAndRound_HPS_QlRl_Ql with F = t Ql, and its
+// reuse Q -> Ql with F = Ql; src/rns.cu:833-890, 920-1001): theta_i =
+// F [(S/s_i)^-1]_{s_i}; frac[j] = [theta_{k+j}]_{s_{k+j}} / s_{k+j};
+// D[i][j] = floor(theta_{k+j} / s_{k+j}) mod s_i, D[i][ns-k] = floor(theta_i / s_i) mod s_i.
+void scale_round_tables(const std::vector<u64> &sv, u32 k, const std::vector<u64> &mult, std::vector<double> &frac,
+                        std::vector<u64> &D) {
+    const u32 ns = (u32)sv.size(), nr = ns - k;
+    frac.assign(nr, 0.0);
+    D.assign((size_t)k * (nr + 1), 0);
+    for (u32 i = 0; i < ns; ++i) {
+        BigU th;
+        for (u64 m : mult) th.mul(m);
+        th.mul(hm::invmod(hm::punctured_mod(sv.data(), ns, i, sv[i]), sv[i]));
+        if (i >= k) frac[i - k] = (double)th.mod(sv[i]) / (double)sv[i];
+        th.div(sv[i]);
+        for (u32 m = 0; m < k; ++m) {
+            if (i >= k) D[(size_t)m * (nr + 1) + (i - k)] = th.mod(sv[m]);
+            else if (i == m) D[(size_t)m * (nr + 1) + nr] = th.mod(sv[m]);
+        }
+    }
+}
+
+// HPS tables for Ql = qp[0 .. k) and Rl = rp[0 .. nr) (src/rns.cu:686-905; the
+// BaseConverter tables of src/host/rns.cu:438-497); overq adds bConv_BEHZ_var1
+// from src[0 .. ks) to Rl and the t/Rl scale-and-round QlRl -> Ql
+HpsOffs build_hps_tables(ConstBuilder &cb, const u64 *qp, u32 k, const u64 *rp, u32 nr, u64 t, bool overq,
+                         const u64 *src, u32 ks) {
+    HpsOffs ho{};
+    const u32 ns = k + nr;
+    std::vector<u64> s(qp, qp + k);
+    s.insert(s.end(), rp, rp + nr);
+    std::vector<u64> qhatinv(k), qhat_r((size_t)nr * k), alphaq_r((size_t)(k + 1) * nr), rhatinv(nr),
+        rhat_q((size_t)k * nr), alphar_q((size_t)(nr + 1) * k), divs((size_t)nr * (k + 1));
+    std::vector<double> qinv(k), rinv(nr), frac(k);
+    for (u32 i = 0; i < k; ++i) {
+        qhatinv[i] = hm::invmod(hm::punctured_mod(qp, k, i, qp[i]), qp[i]);
+        qinv[i] = 1.0 / (double)qp[i];
+        for (u32 j = 0; j < nr; ++j) qhat_r[(size_t)j * k + i] = hm::punctured_mod(qp, k, i, rp[j]);
+    }
+    for (u32 v = 0; v <= k; ++v)
+        for (u32 j = 0; j < nr; ++j) alphaq_r[(size_t)v * nr + j] = hm::mulmod(v, hm::product_mod(qp, k, rp[j]), rp[j]);
+    for (u32 j = 0; j < nr; ++j) {
+        rhatinv[j] = hm::invmod(hm::punctured_mod(rp, nr, j, rp[j]), rp[j]);
+        rinv[j] = 1.0 / (double)rp[j];
+        for (u32 i = 0; i < k; ++i) rhat_q[(size_t)i * nr + j] = hm::punctured_mod(rp, nr, j, qp[i]);
+    }
+    for (u32 v = 0; v <= nr; ++v)
+        for (u32 i = 0; i < k; ++i) alphar_q[(size_t)v * k + i] = hm::mulmod(v, hm::product_mod(rp, nr, qp[i]), qp[i]);
+    // HPS QR -> R: theta_i = t R [(S/s_i)^-1]_{s_i} (S = Q R): frac_i = [theta_i]_{q_i} / q_i,
+    // divs = floor(theta_i / s_i) mod r_j (src/rns.cu:734-781)
+    for (u32 i = 0; i < ns; ++i) {
+        BigU th;
+        th.mul(t);
+        for (u32 j = 0; j < nr; ++j) th.mul(rp[j]);
+        th.mul(hm::invmod(hm::punctured_mod(s.data(), ns, i, s[i]), s[i]));
+        if (i < k) frac[i] = (double)th.mod(s[i]) / (double)s[i];
+        th.div(s[i]);
+        for (u32 j = 0; j < nr; ++j) {
+            if (i < k) divs[(size_t)j * (k + 1) + i] = th.mod(rp[j]);
+            else if (i == k + j) divs[(size_t)j * (k + 1) + k] = th.mod(rp[j]);
+        }
+    }
+    std::vector<DevModulus> mqr(ns);
+    for (u32 i = 0; i < ns; ++i) {
+        u64 lo, hi;
+        hm::barrett_ratio(s[i], lo, hi);
+        mqr[i] = {s[i], lo, hi, 0};
+    }
+    ho.mod_qr = cb.push_struct(mqr);
+    ho.qhatinv_q = cb.push64(qhatinv);
+    ho.qhat_r = cb.push64(qhat_r);
+    ho.qinv = push_doubles(cb, qinv);
+    ho.alphaq_r = cb.push64(alphaq_r);
+    ho.frac = push_doubles(cb, frac);
+    ho.divs_mod_r = cb.push64(divs);
+    ho.rhatinv_r = cb.push64(rhatinv);
+    ho.rhat_q = cb.push64(rhat_q);
+    ho.rinv = push_doubles(cb, rinv);
+    ho.alphar_q = cb.push64(alphar_q);
+    if (overq) {
+        // bConv_BEHZ_var1 src -> Rl (src/host/rns.cu:466-497 with P = Rl):
+        // negrq_i = src_i - [Rl (S'/src_i)^-1]_{src_i} (S' = prod src), qinv_r = src_i^-1 mod r_j
+        std::vector<u64> negrq(ks), qinv_r((size_t)nr * ks);
+        std::vector<DevModulus> msrc(ks);
+        for (u32 i = 0; i < ks; ++i) {
+            const u64 sh = hm::invmod(hm::punctured_mod(src, ks, i, src[i]), src[i]);
+            negrq[i] = src[i] - hm::mulmod(hm::product_mod(rp, nr, src[i]), sh, src[i]);
+            for (u32 j = 0; j < nr; ++j) qinv_r[(size_t)j * ks + i] = hm::invmod(src[i] % rp[j], rp[j]);
+            u64 lo, hi;
+            hm::barrett_ratio(src[i], lo, hi);
+            msrc[i] = {src[i], lo, hi, 0};
+        }
+        std::vector<u64> mult{t};
+        mult.insert(mult.end(), qp, qp + k);
+        std::vector<double> frac_r;
+        std::vector<u64> dq;
+        scale_round_tables(s, k, mult, frac_r, dq);
+        ho.mod_src = cb.push_struct(msrc);
+        ho.negrq = cb.push64(negrq);
+        ho.qinv_r = cb.push64(qinv_r);
+        ho.frac_r = push_doubles(cb, frac_r);
+        ho.divs_mod_q = cb.push64(dq);
+    }
+    return ho;
+}
+
+HpsConsts resolve_hps(const uint8_t *cbb, const HpsOffs &ho, u32 k, u32 nr, bool overq, u32 ks) {
+    auto at = [&](size_t off) { return reinterpret_cast<const u64 *>(cbb + off); };
+    auto atd = [&](size_t off) { return reinterpret_cast<const double *>(cbb + off); };
+    HpsConsts h{};
+    h.k = k;
+    h.nr = nr;
+    h.mod_qr = reinterpret_cast<const DevModulus *>(cbb + ho.mod_qr);
+    h.qhatinv_q = at(ho.qhatinv_q);
+    h.qhat_r = at(ho.qhat_r);
+    h.qinv = atd(ho.qinv);
+    h.alphaq_r = at(ho.alphaq_r);
+    h.frac = atd(ho.frac);
+    h.divs_mod_r = at(ho.divs_mod_r);
+    h.rhatinv_r = at(ho.rhatinv_r);
+    h.rhat_q = at(ho.rhat_q);
+    h.rinv = atd(ho.rinv);
+    h.alphar_q = at(ho.alphar_q);
+    if (overq) {
+        h.overq = 1;
+        h.ks = ks;
+        h.mod_src = reinterpret_cast<const DevModulus *>(cbb + ho.mod_src);
+        h.negrq = at(ho.negrq);
+        h.qinv_r = at(ho.qinv_r);
+        h.frac_r = atd(ho.frac_r);
+        h.divs_mod_q = at(ho.divs_mod_q);
+    }
+    return h;
+}
 }  // namespace
 
 std::vector<uint8_t> bconv_mfma_operands(const std::vector<u64> &mat, u32 ni, u32 no, const std::vector<u64> &omod,
@@ -167,7 +341,7 @@ Context::Context(const Params &p) : params_(p) {
     // ---- HPS base R (src/rns.cu:686-699): size_Q + 1 primes below the smallest q,
     // stepping down by 2N within its bit size (get_primes_below, src/host/numth.cu:235-262);
     // size_Q primes for hps_overq (src/rns.cu:792-801)
-    const bool overq = p.mul_tech == kMulHPSOverQ;
+    const bool overq = p.mul_tech == kMulHPSOverQ || p.mul_tech == kMulHPSOverQLeveled;
     if (p.scheme == kSchemeBFV && (p.mul_tech == kMulHPS || overq)) {
         require(p.plain_modulus > 1, "plain_modulus is not set");
         const u64 qmin = *std::min_element(p.coeff_modulus.begin(), p.coeff_modulus.begin() + size_Q_);
@@ -440,143 +614,47 @@ Context::Context(const Params &p) : params_(p) {
         behz_.bsk_row0 = (u32)QP;
         behz_.t = t;
     }
-    // HPS constants
-    struct HpsOffs {
-        size_t mod_qr, qhatinv_q, qhat_r, qinv, alphaq_r, frac, divs_mod_r, rhatinv_r, rhat_q, rinv, alphar_q;
-        size_t negrq, qinv_r, frac_r, divs_mod_q;
-    } ho{};
+    // HPS constants: the top level, and for hps_overq_leveled one set per number
+    // of dropped levels d = 1 .. size_Q - 1 (Ql = size_Q - d, Rl = the first Ql
+    // primes of R; src/rns.cu:792-1004)
+    std::vector<HpsOffs> hps_offs, drop_offs;
+    std::vector<size_t> expand_offs;
     if (!hps_r_.empty()) {
-        const u32 k = size_Q_, nr = (u32)hps_r_.size(), ns = k + nr;
+        const u32 k0 = size_Q_;
         const u64 t = p.plain_modulus;
-        std::vector<u64> s(qp, qp + k);
-        s.insert(s.end(), hps_r_.begin(), hps_r_.end());
-        const u64 *rp = hps_r_.data();
-        std::vector<u64> qhatinv(k), qhat_r((size_t)nr * k), alphaq_r((size_t)(k + 1) * nr), rhatinv(nr),
-            rhat_q((size_t)k * nr), alphar_q((size_t)(nr + 1) * k), divs((size_t)nr * (k + 1));
-        std::vector<double> qinv(k), rinv(nr), frac(k);
-        for (u32 i = 0; i < k; ++i) {
-            qhatinv[i] = hm::invmod(hm::punctured_mod(qp, k, i, qp[i]), qp[i]);
-            qinv[i] = 1.0 / (double)qp[i];
-            for (u32 j = 0; j < nr; ++j) qhat_r[(size_t)j * k + i] = hm::punctured_mod(qp, k, i, rp[j]);
-        }
-        for (u32 v = 0; v <= k; ++v)
-            for (u32 j = 0; j < nr; ++j)
-                alphaq_r[(size_t)v * nr + j] = hm::mulmod(v, hm::product_mod(qp, k, rp[j]), rp[j]);
-        for (u32 j = 0; j < nr; ++j) {
-            rhatinv[j] = hm::invmod(hm::punctured_mod(rp, nr, j, rp[j]), rp[j]);
-            rinv[j] = 1.0 / (double)rp[j];
-            for (u32 i = 0; i < k; ++i) rhat_q[(size_t)i * nr + j] = hm::punctured_mod(rp, nr, j, qp[i]);
-        }
-        for (u32 v = 0; v <= nr; ++v)
-            for (u32 i = 0; i < k; ++i)
-                alphar_q[(size_t)v * k + i] = hm::mulmod(v, hm::product_mod(rp, nr, qp[i]), qp[i]);
-        // theta_i = t R [(S/s_i)^-1]_{s_i} (S = Q R): frac_i = [theta_i]_{q_i} / q_i,
-        // divs = floor(theta_i / s_i) mod r_j (src/rns.cu:734-781)
-        for (u32 i = 0; i < ns; ++i) {
-            const u64 shinv = hm::invmod(hm::punctured_mod(s.data(), ns, i, s[i]), s[i]);
-            std::vector<u64> th{t};
-            auto mul = [&](u64 m) {
-                u128 c = 0;
-                for (auto &w : th) {
-                    const u128 v = (u128)w * m + c;
-                    w = (u64)v;
-                    c = v >> 64;
+        const u32 levels = p.mul_tech == kMulHPSOverQLeveled ? k0 : 1;
+        for (u32 d = 0; d < levels; ++d) {
+            const u32 Ql = k0 - d;
+            const u32 nr = overq ? Ql : (u32)hps_r_.size();
+            // switch source of the second operand: Ql at the top level, all of Q below
+            hps_offs.push_back(build_hps_tables(cb, qp, Ql, hps_r_.data(), nr, t, overq, qp, d ? k0 : Ql));
+            if (d) {
+                std::vector<u64> sQ(qp, qp + k0), ql(qp, qp + Ql);
+                std::vector<double> frac;
+                std::vector<u64> D;
+                scale_round_tables(sQ, Ql, ql, frac, D);
+                HpsOffs o{};
+                o.frac_r = push_doubles(cb, frac);
+                o.divs_mod_q = cb.push64(D);
+                std::vector<DevModulus> mq(k0);
+                for (u32 i = 0; i < k0; ++i) {
+                    u64 lo, hi;
+                    hm::barrett_ratio(qp[i], lo, hi);
+                    mq[i] = {qp[i], lo, hi, 0};
                 }
-                if (c) th.push_back((u64)c);
-            };
-            auto mod = [&](u64 m) {
-                u128 r = 0;
-                for (size_t w = th.size(); w-- > 0;) r = ((r << 64) | th[w]) % m;
-                return (u64)r;
-            };
-            for (u32 j = 0; j < nr; ++j) mul(rp[j]);
-            mul(shinv);
-            if (i < k) frac[i] = (double)mod(s[i]) / (double)s[i];
-            u128 r = 0;
-            for (size_t w = th.size(); w-- > 0;) {
-                const u128 cur = (r << 64) | th[w];
-                th[w] = (u64)(cur / s[i]);
-                r = cur % s[i];
-            }
-            for (u32 j = 0; j < nr; ++j) {
-                if (i < k) divs[(size_t)j * (k + 1) + i] = mod(rp[j]);
-                else if (i == k + j) divs[(size_t)j * (k + 1) + k] = mod(rp[j]);
-            }
-        }
-        std::vector<DevModulus> mqr(ns);
-        for (u32 i = 0; i < ns; ++i) {
-            u64 lo, hi;
-            hm::barrett_ratio(s[i], lo, hi);
-            mqr[i] = {s[i], lo, hi, 0};
-        }
-        auto pushd = [&](const std::vector<double> &v) {
-            std::vector<u64> w(v.size());
-            std::memcpy(w.data(), v.data(), v.size() * 8);
-            return cb.push64(w);
-        };
-        ho.mod_qr = cb.push_struct(mqr);
-        ho.qhatinv_q = cb.push64(qhatinv);
-        ho.qhat_r = cb.push64(qhat_r);
-        ho.qinv = pushd(qinv);
-        ho.alphaq_r = cb.push64(alphaq_r);
-        ho.frac = pushd(frac);
-        ho.divs_mod_r = cb.push64(divs);
-        ho.rhatinv_r = cb.push64(rhatinv);
-        ho.rhat_q = cb.push64(rhat_q);
-        ho.rinv = pushd(rinv);
-        ho.alphar_q = cb.push64(alphar_q);
-        if (overq) {
-            // bConv_BEHZ_var1 Q -> R tables (src/host/rns.cu:466-497 with P = R) and the
-            // t/R scale-and-round QR -> Q: theta_i = t Q [(S/s_i)^-1]_{s_i},
-            // frac_r_j = [theta_{k+j}]_{r_j} / r_j, D_ij = floor(theta_{k+j} / r_j) mod q_i,
-            // D_ik = floor(theta_i / q_i) mod q_i (src/rns.cu:833-890)
-            std::vector<u64> negrq(k), qinv_r((size_t)nr * k), dq((size_t)k * (nr + 1));
-            std::vector<double> frac_r(nr);
-            for (u32 i = 0; i < k; ++i) {
-                const u64 rq = hm::mulmod(hm::product_mod(rp, nr, qp[i]), qhatinv[i], qp[i]);
-                negrq[i] = qp[i] - rq;
-                for (u32 j = 0; j < nr; ++j) qinv_r[(size_t)j * k + i] = hm::invmod(qp[i] % rp[j], rp[j]);
-            }
-            for (u32 i = 0; i < ns; ++i) {
-                const u64 shinv = hm::invmod(hm::punctured_mod(s.data(), ns, i, s[i]), s[i]);
-                std::vector<u64> th{t};
-                auto mul = [&](u64 m) {
-                    u128 c = 0;
-                    for (auto &w : th) {
-                        const u128 v = (u128)w * m + c;
-                        w = (u64)v;
-                        c = v >> 64;
-                    }
-                    if (c) th.push_back((u64)c);
-                };
-                auto mod = [&](u64 m) {
-                    u128 r = 0;
-                    for (size_t w = th.size(); w-- > 0;) r = ((r << 64) | th[w]) % m;
-                    return (u64)r;
-                };
-                for (u32 m = 0; m < k; ++m) mul(qp[m]);
-                mul(shinv);
-                if (i >= k) frac_r[i - k] = (double)mod(s[i]) / (double)s[i];
-                u128 r = 0;
-                for (size_t w = th.size(); w-- > 0;) {
-                    const u128 cur = (r << 64) | th[w];
-                    th[w] = (u64)(cur / s[i]);
-                    r = cur % s[i];
+                o.mod_qr = cb.push_struct(mq);
+                drop_offs.push_back(o);
+                std::vector<u64> ex(2 * (size_t)Ql);
+                for (u32 i = 0; i < Ql; ++i) {
+                    ex[2 * i] = hm::product_mod(qp + Ql, d, qp[i]);
+                    ex[2 * i + 1] = hm::shoup(ex[2 * i], qp[i]);
                 }
-                for (u32 m = 0; m < k; ++m) {
-                    if (i >= k) dq[(size_t)m * (nr + 1) + (i - k)] = mod(qp[m]);
-                    else if (i == m) dq[(size_t)m * (nr + 1) + nr] = mod(qp[m]);
-                }
+                expand_offs.push_back(cb.push64(ex));
+            } else {
+                drop_offs.push_back(HpsOffs{});
+                expand_offs.push_back(0);
             }
-            ho.negrq = cb.push64(negrq);
-            ho.qinv_r = cb.push64(qinv_r);
-            ho.frac_r = pushd(frac_r);
-            ho.divs_mod_q = cb.push64(dq);
-            hps_.overq = 1;
         }
-        hps_.k = k;
-        hps_.nr = nr;
-        hps_.r_row0 = (u32)(QP + bsk_.size());
     }
     cb.align(256);
     PHX_HIP_CHECK(hipMalloc(&cbuf_, cb.buf.size()));
@@ -592,26 +670,23 @@ Context::Context(const Params &p) : params_(p) {
         L.rescale_qinv = reinterpret_cast<const u64 *>(cbb + offs[c].rescale);
     }
     bigP_mod_q_ = reinterpret_cast<u64 *>(cbb + bp_off);
-    if (hps_.k) {
-        auto at = [&](size_t off) { return reinterpret_cast<const u64 *>(cbb + off); };
-        auto atd = [&](size_t off) { return reinterpret_cast<const double *>(cbb + off); };
-        hps_.mod_qr = reinterpret_cast<const DevModulus *>(cbb + ho.mod_qr);
-        hps_.qhatinv_q = at(ho.qhatinv_q);
-        hps_.qhat_r = at(ho.qhat_r);
-        hps_.qinv = atd(ho.qinv);
-        hps_.alphaq_r = at(ho.alphaq_r);
-        hps_.frac = atd(ho.frac);
-        hps_.divs_mod_r = at(ho.divs_mod_r);
-        hps_.rhatinv_r = at(ho.rhatinv_r);
-        hps_.rhat_q = at(ho.rhat_q);
-        hps_.rinv = atd(ho.rinv);
-        hps_.alphar_q = at(ho.alphar_q);
-        if (hps_.overq) {
-            hps_.negrq = at(ho.negrq);
-            hps_.qinv_r = at(ho.qinv_r);
-            hps_.frac_r = atd(ho.frac_r);
-            hps_.divs_mod_q = at(ho.divs_mod_q);
+    for (size_t d = 0; d < hps_offs.size(); ++d) {
+        const u32 k0 = size_Q_, Ql = k0 - (u32)d;
+        HpsConsts h = resolve_hps(cbb, hps_offs[d], Ql, overq ? Ql : (u32)hps_r_.size(), overq, d ? k0 : Ql);
+        h.r_row0 = (u32)(QP + bsk_.size());
+        if (d == 0) hps_ = h;
+        hps_lv_.push_back(h);
+        HpsConsts dr{};
+        if (d) {
+            dr.k = Ql;
+            dr.nr = (u32)d;
+            dr.mod_qr = reinterpret_cast<const DevModulus *>(cbb + drop_offs[d].mod_qr);
+            dr.frac_r = reinterpret_cast<const double *>(cbb + drop_offs[d].frac_r);
+            dr.divs_mod_q = reinterpret_cast<const u64 *>(cbb + drop_offs[d].divs_mod_q);
+            dr.overq = 1;
         }
+        hps_drop_.push_back(dr);
+        hps_expand_.push_back(d ? reinterpret_cast<const u64 *>(cbb + expand_offs[d]) : nullptr);
     }
     if (behz_.k) {
         auto at = [&](size_t off) { return reinterpret_cast<const u64 *>(cbb + off); };
@@ -655,6 +730,21 @@ void Context::set_tuning(const TuneChoice &t) {
     tables_.fused_g = t.ntt_fused;
     tables_.fused_min = t.ntt_fused_min_limbs;
     tables_.chunk_limbs = t.ntt_chunk_limbs;
+}
+
+const HpsConsts &Context::hps_level(u32 d) const {
+    require(d < hps_lv_.size(), "no HPS tables for this number of dropped levels");
+    return hps_lv_[d];
+}
+
+const HpsConsts &Context::hps_drop(u32 d) const {
+    require(d >= 1 && d < hps_drop_.size(), "no Q -> Ql tables for this number of dropped levels");
+    return hps_drop_[d];
+}
+
+const u64 *Context::hps_expand(u32 d) const {
+    require(d >= 1 && d < hps_expand_.size(), "no expansion factors for this number of dropped levels");
+    return hps_expand_[d];
 }
 
 const LevelConsts &Context::level(u32 chain_index) const {

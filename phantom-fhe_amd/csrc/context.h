@@ -16,7 +16,7 @@
 namespace phx {
 
 enum SchemeType : int { kSchemeNone = 0, kSchemeBGV = 1, kSchemeBFV = 2, kSchemeCKKS = 3 };
-enum MulTech : int { kMulNone = 0, kMulBEHZ = 1, kMulHPS = 2, kMulHPSOverQ = 3 };
+enum MulTech : int { kMulNone = 0, kMulBEHZ = 1, kMulHPS = 2, kMulHPSOverQ = 3, kMulHPSOverQLeveled = 4 };
 
 struct Params {
     int scheme = kSchemeCKKS;
@@ -113,11 +113,13 @@ struct HpsConsts {
     const u64 *rhat_q;          // [k][nr]    Rhat_j mod q_i
     const double *rinv;         // [nr]       1 / r_j
     const u64 *alphar_q;        // [nr+1][k]  v R mod q_i
-    // over Q (mul_tech hps_overq; nr = k): switch Q -> R, t/R scale-and-round QR -> Q
+    // over Q (mul_tech hps_overq / hps_overq_leveled; nr = k): switch of the
+    // second operand from its ks source moduli (Q) to R, t/R scale-and-round QR -> Q
     u32 overq = 0;
-    u32 pad2 = 0;
-    const u64 *negrq;           // [k]        q_i - [R Qhat_i^-1]_{q_i}
-    const u64 *qinv_r;          // [nr][k]    q_i^-1 mod r_j
+    u32 ks = 0;                 // source moduli of the switch (k, or all of Q below the top level)
+    const DevModulus *mod_src;  // [ks]
+    const u64 *negrq;           // [ks]       s_i - [R Shat_i^-1]_{s_i}
+    const u64 *qinv_r;          // [nr][ks]   s_i^-1 mod r_j
     const double *frac_r;       // [nr]       tQlSlHatInvModsDivsFrac
     const u64 *divs_mod_q;      // [k][nr+1]  tQlSlHatInvModsDivsModq
 };
@@ -163,6 +165,13 @@ public:
     // BFV with HPS multiplication: its constants (else null)
     const HpsConsts *hps() const { return hps_.k ? &hps_ : nullptr; }
     const std::vector<u64> &hps_r_primes() const { return hps_r_; }
+    // hps_overq_leveled with d dropped levels (d = 0 is hps()): the tables for
+    // Ql = size_Q - d and Rl = the first Ql primes of R; the Q -> Ql
+    // scale-and-round (k = Ql, nr = d, the dropped moduli as the "R" part); the
+    // expansion factors [Ql][2] = (Q/Ql mod q_i, Shoup)
+    const HpsConsts &hps_level(u32 d) const;
+    const HpsConsts &hps_drop(u32 d) const;
+    const u64 *hps_expand(u32 d) const;
 
     // Galois permutation tables (NTT domain), created on first use
     const u32 *galois_perm(u32 elt) const;
@@ -188,6 +197,8 @@ private:
     std::vector<u64> bsk_;
     HpsConsts hps_{};
     std::vector<u64> hps_r_;
+    std::vector<HpsConsts> hps_lv_, hps_drop_;
+    std::vector<const u64 *> hps_expand_;
 };
 
 // stream-ordered device allocation (pooled, like the reference's cuda_auto_ptr)

@@ -632,8 +632,9 @@ void multiply_and_relin(const Context &ctx, Ciphertext &ct1, const Ciphertext &c
         // hps_overq the reference's fused bfv_mul_relin_hps (818-1027) runs the same
         // two steps
         require(ctx.params().mul_tech == kMulBEHZ || ctx.params().mul_tech == kMulHPS ||
-                    ctx.params().mul_tech == kMulHPSOverQ,
+                    ctx.params().mul_tech == kMulHPSOverQ || ctx.params().mul_tech == kMulHPSOverQLeveled,
                 "unsupported mul tech in BFV mul&relin");
+        if (ctx.params().mul_tech == kMulHPSOverQLeveled) return bfv_mul_relin_leveled(ctx, ct1, ct2, rlk, s);
         bfv_multiply(ctx, ct1, ct2, s);
         bfv_relinearize(ctx, ct1, rlk, s);
         return;
@@ -691,8 +692,9 @@ void rescale_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, 
 // hoisting_inplace for CKKS (src/evaluate.cu:1670-1865): ct <- sum_i rot(ct, steps[i])
 // with one modup of c1; each step's permutation is applied to c0 and, inside
 // the inner product, to the modup'ed digits; one moddown of the accumulated cx.
-void hoisting(const Context &ctx, Ciphertext &ct, const GaloisKeys &gk, const std::vector<int> &steps,
-              hipStream_t s) {
+namespace {
+void hoisting_at(const Context &ctx, Ciphertext &ct, const GaloisKeys &gk, const std::vector<int> &steps,
+                 hipStream_t s) {
     const bool bfv = ctx.params().scheme == kSchemeBFV;
     require(bfv || ctx.params().scheme == kSchemeCKKS, "hoisting: unsupported scheme");
     require(ct.size == 2, "ciphertext size must be 2");
@@ -737,6 +739,29 @@ void hoisting(const Context &ctx, Ciphertext &ct, const GaloisKeys &gk, const st
     dev_free(cx, s);
     dev_free(t, s);
 }
+}  // namespace
+
+void hoisting(const Context &ctx, Ciphertext &ct, const GaloisKeys &gk, const std::vector<int> &steps,
+              hipStream_t s) {
+    // hps_overq_leveled (src/evaluate.cu:1688-1865): c0 and c1 scaled down to the
+    // dropped level, hoisted there, expanded back to Q
+    const u32 d = ct.size == 2 && !ct.is_ntt_form ? bfv_keyswitch_drop(ctx, ct, true) : 0;
+    if (!d) return hoisting_at(ctx, ct, gk, steps, s);
+    const u64 N = ctx.N(), QN0 = (u64)ct.coeff_modulus_size * N;
+    Ciphertext t;
+    t.resize(ctx, ct.chain_index + d, 2, s);
+    t.is_ntt_form = false;
+    const u64 LN = (u64)t.coeff_modulus_size * N;
+    try {
+        hps_scale_q_ql(ctx, d, ct.data, QN0, t.data, LN, 2, s);
+        hoisting_at(ctx, t, gk, steps, s);
+        hps_expand_ql_q(ctx, d, t.data, LN, ct.data, QN0, 2, false, s);
+    } catch (...) {
+        t.release(s);
+        throw;
+    }
+    t.release(s);
+}
 
 // apply_galois_inplace (src/evaluate.cu:1567-1630)
 void apply_galois(const Context &ctx, Ciphertext &ct, u32 elt, const GaloisKeys &gk, hipStream_t s) {
@@ -752,6 +777,33 @@ void apply_galois(const Context &ctx, Ciphertext &ct, u32 elt, const GaloisKeys 
     const u32 Ql = ct.coeff_modulus_size;
     const u64 N = ctx.N(), LN = (u64)Ql * N;
     const u32 *perm = ctx.galois_perm(elt);
+    if (const u32 d = bfv ? bfv_keyswitch_drop(ctx, ct, true) : 0) {
+        // hps_overq_leveled (src/evaluate.cu:1567-1630 with src/eval_key_switch.cu:110-180):
+        // c0 <- pi(c0) + expand(ks0), c1 <- expand(ks1), ks = the key switch of pi(c1)
+        // scaled down to the dropped level; pi in the coefficient domain is the NTT-domain
+        // permutation conjugated by the exact transform
+        const u32 Qd = Ql - d;
+        const u64 DN = (u64)Qd * N;
+        u64 *tmp = static_cast<u64 *>(dev_alloc(2 * LN * 8, s));
+        u64 *c2 = static_cast<u64 *>(dev_alloc(DN * 8, s));
+        u64 *ks = static_cast<u64 *>(dev_alloc(2 * DN * 8, s));
+        ntt_forward(ctx.tables(), limb_map(ct.data, 0, Ql, LN), Ql, 2, s);
+        hipLaunchKernelGGL(k_permute, grid_for(N, 2 * Ql), dim3(kThreads), 0, s, (const u64 *)ct.data, tmp, perm,
+                           (u32)N);
+        PHX_HIP_CHECK(hipGetLastError());
+        ntt_inverse(ctx.tables(), limb_map(tmp, 0, Ql, LN), Ql, 2, s);
+        hps_scale_q_ql(ctx, d, tmp + LN, LN, c2, DN, 1, s);
+        ntt_forward(ctx.tables(), limb_map(c2, 0, Qd), Qd, 1, s);
+        keyswitch(ctx, ct.chain_index + d, c2, key, ks, nullptr, nullptr, s);
+        ntt_inverse(ctx.tables(), limb_map(ks, 0, Qd, DN), Qd, 2, s);
+        PHX_HIP_CHECK(hipMemcpyAsync(ct.data, tmp, LN * 8, hipMemcpyDeviceToDevice, s));
+        hps_expand_ql_q(ctx, d, ks, DN, ct.data, LN, 1, true, s);
+        hps_expand_ql_q(ctx, d, ks + DN, DN, ct.data + LN, LN, 1, false, s);
+        dev_free(ks, s);
+        dev_free(c2, s);
+        dev_free(tmp, s);
+        return;
+    }
     // BFV (coefficient form): the reference's coefficient-domain automorphism and
     // its BFV key switch equal the NTT-domain permutation and key switch
     // conjugated by the exact transform, so the ciphertext is transformed,

@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libphantom_amd.so")
 
 SCHEME_BGV, SCHEME_BFV, SCHEME_CKKS = 1, 2, 3
-MUL_TECH_NONE, MUL_TECH_BEHZ, MUL_TECH_HPS, MUL_TECH_HPS_OVERQ = 0, 1, 2, 3
+MUL_TECH_NONE, MUL_TECH_BEHZ, MUL_TECH_HPS, MUL_TECH_HPS_OVERQ, MUL_TECH_HPS_OVERQ_LEVELED = 0, 1, 2, 3, 4
 
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _u32p = ctypes.POINTER(ctypes.c_uint32)

@@ -76,6 +76,15 @@ def lib():
         L.orc_hps_create2.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int]
         L.orc_hps_modswitch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p, _u64p]
         L.orc_bfv_multiply_hps2.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p, _u64p, ctypes.c_int, _u64p]
+        L.orc_hps_create_level.restype = ctypes.c_void_p
+        L.orc_hps_create_level.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_size_t]
+        L.orc_hps_levels_to_drop.restype = ctypes.c_size_t
+        L.orc_hps_levels_to_drop.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int,
+                                             ctypes.c_int]
+        L.orc_hps_scale_q_ql.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u64p, _u64p]
+        L.orc_hps_expand.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u64p, _u64p]
+        L.orc_bfv_multiply_hps_level.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _u64p, _u64p,
+                                                 ctypes.c_int, ctypes.c_size_t, _u64p]
         L.orc_hps_destroy.argtypes = [ctypes.c_void_p]
         L.orc_salsa_quarterround.argtypes = [ctypes.POINTER(ctypes.c_uint32)]
         L.orc_salsa_hash.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
@@ -322,6 +331,55 @@ class Hps:
         out = np.zeros((len(self.r()), self.octx.N), np.uint64)
         lib().orc_hps_modswitch(self.octx._h, self._h, _p(np.ascontiguousarray(x, np.uint64)), _p(out))
         return out
+
+
+class HpsLevel(Hps):
+    """hps_overq_leveled tool of the level with d dropped data primes."""
+
+    def __init__(self, octx, t, d):
+        self.octx = octx
+        self.t = t
+        self.overq = True
+        self.d = d
+        self._h = lib().orc_hps_create_level(octx._h, t, d)
+        assert self._h
+
+    def multiply(self, ct1, ct2, square=False, expand_polys=3):
+        """[3][size_Q][N]; limbs >= Ql of an unexpanded polynomial are 0"""
+        k = self.octx.size_Q
+        out = np.zeros((3, k, self.octx.N), np.uint64)
+        rc = lib().orc_bfv_multiply_hps_level(self.octx._h, self._h, self.d, _p(np.ascontiguousarray(ct1, np.uint64)),
+                                              _p(np.ascontiguousarray(ct2, np.uint64)), 1 if square else 0,
+                                              expand_polys, _p(out))
+        assert rc == 0
+        return out
+
+    def modswitch(self, x):
+        """bConv_BEHZ_var1 from all of Q to Rl of x [size_Q][N]"""
+        out = np.zeros((len(self.r()), self.octx.N), np.uint64)
+        lib().orc_hps_modswitch(self.octx._h, self._h, _p(np.ascontiguousarray(x, np.uint64)), _p(out))
+        return out
+
+
+def hps_levels_to_drop(octx, t, depth, key_switch=False, asymmetric=False):
+    """FindLevelsToDrop"""
+    return int(lib().orc_hps_levels_to_drop(octx._h, t, depth, int(key_switch), int(asymmetric)))
+
+
+def hps_scale_q_ql(octx, d, x):
+    """scaleAndRound_HPS_Q_Ql: [size_Q][N] -> [size_Q - d][N]"""
+    out = np.zeros((octx.size_Q - d, octx.N), np.uint64)
+    lib().orc_hps_scale_q_ql(octx._h, d, _p(np.ascontiguousarray(x, np.uint64)), _p(out))
+    return out
+
+
+def hps_expand(octx, d, x):
+    """ExpandCRTBasis_Ql_Q: [size_Q - d][N] -> [size_Q][N]"""
+    out = np.zeros((octx.size_Q, octx.N), np.uint64)
+    xx = np.zeros((octx.size_Q, octx.N), np.uint64)
+    xx[:octx.size_Q - d] = x
+    lib().orc_hps_expand(octx._h, d, _p(xx), _p(out))
+    return out
 
 
 def salsa_quarterround(y):

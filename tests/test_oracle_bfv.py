@@ -328,3 +328,145 @@ def test_hps_overq_differs_from_hps():
     a = ob.Hps(octx, T).multiply(ct1, ct2)
     b = ob.Hps(octx, T, overq=True).multiply(ct1, ct2)
     assert not np.array_equal(a, b)
+
+
+# ---- hps_overq_leveled ----
+
+def _prod(xs):
+    v = 1
+    for x in xs:
+        v *= x
+    return v
+
+
+def test_hps_level_r_is_prefix():
+    """the R of the level with d dropped primes is the first Ql primes of the top level's R"""
+    octx, mods, N = setup(5, [30, 30, 30, 30, 31], 1)
+    top = ob.Hps(octx, T, overq=True).r()
+    for d in range(1, 4):
+        assert ob.HpsLevel(octx, T, d).r() == top[:4 - d]
+
+
+@pytest.mark.parametrize("logN,bits", [(5, [30, 30, 30, 30, 31]), (4, [54, 54, 54, 54, 60])])
+def test_hps_scale_q_ql_closed_form(logN, bits):
+    """scaleAndRound_HPS_Q_Ql: round(Ql X / Q) mod q_i (i < Ql) for X = CRT_Q(x), within
+    the FP64 bound of the nu sum"""
+    octx, mods, N = setup(logN, bits, 1)
+    k0 = len(mods) - 1
+    qs = mods[:k0]
+    Q = _prod(qs)
+    rng = np.random.default_rng(logN + 11)
+    x = np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs])
+    x[:, 0] = 0
+    x[:, 1] = [q - 1 for q in qs]
+    qbits = max(q.bit_length() for q in qs)
+    for d in range(1, k0):
+        Ql = _prod(qs[:k0 - d])
+        out = ob.hps_scale_q_ql(octx, d, x)
+        err = d * d * 2.0 ** (qbits - 53)
+        bound = 0 if err < 2.0 ** -10 else int(err) + 1
+        for n in range(N):
+            X = crt([int(v) for v in x[:, n]], qs)[0]
+            w = (2 * Ql * X + Q) // (2 * Q)
+            got = crt([int(out[i, n]) for i in range(k0 - d)], qs[:k0 - d])[0]
+            diff = _centred((got - w) % Ql, Ql)
+            assert abs(diff) <= bound, (d, n, diff)
+        assert crt([int(v) for v in out[:, 0]], qs[:k0 - d])[0] == 0
+
+
+def test_hps_expand_closed_form():
+    """ExpandCRTBasis_Ql_Q: CRT_Q(out) = CRT_Ql(x) (Q / Ql), the dropped limbs zero"""
+    octx, mods, N = setup(4, [30, 30, 30, 30, 31], 1)
+    qs = mods[:4]
+    Q = _prod(qs)
+    rng = np.random.default_rng(5)
+    for d in (1, 2, 3):
+        x = np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs[:4 - d]])
+        out = ob.hps_expand(octx, d, x)
+        assert not out[4 - d:].any()
+        for n in range(N):
+            X = crt([int(v) for v in x[:, n]], qs[:4 - d])[0]
+            assert crt([int(v) for v in out[:, n]], qs)[0] == X * (Q // _prod(qs[:4 - d])) % Q
+
+
+def test_hps_level_modswitch_from_all_of_q():
+    """below the top level the second operand is switched from all of Q to Rl:
+    Z = (Rl X + u) / Q with 0 <= u < size_Q Q"""
+    octx, mods, N = setup(5, [30, 30, 30, 30, 31], 1)
+    qs = mods[:4]
+    Q = _prod(qs)
+    rng = np.random.default_rng(9)
+    x = np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs])
+    for d in (1, 2):
+        h = ob.HpsLevel(octx, T, d)
+        rs = h.r()
+        R = _prod(rs)
+        z = h.modswitch(x)
+        for n in range(N):
+            X = crt([int(v) for v in x[:, n]], qs)[0]
+            Z = crt([int(v) for v in z[:, n]], rs)[0]
+            dd = Z * Q - R * X
+            if dd < 0:
+                dd += R * Q
+            assert 0 <= dd < 4 * Q, (d, n, dd)
+
+
+@pytest.mark.parametrize("square", [False, True])
+def test_hps_level_multiply_matches_integer_round(square):
+    """with d dropped levels: X = the centred round(Ql x1 / Q) (or, squaring, of ct1 twice),
+    Y = the centred switch of ct2 to Rl; the result's first Ql limbs are
+    round(t z / Rl) (Q / Ql) mod q_i, the dropped limbs zero"""
+    octx, mods, N = setup(5, [30, 30, 30, 30, 31], 1)
+    k0 = 4
+    qs = mods[:k0]
+    Q = _prod(qs)
+    rng = np.random.default_rng(17 + square)
+    ct1 = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs]) for _ in range(2)])
+    ct2 = ct1 if square else np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in qs])
+                                       for _ in range(2)])
+    for d in (1, 2):
+        h = ob.HpsLevel(octx, T, d)
+        rs = h.r()
+        R = _prod(rs)
+        ql = qs[:k0 - d]
+        Ql = _prod(ql)
+        out = h.multiply(ct1, ct2, square=square)
+        X = []
+        for p in range(2):
+            s = ob.hps_scale_q_ql(octx, d, ct1[p])
+            X.append([_centred(crt([int(v) for v in s[:, n]], ql)[0], Ql) for n in range(N)])
+        if square:
+            Y = X
+        else:
+            Y = []
+            for p in range(2):
+                z = h.modswitch(ct2[p])
+                Y.append([_centred(crt([int(v) for v in z[:, n]], rs)[0], R) for n in range(N)])
+        zs = (negacyclic_int(X[0], Y[0]),
+              [a + b for a, b in zip(negacyclic_int(X[0], Y[1]), negacyclic_int(X[1], Y[0]))],
+              negacyclic_int(X[1], Y[1]))
+        for p, z in enumerate(zs):
+            assert not out[p, k0 - d:].any()
+            for n in range(N):
+                w = (2 * T * z[n] + R) // (2 * R)
+                got = crt([int(out[p, i, n]) for i in range(k0)], qs)[0]
+                assert got == (w % Ql) * (Q // Ql) % Q, (d, p, n)
+
+
+def test_levels_to_drop_invariants():
+    """FindLevelsToDrop: 0 for a fresh ciphertext, non-decreasing in the depth, at most
+    size_Q - 1, and no more for a key switch or an asymmetric ciphertext (larger
+    extra noise / smaller fresh noise). Its constants are the reference's own
+    arrangement: beyond these invariants the value is not pinned."""
+    octx, mods, N = setup(13, [54, 54, 54, 54, 60], 1)
+    prev = 0
+    vals = []
+    for depth in range(0, 12):
+        v = ob.hps_levels_to_drop(octx, T, depth)
+        assert prev <= v <= 3
+        assert ob.hps_levels_to_drop(octx, T, depth, key_switch=True) <= v
+        assert ob.hps_levels_to_drop(octx, T, depth, asymmetric=True) <= v
+        prev = v
+        vals.append(v)
+    assert vals[0] == 0 and vals[1] == 0
+    assert vals[-1] >= 1
