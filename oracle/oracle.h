@@ -9,6 +9,16 @@
  * 128-bit integer arithmetic, and cites the reference file:line it follows.
  * All outputs are canonical residues in [0, q).
  *
+ * Pinning (tests/test_oracle*.py): every routine is checked against closed
+ * forms, known answers or the specification's vectors, except:
+ *   - the FP64 tie behaviour of the HPS scale-and-round sums (rounding
+ *     boundaries; the closed forms hold within the FP64 bound): parity unpinned;
+ *   - orc_hps_levels_to_drop (FindLevelsToDrop): invariants only, its value is
+ *     parity unpinned;
+ *   - orc_sample_uniform_seeded: the Salsa20 core is pinned by the
+ *     specification's vectors, the seed/nonce layout, rejection bound and
+ *     redraw nonce are the reference's own arrangement: parity unpinned.
+ *
  * Layouts (identical to the reference):
  *   polynomial limb set  [limb][N]            (u64)
  *   ciphertext           [poly][limb][N]
