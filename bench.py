@@ -346,9 +346,16 @@ def cpu_baseline(N, bits, size_P, budget_s):
         el = time.perf_counter() - t0
         if el >= budget_s:
             break
+    # the same op on one thread (one op: ~10-20 s on one core)
+    ob.set_threads(1)
+    t1 = time.perf_counter()
+    octx.multiply_and_relin(Ql, a, b, key)
+    el1 = time.perf_counter() - t1
+    ob.set_threads(threads)
     return {"value": n / el, "unit": "HMult+relin/s", "cores": threads, "kind": "port",
+            "value_1thread": 1.0 / el1,
             "sample": f"{n} HMult+relin at N=2^16, L=35+12, dnum=3 on the oracle (CPU restatement of the "
-                      f"reference algorithm, {threads} threads), {el:.1f} s"}
+                      f"reference algorithm, {threads} threads), {el:.1f} s; plus one op on 1 thread, {el1:.1f} s"}
 
 
 def main():
@@ -391,7 +398,7 @@ def main():
         log(f"bench: autotune {autotune}")
     if args.ntt_only:
         L = ctx.size_Q + ctx.size_P
-        ntt_ms, ntt_min = bench_ntt(ph, ctx, N, L, args.ntt_polys, 10, stream)
+        ntt_ms, ntt_min = bench_ntt(ph, ctx, N, L, args.ntt_polys, 30, stream)
         copy_gbs = bench_copy(ph, 1 << 27, 10, stream)
         print(json.dumps({"ntt_limbs": L * args.ntt_polys, "ntt_ms": ntt_ms, "ntt_bytes_algorithmic":
                           2 * N * 8 * L * args.ntt_polys, "copy_bytes_per_launch": 2 * (1 << 27) * 8,
@@ -416,7 +423,7 @@ def main():
     L = ctx.size_Q + ctx.size_P
     tuned = ctx.tuning()
     ctx.set_tuning(**{k: -1 for k in tuned})
-    ntt_ms, ntt_min = bench_ntt(ph, ctx, N, L, args.ntt_polys, 10, stream)
+    ntt_ms, ntt_min = bench_ntt(ph, ctx, N, L, args.ntt_polys, 30, stream)
     ctx.set_tuning(**tuned)
     ntt_bytes = 2 * N * 8 * L * args.ntt_polys
     achieved = ntt_bytes / (ntt_ms / 1e3) / 1e9
