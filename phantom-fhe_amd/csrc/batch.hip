@@ -4,6 +4,7 @@
 #include <memory>
 
 #include "batch.h"
+#include "ntt_arith.h"
 #include "bfv.h"
 #include "tune.h"
 
@@ -70,6 +71,45 @@ __global__ __launch_bounds__(kThreads) void k_inner_product_batch(const u64 *__r
     const u64 QPN = (u64)QP * N;
     const DevModulus m = mod[row];
     constexpr u32 kMaxBeta = MB;
+    if (m.pad) {
+        // q < 2^50: exact products on the FP64 pipe (FpArith::mulmod, |r| < q), the
+        // partial sums reduced every 4 digits (|sum| < 4.5 q < 2^53), then canonical
+#pragma clang fp contract(off)
+        const FpArith fa{nullptr, (double)m.value, __longlong_as_double((long long)m.pad)};
+        double kb[kMaxBeta], ka[kMaxBeta];
+        const u64 *cur = nullptr;
+        for (u32 item = 0; item < B; ++item) {
+            const u64 *key = keys[item];
+            if (key != cur) {
+                cur = key;
+#pragma unroll
+                for (u32 j = 0; j < kMaxBeta; ++j)
+                    if (j < beta) {
+                        const u64 *p = key + (u64)j * 2 * QPN + (u64)row * N + k;
+                        kb[j] = FpArith::from_canon(p[0]);
+                        ka[j] = FpArith::from_canon(p[QPN]);
+                    }
+            }
+            const u64 *ti = t + (u64)item * beta * QlP * N + (u64)pos * N + k;
+            const u64 *ci = c2 + (u64)item * c2_stride + (u64)pos * N + k;
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (u32 j = 0; j < kMaxBeta; ++j)
+                if (j < beta) {
+                    if (j && (j & 3) == 0) {
+                        s0 = fa.red(s0);
+                        s1 = fa.red(s1);
+                    }
+                    const double x = FpArith::from_canon(j == own ? *ci : ti[(u64)j * QlP * N]);
+                    s0 += fa.mulmod(x, kb[j]);
+                    s1 += fa.mulmod(x, ka[j]);
+                }
+            u64 *co = cx + (u64)item * 2 * QlP * N + (u64)pos * N + k;
+            co[0] = fa.canon(s0);
+            co[(u64)QlP * N] = fa.canon(s1);
+        }
+        return;
+    }
     u64 kb[kMaxBeta], ka[kMaxBeta];
     const u64 *cur = nullptr;
     for (u32 item = 0; item < B; ++item) {

@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "evaluator.h"
+#include "ntt_arith.h"
 #include "prng.h"
 #include "tune.h"
 #include "bfv.h"
@@ -294,6 +295,27 @@ __global__ __launch_bounds__(kThreads) void k_inner_product(const u64 *__restric
     const u32 row = pos < Ql ? pos : size_Q + (pos - Ql);
     const u32 own = c2 && pos < Ql ? pos / alpha : ~0u;
     const u64 QPN = (u64)QP * N;
+    const DevModulus m = mod[row];
+    if (m.pad) {
+        // q < 2^50: exact products on the FP64 pipe (FpArith::mulmod, |r| < q), the
+        // partial sums reduced every 4 digits (|sum| < 4.5 q < 2^53), then canonical
+#pragma clang fp contract(off)
+        const FpArith fa{nullptr, (double)m.value, __longlong_as_double((long long)m.pad)};
+        double s0 = 0.0, s1 = 0.0;
+        for (u32 j = 0; j < beta; ++j) {
+            if (j && (j & 3) == 0) {
+                s0 = fa.red(s0);
+                s1 = fa.red(s1);
+            }
+            const double x = FpArith::from_canon(j == own ? c2[(u64)pos * N + k] : t[((u64)j * QlP + pos) * N + k]);
+            const u64 *kb = key + (u64)j * 2 * QPN + (u64)row * N + k;
+            s0 += fa.mulmod(x, FpArith::from_canon(kb[0]));
+            s1 += fa.mulmod(x, FpArith::from_canon(kb[QPN]));
+        }
+        cx[(u64)pos * N + k] = fa.canon(s0);
+        cx[((u64)QlP + pos) * N + k] = fa.canon(s1);
+        return;
+    }
     u128 a0 = 0, a1 = 0;
     for (u32 j = 0; j < beta; ++j) {
         const u64 x = j == own ? c2[(u64)pos * N + k] : t[((u64)j * QlP + pos) * N + k];
@@ -301,7 +323,6 @@ __global__ __launch_bounds__(kThreads) void k_inner_product(const u64 *__restric
         a0 += (u128)x * kb[0];
         a1 += (u128)x * kb[QPN];
     }
-    const DevModulus m = mod[row];
     cx[(u64)pos * N + k] = barrett128(a0, m.value, m.ratio_lo, m.ratio_hi);
     cx[((u64)QlP + pos) * N + k] = barrett128(a1, m.value, m.ratio_lo, m.ratio_hi);
 }
