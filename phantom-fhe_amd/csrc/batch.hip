@@ -545,14 +545,12 @@ void BatchPlan::after_launch() {
 
 void BatchPlan::run(hipStream_t s) {
     if (transient_) {
-        u64 *arena = static_cast<u64 *>(dev_alloc(std::max<size_t>(scratch_bytes_, 256), s));
-        void *t = dev_alloc(host_tables_.size() * sizeof(void *), s);
-        PHX_HIP_CHECK(hipMemcpyAsync(t, host_tables_.data(), host_tables_.size() * sizeof(void *),
+        Scratch arena(std::max<size_t>(scratch_bytes_, 256), s);
+        Scratch t(host_tables_.size() * sizeof(void *), s);
+        PHX_HIP_CHECK(hipMemcpyAsync(t.get(), host_tables_.data(), host_tables_.size() * sizeof(void *),
                                      hipMemcpyHostToDevice, s));
-        bind(arena, static_cast<const void **>(t));
+        bind(arena.get(), reinterpret_cast<const void **>(t.get()));
         for (const auto &g : groups_) run_group(g, s);
-        dev_free(t, s);
-        dev_free(arena, s);
     } else {
         record(s);
     }

@@ -362,10 +362,13 @@ void bfv_multiply(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, hi
     require(ct1.chain_index == 1, "BEHZ multiplication runs at the top data level");
     const u64 N = ctx.N(), QN = (u64)B.k * N, BN = (u64)B.nb * N;
     const bool square = &ct1 == &ct2 || ct1.data == ct2.data;
-    u64 *q1 = static_cast<u64 *>(dev_alloc(3 * QN * 8, s));
-    u64 *b1 = static_cast<u64 *>(dev_alloc(3 * BN * 8, s));
-    u64 *q2 = square ? q1 : static_cast<u64 *>(dev_alloc(2 * QN * 8, s));
-    u64 *b2 = square ? b1 : static_cast<u64 *>(dev_alloc(2 * BN * 8, s));
+    Scratch q1_buf(3 * QN * 8, s);
+    u64 *q1 = q1_buf.get();
+    Scratch b1_buf(3 * BN * 8, s);
+    u64 *b1 = b1_buf.get();
+    Scratch q2_buf(square ? 0 : 2 * QN * 8, s), b2_buf(square ? 0 : 2 * BN * 8, s);
+    u64 *q2 = square ? q1 : q2_buf.get();
+    u64 *b2 = square ? b1 : b2_buf.get();
     behz_mul_1(ctx, B, ct1, q1, b1, s);
     if (!square) behz_mul_1(ctx, B, ct2, q2, b2, s);
     // (4) tensor in both bases (tensor_prod_2x2_rns_poly, src/polymath.cu:463-498)
@@ -383,12 +386,6 @@ void bfv_multiply(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, hi
     hipLaunchKernelGGL(k_behz_floor_sk, grid2(N, 3), dim3(kThreads), 0, s, (const u64 *)q1, (const u64 *)b1, ct1.data,
                        B, ctx.dev_mod(), (u32)N, QN, BN, QN);
     PHX_HIP_CHECK(hipGetLastError());
-    if (!square) {
-        dev_free(q2, s);
-        dev_free(b2, s);
-    }
-    dev_free(q1, s);
-    dev_free(b1, s);
 }
 
 // ---- hps_overq_leveled (src/evaluate.cu:549-646, 680-800, 845-1020;
@@ -498,8 +495,10 @@ void hps_multiply_core(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct
     const u32 k0 = ctx.size_Q(), Ql = H.k;
     const u64 N = ctx.N(), QN0 = (u64)k0 * N, SN = (u64)(Ql + H.nr) * N;
     const bool square = &ct1 == &ct2 || ct1.data == ct2.data;
-    u64 *a = static_cast<u64 *>(dev_alloc(3 * SN * 8, s));
-    u64 *b = square ? a : static_cast<u64 *>(dev_alloc(2 * SN * 8, s));
+    Scratch a_buf(3 * SN * 8, s);
+    u64 *a = a_buf.get();
+    Scratch b_buf(square ? 0 : 2 * SN * 8, s);
+    u64 *b = square ? a : b_buf.get();
     LimbMap m = limb_map(a, 0, Ql + H.nr, SN);
     m.split = Ql;
     m.special_row = H.r_row0;
@@ -536,8 +535,6 @@ void hps_multiply_core(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct
                            (u32)N, SN, QN0);
     PHX_HIP_CHECK(hipGetLastError());
     if (d && expand_polys) hps_expand_ql_q(ctx, d, ct1.data, QN0, ct1.data, QN0, expand_polys, false, s);
-    if (!square) dev_free(b, s);
-    dev_free(a, s);
 }
 
 // (c0, c1) of ct (Q limbs) += the BFV key switch of c2 (Ql = size_Q - d limbs,
@@ -546,8 +543,10 @@ void bfv_keyswitch_add(const Context &ctx, Ciphertext &ct, const u64 *c2, u32 d,
                        hipStream_t s) {
     const u32 k0 = ct.coeff_modulus_size, Ql = k0 - d;
     const u64 N = ctx.N(), LN = (u64)Ql * N;
-    u64 *c2n = static_cast<u64 *>(dev_alloc(LN * 8, s));
-    u64 *ks = static_cast<u64 *>(dev_alloc(2 * LN * 8, s));
+    Scratch c2n_buf(LN * 8, s);
+    u64 *c2n = c2n_buf.get();
+    Scratch ks_buf(2 * LN * 8, s);
+    u64 *ks = ks_buf.get();
     PHX_HIP_CHECK(hipMemcpyAsync(c2n, c2, LN * 8, hipMemcpyDeviceToDevice, s));
     ntt_forward(ctx.tables(), limb_map(c2n, 0, Ql), Ql, 1, s);
     keyswitch(ctx, ct.chain_index + d, c2n, key, ks, nullptr, nullptr, s);
@@ -559,8 +558,6 @@ void bfv_keyswitch_add(const Context &ctx, Ciphertext &ct, const u64 *c2, u32 d,
                            ctx.dev_mod(), (u32)N, k0);
         PHX_HIP_CHECK(hipGetLastError());
     }
-    dev_free(ks, s);
-    dev_free(c2n, s);
 }
 }  // namespace
 
@@ -690,10 +687,10 @@ void bfv_relinearize(const Context &ctx, Ciphertext &ct, const KSwitchKey &rlk, 
     // hps_overq_leveled: c2 scaled down to the dropped level first (src/eval_key_switch.cu:110-150)
     const u32 d = bfv_keyswitch_drop(ctx, ct, false);
     if (d) {
-        u64 *c2 = static_cast<u64 *>(dev_alloc((u64)(L - d) * N * 8, s));
+        Scratch c2_buf((u64)(L - d) * N * 8, s);
+        u64 *c2 = c2_buf.get();
         hps_scale_q_ql(ctx, d, ct.data + 2 * LN, LN, c2, (u64)(L - d) * N, 1, s);
         bfv_keyswitch_add(ctx, ct, c2, d, rlk, s);
-        dev_free(c2, s);
     } else {
         bfv_keyswitch_add(ctx, ct, ct.data + 2 * LN, 0, rlk, s);
     }

@@ -2,6 +2,7 @@
 #include "../../include/phantom_amd.h"
 
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "batch.h"
@@ -16,20 +17,32 @@
 
 using namespace phx;
 
+// Handles own their device memory: destroying a handle (or a creation that
+// throws half-way) releases it.
 struct phx_context {
     Context *ctx;
+    ~phx_context() { delete ctx; }
 };
 struct phx_ciphertext {
     Ciphertext ct;
+    ~phx_ciphertext() { ct.release(hipStreamPerThread); }
 };
 struct phx_secret_key {
     SecretKey sk;
+    ~phx_secret_key() {
+        dev_free(sk.s, hipStreamPerThread);
+        dev_free(sk.s2, hipStreamPerThread);
+    }
 };
 struct phx_relin_key {
     KSwitchKey key;
+    ~phx_relin_key() { dev_free(key.data, hipStreamPerThread); }
 };
 struct phx_galois_key {
     GaloisKeys gk;
+    ~phx_galois_key() {
+        for (auto &k : gk.keys) dev_free(k.data, hipStreamPerThread);
+    }
 };
 struct phx_batch {
     BatchPlan *plan;
@@ -124,15 +137,14 @@ int phx_context_create(const phx_params *p, phx_context **out) {
         if (p->galois_elts) q.galois_elts.assign(p->galois_elts, p->galois_elts + p->galois_elts_size);
         require(q.scheme == PHX_SCHEME_CKKS || q.scheme == PHX_SCHEME_BFV || q.scheme == PHX_SCHEME_BGV,
                 "unsupported scheme");
-        auto *h = new phx_context{new Context(q)};
-        *out = h;
+        auto h = std::make_unique<phx_context>();
+        h->ctx = new Context(q);
+        *out = h.release();
     });
 }
 
 int phx_context_destroy(phx_context *ctx) {
     return guard([&] {
-        if (!ctx) return;
-        delete ctx->ctx;
         delete ctx;
     });
 }
@@ -165,8 +177,6 @@ int phx_ciphertext_create(phx_ciphertext **out) {
 
 int phx_ciphertext_destroy(phx_ciphertext *ct) {
     return guard([&] {
-        if (!ct) return;
-        ct->ct.release(hipStreamPerThread);
         delete ct;
     });
 }
@@ -256,10 +266,10 @@ int phx_secret_key_create(const phx_context *ctx, uint64_t seed, phx_secret_key 
     return guard([&] {
         nn(ctx, "context");
         nn(out, "out");
-        auto *k = new phx_secret_key{};
+        auto k = std::make_unique<phx_secret_key>();
         secret_key_generate(*ctx->ctx, k->sk, seed, hipStreamPerThread);
         PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
-        *out = k;
+        *out = k.release();
     });
 }
 
@@ -268,10 +278,10 @@ int phx_secret_key_create_from_coeffs(const phx_context *ctx, const int8_t *coef
         nn(ctx, "context");
         nn(coeffs, "coeffs");
         nn(out, "out");
-        auto *k = new phx_secret_key{};
+        auto k = std::make_unique<phx_secret_key>();
         secret_key_upload(*ctx->ctx, k->sk, coeffs, hipStreamPerThread);
         PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
-        *out = k;
+        *out = k.release();
     });
 }
 
@@ -287,9 +297,6 @@ int phx_secret_key_download(const phx_context *ctx, const phx_secret_key *sk, ui
 
 int phx_secret_key_destroy(phx_secret_key *sk) {
     return guard([&] {
-        if (!sk) return;
-        dev_free(sk->sk.s, hipStreamPerThread);
-        dev_free(sk->sk.s2, hipStreamPerThread);
         delete sk;
     });
 }
@@ -299,10 +306,10 @@ int phx_relin_key_create(const phx_context *ctx, phx_secret_key *sk, uint64_t se
         nn(ctx, "context");
         nn(sk, "secret key");
         nn(out, "out");
-        auto *k = new phx_relin_key{};
+        auto k = std::make_unique<phx_relin_key>();
         relin_key_generate(*ctx->ctx, sk->sk, k->key, seed, hipStreamPerThread);
         PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
-        *out = k;
+        *out = k.release();
     });
 }
 
@@ -313,12 +320,12 @@ int phx_relin_key_create_from_host(const phx_context *ctx, const uint64_t *host,
         nn(out, "out");
         require(dnum >= 1, "dnum must be positive");
         const size_t n = dnum * 2 * (size_t)ctx->ctx->size_QP() * ctx->ctx->N();
-        auto *k = new phx_relin_key{};
+        auto k = std::make_unique<phx_relin_key>();
         k->key.dnum = (u32)dnum;
         k->key.data = static_cast<u64 *>(dev_alloc(n * 8, hipStreamPerThread));
         PHX_HIP_CHECK(hipMemcpyAsync(k->key.data, host, n * 8, hipMemcpyHostToDevice, hipStreamPerThread));
         PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
-        *out = k;
+        *out = k.release();
     });
 }
 
@@ -335,8 +342,6 @@ int phx_relin_key_download(const phx_context *ctx, const phx_relin_key *key, uin
 
 int phx_relin_key_destroy(phx_relin_key *key) {
     return guard([&] {
-        if (!key) return;
-        dev_free(key->key.data, hipStreamPerThread);
         delete key;
     });
 }
@@ -348,15 +353,15 @@ int phx_galois_key_create(const phx_context *ctx, const phx_secret_key *sk, cons
         nn(sk, "secret key");
         nn(elts, "elts");
         nn(out, "out");
-        auto *k = new phx_galois_key{};
+        auto k = std::make_unique<phx_galois_key>();
         for (size_t i = 0; i < n; ++i) {
-            KSwitchKey key;
-            galois_key_generate(*ctx->ctx, sk->sk, elts[i], key, seed + 0x1000 * (i + 1), hipStreamPerThread);
             k->gk.elts.push_back(elts[i]);
-            k->gk.keys.push_back(key);
+            k->gk.keys.emplace_back();   // owned by the handle before its memory is allocated
+            galois_key_generate(*ctx->ctx, sk->sk, elts[i], k->gk.keys.back(), seed + 0x1000 * (i + 1),
+                                hipStreamPerThread);
         }
         PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
-        *out = k;
+        *out = k.release();
     });
 }
 
@@ -368,24 +373,22 @@ int phx_galois_key_create_from_host(const phx_context *ctx, const uint32_t *elts
         nn(host, "host");
         nn(out, "out");
         const size_t per = dnum * 2 * (size_t)ctx->ctx->size_QP() * ctx->ctx->N();
-        auto *k = new phx_galois_key{};
+        auto k = std::make_unique<phx_galois_key>();
         for (size_t i = 0; i < n; ++i) {
-            KSwitchKey key;
+            k->gk.elts.push_back(elts[i]);
+            k->gk.keys.emplace_back();
+            KSwitchKey &key = k->gk.keys.back();
             key.dnum = (u32)dnum;
             key.data = static_cast<u64 *>(dev_alloc(per * 8, hipStreamPerThread));
             PHX_HIP_CHECK(hipMemcpyAsync(key.data, host + i * per, per * 8, hipMemcpyHostToDevice, hipStreamPerThread));
-            k->gk.elts.push_back(elts[i]);
-            k->gk.keys.push_back(key);
         }
         PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
-        *out = k;
+        *out = k.release();
     });
 }
 
 int phx_galois_key_destroy(phx_galois_key *key) {
     return guard([&] {
-        if (!key) return;
-        for (auto &k : key->gk.keys) dev_free(k.data, hipStreamPerThread);
         delete key;
     });
 }
@@ -400,10 +403,10 @@ int phx_encrypt_symmetric(const phx_context *ctx, const phx_secret_key *sk, cons
         const Context &c = *ctx->ctx;
         require(chain >= 1 && chain <= c.size_Q(), "invalid chain index");
         const size_t n = (size_t)c.limbs_at((u32)chain) * c.N();
-        u64 *d = static_cast<u64 *>(dev_alloc(n * 8, S(s)));
+        Scratch d_buf(n * 8, S(s));
+        u64 *d = d_buf.get();
         PHX_HIP_CHECK(hipMemcpyAsync(d, plain, n * 8, hipMemcpyHostToDevice, S(s)));
         encrypt_symmetric(c, sk->sk, d, (u32)chain, out->ct, seed, S(s));
-        dev_free(d, S(s));
         PHX_HIP_CHECK(hipStreamSynchronize(S(s)));
     });
 }
@@ -420,10 +423,10 @@ int phx_encrypt_symmetric_seeded(const phx_context *ctx, const phx_secret_key *s
         const Context &c = *ctx->ctx;
         require(chain >= 1 && chain <= c.size_Q(), "invalid chain index");
         const size_t n = (size_t)c.limbs_at((u32)chain) * c.N();
-        u64 *d = static_cast<u64 *>(dev_alloc(n * 8, S(s)));
+        Scratch d_buf(n * 8, S(s));
+        u64 *d = d_buf.get();
         PHX_HIP_CHECK(hipMemcpyAsync(d, plain, n * 8, hipMemcpyHostToDevice, S(s)));
         encrypt_symmetric_seeded(c, sk->sk, d, (u32)chain, out->ct, seed_a, seed_e, S(s));
-        dev_free(d, S(s));
         PHX_HIP_CHECK(hipStreamSynchronize(S(s)));
     });
 }
@@ -436,10 +439,10 @@ int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext
         nn(out, "out");
         const Context &c = *ctx->ctx;
         const size_t n = (size_t)ct->ct.coeff_modulus_size * c.N();
-        u64 *d = static_cast<u64 *>(dev_alloc(n * 8, S(s)));
+        Scratch d_buf(n * 8, S(s));
+        u64 *d = d_buf.get();
         decrypt(c, sk->sk, ct->ct, d, S(s));
         PHX_HIP_CHECK(hipMemcpyAsync(out, d, n * 8, hipMemcpyDeviceToHost, S(s)));
-        dev_free(d, S(s));
         PHX_HIP_CHECK(hipStreamSynchronize(S(s)));
     });
 }
@@ -528,14 +531,9 @@ int phx_secret_key_load(const phx_context *ctx, const void *buf, size_t bytes, p
         nn(ctx, "context");
         nn(buf, "buffer");
         nn(out, "out");
-        auto *k = new phx_secret_key{};
-        try {
-            secret_key_load(*ctx->ctx, k->sk, static_cast<const uint8_t *>(buf), bytes);
-        } catch (...) {
-            delete k;
-            throw;
-        }
-        *out = k;
+        auto k = std::make_unique<phx_secret_key>();
+        secret_key_load(*ctx->ctx, k->sk, static_cast<const uint8_t *>(buf), bytes);
+        *out = k.release();
     });
 }
 
@@ -562,14 +560,9 @@ int phx_relin_key_load(const phx_context *ctx, const void *buf, size_t bytes, ph
         nn(ctx, "context");
         nn(buf, "buffer");
         nn(out, "out");
-        auto *k = new phx_relin_key{};
-        try {
-            kswitch_key_load(*ctx->ctx, k->key, static_cast<const uint8_t *>(buf), bytes);
-        } catch (...) {
-            delete k;
-            throw;
-        }
-        *out = k;
+        auto k = std::make_unique<phx_relin_key>();
+        kswitch_key_load(*ctx->ctx, k->key, static_cast<const uint8_t *>(buf), bytes);
+        *out = k.release();
     });
 }
 
@@ -615,21 +608,15 @@ int phx_galois_key_load(const phx_context *ctx, const uint32_t *elts, size_t n, 
         u64 cnt = 0;
         std::memcpy(&cnt, p, 8);
         require(cnt == n, "the number of Galois keys in the stream differs from the number of elements");
-        auto *g = new phx_galois_key{};
-        try {
-            size_t off = 8;
-            for (size_t i = 0; i < n; ++i) {
-                require((elts[i] & 1) && elts[i] < 2 * ctx->ctx->N(), "Galois element is not valid");
-                g->gk.elts.push_back(elts[i]);
-                g->gk.keys.emplace_back();
-                off += kswitch_key_load(*ctx->ctx, g->gk.keys.back(), p + off, bytes - off);
-            }
-        } catch (...) {
-            for (auto &k : g->gk.keys) dev_free(k.data, hipStreamPerThread);
-            delete g;
-            throw;
+        auto g = std::make_unique<phx_galois_key>();
+        size_t off = 8;
+        for (size_t i = 0; i < n; ++i) {
+            require((elts[i] & 1) && elts[i] < 2 * ctx->ctx->N(), "Galois element is not valid");
+            g->gk.elts.push_back(elts[i]);
+            g->gk.keys.emplace_back();
+            off += kswitch_key_load(*ctx->ctx, g->gk.keys.back(), p + off, bytes - off);
         }
-        *out = g;
+        *out = g.release();
     });
 }
 

@@ -206,4 +206,19 @@ void *dev_alloc(size_t bytes, hipStream_t s);
 void dev_free(void *p, hipStream_t s);
 void pool_init();
 
+// Device scratch owned by one scope: allocated on stream s and released on it
+// when the scope ends, error paths included (0 bytes: no allocation, null).
+class Scratch {
+public:
+    Scratch(size_t bytes, hipStream_t s) : p_(static_cast<u64 *>(dev_alloc(bytes, s))), s_(s) {}
+    ~Scratch() { dev_free(p_, s_); }
+    Scratch(const Scratch &) = delete;
+    Scratch &operator=(const Scratch &) = delete;
+    u64 *get() const { return p_; }
+
+private:
+    u64 *p_;
+    hipStream_t s_;
+};
+
 }  // namespace phx

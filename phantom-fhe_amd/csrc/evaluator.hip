@@ -570,7 +570,8 @@ void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStrea
     const u64 N = ctx.N();
     const u32 Ql = L.size_Ql, QlP = L.size_QlP, beta = L.beta, alpha = ctx.size_P();
     require(alpha <= (u32)kMaxDigitLimbs, "special_modulus_size above 16 is not supported");
-    u64 *t_cks = static_cast<u64 *>(dev_alloc((size_t)Ql * N * 8, s));
+    Scratch t_cks_buf((size_t)Ql * N * 8, s);
+    u64 *t_cks = t_cks_buf.get();
     // INTT(c2) * partQlHatInv (out of place)
     LimbMap mi = limb_map(t_cks, 0, Ql);
     mi.src = c2;
@@ -588,7 +589,6 @@ void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStrea
     mf.nlimbs = QlP;
     const u32 last_sz = Ql - alpha * (beta - 1);
     ntt_forward(ctx.tables(), mf, QlP - last_sz, beta, s);
-    dev_free(t_cks, s);
 }
 
 void inner_product(const Context &ctx, u32 chain, const u64 *t_mod_up, const KSwitchKey &key, u64 *cx,
@@ -613,7 +613,8 @@ void moddown(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, 
     LimbMap mi = limb_map(cx + (u64)Ql * N, ctx.size_Q(), P, (u64)QlP * N);
     mi.scale = L.moddown_scale;
     ntt_inverse(ctx.tables(), mi, P, npolys, s);
-    u64 *delta = static_cast<u64 *>(dev_alloc((size_t)npolys * Ql * N * 8, s));
+    Scratch delta_buf((size_t)npolys * Ql * N * 8, s);
+    u64 *delta = delta_buf.get();
     bconv_launch(ctx, L.moddown_bconv, {L.h_moddown_bconv}, 0u, npolys, cx, (u64)QlP * N, delta, (u64)Ql * N,
                  nullptr, 0, s);
     StoreModDown st{};
@@ -626,7 +627,6 @@ void moddown(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, 
     st.cx_stride = (u64)QlP * N;
     st.N = (u32)N;
     ntt_forward_store(ctx.tables(), limb_map(delta, 0, Ql, (u64)Ql * N), Ql, npolys, st, s);
-    dev_free(delta, s);
 }
 
 // keyswitch_inplace (src/eval_key_switch.cu:95-182)
@@ -636,13 +636,13 @@ void keyswitch(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey &k
     require(key.data != nullptr, "key-switching key is empty");
     require(key.dnum >= L.beta, "key-switching key has too few digits for this level");
     const u64 N = ctx.N();
-    u64 *t_mod_up = static_cast<u64 *>(dev_alloc((size_t)L.beta * L.size_QlP * N * 8, s));
-    u64 *cx = static_cast<u64 *>(dev_alloc((size_t)2 * L.size_QlP * N * 8, s));
+    Scratch t_mod_up_buf((size_t)L.beta * L.size_QlP * N * 8, s);
+    u64 *t_mod_up = t_mod_up_buf.get();
+    Scratch cx_buf((size_t)2 * L.size_QlP * N * 8, s);
+    u64 *cx = cx_buf.get();
     modup(ctx, chain, c2, t_mod_up, s, false);
     inner_product(ctx, chain, t_mod_up, key, cx, s, c2);
     moddown(ctx, chain, cx, ct, base0, base1, 2, s);
-    dev_free(cx, s);
-    dev_free(t_mod_up, s);
 }
 
 // multiply_and_relin_inplace for CKKS (src/evaluate.cu:1061-1104)
@@ -667,10 +667,10 @@ void multiply_and_relin(const Context &ctx, Ciphertext &ct1, const Ciphertext &c
     require(ct1.size == 2 && ct2.size == 2, "poly number mismatch");
     const u32 Ql = ct1.coeff_modulus_size;
     const u64 N = ctx.N(), LN = (u64)Ql * N;
-    u64 *c2 = static_cast<u64 *>(dev_alloc(LN * 8, s));
+    Scratch c2_buf(LN * 8, s);
+    u64 *c2 = c2_buf.get();
     tensor_2x2(ctx, Ql, ct1.data, ct2.data, ct1.data, ct1.data + LN, c2, s);
     keyswitch(ctx, ct1.chain_index, c2, rlk, ct1.data, ct1.data, ct1.data + LN, s);
-    dev_free(c2, s);
     ct1.scale = ct1.scale * ct2.scale;
 }
 
@@ -692,7 +692,8 @@ void rescale_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, 
     const LevelConsts &L = ctx.level(in.chain_index);
     const u32 Ql = L.size_Ql, size = in.size;
     const u64 N = ctx.N();
-    u64 *last = static_cast<u64 *>(dev_alloc((size_t)size * N * 8, s));
+    Scratch last_buf((size_t)size * N * 8, s);
+    u64 *last = last_buf.get();
     LimbMap mi = limb_map(last, Ql - 1, 1, N);
     mi.src = in.data + (u64)(Ql - 1) * N;
     mi.src_poly_stride = (u64)Ql * N;
@@ -702,7 +703,6 @@ void rescale_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, 
     LoadRescale ld{last, N};
     StoreRescale st{in.data, dst.data, L.rescale_qinv, (u64)Ql * N, (u64)(Ql - 1) * N, (u32)N};
     ntt_forward_rescale(ctx.tables(), limb_map(dst.data, 0, Ql - 1, (u64)(Ql - 1) * N), Ql - 1, size, ld, st, s);
-    dev_free(last, s);
     dst.is_ntt_form = in.is_ntt_form;
     dst.scale = in.scale / (double)ctx.moduli()[Ql - 1];
     dst.correction_factor = in.correction_factor;
@@ -740,9 +740,12 @@ void hoisting_at(const Context &ctx, Ciphertext &ct, const GaloisKeys &gk, const
     // runs its BFV modup / moddown; both equal the NTT-domain form conjugated by the
     // exact transform, so the ciphertext is transformed, hoisted and transformed back
     if (bfv) ntt_forward(ctx.tables(), limb_map(ct.data, 0, Ql, LN), Ql, 2, s);
-    u64 *t = static_cast<u64 *>(dev_alloc((size_t)beta * LPN * 8, s));
-    u64 *cx = static_cast<u64 *>(dev_alloc(2 * LPN * 8, s));
-    u64 *acc0 = static_cast<u64 *>(dev_alloc(LN * 8, s));
+    Scratch t_buf((size_t)beta * LPN * 8, s);
+    u64 *t = t_buf.get();
+    Scratch cx_buf(2 * LPN * 8, s);
+    u64 *cx = cx_buf.get();
+    Scratch acc0_buf(LN * 8, s);
+    u64 *acc0 = acc0_buf.get();
     modup(ctx, ct.chain_index, ct.data + LN, t, s, true);
     for (size_t i = 0; i < items.size(); ++i) {
         const int acc = i > 0;
@@ -756,9 +759,6 @@ void hoisting_at(const Context &ctx, Ciphertext &ct, const GaloisKeys &gk, const
     // ct0 = acc0 + moddown(cx0), ct1 = moddown(cx1)
     moddown(ctx, ct.chain_index, cx, ct.data, acc0, nullptr, 2, s);
     if (bfv) ntt_inverse(ctx.tables(), limb_map(ct.data, 0, Ql, LN), Ql, 2, s);
-    dev_free(acc0, s);
-    dev_free(cx, s);
-    dev_free(t, s);
 }
 }  // namespace
 
@@ -805,9 +805,12 @@ void apply_galois(const Context &ctx, Ciphertext &ct, u32 elt, const GaloisKeys 
         // permutation conjugated by the exact transform
         const u32 Qd = Ql - d;
         const u64 DN = (u64)Qd * N;
-        u64 *tmp = static_cast<u64 *>(dev_alloc(2 * LN * 8, s));
-        u64 *c2 = static_cast<u64 *>(dev_alloc(DN * 8, s));
-        u64 *ks = static_cast<u64 *>(dev_alloc(2 * DN * 8, s));
+        Scratch tmp_buf(2 * LN * 8, s);
+        u64 *tmp = tmp_buf.get();
+        Scratch c2_buf(DN * 8, s);
+        u64 *c2 = c2_buf.get();
+        Scratch ks_buf(2 * DN * 8, s);
+        u64 *ks = ks_buf.get();
         ntt_forward(ctx.tables(), limb_map(ct.data, 0, Ql, LN), Ql, 2, s);
         hipLaunchKernelGGL(k_permute, grid_for(N, 2 * Ql), dim3(kThreads), 0, s, (const u64 *)ct.data, tmp, perm,
                            (u32)N);
@@ -820,9 +823,6 @@ void apply_galois(const Context &ctx, Ciphertext &ct, u32 elt, const GaloisKeys 
         PHX_HIP_CHECK(hipMemcpyAsync(ct.data, tmp, LN * 8, hipMemcpyDeviceToDevice, s));
         hps_expand_ql_q(ctx, d, ks, DN, ct.data, LN, 1, true, s);
         hps_expand_ql_q(ctx, d, ks + DN, DN, ct.data + LN, LN, 1, false, s);
-        dev_free(ks, s);
-        dev_free(c2, s);
-        dev_free(tmp, s);
         return;
     }
     // BFV (coefficient form): the reference's coefficient-domain automorphism and
@@ -830,13 +830,13 @@ void apply_galois(const Context &ctx, Ciphertext &ct, u32 elt, const GaloisKeys 
     // conjugated by the exact transform, so the ciphertext is transformed,
     // rotated and transformed back
     if (bfv) ntt_forward(ctx.tables(), limb_map(ct.data, 0, Ql, LN), Ql, 2, s);
-    u64 *tmp = static_cast<u64 *>(dev_alloc(2 * LN * 8, s));
+    Scratch tmp_buf(2 * LN * 8, s);
+    u64 *tmp = tmp_buf.get();
     hipLaunchKernelGGL(k_permute, grid_for(N, 2 * Ql), dim3(kThreads), 0, s, (const u64 *)ct.data, tmp, perm,
                        (u32)N);
     PHX_HIP_CHECK(hipGetLastError());
     // (c0, c1) <- (pi(c0), 0) + keyswitch(pi(c1))
     keyswitch(ctx, ct.chain_index, tmp + LN, key, ct.data, tmp, nullptr, s);
-    dev_free(tmp, s);
     if (bfv) ntt_inverse(ctx.tables(), limb_map(ct.data, 0, Ql, LN), Ql, 2, s);
 }
 
@@ -896,12 +896,12 @@ void secret_key_upload(const Context &ctx, SecretKey &sk, const int8_t *coeffs, 
     const u64 N = ctx.N();
     const u32 QP = ctx.size_QP();
     sk.s = static_cast<u64 *>(dev_alloc((size_t)QP * N * 8, s));
-    int8_t *d = static_cast<int8_t *>(dev_alloc(N, s));
+    Scratch d_buf(N, s);
+    int8_t *d = reinterpret_cast<int8_t *>(d_buf.get());
     PHX_HIP_CHECK(hipMemcpyAsync(d, coeffs, N, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_signed_to_rns, grid_for(N, QP), dim3(kThreads), 0, s, (const int8_t *)d, sk.s,
                        ctx.dev_mod(), (u32)N);
     ntt_forward(ctx.tables(), limb_map(sk.s, 0, QP), QP, 1, s);
-    dev_free(d, s);
 }
 
 // generate_one_kswitch_key (src/secretkey.cu:297-341), dnum = ceil(size_Q / size_P)
@@ -912,7 +912,8 @@ void kswitch_key_generate(const Context &ctx, const SecretKey &sk, const u64 *ne
     const u64 QPN = (u64)QP * N;
     key.dnum = dnum;
     key.data = static_cast<u64 *>(dev_alloc((size_t)dnum * 2 * QPN * 8, s));
-    u64 *e = static_cast<u64 *>(dev_alloc(QPN * 8, s));
+    Scratch e_buf(QPN * 8, s);
+    u64 *e = e_buf.get();
     for (u32 j = 0; j < dnum; ++j) {
         const u64 sj = seed * 0x100000001b3ull + j * 0x9e3779b97f4a7c15ull + 1;
         hipLaunchKernelGGL(k_sample_error, grid_for(N, QP), dim3(kThreads), 0, s, e, ctx.dev_mod(), (u32)N, 0u,
@@ -925,7 +926,6 @@ void kswitch_key_generate(const Context &ctx, const SecretKey &sk, const u64 *ne
                            ctx.dev_mod(), (u32)N, sj, (const u64 *)nullptr);
     }
     PHX_HIP_CHECK(hipGetLastError());
-    dev_free(e, s);
 }
 
 void relin_key_generate(const Context &ctx, SecretKey &sk, KSwitchKey &key, u64 seed, hipStream_t s) {
@@ -944,11 +944,11 @@ void galois_key_generate(const Context &ctx, const SecretKey &sk, u32 elt, KSwit
                          hipStream_t s) {
     const u64 N = ctx.N();
     const u32 QP = ctx.size_QP();
-    u64 *rs = static_cast<u64 *>(dev_alloc((size_t)QP * N * 8, s));
+    Scratch rs_buf((size_t)QP * N * 8, s);
+    u64 *rs = rs_buf.get();
     hipLaunchKernelGGL(k_permute, grid_for(N, QP), dim3(kThreads), 0, s, (const u64 *)sk.s, rs, ctx.galois_perm(elt),
                        (u32)N);
     kswitch_key_generate(ctx, sk, rs, key, seed, s);
-    dev_free(rs, s);
 }
 
 namespace {
@@ -963,8 +963,10 @@ void encrypt_rlwe(const Context &ctx, const SecretKey &sk, const u64 *plain_dev,
         std::memcpy(ct.seed, seed_a, sizeof(ct.seed));
         sample_uniform_seeded(ctx, seed_a, L, ct.data + (u64)L * N, s);
     }
-    u64 *m = static_cast<u64 *>(dev_alloc((size_t)L * N * 8, s));
-    u64 *e = static_cast<u64 *>(dev_alloc((size_t)L * N * 8, s));
+    Scratch m_buf((size_t)L * N * 8, s);
+    u64 *m = m_buf.get();
+    Scratch e_buf((size_t)L * N * 8, s);
+    u64 *e = e_buf.get();
     PHX_HIP_CHECK(hipMemcpyAsync(m, plain_dev, (size_t)L * N * 8, hipMemcpyDeviceToDevice, s));
     ntt_forward(ctx.tables(), limb_map(m, 0, L), L, 1, s);
     hipLaunchKernelGGL(k_sample_error, grid_for(N, L), dim3(kThreads), 0, s, e, ctx.dev_mod(), (u32)N, 0u,
@@ -974,8 +976,6 @@ void encrypt_rlwe(const Context &ctx, const SecretKey &sk, const u64 *plain_dev,
                        (const u64 *)sk.s, (const u64 *)e, (const u64 *)m, (const u64 *)nullptr, (const u64 *)nullptr,
                        0u, 0u, ctx.dev_mod(), (u32)N, seed, seed_a ? (const u64 *)(ct.data + (u64)L * N) : nullptr);
     PHX_HIP_CHECK(hipGetLastError());
-    dev_free(m, s);
-    dev_free(e, s);
 }
 }  // namespace
 
