@@ -90,3 +90,21 @@ def test_set_tuning_rejects():
         ctx.set_tuning(bconv_mfma=2)
     with pytest.raises(ValueError):
         ctx.set_tuning(colour=1)
+
+
+def test_autotune_bfv_leveled_context():
+    """a BFV hps_overq_leveled context has no batch plans: the autotuner times the
+    plain multiply_and_relin only and still applies a valid choice"""
+    N = 1 << 13
+    say("case: autotune bfv hps_overq_leveled")
+    mods = ph.coeff_modulus_create(N, [54] * 4 + [60])
+    parms = ph.EncryptionParameters(ph.SCHEME_BFV)
+    parms.set_poly_modulus_degree(N)
+    parms.set_coeff_modulus(mods)
+    parms.set_special_modulus_size(1)
+    parms.set_plain_modulus(65537)
+    parms.set_mul_tech(ph.MUL_TECH_HPS_OVERQ_LEVELED)
+    ctx = ph.context(parms)
+    rep = ctx.autotune()
+    assert rep["trials"] >= 2 and rep["best_us"] > 0
+    assert set(ctx.tuning()) == {"ntt_fused", "ntt_fused_min_limbs", "ntt_chunk_limbs", "bconv_mfma"}
