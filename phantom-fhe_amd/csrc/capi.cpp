@@ -233,6 +233,10 @@ int phx_ciphertext_copy(const phx_context *ctx, phx_ciphertext *dst, const phx_c
         dst->ct.scale = a.scale;
         dst->ct.is_ntt_form = a.is_ntt_form;
         dst->ct.correction_factor = a.correction_factor;
+        dst->ct.noise_scale_deg = a.noise_scale_deg;
+        dst->ct.is_asymmetric = a.is_asymmetric;
+        dst->ct.has_seed = a.has_seed;
+        std::memcpy(dst->ct.seed, a.seed, sizeof(a.seed));
     });
 }
 

@@ -238,3 +238,22 @@ def test_encrypt_multiply_relin_decrypt(lv, deg):
     q0 = mods[0]
     want = [((int(x) if int(x) <= q0 // 2 else int(x) - q0) % T) for x in pm]
     assert want == dec
+
+
+def test_copy_keeps_noise_degree(lv):
+    """a copied ciphertext keeps its noise degree, so the non-inplace
+    multiply_and_relin (copy, then in place) drops the same levels"""
+    ctx, octx, mods = lv
+    say("case: leveled copy keeps degree")
+    rng = np.random.default_rng(77)
+    key = rand_key(rng, mods, N, ctx.dnum)
+    rk = ph.relin_key(ctx, host=key)
+    a, b = rand_ct(rng, mods[:K], N), rand_ct(rng, mods[:K], N)
+    c1, c2 = with_degree(ctx, a, 6), with_degree(ctx, b, 6)
+    cp = ph.ciphertext()
+    cp.copy_from(ctx, c1)
+    assert degree(cp) == 6
+    out = ph.multiply_and_relin(ctx, c1, c2, rk)
+    assert degree(out) == 7 and degree(c1) == 6
+    ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
+    assert np.array_equal(out.download(N), c1.download(N))
