@@ -52,7 +52,7 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
         u32 baseA[1] = {rest * 256 + c0 + uc};
 #pragma unroll
         for (int e = 0; e < 16; ++e) v[e] = A::from_canon(ld(data, baseA[0] + ((u32)e << 13), limb, z, md));
-        fwd_round<4, 13>(v, baseA, logN, a);
+        fwd_round<4, 13, A, true>(v, baseA, logN, a);
 #pragma unroll
         for (int e = 0; e < 16; ++e) lds[col_addr<TC>(((u32)e << 5) + rest, uc)] = A::to_bits(v[e]);
         __syncthreads();
@@ -101,7 +101,7 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
         for (int e = 0; e < (1 << NA); ++e)
             v[b * (1 << NA) + e] = A::from_canon(ld(data, baseA[b] + ((u32)e << (8 + NB)), limb, z, md));
 
-    fwd_round<NA, 8 + NB>(v, baseA, logN, a);
+    fwd_round<NA, 8 + NB, A, true>(v, baseA, logN, a);
 
     if constexpr (NB > 0) {
 #pragma unroll

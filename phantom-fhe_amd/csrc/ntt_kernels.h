@@ -124,7 +124,10 @@ __device__ __forceinline__ u32 tw_index(u32 m, u32 base, int jj, int e) {
     }
 }
 
-template <int NB, int P, class A>
+// Top = the first round of a column pass: every base is below 2^P, so the
+// group offset base >> (P+jj+1) is 0 and the twiddle index m + k is the same
+// for every thread (a uniform address the compiler can load once per wave).
+template <int NB, int P, class A, bool Top = false>
 __device__ __forceinline__ void fwd_round(typename A::T (&v)[16], const u32 (&base)[16 >> NB], u32 logN,
                                           const A &a) {
 #pragma unroll
@@ -135,7 +138,8 @@ __device__ __forceinline__ void fwd_round(typename A::T (&v)[16], const u32 (&ba
 #pragma unroll
             for (int e = 0; e < (1 << NB); ++e) {
                 if (e & (1 << jj)) continue;
-                a.ct(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], tw_index<P, NB>(m, base[b], jj, e));
+                const u32 idx = Top ? m + ((u32)e >> (jj + 1)) : tw_index<P, NB>(m, base[b], jj, e);
+                a.ct(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], idx);
             }
         }
     }
@@ -161,6 +165,8 @@ __device__ __forceinline__ void inv_round(typename A::T (&v)[16], const u32 (&ba
 // Last inverse round: as inv_round, but the top stage (distance N/2, m = 1)
 // multiplies its outputs by s0 (x output: n^-1 * scale) and s1 (y output:
 // n^-1 * psi^-brv(1) * scale), leaving them ready for canonical form.
+// (the last round of a column pass: every base is below 2^p, so the group
+// offset base >> sh is 0 and the twiddle indices are uniform)
 template <int NB, class A>
 __device__ __forceinline__ void inv_round_last(typename A::T (&v)[16], const u32 (&base)[16 >> NB], int p,
                                                u32 logN, const A &a, u64 s0, u64 s0s, u64 s1, u64 s1s) {
@@ -170,7 +176,7 @@ __device__ __forceinline__ void inv_round_last(typename A::T (&v)[16], const u32
         const u32 m = 1u << (logN - sh);
 #pragma unroll
         for (int b = 0; b < (16 >> NB); ++b) {
-            const u32 g0 = m + (base[b] >> sh);
+            const u32 g0 = m;
 #pragma unroll
             for (int e = 0; e < (1 << NB); ++e) {
                 if (e & (1 << jj)) continue;
