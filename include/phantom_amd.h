@@ -45,6 +45,7 @@ extern "C" {
 typedef struct phx_context phx_context;
 typedef struct phx_ciphertext phx_ciphertext;
 typedef struct phx_secret_key phx_secret_key;
+typedef struct phx_public_key phx_public_key;
 typedef struct phx_relin_key phx_relin_key;
 typedef struct phx_galois_key phx_galois_key;
 typedef void *phx_stream;
@@ -154,6 +155,27 @@ int phx_ciphertext_load(const phx_context *ctx, phx_ciphertext *ct, const void *
 int phx_ciphertext_save_symmetric_size(const phx_ciphertext *ct, size_t *bytes);
 int phx_ciphertext_save_symmetric(const phx_context *ctx, const phx_ciphertext *ct, void *buf, size_t bytes);
 int phx_ciphertext_load_symmetric(const phx_context *ctx, phx_ciphertext *ct, const void *buf, size_t bytes);
+/* PhantomSecretKey::gen_publickey (src/secretkey.cu:382-394): pk = (-(a s + e), a) at the key
+ * level (QP limbs, NTT form); a from the seeded uniform sampler for the 64-byte seed_a (NULL: the
+ * library's counter hash), e from seed_e */
+int phx_public_key_create(const phx_context *ctx, const phx_secret_key *sk, const uint8_t *seed_a, uint64_t seed_e,
+                          phx_public_key **out);
+int phx_public_key_destroy(phx_public_key *pk);
+int phx_public_key_download(const phx_context *ctx, const phx_public_key *pk, uint64_t *host_2_qp_n);
+/* PhantomPublicKey::save / load (include/secretkey.h:85-100): the public key's ciphertext stream */
+int phx_public_key_save_size(const phx_public_key *pk, size_t *bytes);
+int phx_public_key_save(const phx_public_key *pk, void *buf, size_t bytes);
+int phx_public_key_load(const phx_context *ctx, const void *buf, size_t bytes, phx_public_key **out);
+/* PhantomPublicKey::encrypt_asymmetric (src/secretkey.cu:11-186) at the first data level (chain 1):
+ * c_j = moddown_{QP->Q}(pk_j u + e_j), then c0 += the plaintext: CKKS plain = coefficient residues
+ * [size_Q][N] (from phx_ckks_encode), added in NTT form; BFV plain = N coefficients mod t, added as
+ * (Q m + [-Q m]_t) / t (multiply_add_plain_with_scaling_variant). u ternary and e_0, e_1 from `seed`, or,
+ * in the _with_samples form, the given int8 coefficients (N each, host) */
+int phx_encrypt_asymmetric(const phx_context *ctx, const phx_public_key *pk, const uint64_t *plain, size_t chain,
+                           uint64_t seed, phx_ciphertext *out, phx_stream stream);
+int phx_encrypt_asymmetric_with_samples(const phx_context *ctx, const phx_public_key *pk, const uint64_t *plain,
+                                        size_t chain, const int8_t *u, const int8_t *e0, const int8_t *e1,
+                                        uint64_t seed, phx_ciphertext *out, phx_stream stream);
 /* PhantomSecretKey::save / load (include/secretkey.h:347-389): max power, N, QP (8 bytes each),
  * then s (and s^2 when computed) in NTT form over QP; load keeps the first two powers */
 int phx_secret_key_save_size(const phx_context *ctx, const phx_secret_key *sk, size_t *bytes);

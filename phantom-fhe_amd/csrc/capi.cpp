@@ -34,6 +34,10 @@ struct phx_secret_key {
         dev_free(sk.s2, hipStreamPerThread);
     }
 };
+struct phx_public_key {
+    Ciphertext pk;
+    ~phx_public_key() { pk.release(hipStreamPerThread); }
+};
 struct phx_relin_key {
     KSwitchKey key;
     ~phx_relin_key() { dev_free(key.data, hipStreamPerThread); }
@@ -509,6 +513,106 @@ int phx_sample_uniform_poly(const phx_context *ctx, const uint8_t *seed, size_t 
         nn(seed, "seed");
         nn(out, "out");
         sample_uniform_seeded(*ctx->ctx, seed, (u32)coeff_mod_size, out, S(s));
+    });
+}
+
+int phx_public_key_create(const phx_context *ctx, const phx_secret_key *sk, const uint8_t *seed_a, uint64_t seed_e,
+                          phx_public_key **out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(sk, "secret key");
+        nn(out, "out");
+        auto k = std::make_unique<phx_public_key>();
+        public_key_generate(*ctx->ctx, sk->sk, seed_a, seed_e, k->pk, hipStreamPerThread);
+        PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+        *out = k.release();
+    });
+}
+
+int phx_public_key_destroy(phx_public_key *pk) {
+    return guard([&] { delete pk; });
+}
+
+int phx_public_key_download(const phx_context *ctx, const phx_public_key *pk, uint64_t *host_2_qp_n) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(pk, "public key");
+        nn(host_2_qp_n, "out");
+        PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
+        PHX_HIP_CHECK(hipMemcpy(host_2_qp_n, pk->pk.data, (size_t)2 * ctx->ctx->size_QP() * ctx->ctx->N() * 8,
+                                hipMemcpyDeviceToHost));
+    });
+}
+
+int phx_public_key_save_size(const phx_public_key *pk, size_t *bytes) {
+    return guard([&] {
+        nn(pk, "public key");
+        nn(bytes, "bytes");
+        *bytes = ciphertext_bytes(pk->pk);
+    });
+}
+
+int phx_public_key_save(const phx_public_key *pk, void *buf, size_t bytes) {
+    return guard([&] {
+        nn(pk, "public key");
+        nn(buf, "buffer");
+        if (!pk->pk.data) fail(kInvalidArgument, "PhantomPublicKey has not been generated");
+        ciphertext_save(pk->pk, static_cast<uint8_t *>(buf), bytes);
+    });
+}
+
+int phx_public_key_load(const phx_context *ctx, const void *buf, size_t bytes, phx_public_key **out) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(buf, "buffer");
+        nn(out, "out");
+        auto k = std::make_unique<phx_public_key>();
+        ciphertext_load(*ctx->ctx, k->pk, static_cast<const uint8_t *>(buf), bytes);
+        require(k->pk.chain_index == 0 && k->pk.size == 2 && k->pk.is_ntt_form,
+                "the stream does not hold a public key (a 2-polynomial NTT-form ciphertext at the key level)");
+        *out = k.release();
+    });
+}
+
+int phx_encrypt_asymmetric(const phx_context *ctx, const phx_public_key *pk, const uint64_t *plain, size_t chain,
+                           uint64_t seed, phx_ciphertext *out, phx_stream s) {
+    return phx_encrypt_asymmetric_with_samples(ctx, pk, plain, chain, nullptr, nullptr, nullptr, seed, out, s);
+}
+
+int phx_encrypt_asymmetric_with_samples(const phx_context *ctx, const phx_public_key *pk, const uint64_t *plain,
+                                        size_t chain, const int8_t *u, const int8_t *e0, const int8_t *e1,
+                                        uint64_t seed, phx_ciphertext *out, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(pk, "public key");
+        nn(plain, "plain");
+        nn(out, "out");
+        const Context &c = *ctx->ctx;
+        // checked before the plaintext is read: its length follows from the first data level
+        require(chain == 1, "encrypt_asymmetric switches from the key level to the first data level (chain 1) only");
+        const bool bfv = c.params().scheme == kSchemeBFV;
+        const u64 N = c.N();
+        const size_t words = bfv ? N : (size_t)c.size_Q() * N;
+        if (bfv)
+            for (size_t i = 0; i < N; ++i)
+                require(plain[i] < c.params().plain_modulus, "plaintext coefficient is not below plain_modulus");
+        Scratch pd(words * 8, S(s));
+        PHX_HIP_CHECK(hipMemcpyAsync(pd.get(), plain, words * 8, hipMemcpyHostToDevice, S(s)));
+        Scratch sm(u ? 3 * N : 0, S(s));
+        const int8_t *ud = nullptr;
+        const int8_t *ed[2] = {nullptr, nullptr};
+        if (u) {
+            require(e0 && e1, "explicit samples need u, e0 and e1");
+            int8_t *b = reinterpret_cast<int8_t *>(sm.get());
+            PHX_HIP_CHECK(hipMemcpyAsync(b, u, N, hipMemcpyHostToDevice, S(s)));
+            PHX_HIP_CHECK(hipMemcpyAsync(b + N, e0, N, hipMemcpyHostToDevice, S(s)));
+            PHX_HIP_CHECK(hipMemcpyAsync(b + 2 * N, e1, N, hipMemcpyHostToDevice, S(s)));
+            ud = b;
+            ed[0] = b + N;
+            ed[1] = b + 2 * N;
+        }
+        encrypt_asymmetric(c, pk->pk, pd.get(), (u32)chain, ud, ed, seed, out->ct, S(s));
+        PHX_HIP_CHECK(hipStreamSynchronize(S(s)));
     });
 }
 

@@ -552,6 +552,22 @@ Context::Context(const Params &p) : params_(p) {
         bp.push_back(hm::shoup(v, qp[i]));
     }
     const size_t bp_off = cb.push64(bp);
+    // BFV plaintext scaling (multiply_add_plain_with_scaling_variant, src/scalingvariant.cu:10-34;
+    // constants src/rns.cu:296-320): [-Q]_t and t^-1 mod q_i (Shoup) at the first data level
+    size_t tinv_off = 0;
+    if (p.scheme == kSchemeBFV && p.plain_modulus > 1) {
+        const u64 t = p.plain_modulus;
+        const u64 qmodt = hm::product_mod(qp, size_Q_, t);
+        bfv_negq_mod_t_ = qmodt ? t - qmodt : 0;
+        bfv_negq_mod_t_shoup_ = hm::shoup(bfv_negq_mod_t_, t);
+        std::vector<u64> ti;
+        for (u32 i = 0; i < size_Q_; ++i) {
+            const u64 v = hm::invmod(t % qp[i], qp[i]);
+            ti.push_back(v);
+            ti.push_back(hm::shoup(v, qp[i]));
+        }
+        tinv_off = cb.push64(ti);
+    }
     // BEHZ constants (src/rns.cu:424-571)
     struct BehzOffs {
         size_t lift_f, qhat_bsk, qhat_mt, q_mod_bsk, mtinv_bsk, qhatinv_q, qinv_bsk, bhatinv_b, bhat_msk, bhat_q,
@@ -670,6 +686,7 @@ Context::Context(const Params &p) : params_(p) {
         L.rescale_qinv = reinterpret_cast<const u64 *>(cbb + offs[c].rescale);
     }
     bigP_mod_q_ = reinterpret_cast<u64 *>(cbb + bp_off);
+    if (tinv_off) bfv_tinv_q_ = reinterpret_cast<const u64 *>(cbb + tinv_off);
     for (size_t d = 0; d < hps_offs.size(); ++d) {
         const u32 k0 = size_Q_, Ql = k0 - (u32)d;
         HpsConsts h = resolve_hps(cbb, hps_offs[d], Ql, overq ? Ql : (u32)hps_r_.size(), overq, d ? k0 : Ql);

@@ -157,6 +157,11 @@ public:
     const u32 *const_buf32() const { return reinterpret_cast<const u32 *>(cbuf_); }
     // P mod q_i for i < size_Q (keygen gadget factor), [size_Q][2] with Shoup
     const u64 *bigP_mod_q() const { return bigP_mod_q_; }
+    // BFV plaintext scaling: [-Q]_t (and its Shoup companion mod t) and
+    // t^-1 mod q_i, [size_Q][2] with Shoup (null unless BFV with t set)
+    u64 bfv_negq_mod_t() const { return bfv_negq_mod_t_; }
+    u64 bfv_negq_mod_t_shoup() const { return bfv_negq_mod_t_shoup_; }
+    const u64 *bfv_tinv_q() const { return bfv_tinv_q_; }
     const hm::NttTable &host_table(u32 row) const { return host_tables_[row]; }
 
     // BFV with BEHZ multiplication: its constants (else null)
@@ -191,6 +196,8 @@ private:
     void *tw_buf_ = nullptr;     // forward + inverse twiddles, moduli, inverse consts
     u64 *cbuf_ = nullptr;        // per-level constants
     u64 *bigP_mod_q_ = nullptr;
+    u64 bfv_negq_mod_t_ = 0, bfv_negq_mod_t_shoup_ = 0;
+    const u64 *bfv_tinv_q_ = nullptr;
     std::vector<LevelConsts> levels_;
     mutable std::vector<std::pair<u32, u32 *>> galois_tables_;
     BehzConsts behz_{};

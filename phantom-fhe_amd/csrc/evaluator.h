@@ -121,4 +121,18 @@ void encrypt_symmetric_seeded(const Context &ctx, const SecretKey &sk, const u64
 // out: [Ql][N] coefficient-domain residues of c0 + c1 s (+ c2 s^2)
 void decrypt(const Context &ctx, SecretKey &sk, const Ciphertext &ct, u64 *out_dev, hipStream_t s);
 
+// gen_publickey (src/secretkey.cu:382-394): pk = (-(a s + e), a) at the key
+// level (chain 0, QP limbs, NTT form); a from the seeded uniform sampler for
+// seed_a (64 bytes) or, with seed_a null, from the counter hash; e from seed_e.
+void public_key_generate(const Context &ctx, const SecretKey &sk, const uint8_t *seed_a, u64 seed_e,
+                         Ciphertext &pk, hipStream_t s);
+// PhantomPublicKey::encrypt_asymmetric (src/secretkey.cu:11-186) at the first
+// data level: c_j = moddown(pk_j u + e_j) over QP -> Q, then c0 += the plaintext
+// (CKKS: plain_dev = coefficient residues [size_Q][N], added in NTT form; BFV:
+// plain_dev = N coefficients mod t, added as floor(Q m / t)). u ternary and
+// e_0, e_1 come from the int8 samples u_dev / e_dev[2] (device, N each) when
+// given, else from the counter hash with `seed`.
+void encrypt_asymmetric(const Context &ctx, const Ciphertext &pk, const u64 *plain_dev, u32 chain,
+                        const int8_t *u_dev, const int8_t *const e_dev[2], u64 seed, Ciphertext &ct, hipStream_t s);
+
 }  // namespace phx

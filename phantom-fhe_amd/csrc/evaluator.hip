@@ -467,6 +467,39 @@ __global__ void k_sample_error(u64 *e, const DevModulus *mod, u32 N, u32 row0, u
     e[(u64)limb * N + k] = v >= 0 ? (u64)v : q - (u64)(-v);
 }
 
+// out = pk u + e (NTT form), one limb per grid row
+// (multiply_and_add_rns_poly, src/polymath.cu)
+__global__ void k_pk_mul_add(const u64 *pk, const u64 *u, const u64 *e, u64 *out, const DevModulus *mod, u32 N) {
+    const u32 limb = blockIdx.y;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 o = (u64)limb * N + k;
+    const DevModulus m = mod[limb];
+    out[o] = add_mod(mul_mod(pk[o], u[o], m), e[o], m.value);
+}
+
+// a += b, one limb per grid row
+__global__ void k_add_limbs(u64 *a, const u64 *b, const DevModulus *mod, u32 N) {
+    const u32 limb = blockIdx.y;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 o = (u64)limb * N + k;
+    a[o] = add_mod(a[o], b[o], mod[limb].value);
+}
+
+// c0 += (Q m + [-Q m]_t) / t = [[-Q m]_t t^-1]_{q_i}
+// (bfv_add_timesQ_overt_kernel, src/polymath.cu:413-436)
+__global__ void k_bfv_add_plain(u64 *c0, const u64 *pt, u64 negq_t, u64 negq_t_shoup, const u64 *tinv,
+                                const DevModulus *mod, u64 t, u32 N) {
+    const u32 limb = blockIdx.y;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const u64 q = mod[limb].value;
+    const u64 mq = mul_shoup(pt[k], negq_t, negq_t_shoup, t);
+    const u64 o = (u64)limb * N + k;
+    c0[o] = add_mod(c0[o], mul_shoup(mq, tinv[2 * limb], tinv[2 * limb + 1], q), q);
+}
+
 // c1 = a (uniform, NTT form), c0 = -(a s + e) + m (+ gadget * new_key on the
 // digit's limbs). Rows are data rows 0..L-1 (keys: all QP rows).
 // reference: encrypt_zero_symmetric src/secretkey.cu:232-289,
@@ -988,6 +1021,89 @@ void encrypt_symmetric_seeded(const Context &ctx, const SecretKey &sk, const u64
                               Ciphertext &ct, const uint8_t *seed_a, u64 seed_e, hipStream_t s) {
     require(seed_a != nullptr, "seed is null");
     encrypt_rlwe(ctx, sk, plain_dev, chain, ct, seed_e, seed_a, s);
+}
+
+void public_key_generate(const Context &ctx, const SecretKey &sk, const uint8_t *seed_a, u64 seed_e,
+                         Ciphertext &pk, hipStream_t s) {
+    require(sk.s != nullptr, "secret key is empty");
+    const u64 N = ctx.N();
+    const u32 QP = ctx.size_QP();
+    pk.resize(ctx, 0, 2, s);
+    pk.is_ntt_form = true;
+    pk.is_asymmetric = false;
+    pk.has_seed = seed_a != nullptr;
+    u64 *c1 = pk.data + (u64)QP * N;
+    if (seed_a) {
+        std::memcpy(pk.seed, seed_a, sizeof(pk.seed));
+        sample_uniform_seeded(ctx, seed_a, QP, c1, s);
+    }
+    Scratch e_buf((size_t)QP * N * 8, s);
+    u64 *e = e_buf.get();
+    hipLaunchKernelGGL(k_sample_error, grid_for(N, QP), dim3(kThreads), 0, s, e, ctx.dev_mod(), (u32)N, 0u,
+                       seed_e ^ 0x5a5au);
+    ntt_forward(ctx.tables(), limb_map(e, 0, QP), QP, 1, s);
+    hipLaunchKernelGGL(k_rlwe, grid_for(N, QP), dim3(kThreads), 0, s, pk.data, c1, (const u64 *)sk.s,
+                       (const u64 *)e, (const u64 *)nullptr, (const u64 *)nullptr, (const u64 *)nullptr, 0u, 0u,
+                       ctx.dev_mod(), (u32)N, seed_e, seed_a ? (const u64 *)c1 : nullptr);
+    PHX_HIP_CHECK(hipGetLastError());
+}
+
+void encrypt_asymmetric(const Context &ctx, const Ciphertext &pk, const u64 *plain_dev, u32 chain,
+                        const int8_t *u_dev, const int8_t *const e_dev[2], u64 seed, Ciphertext &ct, hipStream_t s) {
+    const bool bfv = ctx.params().scheme == kSchemeBFV;
+    require(bfv || ctx.params().scheme == kSchemeCKKS, "unsupported scheme.");
+    require(pk.data != nullptr && pk.chain_index == 0 && pk.size == 2 && pk.coeff_modulus_size == ctx.size_QP(),
+            "PhantomPublicKey has not been generated");
+    require(chain == 1, "asymmetric encryption runs at the first data level");
+    require(plain_dev != nullptr, "plaintext is null");
+    if (bfv) require(ctx.bfv_tinv_q() != nullptr, "plain_modulus is not set");
+    const u64 N = ctx.N();
+    const u32 QP = ctx.size_QP(), Ql = ctx.size_Q();
+    const u64 QPN = (u64)QP * N, LN = (u64)Ql * N;
+    Scratch u_buf(QPN * 8, s), e_buf(QPN * 8, s), c_buf(2 * QPN * 8, s);
+    u64 *u = u_buf.get(), *e = e_buf.get(), *c = c_buf.get();
+    // u ternary and e_j, shared across limbs (sample_ternary_poly / sample_error_poly)
+    if (u_dev)
+        hipLaunchKernelGGL(k_signed_to_rns, grid_for(N, QP), dim3(kThreads), 0, s, u_dev, u, ctx.dev_mod(), (u32)N);
+    else
+        hipLaunchKernelGGL(k_sample_ternary, grid_for(N, QP), dim3(kThreads), 0, s, u, ctx.dev_mod(), (u32)N,
+                           seed ^ 0x7575u);
+    ntt_forward(ctx.tables(), limb_map(u, 0, QP), QP, 1, s);
+    for (u32 j = 0; j < 2; ++j) {
+        if (e_dev && e_dev[j])
+            hipLaunchKernelGGL(k_signed_to_rns, grid_for(N, QP), dim3(kThreads), 0, s, e_dev[j], e, ctx.dev_mod(),
+                               (u32)N);
+        else
+            hipLaunchKernelGGL(k_sample_error, grid_for(N, QP), dim3(kThreads), 0, s, e, ctx.dev_mod(), (u32)N, 0u,
+                               seed ^ (0x6565u + j));
+        ntt_forward(ctx.tables(), limb_map(e, 0, QP), QP, 1, s);
+        hipLaunchKernelGGL(k_pk_mul_add, grid_for(N, QP), dim3(kThreads), 0, s, (const u64 *)pk.data + j * QPN,
+                           (const u64 *)u, (const u64 *)e, c + j * QPN, ctx.dev_mod(), (u32)N);
+        PHX_HIP_CHECK(hipGetLastError());
+    }
+    // QP -> Q (DRNSTool::moddown, src/rns_bconv.cu:712-760: for BFV the same in the
+    // coefficient domain, i.e. this NTT-domain form conjugated by the exact transform)
+    ct.resize(ctx, chain, 2, s);
+    moddown(ctx, chain, c, ct.data, nullptr, nullptr, 2, s);
+    if (bfv) {
+        ntt_inverse(ctx.tables(), limb_map(ct.data, 0, Ql, LN), Ql, 2, s);
+        hipLaunchKernelGGL(k_bfv_add_plain, grid_for(N, Ql), dim3(kThreads), 0, s, ct.data, plain_dev,
+                           ctx.bfv_negq_mod_t(), ctx.bfv_negq_mod_t_shoup(), ctx.bfv_tinv_q(), ctx.dev_mod(),
+                           ctx.params().plain_modulus, (u32)N);
+    } else {
+        Scratch m_buf(LN * 8, s);
+        PHX_HIP_CHECK(hipMemcpyAsync(m_buf.get(), plain_dev, LN * 8, hipMemcpyDeviceToDevice, s));
+        ntt_forward(ctx.tables(), limb_map(m_buf.get(), 0, Ql), Ql, 1, s);
+        hipLaunchKernelGGL(k_add_limbs, grid_for(N, Ql), dim3(kThreads), 0, s, ct.data, (const u64 *)m_buf.get(),
+                           ctx.dev_mod(), (u32)N);
+    }
+    PHX_HIP_CHECK(hipGetLastError());
+    ct.is_ntt_form = !bfv;
+    ct.is_asymmetric = true;
+    ct.has_seed = false;
+    ct.scale = 1.0;
+    ct.correction_factor = 1;
+    ct.noise_scale_deg = 1;
 }
 
 void decrypt(const Context &ctx, SecretKey &sk, const Ciphertext &ct, u64 *out_dev, hipStream_t s) {

@@ -90,6 +90,16 @@ _SIGS = {
     "phx_secret_key_save_size": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_sz)]),
     "phx_secret_key_save": (ctypes.c_int, [_vp, _vp, _vp, _sz]),
     "phx_secret_key_load": (ctypes.c_int, [_vp, _vp, _sz, ctypes.POINTER(_vp)]),
+    "phx_public_key_create": (ctypes.c_int, [_vp, _vp, ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "phx_public_key_destroy": (ctypes.c_int, [_vp]),
+    "phx_public_key_download": (ctypes.c_int, [_vp, _vp, _u64p]),
+    "phx_public_key_save_size": (ctypes.c_int, [_vp, ctypes.POINTER(_sz)]),
+    "phx_public_key_save": (ctypes.c_int, [_vp, _vp, _sz]),
+    "phx_public_key_load": (ctypes.c_int, [_vp, _vp, _sz, ctypes.POINTER(_vp)]),
+    "phx_encrypt_asymmetric": (ctypes.c_int, [_vp, _vp, _u64p, _sz, ctypes.c_uint64, _vp, _vp]),
+    "phx_encrypt_asymmetric_with_samples": (ctypes.c_int, [_vp, _vp, _u64p, _sz, ctypes.POINTER(ctypes.c_int8),
+                                                           ctypes.POINTER(ctypes.c_int8),
+                                                           ctypes.POINTER(ctypes.c_int8), ctypes.c_uint64, _vp, _vp]),
     "phx_relin_key_save_size": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_sz)]),
     "phx_relin_key_save": (ctypes.c_int, [_vp, _vp, _vp, _sz]),
     "phx_relin_key_load": (ctypes.c_int, [_vp, _vp, _sz, ctypes.POINTER(_vp)]),
@@ -529,6 +539,10 @@ class secret_key:
                                                   _stream(stream)))
         return ct
 
+    def gen_publickey(self, ctx, seed_a=None, seed_e=6):
+        """PhantomSecretKey::gen_publickey: pk = (-(a s + e), a) at the key level"""
+        return public_key(ctx, self, seed_a=seed_a, seed_e=seed_e)
+
     def save(self):
         """PhantomSecretKey::save byte stream"""
         n = _sz()
@@ -554,6 +568,67 @@ class secret_key:
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.phx_secret_key_destroy(self._h)
+            self._h = None
+
+
+class public_key:
+    """PhantomPublicKey (include/secretkey.h:17-100): (-(a s + e), a) over QP, NTT form."""
+
+    def __init__(self, ctx, sk, seed_a=None, seed_e=6):
+        if seed_a is not None and len(seed_a) != 64:
+            raise ValueError("the seed is 64 bytes")
+        h = _vp()
+        _check(lib().phx_public_key_create(ctx._h, sk._h, None if seed_a is None else bytes(seed_a), seed_e,
+                                           ctypes.byref(h)))
+        self._h, self._ctx = h, ctx
+
+    def data(self):
+        out = np.zeros((2, self._ctx.size_QP, self._ctx.N), np.uint64)
+        _check(lib().phx_public_key_download(self._ctx._h, self._h, _p(out)))
+        return out
+
+    def encrypt_asymmetric(self, ctx, plain, chain_index=1, seed=7, u=None, e=None, scale=None, stream=None):
+        """PhantomPublicKey::encrypt_asymmetric. plain: CKKS coefficient residues [size_Q][N]
+        (ckks_encoder.encode), BFV N coefficients mod t. u, e = (e0, e1): optional int8 samples.
+        scale: the CKKS plaintext's scale, carried by the ciphertext."""
+        a = np.ascontiguousarray(plain, np.uint64)
+        ct = ciphertext()
+        if u is None:
+            _check(lib().phx_encrypt_asymmetric(ctx._h, self._h, _p(a), chain_index, seed, ct._h, _stream(stream)))
+        else:
+            i8 = ctypes.POINTER(ctypes.c_int8)
+            uu = np.ascontiguousarray(u, np.int8)
+            e0 = np.ascontiguousarray(e[0], np.int8)
+            e1 = np.ascontiguousarray(e[1], np.int8)
+            for v in (uu, e0, e1):
+                if v.size != ctx.N:
+                    raise ValueError("samples are N int8 coefficients each")
+            _check(lib().phx_encrypt_asymmetric_with_samples(ctx._h, self._h, _p(a), chain_index,
+                                                             uu.ctypes.data_as(i8), e0.ctypes.data_as(i8),
+                                                             e1.ctypes.data_as(i8), seed, ct._h, _stream(stream)))
+        if scale is not None:
+            ct.set_scale(scale)
+        return ct
+
+    def save(self):
+        """PhantomPublicKey::save byte stream"""
+        n = _sz()
+        _check(lib().phx_public_key_save_size(self._h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().phx_public_key_save(self._h, buf, n.value))
+        return buf.raw
+
+    @classmethod
+    def load(cls, ctx, data):
+        k = cls.__new__(cls)
+        h = _vp()
+        _check(lib().phx_public_key_load(ctx._h, data, len(data), ctypes.byref(h)))
+        k._h, k._ctx = h, ctx
+        return k
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.phx_public_key_destroy(self._h)
             self._h = None
 
 
