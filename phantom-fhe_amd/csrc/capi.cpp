@@ -103,6 +103,16 @@ void put_choice(const TuneChoice &t, int c[4]) {
     c[2] = t.ntt_chunk_limbs;
     c[3] = t.bconv_mfma;
 }
+// A ciphertext handed to an evaluator entry point must belong to the context:
+// its ring degree and limb count are the context's at its chain index (the
+// kernels index it by the context's N and level sizes).
+void check_ct(const Context &c, const Ciphertext &ct, const char *what) {
+    require(ct.data != nullptr, std::string(what) + " is empty");
+    require(ct.poly_degree == c.N(), std::string(what) + ": poly_modulus_degree does not match the context");
+    require(ct.chain_index >= 1 && ct.chain_index <= c.size_Q() &&
+                ct.coeff_modulus_size == c.limbs_at(ct.chain_index),
+            std::string(what) + ": chain index / coeff_modulus_size do not match the context");
+}
 }  // namespace
 
 extern "C" {
@@ -445,6 +455,7 @@ int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext
         nn(sk, "secret key");
         nn(ct, "ciphertext");
         nn(out, "out");
+        check_ct(*ctx->ctx, ct->ct, "ciphertext");
         const Context &c = *ctx->ctx;
         const size_t n = (size_t)ct->ct.coeff_modulus_size * c.N();
         Scratch d_buf(n * 8, S(s));
@@ -763,6 +774,8 @@ int phx_multiply_and_relin_inplace(const phx_context *ctx, phx_ciphertext *ct1, 
         nn(ct1, "encrypted1");
         nn(ct2, "encrypted2");
         nn(rlk, "relin_keys");
+        check_ct(*ctx->ctx, ct1->ct, "encrypted1");
+        check_ct(*ctx->ctx, ct2->ct, "encrypted2");
         multiply_and_relin(*ctx->ctx, ct1->ct, ct2->ct, rlk->key, S(s));
     });
 }
@@ -772,6 +785,8 @@ int phx_multiply_inplace(const phx_context *ctx, phx_ciphertext *ct1, const phx_
         nn(ctx, "context");
         nn(ct1, "encrypted1");
         nn(ct2, "encrypted2");
+        check_ct(*ctx->ctx, ct1->ct, "encrypted1");
+        check_ct(*ctx->ctx, ct2->ct, "encrypted2");
         Ciphertext &a = ct1->ct;
         const Ciphertext &b = ct2->ct;
         if (ctx->ctx->params().scheme == kSchemeBFV) {
@@ -802,6 +817,7 @@ int phx_relinearize_inplace(const phx_context *ctx, phx_ciphertext *ct, const ph
         nn(ctx, "context");
         nn(ct, "encrypted");
         nn(rlk, "relin_keys");
+        check_ct(*ctx->ctx, ct->ct, "encrypted");
         relinearize(*ctx->ctx, ct->ct, rlk->key, S(s));
     });
 }
@@ -811,6 +827,7 @@ int phx_rescale_to_next(const phx_context *ctx, const phx_ciphertext *in, phx_ci
         nn(ctx, "context");
         nn(in, "encrypted");
         nn(out, "destination");
+        check_ct(*ctx->ctx, in->ct, "encrypted");
         if (in == out) {
             Ciphertext tmp = in->ct;
             Ciphertext res;
@@ -829,6 +846,7 @@ int phx_apply_galois_inplace(const phx_context *ctx, phx_ciphertext *ct, uint32_
         nn(ctx, "context");
         nn(ct, "encrypted");
         nn(gk, "galois_keys");
+        check_ct(*ctx->ctx, ct->ct, "encrypted");
         apply_galois(*ctx->ctx, ct->ct, elt, gk->gk, S(s));
     });
 }
@@ -838,6 +856,7 @@ int phx_rotate_inplace(const phx_context *ctx, phx_ciphertext *ct, int step, con
         nn(ctx, "context");
         nn(ct, "encrypted");
         nn(gk, "galois_keys");
+        check_ct(*ctx->ctx, ct->ct, "encrypted");
         rotate(*ctx->ctx, ct->ct, step, gk->gk, S(s));
     });
 }
@@ -849,6 +868,7 @@ int phx_hoisting_inplace(const phx_context *ctx, phx_ciphertext *ct, const phx_g
         nn(ct, "encrypted");
         nn(gk, "Galois keys");
         nn(steps, "steps");
+        check_ct(*ctx->ctx, ct->ct, "encrypted");
         hoisting(*ctx->ctx, ct->ct, gk->gk, std::vector<int>(steps, steps + n), S(s));
     });
 }
@@ -860,6 +880,7 @@ int phx_keyswitch_inplace(const phx_context *ctx, phx_ciphertext *ct, const uint
         nn(ct, "encrypted");
         nn(c2, "c2");
         nn(key, "key");
+        check_ct(*ctx->ctx, ct->ct, "encrypted");
         require(ct->ct.size >= 2, "ciphertext must have two polynomials");
         const u64 LN = (u64)ct->ct.coeff_modulus_size * ctx->ctx->N();
         keyswitch(*ctx->ctx, ct->ct.chain_index, c2, key->key, ct->ct.data, ct->ct.data, ct->ct.data + LN, S(s));
@@ -880,6 +901,8 @@ int phx_batch_multiply_and_relin_create(const phx_context *ctx, phx_ciphertext *
         for (size_t i = 0; i < n; ++i) {
             nn(ct1[i], "ct1[i]");
             nn(ct2[i], "ct2[i]");
+            check_ct(*ctx->ctx, ct1[i]->ct, "ct1[i]");
+            check_ct(*ctx->ctx, ct2[i]->ct, "ct2[i]");
             a.push_back(&ct1[i]->ct);
             b.push_back(&ct2[i]->ct);
         }
@@ -899,6 +922,7 @@ int phx_batch_rotate_create(const phx_context *ctx, phx_ciphertext *const *cts, 
         std::vector<Ciphertext *> a;
         for (size_t i = 0; i < n; ++i) {
             nn(cts[i], "cts[i]");
+            check_ct(*ctx->ctx, cts[i]->ct, "cts[i]");
             a.push_back(&cts[i]->ct);
         }
         PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));

@@ -370,3 +370,27 @@ def test_max_degree_multiply_relin_rotate_rescale():
     c1.set_scale(2.0 ** 40)
     r = ph.rescale_to_next(ctx, c1)
     assert np.array_equal(r.download(N), octx.rescale(Ql, rot))
+
+
+def test_ciphertext_of_another_context_rejected(small):
+    """a ciphertext whose ring degree or limb count is not the context's is refused
+    on the host before any kernel reads it"""
+    ctx, octx, mods, key = small
+    say("case: foreign ciphertext rejected")
+    other, _, omods = make_ctx(2 * SMALL["N"], SMALL["bits"], SMALL["P"])
+    rng = np.random.default_rng(5)
+    foreign = upload_ct(other, rand_ct(rng, omods[:6], 2 * SMALL["N"]), 1)
+    mine = upload_ct(ctx, rand_ct(rng, mods[:6], SMALL["N"]), 1)
+    rk = ph.relin_key(ctx, host=key)
+    with pytest.raises(ValueError):
+        ph.multiply_and_relin_inplace(ctx, mine, foreign, rk)
+    with pytest.raises(ValueError):
+        ph.relinearize_inplace(ctx, foreign, rk)
+    with pytest.raises(ValueError):
+        ph.rescale_to_next(ctx, foreign)
+    empty = ph.ciphertext()
+    with pytest.raises(ValueError):
+        ph.multiply_inplace(ctx, mine, empty)
+    # the valid operand is untouched by the refused calls
+    ph.multiply_and_relin_inplace(ctx, mine, mine, rk)
+    assert mine.size() == 2
