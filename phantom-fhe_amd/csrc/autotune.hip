@@ -115,6 +115,8 @@ TuneResult autotune(Context &ctx, hipStream_t s) {
     }
     KSwitchKey rlk;
     rlk.dnum = dnum;
+    rlk.poly_degree = N;
+    rlk.size_QP = QP;
     rlk.data = static_cast<u64 *>(dev_alloc((size_t)dnum * 2 * QP * N * 8, s));
     fill(rlk.data, ctx, QP, 2 * dnum, 7, s);
     auto release = [&] {

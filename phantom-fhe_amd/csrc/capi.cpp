@@ -113,6 +113,16 @@ void check_ct(const Context &c, const Ciphertext &ct, const char *what) {
                 ct.coeff_modulus_size == c.limbs_at(ct.chain_index),
             std::string(what) + ": chain index / coeff_modulus_size do not match the context");
 }
+// a key-switching key of this context: its ring degree and QP limb count, and a
+// digit for every digit the context's key switch reads
+void check_key(const Context &c, const KSwitchKey &k, const char *what) {
+    require(k.data != nullptr, std::string(what) + " is empty");
+    require(k.poly_degree == c.N() && k.size_QP == c.size_QP() && k.dnum >= c.dnum(),
+            std::string(what) + " does not match the context (ring degree, QP limbs or dnum)");
+}
+void check_keys(const Context &c, const GaloisKeys &g) {
+    for (const KSwitchKey &k : g.keys) check_key(c, k, "Galois key");
+}
 }  // namespace
 
 extern "C" {
@@ -340,6 +350,8 @@ int phx_relin_key_create_from_host(const phx_context *ctx, const uint64_t *host,
         const size_t n = dnum * 2 * (size_t)ctx->ctx->size_QP() * ctx->ctx->N();
         auto k = std::make_unique<phx_relin_key>();
         k->key.dnum = (u32)dnum;
+        k->key.poly_degree = ctx->ctx->N();
+        k->key.size_QP = ctx->ctx->size_QP();
         k->key.data = static_cast<u64 *>(dev_alloc(n * 8, hipStreamPerThread));
         PHX_HIP_CHECK(hipMemcpyAsync(k->key.data, host, n * 8, hipMemcpyHostToDevice, hipStreamPerThread));
         PHX_HIP_CHECK(hipStreamSynchronize(hipStreamPerThread));
@@ -397,6 +409,8 @@ int phx_galois_key_create_from_host(const phx_context *ctx, const uint32_t *elts
             k->gk.keys.emplace_back();
             KSwitchKey &key = k->gk.keys.back();
             key.dnum = (u32)dnum;
+            key.poly_degree = ctx->ctx->N();
+            key.size_QP = ctx->ctx->size_QP();
             key.data = static_cast<u64 *>(dev_alloc(per * 8, hipStreamPerThread));
             PHX_HIP_CHECK(hipMemcpyAsync(key.data, host + i * per, per * 8, hipMemcpyHostToDevice, hipStreamPerThread));
         }
@@ -776,6 +790,7 @@ int phx_multiply_and_relin_inplace(const phx_context *ctx, phx_ciphertext *ct1, 
         nn(rlk, "relin_keys");
         check_ct(*ctx->ctx, ct1->ct, "encrypted1");
         check_ct(*ctx->ctx, ct2->ct, "encrypted2");
+        check_key(*ctx->ctx, rlk->key, "relin_keys");
         multiply_and_relin(*ctx->ctx, ct1->ct, ct2->ct, rlk->key, S(s));
     });
 }
@@ -818,6 +833,7 @@ int phx_relinearize_inplace(const phx_context *ctx, phx_ciphertext *ct, const ph
         nn(ct, "encrypted");
         nn(rlk, "relin_keys");
         check_ct(*ctx->ctx, ct->ct, "encrypted");
+        check_key(*ctx->ctx, rlk->key, "relin_keys");
         relinearize(*ctx->ctx, ct->ct, rlk->key, S(s));
     });
 }
@@ -847,6 +863,7 @@ int phx_apply_galois_inplace(const phx_context *ctx, phx_ciphertext *ct, uint32_
         nn(ct, "encrypted");
         nn(gk, "galois_keys");
         check_ct(*ctx->ctx, ct->ct, "encrypted");
+        check_keys(*ctx->ctx, gk->gk);
         apply_galois(*ctx->ctx, ct->ct, elt, gk->gk, S(s));
     });
 }
@@ -857,6 +874,7 @@ int phx_rotate_inplace(const phx_context *ctx, phx_ciphertext *ct, int step, con
         nn(ct, "encrypted");
         nn(gk, "galois_keys");
         check_ct(*ctx->ctx, ct->ct, "encrypted");
+        check_keys(*ctx->ctx, gk->gk);
         rotate(*ctx->ctx, ct->ct, step, gk->gk, S(s));
     });
 }
@@ -869,6 +887,7 @@ int phx_hoisting_inplace(const phx_context *ctx, phx_ciphertext *ct, const phx_g
         nn(gk, "Galois keys");
         nn(steps, "steps");
         check_ct(*ctx->ctx, ct->ct, "encrypted");
+        check_keys(*ctx->ctx, gk->gk);
         hoisting(*ctx->ctx, ct->ct, gk->gk, std::vector<int>(steps, steps + n), S(s));
     });
 }
@@ -881,6 +900,7 @@ int phx_keyswitch_inplace(const phx_context *ctx, phx_ciphertext *ct, const uint
         nn(c2, "c2");
         nn(key, "key");
         check_ct(*ctx->ctx, ct->ct, "encrypted");
+        check_key(*ctx->ctx, key->key, "key");
         require(ct->ct.size >= 2, "ciphertext must have two polynomials");
         const u64 LN = (u64)ct->ct.coeff_modulus_size * ctx->ctx->N();
         keyswitch(*ctx->ctx, ct->ct.chain_index, c2, key->key, ct->ct.data, ct->ct.data, ct->ct.data + LN, S(s));
@@ -896,6 +916,7 @@ int phx_batch_multiply_and_relin_create(const phx_context *ctx, phx_ciphertext *
         nn(ct2, "ct2");
         nn(rlk, "relin_keys");
         nn(out, "out");
+        check_key(*ctx->ctx, rlk->key, "relin_keys");
         std::vector<Ciphertext *> a;
         std::vector<const Ciphertext *> b;
         for (size_t i = 0; i < n; ++i) {
@@ -919,6 +940,7 @@ int phx_batch_rotate_create(const phx_context *ctx, phx_ciphertext *const *cts, 
         nn(steps, "steps");
         nn(gk, "galois_keys");
         nn(out, "out");
+        check_keys(*ctx->ctx, gk->gk);
         std::vector<Ciphertext *> a;
         for (size_t i = 0; i < n; ++i) {
             nn(cts[i], "cts[i]");
@@ -997,6 +1019,8 @@ int phx_server_create(const phx_context *const *ctxs, const phx_relin_key *const
             nn(rlks[i], "relinearisation key");
             nn(gks[i], "Galois keys");
             lanes.push_back({ctxs[i]->ctx, &rlks[i]->key, &gks[i]->gk});
+            check_key(*ctxs[i]->ctx, rlks[i]->key, "relinearisation key");
+            check_keys(*ctxs[i]->ctx, gks[i]->gk);
         }
         require(s != nullptr, "the server needs an explicit stream");
         *out = new phx_server{new Server(lanes, mem_budget, (u32)max_active, (u32)depth, S(s))};

@@ -37,6 +37,8 @@ struct Ciphertext {
 struct KSwitchKey {
     u32 dnum = 0;
     u64 *data = nullptr;
+    u64 poly_degree = 0;   // the context's N and QP limb count at creation
+    u32 size_QP = 0;
 };
 
 struct GaloisKeys {

@@ -944,6 +944,8 @@ void kswitch_key_generate(const Context &ctx, const SecretKey &sk, const u64 *ne
     const u32 QP = ctx.size_QP(), alpha = ctx.size_P(), dnum = ctx.dnum();
     const u64 QPN = (u64)QP * N;
     key.dnum = dnum;
+    key.poly_degree = N;
+    key.size_QP = QP;
     key.data = static_cast<u64 *>(dev_alloc((size_t)dnum * 2 * QPN * 8, s));
     Scratch e_buf(QPN * 8, s);
     u64 *e = e_buf.get();

@@ -189,6 +189,8 @@ size_t kswitch_key_load(const Context &ctx, KSwitchKey &key, const uint8_t *buf,
     if (key.data) dev_free(key.data, hipStreamPerThread);
     key.data = d;
     key.dnum = (u32)dnum;
+    key.poly_degree = ctx.N();
+    key.size_QP = ctx.size_QP();
     return (size_t)(p - buf);
 }
 

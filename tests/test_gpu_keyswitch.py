@@ -391,6 +391,11 @@ def test_ciphertext_of_another_context_rejected(small):
     empty = ph.ciphertext()
     with pytest.raises(ValueError):
         ph.multiply_inplace(ctx, mine, empty)
+    # a key of the other context (same dnum, other ring degree)
+    okey = rand_key(rng, omods, 2 * SMALL["N"], other.dnum)
+    ork = ph.relin_key(other, host=okey)
+    with pytest.raises(ValueError):
+        ph.multiply_and_relin_inplace(ctx, mine, mine, ork)
     # the valid operand is untouched by the refused calls
     ph.multiply_and_relin_inplace(ctx, mine, mine, rk)
     assert mine.size() == 2
