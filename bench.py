@@ -23,7 +23,7 @@ for p in (os.path.join(ROOT, "phantom-fhe_amd"), os.path.join(ROOT, "tests"), RO
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-CONFIG2_BITS = [60] + [50] * 34 + [60] * 12
+BITS_N16_Q35_P12 = [60] + [50] * 34 + [60] * 12
 
 
 def log(msg):
@@ -104,7 +104,7 @@ def bench_hmult_batched(ph, ctx, mods, N, batch, steps, warmup, stream):
     return ev0.elapsed_ms(ev1), bt.scratch_bytes()
 
 
-def bench_bfv_config4(ph, stream, n_ct=64, reps=3):
+def bench_bfv_mixed_rings(ph, stream, n_ct=64, reps=3):
     """BFV multiply + relin (BEHZ) over 64 ciphertexts with N uniform in
     {2^13, 2^14, 2^15} (4/8/16 limbs of 54-bit primes, t = 65537, P = 1): one
     multiply_and_relin_inplace call per ciphertext against one plan per ring
@@ -390,7 +390,7 @@ def main():
         ph.tune_set(k, int(v))
         tune[k] = int(v)
     N = 1 << 16
-    ctx, mods = make_context(ph, N, CONFIG2_BITS, 12)
+    ctx, mods = make_context(ph, N, BITS_N16_Q35_P12, 12)
     stream = ph.stream_create()
     autotune = None
     if not args.no_autotune and not args.tune and not args.ntt_only:
@@ -437,13 +437,13 @@ def main():
 
     rot = bench_rotations(ph, ctx, mods, N, stream)
     log(f"bench: rotations {rot}")
-    bfv4 = bench_bfv_config4(ph, stream)
-    log(f"bench: bfv config4 {bfv4}")
+    bfv4 = bench_bfv_mixed_rings(ph, stream)
+    log(f"bench: bfv mixed rings {bfv4}")
     serving = None
     if args.serve_requests > 0:
         lanes = []
         for n_ in (1 << 15, 1 << 16):
-            c_, m_ = (ctx, mods) if n_ == N else make_context(ph, n_, CONFIG2_BITS, 12)
+            c_, m_ = (ctx, mods) if n_ == N else make_context(ph, n_, BITS_N16_Q35_P12, 12)
             sk_ = ph.secret_key(c_, seed=90)
             lanes.append((c_, m_, sk_.gen_relinkey(c_, seed=91),
                           sk_.create_galois_keys(c_, [ph.galois_elt_from_step(st, n_) for st in ROT_STEPS], seed=92)))
@@ -455,7 +455,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (oracle) ...")
-        cpu = cpu_baseline(N, CONFIG2_BITS, 12, args.cpu_seconds)
+        cpu = cpu_baseline(N, BITS_N16_Q35_P12, 12, args.cpu_seconds)
 
     # algorithmic bytes of one HMult+relin with the key streamed per op:
     # (4*35 in + 2*35 out + 3*2*47 key) limbs x N x 8 B
@@ -491,7 +491,7 @@ def main():
         "cpu_baseline": cpu,
         "ckks_error_vs_fp64": ckks_err,
         "rotations": rot,
-        "bfv_config4": bfv4,
+        "bfv_mixed_rings": bfv4,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -104,11 +104,11 @@ def test_batch_rejects_missing_key_and_aliasing(small):
         ph.batch.multiply_and_relin(ctx, [x, y], [y, x], rk)
 
 
-def test_batch_config2_bit_exact():
+def test_batch_n16_q35_bit_exact():
     """N=2^16, 35 + 12 limbs, dnum 3: four pairs in one batch, graph replay."""
     ob.set_threads(16)
     N = 1 << 16
-    say("case: batch config2")
+    say("case: batch N=2^16, 35 + 12 limbs")
     ctx, octx, mods = make_ctx(N, [60] + [50] * 34 + [60] * 12, 12)
     rng = np.random.default_rng(11)
     key = rand_key(rng, mods, N, 3)
@@ -162,18 +162,18 @@ def test_batch_fused_inner_product_variant(small):
         ph.tune_set("ks_fused_ip", old)
 
 
-CONFIG3_STEPS = [1, 2, 3, 4, 5, 7, 8, 16, 32, 64, 128, 256, 512, 1024, -1, -5]
+ROTATION_STEPS = [1, 2, 3, 4, 5, 7, 8, 16, 32, 64, 128, 256, 512, 1024, -1, -5]
 
 
-def test_config3_batched_rotations_bit_exact():
+def test_128_ciphertexts_batched_rotations_bit_exact():
     """128 ciphertexts at N=2^16 with 5..35 limbs, each rotated by a step drawn
     from 16 resident rotation keys, in one captured batch plan; sampled items
     (smallest and largest level among them) bit-exact against the oracle."""
     ob.set_threads(16)
     N = 1 << 16
-    say("case: config3 batched rotations (128 cts, 16 keys)")
+    say("case: batched rotations (128 cts, 16 keys)")
     ctx, octx, mods = make_ctx(N, [60] + [50] * 34 + [60] * 12, 12)
-    elts = [ph.galois_elt_from_step(s, N) for s in CONFIG3_STEPS]
+    elts = [ph.galois_elt_from_step(s, N) for s in ROTATION_STEPS]
     rng = np.random.default_rng(33)
     key_seeds = [1000 + i for i in range(len(elts))]
 
@@ -190,7 +190,7 @@ def test_config3_batched_rotations_bit_exact():
         a = rand_ct(np.random.default_rng(data_seeds[i]), mods[:int(limbs[i])], N)
         cts.append(upload_ct(ctx, a, chains[i]))
     stream = ph.stream_create()
-    bt = ph.batch.rotate(ctx, cts, [CONFIG3_STEPS[k] for k in steps_idx], gk)
+    bt = ph.batch.rotate(ctx, cts, [ROTATION_STEPS[k] for k in steps_idx], gk)
     bt.capture(stream)
     bt.replay(stream)
     ph.synchronize(stream)
@@ -200,7 +200,7 @@ def test_config3_batched_rotations_bit_exact():
         a = rand_ct(np.random.default_rng(data_seeds[i]), mods[:Ql], N)
         k = int(steps_idx[i])
         want = octx.apply_galois(Ql, a, elts[k], key_of(k))
-        assert np.array_equal(cts[i].download(N), want), (i, Ql, CONFIG3_STEPS[k])
+        assert np.array_equal(cts[i].download(N), want), (i, Ql, ROTATION_STEPS[k])
 
 
 def test_joined_batch_two_ring_degrees():

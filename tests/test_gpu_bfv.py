@@ -85,11 +85,11 @@ def test_bfv_rejects_ntt_form_and_wrong_scheme(small):
         ph.multiply_inplace(ctx, c, c)
 
 
-def test_config4_batch_bit_exact():
+def test_bfv_64_ciphertexts_batch_bit_exact():
     """64 ciphertexts, N uniform in {2^13, 2^14, 2^15} (4/8/16 limbs), multiply
     + relinearise each; items of every ring degree checked against the oracle."""
     ob.set_threads(16)
-    say("case: config4 64 bfv ciphertexts")
+    say("case: 64 bfv ciphertexts")
     rng = np.random.default_rng(44)
     ctxs = {N: make_bfv(N, k) for N, k in LIMBS.items()}
     keys = {N: rand_key(np.random.default_rng(N), v[3], N, v[0].dnum) for N, v in ctxs.items()}
@@ -252,13 +252,13 @@ def test_bfv_batch_rejects():
         ph.batch.multiply_and_relin(ctx, [a], [b], rk)
 
 
-def test_config4_joined_batch_graph():
-    """config 4 in batched form: 64 BFV ciphertexts, N uniform in {2^13, 2^14, 2^15}
+def test_bfv_mixed_rings_joined_batch_graph():
+    """Batched form of 64 BFV ciphertexts, N uniform in {2^13, 2^14, 2^15}
     (4/8/16 limbs), one BEHZ plan per ring degree joined into one captured graph;
     every item equals the plain path, a sample equals the oracle; replayed twice
     with eager work (re-uploads) between the replays."""
     ob.set_threads(16)
-    say("case: config4 joined bfv batch")
+    say("case: joined bfv batch over three ring degrees")
     rng = np.random.default_rng(404)
     ctxs = {N: make_bfv(N, k) for N, k in LIMBS.items()}
     keys = {N: rand_key(np.random.default_rng(N + 1), v[3], N, v[0].dnum) for N, v in ctxs.items()}

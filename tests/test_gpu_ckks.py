@@ -113,9 +113,9 @@ def test_hmult_relin_rescale_rotate_error_small():
     assert e["fresh"] < 1e-8 and e["mul_relin"] < 1e-8 and e["rescale"] < 1e-5 and e["rotate"] < 1e-5
 
 
-def test_config2_error_report():
+def test_n16_q35_error_report():
     """N=2^16, 35 data limbs (60 + 34 x 50 bit), 12 special, dnum 3, scale 2^50."""
-    say("case: ckks chain error config2")
+    say("case: ckks chain error N=2^16")
     ctx, mods = make(1 << 16, [60] + [50] * 34 + [60] * 12, 12)
     e, chain = chain_error(ctx, mods, 2.0 ** 50, 5, 20)
     say("ckks error N=2^16 L=35+12 scale=2^50: " + ", ".join(f"{k} {v:.3e}" for k, v in e.items()))

@@ -127,25 +127,25 @@ def test_missing_galois_key_raises(small):
 
 
 # ---------------------------------------------------------------- full size
-CONFIG2_BITS = [60] + [50] * 34 + [60] * 12
+BITS_N16_Q35_P12 = [60] + [50] * 34 + [60] * 12
 
 
 @pytest.fixture(scope="module")
-def config2():
+def n16_q35():
     ob.set_threads(16)
     N = 1 << 16
-    ctx, octx, mods = make_ctx(N, CONFIG2_BITS, 12)
+    ctx, octx, mods = make_ctx(N, BITS_N16_Q35_P12, 12)
     assert ctx.size_Q == 35 and ctx.dnum == 3
     rng = np.random.default_rng(2026)
     key = rand_key(rng, mods, N, 3)
     return ctx, octx, mods, key
 
 
-def test_config2_multiply_and_relin_bit_exact(config2):
+def test_n16_q35_multiply_and_relin_bit_exact(n16_q35):
     """N=2^16, 35 data limbs + 12 special, dnum = 3 (digits 12, 12, 11)."""
-    ctx, octx, mods, key = config2
+    ctx, octx, mods, key = n16_q35
     N = ctx.N
-    say("case: config2 mulrelin top level")
+    say("case: N=2^16 mulrelin top level")
     rng = np.random.default_rng(5)
     a, b = rand_ct(rng, mods[:35], N), rand_ct(rng, mods[:35], N)
     rk = ph.relin_key(ctx, host=key)
@@ -154,12 +154,12 @@ def test_config2_multiply_and_relin_bit_exact(config2):
     assert np.array_equal(c1.download(N), octx.multiply_and_relin(35, a, b, key))
 
 
-def test_config2_lower_level_and_rotation(config2):
-    ctx, octx, mods, key = config2
+def test_n16_q35_lower_level_and_rotation(n16_q35):
+    ctx, octx, mods, key = n16_q35
     N = ctx.N
     chain = 16   # 20 limbs: digits 12, 8
     Ql = ctx.limbs_at(chain)
-    say(f"case: config2 chain={chain} mulrelin + galois")
+    say(f"case: N=2^16 chain={chain} mulrelin + galois")
     rng = np.random.default_rng(6)
     a, b = rand_ct(rng, mods[:Ql], N), rand_ct(rng, mods[:Ql], N)
     rk = ph.relin_key(ctx, host=key)
@@ -173,10 +173,10 @@ def test_config2_lower_level_and_rotation(config2):
     assert np.array_equal(c3.download(N), octx.apply_galois(Ql, a, elt, key))
 
 
-def test_config2_rescale(config2):
-    ctx, octx, mods, key = config2
+def test_n16_q35_rescale(n16_q35):
+    ctx, octx, mods, key = n16_q35
     N = ctx.N
-    say("case: config2 rescale")
+    say("case: N=2^16 rescale")
     a = rand_ct(np.random.default_rng(8), mods[:35], N)
     c = upload_ct(ctx, a, 1)
     r = ph.rescale_to_next(ctx, c)

@@ -19,9 +19,9 @@ def run_ntt(ctx, data, start, inverse=False):
     return buf.to_array(data.shape)
 
 
-def test_config1_ntt_then_intt_bit_exact():
+def test_n12_three_60bit_ntt_then_intt_bit_exact():
     """N=2^12, 3 limbs of the largest 60-bit primes = 1 mod 2N, uniform input."""
-    say("case: config1 N=4096 L=3")
+    say("case: N=4096 L=3")
     N = 4096
     ctx, octx, mods = make_ctx(N, [60, 60, 60], 1)
     rng = np.random.default_rng(1)

@@ -36,7 +36,7 @@ def test_get_primes_matches_scan():
         assert ob.get_primes(N, bits, cnt) == br.primes_below(bits, N, cnt)
 
 
-def test_create_ordering_config2():
+def test_create_ordering_n16_q35():
     N = 1 << 16
     bits = [60] + [50] * 34 + [60] * 12
     got = ob.coeff_modulus_create(N, bits)
@@ -48,7 +48,7 @@ def test_create_ordering_config2():
     assert len(set(got)) == len(got)
 
 
-def test_create_config1_is_three_largest_ascending():
+def test_create_n12_three_60bit_is_largest_ascending():
     got = ob.coeff_modulus_create(4096, [60, 60, 60])
     assert got == sorted(br.primes_below(60, 4096, 3))
     assert all(q % 8192 == 1 and q < (1 << 60) for q in got)
@@ -101,8 +101,8 @@ def test_ntt_closed_form_small(bits):
         assert [int(x) for x in out[i]] == br.ntt_closed(a[i], q, psi, logN)
 
 
-def test_ntt_config1_sampled_closed_form_and_roundtrip():
-    """Config 1: N=2^12, the 3 largest 60-bit primes = 1 mod 2N, uniform input."""
+def test_ntt_n12_three_60bit_sampled_closed_form_and_roundtrip():
+    """N=2^12, the 3 largest 60-bit primes = 1 mod 2N, uniform input."""
     logN, N = 12, 4096
     qs = ob.coeff_modulus_create(N, [60, 60, 60])
     ctx = ob.Ctx(logN, qs, 1)
