@@ -6,7 +6,7 @@ multiplied and relinearised with the same resident relinearisation key
 (plain path: one ciphertext pair per call). Inputs are synthetic, seeded,
 uniform residues already resident in HBM when timing starts.
 
-Prints one JSON line (rank 0). `--gpus N` with N > 1 exits: this build runs
+Prints one JSON line (rank 0). `--gpus N` with N > 1 exits: the library runs
 on a single GPU.
 """
 import argparse
@@ -379,7 +379,7 @@ def main():
                     help="kernel-variant switch (bconv_mfma, ntt_fp64, ntt_fused, ntt_chunk_limbs, ...)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "the library runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
     import pyphantom as ph
