@@ -137,6 +137,12 @@ int phx_sample_uniform_poly(const phx_context *ctx, const uint8_t *seed, size_t 
 /* PhantomSecretKey::ckks_decrypt (src/secretkey.cu:533-569) followed by the INTT: out = [Ql][N] */
 int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext *ct, uint64_t *out_host,
                 phx_stream stream);
+/* PhantomSecretKey::decrypt of a BFV ciphertext (bfv_decrypt, src/secretkey.cu:571-635): the
+ * plaintext's N coefficients (host), scale-and-round by BEHZ for mul_tech behz
+ * (src/rns.cu:1008-1090), by HPS otherwise (src/rns.cu:1519-1690, its four forms chosen as the
+ * reference chooses them); coefficient-form ciphertexts at the first data level */
+int phx_decrypt_bfv(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext *ct, uint64_t *plain,
+                    phx_stream stream);
 
 /* ---- serialization: the reference's stream layouts, into / out of caller buffers.
  * Ciphertext (PhantomCiphertext::save / load, include/ciphertext.h:173-216): 58-byte

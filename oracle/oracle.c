@@ -1588,3 +1588,88 @@ void orc_sample_uniform_seeded(const orc_ctx *c, const uint8_t seed[64], size_t 
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* BFV decryption scale-and-round                                      */
+/* ------------------------------------------------------------------ */
+
+/* PhantomSecretKey::bfv_decrypt after the phase (src/secretkey.cu:571-635):
+ * phase = canonical residues [size_Q][N] of c0 + c1 s (+ c2 s^2), coefficient
+ * form; out[N] = the plaintext. behz != 0: behz_decrypt_scale_and_round
+ * (src/rns.cu:1008-1090; gamma = the largest 61-bit prime = 1 mod 2N, constants
+ * src/rns.cu:330-390), with the final centring of perform_final_multiplication
+ * (984-1006). Otherwise HPS (hps_decrypt_scale_and_round, src/rns.cu:1634-1690,
+ * kernels 1519-1633; constants 583-673): FP64 sums with fused multiply-adds as
+ * the device compiler contracts them, the form picked from qMSB, tMSB and the
+ * bit count of size_Q exactly as the reference picks it. */
+int orc_bfv_decrypt(const orc_ctx *c, u64 t, int behz, const u64 *phase, u64 *out) {
+    const size_t N = c->N, k = c->size_Q;
+    const u64 *q = c->q;
+    if (behz) {
+        u64 g;
+        if (orc_get_primes(N, 61, 1, &g) != 0) return -1;
+        u64 neg_t = invmod(prod_mod(q, k, t), t), neg_g = invmod(prod_mod(q, k, g), g);
+        neg_t = neg_t ? t - neg_t : 0;
+        neg_g = neg_g ? g - neg_g : 0;
+        const u64 ginv_t = invmod(g % t, t);
+        for (size_t n = 0; n < N; n++) {
+            u64 xt = 0, xg = 0;
+            for (size_t i = 0; i < k; i++) {
+                const u64 x = mulmod(phase[i * N + n], mulmod(t % q[i], g % q[i], q[i]), q[i]);
+                const u64 y = mulmod(x, invmod(punctured_mod(q, k, i, q[i]), q[i]), q[i]);
+                xt = addmod(xt, mulmod(y, punctured_mod(q, k, i, t), t), t);
+                xg = addmod(xg, mulmod(y, punctured_mod(q, k, i, g), g), g);
+            }
+            xt = mulmod(xt, neg_t, t);
+            xg = mulmod(xg, neg_g, g);
+            const u64 v = xg > (g >> 1) ? addmod(xt, (g - xg) % t, t) : submod(xt, xg % t, t);
+            out[n] = mulmod(v, ginv_t, t);
+        }
+        return 0;
+    }
+    u64 qmax = 0;
+    for (size_t i = 0; i < k; i++) qmax = q[i] > qmax ? q[i] : qmax;
+    const int qmsb = bit_count(qmax), sizeqmsb = bit_count((u64)k), tmsb = bit_count(t);
+    const int large = qmsb + sizeqmsb >= 52;
+    const int h = qmsb >> 1;
+    const int lazy = large ? (h + tmsb + sizeqmsb < 52) : (qmsb + tmsb + sizeqmsb < 52);
+    u64 *om = (u64 *)malloc(k * 8), *omb = (u64 *)malloc(k * 8);
+    double *fr = (double *)malloc(k * 8), *frb = (double *)malloc(k * 8);
+    for (size_t i = 0; i < k; i++) {
+        const u64 v = invmod(punctured_mod(q, k, i, q[i]), q[i]);
+        const u128 big = (u128)t * v;
+        om[i] = (u64)((big / q[i]) % t);
+        fr[i] = (double)(u64)(big % q[i]) / (double)q[i];
+        const u64 vb = (u64)(((u128)v << h) % q[i]);
+        const u128 bigb = (u128)t * vb;
+        omb[i] = (u64)((bigb / q[i]) % t);
+        frb[i] = (double)(u64)(bigb % q[i]) / (double)q[i];
+    }
+    const double tinv = 1.0 / (double)t;
+    for (size_t n = 0; n < N; n++) {
+        double fs = 0.0;
+        u64 is = 0;
+        for (size_t i = 0; i < k; i++) {
+            const u64 x = phase[i * N + n];
+            if (!large) {
+                fs = fma((double)x, fr[i], fs);
+                is += lazy ? x * om[i] : mulmod(x, om[i], t);
+            } else {
+                const u64 hi = x >> h, lo = x & (((u64)1 << h) - 1);
+                fs = fma((double)lo, fr[i], fs);
+                fs = fma((double)hi, frb[i], fs);
+                is += lazy ? lo * om[i] : mulmod(lo, om[i], t);
+                is += lazy ? hi * omb[i] : mulmod(hi, omb[i], t);
+            }
+        }
+        fs += (double)is;
+        const u64 quot = (u64)(fs * tinv);
+        fs -= (double)(t * quot);
+        out[n] = (u64)llround(fs);
+    }
+    free(om);
+    free(omb);
+    free(fr);
+    free(frb);
+    return 0;
+}

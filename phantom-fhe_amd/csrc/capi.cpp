@@ -480,6 +480,22 @@ int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext
     });
 }
 
+int phx_decrypt_bfv(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext *ct, uint64_t *plain,
+                    phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(sk, "secret key");
+        nn(ct, "ciphertext");
+        nn(plain, "plain");
+        check_ct(*ctx->ctx, ct->ct, "ciphertext");
+        const size_t N = ctx->ctx->N();
+        Scratch d_buf(N * 8, S(s));
+        bfv_decrypt(*ctx->ctx, sk->sk, ct->ct, d_buf.get(), S(s));
+        PHX_HIP_CHECK(hipMemcpyAsync(plain, d_buf.get(), N * 8, hipMemcpyDeviceToHost, S(s)));
+        PHX_HIP_CHECK(hipStreamSynchronize(S(s)));
+    });
+}
+
 int phx_ciphertext_save_size(const phx_ciphertext *ct, size_t *bytes) {
     return guard([&] {
         nn(ct, "ciphertext");

@@ -568,6 +568,71 @@ Context::Context(const Params &p) : params_(p) {
         }
         tinv_off = cb.push64(ti);
     }
+    // BFV decryption scale-and-round at the first data level (PhantomSecretKey::bfv_decrypt,
+    // src/secretkey.cu:571-635): BEHZ for mul_tech behz (constants src/rns.cu:330-390), HPS
+    // otherwise (constants src/rns.cu:583-673, kernel choice 1634-1690)
+    size_t dec_off = 0;
+    if (p.scheme == kSchemeBFV && p.plain_modulus > 1) {
+        const u64 t = p.plain_modulus;
+        BfvDecrypt &D = bfv_dec_;
+        D.t = t;
+        hm::barrett_ratio(t, D.t_ratio_lo, D.t_ratio_hi);
+        std::vector<u64> tab((size_t)size_Q_ * 8, 0);
+        if (p.mul_tech == kMulBEHZ) {
+            const u64 g = hm::get_primes(N_, 61, 1)[0];
+            D.kind = kBfvDecBEHZ;
+            D.gamma = g;
+            hm::barrett_ratio(g, D.g_ratio_lo, D.g_ratio_hi);
+            const u64 qt = hm::product_mod(qp, size_Q_, t), qg = hm::product_mod(qp, size_Q_, g);
+            const u64 it = hm::invmod(qt, t), ig = hm::invmod(qg, g);
+            D.neg_qinv_t = it ? t - it : 0;
+            D.neg_qinv_g = ig ? g - ig : 0;
+            D.neg_qinv_t_shoup = hm::shoup(D.neg_qinv_t, t);
+            D.neg_qinv_g_shoup = hm::shoup(D.neg_qinv_g, g);
+            D.inv_gamma_t = hm::invmod(g % t, t);
+            D.inv_gamma_t_shoup = hm::shoup(D.inv_gamma_t, t);
+            for (u32 i = 0; i < size_Q_; ++i) {
+                const u64 q = qp[i];
+                const u64 tg = hm::mulmod(t % q, g % q, q);
+                const u64 hinv = hm::invmod(hm::punctured_mod(qp, size_Q_, i, q), q);
+                u64 *r = &tab[(size_t)i * 8];
+                r[0] = tg;
+                r[1] = hm::shoup(tg, q);
+                r[2] = hinv;
+                r[3] = hm::shoup(hinv, q);
+                r[4] = hm::punctured_mod(qp, size_Q_, i, t);
+                r[5] = hm::punctured_mod(qp, size_Q_, i, g);
+            }
+        } else {
+            u64 qmax = 0;
+            for (u32 i = 0; i < size_Q_; ++i) qmax = std::max(qmax, qp[i]);
+            const int qmsb = hm::bit_count(qmax), sizeqmsb = hm::bit_count(size_Q_), tmsb = hm::bit_count(t);
+            const bool large = qmsb + sizeqmsb >= 52;
+            D.qmsb_half = (u32)(qmsb >> 1);
+            if (!large)
+                D.kind = qmsb + tmsb + sizeqmsb < 52 ? kBfvDecHpsSmallLazy : kBfvDecHpsSmall;
+            else
+                D.kind = (int)D.qmsb_half + tmsb + sizeqmsb < 52 ? kBfvDecHpsLargeLazy : kBfvDecHpsLarge;
+            auto dbits = [](double d) { u64 b; std::memcpy(&b, &d, 8); return b; };
+            for (u32 i = 0; i < size_Q_; ++i) {
+                const u64 q = qp[i];
+                const u64 v = hm::invmod(hm::punctured_mod(qp, size_Q_, i, q), q);
+                u64 *r = &tab[(size_t)i * 8];
+                const u128 big = (u128)t * v;
+                r[0] = (u64)((big / q) % t);
+                r[1] = hm::shoup(r[0], t);
+                r[2] = dbits((double)(u64)(big % q) / (double)q);
+                if (large) {
+                    const u64 vb = (u64)(((u128)v << D.qmsb_half) % q);
+                    const u128 bigb = (u128)t * vb;
+                    r[3] = (u64)((bigb / q) % t);
+                    r[4] = hm::shoup(r[3], t);
+                    r[5] = dbits((double)(u64)(bigb % q) / (double)q);
+                }
+            }
+        }
+        dec_off = cb.push64(tab);
+    }
     // BEHZ constants (src/rns.cu:424-571)
     struct BehzOffs {
         size_t lift_f, qhat_bsk, qhat_mt, q_mod_bsk, mtinv_bsk, qhatinv_q, qinv_bsk, bhatinv_b, bhat_msk, bhat_q,
@@ -687,6 +752,7 @@ Context::Context(const Params &p) : params_(p) {
     }
     bigP_mod_q_ = reinterpret_cast<u64 *>(cbb + bp_off);
     if (tinv_off) bfv_tinv_q_ = reinterpret_cast<const u64 *>(cbb + tinv_off);
+    if (dec_off) bfv_dec_.tab = reinterpret_cast<const u64 *>(cbb + dec_off);
     for (size_t d = 0; d < hps_offs.size(); ++d) {
         const u32 k0 = size_Q_, Ql = k0 - (u32)d;
         HpsConsts h = resolve_hps(cbb, hps_offs[d], Ql, overq ? Ql : (u32)hps_r_.size(), overq, d ? k0 : Ql);

@@ -94,6 +94,22 @@ struct BehzConsts {
     const u64 *tscale_bsk;   // [nb][4]
 };
 
+// BFV decryption scale-and-round at the first data level, the reference's
+// choice of method (BEHZ for mul_tech behz; HPS otherwise, in its four forms).
+enum BfvDecKind { kBfvDecNone = 0, kBfvDecBEHZ, kBfvDecHpsSmallLazy, kBfvDecHpsSmall, kBfvDecHpsLargeLazy,
+                  kBfvDecHpsLarge };
+struct BfvDecrypt {
+    int kind = kBfvDecNone;
+    u32 qmsb_half = 0;    // HPS large forms: x = x_hi 2^h + x_lo with h = qMSB / 2
+    // per q_i, 8 words: BEHZ {[t gamma]_q, Shoup, Qhat^-1 mod q, Shoup, Qhat mod t, Qhat mod gamma, 0, 0};
+    // HPS {[t Qhat^-1 / q]_t, Shoup, frac (FP64 bits), the same for Qhat^-1 2^h, 0, 0}
+    const u64 *tab = nullptr;
+    u64 t = 0, t_ratio_lo = 0, t_ratio_hi = 0;
+    u64 gamma = 0, g_ratio_lo = 0, g_ratio_hi = 0;
+    u64 neg_qinv_t = 0, neg_qinv_t_shoup = 0, neg_qinv_g = 0, neg_qinv_g_shoup = 0;
+    u64 inv_gamma_t = 0, inv_gamma_t_shoup = 0;
+};
+
 // HPS constants of the top data level (BFV multiply with mul_tech hps;
 // reference DRNSTool ctor, src/rns.cu:686-783, and the BaseConverter tables of
 // src/host/rns.cu:438-497): R = size_Q + 1 primes below the smallest q, as
@@ -162,6 +178,7 @@ public:
     u64 bfv_negq_mod_t() const { return bfv_negq_mod_t_; }
     u64 bfv_negq_mod_t_shoup() const { return bfv_negq_mod_t_shoup_; }
     const u64 *bfv_tinv_q() const { return bfv_tinv_q_; }
+    const BfvDecrypt &bfv_decrypt_consts() const { return bfv_dec_; }
     const hm::NttTable &host_table(u32 row) const { return host_tables_[row]; }
 
     // BFV with BEHZ multiplication: its constants (else null)
@@ -198,6 +215,7 @@ private:
     u64 *bigP_mod_q_ = nullptr;
     u64 bfv_negq_mod_t_ = 0, bfv_negq_mod_t_shoup_ = 0;
     const u64 *bfv_tinv_q_ = nullptr;
+    BfvDecrypt bfv_dec_{};
     std::vector<LevelConsts> levels_;
     mutable std::vector<std::pair<u32, u32 *>> galois_tables_;
     BehzConsts behz_{};

@@ -1108,6 +1108,96 @@ void encrypt_asymmetric(const Context &ctx, const Ciphertext &pk, const u64 *pla
     ct.noise_scale_deg = 1;
 }
 
+// BFV scale-and-round of the phase x (canonical residues [k][N], coefficient form)
+// to the plaintext, one thread per coefficient: BEHZ (behz_decrypt_scale_and_round,
+// src/rns.cu:1008-1090 with perform_final_multiplication 984-1006) or HPS
+// (hps_decrypt_scale_and_round_kernel_small / _small_lazy / _large / _large_lazy,
+// src/rns.cu:1519-1633). The HPS forms can return t itself (the reference's llround
+// of a sum just below t), kept as the reference returns it.
+__global__ void k_bfv_decrypt(const u64 *phase, u64 *out, BfvDecrypt D, const DevModulus *mod, u32 N, u32 k) {
+#pragma clang fp contract(off)
+    const u32 c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= N) return;
+    const u64 *tab = D.tab;
+    const u64 t = D.t;
+    if (D.kind == kBfvDecBEHZ) {
+        // [t gamma]_q x, then FastBConv Q -> {t, gamma}
+        u128 at = 0, ag = 0;
+        for (u32 i = 0; i < k; ++i) {
+            const u64 q = mod[i].value;
+            const u64 *r = tab + 8 * i;
+            const u64 x = mul_shoup(phase[(u64)i * N + c], r[0], r[1], q);
+            const u64 y = mul_shoup(x, r[2], r[3], q);
+            at += (u128)y * r[4];
+            ag += (u128)y * r[5];
+        }
+        u64 xt = barrett128(at, t, D.t_ratio_lo, D.t_ratio_hi);
+        u64 xg = barrett128(ag, D.gamma, D.g_ratio_lo, D.g_ratio_hi);
+        xt = mul_shoup(xt, D.neg_qinv_t, D.neg_qinv_t_shoup, t);
+        xg = mul_shoup(xg, D.neg_qinv_g, D.neg_qinv_g_shoup, D.gamma);
+        // centred gamma part removed, then gamma^-1 mod t
+        const u64 v = xg > (D.gamma >> 1) ? add_mod(xt, (D.gamma - xg) % t, t) : sub_mod(xt, xg % t, t);
+        out[c] = mul_shoup(v, D.inv_gamma_t, D.inv_gamma_t_shoup, t);
+        return;
+    }
+    const bool lazy = D.kind == kBfvDecHpsSmallLazy || D.kind == kBfvDecHpsLargeLazy;
+    const bool large = D.kind == kBfvDecHpsLargeLazy || D.kind == kBfvDecHpsLarge;
+    const u32 h = D.qmsb_half;
+    const double tinv = 1.0 / (double)t;
+    double fs = 0.0;
+    u64 is = 0;
+    for (u32 i = 0; i < k; ++i) {
+        const u64 x = phase[(u64)i * N + c];
+        const u64 *r = tab + 8 * i;
+        if (!large) {
+            fs = __builtin_fma((double)x, __longlong_as_double((long long)r[2]), fs);
+            is += lazy ? x * r[0] : mul_shoup(x, r[0], r[1], t);
+        } else {
+            const u64 hi = x >> h, lo = x & ((1ull << h) - 1);
+            fs = __builtin_fma((double)lo, __longlong_as_double((long long)r[2]), fs);
+            fs = __builtin_fma((double)hi, __longlong_as_double((long long)r[5]), fs);
+            if (lazy) {
+                is += lo * r[0];
+                is += hi * r[3];
+            } else {
+                is += mul_shoup(lo, r[0], r[1], t);
+                is += mul_shoup(hi, r[3], r[4], t);
+            }
+        }
+    }
+    fs += (double)is;
+    const u64 quot = (u64)(fs * tinv);
+    fs -= (double)(t * quot);
+    out[c] = (u64)__builtin_llround(fs);
+}
+
+void bfv_decrypt(const Context &ctx, SecretKey &sk, const Ciphertext &ct, u64 *out_dev, hipStream_t s) {
+    const BfvDecrypt &D = ctx.bfv_decrypt_consts();
+    require(ctx.params().scheme == kSchemeBFV, "bfv_decrypt: the context is not BFV");
+    require(D.kind != kBfvDecNone, "plain_modulus is not set");
+    require(ct.chain_index == 1, "BFV decryption runs at the first data level");
+    require(!ct.is_ntt_form, "BFV encrypted must be in coefficient form");
+    const u64 N = ctx.N();
+    const u32 L = ct.coeff_modulus_size;
+    const u64 LN = (u64)L * N;
+    // phase c0 + c1 s (+ c2 s^2), via the NTT-form product
+    Scratch cp_buf((size_t)ct.size * LN * 8, s), ph_buf(LN * 8, s);
+    PHX_HIP_CHECK(hipMemcpyAsync(cp_buf.get(), ct.data, (size_t)ct.size * LN * 8, hipMemcpyDeviceToDevice, s));
+    ntt_forward(ctx.tables(), limb_map(cp_buf.get(), 0, L, LN), L, ct.size, s);
+    Ciphertext tmp;
+    tmp.chain_index = ct.chain_index;
+    tmp.size = ct.size;
+    tmp.poly_degree = N;
+    tmp.coeff_modulus_size = L;
+    tmp.is_ntt_form = true;
+    tmp.data = cp_buf.get();
+    tmp.owns = false;
+    decrypt(ctx, sk, tmp, ph_buf.get(), s);
+    hipLaunchKernelGGL(k_bfv_decrypt, grid_for(N, 1), dim3(kThreads), 0, s, (const u64 *)ph_buf.get(), out_dev, D,
+                       ctx.dev_mod(), (u32)N, L);
+    PHX_HIP_CHECK(hipGetLastError());
+}
+
 void decrypt(const Context &ctx, SecretKey &sk, const Ciphertext &ct, u64 *out_dev, hipStream_t s) {
     const u64 N = ctx.N();
     const u32 L = ct.coeff_modulus_size;

@@ -90,6 +90,7 @@ _SIGS = {
     "phx_secret_key_save_size": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_sz)]),
     "phx_secret_key_save": (ctypes.c_int, [_vp, _vp, _vp, _sz]),
     "phx_secret_key_load": (ctypes.c_int, [_vp, _vp, _sz, ctypes.POINTER(_vp)]),
+    "phx_decrypt_bfv": (ctypes.c_int, [_vp, _vp, _vp, _u64p, _vp]),
     "phx_public_key_create": (ctypes.c_int, [_vp, _vp, ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "phx_public_key_destroy": (ctypes.c_int, [_vp]),
     "phx_public_key_download": (ctypes.c_int, [_vp, _vp, _u64p]),
@@ -563,6 +564,12 @@ class secret_key:
         L = ct.coeff_modulus_size()
         out = np.zeros((L, ctx.N), np.uint64)
         _check(lib().phx_decrypt(ctx._h, self._h, ct._h, _p(out), _stream(stream)))
+        return out
+
+    def decrypt_bfv(self, ctx, ct, stream=None):
+        """PhantomSecretKey::decrypt of a BFV ciphertext: the plaintext's N coefficients"""
+        out = np.zeros(ctx.N, np.uint64)
+        _check(lib().phx_decrypt_bfv(ctx._h, self._h, ct._h, _p(out), _stream(stream)))
         return out
 
     def __del__(self):

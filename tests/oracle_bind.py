@@ -86,6 +86,7 @@ def lib():
         L.orc_bfv_multiply_hps_level.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _u64p, _u64p,
                                                  ctypes.c_int, ctypes.c_size_t, _u64p]
         L.orc_hps_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_bfv_decrypt.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, _u64p, _u64p]
         L.orc_salsa_quarterround.argtypes = [ctypes.POINTER(ctypes.c_uint32)]
         L.orc_salsa_hash.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
         L.orc_sample_uniform_seeded.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _sz, _u64p]
@@ -359,6 +360,13 @@ class HpsLevel(Hps):
         out = np.zeros((len(self.r()), self.octx.N), np.uint64)
         lib().orc_hps_modswitch(self.octx._h, self._h, _p(np.ascontiguousarray(x, np.uint64)), _p(out))
         return out
+
+
+def bfv_decrypt(octx, t, phase, behz):
+    """BFV scale-and-round of a coefficient-form phase [size_Q][N] -> N plaintext coefficients"""
+    out = np.zeros(octx.N, np.uint64)
+    assert lib().orc_bfv_decrypt(octx._h, t, int(behz), _p(np.ascontiguousarray(phase, np.uint64)), _p(out)) == 0
+    return out
 
 
 def hps_levels_to_drop(octx, t, depth, key_switch=False, asymmetric=False):
