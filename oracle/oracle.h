@@ -11,8 +11,9 @@
  *
  * Pinning (tests/test_oracle*.py): every routine is checked against closed
  * forms, known answers or the specification's vectors, except:
- *   - the FP64 tie behaviour of the HPS scale-and-round sums (rounding
- *     boundaries; the closed forms hold within the FP64 bound): parity unpinned;
+ *   - the FP64 tie behaviour of the HPS scale-and-round sums, in multiply
+ *     and in BFV decryption (rounding boundaries; the closed forms hold within
+ *     the FP64 bound): parity unpinned;
  *   - orc_hps_levels_to_drop (FindLevelsToDrop): invariants only, its value is
  *     parity unpinned;
  *   - orc_sample_uniform_seeded: the Salsa20 core is pinned by the
