@@ -872,6 +872,54 @@ int phx_rescale_to_next(const phx_context *ctx, const phx_ciphertext *in, phx_ci
     });
 }
 
+int phx_mod_switch_to_next(const phx_context *ctx, const phx_ciphertext *in, phx_ciphertext *out, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(in, "encrypted");
+        nn(out, "destination");
+        check_ct(*ctx->ctx, in->ct, "encrypted");
+        if (in == out) {
+            Ciphertext res;
+            mod_switch_to_next(*ctx->ctx, in->ct, res, S(s));
+            out->ct.release(S(s));
+            out->ct = res;
+        } else {
+            mod_switch_to_next(*ctx->ctx, in->ct, out->ct, S(s));
+        }
+    });
+}
+
+int phx_mod_switch_to_inplace(const phx_context *ctx, phx_ciphertext *ct, size_t chain_index, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct, "encrypted");
+        check_ct(*ctx->ctx, ct->ct, "encrypted");
+        require(ct->ct.chain_index <= chain_index, "cannot switch to higher level modulus");
+        require(chain_index <= ctx->ctx->size_Q(), "end of modulus switching chain reached");
+        if (ct->ct.chain_index == chain_index) return;
+        // one copy of the kept limbs (the same limbs as repeated drops)
+        const Context &c = *ctx->ctx;
+        require(c.params().scheme == kSchemeCKKS,
+                "modulus switching with scaling (BFV) is not provided; CKKS drops the last limb");
+        require(ct->ct.is_ntt_form, "CKKS encrypted must be in NTT form");
+        const u64 N = c.N();
+        const Ciphertext &in = ct->ct;
+        Ciphertext res;
+        res.resize(c, (u32)chain_index, in.size, S(s));
+        const u32 L = in.coeff_modulus_size, Ln = res.coeff_modulus_size;
+        for (u32 p = 0; p < in.size; ++p)
+            PHX_HIP_CHECK(hipMemcpyAsync(res.data + (u64)p * Ln * N, in.data + (u64)p * L * N, (size_t)Ln * N * 8,
+                                         hipMemcpyDeviceToDevice, S(s)));
+        res.scale = in.scale;
+        res.is_ntt_form = true;
+        res.correction_factor = in.correction_factor;
+        res.noise_scale_deg = in.noise_scale_deg;
+        res.is_asymmetric = in.is_asymmetric;
+        ct->ct.release(S(s));
+        ct->ct = res;
+    });
+}
+
 int phx_apply_galois_inplace(const phx_context *ctx, phx_ciphertext *ct, uint32_t elt, const phx_galois_key *gk,
                              phx_stream s) {
     return guard([&] {

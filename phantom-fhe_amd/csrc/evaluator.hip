@@ -718,6 +718,28 @@ void relinearize(const Context &ctx, Ciphertext &ct, const KSwitchKey &rlk, hipS
 }
 
 // rescale_to_next (src/evaluate.cu:1545-1565, 1376-1427; src/rns.cu:1160-1184)
+void mod_switch_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, hipStream_t s) {
+    require(in.chain_index >= 1, "invalid chain index");
+    require(in.chain_index < ctx.size_Q(), "end of modulus switching chain reached");
+    require(ctx.params().scheme == kSchemeCKKS,
+            "modulus switching with scaling (BFV) is not provided; CKKS drops the last limb");
+    require(in.is_ntt_form, "CKKS encrypted must be in NTT form");
+    require(&in != &out, "mod_switch_to_next: in and out must differ");
+    const u64 N = ctx.N();
+    const u32 L = in.coeff_modulus_size, size = in.size;
+    out.resize(ctx, in.chain_index + 1, size, s);
+    const u32 Ln = out.coeff_modulus_size;
+    for (u32 p = 0; p < size; ++p)
+        PHX_HIP_CHECK(hipMemcpyAsync(out.data + (u64)p * Ln * N, in.data + (u64)p * L * N, (size_t)Ln * N * 8,
+                                     hipMemcpyDeviceToDevice, s));
+    out.scale = in.scale;
+    out.is_ntt_form = true;
+    out.correction_factor = in.correction_factor;
+    out.noise_scale_deg = in.noise_scale_deg;
+    out.is_asymmetric = in.is_asymmetric;
+    out.has_seed = false;
+}
+
 void rescale_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, hipStream_t s) {
     require(ctx.params().scheme == kSchemeCKKS, "unsupported scheme");
     require(in.chain_index >= 1, "invalid chain index");

@@ -124,6 +124,8 @@ _SIGS = {
     "phx_multiply_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "phx_relinearize_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "phx_rescale_to_next": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "phx_mod_switch_to_next": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "phx_mod_switch_to_inplace": (ctypes.c_int, [_vp, _vp, _sz, _vp]),
     "phx_apply_galois_inplace": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "phx_rotate_inplace": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
     "phx_keyswitch_inplace": (ctypes.c_int, [_vp, _vp, _u64p, _vp, ctypes.c_int, _vp]),
@@ -746,6 +748,21 @@ def rescale_to_next(ctx, ct, stream=None):
     out = ciphertext()
     _check(lib().phx_rescale_to_next(ctx._h, ct._h, out._h, _stream(stream)))
     return out
+
+
+def mod_switch_to_next(ctx, ct, stream=None):
+    """CKKS: the ciphertext with its last limb dropped"""
+    out = ciphertext()
+    _check(lib().phx_mod_switch_to_next(ctx._h, ct._h, out._h, _stream(stream)))
+    return out
+
+
+def mod_switch_to_next_inplace(ctx, ct, stream=None):
+    _check(lib().phx_mod_switch_to_next(ctx._h, ct._h, ct._h, _stream(stream)))
+
+
+def mod_switch_to_inplace(ctx, ct, chain_index, stream=None):
+    _check(lib().phx_mod_switch_to_inplace(ctx._h, ct._h, chain_index, _stream(stream)))
 
 
 def rescale_to_next_inplace(ctx, ct, stream=None):

@@ -399,3 +399,33 @@ def test_ciphertext_of_another_context_rejected(small):
     # the valid operand is untouched by the refused calls
     ph.multiply_and_relin_inplace(ctx, mine, mine, rk)
     assert mine.size() == 2
+
+
+def test_mod_switch_drops_limbs(small):
+    """mod_switch_to_next / mod_switch_to_inplace (CKKS): the first limbs of each
+    polynomial, NTT form and scale kept; the switched ciphertext multiplies with
+    one born at that level as the oracle does"""
+    ctx, octx, mods, key = small
+    N = ctx.N
+    say("case: mod switch drop")
+    rng = np.random.default_rng(9)
+    a = rand_ct(rng, mods[:6], N)
+    c = upload_ct(ctx, a, 1)
+    c.set_scale(2.0 ** 40)
+    d = ph.mod_switch_to_next(ctx, c)
+    assert d.chain_index() == 2 and d.is_ntt_form() and d.scale() == 2.0 ** 40
+    assert np.array_equal(d.download(N), a[:, :5])
+    ph.mod_switch_to_next_inplace(ctx, c)
+    assert np.array_equal(c.download(N), a[:, :5])
+    ph.mod_switch_to_inplace(ctx, c, 4)
+    assert c.chain_index() == 4 and np.array_equal(c.download(N), a[:, :3])
+    b = rand_ct(rng, mods[:3], N)
+    e = upload_ct(ctx, b, 4)
+    rk = ph.relin_key(ctx, host=key)
+    ph.multiply_and_relin_inplace(ctx, c, e, rk)
+    assert np.array_equal(c.download(N), octx.multiply_and_relin(3, a[:, :3], b, key))
+    with pytest.raises(ValueError):
+        ph.mod_switch_to_inplace(ctx, c, 2)          # higher level
+    last = upload_ct(ctx, rand_ct(rng, mods[:1], N), 6)
+    with pytest.raises(ValueError):
+        ph.mod_switch_to_next(ctx, last)              # end of the chain
