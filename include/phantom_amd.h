@@ -214,9 +214,9 @@ int phx_multiply_and_relin_inplace(const phx_context *ctx, phx_ciphertext *ct1, 
 int phx_multiply_inplace(const phx_context *ctx, phx_ciphertext *ct1, const phx_ciphertext *ct2, phx_stream stream);
 /* relinearize_inplace (include/evaluate.cuh:125-126, src/evaluate.cu:1342-1374) */
 int phx_relinearize_inplace(const phx_context *ctx, phx_ciphertext *ct, const phx_relin_key *rlk, phx_stream stream);
-/* mod_switch_to_next / mod_switch_to_inplace for CKKS (include/evaluate.cuh:145-178,
- * mod_switch_drop_to_next src/evaluate.cu:1429-1472): drop the last limb(s), NTT form, scale kept.
- * in == out allowed. BFV's modulus switching with scaling is not provided (PHX_EINVAL). */
+/* mod_switch_to_next / mod_switch_to_inplace (include/evaluate.cuh:145-178, src/evaluate.cu:1376-1540):
+ * CKKS drops the last limb(s) (NTT form, scale kept); BFV divides by the last prime per level
+ * (coefficient form, floor as the reference's divide_and_round_q_last). in == out allowed. */
 int phx_mod_switch_to_next(const phx_context *ctx, const phx_ciphertext *in, phx_ciphertext *out, phx_stream stream);
 int phx_mod_switch_to_inplace(const phx_context *ctx, phx_ciphertext *ct, size_t chain_index, phx_stream stream);
 /* rescale_to_next (include/evaluate.cuh:137, src/evaluate.cu:1545-1565) */

@@ -897,10 +897,19 @@ int phx_mod_switch_to_inplace(const phx_context *ctx, phx_ciphertext *ct, size_t
         require(ct->ct.chain_index <= chain_index, "cannot switch to higher level modulus");
         require(chain_index <= ctx->ctx->size_Q(), "end of modulus switching chain reached");
         if (ct->ct.chain_index == chain_index) return;
-        // one copy of the kept limbs (the same limbs as repeated drops)
         const Context &c = *ctx->ctx;
-        require(c.params().scheme == kSchemeCKKS,
-                "modulus switching with scaling (BFV) is not provided; CKKS drops the last limb");
+        if (c.params().scheme == kSchemeBFV) {
+            // one division by the last prime per level, as the reference's loop does
+            while (ct->ct.chain_index < chain_index) {
+                Ciphertext res;
+                mod_switch_to_next(c, ct->ct, res, S(s));
+                ct->ct.release(S(s));
+                ct->ct = res;
+            }
+            return;
+        }
+        // CKKS: one copy of the kept limbs (the same limbs as repeated drops)
+        require(c.params().scheme == kSchemeCKKS, "unsupported scheme");
         require(ct->ct.is_ntt_form, "CKKS encrypted must be in NTT form");
         const u64 N = c.N();
         const Ciphertext &in = ct->ct;

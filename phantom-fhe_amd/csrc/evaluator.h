@@ -84,8 +84,9 @@ void multiply_and_relin(const Context &ctx, Ciphertext &ct1, const Ciphertext &c
                         hipStream_t s);
 void relinearize(const Context &ctx, Ciphertext &ct, const KSwitchKey &rlk, hipStream_t s);
 void rescale_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, hipStream_t s);
-// mod_switch_to_next for CKKS (mod_switch_drop_to_next, src/evaluate.cu:1429-1472,
-// 1505-1540): the first chain-(i+1) limbs of each polynomial, NTT form; in != out
+// mod_switch_to_next (src/evaluate.cu:1505-1540): CKKS drops the last limb
+// (mod_switch_drop_to_next, 1429-1472), BFV divides by it (mod_switch_scale_to_next,
+// 1376-1427, coefficient form); in != out
 void mod_switch_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, hipStream_t s);
 void apply_galois(const Context &ctx, Ciphertext &ct, u32 elt, const GaloisKeys &gk, hipStream_t s);
 void rotate(const Context &ctx, Ciphertext &ct, int step, const GaloisKeys &gk, hipStream_t s);

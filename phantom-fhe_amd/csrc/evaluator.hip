@@ -718,22 +718,47 @@ void relinearize(const Context &ctx, Ciphertext &ct, const KSwitchKey &rlk, hipS
 }
 
 // rescale_to_next (src/evaluate.cu:1545-1565, 1376-1427; src/rns.cu:1160-1184)
+// BFV modulus switch: c_j' = (c_j - [c_last]_{q_j}) q_last^-1 mod q_j on the
+// coefficient form, one grid row per kept limb, one z slice per polynomial
+// (divide_and_round_q_last_kernel, src/rns.cu:1082-1126: despite its name a floor)
+__global__ void k_divide_q_last(const u64 *in, u64 *out, const u64 *qinv, const DevModulus *mod, u32 N, u32 Ql) {
+    const u32 j = blockIdx.y, poly = blockIdx.z;
+    const u32 k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= N) return;
+    const DevModulus m = mod[j];
+    const u64 *src = in + (u64)poly * Ql * N;
+    const u64 last = barrett64(src[(u64)(Ql - 1) * N + k], m.value, m.ratio_hi);
+    const u64 d = sub_mod(src[(u64)j * N + k], last, m.value);
+    out[((u64)poly * (Ql - 1) + j) * N + k] = mul_shoup(d, qinv[2 * j], qinv[2 * j + 1], m.value);
+}
+
 void mod_switch_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, hipStream_t s) {
     require(in.chain_index >= 1, "invalid chain index");
     require(in.chain_index < ctx.size_Q(), "end of modulus switching chain reached");
-    require(ctx.params().scheme == kSchemeCKKS,
-            "modulus switching with scaling (BFV) is not provided; CKKS drops the last limb");
-    require(in.is_ntt_form, "CKKS encrypted must be in NTT form");
     require(&in != &out, "mod_switch_to_next: in and out must differ");
+    const bool bfv = ctx.params().scheme == kSchemeBFV;
+    require(bfv || ctx.params().scheme == kSchemeCKKS, "unsupported scheme");
+    if (bfv)
+        require(!in.is_ntt_form, "BFV encrypted cannot be in NTT form");
+    else
+        require(in.is_ntt_form, "CKKS encrypted must be in NTT form");
     const u64 N = ctx.N();
     const u32 L = in.coeff_modulus_size, size = in.size;
     out.resize(ctx, in.chain_index + 1, size, s);
     const u32 Ln = out.coeff_modulus_size;
-    for (u32 p = 0; p < size; ++p)
-        PHX_HIP_CHECK(hipMemcpyAsync(out.data + (u64)p * Ln * N, in.data + (u64)p * L * N, (size_t)Ln * N * 8,
-                                     hipMemcpyDeviceToDevice, s));
+    if (bfv) {
+        // mod_switch_scale_to_next (src/evaluate.cu:1376-1427)
+        hipLaunchKernelGGL(k_divide_q_last, grid_for(N, Ln, size), dim3(kThreads), 0, s, (const u64 *)in.data,
+                           out.data, ctx.level(in.chain_index).rescale_qinv, ctx.dev_mod(), (u32)N, L);
+        PHX_HIP_CHECK(hipGetLastError());
+    } else {
+        // mod_switch_drop_to_next (src/evaluate.cu:1429-1472)
+        for (u32 p = 0; p < size; ++p)
+            PHX_HIP_CHECK(hipMemcpyAsync(out.data + (u64)p * Ln * N, in.data + (u64)p * L * N, (size_t)Ln * N * 8,
+                                         hipMemcpyDeviceToDevice, s));
+    }
     out.scale = in.scale;
-    out.is_ntt_form = true;
+    out.is_ntt_form = !bfv;
     out.correction_factor = in.correction_factor;
     out.noise_scale_deg = in.noise_scale_deg;
     out.is_asymmetric = in.is_asymmetric;

@@ -115,3 +115,26 @@ def test_bfv_decrypt_rejects():
     cct = ph.ciphertext(cctx, 1, 2)
     with pytest.raises(ValueError):
         csk.decrypt_bfv(cctx, cct)
+
+
+def test_bfv_mod_switch_floor_divides():
+    """BFV mod_switch_to_next: each coefficient's CRT integer X becomes
+    floor(X / q_last) in the remaining primes (the reference's divide_and_round_q_last
+    kernel floors); mod_switch_to_inplace two levels down is floor(X / (q_k q_{k-1}))"""
+    from bigint_ref import crt
+    say("case: bfv mod switch")
+    ctx, octx, data, t = make(54, 4, None, ph.MUL_TECH_BEHZ)
+    rng = np.random.default_rng(11)
+    c = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in data]) for _ in range(2)])
+    ct = ph.ciphertext(ctx, 1, 2)
+    ct.upload(c)
+    d = ph.mod_switch_to_next(ctx, ct)
+    assert d.chain_index() == 2 and not d.is_ntt_form()
+    got = d.download(N)
+    ph.mod_switch_to_inplace(ctx, ct, 3)
+    got2 = ct.download(N)
+    for p in range(2):
+        for n in range(0, N, 7):
+            X = crt([int(c[p, i, n]) for i in range(4)], data)[0]
+            assert [int(v) for v in got[p, :, n]] == [(X // data[3]) % q for q in data[:3]], (p, n)
+            assert [int(v) for v in got2[p, :, n]] == [(X // (data[3] * data[2])) % q for q in data[:2]], (p, n)
