@@ -140,7 +140,7 @@ int phx_decrypt(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext
 /* PhantomSecretKey::decrypt of a BFV ciphertext (bfv_decrypt, src/secretkey.cu:571-635): the
  * plaintext's N coefficients (host), scale-and-round by BEHZ for mul_tech behz
  * (src/rns.cu:1008-1090), by HPS otherwise (src/rns.cu:1519-1690, its four forms chosen as the
- * reference chooses them); coefficient-form ciphertexts at the first data level */
+ * reference chooses them, with the level's constants); coefficient-form ciphertexts at any data level */
 int phx_decrypt_bfv(const phx_context *ctx, phx_secret_key *sk, const phx_ciphertext *ct, uint64_t *plain,
                     phx_stream stream);
 

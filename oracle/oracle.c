@@ -1594,16 +1594,17 @@ void orc_sample_uniform_seeded(const orc_ctx *c, const uint8_t seed[64], size_t 
 /* ------------------------------------------------------------------ */
 
 /* PhantomSecretKey::bfv_decrypt after the phase (src/secretkey.cu:571-635):
- * phase = canonical residues [size_Q][N] of c0 + c1 s (+ c2 s^2), coefficient
- * form; out[N] = the plaintext. behz != 0: behz_decrypt_scale_and_round
+ * phase = canonical residues [size_Ql][N] of c0 + c1 s (+ c2 s^2), coefficient
+ * form, at the level of the first size_Ql primes; out[N] = the plaintext. behz != 0: behz_decrypt_scale_and_round
  * (src/rns.cu:1008-1090; gamma = the largest 61-bit prime = 1 mod 2N, constants
  * src/rns.cu:330-390), with the final centring of perform_final_multiplication
  * (984-1006). Otherwise HPS (hps_decrypt_scale_and_round, src/rns.cu:1634-1690,
  * kernels 1519-1633; constants 583-673): FP64 sums with fused multiply-adds as
  * the device compiler contracts them, the form picked from qMSB, tMSB and the
  * bit count of size_Q exactly as the reference picks it. */
-int orc_bfv_decrypt(const orc_ctx *c, u64 t, int behz, const u64 *phase, u64 *out) {
-    const size_t N = c->N, k = c->size_Q;
+int orc_bfv_decrypt(const orc_ctx *c, size_t size_Ql, u64 t, int behz, const u64 *phase, u64 *out) {
+    const size_t N = c->N, k = size_Ql;
+    if (size_Ql < 1 || size_Ql > c->size_Q) return -1;
     const u64 *q = c->q;
     if (behz) {
         u64 g;
@@ -1627,8 +1628,8 @@ int orc_bfv_decrypt(const orc_ctx *c, u64 t, int behz, const u64 *phase, u64 *ou
         }
         return 0;
     }
-    u64 qmax = 0;
-    for (size_t i = 0; i < k; i++) qmax = q[i] > qmax ? q[i] : qmax;
+    u64 qmax = 0;   /* over all of Q; the size term over the level's primes (src/rns.cu:576-592) */
+    for (size_t i = 0; i < c->size_Q; i++) qmax = q[i] > qmax ? q[i] : qmax;
     const int qmsb = bit_count(qmax), sizeqmsb = bit_count((u64)k), tmsb = bit_count(t);
     const int large = qmsb + sizeqmsb >= 52;
     const int h = qmsb >> 1;

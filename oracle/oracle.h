@@ -145,8 +145,8 @@ void orc_salsa_quarterround(uint32_t y[4]);
 void orc_salsa_hash(const uint8_t in[64], uint8_t out[64]);
 void orc_sample_uniform_seeded(const orc_ctx *c, const uint8_t seed[64], size_t L, uint64_t *out);
 
-/* BFV decryption scale-and-round of a coefficient-form phase [size_Q][N] (BEHZ or HPS) */
-int orc_bfv_decrypt(const orc_ctx *c, uint64_t t, int behz, const uint64_t *phase, uint64_t *out);
+/* BFV decryption scale-and-round of a coefficient-form phase [size_Ql][N] (BEHZ or HPS) */
+int orc_bfv_decrypt(const orc_ctx *c, size_t size_Ql, uint64_t t, int behz, const uint64_t *phase, uint64_t *out);
 
 #ifdef __cplusplus
 }

@@ -571,19 +571,23 @@ Context::Context(const Params &p) : params_(p) {
     // BFV decryption scale-and-round at the first data level (PhantomSecretKey::bfv_decrypt,
     // src/secretkey.cu:571-635): BEHZ for mul_tech behz (constants src/rns.cu:330-390), HPS
     // otherwise (constants src/rns.cu:583-673, kernel choice 1634-1690)
-    size_t dec_off = 0;
-    if (p.scheme == kSchemeBFV && p.plain_modulus > 1) {
+    std::vector<size_t> dec_off(size_Q_ + 1, 0);
+    bfv_dec_.assign(size_Q_ + 1, BfvDecrypt{});
+    u64 dec_qmax = 0;
+    for (u32 i = 0; i < size_Q_; ++i) dec_qmax = std::max(dec_qmax, qp[i]);
+    for (u32 lev = 1; p.scheme == kSchemeBFV && p.plain_modulus > 1 && lev <= size_Q_; ++lev) {
+        const u32 Ql = size_Q_ - (lev - 1);
         const u64 t = p.plain_modulus;
-        BfvDecrypt &D = bfv_dec_;
+        BfvDecrypt &D = bfv_dec_[lev];
         D.t = t;
         hm::barrett_ratio(t, D.t_ratio_lo, D.t_ratio_hi);
-        std::vector<u64> tab((size_t)size_Q_ * 8, 0);
+        std::vector<u64> tab((size_t)Ql * 8, 0);
         if (p.mul_tech == kMulBEHZ) {
             const u64 g = hm::get_primes(N_, 61, 1)[0];
             D.kind = kBfvDecBEHZ;
             D.gamma = g;
             hm::barrett_ratio(g, D.g_ratio_lo, D.g_ratio_hi);
-            const u64 qt = hm::product_mod(qp, size_Q_, t), qg = hm::product_mod(qp, size_Q_, g);
+            const u64 qt = hm::product_mod(qp, Ql, t), qg = hm::product_mod(qp, Ql, g);
             const u64 it = hm::invmod(qt, t), ig = hm::invmod(qg, g);
             D.neg_qinv_t = it ? t - it : 0;
             D.neg_qinv_g = ig ? g - ig : 0;
@@ -591,22 +595,21 @@ Context::Context(const Params &p) : params_(p) {
             D.neg_qinv_g_shoup = hm::shoup(D.neg_qinv_g, g);
             D.inv_gamma_t = hm::invmod(g % t, t);
             D.inv_gamma_t_shoup = hm::shoup(D.inv_gamma_t, t);
-            for (u32 i = 0; i < size_Q_; ++i) {
+            for (u32 i = 0; i < Ql; ++i) {
                 const u64 q = qp[i];
                 const u64 tg = hm::mulmod(t % q, g % q, q);
-                const u64 hinv = hm::invmod(hm::punctured_mod(qp, size_Q_, i, q), q);
+                const u64 hinv = hm::invmod(hm::punctured_mod(qp, Ql, i, q), q);
                 u64 *r = &tab[(size_t)i * 8];
                 r[0] = tg;
                 r[1] = hm::shoup(tg, q);
                 r[2] = hinv;
                 r[3] = hm::shoup(hinv, q);
-                r[4] = hm::punctured_mod(qp, size_Q_, i, t);
-                r[5] = hm::punctured_mod(qp, size_Q_, i, g);
+                r[4] = hm::punctured_mod(qp, Ql, i, t);
+                r[5] = hm::punctured_mod(qp, Ql, i, g);
             }
         } else {
-            u64 qmax = 0;
-            for (u32 i = 0; i < size_Q_; ++i) qmax = std::max(qmax, qp[i]);
-            const int qmsb = hm::bit_count(qmax), sizeqmsb = hm::bit_count(size_Q_), tmsb = hm::bit_count(t);
+            // qMSB over all of Q, the size term over this level's primes (src/rns.cu:576-592)
+            const int qmsb = hm::bit_count(dec_qmax), sizeqmsb = hm::bit_count(Ql), tmsb = hm::bit_count(t);
             const bool large = qmsb + sizeqmsb >= 52;
             D.qmsb_half = (u32)(qmsb >> 1);
             if (!large)
@@ -614,9 +617,9 @@ Context::Context(const Params &p) : params_(p) {
             else
                 D.kind = (int)D.qmsb_half + tmsb + sizeqmsb < 52 ? kBfvDecHpsLargeLazy : kBfvDecHpsLarge;
             auto dbits = [](double d) { u64 b; std::memcpy(&b, &d, 8); return b; };
-            for (u32 i = 0; i < size_Q_; ++i) {
+            for (u32 i = 0; i < Ql; ++i) {
                 const u64 q = qp[i];
-                const u64 v = hm::invmod(hm::punctured_mod(qp, size_Q_, i, q), q);
+                const u64 v = hm::invmod(hm::punctured_mod(qp, Ql, i, q), q);
                 u64 *r = &tab[(size_t)i * 8];
                 const u128 big = (u128)t * v;
                 r[0] = (u64)((big / q) % t);
@@ -631,7 +634,7 @@ Context::Context(const Params &p) : params_(p) {
                 }
             }
         }
-        dec_off = cb.push64(tab);
+        dec_off[lev] = cb.push64(tab);
     }
     // BEHZ constants (src/rns.cu:424-571)
     struct BehzOffs {
@@ -752,7 +755,8 @@ Context::Context(const Params &p) : params_(p) {
     }
     bigP_mod_q_ = reinterpret_cast<u64 *>(cbb + bp_off);
     if (tinv_off) bfv_tinv_q_ = reinterpret_cast<const u64 *>(cbb + tinv_off);
-    if (dec_off) bfv_dec_.tab = reinterpret_cast<const u64 *>(cbb + dec_off);
+    for (size_t lev = 1; lev < dec_off.size(); ++lev)
+        if (dec_off[lev]) bfv_dec_[lev].tab = reinterpret_cast<const u64 *>(cbb + dec_off[lev]);
     for (size_t d = 0; d < hps_offs.size(); ++d) {
         const u32 k0 = size_Q_, Ql = k0 - (u32)d;
         HpsConsts h = resolve_hps(cbb, hps_offs[d], Ql, overq ? Ql : (u32)hps_r_.size(), overq, d ? k0 : Ql);

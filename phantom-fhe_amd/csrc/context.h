@@ -94,7 +94,7 @@ struct BehzConsts {
     const u64 *tscale_bsk;   // [nb][4]
 };
 
-// BFV decryption scale-and-round at the first data level, the reference's
+// BFV decryption scale-and-round at one data level, the reference's
 // choice of method (BEHZ for mul_tech behz; HPS otherwise, in its four forms).
 enum BfvDecKind { kBfvDecNone = 0, kBfvDecBEHZ, kBfvDecHpsSmallLazy, kBfvDecHpsSmall, kBfvDecHpsLargeLazy,
                   kBfvDecHpsLarge };
@@ -178,7 +178,8 @@ public:
     u64 bfv_negq_mod_t() const { return bfv_negq_mod_t_; }
     u64 bfv_negq_mod_t_shoup() const { return bfv_negq_mod_t_shoup_; }
     const u64 *bfv_tinv_q() const { return bfv_tinv_q_; }
-    const BfvDecrypt &bfv_decrypt_consts() const { return bfv_dec_; }
+    // per chain index 1 .. size_Q (kind kBfvDecNone unless BFV with t set)
+    const BfvDecrypt &bfv_decrypt_consts(u32 chain) const { return bfv_dec_[chain]; }
     const hm::NttTable &host_table(u32 row) const { return host_tables_[row]; }
 
     // BFV with BEHZ multiplication: its constants (else null)
@@ -215,7 +216,7 @@ private:
     u64 *bigP_mod_q_ = nullptr;
     u64 bfv_negq_mod_t_ = 0, bfv_negq_mod_t_shoup_ = 0;
     const u64 *bfv_tinv_q_ = nullptr;
-    BfvDecrypt bfv_dec_{};
+    std::vector<BfvDecrypt> bfv_dec_;
     std::vector<LevelConsts> levels_;
     mutable std::vector<std::pair<u32, u32 *>> galois_tables_;
     BehzConsts behz_{};

@@ -124,8 +124,8 @@ void encrypt_symmetric(const Context &ctx, const SecretKey &sk, const u64 *plain
 // the ciphertext keeps the seed (save_symmetric). e comes from the counter hash (seed_e).
 void encrypt_symmetric_seeded(const Context &ctx, const SecretKey &sk, const u64 *plain_dev, u32 chain,
                               Ciphertext &ct, const uint8_t *seed_a, u64 seed_e, hipStream_t s);
-// BFV: the plaintext (N coefficients) of a coefficient-form ciphertext at the first
-// data level (PhantomSecretKey::bfv_decrypt, src/secretkey.cu:571-635)
+// BFV: the plaintext (N coefficients) of a coefficient-form ciphertext at any data
+// level (PhantomSecretKey::bfv_decrypt, src/secretkey.cu:571-635)
 void bfv_decrypt(const Context &ctx, SecretKey &sk, const Ciphertext &ct, u64 *out_dev, hipStream_t s);
 // out: [Ql][N] coefficient-domain residues of c0 + c1 s (+ c2 s^2)
 void decrypt(const Context &ctx, SecretKey &sk, const Ciphertext &ct, u64 *out_dev, hipStream_t s);

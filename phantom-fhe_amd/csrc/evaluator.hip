@@ -1219,10 +1219,10 @@ __global__ void k_bfv_decrypt(const u64 *phase, u64 *out, BfvDecrypt D, const De
 }
 
 void bfv_decrypt(const Context &ctx, SecretKey &sk, const Ciphertext &ct, u64 *out_dev, hipStream_t s) {
-    const BfvDecrypt &D = ctx.bfv_decrypt_consts();
     require(ctx.params().scheme == kSchemeBFV, "bfv_decrypt: the context is not BFV");
+    require(ct.chain_index >= 1 && ct.chain_index <= ctx.size_Q(), "invalid chain index");
+    const BfvDecrypt &D = ctx.bfv_decrypt_consts(ct.chain_index);
     require(D.kind != kBfvDecNone, "plain_modulus is not set");
-    require(ct.chain_index == 1, "BFV decryption runs at the first data level");
     require(!ct.is_ntt_form, "BFV encrypted must be in coefficient form");
     const u64 N = ctx.N();
     const u32 L = ct.coeff_modulus_size;

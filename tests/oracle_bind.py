@@ -86,7 +86,7 @@ def lib():
         L.orc_bfv_multiply_hps_level.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _u64p, _u64p,
                                                  ctypes.c_int, ctypes.c_size_t, _u64p]
         L.orc_hps_destroy.argtypes = [ctypes.c_void_p]
-        L.orc_bfv_decrypt.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, _u64p, _u64p]
+        L.orc_bfv_decrypt.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int, _u64p, _u64p]
         L.orc_salsa_quarterround.argtypes = [ctypes.POINTER(ctypes.c_uint32)]
         L.orc_salsa_hash.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
         L.orc_sample_uniform_seeded.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _sz, _u64p]
@@ -363,9 +363,11 @@ class HpsLevel(Hps):
 
 
 def bfv_decrypt(octx, t, phase, behz):
-    """BFV scale-and-round of a coefficient-form phase [size_Q][N] -> N plaintext coefficients"""
+    """BFV scale-and-round of a coefficient-form phase [size_Ql][N] (the level of
+    its first size_Ql primes) -> N plaintext coefficients"""
     out = np.zeros(octx.N, np.uint64)
-    assert lib().orc_bfv_decrypt(octx._h, t, int(behz), _p(np.ascontiguousarray(phase, np.uint64)), _p(out)) == 0
+    phase = np.ascontiguousarray(phase, np.uint64)
+    assert lib().orc_bfv_decrypt(octx._h, phase.shape[0], t, int(behz), _p(phase), _p(out)) == 0
     return out
 
 

@@ -138,3 +138,27 @@ def test_bfv_mod_switch_floor_divides():
             X = crt([int(c[p, i, n]) for i in range(4)], data)[0]
             assert [int(v) for v in got[p, :, n]] == [(X // data[3]) % q for q in data[:3]], (p, n)
             assert [int(v) for v in got2[p, :, n]] == [(X // (data[3] * data[2])) % q for q in data[:2]], (p, n)
+
+
+@pytest.mark.parametrize("tech", [ph.MUL_TECH_BEHZ, ph.MUL_TECH_HPS])
+def test_bfv_decrypt_below_the_top_level(tech):
+    """decryption at chain 2 and 3 (the level's own constants; HPS's qMSB still over
+    all of Q): bit-exact against the oracle on uniform ciphertexts, and an
+    encryption switched down by mod_switch_to_next still decrypts to m"""
+    say(f"case: bfv decrypt below the top level tech={tech}")
+    ctx, octx, data, t = make(54, 4, None, tech)
+    sk = ph.secret_key(ctx, seed=45)
+    s_ntt = sk.ntt_form()
+    rng = np.random.default_rng(tech + 20)
+    for chain in (2, 3):
+        kl = 5 - chain
+        c = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in data[:kl]]) for _ in range(2)])
+        ct = ph.ciphertext(ctx, chain, 2)
+        ct.upload(c)
+        want = ob.bfv_decrypt(octx, t, phase(octx, data[:kl], c, s_ntt), tech == ph.MUL_TECH_BEHZ)
+        assert np.array_equal(sk.decrypt_bfv(ctx, ct), want), chain
+    pk = sk.gen_publickey(ctx)
+    m = rng.integers(0, T, N).astype(np.uint64)
+    e = pk.encrypt_asymmetric(ctx, m, 1, seed=8)
+    ph.mod_switch_to_inplace(ctx, e, 3)
+    assert np.array_equal(sk.decrypt_bfv(ctx, e) % T, m)
