@@ -1103,6 +1103,7 @@ void encrypt_asymmetric(const Context &ctx, const Ciphertext &pk, const u64 *pla
     require(bfv || ctx.params().scheme == kSchemeCKKS, "unsupported scheme.");
     require(pk.data != nullptr && pk.chain_index == 0 && pk.size == 2 && pk.coeff_modulus_size == ctx.size_QP(),
             "PhantomPublicKey has not been generated");
+    require(pk.poly_degree == ctx.N(), "the public key does not match the context (ring degree)");
     require(chain == 1, "asymmetric encryption runs at the first data level");
     require(plain_dev != nullptr, "plaintext is null");
     if (bfv) require(ctx.bfv_tinv_q() != nullptr, "plain_modulus is not set");
