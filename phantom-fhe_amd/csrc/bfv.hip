@@ -439,7 +439,13 @@ u32 hps_levels_to_drop(const Context &ctx, u64 depth, bool key_switch, bool asym
 
 u32 bfv_keyswitch_drop(const Context &ctx, const Ciphertext &ct, bool key_switch) {
     if (ctx.params().scheme != kSchemeBFV || ctx.params().mul_tech != kMulHPSOverQLeveled) return 0;
-    return hps_levels_to_drop(ctx, (u64)ct.noise_scale_deg - 1, key_switch, ct.is_asymmetric);
+    const u32 d = hps_levels_to_drop(ctx, (u64)ct.noise_scale_deg - 1, key_switch, ct.is_asymmetric);
+    // the level drop scales all of Q down to Q_(size_Q - d) and expands back to Q
+    // (src/eval_key_switch.cu:110-150 reads size_Q limbs of c2): it is defined for
+    // ciphertexts at the first data level only
+    require(d == 0 || ct.chain_index == 1,
+            "hps_overq_leveled key switching with dropped levels needs a ciphertext at the first data level");
+    return d;
 }
 
 // ExpandCRTBasis_Ql_Q (src/rns.cu:1808-1834): out_i = [x_i (Q/Ql)]_{q_i} for i < Ql and 0

@@ -120,6 +120,24 @@ void check_key(const Context &c, const KSwitchKey &k, const char *what) {
     require(k.poly_degree == c.N() && k.size_QP == c.size_QP() && k.dnum >= c.dnum(),
             std::string(what) + " does not match the context (ring degree, QP limbs or dnum)");
 }
+// A ciphertext built inside an entry point before it replaces the caller's: its
+// device buffer is released if the call fails before the hand-over.
+struct TempCt {
+    Ciphertext ct;
+    hipStream_t s;
+    explicit TempCt(hipStream_t st) : s(st) {}
+    TempCt(const TempCt &) = delete;
+    TempCt &operator=(const TempCt &) = delete;
+    ~TempCt() { ct.release(s); }
+    // moves the ciphertext into dst, releasing dst's old buffer
+    void hand_over(Ciphertext &dst) {
+        dst.release(s);
+        dst = ct;
+        ct.data = nullptr;
+        ct.owns = false;
+        ct.capacity = 0;
+    }
+};
 void check_keys(const Context &c, const GaloisKeys &g) {
     for (const KSwitchKey &k : g.keys) check_key(c, k, "Galois key");
 }
@@ -861,11 +879,9 @@ int phx_rescale_to_next(const phx_context *ctx, const phx_ciphertext *in, phx_ci
         nn(out, "destination");
         check_ct(*ctx->ctx, in->ct, "encrypted");
         if (in == out) {
-            Ciphertext tmp = in->ct;
-            Ciphertext res;
-            rescale_to_next(*ctx->ctx, tmp, res, S(s));
-            out->ct.release(S(s));
-            out->ct = res;
+            TempCt res(S(s));
+            rescale_to_next(*ctx->ctx, in->ct, res.ct, S(s));
+            res.hand_over(out->ct);
         } else {
             rescale_to_next(*ctx->ctx, in->ct, out->ct, S(s));
         }
@@ -879,10 +895,9 @@ int phx_mod_switch_to_next(const phx_context *ctx, const phx_ciphertext *in, phx
         nn(out, "destination");
         check_ct(*ctx->ctx, in->ct, "encrypted");
         if (in == out) {
-            Ciphertext res;
-            mod_switch_to_next(*ctx->ctx, in->ct, res, S(s));
-            out->ct.release(S(s));
-            out->ct = res;
+            TempCt res(S(s));
+            mod_switch_to_next(*ctx->ctx, in->ct, res.ct, S(s));
+            res.hand_over(out->ct);
         } else {
             mod_switch_to_next(*ctx->ctx, in->ct, out->ct, S(s));
         }
@@ -901,10 +916,9 @@ int phx_mod_switch_to_inplace(const phx_context *ctx, phx_ciphertext *ct, size_t
         if (c.params().scheme == kSchemeBFV) {
             // one division by the last prime per level, as the reference's loop does
             while (ct->ct.chain_index < chain_index) {
-                Ciphertext res;
-                mod_switch_to_next(c, ct->ct, res, S(s));
-                ct->ct.release(S(s));
-                ct->ct = res;
+                TempCt res(S(s));
+                mod_switch_to_next(c, ct->ct, res.ct, S(s));
+                res.hand_over(ct->ct);
             }
             return;
         }
@@ -913,7 +927,8 @@ int phx_mod_switch_to_inplace(const phx_context *ctx, phx_ciphertext *ct, size_t
         require(ct->ct.is_ntt_form, "CKKS encrypted must be in NTT form");
         const u64 N = c.N();
         const Ciphertext &in = ct->ct;
-        Ciphertext res;
+        TempCt tmp(S(s));
+        Ciphertext &res = tmp.ct;
         res.resize(c, (u32)chain_index, in.size, S(s));
         const u32 L = in.coeff_modulus_size, Ln = res.coeff_modulus_size;
         for (u32 p = 0; p < in.size; ++p)
@@ -924,8 +939,7 @@ int phx_mod_switch_to_inplace(const phx_context *ctx, phx_ciphertext *ct, size_t
         res.correction_factor = in.correction_factor;
         res.noise_scale_deg = in.noise_scale_deg;
         res.is_asymmetric = in.is_asymmetric;
-        ct->ct.release(S(s));
-        ct->ct = res;
+        tmp.hand_over(ct->ct);
     });
 }
 

@@ -374,8 +374,9 @@ __global__ __launch_bounds__(kThreads) void k_permute_acc(const u64 *__restrict_
 }
 
 // tensor_prod_2x2_rns_poly (src/polymath.cu:463-498), Karatsuba middle term
-__global__ __launch_bounds__(kThreads) void k_tensor(const u64 *__restrict__ a, const u64 *__restrict__ b,
-                                                     u64 *d0, u64 *d1, u64 *__restrict__ d2,
+// a and b may be d0 / d1 (the in-place multiply): only d2 is declared unaliased
+__global__ __launch_bounds__(kThreads) void k_tensor(const u64 *a, const u64 *b, u64 *d0, u64 *d1,
+                                                     u64 *__restrict__ d2,
                                                      const DevModulus *__restrict__ mod, u32 N, u64 LN) {
     const u32 limb = blockIdx.y;
     const u32 k = blockIdx.x * kThreads + threadIdx.x;
@@ -533,7 +534,9 @@ __global__ void k_dot_secret(const u64 *ct, const u64 *s, const u64 *s2, u64 *ou
     const u64 o = (u64)limb * N + k;
     const u64 LN = (u64)L * N;
     const DevModulus md = mod[limb];
-    u64 v = add_mod(ct[o], mul_mod(ct[LN + o], s[o], md), md.value);
+    // a size-1 ciphertext is its c0 alone (needed_sk_power 0)
+    u64 v = ct[o];
+    if (size > 1) v = add_mod(v, mul_mod(ct[LN + o], s[o], md), md.value);
     if (size > 2) v = add_mod(v, mul_mod(ct[2 * LN + o], s2[o], md), md.value);
     out[o] = v;
 }
@@ -780,9 +783,15 @@ void rescale_to_next(const Context &ctx, const Ciphertext &in, Ciphertext &out, 
     ntt_inverse(ctx.tables(), mi, 1, size, s);
     Ciphertext dst;
     dst.resize(ctx, in.chain_index + 1, size, s);
-    LoadRescale ld{last, N};
-    StoreRescale st{in.data, dst.data, L.rescale_qinv, (u64)Ql * N, (u64)(Ql - 1) * N, (u32)N};
-    ntt_forward_rescale(ctx.tables(), limb_map(dst.data, 0, Ql - 1, (u64)(Ql - 1) * N), Ql - 1, size, ld, st, s);
+    try {
+        LoadRescale ld{last, N};
+        StoreRescale st{in.data, dst.data, L.rescale_qinv, (u64)Ql * N, (u64)(Ql - 1) * N, (u32)N};
+        ntt_forward_rescale(ctx.tables(), limb_map(dst.data, 0, Ql - 1, (u64)(Ql - 1) * N), Ql - 1, size, ld, st,
+                            s);
+    } catch (...) {
+        dst.release(s);
+        throw;
+    }
     dst.is_ntt_form = in.is_ntt_form;
     dst.scale = in.scale / (double)ctx.moduli()[Ql - 1];
     dst.correction_factor = in.correction_factor;
@@ -866,7 +875,7 @@ void hoisting(const Context &ctx, Ciphertext &ct, const GaloisKeys &gk, const st
 
 // apply_galois_inplace (src/evaluate.cu:1567-1630)
 void apply_galois(const Context &ctx, Ciphertext &ct, u32 elt, const GaloisKeys &gk, hipStream_t s) {
-    require(ct.size <= 2, "encrypted size must be 2");
+    require(ct.size == 2, "encrypted size must be 2");
     auto it = std::find(gk.elts.begin(), gk.elts.end(), elt);
     require(it != gk.elts.end(), "Galois elt not present");
     const KSwitchKey &key = gk.keys[it - gk.elts.begin()];
@@ -1250,6 +1259,7 @@ void decrypt(const Context &ctx, SecretKey &sk, const Ciphertext &ct, u64 *out_d
     const u64 N = ctx.N();
     const u32 L = ct.coeff_modulus_size;
     require(ct.is_ntt_form, "encrypted must be in NTT form");
+    require(ct.size >= 1 && ct.size <= 3, "encrypted size must be 1..3");
     if (ct.size > 2 && !sk.s2) {
         const u32 QP = ctx.size_QP();
         sk.s2 = static_cast<u64 *>(dev_alloc((size_t)QP * N * 8, s));

@@ -751,7 +751,9 @@ def rescale_to_next(ctx, ct, stream=None):
 
 
 def mod_switch_to_next(ctx, ct, stream=None):
-    """CKKS: the ciphertext with its last limb dropped"""
+    """the ciphertext at the next level: CKKS drops the last limb (NTT form, scale
+    kept); BFV divides by the last prime (coefficient form, floor division as the
+    reference's divide_and_round_q_last)"""
     out = ciphertext()
     _check(lib().phx_mod_switch_to_next(ctx._h, ct._h, out._h, _stream(stream)))
     return out
