@@ -1162,30 +1162,52 @@ void orc_hps_destroy(orc_hps *h) {
 size_t orc_hps_r_size(const orc_hps *h) { return h->nr; }
 void orc_hps_r(const orc_hps *h, u64 *out) { memcpy(out, h->r, h->nr * 8); }
 
+/* The FP64 sums of the HPS path, all in one form: acc = init, then for each
+ * term in input order acc = fma((double) x_i, w_i, acc) -- the reference's
+ * `acc += static_cast<double>(x_i) * w_i` (src/rns_bconv.cu:170-195,
+ * src/rns.cu:1708-1714, 1757-1763) as nvcc contracts it (multiply-add fused,
+ * u64 -> double rounded to nearest even). x_i = x[i * stride]. */
+static double fma_chain(double init, const u64 *x, size_t stride, const double *w, size_t n) {
+    double acc = init;
+    for (size_t i = 0; i < n; i++) acc = fma((double)x[i * stride], w[i], acc);
+    return acc;
+}
+
+void orc_fma_chain(const u64 *x, size_t nterms, size_t N, const double *w, double init, double *out) {
+    for (size_t n = 0; n < N; n++) out[n] = fma_chain(init, x + n, N, w, nterms);
+}
+
 /* DBaseConverter::bConv_HPS (src/rns_bconv.cu:277-372): y_i = [x_i ihat_i^-1]_{i};
- * out_p = sum_i y_i [ihat_i]_p - v [I]_p with v = llround(sum_i y_i / i_i), the
- * FP64 sum accumulated in input order with fused multiply-adds */
-static void bconv_hps(size_t N, const u64 *ib, size_t ni, const u64 *ob, size_t no, const u64 *in, u64 *out) {
+ * out_p = sum_i y_i [ihat_i]_p - v [I]_p with v = llround(sum_i y_i fl(1 / i_i))
+ * (the FP64 sum from 0 by fma_chain; ibase_.qiInv() = 1.0 / (double) q_i,
+ * src/host/rns.cu:319-323). sum (may be null): the FP64 sum per coefficient */
+void orc_bconv_hps(size_t N, const u64 *ib, size_t ni, const u64 *ob, size_t no, const u64 *in, u64 *out,
+                   double *sum) {
     u64 *y = (u64 *)malloc(ni * N * 8);
+    double *inv = (double *)malloc(ni * sizeof(double));
+    double *fs = (double *)malloc(N * sizeof(double));
     for (size_t i = 0; i < ni; i++) {
         u64 hinv = invmod(punctured_mod(ib, ni, i, ib[i]), ib[i]);
+        inv[i] = 1.0 / (double)ib[i];
         for (size_t n = 0; n < N; n++) y[i * N + n] = mulmod(in[i * N + n], hinv, ib[i]);
     }
+    for (size_t n = 0; n < N; n++) fs[n] = fma_chain(0.0, y + n, N, inv, ni);
+    if (sum) memcpy(sum, fs, N * sizeof(double));
     for (size_t o = 0; o < no; o++) {
         u64 p = ob[o], imodp = prod_mod(ib, ni, p);
         for (size_t n = 0; n < N; n++) {
             u128 acc = 0;
-            double frac = 0.0;
-            for (size_t i = 0; i < ni; i++) {
-                acc += (u128)y[i * N + n] * punctured_mod(ib, ni, i, p);
-                frac = fma((double)y[i * N + n], 1.0 / (double)ib[i], frac);
-            }
-            long long v = llround(frac);
+            for (size_t i = 0; i < ni; i++) acc += (u128)y[i * N + n] * punctured_mod(ib, ni, i, p);
+            long long v = llround(fs[n]);
             u64 sub = mulmod((u64)v % p, imodp, p);
             out[o * N + n] = submod((u64)(acc % p), sub, p);
         }
     }
-    free(y);
+    free(y); free(inv); free(fs);
+}
+
+static void bconv_hps(size_t N, const u64 *ib, size_t ni, const u64 *ob, size_t no, const u64 *in, u64 *out) {
+    orc_bconv_hps(N, ib, ni, ob, no, in, out, NULL);
 }
 
 static void r_ntt(const orc_hps *h, size_t N, u64 *data, int inverse) {
@@ -1212,6 +1234,49 @@ void orc_hps_modswitch(const orc_ctx *c, const orc_hps *h, const u64 *x, u64 *ou
             out[j * N + n] = (u64)(acc % h->r[j]);
         }
     }
+}
+
+/* The scale-and-round step of one polynomial d [k + nr][N] (Q part, then R part).
+ * mul_tech hps: scaleAndRound_HPS_QR_R (src/rns.cu:1700-1746): nu = 0.5 +
+ * sum_i x_{q_i} frac_i (fma_chain), alpha = (u64) nu, out_j = [sum_i x_{q_i} D_ji +
+ * x_{r_j} D_jk + alpha]_{r_j}, out [nr][N]. hps_overq / leveled:
+ * scaleAndRound_HPS_QlRl_Ql (src/rns.cu:1749-1796): nu = 0.5 + sum_j x_{r_j}
+ * frac_j, out_i = [sum_j x_{r_j} D_ij + x_{q_i} D_ik + alpha]_{q_i}, out [k][N].
+ * nu (may be null): the FP64 sum per coefficient */
+void orc_hps_scale_round(const orc_ctx *c, const orc_hps *h, const u64 *d, u64 *out, double *nu_out) {
+    size_t N = c->N, k = h->k, nr = h->nr, QN = k * N;
+    for (size_t n = 0; n < N; n++) {
+        if (h->overq) {
+            double nu = fma_chain(0.5, d + QN + n, N, h->frac_r, nr);
+            if (nu_out) nu_out[n] = nu;
+            u64 alpha = (u64)nu;
+            for (size_t i = 0; i < k; i++) {
+                u64 qi = c->q[i];
+                const u64 *dm = h->divs_mod_q + i * (nr + 1);
+                u128 acc = 0;
+                for (size_t j = 0; j < nr; j++) acc += (u128)d[QN + j * N + n] * dm[j];
+                acc += (u128)d[i * N + n] * dm[nr];
+                out[i * N + n] = addmod((u64)(acc % qi), alpha % qi, qi);
+            }
+            continue;
+        }
+        double nu = fma_chain(0.5, d + n, N, h->frac, k);
+        if (nu_out) nu_out[n] = nu;
+        u64 alpha = (u64)nu;
+        for (size_t j = 0; j < nr; j++) {
+            u64 rj = h->r[j];
+            const u64 *dm = h->divs_mod_r + j * (k + 1);
+            u128 acc = 0;
+            for (size_t i = 0; i < k; i++) acc += (u128)d[i * N + n] * dm[i];
+            acc += (u128)d[QN + j * N + n] * dm[k];
+            out[j * N + n] = addmod((u64)(acc % rj), alpha % rj, rj);
+        }
+    }
+}
+
+void orc_hps_frac(const orc_hps *h, double *frac_q, double *frac_r) {
+    if (frac_q) memcpy(frac_q, h->frac, h->k * sizeof(double));
+    if (frac_r && h->overq) memcpy(frac_r, h->frac_r, h->nr * sizeof(double));
 }
 
 int orc_bfv_multiply_hps(const orc_ctx *c, const orc_hps *h, const u64 *ct1, const u64 *ct2, u64 *out3) {
@@ -1263,34 +1328,10 @@ int orc_bfv_multiply_hps2(const orc_ctx *c, const orc_hps *h, const u64 *ct1, co
         orc_ntt_inverse(c, d, k, 0);
         r_ntt(h, N, d + QN, 1);
         if (h->overq) {
-            for (size_t n = 0; n < N; n++) {
-                double nu = 0.5;
-                for (size_t j = 0; j < nr; j++) nu = fma((double)d[QN + j * N + n], h->frac_r[j], nu);
-                u64 alpha = (u64)nu;
-                for (size_t i = 0; i < k; i++) {
-                    u64 qi = c->q[i];
-                    const u64 *dm = h->divs_mod_q + i * (nr + 1);
-                    u128 acc = 0;
-                    for (size_t j = 0; j < nr; j++) acc += (u128)d[QN + j * N + n] * dm[j];
-                    acc += (u128)d[i * N + n] * dm[nr];
-                    out3[p * QN + i * N + n] = addmod((u64)(acc % qi), alpha % qi, qi);
-                }
-            }
+            orc_hps_scale_round(c, h, d, out3 + p * QN, NULL);
             continue;
         }
-        for (size_t n = 0; n < N; n++) {
-            double nu = 0.5;
-            for (size_t i = 0; i < k; i++) nu = fma((double)d[i * N + n], h->frac[i], nu);
-            u64 alpha = (u64)nu;
-            for (size_t j = 0; j < nr; j++) {
-                u64 rj = h->r[j];
-                const u64 *dm = h->divs_mod_r + j * (k + 1);
-                u128 acc = 0;
-                for (size_t i = 0; i < k; i++) acc += (u128)d[i * N + n] * dm[i];
-                acc += (u128)d[QN + j * N + n] * dm[k];
-                tr[j * N + n] = addmod((u64)(acc % rj), alpha % rj, rj);
-            }
-        }
+        orc_hps_scale_round(c, h, d, tr, NULL);
         bconv_hps(N, h->r, nr, c->q, k, tr, out3 + p * QN);
     }
     free(a); free(b); free(tr);
@@ -1335,6 +1376,10 @@ size_t orc_hps_levels_to_drop(const orc_ctx *c, u64 t, size_t depth, int key_swi
  * order), alpha = (u64) nu; out_i = [sum_j x_{Ql+j} D_ij + x_i D_id + alpha]_{q_i},
  * i.e. round(Ql X / Q) for X = CRT_Q(x). x [k0][N] -> out [Ql][N], d >= 1 */
 void orc_hps_scale_q_ql(const orc_ctx *c, size_t d, const u64 *x, u64 *out) {
+    orc_hps_scale_q_ql_nu(c, d, x, out, NULL);
+}
+
+void orc_hps_scale_q_ql_nu(const orc_ctx *c, size_t d, const u64 *x, u64 *out, double *nu_out) {
     size_t N = c->N, k0 = c->size_Q, Ql = k0 - d;
     double *frac = (double *)malloc(d * sizeof(double));
     u64 *D = (u64 *)malloc(Ql * (d + 1) * 8);
@@ -1351,8 +1396,8 @@ void orc_hps_scale_q_ql(const orc_ctx *c, size_t d, const u64 *x, u64 *out) {
         }
     }
     for (size_t n = 0; n < N; n++) {
-        double nu = 0.5;
-        for (size_t j = 0; j < d; j++) nu = fma((double)x[(Ql + j) * N + n], frac[j], nu);
+        double nu = fma_chain(0.5, x + Ql * N + n, N, frac, d);
+        if (nu_out) nu_out[n] = nu;
         u64 alpha = (u64)nu;
         for (size_t i = 0; i < Ql; i++) {
             u64 qi = c->q[i];
@@ -1420,18 +1465,7 @@ int orc_bfv_multiply_hps_level(const orc_ctx *c, const orc_hps *h, size_t d, con
         u64 *dd = a + p * SN, *o = out3 + p * QN0;
         orc_ntt_inverse(c, dd, k, 0);
         r_ntt(h, N, dd + QN, 1);
-        for (size_t n = 0; n < N; n++) {
-            double nu = 0.5;
-            for (size_t j = 0; j < nr; j++) nu = fma((double)dd[QN + j * N + n], h->frac_r[j], nu);
-            u64 alpha = (u64)nu;
-            for (size_t i = 0; i < k; i++) {
-                u64 qi = c->q[i];
-                const u64 *dm = h->divs_mod_q + i * (nr + 1);
-                u128 acc = (u128)dd[i * N + n] * dm[nr];
-                for (size_t j = 0; j < nr; j++) acc += (u128)dd[QN + j * N + n] * dm[j];
-                o[i * N + n] = addmod((u64)(acc % qi), alpha % qi, qi);
-            }
-        }
+        orc_hps_scale_round(c, h, dd, o, NULL);
         if (d && (size_t)p < expand_polys) orc_hps_expand(c, d, o, o);
     }
     free(a); free(b);

@@ -10,10 +10,14 @@
  * All outputs are canonical residues in [0, q).
  *
  * Pinning (tests/test_oracle*.py): every routine is checked against closed
- * forms, known answers or the specification's vectors, except:
- *   - the FP64 tie behaviour of the HPS scale-and-round sums, in multiply
- *     and in BFV decryption (rounding boundaries; the closed forms hold within
- *     the FP64 bound): parity unpinned;
+ * forms, known answers or the specification's vectors. The FP64 sums of the
+ * HPS multiply (bConv_HPS's v, the scale-and-round nu of QR -> R, QlRl -> Ql
+ * and Q -> Ql) are pinned bit for bit by an exact-rational model of fused
+ * multiply-adds in input order (tests/test_oracle_hps_fp64.py), which holds if
+ * nvcc contracts the reference's `nu += (double) x * frac` that way (its
+ * default). Not pinned:
+ *   - the FP64 sum of the HPS forms of BFV decryption (rounding boundaries;
+ *     the closed form holds within the FP64 bound): parity unpinned;
  *   - orc_hps_levels_to_drop (FindLevelsToDrop): invariants only, its value is
  *     parity unpinned;
  *   - orc_sample_uniform_seeded: the Salsa20 core is pinned by the
@@ -128,6 +132,20 @@ int orc_bfv_multiply_hps2(const orc_ctx *c, const orc_hps *h, const uint64_t *ct
 int orc_bfv_multiply_hps(const orc_ctx *c, const orc_hps *h, const uint64_t *ct1, const uint64_t *ct2,
                          uint64_t *out3);
 
+/* The FP64 sums of the HPS path and the steps that use them, exported so that the
+ * tests can check every sum against an exact-rational model of sequential fused
+ * multiply-adds. orc_fma_chain: out[n] = fma(x[m-1][n], w[m-1], ... fma(x[0][n], w[0],
+ * init)) over nterms terms, x [nterms][N]. orc_bconv_hps: bConv_HPS of [ni][N] -> [no][N]
+ * with its sum (llround'ed to v) per coefficient. orc_hps_scale_round: the scale-and-round
+ * step of one polynomial [k + nr][N] (QR -> R into [nr][N] for hps, QlRl -> Ql into [k][N]
+ * for hps_overq and levels) with nu per coefficient. orc_hps_frac: the frac constants
+ * (tRSHatInvModsDivsFrac [k]; tQlSlHatInvModsDivsFrac [nr] for over-Q tools). */
+void orc_fma_chain(const uint64_t *x, size_t nterms, size_t N, const double *w, double init, double *out);
+void orc_bconv_hps(size_t N, const uint64_t *ib, size_t ni, const uint64_t *ob, size_t no, const uint64_t *in,
+                   uint64_t *out, double *sum);
+void orc_hps_scale_round(const orc_ctx *c, const orc_hps *h, const uint64_t *d, uint64_t *out, double *nu);
+void orc_hps_frac(const orc_hps *h, double *frac_q, double *frac_r);
+
 /* hps_overq_leveled: the HPS tool of the level with d dropped primes (Ql = size_Q - d,
  * Rl = the first Ql primes of R, the second operand switched from all of Q);
  * FindLevelsToDrop; scaleAndRound_HPS_Q_Ql ([size_Q][N] -> [Ql][N]); ExpandCRTBasis_Ql_Q
@@ -135,6 +153,8 @@ int orc_bfv_multiply_hps(const orc_ctx *c, const orc_hps *h, const uint64_t *ct1
 orc_hps *orc_hps_create_level(const orc_ctx *c, uint64_t t, size_t d);
 size_t orc_hps_levels_to_drop(const orc_ctx *c, uint64_t t, size_t depth, int key_switch, int asymmetric);
 void orc_hps_scale_q_ql(const orc_ctx *c, size_t d, const uint64_t *x, uint64_t *out);
+/* the same with nu (the FP64 sum of scaleAndRound_HPS_Q_Ql) per coefficient */
+void orc_hps_scale_q_ql_nu(const orc_ctx *c, size_t d, const uint64_t *x, uint64_t *out, double *nu);
 void orc_hps_expand(const orc_ctx *c, size_t d, const uint64_t *x, uint64_t *out);
 int orc_bfv_multiply_hps_level(const orc_ctx *c, const orc_hps *h, size_t d, const uint64_t *ct1,
                                const uint64_t *ct2, int square, size_t expand_polys, uint64_t *out3);

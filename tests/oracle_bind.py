@@ -94,6 +94,12 @@ def lib():
         L.orc_hps_r_size.argtypes = [ctypes.c_void_p]
         L.orc_hps_r.argtypes = [ctypes.c_void_p, _u64p]
         L.orc_bfv_multiply_hps.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p, _u64p, _u64p]
+        _dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_fma_chain.argtypes = [_u64p, _sz, _sz, _dp, ctypes.c_double, _dp]
+        L.orc_bconv_hps.argtypes = [_sz, _u64p, _sz, _u64p, _sz, _u64p, _u64p, _dp]
+        L.orc_hps_scale_round.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _u64p, _u64p, _dp]
+        L.orc_hps_frac.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_hps_scale_q_ql_nu.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u64p, _u64p, _dp]
         _lib = L
     return _lib
 
@@ -101,6 +107,32 @@ def lib():
 def _p(a):
     assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
     return a.ctypes.data_as(_u64p)
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def fma_chain(x, w, init):
+    """the oracle's FP64 sum of the HPS path: fma((double) x_i, w_i, acc) in order,
+    x [nterms][N] -> [N]"""
+    x = np.ascontiguousarray(x, np.uint64)
+    w = np.ascontiguousarray(w, np.float64)
+    out = np.zeros(x.shape[1], np.float64)
+    lib().orc_fma_chain(_p(x), x.shape[0], x.shape[1], _d(w), float(init), _d(out))
+    return out
+
+
+def bconv_hps(ib, ob, x):
+    """bConv_HPS [len(ib)][N] -> ([len(ob)][N], the FP64 sum per coefficient)"""
+    x = np.ascontiguousarray(x, np.uint64)
+    N = x.shape[1]
+    ib_a, ob_a = np.array(ib, np.uint64), np.array(ob, np.uint64)
+    out = np.zeros((len(ob), N), np.uint64)
+    s = np.zeros(N, np.float64)
+    lib().orc_bconv_hps(N, _p(ib_a), len(ib), _p(ob_a), len(ob), _p(x), _p(out), _d(s))
+    return out, s
 
 
 def set_threads(n):
@@ -327,6 +359,25 @@ class Hps:
         assert rc == 0
         return out
 
+    def frac(self):
+        """(tRSHatInvModsDivsFrac [k], tQlSlHatInvModsDivsFrac [nr] or None)"""
+        k = self.octx.size_QP - self.octx.size_P if not isinstance(self, HpsLevel) else self.octx.size_Q - self.d
+        fq = np.zeros(k, np.float64)
+        fr = np.zeros(len(self.r()), np.float64)
+        lib().orc_hps_frac(self._h, _d(fq), _d(fr))
+        return fq, (fr if self.overq else None)
+
+    def scale_round(self, d):
+        """the scale-and-round step of one polynomial d [k + nr][N]: (out, nu); out is
+        [nr][N] (QR -> R) for hps, [k][N] (QlRl -> Ql) for hps_overq and levels"""
+        d = np.ascontiguousarray(d, np.uint64)
+        nr = len(self.r())
+        k = d.shape[0] - nr
+        out = np.zeros((k if self.overq else nr, self.octx.N), np.uint64)
+        nu = np.zeros(self.octx.N, np.float64)
+        lib().orc_hps_scale_round(self.octx._h, self._h, _p(d), _p(out), _d(nu))
+        return out, nu
+
     def modswitch(self, x):
         """bConv_BEHZ_var1 Q -> R of x [k][N] (over-Q tool)"""
         out = np.zeros((len(self.r()), self.octx.N), np.uint64)
@@ -381,6 +432,14 @@ def hps_scale_q_ql(octx, d, x):
     out = np.zeros((octx.size_Q - d, octx.N), np.uint64)
     lib().orc_hps_scale_q_ql(octx._h, d, _p(np.ascontiguousarray(x, np.uint64)), _p(out))
     return out
+
+
+def hps_scale_q_ql_nu(octx, d, x):
+    """scaleAndRound_HPS_Q_Ql with its FP64 sum: ([size_Q - d][N], nu [N])"""
+    out = np.zeros((octx.size_Q - d, octx.N), np.uint64)
+    nu = np.zeros(octx.N, np.float64)
+    lib().orc_hps_scale_q_ql_nu(octx._h, d, _p(np.ascontiguousarray(x, np.uint64)), _p(out), _d(nu))
+    return out, nu
 
 
 def hps_expand(octx, d, x):
