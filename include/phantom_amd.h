@@ -90,6 +90,9 @@ int phx_ciphertext_resize(const phx_context *ctx, phx_ciphertext *ct, size_t cha
 int phx_ciphertext_upload(phx_ciphertext *ct, const uint64_t *host, size_t n_elems, phx_stream stream);
 int phx_ciphertext_download(const phx_ciphertext *ct, uint64_t *host, size_t n_elems, phx_stream stream);
 int phx_ciphertext_copy(const phx_context *ctx, phx_ciphertext *dst, const phx_ciphertext *src, phx_stream stream);
+/* PhantomCiphertext's copy constructor (include/ciphertext.h:31, a deep copy of the
+ * device buffer and every field): *out is a new handle */
+int phx_ciphertext_clone(const phx_ciphertext *src, phx_ciphertext **out, phx_stream stream);
 int phx_ciphertext_info(const phx_ciphertext *ct, size_t *chain_index, size_t *size, size_t *coeff_modulus_size,
                         double *scale, int *is_ntt_form);
 int phx_ciphertext_set_scale(phx_ciphertext *ct, double scale);

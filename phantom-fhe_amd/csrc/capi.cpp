@@ -282,6 +282,26 @@ int phx_ciphertext_copy(const phx_context *ctx, phx_ciphertext *dst, const phx_c
     });
 }
 
+int phx_ciphertext_clone(const phx_ciphertext *src, phx_ciphertext **out, phx_stream s) {
+    return guard([&] {
+        nn(src, "src");
+        nn(out, "out");
+        std::unique_ptr<phx_ciphertext> c(new phx_ciphertext{});
+        const Ciphertext &a = src->ct;
+        c->ct = a;
+        c->ct.data = nullptr;
+        c->ct.capacity = 0;
+        c->ct.owns = true;
+        const size_t words = (size_t)a.size * a.coeff_modulus_size * a.poly_degree;
+        if (a.data && words) {
+            c->ct.data = static_cast<u64 *>(dev_alloc(words * 8, S(s)));
+            c->ct.capacity = words;
+            PHX_HIP_CHECK(hipMemcpyAsync(c->ct.data, a.data, words * 8, hipMemcpyDeviceToDevice, S(s)));
+        }
+        *out = c.release();
+    });
+}
+
 int phx_ciphertext_info(const phx_ciphertext *ct, size_t *chain, size_t *size, size_t *L, double *scale, int *ntt) {
     return guard([&] {
         nn(ct, "ciphertext");

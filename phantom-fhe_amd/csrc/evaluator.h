@@ -96,14 +96,28 @@ void hoisting(const Context &ctx, Ciphertext &ct, const GaloisKeys &gk, const st
               hipStream_t s);
 u32 galois_elt_from_step(int step, u64 N);
 
+// the same operations in a caller's workspace (device memory of *_ws_words u64
+// words at the ciphertext's chain index): nothing is allocated
+size_t keyswitch_ws_words(const Context &ctx, u32 chain);
+void keyswitch_ws(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey &key, u64 *ct,
+                  const u64 *base0, const u64 *base1, u64 *ws, hipStream_t s);
+size_t multiply_and_relin_ws_words(const Context &ctx, u32 chain);
+// CKKS only
+void multiply_and_relin_ws(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, const KSwitchKey &rlk,
+                           u64 *ws, hipStream_t s);
+
 // ---- pieces exposed for tests / the low-level API ----
 // copy_own = false: the digits' own limbs of t_mod_up are left unwritten
 void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStream_t s, bool copy_own = true);
+void modup_ws(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, u64 *t_cks, hipStream_t s,
+              bool copy_own);
 // c2 given: the digits' own limbs are read from c2 (NTT form) instead of t_mod_up
 void inner_product(const Context &ctx, u32 chain, const u64 *t_mod_up, const KSwitchKey &key, u64 *cx,
                    hipStream_t s, const u64 *c2 = nullptr);
 void moddown(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, const u64 *base1, u32 npolys,
              hipStream_t s);
+void moddown_ws(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, const u64 *base1, u32 npolys,
+                u64 *delta, hipStream_t s);
 
 // streaming copy (bandwidth calibration)
 void copy_u64(u64 *dst, const u64 *src, size_t n, hipStream_t s);

@@ -600,14 +600,14 @@ void tensor_2x2_mod(const DevModulus *mod, u64 N, u32 L, const u64 *a, const u64
 }
 
 // DRNSTool::modup for CKKS (src/rns_bconv.cu:530-628); copy_own = false leaves
-// the digits' own limbs of t_mod_up unwritten (the inner product then reads c2)
-void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStream_t s, bool copy_own) {
+// the digits' own limbs of t_mod_up unwritten (the inner product then reads c2).
+// t_cks: Ql limbs of workspace for the scaled INTT of c2.
+void modup_ws(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, u64 *t_cks, hipStream_t s,
+              bool copy_own) {
     const LevelConsts &L = ctx.level(chain);
     const u64 N = ctx.N();
     const u32 Ql = L.size_Ql, QlP = L.size_QlP, beta = L.beta, alpha = ctx.size_P();
     require(alpha <= (u32)kMaxDigitLimbs, "special_modulus_size above 16 is not supported");
-    Scratch t_cks_buf((size_t)Ql * N * 8, s);
-    u64 *t_cks = t_cks_buf.get();
     // INTT(c2) * partQlHatInv (out of place)
     LimbMap mi = limb_map(t_cks, 0, Ql);
     mi.src = c2;
@@ -627,6 +627,11 @@ void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStrea
     ntt_forward(ctx.tables(), mf, QlP - last_sz, beta, s);
 }
 
+void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStream_t s, bool copy_own) {
+    Scratch t_cks((size_t)ctx.level(chain).size_Ql * ctx.N() * 8, s);
+    modup_ws(ctx, chain, c2, t_mod_up, t_cks.get(), s, copy_own);
+}
+
 void inner_product(const Context &ctx, u32 chain, const u64 *t_mod_up, const KSwitchKey &key, u64 *cx,
                    hipStream_t s, const u64 *c2) {
     const LevelConsts &L = ctx.level(chain);
@@ -639,9 +644,10 @@ void inner_product(const Context &ctx, u32 chain, const u64 *t_mod_up, const KSw
 }
 
 // DRNSTool::moddown_from_NTT for CKKS (src/rns_bconv.cu:776-828), both halves
-// in one launch per step, the tail fused into the last NTT pass.
-void moddown(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, const u64 *base1, u32 npolys,
-             hipStream_t s) {
+// in one launch per step, the tail fused into the last NTT pass. delta:
+// npolys x Ql limbs of workspace for the base conversion P -> Ql.
+void moddown_ws(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, const u64 *base1, u32 npolys,
+                u64 *delta, hipStream_t s) {
     const LevelConsts &L = ctx.level(chain);
     const u64 N = ctx.N();
     const u32 Ql = L.size_Ql, QlP = L.size_QlP, P = ctx.size_P();
@@ -649,8 +655,6 @@ void moddown(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, 
     LimbMap mi = limb_map(cx + (u64)Ql * N, ctx.size_Q(), P, (u64)QlP * N);
     mi.scale = L.moddown_scale;
     ntt_inverse(ctx.tables(), mi, P, npolys, s);
-    Scratch delta_buf((size_t)npolys * Ql * N * 8, s);
-    u64 *delta = delta_buf.get();
     bconv_launch(ctx, L.moddown_bconv, {L.h_moddown_bconv}, 0u, npolys, cx, (u64)QlP * N, delta, (u64)Ql * N,
                  nullptr, 0, s);
     StoreModDown st{};
@@ -665,20 +669,60 @@ void moddown(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, 
     ntt_forward_store(ctx.tables(), limb_map(delta, 0, Ql, (u64)Ql * N), Ql, npolys, st, s);
 }
 
-// keyswitch_inplace (src/eval_key_switch.cu:95-182)
-void keyswitch(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey &key, u64 *ct, const u64 *base0,
-               const u64 *base1, hipStream_t s) {
+void moddown(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, const u64 *base1, u32 npolys,
+             hipStream_t s) {
+    Scratch delta((size_t)npolys * ctx.level(chain).size_Ql * ctx.N() * 8, s);
+    moddown_ws(ctx, chain, cx, ct, base0, base1, npolys, delta.get(), s);
+}
+
+// workspace of one key switch at chain: t_mod_up (beta x QlP limbs), cx (2 x QlP),
+// the scaled INTT of c2 (Ql) and delta (2 x Ql)
+size_t keyswitch_ws_words(const Context &ctx, u32 chain) {
+    const LevelConsts &L = ctx.level(chain);
+    return (size_t)ctx.N() * ((size_t)L.beta * L.size_QlP + 2ull * L.size_QlP + 3ull * L.size_Ql);
+}
+
+// keyswitch_inplace (src/eval_key_switch.cu:95-182) in a caller's workspace
+void keyswitch_ws(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey &key, u64 *ct, const u64 *base0,
+                  const u64 *base1, u64 *ws, hipStream_t s) {
     const LevelConsts &L = ctx.level(chain);
     require(key.data != nullptr, "key-switching key is empty");
     require(key.dnum >= L.beta, "key-switching key has too few digits for this level");
     const u64 N = ctx.N();
-    Scratch t_mod_up_buf((size_t)L.beta * L.size_QlP * N * 8, s);
-    u64 *t_mod_up = t_mod_up_buf.get();
-    Scratch cx_buf((size_t)2 * L.size_QlP * N * 8, s);
-    u64 *cx = cx_buf.get();
-    modup(ctx, chain, c2, t_mod_up, s, false);
+    u64 *t_mod_up = ws;
+    u64 *cx = t_mod_up + (u64)L.beta * L.size_QlP * N;
+    u64 *t_cks = cx + 2ull * L.size_QlP * N;
+    u64 *delta = t_cks + (u64)L.size_Ql * N;
+    modup_ws(ctx, chain, c2, t_mod_up, t_cks, s, false);
     inner_product(ctx, chain, t_mod_up, key, cx, s, c2);
-    moddown(ctx, chain, cx, ct, base0, base1, 2, s);
+    moddown_ws(ctx, chain, cx, ct, base0, base1, 2, delta, s);
+}
+
+void keyswitch(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey &key, u64 *ct, const u64 *base0,
+               const u64 *base1, hipStream_t s) {
+    Scratch ws(keyswitch_ws_words(ctx, chain) * 8, s);
+    keyswitch_ws(ctx, chain, c2, key, ct, base0, base1, ws.get(), s);
+}
+
+size_t multiply_and_relin_ws_words(const Context &ctx, u32 chain) {
+    return (size_t)ctx.level(chain).size_Ql * ctx.N() + keyswitch_ws_words(ctx, chain);
+}
+
+// CKKS multiply_and_relin_inplace in a caller's workspace: c2 (Ql limbs) then
+// the key switch's workspace
+void multiply_and_relin_ws(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, const KSwitchKey &rlk,
+                           u64 *ws, hipStream_t s) {
+    require(ctx.params().scheme == kSchemeCKKS, "multiply_and_relin: unsupported scheme");
+    require(ct1.chain_index == ct2.chain_index, "encrypted1 and encrypted2 parameter mismatch");
+    require(ct1.is_ntt_form == ct2.is_ntt_form, "NTT form mismatch");
+    require(ct1.is_ntt_form, "encrypted1 and encrypted2 must be in NTT form");
+    require(ct1.size == 2 && ct2.size == 2, "poly number mismatch");
+    const u32 Ql = ct1.coeff_modulus_size;
+    const u64 N = ctx.N(), LN = (u64)Ql * N;
+    u64 *c2 = ws;
+    tensor_2x2(ctx, Ql, ct1.data, ct2.data, ct1.data, ct1.data + LN, c2, s);
+    keyswitch_ws(ctx, ct1.chain_index, c2, rlk, ct1.data, ct1.data, ct1.data + LN, ws + LN, s);
+    ct1.scale = ct1.scale * ct2.scale;
 }
 
 // multiply_and_relin_inplace for CKKS (src/evaluate.cu:1061-1104)
@@ -697,17 +741,8 @@ void multiply_and_relin(const Context &ctx, Ciphertext &ct1, const Ciphertext &c
         return;
     }
     require(ctx.params().scheme == kSchemeCKKS, "multiply_and_relin: unsupported scheme");
-    require(ct1.chain_index == ct2.chain_index, "encrypted1 and encrypted2 parameter mismatch");
-    require(ct1.is_ntt_form == ct2.is_ntt_form, "NTT form mismatch");
-    require(ct1.is_ntt_form, "encrypted1 and encrypted2 must be in NTT form");
-    require(ct1.size == 2 && ct2.size == 2, "poly number mismatch");
-    const u32 Ql = ct1.coeff_modulus_size;
-    const u64 N = ctx.N(), LN = (u64)Ql * N;
-    Scratch c2_buf(LN * 8, s);
-    u64 *c2 = c2_buf.get();
-    tensor_2x2(ctx, Ql, ct1.data, ct2.data, ct1.data, ct1.data + LN, c2, s);
-    keyswitch(ctx, ct1.chain_index, c2, rlk, ct1.data, ct1.data, ct1.data + LN, s);
-    ct1.scale = ct1.scale * ct2.scale;
+    Scratch ws(multiply_and_relin_ws_words(ctx, ct1.chain_index) * 8, s);
+    multiply_and_relin_ws(ctx, ct1, ct2, rlk, ws.get(), s);
 }
 
 // relinearize_inplace (src/evaluate.cu:1342-1374)

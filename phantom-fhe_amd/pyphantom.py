@@ -121,6 +121,7 @@ _SIGS = {
     "phx_ckks_encode": (ctypes.c_int, [_vp, _dp, _dp, _sz, ctypes.c_double, _sz, _u64p]),
     "phx_ckks_decode": (ctypes.c_int, [_vp, _u64p, _sz, ctypes.c_double, _dp, _dp]),
     "phx_multiply_and_relin_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "phx_ciphertext_clone": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
     "phx_multiply_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "phx_relinearize_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "phx_rescale_to_next": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
@@ -324,6 +325,14 @@ class ciphertext:
 
     def resize(self, ctx, chain_index, size, stream=None):
         _check(lib().phx_ciphertext_resize(ctx._h, self._h, chain_index, size, _stream(stream)))
+
+    def clone(self, stream=None):
+        """deep copy (the reference's copy constructor)"""
+        c = ciphertext.__new__(ciphertext)
+        h = _vp()
+        _check(lib().phx_ciphertext_clone(self._h, ctypes.byref(h), _stream(stream)))
+        c._h = h
+        return c
 
     def info(self):
         c, s, L = _sz(), _sz(), _sz()

@@ -87,3 +87,16 @@ def test_tune_cache_roundtrip(tmp_path):
         f.write('{"entries": [\n  {"key": "scheme=ckks,N=4096,Q=2,P=1", "ntt_fused": 1}\n]}\n')
     with pytest.raises(ValueError):
         ph.tune_cache_lookup(path, ph.SCHEME_CKKS, 4096, 2, 1)
+
+
+def test_cpp_facade_header_compiles(tmp_path):
+    """include/phantom.hpp (the reference's class names over the C ABI) compiles as a
+    translation unit of its own with the host compiler"""
+    import subprocess
+    src = tmp_path / "tu.cpp"
+    src.write_text('#include "phantom.hpp"\nint main() { phantom::EncryptionParameters p(phantom::scheme_type::ckks);'
+                   ' (void)p; return 0; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        "-I" + os.path.join(ROOT, "include"), "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

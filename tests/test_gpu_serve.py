@@ -130,3 +130,100 @@ def test_serving_rejects_bad_requests(lanes):
     tiny = ph.server([(c, r, g) for c, r, g, _ in lanes], mem_budget=1 << 10)
     with pytest.raises(ValueError):
         tiny.submit(0, ct, [1])                 # larger than the whole budget
+
+
+# ---- the serving workload at its full shape: N in {2^15, 2^16}, 35 data limbs
+# (60 + 34 x 50 bit) + 12 special (60 bit), dnum 3, start level 10..35 limbs,
+# chains of k in 1..8 links, Poisson arrivals ----
+SHAPE_BITS = [60] + [50] * 34 + [60] * 12
+SHAPE_STEPS = [1, 2, -1, 5, 16, -7]
+
+
+@pytest.fixture(scope="module")
+def shape_lanes():
+    import oracle_bind as ob
+
+    ob.set_threads(16)
+    out = []
+    for lane, N in enumerate((1 << 15, 1 << 16)):
+        ctx, octx, mods = make_ctx(N, SHAPE_BITS, 12)
+        rng = np.random.default_rng(500 + lane)
+        rkey = rand_key(rng, mods, N, ctx.dnum)
+        rk = ph.relin_key(ctx, host=rkey)
+        elts = [ph.galois_elt_from_step(s, N) for s in SHAPE_STEPS]
+        gkeys = [rand_key(rng, mods, N, ctx.dnum) for _ in elts]
+        gk = ph.galois_key(ctx, elts, host=np.stack(gkeys))
+        out.append((ctx, rk, gk, mods, octx, rkey, dict(zip(SHAPE_STEPS, zip(elts, gkeys)))))
+    return out
+
+
+def oracle_chain(octx, rkey, gkeys, data, steps):
+    """the chain on the oracle: (multiply_and_relin of ct with itself, rescale,
+    apply_galois with the step's key) per link (src/evaluate.cu:1061-1104,
+    1545-1565, 1567-1630)"""
+    ct = data
+    for st in steps:
+        Ql = ct.shape[1]
+        ct = octx.multiply_and_relin(Ql, ct, ct, rkey)
+        ct = octx.rescale(Ql, ct)
+        elt, key = gkeys[st]
+        ct = octx.apply_galois(Ql - 1, ct, elt, key)
+    return ct
+
+
+def test_serving_full_shape_mixed_equals_alone_and_oracle(shape_lanes):
+    say("case: serving at N in {2^15, 2^16}, 10..35 limbs, k in 1..8, Poisson 400/s")
+    rng = np.random.default_rng(21)
+    n_req = 24
+    reqs = []
+    for _ in range(n_req):
+        lane = int(rng.integers(0, 2))
+        ctx, _, _, mods = shape_lanes[lane][:4]
+        k = int(rng.integers(1, 9))
+        limbs = int(rng.integers(max(10, k + 1), 36))
+        chain = ctx.size_Q - limbs + 1
+        data = rand_ct(rng, mods[:limbs], ctx.N)
+        steps = [SHAPE_STEPS[j] for j in rng.integers(0, len(SHAPE_STEPS), k)]
+        reqs.append((lane, chain, data, steps))
+    assert {r[0] for r in reqs} == {0, 1} and max(len(r[3]) for r in reqs) >= 7
+    srv = ph.server([(c, r, g) for c, r, g in (l[:3] for l in shape_lanes)], mem_budget=24 << 30,
+                    max_active=256, depth=2)
+    arrivals = np.cumsum(rng.exponential(1 / 400.0, n_req))
+    t0 = srv.now()
+    ids, i = [], 0
+    while i < n_req:
+        now = srv.now() - t0
+        while i < n_req and arrivals[i] <= now:
+            lane, chain, data, steps = reqs[i]
+            ct = upload_ct(shape_lanes[lane][0], data, chain)
+            ct.set_scale(2.0 ** 50)
+            ids.append(srv.submit(lane, ct, steps, t0 + arrivals[i]))
+            i += 1
+        if srv.step() == 0:
+            time.sleep(0.0005)
+    srv.drain()
+    st = srv.stats()
+    say(f"serving stats: {st}")
+    assert st["completed"] == n_req and st["peak_active"] > 1
+    results = []
+    for rid, (lane, chain, data, steps) in zip(ids, reqs):
+        got, lat = srv.take(rid)
+        ctx, rk, gk = shape_lanes[lane][:3]
+        ref = upload_ct(ctx, data, chain)
+        ref.set_scale(2.0 ** 50)
+        ref = alone(ctx, rk, gk, ref, steps)
+        g = got.download(ctx.N)
+        assert got.chain_index() == chain + len(steps)
+        assert np.array_equal(g, ref.download(ctx.N)), (rid, lane, chain, steps)
+        results.append(g)
+    # two requests per lane against the oracle's chain (the cheapest ones: the
+    # oracle's key switch at N = 2^16 takes about a second per link)
+    for lane in (0, 1):
+        idx = [j for j in range(n_req) if reqs[j][0] == lane]
+        idx.sort(key=lambda j: len(reqs[j][3]) * reqs[j][2].shape[1])
+        for j in idx[:2]:
+            _, chain, data, steps = reqs[j]
+            octx, rkey, gkeys = shape_lanes[lane][4:]
+            say(f"case: oracle chain lane {lane} limbs {data.shape[1]} steps {steps}")
+            want = oracle_chain(octx, rkey, gkeys, data, steps)
+            assert np.array_equal(results[j], want), (j, lane, steps)
