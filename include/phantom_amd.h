@@ -213,6 +213,15 @@ int phx_ckks_decode(const phx_context *ctx, const uint64_t *in_host, size_t chai
 /* multiply_and_relin_inplace (include/evaluate.cuh:115-116, src/evaluate.cu:1061-1104) */
 int phx_multiply_and_relin_inplace(const phx_context *ctx, phx_ciphertext *ct1, const phx_ciphertext *ct2,
                                    const phx_relin_key *rlk, phx_stream stream);
+/* No counterpart in the reference: multiply_and_relin_inplace for CKKS that allocates
+ * nothing, for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Its workspace is a static device
+ * array of 192 MiB per GPU (enough for N = 2^16 with 35 data limbs, 12 special limbs and
+ * 3 digits); larger shapes return PHX_EINVAL before any kernel runs. Calls on one GPU
+ * share that array: overlapping calls on one GPU race. Results equal
+ * phx_multiply_and_relin_inplace's bit for bit. */
+int phx_multiply_and_relin_inplace_noalloc(const phx_context *ctx, phx_ciphertext *ct1, const phx_ciphertext *ct2,
+                                           const phx_relin_key *rlk, phx_stream stream);
 /* multiply_inplace (include/evaluate.cuh:104-105): tensor only, result size 3 */
 int phx_multiply_inplace(const phx_context *ctx, phx_ciphertext *ct1, const phx_ciphertext *ct2, phx_stream stream);
 /* relinearize_inplace (include/evaluate.cuh:125-126, src/evaluate.cu:1342-1374) */

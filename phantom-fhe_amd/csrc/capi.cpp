@@ -849,6 +849,20 @@ int phx_multiply_and_relin_inplace(const phx_context *ctx, phx_ciphertext *ct1, 
     });
 }
 
+int phx_multiply_and_relin_inplace_noalloc(const phx_context *ctx, phx_ciphertext *ct1, const phx_ciphertext *ct2,
+                                           const phx_relin_key *rlk, phx_stream s) {
+    return guard([&] {
+        nn(ctx, "context");
+        nn(ct1, "encrypted1");
+        nn(ct2, "encrypted2");
+        nn(rlk, "relin_keys");
+        check_ct(*ctx->ctx, ct1->ct, "encrypted1");
+        check_ct(*ctx->ctx, ct2->ct, "encrypted2");
+        check_key(*ctx->ctx, rlk->key, "relin_keys");
+        multiply_and_relin_noalloc(*ctx->ctx, ct1->ct, ct2->ct, rlk->key, S(s));
+    });
+}
+
 int phx_multiply_inplace(const phx_context *ctx, phx_ciphertext *ct1, const phx_ciphertext *ct2, phx_stream s) {
     return guard([&] {
         nn(ctx, "context");

@@ -105,6 +105,10 @@ size_t multiply_and_relin_ws_words(const Context &ctx, u32 chain);
 // CKKS only
 void multiply_and_relin_ws(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, const KSwitchKey &rlk,
                            u64 *ws, hipStream_t s);
+// multiply_and_relin_ws in the library's static device scratch (no allocation);
+// sizes beyond it are refused
+void multiply_and_relin_noalloc(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, const KSwitchKey &rlk,
+                                hipStream_t s);
 
 // ---- pieces exposed for tests / the low-level API ----
 // copy_own = false: the digits' own limbs of t_mod_up are left unwritten

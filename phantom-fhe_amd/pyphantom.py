@@ -121,6 +121,7 @@ _SIGS = {
     "phx_ckks_encode": (ctypes.c_int, [_vp, _dp, _dp, _sz, ctypes.c_double, _sz, _u64p]),
     "phx_ckks_decode": (ctypes.c_int, [_vp, _u64p, _sz, ctypes.c_double, _dp, _dp]),
     "phx_multiply_and_relin_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "phx_multiply_and_relin_inplace_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "phx_ciphertext_clone": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
     "phx_multiply_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "phx_relinearize_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
@@ -736,6 +737,11 @@ class galois_key:
 # ------------------------------------------------------------------ evaluator
 def multiply_and_relin_inplace(ctx, ct1, ct2, rlk, stream=None):
     _check(lib().phx_multiply_and_relin_inplace(ctx._h, ct1._h, ct2._h, rlk._h, _stream(stream)))
+
+
+def multiply_and_relin_inplace_noalloc(ctx, ct1, ct2, rlk, stream=None):
+    """CKKS multiply_and_relin_inplace in the library's static device scratch (no allocation)"""
+    _check(lib().phx_multiply_and_relin_inplace_noalloc(ctx._h, ct1._h, ct2._h, rlk._h, _stream(stream)))
 
 
 def multiply_and_relin(ctx, ct1, ct2, rlk, stream=None):
