@@ -328,7 +328,9 @@ def bench_copy(ph, n_u64, iters, stream):
 def cpu_baseline(N, bits, size_P, budget_s):
     import oracle_bind as ob
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    # the host's CPU share for one GPU is 16 threads; all of them when fewer exist
+    host_cpus = os.cpu_count() or 1
+    threads = max(1, min(16, host_cpus))
     ob.set_threads(threads)
     mods = ob.coeff_modulus_create(N, bits)
     octx = ob.Ctx(N.bit_length() - 1, mods, size_P)
@@ -353,7 +355,7 @@ def cpu_baseline(N, bits, size_P, budget_s):
     el1 = time.perf_counter() - t1
     ob.set_threads(threads)
     return {"value": n / el, "unit": "HMult+relin/s", "cores": threads, "kind": "port",
-            "value_1thread": 1.0 / el1,
+            "host_cpus": host_cpus, "value_1thread": 1.0 / el1,
             "sample": f"{n} HMult+relin at N=2^16, L=35+12, dnum=3 on the oracle (CPU restatement of the "
                       f"reference algorithm, {threads} threads), {el:.1f} s; plus one op on 1 thread, {el1:.1f} s"}
 
@@ -373,6 +375,9 @@ def main():
                     help="only the plain and batched HMult+relin (for kernel-trace runs)")
     ap.add_argument("--serve-qps", type=float, default=400.0)
     ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-sweep", type=float, nargs="*", default=[200.0, 400.0, 800.0, 1600.0],
+                    help="offered loads (requests/s) of the serving sweep")
+    ap.add_argument("--serve-sweep-requests", type=int, default=300)
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the autotuner (kernel variants stay at the process-wide switches)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
@@ -449,6 +454,16 @@ def main():
                           sk_.create_galois_keys(c_, [ph.galois_elt_from_step(st, n_) for st in ROT_STEPS], seed=92)))
         serving = bench_serving(ph, lanes, args.serve_qps, args.serve_requests)
         log(f"bench: serving {serving}")
+        # offered-load sweep: p50/p99 and throughput at each load, the saturation rate
+        sweep = []
+        for qps in args.serve_sweep:
+            r = bench_serving(ph, lanes, qps, args.serve_sweep_requests, seed=56)
+            sweep.append({"qps": qps, "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"], "req_per_s": r["req_per_s"],
+                          "ops_per_s": r["ops_per_s"], "peak_active": r["peak_active"]})
+            log(f"bench: serving sweep {sweep[-1]}")
+        if sweep:
+            serving["sweep"] = sweep
+            serving["saturation_req_per_s"] = max(x["req_per_s"] for x in sweep)
     ckks_err = ckks_error_report(ph, ctx)
     log(f"bench: CKKS max slot error vs FP64 (scale 2^50): {ckks_err}")
 

@@ -252,8 +252,8 @@ int phx_keyswitch_inplace(const phx_context *ctx, phx_ciphertext *ct, const uint
 /* ---- batched hot path (no single reference counterpart: it runs the reference's
  * multiply_and_relin_inplace / rotate_inplace over many ciphertexts at once) -----------
  * A batch binds a fixed list of ciphertexts (their device buffers must not be resized
- * while the batch lives) to preallocated scratch; ciphertexts are grouped by level and
- * every stage of the pipeline is one launch per group. run() launches eagerly,
+ * while the batch lives) to preallocated scratch and device work lists; every stage
+ * of the pipeline is one launch over all items, whatever their levels. run() launches eagerly,
  * capture() records the launches into a hipGraph on an explicit (non-NULL) stream and
  * replay() launches that graph. Results are bit-identical to the one-at-a-time calls. */
 typedef struct phx_batch phx_batch;
@@ -277,6 +277,9 @@ int phx_batch_capture(phx_batch *batch, phx_stream stream);
 int phx_batch_replay(phx_batch *batch, phx_stream stream);
 /* bytes of device scratch the batch holds */
 int phx_batch_scratch_bytes(const phx_batch *batch, size_t *bytes);
+/* kernel launches one run issues (the kernel nodes of the captured graph): one per
+ * pipeline stage across all items and levels of the batch */
+int phx_batch_launches(const phx_batch *batch, size_t *launches);
 int phx_batch_destroy(phx_batch *batch);
 
 /* ---- serving loop (no single reference counterpart: a request loop around the

@@ -1,19 +1,22 @@
 // batch.h -- batched hot path: many ciphertexts per launch, hipGraph replay.
 //
-// A BatchPlan binds a fixed list of ciphertexts (and keys) to scratch and
-// device pointer tables. Ciphertexts are grouped by level; each pipeline
-// stage of each group is one launch over the whole group, so the small
-// per-ciphertext launches of the plain path become a few large ones, and the
-// key inner product reads a shared key once per coefficient for the whole
-// group. The groups run one after another on one stream, so they share one
-// scratch arena sized for the largest group.
+// A BatchPlan binds a fixed list of ciphertexts (and keys) to scratch and to
+// device work lists. Every pipeline stage is ONE launch over all items of the
+// plan, whatever their levels: the NTT and FastBConv launches take a table of
+// (polynomial, limb) entries with their own buffers, table rows and level
+// constants; the tensor / permutation and key inner product take an item table
+// and skip the limbs an item's level does not have. So a plan over ciphertexts
+// of thirty different levels issues the same dozen launches as a plan over
+// one level, and the key inner product reads a key shared by items of any level
+// once per coefficient.
 //
-// Persistent plans allocate the arena and tables once at creation: run()
-// launches eagerly; capture() records run() into a hipGraph that replay()
-// launches with one call (nothing is allocated, set or synchronised inside
-// run()). Transient plans (the serving loop) take the arena from the
-// stream-ordered pool inside run() and return it at the end of the same
-// stream's work; the plan object must outlive that work.
+// Persistent plans allocate the arena (one scratch region per item) and the
+// device tables at creation: run() launches eagerly; capture() records run()
+// into a hipGraph that replay() launches with one call (nothing is allocated,
+// set or synchronised inside run()). Transient plans (the serving loop) run in
+// memory the caller hands over (run_with) or take it from the stream-ordered
+// pool inside run(); they upload their tables on the same stream, and the plan
+// object must outlive that work.
 //
 // BFV plans (multiply + relinearise at the top level) stage both operands of
 // every item contiguously, run each multiply stage once over all items, key-
@@ -26,12 +29,13 @@
 #include <vector>
 
 #include "evaluator.h"
+#include "ntt.h"
 
 namespace phx {
 
 class BatchPlan {
 public:
-    enum Kind { kMulRelin = 1, kRotate = 2, kRelin = 3, kBfvMulRelin = 4, kJoin = 5 };
+    enum Kind { kMulRelin = 1, kRotate = 2, kRelin = 3, kBfvMulRelin = 4, kJoin = 5, kRescale = 6 };
 
     // ct1[i] *= ct2[i], relinearised with rlk. CKKS: all in NTT form, any level.
     // BFV (persistent plans only): coefficient form at the top level, multiplied
@@ -44,63 +48,89 @@ public:
     // cts[i] <- rotate(cts[i], steps[i]); every step's Galois element must have a key in gk
     static BatchPlan *rotate(const Context &ctx, const std::vector<Ciphertext *> &cts, const std::vector<int> &steps,
                              const GaloisKeys &gk, bool transient = false);
+    // outs[i] <- rescale_to_next(ins[i]) (CKKS, NTT form, any levels); outs[i] must
+    // already be sized for chain index ins[i].chain_index + 1 and differ from ins[i]
+    static BatchPlan *rescale(const Context &ctx, const std::vector<const Ciphertext *> &ins,
+                              const std::vector<Ciphertext *> &outs, bool transient = false);
     // scratch bytes one item at chain index `chain` needs (admission budgets)
     static size_t item_scratch_bytes(const Context &ctx, u32 chain, Kind k);
     ~BatchPlan();
 
     void run(hipStream_t s);
+    // transient plans: run in caller memory of at least scratch_bytes() and
+    // table_bytes() (device), free of other work on s from here on
+    void run_with(hipStream_t s, u64 *arena, uint8_t *tables);
+    size_t table_bytes() const;
     void capture(hipStream_t s);
     void replay(hipStream_t s);
-    size_t size() const { return items_; }
+    size_t size() const { return items_.size() + joined_items_; }
     size_t scratch_bytes() const { return scratch_bytes_; }
+    // kernel launches one run issues (the graph's kernel nodes)
+    size_t launches() const;
 
 private:
-    static constexpr u32 kTables = 6;   // a, b, out, keys, perms, key-switch outputs (BFV)
-    struct Group {
-        u32 chain = 0, Ql = 0, QlP = 0, beta = 0, B = 0;
-        std::vector<u32> items;
-        size_t table0 = 0;             // first entry of this group's tables
-        // device pointer tables [B]
-        const u64 **a = nullptr;       // operand 1 / ciphertext data
-        const u64 **b = nullptr;       // operand 2 (mul-relin)
-        u64 **out = nullptr;           // output ciphertext data
-        const u64 **keys = nullptr;    // key data per item
-        const u32 **perms = nullptr;   // Galois permutation per item
-        u64 **ks_out = nullptr;        // BFV: the key switch adds into these (NTT form)
-        // scratch (views into the shared arena)
-        u64 *c2 = nullptr, *t_cks = nullptr, *t_mod_up = nullptr, *cx = nullptr, *delta = nullptr,
-            *base0 = nullptr;
-        u64 c2_stride = 0;             // words between items' c2
-        // BFV: staged operands [B][2][Ql][N], multiply work, product [B][3][Ql][N]
+    struct Item {
+        u32 chain = 0, Ql = 0, QlP = 0, beta = 0;
+        const u64 *a = nullptr;      // operand 1 / the ciphertext (device)
+        const u64 *b = nullptr;      // operand 2 (mul-relin)
+        u64 *out = nullptr;          // output ciphertext data
+        const u64 *key = nullptr;    // key-switching key
+        const u32 *perm = nullptr;   // Galois permutation (rotate)
+        size_t off = 0;              // first word of the item's scratch in the arena
+    };
+    // BFV staging (one group at the top level): operands, multiply work, product
+    struct BfvStage {
         u64 *in1 = nullptr, *in2 = nullptr, *work = nullptr, *c3 = nullptr;
     };
+
     BatchPlan(const Context &ctx, Kind k, bool transient) : ctx_(ctx), kind_(k), transient_(transient) {}
-    void build_groups(const std::vector<u32> &chains);
-    // host tables for group g: item i's a, b, out, key, perm
-    void set_tables(Group &g, const std::vector<const void *> &a, const std::vector<const void *> &b,
-                    const std::vector<const void *> &out, const std::vector<const void *> &keys,
-                    const std::vector<const void *> &perms);
-    size_t group_scratch_words(const Group &g) const;
-    void layout();                               // table offsets, arena size
-    void bind(u64 *arena, const void **tables);  // group views into device memory
-    // persistent plans: bind to `arena`, point arena-relative table entries at it and
-    // upload the tables
-    void rebind(u64 *arena);
-    void run_group(const Group &g, hipStream_t s);
-    void keyswitch_group(const Group &g, hipStream_t s);
-    void record(hipStream_t s);                  // every launch of the plan
-    void after_launch();                         // host-side metadata of the outputs
+    size_t item_words(const Item &it) const;
+    void layout();                           // scratch offsets, arena size; persistent: allocate and bind
+    void bind(u64 *arena);                   // build the host work lists against an arena
+    void upload_tables(hipStream_t s, bool async);
+    void record(hipStream_t s);              // every launch of the plan
+    void record_keyswitch(hipStream_t s);
+    void after_launch();                     // host-side metadata of the outputs
 
     const Context &ctx_;
     Kind kind_;
     bool transient_ = false;
-    size_t items_ = 0;
+    std::vector<Item> items_;
+    size_t joined_items_ = 0;
     size_t scratch_bytes_ = 0;       // arena bytes
-    std::vector<Group> groups_;
-    std::vector<const void *> host_tables_;   // [group tables][kTables][B]
-    u64 *arena_ = nullptr;                    // persistent plans
-    const void **tables_ = nullptr;
+    size_t bfv_words_ = 0;           // BFV staging words at the start of the arena
+    BfvStage bfv_;
+    u64 *arena_ = nullptr;           // persistent plans
+
+    // host work lists and their device copy (one buffer, sections at offsets)
+    struct PreItem {
+        const u64 *a, *b;
+        u64 *out;
+        u64 *c2, *base0;
+        const u32 *perm;
+        u32 Ql, pad;
+    };
+    struct IpItem {
+        const u64 *t, *c2;
+        u64 *cx;
+        const u64 *key;
+        u32 Ql, QlP, beta, pad;
+    };
+    std::vector<PreItem> pre_;
+    std::vector<IpItem> ip_;
+    std::vector<NttJob> mu_inv_, mu_fwd_, md_inv_, md_fwd_;
+    std::vector<BconvRow> mu_bc_, md_bc_;
+    std::vector<BconvDesc> mu_desc_host_, md_desc_host_;   // every descriptor the rows use
+    u32 max_Ql_ = 0, max_beta_ = 0;
+    std::vector<uint8_t> host_tables_;
+    uint8_t *tables_ = nullptr;      // device copy (persistent plans)
+    uint8_t *tables_bound_ = nullptr;
+    size_t off_pre_ = 0, off_ip_ = 0, off_mu_inv_ = 0, off_mu_fwd_ = 0, off_md_inv_ = 0, off_md_fwd_ = 0,
+           off_mu_bc_ = 0, off_md_bc_ = 0, off_bfv_sq_ = 0;
+    std::vector<u64> bfv_square_;    // BFV: per item 1 when both operands are one ciphertext
+
     std::vector<Ciphertext *> cts_;
+    std::vector<const Ciphertext *> ins_;   // rescale inputs
     std::vector<double> scale_mul_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;

@@ -1,157 +1,144 @@
-// batch.hip -- batched multiply-and-relinearise and rotation (see batch.h).
+// batch.hip -- batched multiply-and-relinearise, rotation and rescale over work
+// lists (see batch.h).
 #include <algorithm>
-#include <map>
+#include <cstring>
 #include <memory>
 
 #include "batch.h"
 #include "ntt_arith.h"
 #include "bfv.h"
-#include "tune.h"
 
 namespace phx {
 
 namespace {
 constexpr int kThreads = 256;
 inline dim3 grid3(u64 N, u32 y, u32 z) { return dim3((u32)((N + kThreads - 1) / kThreads), y, z); }
+constexpr u32 kMaxGridY = 65535;
 }  // namespace
 
-// tensor of item z: (d0, d1) into out[z], d2 into c2[z]
-// (tensor_prod_2x2_rns_poly, src/polymath.cu:463-498)
-__global__ __launch_bounds__(kThreads) void k_tensor_batch(const u64 *const *a, const u64 *const *b, u64 *const *out,
-                                                           u64 *__restrict__ c2, const DevModulus *__restrict__ mod,
-                                                           u32 N, u32 Ql) {
-    const u32 limb = blockIdx.y, item = blockIdx.z;
+// tensor of item z at limb y < its Ql: (d0, d1) into out, d2 into c2
+// (tensor_prod_2x2_rns_poly, src/polymath.cu:463-498). out may be a (in place).
+template <class PreItem>
+__global__ __launch_bounds__(kThreads) void k_tensor_items(const PreItem *items, const DevModulus *__restrict__ mod,
+                                                           u32 N) {
+    const PreItem it = items[blockIdx.z];
+    const u32 limb = blockIdx.y;
     const u32 k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= N) return;
-    const u64 LN = (u64)Ql * N, o = (u64)limb * N + k;
-    const u64 *pa = a[item], *pb = b[item];
+    if (limb >= it.Ql || k >= N) return;
+    const u64 LN = (u64)it.Ql * N, o = (u64)limb * N + k;
     const DevModulus m = mod[limb];
     const u64 q = m.value;
-    const u64 a0 = pa[o], a1 = pa[LN + o], b0 = pb[o], b1 = pb[LN + o];
+    const u64 a0 = it.a[o], a1 = it.a[LN + o], b0 = it.b[o], b1 = it.b[LN + o];
     const u64 r0 = mul_mod(a0, b0, m);
     const u64 r2 = mul_mod(a1, b1, m);
     u64 r1 = mul_mod(add_mod(a0, a1, q), add_mod(b0, b1, q), m);
     r1 = sub_mod(sub_mod(r1, r0, q), r2, q);
-    u64 *po = out[item];
-    po[o] = r0;
-    po[LN + o] = r1;
-    c2[(u64)item * LN + o] = r2;
+    it.out[o] = r0;
+    it.out[LN + o] = r1;
+    it.c2[o] = r2;
 }
 
 // Galois permutation of item z (apply_galois_ntt_permutation, src/galois.cu:11-18):
-// base0[z] = pi(c0), c2[z] = pi(c1)
-__global__ __launch_bounds__(kThreads) void k_permute_batch(const u64 *const *ct, const u32 *const *perm,
-                                                            u64 *__restrict__ base0, u64 *__restrict__ c2, u32 N,
-                                                            u32 Ql) {
-    const u32 limb = blockIdx.y, item = blockIdx.z;
+// base0 = pi(c0), c2 = pi(c1)
+template <class PreItem>
+__global__ __launch_bounds__(kThreads) void k_permute_items(const PreItem *items, u32 N) {
+    const PreItem it = items[blockIdx.z];
+    const u32 limb = blockIdx.y;
     const u32 k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= N) return;
-    const u64 LN = (u64)Ql * N, o = (u64)limb * N;
-    const u64 *src = ct[item];
-    const u32 pk = perm[item][k];
-    base0[(u64)item * LN + o + k] = src[o + pk];
-    c2[(u64)item * LN + o + k] = src[LN + o + pk];
+    if (limb >= it.Ql || k >= N) return;
+    const u64 LN = (u64)it.Ql * N, o = (u64)limb * N;
+    const u32 pk = it.perm[k];
+    it.base0[o + k] = it.a[o + pk];
+    it.c2[o + k] = it.a[LN + o + pk];
 }
 
-// key inner product over a group (key_switch_inner_prod_c2_and_evk,
-// src/eval_key_switch.cu:14-69): items sharing a key read it once per coefficient;
-// digit j's own limbs (pos / alpha == j) are read from c2 (NTT form) directly.
-// MB = the digit capacity (registers caching a shared key's digits)
-template <u32 MB>
-__global__ __launch_bounds__(kThreads) void k_inner_product_batch(const u64 *__restrict__ t, const u64 *__restrict__ c2,
-                                                                  const u64 *const *keys, u64 *__restrict__ cx,
-                                                                  const DevModulus *__restrict__ mod, u32 N, u32 Ql,
-                                                                  u32 QlP, u32 size_Q, u32 QP, u32 beta, u32 alpha,
-                                                                  u32 B, u64 c2_stride) {
-    const u32 pos = blockIdx.y;
-    const u32 own = pos < Ql ? pos / alpha : ~0u;
+// key inner product over the items of a plan (key_switch_inner_prod_c2_and_evk,
+// src/eval_key_switch.cu:14-69), grid row = table row r of the QP chain: item i
+// has r as data limb pos = r (r < its Ql) or special limb pos = Ql + (r - size_Q);
+// cx[pos] = sum_j t[j][pos] * key_j.b[r], cx[QlP + pos] likewise with key_j.a.
+// Consecutive items sharing a key read it once per coefficient (the first item of
+// a run of equal keys has the most digits). Digit j's own limbs (pos / alpha == j)
+// are read from c2 (NTT form). MB = the digit capacity of the key cache.
+template <u32 MB, class IpItem>
+__global__ __launch_bounds__(kThreads) void k_inner_product_items(const IpItem *items, u32 nitems,
+                                                                  const DevModulus *__restrict__ mod, u32 N,
+                                                                  u32 size_Q, u32 QP, u32 alpha) {
+#pragma clang fp contract(off)
+    const u32 r = blockIdx.y;
     const u32 k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= N) return;
-    const u32 row = pos < Ql ? pos : size_Q + (pos - Ql);
     const u64 QPN = (u64)QP * N;
-    const DevModulus m = mod[row];
-    constexpr u32 kMaxBeta = MB;
-    if (m.pad) {
-        // q < 2^50: exact products on the FP64 pipe (FpArith::mulmod, |r| < q), the
-        // partial sums reduced every 4 digits (|sum| < 4.5 q < 2^53), then canonical
-#pragma clang fp contract(off)
-        const FpArith fa{nullptr, (double)m.value, __longlong_as_double((long long)m.pad)};
-        double kb[kMaxBeta], ka[kMaxBeta];
-        const u64 *cur = nullptr;
-        for (u32 item = 0; item < B; ++item) {
-            const u64 *key = keys[item];
-            if (key != cur) {
-                cur = key;
-#pragma unroll
-                for (u32 j = 0; j < kMaxBeta; ++j)
-                    if (j < beta) {
-                        const u64 *p = key + (u64)j * 2 * QPN + (u64)row * N + k;
-                        kb[j] = FpArith::from_canon(p[0]);
-                        ka[j] = FpArith::from_canon(p[QPN]);
-                    }
-            }
-            const u64 *ti = t + (u64)item * beta * QlP * N + (u64)pos * N + k;
-            const u64 *ci = c2 + (u64)item * c2_stride + (u64)pos * N + k;
-            double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-            for (u32 j = 0; j < kMaxBeta; ++j)
-                if (j < beta) {
-                    if (j && (j & 3) == 0) {
-                        s0 = fa.red(s0);
-                        s1 = fa.red(s1);
-                    }
-                    const double x = FpArith::from_canon(j == own ? *ci : ti[(u64)j * QlP * N]);
-                    s0 += fa.mulmod(x, kb[j]);
-                    s1 += fa.mulmod(x, ka[j]);
-                }
-            u64 *co = cx + (u64)item * 2 * QlP * N + (u64)pos * N + k;
-            co[0] = fa.canon(s0);
-            co[(u64)QlP * N] = fa.canon(s1);
-        }
-        return;
-    }
-    u64 kb[kMaxBeta], ka[kMaxBeta];
+    const DevModulus m = mod[r];
+    const bool fp = m.pad != 0;
+    const FpArith fa{nullptr, (double)m.value, __longlong_as_double((long long)m.pad)};
+    u64 kb[MB], ka[MB];
     const u64 *cur = nullptr;
-    for (u32 item = 0; item < B; ++item) {
-        const u64 *key = keys[item];
-        if (key != cur) {
-            cur = key;
+    for (u32 i = 0; i < nitems; ++i) {
+        const IpItem it = items[i];
+        if (r < size_Q && r >= it.Ql) continue;
+        const u32 pos = r < size_Q ? r : it.Ql + (r - size_Q);
+        const u32 own = pos < it.Ql ? pos / alpha : ~0u;
+        if (it.key != cur) {
+            cur = it.key;
 #pragma unroll
-            for (u32 j = 0; j < kMaxBeta; ++j)
-                if (j < beta) {
-                    const u64 *p = key + (u64)j * 2 * QPN + (u64)row * N + k;
+            for (u32 j = 0; j < MB; ++j)
+                if (j < it.beta) {
+                    const u64 *p = it.key + (u64)j * 2 * QPN + (u64)r * N + k;
                     kb[j] = p[0];
                     ka[j] = p[QPN];
                 }
         }
-        const u64 *ti = t + (u64)item * beta * QlP * N + (u64)pos * N + k;
-        const u64 *ci = c2 + (u64)item * c2_stride + (u64)pos * N + k;
-        u128 a0 = 0, a1 = 0;
+        const u64 *ti = it.t + (u64)pos * N + k;
+        const u64 LPN = (u64)it.QlP * N;
+        const u64 xo = own != ~0u ? it.c2[(u64)pos * N + k] : 0;
+        u64 *co = it.cx + (u64)pos * N + k;
+        if (fp) {
+            // q < 2^50: exact products on the FP64 pipe (FpArith::mulmod, |r| < q), the
+            // partial sums reduced every 4 digits (|sum| < 4.5 q < 2^53), then canonical
+            double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-        for (u32 j = 0; j < kMaxBeta; ++j)
-            if (j < beta) {
-                const u64 x = j == own ? *ci : ti[(u64)j * QlP * N];
-                a0 += (u128)x * kb[j];
-                a1 += (u128)x * ka[j];
-            }
-        u64 *co = cx + (u64)item * 2 * QlP * N + (u64)pos * N + k;
-        co[0] = barrett128(a0, m.value, m.ratio_lo, m.ratio_hi);
-        co[(u64)QlP * N] = barrett128(a1, m.value, m.ratio_lo, m.ratio_hi);
+            for (u32 j = 0; j < MB; ++j)
+                if (j < it.beta) {
+                    if (j && (j & 3) == 0) {
+                        s0 = fa.red(s0);
+                        s1 = fa.red(s1);
+                    }
+                    const double x = FpArith::from_canon(j == own ? xo : ti[(u64)j * LPN]);
+                    s0 += fa.mulmod(x, FpArith::from_canon(kb[j]));
+                    s1 += fa.mulmod(x, FpArith::from_canon(ka[j]));
+                }
+            co[0] = fa.canon(s0);
+            co[LPN] = fa.canon(s1);
+        } else {
+            u128 a0 = 0, a1 = 0;
+#pragma unroll
+            for (u32 j = 0; j < MB; ++j)
+                if (j < it.beta) {
+                    const u64 x = j == own ? xo : ti[(u64)j * LPN];
+                    a0 += (u128)x * kb[j];
+                    a1 += (u128)x * ka[j];
+                }
+            co[0] = barrett128(a0, m.value, m.ratio_lo, m.ratio_hi);
+            co[LPN] = barrett128(a1, m.value, m.ratio_lo, m.ratio_hi);
+        }
     }
 }
 
-// dst[z][0 .. words) = src_z[0 .. words) for item z = blockIdx.y
-__global__ __launch_bounds__(kThreads) void k_gather_items(const u64 *const *src, u64 *__restrict__ dst, u64 words,
-                                                           u64 dst_stride) {
-    const u64 *a = src[blockIdx.y];
+// BFV staging: dst[z][0 .. words) = (which ? items[z].b : items[z].a)[0 .. words)
+template <class PreItem>
+__global__ __launch_bounds__(kThreads) void k_gather_items(const PreItem *items, int which, u64 *__restrict__ dst,
+                                                           u64 words, u64 dst_stride) {
+    const PreItem it = items[blockIdx.y];
+    const u64 *a = which ? it.b : it.a;
     u64 *d = dst + blockIdx.y * dst_stride;
     for (u64 i = (u64)blockIdx.x * kThreads + threadIdx.x; i < words; i += (u64)gridDim.x * kThreads) d[i] = a[i];
 }
 
-// dst_z[0 .. words) = src[z][0 .. words)
-__global__ __launch_bounds__(kThreads) void k_scatter_items(u64 *const *dst, const u64 *__restrict__ src, u64 words,
-                                                            u64 src_stride) {
-    u64 *d = dst[blockIdx.y];
+// items[z].out[0 .. words) = src[z][0 .. words)
+template <class PreItem>
+__global__ __launch_bounds__(kThreads) void k_scatter_items(const PreItem *items, const u64 *__restrict__ src,
+                                                            u64 words, u64 src_stride) {
+    u64 *d = items[blockIdx.y].out;
     const u64 *a = src + blockIdx.y * src_stride;
     for (u64 i = (u64)blockIdx.x * kThreads + threadIdx.x; i < words; i += (u64)gridDim.x * kThreads) d[i] = a[i];
 }
@@ -159,6 +146,26 @@ __global__ __launch_bounds__(kThreads) void k_scatter_items(u64 *const *dst, con
 namespace {
 dim3 items_grid(u64 words, u32 items) {
     return dim3((u32)std::min<u64>((words + kThreads - 1) / kThreads, 1024), items);
+}
+
+// scratch words of one item: c2 (CKKS), t_cks, t_mod_up, cx, delta (+ base0 for rotations)
+size_t ks_item_words(u64 N, u32 Ql, u32 QlP, u32 beta, bool own_c2, bool rotate) {
+    return (size_t)N * ((u64)Ql * ((own_c2 ? 1 : 0) + 1 + (rotate ? 1 : 0)) + (u64)beta * QlP + 2ull * QlP +
+                        2ull * Ql);
+}
+
+template <class T>
+size_t append(std::vector<uint8_t> &buf, const std::vector<T> &v) {
+    const size_t off = (buf.size() + 255) & ~size_t{255};
+    buf.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+}
+
+// stable order by table row: entries of one prime run next to each other, so
+// the prime's twiddles stay in L2
+void sort_by_row(std::vector<NttJob> &v) {
+    std::stable_sort(v.begin(), v.end(), [](const NttJob &x, const NttJob &y) { return x.row < y.row; });
 }
 }  // namespace
 
@@ -169,116 +176,168 @@ BatchPlan::~BatchPlan() {
     if (arena_ || tables_) {
         (void)hipDeviceSynchronize();
         (void)hipFree(arena_);
-        (void)hipFree(const_cast<void **>(tables_));
+        (void)hipFree(tables_);
     }
 }
-
-void BatchPlan::build_groups(const std::vector<u32> &chains) {
-    std::map<u32, std::vector<u32>> by;
-    for (u32 i = 0; i < chains.size(); ++i) by[chains[i]].push_back(i);
-    for (auto &kv : by) {
-        Group g;
-        g.chain = kv.first;
-        const LevelConsts &L = ctx_.level(kv.first);
-        g.Ql = L.size_Ql;
-        g.QlP = L.size_QlP;
-        g.beta = L.beta;
-        g.items = kv.second;
-        g.B = (u32)g.items.size();
-        groups_.push_back(g);
-    }
-}
-
-namespace {
-// scratch words of one item: c2, t_cks, t_mod_up, cx, delta (+ base0 for rotations)
-size_t item_words(u64 N, u32 Ql, u32 QlP, u32 beta, bool rotate) {
-    return (u64)N * ((u64)Ql * (2 + (rotate ? 1 : 0)) + (u64)beta * QlP + 2ull * QlP + 2ull * Ql);
-}
-}  // namespace
 
 size_t BatchPlan::item_scratch_bytes(const Context &ctx, u32 chain, Kind k) {
     const LevelConsts &L = ctx.level(chain);
-    return 8 * item_words(ctx.N(), L.size_Ql, L.size_QlP, L.beta, k == kRotate);
+    if (k == kRescale) return 8 * 2 * (size_t)ctx.N();
+    return 8 * ks_item_words(ctx.N(), L.size_Ql, L.size_QlP, L.beta, k != kBfvMulRelin, k == kRotate);
 }
 
-size_t BatchPlan::group_scratch_words(const Group &g) const {
-    size_t w = (size_t)g.B * item_words(ctx_.N(), g.Ql, g.QlP, g.beta, kind_ == kRotate);
-    if (kind_ == kBfvMulRelin)   // staged operands, product, multiply work
-        w += (size_t)g.B * (7ull * g.Ql * ctx_.N() + bfv_batch_work_words(ctx_));
-    return w;
-}
-
-void BatchPlan::set_tables(Group &g, const std::vector<const void *> &a, const std::vector<const void *> &b,
-                           const std::vector<const void *> &out, const std::vector<const void *> &keys,
-                           const std::vector<const void *> &perms) {
-    g.table0 = host_tables_.size();
-    for (const auto *v : {&a, &b, &out, &keys, &perms})
-        for (u32 i = 0; i < g.B; ++i) host_tables_.push_back(v->empty() ? nullptr : (*v)[i]);
-    for (u32 i = 0; i < g.B; ++i) host_tables_.push_back(nullptr);   // key-switch outputs, set by rebind
+size_t BatchPlan::item_words(const Item &it) const {
+    if (kind_ == kRescale) return 2 * (size_t)ctx_.N();
+    return ks_item_words(ctx_.N(), it.Ql, it.QlP, it.beta, kind_ != kBfvMulRelin, kind_ == kRotate);
 }
 
 void BatchPlan::layout() {
-    size_t words = 0;
-    for (const auto &g : groups_) words = std::max(words, group_scratch_words(g));
-    scratch_bytes_ = words * 8;
+    const u64 N = ctx_.N();
+    bfv_words_ = 0;
+    if (kind_ == kBfvMulRelin) {
+        const u64 LN = (u64)items_[0].Ql * N;
+        bfv_words_ = items_.size() * (7 * LN + bfv_batch_work_words(ctx_));
+    }
+    size_t w = bfv_words_;
+    for (auto &it : items_) {
+        it.off = w;
+        w += item_words(it);
+    }
+    scratch_bytes_ = w * 8;
     if (!transient_) {
         PHX_HIP_CHECK(hipMalloc(&arena_, std::max<size_t>(scratch_bytes_, 256)));
-        void *t = nullptr;
-        PHX_HIP_CHECK(hipMalloc(&t, std::max<size_t>(host_tables_.size() * sizeof(void *), 256)));
-        tables_ = static_cast<const void **>(t);
-        rebind(arena_);
+        bind(arena_);
+        PHX_HIP_CHECK(hipMalloc(&tables_, std::max<size_t>(host_tables_.size(), 256)));
+        upload_tables(nullptr, false);
     }
 }
 
-void BatchPlan::rebind(u64 *arena) {
-    bind(arena, tables_);
+void BatchPlan::bind(u64 *arena) {
+    const Context &ctx = ctx_;
+    const u64 N = ctx.N();
+    const u32 size_Q = ctx.size_Q(), P = ctx.size_P(), alpha = ctx.size_P();
+    pre_.clear();
+    ip_.clear();
+    mu_inv_.clear();
+    mu_fwd_.clear();
+    md_inv_.clear();
+    md_fwd_.clear();
+    mu_bc_.clear();
+    md_bc_.clear();
+    mu_desc_host_.clear();
+    md_desc_host_.clear();
+    max_Ql_ = max_beta_ = 0;
     if (kind_ == kBfvMulRelin) {
-        // outputs of the key switch: polynomials 0 and 1 of each item's product
-        for (const auto &g : groups_) {
-            const u64 LN = (u64)g.Ql * ctx_.N();
-            for (u32 i = 0; i < g.B; ++i) host_tables_[g.table0 + 5 * g.B + i] = g.c3 + (u64)i * 3 * LN;
+        const u64 B = items_.size(), LN = (u64)items_[0].Ql * N;
+        bfv_.in1 = arena;
+        bfv_.in2 = arena + B * 2 * LN;
+        bfv_.c3 = arena + B * 4 * LN;
+        bfv_.work = arena + B * 7 * LN;
+    }
+    std::vector<u32> seen_levels;
+    for (size_t i = 0; i < items_.size(); ++i) {
+        const Item &it = items_[i];
+        const LevelConsts &L = ctx.level(it.chain);
+        const u32 Ql = it.Ql, QlP = it.QlP, beta = it.beta;
+        const u64 LN = (u64)Ql * N, LPN = (u64)QlP * N;
+        max_Ql_ = std::max(max_Ql_, Ql);
+        max_beta_ = std::max(max_beta_, beta);
+        u64 *p = arena + it.off;
+        if (kind_ == kRescale) {
+            // INTT of the last limb of both polynomials, then the forward NTT of
+            // [c_last]_{q_j} with the floor-division tail
+            // (DRNSTool::divide_and_round_q_last_ntt, src/rns.cu:1128-1184)
+            u64 *last = p;
+            u64 *out = const_cast<u64 *>(it.b);   // the output ciphertext's buffer
+            for (u32 h = 0; h < 2; ++h) {
+                mu_inv_.push_back(NttJob{last + h * N, it.a + (h * Ql + Ql - 1) * N, nullptr, nullptr, nullptr,
+                                         Ql - 1, 0});
+                for (u32 l = 0; l + 1 < Ql; ++l) {
+                    u64 *d = out + ((u64)h * (Ql - 1) + l) * N;
+                    md_fwd_.push_back(NttJob{d, it.a + ((u64)h * Ql + l) * N, d, last + h * N,
+                                             L.rescale_qinv + 2 * l, l, 0});
+                }
+            }
+            continue;
+        }
+        u64 *c2;
+        if (kind_ == kBfvMulRelin) {
+            c2 = bfv_.c3 + i * 3 * LN + 2 * LN;
+        } else {
+            c2 = p;
+            p += LN;
+        }
+        u64 *t_cks = p;
+        p += LN;
+        u64 *t_mod_up = p;
+        p += (u64)beta * LPN;
+        u64 *cx = p;
+        p += 2 * LPN;
+        u64 *delta = p;
+        p += 2 * LN;
+        u64 *base0 = kind_ == kRotate ? p : nullptr;
+        u64 *dst = kind_ == kBfvMulRelin ? bfv_.c3 + i * 3 * LN : it.out;
+        pre_.push_back(PreItem{it.a, it.b, it.out, c2, base0, it.perm, Ql, 0});
+        // modup: INTT(c2) * partQlHatInv, FastBConv of every digit, NTT of the complements
+        for (u32 l = 0; l < Ql; ++l)
+            mu_inv_.push_back(NttJob{t_cks + l * N, c2 + l * N, nullptr, nullptr, L.modup_scale + 4 * l, l, 0});
+        for (u32 j = 0; j < beta; ++j) {
+            mu_bc_.push_back(BconvRow{L.modup_bconv + j, t_cks, t_mod_up + (u64)j * LPN});
+            const u32 lo = alpha * j, hi = std::min(alpha * (j + 1), Ql);
+            for (u32 pos = 0; pos < QlP; ++pos) {
+                if (pos >= lo && pos < hi) continue;
+                const u32 row = pos < Ql ? pos : size_Q + (pos - Ql);
+                mu_fwd_.push_back(NttJob{t_mod_up + (u64)j * LPN + (u64)pos * N, nullptr, nullptr, nullptr, nullptr,
+                                         row, 0});
+            }
+        }
+        if (std::find(seen_levels.begin(), seen_levels.end(), it.chain) == seen_levels.end()) {
+            seen_levels.push_back(it.chain);
+            mu_desc_host_.insert(mu_desc_host_.end(), L.h_modup_bconv.begin(), L.h_modup_bconv.end());
+            md_desc_host_.push_back(L.h_moddown_bconv);
+        }
+        ip_.push_back(IpItem{t_mod_up, c2, cx, it.key, Ql, QlP, beta, it.chain});
+        // moddown of both halves: INTT of the P limbs * PHatInv, FastBConv P -> Ql,
+        // NTT of delta with the tail dst = base + (cx - NTT(delta)) * P^-1
+        for (u32 h = 0; h < 2; ++h) {
+            for (u32 pj = 0; pj < P; ++pj)
+                md_inv_.push_back(NttJob{cx + (u64)h * LPN + (u64)(Ql + pj) * N, nullptr, nullptr, nullptr,
+                                         L.moddown_scale + 4 * pj, size_Q + pj, 0});
+            md_bc_.push_back(BconvRow{L.moddown_bconv, cx + (u64)h * LPN, delta + (u64)h * LN});
+            for (u32 l = 0; l < Ql; ++l) {
+                const u64 *base = kind_ == kRotate && h == 0 ? base0 + (u64)l * N : nullptr;
+                md_fwd_.push_back(NttJob{delta + (u64)h * LN + (u64)l * N, cx + (u64)h * LPN + (u64)l * N,
+                                         dst + (u64)h * LN + (u64)l * N, base, L.pinv + 2 * l, l,
+                                         kind_ == kRotate ? 0u : 1u});
+            }
         }
     }
-    PHX_HIP_CHECK(hipMemcpy(const_cast<void **>(tables_), host_tables_.data(), host_tables_.size() * sizeof(void *),
-                            hipMemcpyHostToDevice));
+    // items sharing a key next to each other, the most digits first (IpItem.pad holds the chain)
+    std::stable_sort(ip_.begin(), ip_.end(), [](const IpItem &x, const IpItem &y) {
+        return x.key != y.key ? x.key < y.key : x.pad < y.pad;
+    });
+    sort_by_row(mu_inv_);
+    sort_by_row(mu_fwd_);
+    sort_by_row(md_inv_);
+    sort_by_row(md_fwd_);
+    host_tables_.clear();
+    off_pre_ = append(host_tables_, pre_);
+    off_ip_ = append(host_tables_, ip_);
+    off_mu_inv_ = append(host_tables_, mu_inv_);
+    off_mu_fwd_ = append(host_tables_, mu_fwd_);
+    off_md_inv_ = append(host_tables_, md_inv_);
+    off_md_fwd_ = append(host_tables_, md_fwd_);
+    off_mu_bc_ = append(host_tables_, mu_bc_);
+    off_md_bc_ = append(host_tables_, md_bc_);
+    off_bfv_sq_ = append(host_tables_, bfv_square_);
 }
 
-void BatchPlan::bind(u64 *arena, const void **tables) {
-    const u64 N = ctx_.N();
-    for (auto &g : groups_) {
-        const u64 LN = (u64)g.Ql * N, LPN = (u64)g.QlP * N;
-        u64 *p = arena;
-        auto take = [&](u64 words) {
-            u64 *r = p;
-            p += words;
-            return r;
-        };
-        g.c2_stride = LN;
-        if (kind_ == kBfvMulRelin) {
-            g.in1 = take((u64)g.B * 2 * LN);
-            g.in2 = take((u64)g.B * 2 * LN);
-            g.c3 = take((u64)g.B * 3 * LN);
-            g.work = take((u64)g.B * bfv_batch_work_words(ctx_));
-        }
-        g.c2 = take(g.B * LN);
-        if (kind_ == kBfvMulRelin) {
-            // the key switch reads c2 where the product holds it (NTT form)
-            g.c2 = g.c3 + 2 * LN;
-            g.c2_stride = 3 * LN;
-        }
-        g.t_cks = take(g.B * LN);
-        g.t_mod_up = take((u64)g.B * g.beta * LPN);
-        g.cx = take((u64)g.B * 2 * LPN);
-        g.delta = take((u64)g.B * 2 * LN);
-        g.base0 = kind_ == kRotate ? take(g.B * LN) : nullptr;
-        const void **t = tables + g.table0;
-        g.a = reinterpret_cast<const u64 **>(t);
-        g.b = reinterpret_cast<const u64 **>(t + g.B);
-        g.out = reinterpret_cast<u64 **>(const_cast<void **>(t + 2 * g.B));
-        g.keys = reinterpret_cast<const u64 **>(t + 3 * g.B);
-        g.perms = reinterpret_cast<const u32 **>(t + 4 * g.B);
-        g.ks_out = reinterpret_cast<u64 **>(const_cast<void **>(t + 5 * g.B));
-    }
+void BatchPlan::upload_tables(hipStream_t s, bool async) {
+    if (async)
+        PHX_HIP_CHECK(hipMemcpyAsync(tables_bound_, host_tables_.data(), host_tables_.size(), hipMemcpyHostToDevice, s));
+    else
+        PHX_HIP_CHECK(hipMemcpy(tables_, host_tables_.data(), host_tables_.size(), hipMemcpyHostToDevice));
+    if (!async) tables_bound_ = tables_;
 }
 
 BatchPlan *BatchPlan::mul_relin(const Context &ctx, const std::vector<Ciphertext *> &ct1,
@@ -287,10 +346,10 @@ BatchPlan *BatchPlan::mul_relin(const Context &ctx, const std::vector<Ciphertext
     require(bfv || ctx.params().scheme == kSchemeCKKS, "batched multiply_and_relin: unsupported scheme");
     require(!(bfv && transient), "BFV batches are persistent plans");
     require(ct1.size() == ct2.size() && !ct1.empty(), "batch operands must be non-empty and of equal count");
+    require(ct1.size() <= kMaxGridY, "too many ciphertexts in one batch");
     require(rlk.data != nullptr, "relinearisation key is empty");
     if (bfv) bfv_batch_work_words(ctx);   // rejects a context without its multiply tool
     std::unique_ptr<BatchPlan> p(new BatchPlan(ctx, bfv ? kBfvMulRelin : kMulRelin, transient));
-    std::vector<u32> chains;
     for (size_t i = 0; i < ct1.size(); ++i) {
         require(ct1[i] && ct2[i] && ct1[i]->data && ct2[i]->data, "batch ciphertext is empty");
         require(ct1[i]->chain_index == ct2[i]->chain_index, "encrypted1 and encrypted2 parameter mismatch");
@@ -307,23 +366,22 @@ BatchPlan *BatchPlan::mul_relin(const Context &ctx, const std::vector<Ciphertext
             // an output may only alias its own second operand (squaring)
             if (j != i) require(ct2[j] != ct1[i], "a batch output is another item's operand");
         }
-        chains.push_back(ct1[i]->chain_index);
+        const LevelConsts &L = ctx.level(ct1[i]->chain_index);
+        require(L.beta <= 32, "batched key switching supports at most 32 digits");
+        Item it;
+        it.chain = ct1[i]->chain_index;
+        it.Ql = L.size_Ql;
+        it.QlP = L.size_QlP;
+        it.beta = L.beta;
+        it.a = ct1[i]->data;
+        it.b = ct2[i]->data;
+        it.out = ct1[i]->data;
+        it.key = rlk.data;
+        p->items_.push_back(it);
         p->cts_.push_back(ct1[i]);
         p->scale_mul_.push_back(bfv ? 1.0 : ct2[i]->scale);
-    }
-    p->items_ = ct1.size();
-    p->build_groups(chains);
-    for (auto &g : p->groups_) {
-        std::vector<const void *> a, b, out, keys, square;
-        for (u32 i : g.items) {
-            a.push_back(ct1[i]->data);
-            b.push_back(ct2[i]->data);
-            out.push_back(ct1[i]->data);
-            keys.push_back(rlk.data);
-            // BFV: squaring flag (the same ciphertext as both operands)
-            square.push_back(ct1[i]->data == ct2[i]->data ? reinterpret_cast<const void *>(uintptr_t{1}) : nullptr);
-        }
-        p->set_tables(g, a, b, out, keys, bfv ? square : std::vector<const void *>{});
+        // BFV: squaring flag (the same ciphertext as both operands)
+        if (bfv) p->bfv_square_.push_back(ct1[i]->data == ct2[i]->data ? 1 : 0);
     }
     p->layout();
     return p.release();
@@ -340,7 +398,7 @@ BatchPlan *BatchPlan::join(const std::vector<BatchPlan *> &parts) {
     size_t bytes = 0;
     for (auto *q : parts) {
         bytes = std::max(bytes, q->scratch_bytes_);
-        p->items_ += q->items_;
+        p->joined_items_ += q->size();
     }
     // the parts run one after another on one stream: one arena of the largest size
     PHX_HIP_CHECK(hipDeviceSynchronize());
@@ -349,7 +407,8 @@ BatchPlan *BatchPlan::join(const std::vector<BatchPlan *> &parts) {
     for (auto *q : parts) {
         PHX_HIP_CHECK(hipFree(q->arena_));
         q->arena_ = nullptr;
-        q->rebind(p->arena_);
+        q->bind(p->arena_);
+        q->upload_tables(nullptr, false);
         if (q->exec_) (void)hipGraphExecDestroy(q->exec_);
         if (q->graph_) (void)hipGraphDestroy(q->graph_);
         q->exec_ = nullptr;
@@ -362,198 +421,218 @@ BatchPlan *BatchPlan::join(const std::vector<BatchPlan *> &parts) {
 BatchPlan *BatchPlan::rotate(const Context &ctx, const std::vector<Ciphertext *> &cts, const std::vector<int> &steps,
                              const GaloisKeys &gk, bool transient) {
     require(cts.size() == steps.size() && !cts.empty(), "batch operands must be non-empty and of equal count");
+    require(cts.size() <= kMaxGridY, "too many ciphertexts in one batch");
+    require(ctx.params().scheme == kSchemeCKKS, "batched rotation runs CKKS ciphertexts");
     std::unique_ptr<BatchPlan> p(new BatchPlan(ctx, kRotate, transient));
-    std::vector<u32> chains;
-    std::vector<const u64 *> item_key(cts.size());
-    std::vector<const u32 *> item_perm(cts.size());
     for (size_t i = 0; i < cts.size(); ++i) {
         require(cts[i] && cts[i]->data, "batch ciphertext is empty");
         require(cts[i]->size == 2 && cts[i]->is_ntt_form, "encrypted size must be 2 (NTT form)");
         for (size_t j = 0; j < i; ++j) require(cts[j] != cts[i], "a ciphertext appears twice in a batch");
         const u32 elt = galois_elt_from_step(steps[i], ctx.N());
-        auto it = std::find(gk.elts.begin(), gk.elts.end(), elt);
-        require(it != gk.elts.end(), "Galois elt not present");
-        const KSwitchKey &key = gk.keys[it - gk.elts.begin()];
-        require(key.dnum >= ctx.level(cts[i]->chain_index).beta, "key-switching key has too few digits");
-        item_key[i] = key.data;
-        item_perm[i] = ctx.galois_perm(elt);
-        chains.push_back(cts[i]->chain_index);
+        auto itk = std::find(gk.elts.begin(), gk.elts.end(), elt);
+        require(itk != gk.elts.end(), "Galois elt not present");
+        const KSwitchKey &key = gk.keys[itk - gk.elts.begin()];
+        const LevelConsts &L = ctx.level(cts[i]->chain_index);
+        require(key.dnum >= L.beta, "key-switching key has too few digits");
+        require(L.beta <= 32, "batched key switching supports at most 32 digits");
+        Item it;
+        it.chain = cts[i]->chain_index;
+        it.Ql = L.size_Ql;
+        it.QlP = L.size_QlP;
+        it.beta = L.beta;
+        it.a = cts[i]->data;
+        it.out = cts[i]->data;
+        it.key = key.data;
+        it.perm = ctx.galois_perm(elt);
+        p->items_.push_back(it);
         p->cts_.push_back(cts[i]);
-    }
-    p->items_ = cts.size();
-    p->build_groups(chains);
-    for (auto &g : p->groups_) {
-        // items sharing a key are placed next to each other so the inner product reuses it
-        std::stable_sort(g.items.begin(), g.items.end(),
-                         [&](u32 x, u32 y) { return item_key[x] < item_key[y]; });
-        std::vector<const void *> a, out, keys, perms;
-        for (u32 i : g.items) {
-            a.push_back(cts[i]->data);
-            out.push_back(cts[i]->data);
-            keys.push_back(item_key[i]);
-            perms.push_back(item_perm[i]);
-        }
-        p->set_tables(g, a, {}, out, keys, perms);
     }
     p->layout();
     return p.release();
 }
 
-// The same pipeline as keyswitch() in evaluator.hip, each stage over the group.
-void BatchPlan::run_group(const Group &g, hipStream_t s) {
-    const Context &ctx = ctx_;
-    const u64 N = ctx.N();
-    const u32 Ql = g.Ql, B = g.B;
-    const u64 LN = (u64)Ql * N;
-    if (kind_ == kBfvMulRelin) {
-        // BFV (evaluate.cu: bfv_multiply_behz / bfv_multiply_hps, then relinearize_inplace):
-        // product of every item, its NTT, the key switch of c2 added to (c0, c1) in the
-        // NTT domain, and back: INTT(NTT(c) + ks) = c + INTT(ks) exactly
-        hipLaunchKernelGGL(k_gather_items, items_grid(2 * LN, B), dim3(kThreads), 0, s, g.a, g.in1, 2 * LN, 2 * LN);
-        PHX_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(k_gather_items, items_grid(2 * LN, B), dim3(kThreads), 0, s, g.b, g.in2, 2 * LN, 2 * LN);
-        PHX_HIP_CHECK(hipGetLastError());
-        // the perms table of a BFV plan holds each item's squaring flag
-        bfv_multiply_batch(ctx, g.in1, g.in2, B, reinterpret_cast<const u64 *>(g.perms), g.work, g.c3, s);
-        ntt_forward(ctx.tables(), limb_map(g.c3, 0, Ql, LN), Ql, 3 * B, s);
-        keyswitch_group(g, s);
-        ntt_inverse(ctx.tables(), limb_map(g.c3, 0, Ql, 3 * LN), Ql, B, s);
-        ntt_inverse(ctx.tables(), limb_map(g.c3 + LN, 0, Ql, 3 * LN), Ql, B, s);
-        hipLaunchKernelGGL(k_scatter_items, items_grid(2 * LN, B), dim3(kThreads), 0, s, g.out, (const u64 *)g.c3,
-                           2 * LN, 3 * LN);
-        PHX_HIP_CHECK(hipGetLastError());
-        return;
+BatchPlan *BatchPlan::rescale(const Context &ctx, const std::vector<const Ciphertext *> &ins,
+                              const std::vector<Ciphertext *> &outs, bool transient) {
+    require(ins.size() == outs.size() && !ins.empty(), "batch operands must be non-empty and of equal count");
+    require(ins.size() <= kMaxGridY, "too many ciphertexts in one batch");
+    require(ctx.params().scheme == kSchemeCKKS, "rescale_to_next: unsupported scheme");
+    std::unique_ptr<BatchPlan> p(new BatchPlan(ctx, kRescale, transient));
+    for (size_t i = 0; i < ins.size(); ++i) {
+        const Ciphertext *in = ins[i];
+        Ciphertext *out = outs[i];
+        require(in && out && in->data && out->data && in != out, "rescale needs distinct input and output");
+        require(in->size == 2 && in->is_ntt_form, "encrypted size must be 2 (NTT form)");
+        require(in->chain_index >= 1 && in->chain_index < ctx.size_Q(), "end of modulus switching chain reached");
+        require(out->chain_index == in->chain_index + 1 && out->size == 2 &&
+                    out->coeff_modulus_size == in->coeff_modulus_size - 1,
+                "rescale output must be sized for the next level");
+        Item it;
+        it.chain = in->chain_index;
+        it.Ql = in->coeff_modulus_size;
+        it.a = in->data;
+        it.b = out->data;
+        p->items_.push_back(it);
+        p->ins_.push_back(in);
+        p->cts_.push_back(out);
     }
-    if (kind_ == kMulRelin) {
-        hipLaunchKernelGGL(k_tensor_batch, grid3(N, Ql, B), dim3(kThreads), 0, s, g.a, g.b, g.out, g.c2,
-                           ctx.dev_mod(), (u32)N, Ql);
-    } else {
-        hipLaunchKernelGGL(k_permute_batch, grid3(N, Ql, B), dim3(kThreads), 0, s, g.a, g.perms, g.base0, g.c2,
-                           (u32)N, Ql);
-    }
-    PHX_HIP_CHECK(hipGetLastError());
-    keyswitch_group(g, s);
+    p->layout();
+    return p.release();
 }
 
-// out[item] (+)= keyswitch(c2[item]) in NTT form: the same pipeline as keyswitch()
-// in evaluator.hip, each stage over the group
-void BatchPlan::keyswitch_group(const Group &g, hipStream_t s) {
+size_t BatchPlan::launches() const {
+    size_t n = 0;
+    for (const auto &q : parts_) n += q->launches();
+    if (kind_ == kJoin) return n;
+    auto ntt = [](size_t jobs) { return 2 * ((jobs + kMaxGridY - 1) / kMaxGridY); };
+    if (kind_ == kRescale) return ntt(mu_inv_.size()) + ntt(md_fwd_.size());
+    n += 1 + ntt(mu_inv_.size()) + 1 + ntt(mu_fwd_.size()) + 1 + ntt(md_inv_.size()) + 1 + ntt(md_fwd_.size());
+    return n;
+}
+
+// out[item] (+)= keyswitch(c2[item]) in NTT form for every item: the pipeline of
+// keyswitch() in evaluator.hip, each stage one launch over the work lists
+void BatchPlan::record_keyswitch(hipStream_t s) {
     const Context &ctx = ctx_;
-    const LevelConsts &L = ctx.level(g.chain);
     const u64 N = ctx.N();
-    const u32 Ql = g.Ql, QlP = g.QlP, beta = g.beta, B = g.B, P = ctx.size_P(), alpha = ctx.size_P();
-    const u64 LN = (u64)Ql * N, LPN = (u64)QlP * N;
-    // modup: INTT(c2) * partQlHatInv for every item
-    LimbMap mi = limb_map(g.t_cks, 0, Ql, LN);
-    mi.src = g.c2;
-    mi.src_poly_stride = g.c2_stride;
-    mi.scale = L.modup_scale;
-    ntt_inverse(ctx.tables(), mi, Ql, B, s);
-    // FastBConv of every digit of every item
+    const NttTables &T = ctx.tables();
+    auto tab = [&](size_t off) { return tables_bound_ + off; };
+    // modup
+    ntt_inverse_jobs(T, reinterpret_cast<const NttJob *>(tab(off_mu_inv_)), (u32)mu_inv_.size(), s);
+    require(mu_bc_.size() <= kMaxGridY && md_bc_.size() <= kMaxGridY, "too many base conversions in one batch");
     BconvArgs ba{};
+    ba.rows = reinterpret_cast<const BconvRow *>(tab(off_mu_bc_));
     ba.cbuf = ctx.const_buf();
-    ba.descs = L.modup_bconv;
-    ba.desc_stride = 1;
-    ba.jobs_per_batch = beta;
-    ba.in = g.t_cks;
-    ba.in_job_stride = 0;
-    ba.in_batch_stride = LN;
-    ba.out = g.t_mod_up;
-    ba.out_job_stride = LPN;
-    ba.out_batch_stride = (u64)beta * LPN;
-    // the inner product reads the digits' own limbs from c2 directly: no copy
-    const bool fused_ip = tune_value("ks_fused_ip") != 0;
-    ba.copy_src = nullptr;
-    ba.copy_job_stride = 0;
-    ba.copy_batch_stride = g.c2_stride;
     ba.mod = ctx.dev_mod();
     ba.N = (u32)N;
-    bconv_run(ctx, L.h_modup_bconv, ba, B * beta, s);
-    // NTT of the complement limbs of every (item, digit)
-    LimbMap mf = limb_map(g.t_mod_up, 0, QlP, LPN);
-    mf.split = Ql;
-    mf.special_row = ctx.size_Q();
-    mf.skip_alpha = alpha;
-    mf.skip_cap = Ql;
-    mf.skip_mod = beta;
-    mf.nlimbs = QlP;
-    const u32 last_sz = Ql - alpha * (beta - 1);
-    require(beta <= 32, "batched key switching supports at most 32 digits");
-    require(!fused_ip || beta <= 8, "the fused inner product supports at most 8 digits");
-    if (fused_ip) {
-        // pass 1 here; pass 2 fused with the inner product
-        ntt_forward_cols(ctx.tables(), mf, QlP - last_sz, B * beta, s);
-        RowsIpArgs ip{};
-        ip.t = g.t_mod_up;
-        ip.c2 = g.c2;
-        ip.c2_stride = g.c2_stride;
-        ip.keys = g.keys;
-        ip.cx = g.cx;
-        ip.Ql = Ql;
-        ip.QlP = QlP;
-        ip.size_Q = ctx.size_Q();
-        ip.QP = ctx.size_QP();
-        ip.beta = beta;
-        ip.alpha = alpha;
-        ntt_forward_rows_ip(ctx.tables(), ip, B, s);
-    } else {
-        ntt_forward(ctx.tables(), mf, QlP - last_sz, B * beta, s);
-        // inner product (a shared key is read once per coefficient for the group)
-        auto ip = beta <= 8 ? k_inner_product_batch<8> : beta <= 16 ? k_inner_product_batch<16>
-                                                                    : k_inner_product_batch<32>;
-        hipLaunchKernelGGL(ip, dim3((u32)(N / kThreads), QlP), dim3(kThreads), 0, s, g.t_mod_up, g.c2, g.keys, g.cx,
-                           ctx.dev_mod(), (u32)N, Ql, QlP, ctx.size_Q(), ctx.size_QP(), beta, alpha, B, g.c2_stride);
-        PHX_HIP_CHECK(hipGetLastError());
-    }
-    // moddown of both halves of every item
-    LimbMap md = limb_map(g.cx + LN, ctx.size_Q(), P, LPN);
-    md.scale = L.moddown_scale;
-    ntt_inverse(ctx.tables(), md, P, 2 * B, s);
+    bconv_run(ctx, mu_desc_host_, ba, (u32)mu_bc_.size(), s);
+    ntt_forward_jobs(T, reinterpret_cast<const NttJob *>(tab(off_mu_fwd_)), (u32)mu_fwd_.size(), s);
+    // key inner product
+    auto ip = max_beta_ <= 8 ? k_inner_product_items<8, IpItem>
+                             : max_beta_ <= 16 ? k_inner_product_items<16, IpItem> : k_inner_product_items<32, IpItem>;
+    hipLaunchKernelGGL(ip, dim3((u32)(N / kThreads), ctx.size_QP()), dim3(kThreads), 0, s,
+                       reinterpret_cast<const IpItem *>(tab(off_ip_)), (u32)ip_.size(), ctx.dev_mod(), (u32)N,
+                       ctx.size_Q(), ctx.size_QP(), ctx.size_P());
+    PHX_HIP_CHECK(hipGetLastError());
+    // moddown
+    ntt_inverse_jobs(T, reinterpret_cast<const NttJob *>(tab(off_md_inv_)), (u32)md_inv_.size(), s);
     BconvArgs bd{};
+    bd.rows = reinterpret_cast<const BconvRow *>(tab(off_md_bc_));
     bd.cbuf = ctx.const_buf();
-    bd.descs = L.moddown_bconv;
-    bd.desc_stride = 0;
-    bd.jobs_per_batch = 1;
-    bd.in = g.cx;
-    bd.in_batch_stride = LPN;
-    bd.out = g.delta;
-    bd.out_batch_stride = LN;
     bd.mod = ctx.dev_mod();
     bd.N = (u32)N;
-    bconv_run(ctx, {L.h_moddown_bconv}, bd, 2 * B, s);
-    StoreModDownBatch st{};
-    st.out = kind_ == kBfvMulRelin ? g.ks_out : g.out;
-    st.base0 = kind_ == kRotate ? g.base0 : nullptr;
-    st.cx = g.cx;
-    st.pinv = L.pinv;
-    st.cx_stride = LPN;
-    st.N = (u32)N;
-    st.Ql = Ql;
-    ntt_forward_store_batch(ctx.tables(), limb_map(g.delta, 0, Ql, LN), Ql, 2 * B, st, s);
+    bconv_run(ctx, md_desc_host_, bd, (u32)md_bc_.size(), s);
+    ntt_forward_jobs_moddown(T, reinterpret_cast<const NttJob *>(tab(off_md_fwd_)), (u32)md_fwd_.size(), s);
 }
 
 void BatchPlan::record(hipStream_t s) {
     for (const auto &q : parts_) q->record(s);
-    for (const auto &g : groups_) run_group(g, s);
+    if (kind_ == kJoin) return;
+    const Context &ctx = ctx_;
+    const u64 N = ctx.N();
+    const NttTables &T = ctx.tables();
+    const PreItem *pre = reinterpret_cast<const PreItem *>(tables_bound_ + off_pre_);
+    const u32 B = (u32)items_.size();
+    if (kind_ == kRescale) {
+        ntt_inverse_jobs(T, reinterpret_cast<const NttJob *>(tables_bound_ + off_mu_inv_), (u32)mu_inv_.size(), s);
+        ntt_forward_jobs_rescale(T, reinterpret_cast<const NttJob *>(tables_bound_ + off_md_fwd_),
+                                 (u32)md_fwd_.size(), s);
+        return;
+    }
+    if (kind_ == kBfvMulRelin) {
+        // BFV (evaluate.cu: bfv_multiply_behz / bfv_multiply_hps, then relinearize_inplace):
+        // product of every item, its NTT, the key switch of c2 added to (c0, c1) in the
+        // NTT domain, and back: INTT(NTT(c) + ks) = c + INTT(ks) exactly
+        const u32 Ql = items_[0].Ql;
+        const u64 LN = (u64)Ql * N;
+        hipLaunchKernelGGL(k_gather_items<PreItem>, items_grid(2 * LN, B), dim3(kThreads), 0, s, pre, 0, bfv_.in1,
+                           2 * LN, 2 * LN);
+        PHX_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_gather_items<PreItem>, items_grid(2 * LN, B), dim3(kThreads), 0, s, pre, 1, bfv_.in2,
+                           2 * LN, 2 * LN);
+        PHX_HIP_CHECK(hipGetLastError());
+        bfv_multiply_batch(ctx, bfv_.in1, bfv_.in2, B, reinterpret_cast<const u64 *>(tables_bound_ + off_bfv_sq_),
+                           bfv_.work, bfv_.c3, s);
+        ntt_forward(T, limb_map(bfv_.c3, 0, Ql, LN), Ql, 3 * B, s);
+        record_keyswitch(s);
+        ntt_inverse(T, limb_map(bfv_.c3, 0, Ql, 3 * LN), Ql, B, s);
+        ntt_inverse(T, limb_map(bfv_.c3 + LN, 0, Ql, 3 * LN), Ql, B, s);
+        hipLaunchKernelGGL(k_scatter_items<PreItem>, items_grid(2 * LN, B), dim3(kThreads), 0, s, pre,
+                           (const u64 *)bfv_.c3, 2 * LN, 3 * LN);
+        PHX_HIP_CHECK(hipGetLastError());
+        return;
+    }
+    if (kind_ == kMulRelin)
+        hipLaunchKernelGGL(k_tensor_items<PreItem>, grid3(N, max_Ql_, B), dim3(kThreads), 0, s, pre, ctx.dev_mod(),
+                           (u32)N);
+    else
+        hipLaunchKernelGGL(k_permute_items<PreItem>, grid3(N, max_Ql_, B), dim3(kThreads), 0, s, pre, (u32)N);
+    PHX_HIP_CHECK(hipGetLastError());
+    record_keyswitch(s);
 }
 
 void BatchPlan::after_launch() {
     for (const auto &q : parts_) q->after_launch();
     if (kind_ == kMulRelin)
         for (size_t i = 0; i < cts_.size(); ++i) cts_[i]->scale *= scale_mul_[i];
+    if (kind_ == kRescale)
+        for (size_t i = 0; i < cts_.size(); ++i) {
+            const Ciphertext &in = *ins_[i];
+            Ciphertext &out = *cts_[i];
+            out.scale = in.scale / (double)ctx_.moduli()[in.coeff_modulus_size - 1];
+            out.is_ntt_form = true;
+            out.correction_factor = in.correction_factor;
+            out.noise_scale_deg = in.noise_scale_deg;
+            out.is_asymmetric = in.is_asymmetric;
+            out.has_seed = false;
+        }
+}
+
+size_t BatchPlan::table_bytes() const {
+    // the sections bind() writes, each aligned to 256 bytes
+    size_t pre = 0, ip = 0, mu_inv = 0, mu_fwd = 0, md_inv = 0, md_fwd = 0, mu_bc = 0, md_bc = 0;
+    const size_t P = ctx_.size_P();
+    for (const Item &it : items_) {
+        if (kind_ == kRescale) {
+            mu_inv += 2;
+            md_fwd += 2 * (it.Ql - 1);
+            continue;
+        }
+        pre += 1;
+        ip += 1;
+        mu_inv += it.Ql;
+        mu_fwd += (size_t)it.beta * it.QlP - it.Ql;
+        md_inv += 2 * P;
+        md_fwd += 2 * it.Ql;
+        mu_bc += it.beta;
+        md_bc += 2;
+    }
+    const size_t sizes[9] = {pre * sizeof(PreItem), ip * sizeof(IpItem), mu_inv * sizeof(NttJob),
+                             mu_fwd * sizeof(NttJob), md_inv * sizeof(NttJob), md_fwd * sizeof(NttJob),
+                             mu_bc * sizeof(BconvRow), md_bc * sizeof(BconvRow), bfv_square_.size() * sizeof(u64)};
+    size_t off = 0;
+    for (size_t v : sizes) off = ((off + 255) & ~size_t{255}) + v;
+    return std::max<size_t>(off, 256);
+}
+
+void BatchPlan::run_with(hipStream_t s, u64 *arena, uint8_t *tables) {
+    require(transient_, "only transient plans run in caller memory");
+    bind(arena);
+    require(host_tables_.size() <= table_bytes(), "batch work lists larger than their buffer");
+    tables_bound_ = tables;
+    upload_tables(s, true);
+    record(s);
+    tables_bound_ = nullptr;
+    after_launch();
 }
 
 void BatchPlan::run(hipStream_t s) {
     if (transient_) {
         Scratch arena(std::max<size_t>(scratch_bytes_, 256), s);
-        Scratch t(host_tables_.size() * sizeof(void *), s);
-        PHX_HIP_CHECK(hipMemcpyAsync(t.get(), host_tables_.data(), host_tables_.size() * sizeof(void *),
-                                     hipMemcpyHostToDevice, s));
-        bind(arena.get(), reinterpret_cast<const void **>(t.get()));
-        for (const auto &g : groups_) run_group(g, s);
-    } else {
-        record(s);
+        Scratch t(table_bytes(), s);
+        run_with(s, arena.get(), reinterpret_cast<uint8_t *>(t.get()));
+        return;
     }
+    record(s);
     after_launch();
 }
 

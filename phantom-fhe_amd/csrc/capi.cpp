@@ -1118,6 +1118,14 @@ int phx_batch_scratch_bytes(const phx_batch *b, size_t *bytes) {
     });
 }
 
+int phx_batch_launches(const phx_batch *b, size_t *launches) {
+    return guard([&] {
+        nn(b, "batch");
+        nn(launches, "launches");
+        *launches = b->plan->launches();
+    });
+}
+
 int phx_batch_destroy(phx_batch *b) {
     return guard([&] {
         if (!b) return;

@@ -51,9 +51,19 @@ struct SecretKey {
     u64 *s2 = nullptr;   // s^2, [QP][N] NTT form (lazily computed)
 };
 
+// One grid row of a work-list FastBConv launch: the descriptor of the row's
+// conversion (its level's), the row's input limb set and output limb set.
+struct BconvRow {
+    const BconvDesc *desc;
+    const u64 *in;
+    u64 *out;
+};
+
 // FastBConv launch arguments. Grid row y = batch * jobs_per_batch + job; job j
-// uses descs[j * desc_stride]; pointers advance by job and batch strides.
+// uses descs[j * desc_stride]; pointers advance by job and batch strides. With
+// rows set, grid row y is rows[y] instead (copy_src unused).
 struct BconvArgs {
+    const BconvRow *rows;
     const u64 *cbuf;
     const BconvDesc *descs;
     u32 desc_stride;

@@ -29,14 +29,35 @@ inline dim3 grid_for(u64 N, u32 y, u32 z = 1) { return dim3((u32)((N + kThreads 
 // reference: bconv_matmul_padded_unroll2_kernel src/rns_bconv.cu:455-485,
 //            bconv_matmul_unroll2_kernel 143-168, modup_copy_partQl 522-528
 // ---------------------------------------------------------------------------
+// the descriptor and limb sets of grid row blockIdx.y
+__device__ __forceinline__ void bconv_row(const BconvArgs &A, BconvDesc &d, const u64 *&in, u64 *&out, u32 &j,
+                                          u32 &bt) {
+    if (A.rows) {
+        const BconvRow r = A.rows[blockIdx.y];
+        d = *r.desc;
+        in = r.in;
+        out = r.out;
+        j = bt = 0;
+        return;
+    }
+    j = blockIdx.y % A.jobs_per_batch;
+    bt = blockIdx.y / A.jobs_per_batch;
+    d = A.descs[j * A.desc_stride];
+    in = A.in + bt * A.in_batch_stride + j * A.in_job_stride;
+    out = A.out + bt * A.out_batch_stride + j * A.out_job_stride;
+}
+
 __global__ __launch_bounds__(kThreads) void k_bconv(BconvArgs A) {
-    const u32 j = blockIdx.y % A.jobs_per_batch, bt = blockIdx.y / A.jobs_per_batch;
-    const BconvDesc d = A.descs[j * A.desc_stride];
+    BconvDesc d;
+    const u64 *inb;
+    u64 *outb;
+    u32 j, bt;
+    bconv_row(A, d, inb, outb, j, bt);
     const u32 N = A.N;
     const u32 k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= N) return;
-    const u64 *inj = A.in + bt * A.in_batch_stride + j * A.in_job_stride + (u64)d.in_first * N + k;
-    u64 *outj = A.out + bt * A.out_batch_stride + j * A.out_job_stride + k;
+    const u64 *inj = inb + (u64)d.in_first * N + k;
+    u64 *outj = outb + k;
     const u64 *mat = A.cbuf + d.mat_off;
     const u32 *pos = reinterpret_cast<const u32 *>(A.cbuf) + d.pos_off;
     const u32 *rows = reinterpret_cast<const u32 *>(A.cbuf) + d.row_off;
@@ -81,8 +102,11 @@ constexpr u32 kBconvThreads = 512;   // 8 waves share one staged copy of the ope
 
 __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
     extern __shared__ __align__(16) uint8_t smem[];
-    const u32 j = blockIdx.y % A.jobs_per_batch, bt = blockIdx.y / A.jobs_per_batch;
-    const BconvDesc d = A.descs[j * A.desc_stride];
+    BconvDesc d;
+    const u64 *inb;
+    u64 *outb;
+    u32 j, bt;
+    bconv_row(A, d, inb, outb, j, bt);
     const u64 *cbuf = A.cbuf;
     const DevModulus *mod = A.mod;
     const u32 N = A.N;
@@ -113,8 +137,8 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
     }
     __syncthreads();
 
-    const u64 *inj = A.in + bt * A.in_batch_stride + j * A.in_job_stride + (u64)d.in_first * N;
-    u64 *outj = A.out + bt * A.out_batch_stride + j * A.out_job_stride;
+    const u64 *inj = inb + (u64)d.in_first * N;
+    u64 *outj = outb;
     const u32 wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
     const u32 ntiles = N / kBconvTileCols;
     const u32 groups = rt / 3;

@@ -95,6 +95,63 @@ struct StoreRescale {
     }
 };
 
+// ---- policies of work-list launches: bound to their entry's operands ----
+// Moddown tail: out[i] = base + (cx[i] - NTT(delta)[i]) * P^-1 with aux0 = the cx
+// limb, aux1 = the output limb, aux2 = the base limb (null: the output's own value
+// when flags & 1, else zero), scale = (P^-1 mod q, Shoup)
+struct StoreModDownJob {
+    static constexpr bool kBind = true;
+    struct Bound {
+        const u64 *cx;
+        u64 *out;
+        const u64 *base;
+        u64 p, ps;
+        u32 add_out;
+        __device__ __forceinline__ void operator()(u64 *, u32 i, u64 v, u32, u32, const DevModulus &m) const {
+            const u64 q = m.value;
+            const u64 r = mul_shoup(sub_mod(cx[i], v, q), p, ps, q);
+            const u64 b = base ? base[i] : (add_out ? out[i] : 0);
+            out[i] = add_mod(b, r, q);
+        }
+    };
+    __device__ __forceinline__ Bound bind(const NttJob &J) const {
+        return Bound{J.aux0, J.aux1, J.aux2, J.scale[0], J.scale[1], J.flags & 1u};
+    }
+};
+// Rescale: the NTT input is [c_last]_q from aux2 (the coefficient-form last limb of
+// the polynomial); the output aux1[i] = (aux0[i] - NTT(.)[i]) * q_last^-1, scale =
+// (q_last^-1 mod q, Shoup)
+struct LoadRescaleJob {
+    static constexpr bool kBind = true;
+    struct Bound {
+        const u64 *last;
+        __device__ __forceinline__ u64 operator()(const u64 *, u32 i, u32, u32, const DevModulus &m) const {
+            return barrett64(last[i], m.value, m.ratio_hi);
+        }
+    };
+    __device__ __forceinline__ Bound bind(const NttJob &J) const { return Bound{J.aux2}; }
+};
+struct StoreRescaleJob {
+    static constexpr bool kBind = true;
+    struct Bound {
+        const u64 *in;
+        u64 *out;
+        u64 p, ps;
+        __device__ __forceinline__ void operator()(u64 *, u32 i, u64 v, u32, u32, const DevModulus &m) const {
+            const u64 q = m.value;
+            out[i] = mul_shoup(sub_mod(in[i], v, q), p, ps, q);
+        }
+    };
+    __device__ __forceinline__ Bound bind(const NttJob &J) const { return Bound{J.aux0, J.aux1, J.scale[0], J.scale[1]}; }
+};
+
+// work-list launches over njobs entries of a device table (two passes each)
+void ntt_forward_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
+void ntt_forward_jobs_moddown(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
+void ntt_forward_jobs_rescale(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
+// inverse: entries with aux0 read their first pass from aux0 (out of place)
+void ntt_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
+
 void ntt_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s);
 void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s);
 void ntt_forward_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const StoreModDown &st,
@@ -103,23 +160,6 @@ void ntt_forward_store_batch(const NttTables &T, const LimbMap &M, u32 nlimbs, u
                              const StoreModDownBatch &st, hipStream_t s);
 void ntt_forward_rescale(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const LoadRescale &ld,
                          const StoreRescale &st, hipStream_t s);
-
-// Pass 2 of the modup NTT fused with the key inner product, over a batch:
-//   cx[i][h][pos] = sum_j NTT(t[i][j][pos]) * key_i[j][h][row(pos)]  (mod q_row)
-// where t[i][j][pos] holds pass-1 output, except for the digit's own limbs
-// (pos / alpha == j, pos < Ql), which are read from c2[i][pos] (NTT form).
-struct RowsIpArgs {
-    const u64 *t;            // [item][beta][QlP][N], pass-1 output
-    const u64 *c2;           // [item] at stride c2_stride: [Ql][N]
-    u64 c2_stride;
-    const u64 *const *keys;  // [item] -> key [dnum][2][QP][N]
-    u64 *cx;                 // [item][2][QlP][N]
-    u32 Ql, QlP, size_Q, QP, beta, alpha;
-};
-
-// pass 1 of the forward NTT only (in place)
-void ntt_forward_cols(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s);
-void ntt_forward_rows_ip(const NttTables &T, const RowsIpArgs &P, u32 nitems, hipStream_t s);
 
 // contiguous limbs [0, L) of one polynomial with rows first_row + i
 inline LimbMap limb_map(u64 *data, u32 first_row, u32 L, u64 poly_stride = 0) {

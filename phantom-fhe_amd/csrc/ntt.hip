@@ -7,6 +7,8 @@
 #include "ntt.h"
 #include "tune.h"
 
+#include <type_traits>
+
 namespace phx {
 
 namespace {
@@ -399,52 +401,85 @@ __global__ __launch_bounds__(kNttThreads) void k_inv_cols(NttTables T, LimbMap M
 }
 
 // ---------------------------------------------------------------------------
-// Pass 2 of the modup NTT fused with the key inner product
-// (key_switch_inner_prod_c2_and_evk, src/eval_key_switch.cu:14-69). Workgroup
-// (tile, item, limb pos) runs, for every digit j, the row pass on the pass-1
-// output of t[item][j][pos] -- or, for the digit's own limbs, takes c2 (already
-// in NTT form) -- multiplies by key_j at pos and accumulates, then writes the
-// two outputs cx[item][0 / 1][pos]. The transformed digits never reach HBM.
+// work-list kernels: entry blockIdx.y of the table gives the limb's buffer and
+// row; policies that carry per-entry operands are bound to the entry once
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kNttThreads) void k_fwd_rows_ip(NttTables T, RowsIpArgs P) {
+namespace {
+template <class P, class = void>
+struct BindsJob : std::false_type {};
+template <class P>
+struct BindsJob<P, std::void_t<decltype(P::kBind)>> : std::true_type {};
+
+template <class P>
+__device__ __forceinline__ auto bind_job(const P &p, const NttJob &J) {
+    if constexpr (BindsJob<P>::value)
+        return p.bind(J);
+    else
+        return p;
+}
+}  // namespace
+
+template <int SR, class Load>
+__global__ __launch_bounds__(kNttThreads) void k_fwd_cols_jobs(NttTables T, const NttJob *jobs, Load ld) {
+    constexpr int R = 1 << SR;
+    constexpr int TC = kNttTile / R;
+    __shared__ u64 lds[R * (TC + 1)];
+    const NttJob J = jobs[blockIdx.y];
+    const DevModulus md = T.mod[J.row];
+    const auto lb = bind_job(ld, J);
+    const u32 c0 = blockIdx.x * TC;
+    if (md.pad)
+        fwd_cols_body<SR>(fp_arith(T.twd + (u64)J.row * T.N, md), lds, J.data, T.logN, c0, 0u, 0u, md, lb,
+                          threadIdx.x);
+    else
+        fwd_cols_body<SR>(int_arith(T.tw + (u64)J.row * T.N, md.value), lds, J.data, T.logN, c0, 0u, 0u, md, lb,
+                          threadIdx.x);
+}
+
+template <class Store>
+__global__ __launch_bounds__(kNttThreads) void k_fwd_rows_jobs(NttTables T, const NttJob *jobs, Store st) {
     __shared__ u64 lds[16 * 272];
-    const u32 pos = blockIdx.z, item = blockIdx.y, r0 = blockIdx.x * 16, t = threadIdx.x;
-    const u32 row = pos < P.Ql ? pos : P.size_Q + (pos - P.Ql);
-    const DevModulus md = T.mod[row];
-    const u64 N = T.N, QPN = (u64)P.QP * N;
-    const u64 *key = P.keys[item] + (u64)row * N + r0 * 256 + t;
-    const u64 *t_item = P.t + (u64)item * P.beta * P.QlP * N + (u64)pos * N;
-    u64 acc0[16], acc1[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) acc0[k] = acc1[k] = 0;
-    for (u32 j = 0; j < P.beta; ++j) {
-        u64 x[16];
-        if (pos < P.Ql && pos / P.alpha == j) {
-            const u64 *c2 = P.c2 + (u64)item * P.c2_stride + (u64)pos * N + r0 * 256 + t;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) x[k] = c2[kNttThreads * k];
-        } else {
-            const u64 *src = t_item + (u64)j * P.QlP * N;
-            if (md.pad)
-                fwd_rows_core(fp_arith(T.twd + (u64)row * N, md), lds, src, T.logN, r0, t, x);
-            else
-                fwd_rows_core(int_arith(T.tw + (u64)row * N, md.value), lds, src, T.logN, r0, t, x);
-            __syncthreads();
-        }
-        const u64 *kj = key + (u64)j * 2 * QPN;
-        // each product is reduced, so 8 digits of partial sums stay below 2^63
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            acc0[k] += mul_mod(x[k], kj[kNttThreads * k], md);
-            acc1[k] += mul_mod(x[k], kj[QPN + kNttThreads * k], md);
-        }
-    }
-    u64 *out = P.cx + (u64)item * 2 * P.QlP * N + (u64)pos * N + r0 * 256 + t;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        out[kNttThreads * k] = barrett64(acc0[k], md.value, md.ratio_hi);
-        out[(u64)P.QlP * N + kNttThreads * k] = barrett64(acc1[k], md.value, md.ratio_hi);
-    }
+    const NttJob J = jobs[blockIdx.y];
+    const DevModulus md = T.mod[J.row];
+    const auto sb = bind_job(st, J);
+    const u32 r0 = blockIdx.x * 16;
+    if (md.pad)
+        fwd_rows_body(fp_arith(T.twd + (u64)J.row * T.N, md), lds, J.data, T.logN, r0, 0u, 0u, md, sb, threadIdx.x);
+    else
+        fwd_rows_body(int_arith(T.tw + (u64)J.row * T.N, md.value), lds, J.data, T.logN, r0, 0u, 0u, md, sb,
+                      threadIdx.x);
+}
+
+__global__ __launch_bounds__(kNttThreads) void k_inv_rows_jobs(NttTables T, const NttJob *jobs) {
+    __shared__ u64 lds[16 * 272];
+    const NttJob J = jobs[blockIdx.y];
+    const DevModulus md = T.mod[J.row];
+    const u64 *src = J.aux0 ? J.aux0 : J.data;
+    const u32 r0 = blockIdx.x * 16;
+    if (md.pad)
+        inv_rows_body(fp_arith(T.itwd + (u64)J.row * T.N, md), lds, J.data, src, T.logN, r0, 0u, 0u, md,
+                      LoadPlain{}, threadIdx.x);
+    else
+        inv_rows_body(int_arith(T.itw + (u64)J.row * T.N, md.value), lds, J.data, src, T.logN, r0, 0u, 0u, md,
+                      LoadPlain{}, threadIdx.x);
+}
+
+template <int SR>
+__global__ __launch_bounds__(kNttThreads) void k_inv_cols_jobs(NttTables T, const NttJob *jobs) {
+    constexpr int R = 1 << SR;
+    constexpr int TC = kNttTile / R;
+    __shared__ u64 lds[R * (TC + 1)];
+    const NttJob J = jobs[blockIdx.y];
+    const DevModulus md = T.mod[J.row];
+    const u64 *c = J.scale ? J.scale : T.inv_consts + 4 * (u64)J.row;
+    const u64 s0 = c[0], s0s = c[1], s1 = c[2], s1s = c[3];
+    const u32 c0 = blockIdx.x * TC;
+    if (md.pad)
+        inv_cols_body<SR>(fp_arith(T.itwd + (u64)J.row * T.N, md), lds, J.data, T.logN, c0, 0u, 0u, md, StorePlain{},
+                          s0, s0s, s1, s1s, threadIdx.x);
+    else
+        inv_cols_body<SR>(int_arith(T.itw + (u64)J.row * T.N, md.value), lds, J.data, T.logN, c0, 0u, 0u, md,
+                          StorePlain{}, s0, s0s, s1, s1s, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -616,6 +651,51 @@ void ntt_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, h
     launch_forward(T, M, nlimbs, npolys, LoadPlain{}, StorePlain{}, s);
 }
 
+namespace {
+// grid rows per work-list launch (the limit of gridDim.y)
+constexpr u32 kMaxJobsPerLaunch = 65535;
+
+template <class Load, class Store>
+void launch_forward_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, const Load &ld, const Store &st,
+                         hipStream_t s) {
+    const u32 tiles = T.N / kNttTile;
+    for (u32 j0 = 0; j0 < njobs; j0 += kMaxJobsPerLaunch) {
+        const dim3 grid(tiles, std::min(kMaxJobsPerLaunch, njobs - j0));
+        dispatch_sr(T.logN, [&](auto sr) {
+            constexpr int SR = decltype(sr)::value;
+            hipLaunchKernelGGL((k_fwd_cols_jobs<SR, Load>), grid, dim3(kNttThreads), 0, s, T, jobs + j0, ld);
+        });
+        hipLaunchKernelGGL((k_fwd_rows_jobs<Store>), grid, dim3(kNttThreads), 0, s, T, jobs + j0, st);
+    }
+    PHX_HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+void ntt_forward_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s) {
+    launch_forward_jobs(T, jobs, njobs, LoadPlain{}, StorePlain{}, s);
+}
+
+void ntt_forward_jobs_moddown(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s) {
+    launch_forward_jobs(T, jobs, njobs, LoadPlain{}, StoreModDownJob{}, s);
+}
+
+void ntt_forward_jobs_rescale(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s) {
+    launch_forward_jobs(T, jobs, njobs, LoadRescaleJob{}, StoreRescaleJob{}, s);
+}
+
+void ntt_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s) {
+    const u32 tiles = T.N / kNttTile;
+    for (u32 j0 = 0; j0 < njobs; j0 += kMaxJobsPerLaunch) {
+        const dim3 grid(tiles, std::min(kMaxJobsPerLaunch, njobs - j0));
+        hipLaunchKernelGGL(k_inv_rows_jobs, grid, dim3(kNttThreads), 0, s, T, jobs + j0);
+        dispatch_sr(T.logN, [&](auto sr) {
+            constexpr int SR = decltype(sr)::value;
+            hipLaunchKernelGGL((k_inv_cols_jobs<SR>), grid, dim3(kNttThreads), 0, s, T, jobs + j0);
+        });
+    }
+    PHX_HIP_CHECK(hipGetLastError());
+}
+
 void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s) {
     if (nlimbs == 0 || npolys == 0) return;
     const u32 tiles = T.N / kNttTile;
@@ -638,23 +718,6 @@ void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, h
             hipLaunchKernelGGL((k_inv_cols<SR, StorePlain>), grid, dim3(kNttThreads), 0, s, T, Mc, StorePlain{});
         });
     }
-    PHX_HIP_CHECK(hipGetLastError());
-}
-
-void ntt_forward_cols(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s) {
-    if (nlimbs == 0 || npolys == 0) return;
-    dim3 grid(T.N / kNttTile, npolys, nlimbs);
-    dispatch_sr(T.logN, [&](auto sr) {
-        constexpr int SR = decltype(sr)::value;
-        hipLaunchKernelGGL((k_fwd_cols<SR, LoadPlain>), grid, dim3(kNttThreads), 0, s, T, M, LoadPlain{});
-    });
-    PHX_HIP_CHECK(hipGetLastError());
-}
-
-void ntt_forward_rows_ip(const NttTables &T, const RowsIpArgs &P, u32 nitems, hipStream_t s) {
-    if (nitems == 0) return;
-    require(P.beta >= 1 && P.beta <= 8, "fused key inner product supports 1 .. 8 digits");
-    hipLaunchKernelGGL(k_fwd_rows_ip, dim3(T.N / kNttTile, nitems, P.QlP), dim3(kNttThreads), 0, s, T, P);
     PHX_HIP_CHECK(hipGetLastError());
 }
 

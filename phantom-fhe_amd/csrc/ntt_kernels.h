@@ -70,6 +70,23 @@ struct LimbMap {
     }
 };
 
+// Work-list form of a launch: one entry per (polynomial, limb) to transform,
+// each with its own buffer and table row, so that one launch covers items of
+// different levels (ragged limb counts, separate buffers). Grid row y runs
+// entry y. The aux pointers and flags carry the load/store policy's operands
+// (out-of-place source of an inverse first pass; the moddown or rescale tail's
+// inputs and output); scale: the final inverse stage's [s0, s0', s1, s1']
+// (null = the row's n^-1 constants) or a policy's constant pair.
+struct NttJob {
+    u64 *data;          // the limb transformed in place
+    const u64 *aux0;
+    u64 *aux1;
+    const u64 *aux2;
+    const u64 *scale;
+    u32 row;            // table row of the limb's prime
+    u32 flags;
+};
+
 // DevModulus.pad of a row holds the bits of fl(1/q) when the row takes the
 // FP64 arithmetic (q < 2^50), 0 otherwise.
 struct NttTables {

@@ -22,7 +22,30 @@ Server::~Server() {
     } catch (...) {
     }
     for (auto &kv : reqs_) kv.second->ct.release(s_);
+    dev_free(arena_, s_);
+    dev_free(tables_, s_);
     (void)hipStreamSynchronize(s_);
+}
+
+void Server::run_plan(BatchPlan &p) {
+    // growth frees the old buffer after the work queued on the stream (stream-ordered)
+    if (p.scratch_bytes() > arena_bytes_) {
+        dev_free(arena_, s_);
+        arena_ = nullptr;
+        arena_bytes_ = 0;
+        const size_t want = p.scratch_bytes() + p.scratch_bytes() / 4;
+        arena_ = static_cast<u64 *>(dev_alloc(want, s_));
+        arena_bytes_ = want;
+    }
+    if (p.table_bytes() > tables_bytes_) {
+        dev_free(tables_, s_);
+        tables_ = nullptr;
+        tables_bytes_ = 0;
+        const size_t want = p.table_bytes() * 2;
+        tables_ = static_cast<uint8_t *>(dev_alloc(want, s_));
+        tables_bytes_ = want;
+    }
+    p.run_with(s_, arena_, tables_);
 }
 
 double Server::now() const {
@@ -133,13 +156,30 @@ u32 Server::step() {
                 b.push_back(&r->ct);   // squaring
             }
             tick.plans.emplace_back(BatchPlan::mul_relin(*ln.ctx, a, b, *ln.rlk, true));
-            tick.plans.back()->run(s_);
+            run_plan(*tick.plans.back());
         }
-        for (Request *r : res) {
-            Ciphertext out;
-            rescale_to_next(*ln.ctx, r->ct, out, s_);
-            r->ct.release(s_);
-            r->ct = out;
+        if (!res.empty()) {
+            // every rescale of the lane in one batched launch set: outputs at the next
+            // level from the pool, the inputs released after the plan's work (same stream)
+            std::vector<Ciphertext> outs(res.size());
+            std::vector<const Ciphertext *> ins;
+            std::vector<Ciphertext *> outp;
+            try {
+                for (size_t i = 0; i < res.size(); ++i) {
+                    outs[i].resize(*ln.ctx, res[i]->ct.chain_index + 1, 2, s_);
+                    ins.push_back(&res[i]->ct);
+                    outp.push_back(&outs[i]);
+                }
+                tick.plans.emplace_back(BatchPlan::rescale(*ln.ctx, ins, outp, true));
+                run_plan(*tick.plans.back());
+            } catch (...) {
+                for (auto &o : outs) o.release(s_);
+                throw;
+            }
+            for (size_t i = 0; i < res.size(); ++i) {
+                res[i]->ct.release(s_);
+                res[i]->ct = outs[i];
+            }
         }
         if (!rot.empty()) {
             std::vector<Ciphertext *> a;
@@ -149,7 +189,7 @@ u32 Server::step() {
                 steps.push_back(r->steps[r->next_op / 3]);
             }
             tick.plans.emplace_back(BatchPlan::rotate(*ln.ctx, a, steps, *ln.gk, true));
-            tick.plans.back()->run(s_);
+            run_plan(*tick.plans.back());
         }
         for (auto *v : {&mul, &res, &rot})
             for (Request *r : *v) {

@@ -8,12 +8,12 @@
 // Every tick (step()) admits pending requests in arrival order while the
 // memory they reserve (their ciphertext plus the per-item scratch of their
 // level) fits the budget and a slot is free, then issues the next operation
-// of every active request: the multiplications of one context go as one
-// transient batch plan, the rescales one after another, the rotations as one
-// batch plan. Requests admitted now share the launches of requests already
-// in flight. Work of consecutive ticks is queued on one stream, so the host
-// runs ahead of the device by at most `depth` ticks; a request is complete
-// when the event of the tick that ran its last operation has fired.
+// of every active request: the multiplications, the rescales and the rotations
+// of one context each go as one transient batch plan. Requests admitted now
+// share the launches of requests already in flight. Work of consecutive ticks
+// is queued on one stream, so the host runs ahead of the device by at most
+// `depth` ticks; a request is complete when the event of the tick that ran its
+// last operation has fired.
 #pragma once
 
 #include <chrono>
@@ -75,6 +75,8 @@ private:
         std::vector<u64> finishing;
     };
     void harvest(bool wait_oldest);
+    // runs a transient plan in the server's work memory (grown when too small)
+    void run_plan(BatchPlan &p);
 
     std::vector<Lane> lanes_;
     size_t budget_;
@@ -87,6 +89,12 @@ private:
     std::vector<u64> active_;
     std::deque<Tick> inflight_;
     size_t reserved_ = 0;
+    // work memory of the plans: ticks run one after another on one stream, so
+    // every plan uses the same arena and table buffer (stream-ordered)
+    u64 *arena_ = nullptr;
+    size_t arena_bytes_ = 0;
+    uint8_t *tables_ = nullptr;
+    size_t tables_bytes_ = 0;
     ServerStats st_;
     std::vector<double> latencies_;
     double first_arrival_ = -1, last_done_ = 0;

@@ -9,9 +9,6 @@
 //   ntt_fused_min_limbs  launches over fewer limbs than this use two passes
 //   ntt_chunk_limbs  two-pass launches over more limbs than this are split into
 //                slices of whole polynomials of about this many limbs (0 = never)
-//   ks_fused_ip  1: batched key switching runs the modup NTT's second pass fused
-//                with the key inner product; 0: separate kernels (default, faster
-//                on MI355X: the fused kernel needs 242 VGPRs)
 // Environment defaults: PHX_BCONV_VALU=1 -> bconv_mfma=0, PHX_NTT_INT_ONLY=1 -> ntt_fp64=0.
 #include <atomic>
 #include <cstdlib>
@@ -31,7 +28,6 @@ std::atomic<int> g_ntt_fp64{env_is_one("PHX_NTT_INT_ONLY") ? 0 : 1};
 std::atomic<int> g_ntt_fused{0};
 std::atomic<int> g_ntt_fused_min{512};
 std::atomic<int> g_ntt_chunk{0};
-std::atomic<int> g_ks_fused_ip{0};
 
 std::atomic<int> *slot(const char *key) {
     if (!key) return nullptr;
@@ -40,7 +36,6 @@ std::atomic<int> *slot(const char *key) {
     if (!std::strcmp(key, "ntt_fused")) return &g_ntt_fused;
     if (!std::strcmp(key, "ntt_fused_min_limbs")) return &g_ntt_fused_min;
     if (!std::strcmp(key, "ntt_chunk_limbs")) return &g_ntt_chunk;
-    if (!std::strcmp(key, "ks_fused_ip")) return &g_ks_fused_ip;
     return nullptr;
 }
 }  // namespace

@@ -122,6 +122,7 @@ _SIGS = {
     "phx_ckks_decode": (ctypes.c_int, [_vp, _u64p, _sz, ctypes.c_double, _dp, _dp]),
     "phx_multiply_and_relin_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "phx_multiply_and_relin_inplace_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "phx_batch_launches": (ctypes.c_int, [_vp, ctypes.POINTER(_sz)]),
     "phx_ciphertext_clone": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
     "phx_multiply_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "phx_relinearize_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
@@ -848,6 +849,12 @@ class batch:
 
     def replay(self, stream=None):
         _check(lib().phx_batch_replay(self._h, _stream(stream)))
+
+    def launches(self):
+        """kernel launches of one run (kernel nodes of the captured graph)"""
+        n = _sz()
+        _check(lib().phx_batch_launches(self._h, ctypes.byref(n)))
+        return n.value
 
     def scratch_bytes(self):
         v = _sz()

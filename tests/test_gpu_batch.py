@@ -127,39 +127,39 @@ def test_batch_n16_q35_bit_exact():
         assert np.array_equal(c1[i].download(N), octx.multiply_and_relin(35, a[i], b[i], key)), i
 
 
-def test_batch_fused_inner_product_variant(small):
-    """ks_fused_ip=1: the modup NTT's second pass fused with the key inner
-    product gives the same outputs (mixed levels, squaring, rotation)."""
+def test_batch_one_launch_per_stage_across_levels(small):
+    """a plan over items of four levels (with a squaring) issues one launch per
+    pipeline stage, the same count as a one-level plan, and stays bit-exact; so
+    does a rotation plan over two levels"""
     ctx, octx, mods, key = small
     N = ctx.N
-    say("case: batch fused inner product")
-    old = ph.tune_get("ks_fused_ip")
-    try:
-        ph.tune_set("ks_fused_ip", 1)
-        rng = np.random.default_rng(21)
-        chains = [1, 3, 1, 7]
-        data = [(c, ctx.limbs_at(c), rand_ct(rng, mods[:ctx.limbs_at(c)], N),
-                 rand_ct(rng, mods[:ctx.limbs_at(c)], N)) for c in chains]
-        rk = ph.relin_key(ctx, host=key)
-        c1 = [upload_ct(ctx, a, c) for c, _, a, _ in data]
-        c2 = [upload_ct(ctx, b, c) for c, _, _, b in data]
-        c2[2] = c1[2]
-        bt = ph.batch.multiply_and_relin(ctx, c1, c2, rk)
-        bt.run()
-        ph.device_synchronize()
-        for i, (c, Ql, a, b) in enumerate(data):
-            other = a if i == 2 else b
-            assert np.array_equal(c1[i].download(N), octx.multiply_and_relin(Ql, a, other, key)), i
-        elt = ph.galois_elt_from_step(2, N)
-        gk = ph.galois_key(ctx, [elt], host=key[None])
-        x = rand_ct(rng, mods[:5], N)
-        cx = [upload_ct(ctx, x, 3)]
-        bt = ph.batch.rotate(ctx, cx, [2], gk)
-        bt.run()
-        ph.device_synchronize()
-        assert np.array_equal(cx[0].download(N), octx.apply_galois(5, x, elt, key))
-    finally:
-        ph.tune_set("ks_fused_ip", old)
+    say("case: batch over four levels, one launch per stage")
+    rng = np.random.default_rng(21)
+    chains = [1, 3, 1, 7]
+    data = [(c, ctx.limbs_at(c), rand_ct(rng, mods[:ctx.limbs_at(c)], N),
+             rand_ct(rng, mods[:ctx.limbs_at(c)], N)) for c in chains]
+    rk = ph.relin_key(ctx, host=key)
+    c1 = [upload_ct(ctx, a, c) for c, _, a, _ in data]
+    c2 = [upload_ct(ctx, b, c) for c, _, _, b in data]
+    c2[2] = c1[2]
+    bt = ph.batch.multiply_and_relin(ctx, c1, c2, rk)
+    one = ph.batch.multiply_and_relin(ctx, [upload_ct(ctx, data[0][2], 1)], [upload_ct(ctx, data[0][3], 1)], rk)
+    assert bt.launches() == one.launches() == 12
+    bt.run()
+    ph.device_synchronize()
+    for i, (c, Ql, a, b) in enumerate(data):
+        other = a if i == 2 else b
+        assert np.array_equal(c1[i].download(N), octx.multiply_and_relin(Ql, a, other, key)), i
+    elt = ph.galois_elt_from_step(2, N)
+    gk = ph.galois_key(ctx, [elt], host=key[None])
+    x, y = rand_ct(rng, mods[:5], N), rand_ct(rng, mods[:2], N)
+    cx = [upload_ct(ctx, x, 3), upload_ct(ctx, y, 6)]
+    bt = ph.batch.rotate(ctx, cx, [2, 2], gk)
+    assert bt.launches() == 12
+    bt.run()
+    ph.device_synchronize()
+    assert np.array_equal(cx[0].download(N), octx.apply_galois(5, x, elt, key))
+    assert np.array_equal(cx[1].download(N), octx.apply_galois(2, y, elt, key))
 
 
 ROTATION_STEPS = [1, 2, 3, 4, 5, 7, 8, 16, 32, 64, 128, 256, 512, 1024, -1, -5]
@@ -191,6 +191,7 @@ def test_128_ciphertexts_batched_rotations_bit_exact():
         cts.append(upload_ct(ctx, a, chains[i]))
     stream = ph.stream_create()
     bt = ph.batch.rotate(ctx, cts, [ROTATION_STEPS[k] for k in steps_idx], gk)
+    assert bt.launches() == 12      # one launch per stage over all 31 levels
     bt.capture(stream)
     bt.replay(stream)
     ph.synchronize(stream)
