@@ -360,40 +360,53 @@ int phx_device_count(int *n);
  *   "bconv_mfma" 1 = FastBConv on the matrix cores (default), 0 = VALU kernel
  *   "ntt_fp64"   1 = exact FP64 butterflies for primes < 2^50 (default), 0 = 64-bit integer
  *                only; takes effect for contexts created afterwards
+ *   "ntt_fused", "ntt_fused_min_limbs", "ntt_chunk_limbs", "ntt_cols_threads",
+ *   "ntt_rows_threads", "bconv_wgs_per_cu": the process-wide defaults of the
+ *                per-context variants below
  * unknown keys return PHX_EINVAL */
 int phx_tune_set(const char *key, int value);
 int phx_tune_get(const char *key, int *value);
 /* Per-context kernel variants (no reference counterpart: the reference fixes its
- * launch shapes at compile time). choice[4] = {ntt_fused, ntt_fused_min_limbs,
- * ntt_chunk_limbs, bconv_mfma}; -1 = follow the process-wide switch of that name.
+ * launch shapes at compile time). choice[PHX_TUNE_AXES] = {ntt_fused,
+ * ntt_fused_min_limbs, ntt_chunk_limbs, bconv_mfma, ntt_cols_threads,
+ * ntt_rows_threads, bconv_wgs_per_cu}; -1 = follow the process-wide switch of that name.
  *   ntt_fused           0 = two launches per NTT; 1, 2, 4 = one workgroup of that
  *                       many x 256 threads per limb (N / 4096 a multiple of it)
  *   ntt_fused_min_limbs launches over fewer limbs keep two launches
  *   ntt_chunk_limbs     two-launch NTTs over more limbs go in slices of about this
  *                       many limbs (0 = one launch pair)
- *   bconv_mfma          1 = FastBConv on the matrix cores, 0 = VALU kernel */
+ *   bconv_mfma          1 = FastBConv on the matrix cores, 0 = VALU kernel
+ *   ntt_cols_threads    tile shape of the column passes: 256 or 512 threads x 16
+ *                       coefficients (16 or 32 columns at N = 2^16)
+ *   ntt_rows_threads    tile shape of the row passes: 128, 256 or 512 threads
+ *                       (8, 16 or 32 rows of 256 coefficients)
+ *   bconv_wgs_per_cu    matrix-core FastBConv workgroups per CU: 1, 2, 4 (0 = as many
+ *                       as the LDS holds) */
+#define PHX_TUNE_AXES 7
 typedef struct {
-    int choice[4];       /* the variants applied */
+    int choice[PHX_TUNE_AXES];   /* the variants applied */
     double default_us;   /* objective with every switch at -1: one HMult+relin at the top level plus
                             one op of an 8-pair batch plan, microseconds */
     double best_us;      /* the same with `choice` */
     uint32_t trials;     /* candidates timed */
+    char trials_json[8192];   /* every candidate and its objective: [{"choice": "...", "us": x}, ...] */
 } phx_tune_report;
-int phx_context_get_tuning(const phx_context *ctx, int choice[4]);
-int phx_context_set_tuning(phx_context *ctx, const int choice[4]);
+int phx_context_get_tuning(const phx_context *ctx, int choice[PHX_TUNE_AXES]);
+int phx_context_set_tuning(phx_context *ctx, const int choice[PHX_TUNE_AXES]);
 /* Autotuner: times HMult+relin at the top level, plain and batched (synthetic
- * residues in HBM), for each candidate, applies the fastest (a candidate must beat
- * the incumbent by 2%) to ctx and, when cache_path is non-empty,
- * stores it in that JSON cache under (scheme, N, size_Q, size_P). Contexts created
- * while the environment variable PHX_TUNE_CACHE names such a file apply its entry. */
+ * residues in HBM), for each candidate (NTT launch form, NTT tile shape, FastBConv
+ * kind and occupancy), applies the fastest (a candidate must beat the incumbent by
+ * 2%) to ctx and, when cache_path is non-empty, stores it in that JSON cache under
+ * (scheme, N, size_Q, size_P). Contexts created while the environment variable
+ * PHX_TUNE_CACHE names such a file apply its entry. */
 int phx_autotune(phx_context *ctx, const char *cache_path, phx_tune_report *report, phx_stream stream);
 /* applies the cache entry for ctx's shape if there is one (*found = 0 or 1) */
 int phx_tune_load(phx_context *ctx, const char *cache_path, int *found);
 /* host-only cache access (no GPU needed) */
 int phx_tune_cache_store(const char *cache_path, int scheme, uint64_t N, size_t size_Q, size_t size_P,
-                         const int choice[4], double hmult_relin_us);
+                         const int choice[PHX_TUNE_AXES], double hmult_relin_us);
 int phx_tune_cache_lookup(const char *cache_path, int scheme, uint64_t N, size_t size_Q, size_t size_P,
-                          int choice[4], int *found);
+                          int choice[PHX_TUNE_AXES], int *found);
 /* streaming copy kernel of n u64 elements (no reference counterpart): the known-size
  * copy that HBM counters and bandwidth figures are calibrated against */
 int phx_copy_u64(uint64_t *dst, const uint64_t *src, size_t n, phx_stream stream);

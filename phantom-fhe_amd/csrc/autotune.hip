@@ -5,10 +5,16 @@
 //      sliced to {all, 128, 256, 512} limbs, or one workgroup of G x 256
 //      threads per limb (G in {1, 2, 4} where N / 4096 is a multiple of G) for
 //      launches of at least {0, 64, 256} limbs;
-//   2. the FastBConv kernel: matrix cores or VALU.
-// The radix split of a transform is not searched: the 4096-coefficient tile
-// (16 registers x 256 lanes) fixes the second pass at 2^8 points per column
-// block and the first at N / 2^8.
+//   2. the tile shape of the two passes: row-pass workgroups of {128, 256, 512}
+//      threads (8, 16 or 32 rows of 256 coefficients), then column-pass
+//      workgroups of {256, 512} threads (16 or 32 columns at N = 2^16); each
+//      thread holds 16 coefficients, so the tile is 16 x threads coefficients;
+//   3. the FastBConv kernel: matrix cores or VALU;
+//   4. for the matrix-core FastBConv, its workgroups per CU: {1, 2, as many as
+//      the LDS holds}.
+// The radix split of a transform follows from the tile: the column pass runs
+// the first log2(N / 256) stages, the row pass the last 8, each as radix-16
+// register rounds.
 // The objective is the time of one HMult+relin at the top level plus the
 // per-pair time of an 8-pair batch plan of the same (its launches are the
 // large ones: a candidate that only changes launches above some size must be
@@ -92,7 +98,9 @@ private:
 std::string describe(const TuneChoice &c) {
     std::ostringstream o;
     o << "ntt_fused=" << c.ntt_fused << ",ntt_fused_min_limbs=" << c.ntt_fused_min_limbs
-      << ",ntt_chunk_limbs=" << c.ntt_chunk_limbs << ",bconv_mfma=" << c.bconv_mfma;
+      << ",ntt_chunk_limbs=" << c.ntt_chunk_limbs << ",bconv_mfma=" << c.bconv_mfma
+      << ",ntt_cols_threads=" << c.ntt_cols_threads << ",ntt_rows_threads=" << c.ntt_rows_threads
+      << ",bconv_wgs_per_cu=" << c.bconv_wgs_per_cu;
     return o.str();
 }
 }  // namespace
@@ -172,7 +180,34 @@ TuneResult autotune(Context &ctx, hipStream_t s) {
             const double us = time_choice(cand);
             if (us < kMargin * best_us) best_us = us, best = cand;
         }
-        // 2. FastBConv kernel: the matrix-core form stays unless the VALU form wins
+        // 2. tile shape: the row passes, then the column passes (shapes whose tile
+        //    exceeds a limb are left out)
+        std::vector<int> rows, cols;
+        for (int t : {128, 256, 512})
+            if ((u64)t * kNttRegs <= N) rows.push_back(t);
+        for (int t : {256, 512})
+            if ((u64)t * kNttRegs <= N) cols.push_back(t);
+        auto axis = [&](int TuneChoice::*field, const std::vector<int> &v, int inc) {
+            if (v.size() < 2) {
+                best.*field = inc;
+                return;
+            }
+            TuneChoice inc_c = best;
+            inc_c.*field = inc;
+            double inc_us = time_choice(inc_c);
+            for (int x : v) {
+                if (x == inc) continue;
+                TuneChoice cand = best;
+                cand.*field = x;
+                const double us = time_choice(cand);
+                if (us < kMargin * inc_us) inc_us = us, inc_c = cand;
+            }
+            best = inc_c;
+            best_us = inc_us;
+        };
+        axis(&TuneChoice::ntt_rows_threads, rows, 256);
+        axis(&TuneChoice::ntt_cols_threads, cols, 256);
+        // 3. FastBConv kernel: the matrix-core form stays unless the VALU form wins
         TuneChoice mf = best;
         mf.bconv_mfma = 1;
         const double mf_us = time_choice(mf);
@@ -180,8 +215,11 @@ TuneResult autotune(Context &ctx, hipStream_t s) {
         va.bconv_mfma = 0;
         const double va_us = time_choice(va);
         best = va_us < kMargin * mf_us ? va : mf;
+        best_us = va_us < kMargin * mf_us ? va_us : mf_us;
+        // 4. the matrix-core FastBConv's workgroups per CU
+        if (best.bconv_mfma == 1) axis(&TuneChoice::bconv_wgs_per_cu, {0, 1, 2}, 0);
         res.choice = best;
-        res.best_us = va_us < kMargin * mf_us ? va_us : mf_us;
+        res.best_us = best_us;
     } catch (...) {
         ctx.set_tuning(saved);
         plan.reset();
@@ -232,7 +270,8 @@ void tune_cache_store(const std::string &path, const std::string &key, const Tun
     std::ostringstream e;
     e << "  {" << tag << ", \"ntt_fused\": " << c.ntt_fused << ", \"ntt_fused_min_limbs\": " << c.ntt_fused_min_limbs
       << ", \"ntt_chunk_limbs\": " << c.ntt_chunk_limbs << ", \"bconv_mfma\": " << c.bconv_mfma
-      << ", \"hmult_relin_us\": " << us << "}";
+      << ", \"ntt_cols_threads\": " << c.ntt_cols_threads << ", \"ntt_rows_threads\": " << c.ntt_rows_threads
+      << ", \"bconv_wgs_per_cu\": " << c.bconv_wgs_per_cu << ", \"hmult_relin_us\": " << us << "}";
     kept.push_back(e.str());
     std::ofstream out(path, std::ios::trunc);
     require((bool)out, "cannot write the tuning cache " + path);
@@ -255,6 +294,10 @@ bool tune_cache_lookup(const std::string &path, const std::string &key, TuneChoi
             !entry_field(line, "ntt_chunk_limbs", &c.ntt_chunk_limbs) ||
             !entry_field(line, "bconv_mfma", &c.bconv_mfma))
             fail(kInvalidArgument, "malformed tuning cache entry for " + key + " in " + path);
+        // axes added later are absent from older entries: they follow the process-wide switches
+        entry_field(line, "ntt_cols_threads", &c.ntt_cols_threads);
+        entry_field(line, "ntt_rows_threads", &c.ntt_rows_threads);
+        entry_field(line, "bconv_wgs_per_cu", &c.bconv_wgs_per_cu);
         *out = c;
         return true;
     }

@@ -15,8 +15,8 @@
 // into a hipGraph that replay() launches with one call (nothing is allocated,
 // set or synchronised inside run()). Transient plans (the serving loop) run in
 // memory the caller hands over (run_with) or take it from the stream-ordered
-// pool inside run(); they upload their tables on the same stream, and the plan
-// object must outlive that work.
+// pool inside run(); they upload their tables on the same stream from pinned
+// host memory, and the plan object must outlive that work.
 //
 // BFV plans (multiply + relinearise at the top level) stage both operands of
 // every item contiguously, run each multiply stage once over all items, key-
@@ -58,8 +58,11 @@ public:
 
     void run(hipStream_t s);
     // transient plans: run in caller memory of at least scratch_bytes() and
-    // table_bytes() (device), free of other work on s from here on
-    void run_with(hipStream_t s, u64 *arena, uint8_t *tables);
+    // table_bytes() (device), free of other work on s from here on; the work
+    // lists go through `staging`, pinned host memory of table_bytes() that must
+    // stay untouched until the stream has passed this call's work (the copy
+    // from it is ordered on s after the work queued before it)
+    void run_with(hipStream_t s, u64 *arena, uint8_t *tables, uint8_t *staging);
     size_t table_bytes() const;
     void capture(hipStream_t s);
     void replay(hipStream_t s);
@@ -87,7 +90,7 @@ private:
     size_t item_words(const Item &it) const;
     void layout();                           // scratch offsets, arena size; persistent: allocate and bind
     void bind(u64 *arena);                   // build the host work lists against an arena
-    void upload_tables(hipStream_t s, bool async);
+    void upload_tables(hipStream_t s, uint8_t *staging);
     void record(hipStream_t s);              // every launch of the plan
     void record_keyswitch(hipStream_t s);
     void after_launch();                     // host-side metadata of the outputs

@@ -208,7 +208,7 @@ void BatchPlan::layout() {
         PHX_HIP_CHECK(hipMalloc(&arena_, std::max<size_t>(scratch_bytes_, 256)));
         bind(arena_);
         PHX_HIP_CHECK(hipMalloc(&tables_, std::max<size_t>(host_tables_.size(), 256)));
-        upload_tables(nullptr, false);
+        upload_tables(nullptr, nullptr);
     }
 }
 
@@ -332,12 +332,16 @@ void BatchPlan::bind(u64 *arena) {
     off_bfv_sq_ = append(host_tables_, bfv_square_);
 }
 
-void BatchPlan::upload_tables(hipStream_t s, bool async) {
-    if (async)
-        PHX_HIP_CHECK(hipMemcpyAsync(tables_bound_, host_tables_.data(), host_tables_.size(), hipMemcpyHostToDevice, s));
-    else
+// staging null: a blocking copy into the plan's own tables (persistent plans);
+// else a copy from the pinned staging buffer ordered on s (transient plans)
+void BatchPlan::upload_tables(hipStream_t s, uint8_t *staging) {
+    if (staging) {
+        std::memcpy(staging, host_tables_.data(), host_tables_.size());
+        PHX_HIP_CHECK(hipMemcpyAsync(tables_bound_, staging, host_tables_.size(), hipMemcpyHostToDevice, s));
+    } else {
         PHX_HIP_CHECK(hipMemcpy(tables_, host_tables_.data(), host_tables_.size(), hipMemcpyHostToDevice));
-    if (!async) tables_bound_ = tables_;
+        tables_bound_ = tables_;
+    }
 }
 
 BatchPlan *BatchPlan::mul_relin(const Context &ctx, const std::vector<Ciphertext *> &ct1,
@@ -408,7 +412,7 @@ BatchPlan *BatchPlan::join(const std::vector<BatchPlan *> &parts) {
         PHX_HIP_CHECK(hipFree(q->arena_));
         q->arena_ = nullptr;
         q->bind(p->arena_);
-        q->upload_tables(nullptr, false);
+        q->upload_tables(nullptr, nullptr);
         if (q->exec_) (void)hipGraphExecDestroy(q->exec_);
         if (q->graph_) (void)hipGraphDestroy(q->graph_);
         q->exec_ = nullptr;
@@ -614,12 +618,13 @@ size_t BatchPlan::table_bytes() const {
     return std::max<size_t>(off, 256);
 }
 
-void BatchPlan::run_with(hipStream_t s, u64 *arena, uint8_t *tables) {
+void BatchPlan::run_with(hipStream_t s, u64 *arena, uint8_t *tables, uint8_t *staging) {
     require(transient_, "only transient plans run in caller memory");
+    require(staging != nullptr, "a transient plan needs pinned staging for its work lists");
     bind(arena);
     require(host_tables_.size() <= table_bytes(), "batch work lists larger than their buffer");
     tables_bound_ = tables;
-    upload_tables(s, true);
+    upload_tables(s, staging);
     record(s);
     tables_bound_ = nullptr;
     after_launch();
@@ -627,9 +632,21 @@ void BatchPlan::run_with(hipStream_t s, u64 *arena, uint8_t *tables) {
 
 void BatchPlan::run(hipStream_t s) {
     if (transient_) {
+        // a one-off run: device memory from the pool, the staging copy completed
+        // before the staging buffer is freed
         Scratch arena(std::max<size_t>(scratch_bytes_, 256), s);
         Scratch t(table_bytes(), s);
-        run_with(s, arena.get(), reinterpret_cast<uint8_t *>(t.get()));
+        void *staging = nullptr;
+        PHX_HIP_CHECK(hipHostMalloc(&staging, table_bytes(), hipHostMallocDefault));
+        try {
+            run_with(s, arena.get(), reinterpret_cast<uint8_t *>(t.get()), static_cast<uint8_t *>(staging));
+            PHX_HIP_CHECK(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            (void)hipHostFree(staging);
+            throw;
+        }
+        PHX_HIP_CHECK(hipHostFree(staging));
         return;
     }
     record(s);

@@ -1,6 +1,7 @@
 // capi.cpp -- the extern "C" boundary (include/phantom_amd.h).
 #include "../../include/phantom_amd.h"
 
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -88,20 +89,27 @@ inline void nn(const void *p, const char *what) {
 }  // namespace
 
 namespace {
-// tuning choice <-> the C array {ntt_fused, ntt_fused_min_limbs, ntt_chunk_limbs, bconv_mfma}
-TuneChoice choice_of(const int c[4]) {
+// tuning choice <-> the C array of PHX_TUNE_AXES entries {ntt_fused, ntt_fused_min_limbs,
+// ntt_chunk_limbs, bconv_mfma, ntt_cols_threads, ntt_rows_threads, bconv_wgs_per_cu}
+TuneChoice choice_of(const int c[PHX_TUNE_AXES]) {
     TuneChoice t;
     t.ntt_fused = c[0];
     t.ntt_fused_min_limbs = c[1];
     t.ntt_chunk_limbs = c[2];
     t.bconv_mfma = c[3];
+    t.ntt_cols_threads = c[4];
+    t.ntt_rows_threads = c[5];
+    t.bconv_wgs_per_cu = c[6];
     return t;
 }
-void put_choice(const TuneChoice &t, int c[4]) {
+void put_choice(const TuneChoice &t, int c[PHX_TUNE_AXES]) {
     c[0] = t.ntt_fused;
     c[1] = t.ntt_fused_min_limbs;
     c[2] = t.ntt_chunk_limbs;
     c[3] = t.bconv_mfma;
+    c[4] = t.ntt_cols_threads;
+    c[5] = t.ntt_rows_threads;
+    c[6] = t.bconv_wgs_per_cu;
 }
 // A ciphertext handed to an evaluator entry point must belong to the context:
 // its ring degree and limb count are the context's at its chain index (the
@@ -1370,7 +1378,7 @@ int phx_tune_get(const char *key, int *value) {
     });
 }
 
-int phx_context_get_tuning(const phx_context *ctx, int choice[4]) {
+int phx_context_get_tuning(const phx_context *ctx, int choice[PHX_TUNE_AXES]) {
     return guard([&] {
         nn(ctx, "context");
         nn(choice, "choice");
@@ -1378,7 +1386,7 @@ int phx_context_get_tuning(const phx_context *ctx, int choice[4]) {
     });
 }
 
-int phx_context_set_tuning(phx_context *ctx, const int choice[4]) {
+int phx_context_set_tuning(phx_context *ctx, const int choice[PHX_TUNE_AXES]) {
     return guard([&] {
         nn(ctx, "context");
         nn(choice, "choice");
@@ -1399,6 +1407,18 @@ int phx_autotune(phx_context *ctx, const char *cache_path, phx_tune_report *repo
             report->default_us = r.default_us;
             report->best_us = r.best_us;
             report->trials = (uint32_t)r.trials.size();
+            // every candidate with its objective, as a JSON array (truncated to the buffer)
+            std::string j = "[";
+            for (size_t i = 0; i < r.trials.size(); ++i) {
+                char us[32];
+                std::snprintf(us, sizeof us, "%.2f", r.trials[i].second);
+                const std::string e = std::string(i ? ", " : "") + "{\"choice\": \"" + r.trials[i].first +
+                                      "\", \"us\": " + us + "}";
+                if (j.size() + e.size() + 2 > sizeof report->trials_json) break;
+                j += e;
+            }
+            j += "]";
+            std::snprintf(report->trials_json, sizeof report->trials_json, "%s", j.c_str());
         }
     });
 }
@@ -1415,7 +1435,7 @@ int phx_tune_load(phx_context *ctx, const char *cache_path, int *found) {
 }
 
 int phx_tune_cache_store(const char *cache_path, int scheme, uint64_t N, size_t size_Q, size_t size_P,
-                         const int choice[4], double hmult_relin_us) {
+                         const int choice[PHX_TUNE_AXES], double hmult_relin_us) {
     return guard([&] {
         nn(cache_path, "cache_path");
         nn(choice, "choice");
@@ -1425,7 +1445,7 @@ int phx_tune_cache_store(const char *cache_path, int scheme, uint64_t N, size_t 
 }
 
 int phx_tune_cache_lookup(const char *cache_path, int scheme, uint64_t N, size_t size_Q, size_t size_P,
-                          int choice[4], int *found) {
+                          int choice[PHX_TUNE_AXES], int *found) {
     return guard([&] {
         nn(cache_path, "cache_path");
         nn(choice, "choice");

@@ -813,10 +813,19 @@ void Context::set_tuning(const TuneChoice &t) {
             "ntt_fused must be -1, 0, 1, 2 or 4");
     require(t.bconv_mfma >= -1 && t.bconv_mfma <= 1, "bconv_mfma must be -1, 0 or 1");
     require(t.ntt_chunk_limbs >= -1 && t.ntt_fused_min_limbs >= -1, "limb counts must be -1 or non-negative");
+    require(t.ntt_cols_threads == -1 || t.ntt_cols_threads == 256 || t.ntt_cols_threads == 512,
+            "ntt_cols_threads must be -1, 256 or 512");
+    require(t.ntt_rows_threads == -1 || t.ntt_rows_threads == 128 || t.ntt_rows_threads == 256 ||
+                t.ntt_rows_threads == 512,
+            "ntt_rows_threads must be -1, 128, 256 or 512");
+    require(t.bconv_wgs_per_cu >= -1 && t.bconv_wgs_per_cu <= 4 && t.bconv_wgs_per_cu != 3,
+            "bconv_wgs_per_cu must be -1, 0, 1, 2 or 4");
     tune_ = t;
     tables_.fused_g = t.ntt_fused;
     tables_.fused_min = t.ntt_fused_min_limbs;
     tables_.chunk_limbs = t.ntt_chunk_limbs;
+    tables_.cols_nt = t.ntt_cols_threads;
+    tables_.rows_nt = t.ntt_rows_threads;
 }
 
 const HpsConsts &Context::hps_level(u32 d) const {

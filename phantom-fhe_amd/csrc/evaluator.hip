@@ -273,7 +273,10 @@ void bconv_run(const Context &ctx, const std::vector<BconvDesc> &descs_host, con
             (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
             return (size_t)std::max(v, 65536);
         }();
-        const u32 per_cu = (u32)std::max<size_t>(1, std::min<size_t>(lds_cu / lds, 2048 / kBconvThreads));
+        u32 per_cu = (u32)std::max<size_t>(1, std::min<size_t>(lds_cu / lds, 2048 / kBconvThreads));
+        const int cap = ctx.tuning().bconv_wgs_per_cu >= 0 ? ctx.tuning().bconv_wgs_per_cu
+                                                           : tune_value("bconv_wgs_per_cu");
+        if (cap > 0) per_cu = std::min<u32>(per_cu, (u32)cap);
         const u32 tiles = (u32)(N / kBconvTileCols);
         const u32 waves = kBconvThreads / 64;
         const u32 gx = std::max<u32>(1, std::min<u32>((tiles + waves - 1) / waves, cus * per_cu / jobs));

@@ -42,34 +42,6 @@ struct StoreModDown {
     }
 };
 
-// Batched moddown tail: polynomial z = 2*item + half of the batch writes
-// out[item] + half*Ql*N  =  base + (cx - NTT(delta)) * P^-1, where base is the
-// output itself (in place: relinearisation) or, with base0 set, base0's item
-// slice for half 0 and zero for half 1 (Galois rotation).
-struct StoreModDownBatch {
-    u64 *const *out;     // [items] ciphertext data, [2][Ql][N]
-    const u64 *base0;    // [items][Ql][N] or null
-    const u64 *cx;       // [2*items][QlP][N]
-    const u64 *pinv;     // [Ql][2]
-    u64 cx_stride;
-    u32 N, Ql;
-    __device__ __forceinline__ void operator()(u64 *, u32 i, u64 v, u32 limb, u32 poly,
-                                               const DevModulus &m) const {
-        const u64 q = m.value;
-        const u64 off = (u64)limb * N + i;
-        const u32 item = poly >> 1, half = poly & 1;
-        const u64 d = sub_mod(cx[poly * cx_stride + off], v, q);
-        const u64 r = mul_shoup(d, pinv[2 * limb], pinv[2 * limb + 1], q);
-        u64 *o = out[item] + (u64)half * Ql * N + off;
-        u64 base;
-        if (base0)
-            base = half ? 0 : base0[(u64)item * Ql * N + off];
-        else
-            base = *o;
-        *o = add_mod(base, r, q);
-    }
-};
-
 // Rescale (CKKS, floor): the NTT input for limb j is [c_last]_{q_j} taken from
 // the coefficient-form last limb; the output is (c_j - NTT(.)) * q_last^-1.
 // (reference: DRNSTool::divide_and_round_q_last_ntt, src/rns.cu:1128-1184)
@@ -156,8 +128,6 @@ void ntt_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, h
 void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s);
 void ntt_forward_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const StoreModDown &st,
                        hipStream_t s);
-void ntt_forward_store_batch(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys,
-                             const StoreModDownBatch &st, hipStream_t s);
 void ntt_forward_rescale(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const LoadRescale &ld,
                          const StoreRescale &st, hipStream_t s);
 

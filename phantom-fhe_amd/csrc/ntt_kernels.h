@@ -104,6 +104,10 @@ struct NttTables {
     int fused_g;
     int fused_min;
     int chunk_limbs;
+    // tile shape: threads of the column / row pass workgroups (128, 256, 512; -1 =
+    // the process-wide switches ntt_cols_threads / ntt_rows_threads)
+    int cols_nt;
+    int rows_nt;
 };
 
 // Storage slot of table entry k (N >= 4096): entries below N/16 in place; the

@@ -22,30 +22,51 @@ Server::~Server() {
     } catch (...) {
     }
     for (auto &kv : reqs_) kv.second->ct.release(s_);
-    dev_free(arena_, s_);
-    dev_free(tables_, s_);
     (void)hipStreamSynchronize(s_);
+    (void)hipFree(arena_);
+    (void)hipFree(tables_);
+    for (auto &t : inflight_)
+        for (auto &b : t.staging) (void)hipHostFree(b.p);
+    for (auto &b : staging_free_) (void)hipHostFree(b.p);
 }
 
-void Server::run_plan(BatchPlan &p) {
-    // growth frees the old buffer after the work queued on the stream (stream-ordered)
-    if (p.scratch_bytes() > arena_bytes_) {
-        dev_free(arena_, s_);
-        arena_ = nullptr;
-        arena_bytes_ = 0;
-        const size_t want = p.scratch_bytes() + p.scratch_bytes() / 4;
-        arena_ = static_cast<u64 *>(dev_alloc(want, s_));
-        arena_bytes_ = want;
+void Server::run_plan(Tick &t, BatchPlan &p) {
+    // growth waits for the work queued on the stream, then replaces the buffer
+    if (p.scratch_bytes() > arena_bytes_ || p.table_bytes() > tables_bytes_) {
+        PHX_HIP_CHECK(hipStreamSynchronize(s_));
+        if (p.scratch_bytes() > arena_bytes_) {
+            PHX_HIP_CHECK(hipFree(arena_));
+            arena_ = nullptr;
+            arena_bytes_ = 0;
+            const size_t want = p.scratch_bytes() + p.scratch_bytes() / 4;
+            PHX_HIP_CHECK(hipMalloc(&arena_, want));
+            arena_bytes_ = want;
+        }
+        if (p.table_bytes() > tables_bytes_) {
+            PHX_HIP_CHECK(hipFree(tables_));
+            tables_ = nullptr;
+            tables_bytes_ = 0;
+            const size_t want = p.table_bytes() * 2;
+            PHX_HIP_CHECK(hipMalloc(&tables_, want));
+            tables_bytes_ = want;
+        }
     }
-    if (p.table_bytes() > tables_bytes_) {
-        dev_free(tables_, s_);
-        tables_ = nullptr;
-        tables_bytes_ = 0;
-        const size_t want = p.table_bytes() * 2;
-        tables_ = static_cast<uint8_t *>(dev_alloc(want, s_));
-        tables_bytes_ = want;
+    // a staging buffer of the free list that is large enough, else a new one
+    Staging st;
+    for (size_t i = 0; i < staging_free_.size(); ++i)
+        if (staging_free_[i].bytes >= p.table_bytes()) {
+            st = staging_free_[i];
+            staging_free_.erase(staging_free_.begin() + (long)i);
+            break;
+        }
+    if (!st.p) {
+        st.bytes = p.table_bytes() * 2;
+        void *h = nullptr;
+        PHX_HIP_CHECK(hipHostMalloc(&h, st.bytes, hipHostMallocDefault));
+        st.p = static_cast<uint8_t *>(h);
     }
-    p.run_with(s_, arena_, tables_);
+    t.staging.push_back(st);
+    p.run_with(s_, arena_, tables_, st.p);
 }
 
 double Server::now() const {
@@ -116,6 +137,7 @@ void Server::harvest(bool wait_oldest) {
             ++st_.completed;
         }
         (void)hipEventDestroy(t.ev);
+        for (auto &b : t.staging) staging_free_.push_back(b);
         inflight_.pop_front();
     }
 }
@@ -156,7 +178,7 @@ u32 Server::step() {
                 b.push_back(&r->ct);   // squaring
             }
             tick.plans.emplace_back(BatchPlan::mul_relin(*ln.ctx, a, b, *ln.rlk, true));
-            run_plan(*tick.plans.back());
+            run_plan(tick, *tick.plans.back());
         }
         if (!res.empty()) {
             // every rescale of the lane in one batched launch set: outputs at the next
@@ -171,7 +193,7 @@ u32 Server::step() {
                     outp.push_back(&outs[i]);
                 }
                 tick.plans.emplace_back(BatchPlan::rescale(*ln.ctx, ins, outp, true));
-                run_plan(*tick.plans.back());
+                run_plan(tick, *tick.plans.back());
             } catch (...) {
                 for (auto &o : outs) o.release(s_);
                 throw;
@@ -189,7 +211,7 @@ u32 Server::step() {
                 steps.push_back(r->steps[r->next_op / 3]);
             }
             tick.plans.emplace_back(BatchPlan::rotate(*ln.ctx, a, steps, *ln.gk, true));
-            run_plan(*tick.plans.back());
+            run_plan(tick, *tick.plans.back());
         }
         for (auto *v : {&mul, &res, &rot})
             for (Request *r : *v) {

@@ -69,14 +69,20 @@ private:
         size_t state_bytes = 0, scratch_bytes = 0;   // reserved while active (state until taken)
         int state = 0;
     };
+    // pinned host buffer the work lists of one plan are copied from
+    struct Staging {
+        uint8_t *p = nullptr;
+        size_t bytes = 0;
+    };
     struct Tick {
         hipEvent_t ev = nullptr;
         std::vector<std::unique_ptr<BatchPlan>> plans;
+        std::vector<Staging> staging;   // returned to the free list when the tick is harvested
         std::vector<u64> finishing;
     };
     void harvest(bool wait_oldest);
-    // runs a transient plan in the server's work memory (grown when too small)
-    void run_plan(BatchPlan &p);
+    // runs a transient plan of `t` in the server's work memory (grown when too small)
+    void run_plan(Tick &t, BatchPlan &p);
 
     std::vector<Lane> lanes_;
     size_t budget_;
@@ -90,11 +96,13 @@ private:
     std::deque<Tick> inflight_;
     size_t reserved_ = 0;
     // work memory of the plans: ticks run one after another on one stream, so
-    // every plan uses the same arena and table buffer (stream-ordered)
+    // every plan uses the same arena and table buffer, the table copies ordered
+    // on the stream from pinned staging buffers that are recycled per tick
     u64 *arena_ = nullptr;
     size_t arena_bytes_ = 0;
     uint8_t *tables_ = nullptr;
     size_t tables_bytes_ = 0;
+    std::vector<Staging> staging_free_;
     ServerStats st_;
     std::vector<double> latencies_;
     double first_arrival_ = -1, last_done_ = 0;

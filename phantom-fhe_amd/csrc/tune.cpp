@@ -9,6 +9,11 @@
 //   ntt_fused_min_limbs  launches over fewer limbs than this use two passes
 //   ntt_chunk_limbs  two-pass launches over more limbs than this are split into
 //                slices of whole polynomials of about this many limbs (0 = never)
+//   ntt_cols_threads / ntt_rows_threads  tile shape of the two passes: threads per
+//                workgroup (16 registers each): columns 256 (default) or 512, rows
+//                128, 256 (default) or 512
+//   bconv_wgs_per_cu  FastBConv on the matrix cores: workgroups per CU of its grid
+//                (1, 2, 4; 0 = as many as the LDS holds, the default)
 // Environment defaults: PHX_BCONV_VALU=1 -> bconv_mfma=0, PHX_NTT_INT_ONLY=1 -> ntt_fp64=0.
 #include <atomic>
 #include <cstdlib>
@@ -28,6 +33,9 @@ std::atomic<int> g_ntt_fp64{env_is_one("PHX_NTT_INT_ONLY") ? 0 : 1};
 std::atomic<int> g_ntt_fused{0};
 std::atomic<int> g_ntt_fused_min{512};
 std::atomic<int> g_ntt_chunk{0};
+std::atomic<int> g_ntt_cols{256};
+std::atomic<int> g_ntt_rows{256};
+std::atomic<int> g_bconv_wgs{0};
 
 std::atomic<int> *slot(const char *key) {
     if (!key) return nullptr;
@@ -36,6 +44,9 @@ std::atomic<int> *slot(const char *key) {
     if (!std::strcmp(key, "ntt_fused")) return &g_ntt_fused;
     if (!std::strcmp(key, "ntt_fused_min_limbs")) return &g_ntt_fused_min;
     if (!std::strcmp(key, "ntt_chunk_limbs")) return &g_ntt_chunk;
+    if (!std::strcmp(key, "ntt_cols_threads")) return &g_ntt_cols;
+    if (!std::strcmp(key, "ntt_rows_threads")) return &g_ntt_rows;
+    if (!std::strcmp(key, "bconv_wgs_per_cu")) return &g_bconv_wgs;
     return nullptr;
 }
 }  // namespace

@@ -282,7 +282,7 @@ class context:
 
     def tuning(self):
         """kernel variants of this context (-1 = the process-wide switch)"""
-        c = (ctypes.c_int * 4)()
+        c = (ctypes.c_int * len(TUNE_KEYS))()
         _check(lib().phx_context_get_tuning(self._h, c))
         return dict(zip(TUNE_KEYS, list(c)))
 
@@ -292,7 +292,7 @@ class context:
             if k not in cur:
                 raise ValueError(f"unknown tuning key: {k}")
             cur[k] = int(v)
-        c = (ctypes.c_int * 4)(*[cur[k] for k in TUNE_KEYS])
+        c = (ctypes.c_int * len(TUNE_KEYS))(*[cur[k] for k in TUNE_KEYS])
         _check(lib().phx_context_set_tuning(self._h, c))
 
     def autotune(self, cache_path=None, stream=None):
@@ -301,8 +301,9 @@ class context:
         r = _TuneReport()
         _check(lib().phx_autotune(self._h, cache_path.encode() if cache_path else None, ctypes.byref(r),
                                   _stream(stream)))
+        import json
         return dict(choice=dict(zip(TUNE_KEYS, list(r.choice))), default_us=r.default_us, best_us=r.best_us,
-                    trials=r.trials)
+                    trials=r.trials, timings=json.loads(r.trials_json.decode() or "[]"))
 
     def tune_load(self, cache_path):
         found = ctypes.c_int()
@@ -961,23 +962,24 @@ class Event:
             self._h = None
 
 
-TUNE_KEYS = ("ntt_fused", "ntt_fused_min_limbs", "ntt_chunk_limbs", "bconv_mfma")
+TUNE_KEYS = ("ntt_fused", "ntt_fused_min_limbs", "ntt_chunk_limbs", "bconv_mfma", "ntt_cols_threads",
+             "ntt_rows_threads", "bconv_wgs_per_cu")
 
 
 class _TuneReport(ctypes.Structure):
-    _fields_ = [("choice", ctypes.c_int * 4), ("default_us", ctypes.c_double), ("best_us", ctypes.c_double),
-                ("trials", ctypes.c_uint32)]
+    _fields_ = [("choice", ctypes.c_int * len(TUNE_KEYS)), ("default_us", ctypes.c_double),
+                ("best_us", ctypes.c_double), ("trials", ctypes.c_uint32), ("trials_json", ctypes.c_char * 8192)]
 
 
 def tune_cache_store(path, scheme, N, size_Q, size_P, choice, us=0.0):
     """writes one autotuner cache entry (host only)"""
-    c = (ctypes.c_int * 4)(*[int(choice[k]) for k in TUNE_KEYS])
+    c = (ctypes.c_int * len(TUNE_KEYS))(*[int(choice.get(k, -1)) for k in TUNE_KEYS])
     _check(lib().phx_tune_cache_store(path.encode(), scheme, N, size_Q, size_P, c, float(us)))
 
 
 def tune_cache_lookup(path, scheme, N, size_Q, size_P):
     """the cached choice for a shape, or None (host only)"""
-    c = (ctypes.c_int * 4)()
+    c = (ctypes.c_int * len(TUNE_KEYS))()
     found = ctypes.c_int()
     _check(lib().phx_tune_cache_lookup(path.encode(), scheme, N, size_Q, size_P, c, ctypes.byref(found)))
     return dict(zip(TUNE_KEYS, list(c))) if found.value else None

@@ -68,8 +68,10 @@ def test_tune_cache_roundtrip(tmp_path):
     """the autotuner's JSON cache: store, look up, replace one entry, keep others"""
     path = str(tmp_path / "tune.json")
     assert ph.tune_cache_lookup(path, ph.SCHEME_CKKS, 65536, 35, 12) is None
-    a = dict(ntt_fused=0, ntt_fused_min_limbs=0, ntt_chunk_limbs=256, bconv_mfma=1)
-    b = dict(ntt_fused=2, ntt_fused_min_limbs=64, ntt_chunk_limbs=0, bconv_mfma=0)
+    a = dict(ntt_fused=0, ntt_fused_min_limbs=0, ntt_chunk_limbs=256, bconv_mfma=1, ntt_cols_threads=512,
+             ntt_rows_threads=128, bconv_wgs_per_cu=2)
+    b = dict(ntt_fused=2, ntt_fused_min_limbs=64, ntt_chunk_limbs=0, bconv_mfma=0, ntt_cols_threads=-1,
+             ntt_rows_threads=256, bconv_wgs_per_cu=0)
     ph.tune_cache_store(path, ph.SCHEME_CKKS, 65536, 35, 12, a, 340.5)
     ph.tune_cache_store(path, ph.SCHEME_BFV, 8192, 4, 1, b, 12.0)
     assert ph.tune_cache_lookup(path, ph.SCHEME_CKKS, 65536, 35, 12) == a
@@ -83,6 +85,12 @@ def test_tune_cache_roundtrip(tmp_path):
     doc = json.load(open(path))
     assert len(doc["entries"]) == 2
     assert {e["key"] for e in doc["entries"]} == {"scheme=ckks,N=65536,Q=35,P=12", "scheme=bfv,N=8192,Q=4,P=1"}
+    # an entry written before the tile-shape axes existed reads them as -1
+    with open(path, "w") as f:
+        f.write('{"entries": [\n  {"key": "scheme=ckks,N=4096,Q=2,P=1", "ntt_fused": 1, "ntt_fused_min_limbs": 0, '
+                '"ntt_chunk_limbs": 0, "bconv_mfma": 1, "hmult_relin_us": 10}\n]}\n')
+    old = ph.tune_cache_lookup(path, ph.SCHEME_CKKS, 4096, 2, 1)
+    assert old["ntt_fused"] == 1 and old["ntt_cols_threads"] == old["ntt_rows_threads"] == -1
     with open(path, "w") as f:
         f.write('{"entries": [\n  {"key": "scheme=ckks,N=4096,Q=2,P=1", "ntt_fused": 1}\n]}\n')
     with pytest.raises(ValueError):

@@ -9,7 +9,7 @@ import pytest
 
 import oracle_bind as ob
 import pyphantom as ph
-from gpu_helpers import make_ctx, rand_ct, rand_key, say
+from gpu_helpers import make_ctx, rand_ct, rand_key, say, upload_ct
 
 pytestmark = pytest.mark.gpu
 
@@ -36,14 +36,21 @@ def test_autotune_applies_and_caches(tmp_path):
     N = 1 << 14
     say("case: autotune N=2^14 L=9")
     ctx, octx, mods = make_ctx(N, BITS, 2)
-    assert ctx.tuning() == dict(ntt_fused=-1, ntt_fused_min_limbs=-1, ntt_chunk_limbs=-1, bconv_mfma=-1)
+    assert ctx.tuning() == {k: -1 for k in ph.TUNE_KEYS}
     path = str(tmp_path / "tune.json")
     rep = ctx.autotune(path)
     say(f"autotune: {rep}")
     ch = rep["choice"]
     assert ch["ntt_fused"] in (0, 1, 2, 4) and ch["bconv_mfma"] in (0, 1)
     assert ch["ntt_chunk_limbs"] >= 0 and ch["ntt_fused_min_limbs"] >= 0
-    assert rep["trials"] >= 8 and rep["best_us"] > 0 and rep["default_us"] > 0
+    assert ch["ntt_cols_threads"] in (256, 512) and ch["ntt_rows_threads"] in (128, 256, 512)
+    assert ch["bconv_wgs_per_cu"] in (-1, 0, 1, 2)
+    assert rep["trials"] >= 12 and rep["best_us"] > 0 and rep["default_us"] > 0
+    # the report lists every candidate with its timing, each axis among them
+    assert len(rep["timings"]) == rep["trials"]
+    for axis in ("ntt_rows_threads=128", "ntt_rows_threads=512", "ntt_cols_threads=512", "bconv_mfma=0",
+                 "bconv_wgs_per_cu=1", "ntt_fused=2", "ntt_chunk_limbs=128"):
+        assert any(axis in t["choice"] for t in rep["timings"]), axis
     # the winner is no slower than the default beyond timing noise
     assert rep["best_us"] <= 1.1 * rep["default_us"]
     assert ctx.tuning() == ch
@@ -65,21 +72,58 @@ def test_autotune_applies_and_caches(tmp_path):
     dict(ntt_fused=0, ntt_fused_min_limbs=0, ntt_chunk_limbs=128, bconv_mfma=1),
     dict(ntt_fused=1, ntt_fused_min_limbs=0, ntt_chunk_limbs=0, bconv_mfma=0),
     dict(ntt_fused=4, ntt_fused_min_limbs=0, ntt_chunk_limbs=0, bconv_mfma=1),
+    dict(ntt_fused=0, ntt_cols_threads=512, ntt_rows_threads=128, bconv_mfma=1, bconv_wgs_per_cu=1),
+    dict(ntt_fused=0, ntt_cols_threads=256, ntt_rows_threads=512, bconv_mfma=1, bconv_wgs_per_cu=2),
 ])
 def test_per_context_choice_bit_exact(choice):
-    """a context's own choice drives its launches (process-wide switches untouched)"""
+    """a context's own choice drives its launches (process-wide switches untouched),
+    plain and in a batch plan over two levels"""
     N = 1 << 14
     say(f"case: per-context tuning {choice}")
     ctx, octx, mods = make_ctx(N, BITS, 2)
     ctx.set_tuning(**choice)
-    before = {k: ph.tune_get(k) for k in ("ntt_fused", "ntt_chunk_limbs", "bconv_mfma")}
+    before = {k: ph.tune_get(k) for k in ph.TUNE_KEYS}
     check_mul_relin(ctx, octx, mods, 2)
     x = np.stack([np.random.default_rng(3).integers(0, q, N, dtype=np.uint64) for q in mods])
     buf = ph.DeviceBuffer.from_array(x)
     ph.ntt_forward(ctx, buf, len(mods), 0)
     ph.device_synchronize()
     assert np.array_equal(buf.to_array(x.shape), octx.ntt_forward(x, 0))
+    ph.ntt_inverse(ctx, buf, len(mods), 0)
+    ph.device_synchronize()
+    assert np.array_equal(buf.to_array(x.shape), x)
+    rng = np.random.default_rng(4)
+    key = rand_key(rng, mods, N, ctx.dnum)
+    rk = ph.relin_key(ctx, host=key)
+    items = [(1, ctx.limbs_at(1)), (3, ctx.limbs_at(3))]
+    data = [(c, L, rand_ct(rng, mods[:L], N), rand_ct(rng, mods[:L], N)) for c, L in items]
+    c1 = [upload_ct(ctx, a, c) for c, _, a, _ in data]
+    c2 = [upload_ct(ctx, b, c) for c, _, _, b in data]
+    bt = ph.batch.multiply_and_relin(ctx, c1, c2, rk)
+    bt.run()
+    ph.device_synchronize()
+    for i, (c, L, a, b) in enumerate(data):
+        assert np.array_equal(c1[i].download(N), octx.multiply_and_relin(L, a, b, key)), i
     assert {k: ph.tune_get(k) for k in before} == before
+
+
+@pytest.mark.parametrize("logN", [12, 13, 15, 16, 17])
+@pytest.mark.parametrize("cols,rows", [(256, 128), (512, 256), (512, 512)])
+def test_tile_shapes_every_degree(logN, cols, rows):
+    """every tile shape at every ring degree (shapes whose tile exceeds a limb fall
+    back to a smaller one): forward and inverse NTT bit-exact against the oracle"""
+    N = 1 << logN
+    say(f"case: NTT tile shape cols={cols} rows={rows} N=2^{logN}")
+    ctx, octx, mods = make_ctx(N, [60, 50, 60], 1)
+    ctx.set_tuning(ntt_cols_threads=cols, ntt_rows_threads=rows)
+    x = np.stack([np.random.default_rng(logN).integers(0, q, N, dtype=np.uint64) for q in mods])
+    buf = ph.DeviceBuffer.from_array(x)
+    ph.ntt_forward(ctx, buf, len(mods), 0)
+    ph.device_synchronize()
+    assert np.array_equal(buf.to_array(x.shape), octx.ntt_forward(x, 0))
+    ph.ntt_inverse(ctx, buf, len(mods), 0)
+    ph.device_synchronize()
+    assert np.array_equal(buf.to_array(x.shape), x)
 
 
 def test_set_tuning_rejects():
@@ -90,6 +134,12 @@ def test_set_tuning_rejects():
         ctx.set_tuning(bconv_mfma=2)
     with pytest.raises(ValueError):
         ctx.set_tuning(colour=1)
+    with pytest.raises(ValueError):
+        ctx.set_tuning(ntt_cols_threads=128)
+    with pytest.raises(ValueError):
+        ctx.set_tuning(ntt_rows_threads=1024)
+    with pytest.raises(ValueError):
+        ctx.set_tuning(bconv_wgs_per_cu=3)
 
 
 def test_autotune_bfv_leveled_context():
@@ -107,4 +157,4 @@ def test_autotune_bfv_leveled_context():
     ctx = ph.context(parms)
     rep = ctx.autotune()
     assert rep["trials"] >= 2 and rep["best_us"] > 0
-    assert set(ctx.tuning()) == {"ntt_fused", "ntt_fused_min_limbs", "ntt_chunk_limbs", "bconv_mfma"}
+    assert set(ctx.tuning()) == set(ph.TUNE_KEYS)
