@@ -16,28 +16,6 @@ inline dim3 grid3(u64 N, u32 y, u32 z) { return dim3((u32)((N + kThreads - 1) / 
 constexpr u32 kMaxGridY = 65535;
 }  // namespace
 
-// tensor of item z at limb y < its Ql: (d0, d1) into out, d2 into c2
-// (tensor_prod_2x2_rns_poly, src/polymath.cu:463-498). out may be a (in place).
-template <class PreItem>
-__global__ __launch_bounds__(kThreads) void k_tensor_items(const PreItem *items, const DevModulus *__restrict__ mod,
-                                                           u32 N) {
-    const PreItem it = items[blockIdx.z];
-    const u32 limb = blockIdx.y;
-    const u32 k = blockIdx.x * kThreads + threadIdx.x;
-    if (limb >= it.Ql || k >= N) return;
-    const u64 LN = (u64)it.Ql * N, o = (u64)limb * N + k;
-    const DevModulus m = mod[limb];
-    const u64 q = m.value;
-    const u64 a0 = it.a[o], a1 = it.a[LN + o], b0 = it.b[o], b1 = it.b[LN + o];
-    const u64 r0 = mul_mod(a0, b0, m);
-    const u64 r2 = mul_mod(a1, b1, m);
-    u64 r1 = mul_mod(add_mod(a0, a1, q), add_mod(b0, b1, q), m);
-    r1 = sub_mod(sub_mod(r1, r0, q), r2, q);
-    it.out[o] = r0;
-    it.out[LN + o] = r1;
-    it.c2[o] = r2;
-}
-
 // Galois permutation of item z (apply_galois_ntt_permutation, src/galois.cu:11-18):
 // base0 = pi(c0), c2 = pi(c1)
 template <class PreItem>
@@ -279,8 +257,15 @@ void BatchPlan::bind(u64 *arena) {
         u64 *dst = kind_ == kBfvMulRelin ? bfv_.c3 + i * 3 * LN : it.out;
         pre_.push_back(PreItem{it.a, it.b, it.out, c2, base0, it.perm, Ql, 0});
         // modup: INTT(c2) * partQlHatInv, FastBConv of every digit, NTT of the complements
-        for (u32 l = 0; l < Ql; ++l)
-            mu_inv_.push_back(NttJob{t_cks + l * N, c2 + l * N, nullptr, nullptr, L.modup_scale + 4 * l, l, 0});
+        // modup INTT: for a multiplication its first pass loads the tensor product
+        // (writing d0, d1 to the output and d2 to c2), else it reads c2
+        for (u32 l = 0; l < Ql; ++l) {
+            if (kind_ == kMulRelin)
+                mu_inv_.push_back(NttJob{t_cks + l * N, it.a + l * N, it.out + l * N, it.b + l * N,
+                                         L.modup_scale + 4 * l, l, Ql});
+            else
+                mu_inv_.push_back(NttJob{t_cks + l * N, c2 + l * N, nullptr, nullptr, L.modup_scale + 4 * l, l, 0});
+        }
         for (u32 j = 0; j < beta; ++j) {
             mu_bc_.push_back(BconvRow{L.modup_bconv + j, t_cks, t_mod_up + (u64)j * LPN});
             const u32 lo = alpha * j, hi = std::min(alpha * (j + 1), Ql);
@@ -489,7 +474,8 @@ size_t BatchPlan::launches() const {
     if (kind_ == kJoin) return n;
     auto ntt = [](size_t jobs) { return 2 * ((jobs + kMaxGridY - 1) / kMaxGridY); };
     if (kind_ == kRescale) return ntt(mu_inv_.size()) + ntt(md_fwd_.size());
-    n += 1 + ntt(mu_inv_.size()) + 1 + ntt(mu_fwd_.size()) + 1 + ntt(md_inv_.size()) + 1 + ntt(md_fwd_.size());
+    n += (kind_ == kRotate ? 1 : 0) + ntt(mu_inv_.size()) + 1 + ntt(mu_fwd_.size()) + 1 + ntt(md_inv_.size()) + 1 +
+         ntt(md_fwd_.size());
     return n;
 }
 
@@ -500,8 +486,11 @@ void BatchPlan::record_keyswitch(hipStream_t s) {
     const u64 N = ctx.N();
     const NttTables &T = ctx.tables();
     auto tab = [&](size_t off) { return tables_bound_ + off; };
-    // modup
-    ntt_inverse_jobs(T, reinterpret_cast<const NttJob *>(tab(off_mu_inv_)), (u32)mu_inv_.size(), s);
+    // modup (for a multiplication the tensor product is the INTT's first load)
+    if (kind_ == kMulRelin)
+        ntt_inverse_jobs_tensor(T, reinterpret_cast<const NttJob *>(tab(off_mu_inv_)), (u32)mu_inv_.size(), s);
+    else
+        ntt_inverse_jobs(T, reinterpret_cast<const NttJob *>(tab(off_mu_inv_)), (u32)mu_inv_.size(), s);
     require(mu_bc_.size() <= kMaxGridY && md_bc_.size() <= kMaxGridY, "too many base conversions in one batch");
     BconvArgs ba{};
     ba.rows = reinterpret_cast<const BconvRow *>(tab(off_mu_bc_));
@@ -565,12 +554,10 @@ void BatchPlan::record(hipStream_t s) {
         PHX_HIP_CHECK(hipGetLastError());
         return;
     }
-    if (kind_ == kMulRelin)
-        hipLaunchKernelGGL(k_tensor_items<PreItem>, grid3(N, max_Ql_, B), dim3(kThreads), 0, s, pre, ctx.dev_mod(),
-                           (u32)N);
-    else
+    if (kind_ == kRotate) {
         hipLaunchKernelGGL(k_permute_items<PreItem>, grid3(N, max_Ql_, B), dim3(kThreads), 0, s, pre, (u32)N);
-    PHX_HIP_CHECK(hipGetLastError());
+        PHX_HIP_CHECK(hipGetLastError());
+    }
     record_keyswitch(s);
 }
 

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "context.h"
+#include "ntt.h"
 
 namespace phx {
 
@@ -109,8 +110,9 @@ u32 galois_elt_from_step(int step, u64 N);
 // the same operations in a caller's workspace (device memory of *_ws_words u64
 // words at the ciphertext's chain index): nothing is allocated
 size_t keyswitch_ws_words(const Context &ctx, u32 chain);
+// tensor (optional): the modup INTT's first pass loads the tensor product and writes c2
 void keyswitch_ws(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey &key, u64 *ct,
-                  const u64 *base0, const u64 *base1, u64 *ws, hipStream_t s);
+                  const u64 *base0, const u64 *base1, u64 *ws, hipStream_t s, const LoadTensor *tensor = nullptr);
 size_t multiply_and_relin_ws_words(const Context &ctx, u32 chain);
 // CKKS only
 void multiply_and_relin_ws(const Context &ctx, Ciphertext &ct1, const Ciphertext &ct2, const KSwitchKey &rlk,
@@ -124,7 +126,7 @@ void multiply_and_relin_noalloc(const Context &ctx, Ciphertext &ct1, const Ciphe
 // copy_own = false: the digits' own limbs of t_mod_up are left unwritten
 void modup(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, hipStream_t s, bool copy_own = true);
 void modup_ws(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, u64 *t_cks, hipStream_t s,
-              bool copy_own);
+              bool copy_own, const LoadTensor *tensor = nullptr);
 // c2 given: the digits' own limbs are read from c2 (NTT form) instead of t_mod_up
 void inner_product(const Context &ctx, u32 chain, const u64 *t_mod_up, const KSwitchKey &key, u64 *cx,
                    hipStream_t s, const u64 *c2 = nullptr);

@@ -630,16 +630,20 @@ void tensor_2x2_mod(const DevModulus *mod, u64 N, u32 L, const u64 *a, const u64
 // the digits' own limbs of t_mod_up unwritten (the inner product then reads c2).
 // t_cks: Ql limbs of workspace for the scaled INTT of c2.
 void modup_ws(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, u64 *t_cks, hipStream_t s,
-              bool copy_own) {
+              bool copy_own, const LoadTensor *tensor) {
     const LevelConsts &L = ctx.level(chain);
     const u64 N = ctx.N();
     const u32 Ql = L.size_Ql, QlP = L.size_QlP, beta = L.beta, alpha = ctx.size_P();
     require(alpha <= (u32)kMaxDigitLimbs, "special_modulus_size above 16 is not supported");
-    // INTT(c2) * partQlHatInv (out of place)
+    // INTT(c2) * partQlHatInv (out of place); with `tensor`, c2 is the tensor
+    // product computed (and written) by the first pass's load
     LimbMap mi = limb_map(t_cks, 0, Ql);
     mi.src = c2;
     mi.scale = L.modup_scale;
-    ntt_inverse(ctx.tables(), mi, Ql, 1, s);
+    if (tensor)
+        ntt_inverse_tensor(ctx.tables(), mi, Ql, *tensor, s);
+    else
+        ntt_inverse(ctx.tables(), mi, Ql, 1, s);
     // FastBConv of every digit to its complement, own limbs copied
     bconv_launch(ctx, L.modup_bconv, L.h_modup_bconv, 1u, beta, t_cks, 0, t_mod_up, (u64)QlP * N,
                  copy_own ? c2 : nullptr, 0, s);
@@ -711,7 +715,7 @@ size_t keyswitch_ws_words(const Context &ctx, u32 chain) {
 
 // keyswitch_inplace (src/eval_key_switch.cu:95-182) in a caller's workspace
 void keyswitch_ws(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey &key, u64 *ct, const u64 *base0,
-                  const u64 *base1, u64 *ws, hipStream_t s) {
+                  const u64 *base1, u64 *ws, hipStream_t s, const LoadTensor *tensor) {
     const LevelConsts &L = ctx.level(chain);
     require(key.data != nullptr, "key-switching key is empty");
     require(key.dnum >= L.beta, "key-switching key has too few digits for this level");
@@ -720,7 +724,7 @@ void keyswitch_ws(const Context &ctx, u32 chain, const u64 *c2, const KSwitchKey
     u64 *cx = t_mod_up + (u64)L.beta * L.size_QlP * N;
     u64 *t_cks = cx + 2ull * L.size_QlP * N;
     u64 *delta = t_cks + (u64)L.size_Ql * N;
-    modup_ws(ctx, chain, c2, t_mod_up, t_cks, s, false);
+    modup_ws(ctx, chain, c2, t_mod_up, t_cks, s, false, tensor);
     inner_product(ctx, chain, t_mod_up, key, cx, s, c2);
     moddown_ws(ctx, chain, cx, ct, base0, base1, 2, delta, s);
 }
@@ -747,8 +751,9 @@ void multiply_and_relin_ws(const Context &ctx, Ciphertext &ct1, const Ciphertext
     const u32 Ql = ct1.coeff_modulus_size;
     const u64 N = ctx.N(), LN = (u64)Ql * N;
     u64 *c2 = ws;
-    tensor_2x2(ctx, Ql, ct1.data, ct2.data, ct1.data, ct1.data + LN, c2, s);
-    keyswitch_ws(ctx, ct1.chain_index, c2, rlk, ct1.data, ct1.data, ct1.data + LN, ws + LN, s);
+    // the tensor product is the first load of the modup INTT (d0, d1 into ct1, d2 into c2)
+    const LoadTensor tl{ct1.data, ct2.data, ct1.data, c2, LN, (u32)N};
+    keyswitch_ws(ctx, ct1.chain_index, c2, rlk, ct1.data, ct1.data, ct1.data + LN, ws + LN, s, &tl);
     ct1.scale = ct1.scale * ct2.scale;
 }
 

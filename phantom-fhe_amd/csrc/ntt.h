@@ -42,6 +42,28 @@ struct StoreModDown {
     }
 };
 
+// Tensor product as the load of an inverse first pass over one ciphertext pair
+// (the plain multiply's modup): writes d0, d1 to out, d2 to c2 and returns d2
+// (tensor_prod_2x2_rns_poly, src/polymath.cu:463-498); out may be a, b may be a
+struct LoadTensor {
+    const u64 *a, *b;
+    u64 *out, *c2;
+    u64 LN;
+    u32 N;
+    __device__ __forceinline__ u64 operator()(const u64 *, u32 i, u32 limb, u32, const DevModulus &m) const {
+        const u64 o = (u64)limb * N + i, q = m.value;
+        const u64 a0 = a[o], a1 = a[LN + o], b0 = b[o], b1 = b[LN + o];
+        const u64 r0 = mul_mod(a0, b0, m);
+        const u64 r2 = mul_mod(a1, b1, m);
+        u64 r1 = mul_mod(add_mod(a0, a1, q), add_mod(b0, b1, q), m);
+        r1 = sub_mod(sub_mod(r1, r0, q), r2, q);
+        out[o] = r0;
+        out[LN + o] = r1;
+        c2[o] = r2;
+        return r2;
+    }
+};
+
 // Rescale (CKKS, floor): the NTT input for limb j is [c_last]_{q_j} taken from
 // the coefficient-form last limb; the output is (c_j - NTT(.)) * q_last^-1.
 // (reference: DRNSTool::divide_and_round_q_last_ntt, src/rns.cu:1128-1184)
@@ -117,12 +139,46 @@ struct StoreRescaleJob {
     __device__ __forceinline__ Bound bind(const NttJob &J) const { return Bound{J.aux0, J.aux1, J.scale[0], J.scale[1]}; }
 };
 
+// Tensor product as the load of the modup INTT's first pass (tensor_prod_2x2_rns_poly,
+// src/polymath.cu:463-498): for limb l of an item, aux0 = a's limb l (c1 at + Ql N),
+// aux2 = b's limb l, aux1 = the output's limb l, flags = Ql. The load writes d0, d1 to
+// the output and d2 (NTT form) to data - Ql N (the item's c2, which the key inner
+// product reads), and returns d2 for the inverse transform. The output may be a
+// (in place) and b may be a (squaring): each coefficient is read before it is written.
+struct LoadTensorJob {
+    static constexpr bool kBind = true;
+    struct Bound {
+        const u64 *a, *b;
+        u64 *out, *c2;
+        u64 LN;
+        __device__ __forceinline__ u64 operator()(const u64 *, u32 i, u32, u32, const DevModulus &m) const {
+            const u64 q = m.value;
+            const u64 a0 = a[i], a1 = a[LN + i], b0 = b[i], b1 = b[LN + i];
+            const u64 r0 = mul_mod(a0, b0, m);
+            const u64 r2 = mul_mod(a1, b1, m);
+            u64 r1 = mul_mod(add_mod(a0, a1, q), add_mod(b0, b1, q), m);
+            r1 = sub_mod(sub_mod(r1, r0, q), r2, q);
+            out[i] = r0;
+            out[LN + i] = r1;
+            c2[i] = r2;
+            return r2;
+        }
+    };
+    u64 N;
+    __device__ __forceinline__ Bound bind(const NttJob &J) const {
+        const u64 LN = (u64)J.flags * N;
+        return Bound{J.aux0, J.aux2, J.aux1, J.data - LN, LN};
+    }
+};
+
 // work-list launches over njobs entries of a device table (two passes each)
 void ntt_forward_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
 void ntt_forward_jobs_moddown(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
 void ntt_forward_jobs_rescale(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
 // inverse: entries with aux0 read their first pass from aux0 (out of place)
 void ntt_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
+// inverse over entries whose first pass loads the tensor product (LoadTensorJob)
+void ntt_inverse_jobs_tensor(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
 
 void ntt_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s);
 void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s);
@@ -130,6 +186,8 @@ void ntt_forward_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npo
                        hipStream_t s);
 void ntt_forward_rescale(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const LoadRescale &ld,
                          const StoreRescale &st, hipStream_t s);
+// inverse of one polynomial whose first pass loads the tensor product (always two passes)
+void ntt_inverse_tensor(const NttTables &T, const LimbMap &M, u32 nlimbs, const LoadTensor &ld, hipStream_t s);
 
 // contiguous limbs [0, L) of one polynomial with rows first_row + i
 inline LimbMap limb_map(u64 *data, u32 first_row, u32 L, u64 poly_stride = 0) {

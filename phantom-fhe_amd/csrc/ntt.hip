@@ -154,12 +154,9 @@ void ntt_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, h
     launch_forward(T, M, nlimbs, npolys, LoadPlain{}, StorePlain{}, s);
 }
 
-void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s) {
-    if (nlimbs == 0 || npolys == 0) return;
-    if (const u32 G = fused_groups(T, nlimbs, npolys)) {
-        fused_inverse(G, T, M, nlimbs, npolys, s);
-        return;
-    }
+namespace {
+template <class Load>
+void launch_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const Load &ld, hipStream_t s) {
     const u32 ntc = cols_threads(T), ntr = rows_threads(T);
     // launch order and slicing as in launch_forward
     const u32 step = chunk_polys(T, nlimbs, npolys);
@@ -169,8 +166,8 @@ void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, h
         const u32 np = std::min(step, npolys - p0);
         dispatch_rows_nt(ntr, [&](auto nt) {
             constexpr int NT = decltype(nt)::value;
-            hipLaunchKernelGGL((k_inv_rows<NT, LoadPlain>), dim3(T.N / (NT * kNttRegs), np, nlimbs), dim3(NT), 0, s,
-                               T, Mc, LoadPlain{});
+            hipLaunchKernelGGL((k_inv_rows<NT, Load>), dim3(T.N / (NT * kNttRegs), np, nlimbs), dim3(NT), 0, s, T,
+                               Mc, ld);
         });
         dispatch_sr(T.logN, [&](auto sr) {
             constexpr int SR = decltype(sr)::value;
@@ -182,6 +179,21 @@ void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, h
         });
     }
     PHX_HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s) {
+    if (nlimbs == 0 || npolys == 0) return;
+    if (const u32 G = fused_groups(T, nlimbs, npolys)) {
+        fused_inverse(G, T, M, nlimbs, npolys, s);
+        return;
+    }
+    launch_inverse(T, M, nlimbs, npolys, LoadPlain{}, s);
+}
+
+void ntt_inverse_tensor(const NttTables &T, const LimbMap &M, u32 nlimbs, const LoadTensor &ld, hipStream_t s) {
+    if (nlimbs == 0) return;
+    launch_inverse(T, M, nlimbs, 1, ld, s);
 }
 
 void ntt_forward_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const StoreModDown &st,

@@ -37,19 +37,21 @@ __global__ __launch_bounds__(NT) void k_fwd_rows_jobs(NttTables T, const NttJob 
                           threadIdx.x);
 }
 
-template <int NT>
-__global__ __launch_bounds__(NT) void k_inv_rows_jobs(NttTables T, const NttJob *jobs) {
+template <int NT, class Load>
+__global__ __launch_bounds__(NT) void k_inv_rows_jobs(NttTables T, const NttJob *jobs, Load ld) {
     __shared__ u64 lds[(NT / 16) * 272];
     const NttJob J = jobs[blockIdx.y];
     const DevModulus md = T.mod[J.row];
-    const u64 *src = J.aux0 ? J.aux0 : J.data;
+    // a plain load reads aux0 (out of place) when it is set; a bound load ignores src
+    const u64 *src = !BindsJob<Load>::value && J.aux0 ? J.aux0 : J.data;
+    const auto lb = bind_job(ld, J);
     const u32 r0 = blockIdx.x * (NT / 16);
     if (md.pad)
-        inv_rows_body<NT>(fp_arith(T.itwd + (u64)J.row * T.N, md), lds, J.data, src, T.logN, r0, 0u, 0u, md,
-                          LoadPlain{}, threadIdx.x);
+        inv_rows_body<NT>(fp_arith(T.itwd + (u64)J.row * T.N, md), lds, J.data, src, T.logN, r0, 0u, 0u, md, lb,
+                          threadIdx.x);
     else
         inv_rows_body<NT>(int_arith(T.itw + (u64)J.row * T.N, md.value), lds, J.data, src, T.logN, r0, 0u, 0u, md,
-                          LoadPlain{}, threadIdx.x);
+                          lb, threadIdx.x);
 }
 
 template <int SR, int NT>
@@ -110,13 +112,16 @@ void ntt_forward_jobs_rescale(const NttTables &T, const NttJob *jobs, u32 njobs,
     launch_forward_jobs(T, jobs, njobs, LoadRescaleJob{}, StoreRescaleJob{}, s);
 }
 
-void ntt_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s) {
+namespace {
+template <class Load>
+void launch_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, const Load &ld, hipStream_t s) {
     const u32 ntc = cols_threads(T), ntr = rows_threads(T);
     for (u32 j0 = 0; j0 < njobs; j0 += kMaxJobsPerLaunch) {
         const u32 nj = std::min(kMaxJobsPerLaunch, njobs - j0);
         dispatch_rows_nt(ntr, [&](auto nt) {
             constexpr int NT = decltype(nt)::value;
-            hipLaunchKernelGGL((k_inv_rows_jobs<NT>), dim3(T.N / (NT * kNttRegs), nj), dim3(NT), 0, s, T, jobs + j0);
+            hipLaunchKernelGGL((k_inv_rows_jobs<NT, Load>), dim3(T.N / (NT * kNttRegs), nj), dim3(NT), 0, s, T,
+                               jobs + j0, ld);
         });
         dispatch_sr(T.logN, [&](auto sr) {
             constexpr int SR = decltype(sr)::value;
@@ -128,6 +133,15 @@ void ntt_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStre
         });
     }
     PHX_HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+void ntt_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s) {
+    launch_inverse_jobs(T, jobs, njobs, LoadPlain{}, s);
+}
+
+void ntt_inverse_jobs_tensor(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s) {
+    launch_inverse_jobs(T, jobs, njobs, LoadTensorJob{T.N}, s);
 }
 
 }  // namespace phx

@@ -130,7 +130,7 @@ def test_batch_n16_q35_bit_exact():
 def test_batch_one_launch_per_stage_across_levels(small):
     """a plan over items of four levels (with a squaring) issues one launch per
     pipeline stage, the same count as a one-level plan, and stays bit-exact; so
-    does a rotation plan over two levels"""
+    does a rotation plan over two levels (one launch more: the permutation)"""
     ctx, octx, mods, key = small
     N = ctx.N
     say("case: batch over four levels, one launch per stage")
@@ -144,7 +144,8 @@ def test_batch_one_launch_per_stage_across_levels(small):
     c2[2] = c1[2]
     bt = ph.batch.multiply_and_relin(ctx, c1, c2, rk)
     one = ph.batch.multiply_and_relin(ctx, [upload_ct(ctx, data[0][2], 1)], [upload_ct(ctx, data[0][3], 1)], rk)
-    assert bt.launches() == one.launches() == 12
+    # 11 launches: the tensor product is the first load of the modup INTT
+    assert bt.launches() == one.launches() == 11
     bt.run()
     ph.device_synchronize()
     for i, (c, Ql, a, b) in enumerate(data):
