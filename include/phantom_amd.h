@@ -255,7 +255,12 @@ int phx_keyswitch_inplace(const phx_context *ctx, phx_ciphertext *ct, const uint
  * while the batch lives) to preallocated scratch and device work lists; every stage
  * of the pipeline is one launch over all items, whatever their levels. run() launches eagerly,
  * capture() records the launches into a hipGraph on an explicit (non-NULL) stream and
- * replay() launches that graph. Results are bit-identical to the one-at-a-time calls. */
+ * replay() launches that graph. Results are bit-identical to the one-at-a-time calls.
+ * A batch of at least 2 x B items (B = the "batch_branches" switch, default 2) is split
+ * into B runs of items of about equal work that run concurrently: branch 0 on the
+ * caller's stream, the others on batch-owned streams forked from it and joined back
+ * into it, so run(), capture() and replay() still order everything on the caller's
+ * stream. */
 typedef struct phx_batch phx_batch;
 /* ct1[i] = relin(ct1[i] * ct2[i]); ct1[i] may equal ct2[i] (squaring).
  * CKKS: NTT form, any level. BFV: coefficient form at the top level, multiplied with
@@ -267,8 +272,10 @@ int phx_batch_multiply_and_relin_create(const phx_context *ctx, phx_ciphertext *
 /* cts[i] = rotate(cts[i], steps[i]); each step's Galois element must have a key in gk */
 int phx_batch_rotate_create(const phx_context *ctx, phx_ciphertext *const *cts, const int *steps, size_t n,
                             const phx_galois_key *gk, phx_batch **out);
-/* one batch that runs `parts` in order (one graph when captured) with one scratch arena
- * of the largest part's size; parts may belong to different contexts (ring degrees).
+/* one batch that runs `parts` (one graph when captured); parts may belong to different
+ * contexts (ring degrees), joined batches are flattened into their parts. The parts are
+ * spread over B concurrent branches (largest first onto the least loaded branch); the
+ * parts of one branch run in order in one scratch arena of the largest one's size.
  * Takes ownership: the parts' handles are released and must not be used again; on
  * error they are left untouched. */
 int phx_batch_join(phx_batch *const *parts, size_t n, phx_batch **out);
@@ -280,6 +287,8 @@ int phx_batch_scratch_bytes(const phx_batch *batch, size_t *bytes);
 /* kernel launches one run issues (the kernel nodes of the captured graph): one per
  * pipeline stage across all items and levels of the batch */
 int phx_batch_launches(const phx_batch *batch, size_t *launches);
+/* concurrent branches of the batch (1 when it runs on the caller's stream alone) */
+int phx_batch_branches(const phx_batch *batch, size_t *branches);
 int phx_batch_destroy(phx_batch *batch);
 
 /* ---- serving loop (no single reference counterpart: a request loop around the

@@ -20,9 +20,16 @@
 //
 // BFV plans (multiply + relinearise at the top level) stage both operands of
 // every item contiguously, run each multiply stage once over all items, key-
-// switch them as above and write the results back. A joined plan runs plans of
-// different contexts (ring degrees) one after another in one graph, sharing one
-// scratch arena.
+// switch them as above and write the results back.
+//
+// Concurrent branches: a joined plan runs its parts (plans of any contexts /
+// ring degrees) on up to `batch_branches` branches: branch 0 on the caller's
+// stream, every other branch on a plan-owned stream forked from it by an event
+// and joined back into it by another, so one graph holds parallel chains of
+// launches whose ramp-up and tail overlap. Parts of one branch run one after
+// another in one arena of the largest part's size. A persistent plan of at least
+// 2 x batch_branches items is built as such a joined plan of contiguous runs of
+// items of about equal work, each with its own work lists.
 #pragma once
 
 #include <memory>
@@ -43,7 +50,8 @@ public:
     static BatchPlan *mul_relin(const Context &ctx, const std::vector<Ciphertext *> &ct1,
                                 const std::vector<const Ciphertext *> &ct2, const KSwitchKey &rlk,
                                 bool transient = false);
-    // one plan running `parts` (persistent, any contexts) in order; takes ownership
+    // one plan running `parts` (persistent, any contexts; joined plans are
+    // flattened into their parts) on concurrent branches; takes ownership
     static BatchPlan *join(const std::vector<BatchPlan *> &parts);
     // cts[i] <- rotate(cts[i], steps[i]); every step's Galois element must have a key in gk
     static BatchPlan *rotate(const Context &ctx, const std::vector<Ciphertext *> &cts, const std::vector<int> &steps,
@@ -70,6 +78,8 @@ public:
     size_t scratch_bytes() const { return scratch_bytes_; }
     // kernel launches one run issues (the graph's kernel nodes)
     size_t launches() const;
+    // concurrent branches (1 unless joined)
+    size_t branches() const { return kind_ == kJoin ? nbranch_ : 1; }
 
 private:
     struct Item {
@@ -87,6 +97,11 @@ private:
     };
 
     BatchPlan(const Context &ctx, Kind k, bool transient) : ctx_(ctx), kind_(k), transient_(transient) {}
+    // a plan with its items set: laid out, or split into concurrent parts
+    static BatchPlan *finish(std::unique_ptr<BatchPlan> p);
+    // the joined plan of `parts` (persistent, unjoined) on nb branches; it owns
+    // them when it returns, and nothing has changed when it throws
+    static BatchPlan *join_parts(const std::vector<BatchPlan *> &parts, u32 nb);
     size_t item_words(const Item &it) const;
     void layout();                           // scratch offsets, arena size; persistent: allocate and bind
     void bind(u64 *arena);                   // build the host work lists against an arena
@@ -138,6 +153,14 @@ private:
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
     std::vector<std::unique_ptr<BatchPlan>> parts_;   // joined plans
+    // joined plans: part i runs on branch branch_[i]; branch 0 on the caller's
+    // stream, branch b > 0 on aux_[b - 1]
+    std::vector<u32> branch_;
+    u32 nbranch_ = 1;
+    std::vector<u64 *> branch_arenas_;   // shared arenas of branches with several parts
+    std::vector<hipStream_t> aux_;
+    hipEvent_t fork_ = nullptr;
+    std::vector<hipEvent_t> joins_;
 };
 
 }  // namespace phx

@@ -3,10 +3,12 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <numeric>
 
 #include "batch.h"
 #include "ntt_arith.h"
 #include "bfv.h"
+#include "tune.h"
 
 namespace phx {
 
@@ -151,11 +153,15 @@ void sort_by_row(std::vector<NttJob> &v) {
 BatchPlan::~BatchPlan() {
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
-    if (arena_ || tables_) {
+    if (arena_ || tables_ || !branch_arenas_.empty() || !aux_.empty()) {
         (void)hipDeviceSynchronize();
         (void)hipFree(arena_);
         (void)hipFree(tables_);
+        for (u64 *a : branch_arenas_) (void)hipFree(a);
     }
+    for (hipStream_t st : aux_) (void)hipStreamDestroy(st);
+    for (hipEvent_t e : joins_) (void)hipEventDestroy(e);
+    if (fork_) (void)hipEventDestroy(fork_);
 }
 
 size_t BatchPlan::item_scratch_bytes(const Context &ctx, u32 chain, Kind k) {
@@ -372,38 +378,157 @@ BatchPlan *BatchPlan::mul_relin(const Context &ctx, const std::vector<Ciphertext
         // BFV: squaring flag (the same ciphertext as both operands)
         if (bfv) p->bfv_square_.push_back(ct1[i]->data == ct2[i]->data ? 1 : 0);
     }
-    p->layout();
-    return p.release();
+    return finish(std::move(p));
+}
+
+namespace {
+u32 branches_wanted() { return (u32)std::clamp(tune_value("batch_branches"), 1, 8); }
+}  // namespace
+
+BatchPlan *BatchPlan::finish(std::unique_ptr<BatchPlan> p) {
+    const size_t n = p->items_.size();
+    const u32 nb = p->transient_ ? 1u : (u32)std::min<size_t>(branches_wanted(), n / 2);
+    if (nb <= 1) {
+        p->layout();
+        return p.release();
+    }
+    // contiguous runs of items of about equal work (scratch words as its measure):
+    // an item joins the run when its midpoint lies below the run's share of the
+    // total; every run keeps at least one item
+    std::vector<size_t> w(n);
+    for (size_t i = 0; i < n; ++i) w[i] = p->item_words(p->items_[i]);
+    const double total = (double)std::accumulate(w.begin(), w.end(), size_t{0});
+    std::vector<size_t> cut(nb + 1, 0);
+    cut[nb] = n;
+    double acc = 0;
+    size_t i = 0;
+    for (u32 b = 1; b < nb; ++b) {
+        const double target = total * b / nb;
+        while (i < n - (nb - b) && (i == cut[b - 1] || acc + 0.5 * (double)w[i] <= target)) acc += (double)w[i++];
+        cut[b] = i;
+    }
+    std::vector<std::unique_ptr<BatchPlan>> parts;
+    for (u32 b = 0; b < nb; ++b) {
+        std::unique_ptr<BatchPlan> q(new BatchPlan(p->ctx_, p->kind_, false));
+        const size_t lo = cut[b], hi = cut[b + 1];
+        auto slice = [&](const auto &from, auto &to) {
+            if (!from.empty()) to.assign(from.begin() + lo, from.begin() + hi);
+        };
+        slice(p->items_, q->items_);
+        slice(p->cts_, q->cts_);
+        slice(p->ins_, q->ins_);
+        slice(p->scale_mul_, q->scale_mul_);
+        slice(p->bfv_square_, q->bfv_square_);
+        q->layout();
+        parts.push_back(std::move(q));
+    }
+    std::vector<BatchPlan *> raw;
+    for (auto &q : parts) raw.push_back(q.get());
+    BatchPlan *joined = join_parts(raw, nb);
+    for (auto &q : parts) (void)q.release();   // now owned by the joined plan
+    return joined;
 }
 
 BatchPlan *BatchPlan::join(const std::vector<BatchPlan *> &parts) {
     require(!parts.empty(), "nothing to join");
     for (size_t i = 0; i < parts.size(); ++i) {
         require(parts[i] != nullptr, "batch is null");
-        require(!parts[i]->transient_ && parts[i]->kind_ != kJoin, "only persistent, unjoined batches can be joined");
+        require(!parts[i]->transient_, "only persistent batches can be joined");
         for (size_t j = 0; j < i; ++j) require(parts[j] != parts[i], "a batch appears twice in a join");
     }
-    std::unique_ptr<BatchPlan> p(new BatchPlan(parts[0]->ctx_, kJoin, false));
-    size_t bytes = 0;
+    // joined plans are flattened into their parts
+    std::vector<BatchPlan *> flat;
     for (auto *q : parts) {
-        bytes = std::max(bytes, q->scratch_bytes_);
-        p->joined_items_ += q->size();
+        if (q->kind_ != kJoin) flat.push_back(q);
+        else
+            for (auto &x : q->parts_) flat.push_back(x.get());
     }
-    // the parts run one after another on one stream: one arena of the largest size
-    PHX_HIP_CHECK(hipDeviceSynchronize());
-    PHX_HIP_CHECK(hipMalloc(&p->arena_, std::max<size_t>(bytes, 256)));
-    p->scratch_bytes_ = bytes;
+    BatchPlan *joined = join_parts(flat, (u32)std::min<size_t>(branches_wanted(), flat.size()));
+    // the joined plan owns every part now; flattened plans are emptied and freed
     for (auto *q : parts) {
-        PHX_HIP_CHECK(hipFree(q->arena_));
-        q->arena_ = nullptr;
-        q->bind(p->arena_);
-        q->upload_tables(nullptr, nullptr);
+        if (q->kind_ != kJoin) continue;
+        for (auto &x : q->parts_) (void)x.release();
+        q->parts_.clear();
+        delete q;
+    }
+    return joined;
+}
+
+BatchPlan *BatchPlan::join_parts(const std::vector<BatchPlan *> &parts, u32 nb) {
+    std::unique_ptr<BatchPlan> p(new BatchPlan(parts[0]->ctx_, kJoin, false));
+    nb = std::max<u32>(1, std::min<u32>(nb, (u32)parts.size()));
+    // the largest part first onto the least loaded branch (scratch bytes as the
+    // measure of work)
+    std::vector<size_t> order(parts.size());
+    std::iota(order.begin(), order.end(), size_t{0});
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return parts[x]->scratch_bytes_ > parts[y]->scratch_bytes_; });
+    std::vector<size_t> load(nb, 0);
+    std::vector<u32> br(parts.size(), 0);
+    for (size_t i : order) {
+        const u32 b = (u32)(std::min_element(load.begin(), load.end()) - load.begin());
+        br[i] = b;
+        load[b] += parts[i]->scratch_bytes_;
+    }
+    // a branch of one part keeps that part's own arena; the parts of a branch of
+    // several share one arena of the largest part's size. Every allocation comes
+    // first, so a failure leaves the parts as they were.
+    std::vector<std::vector<BatchPlan *>> members(nb);
+    for (size_t i = 0; i < parts.size(); ++i) members[br[i]].push_back(parts[i]);
+    std::vector<u64 *> fresh(nb, nullptr);
+    size_t total = 0;
+    try {
+        for (u32 b = 0; b < nb; ++b) {
+            size_t bytes = 0;
+            for (auto *q : members[b]) bytes = std::max(bytes, q->scratch_bytes_);
+            total += bytes;
+            if (members[b].size() == 1 && members[b][0]->arena_) continue;
+            PHX_HIP_CHECK(hipMalloc(&fresh[b], std::max<size_t>(bytes, 256)));
+        }
+        if (nb > 1) {
+            PHX_HIP_CHECK(hipEventCreateWithFlags(&p->fork_, hipEventDisableTiming));
+            for (u32 b = 1; b < nb; ++b) {
+                hipStream_t st = nullptr;
+                PHX_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+                p->aux_.push_back(st);
+                hipEvent_t e = nullptr;
+                PHX_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                p->joins_.push_back(e);
+            }
+        }
+    } catch (...) {
+        for (u64 *a : fresh) (void)hipFree(a);
+        throw;
+    }
+    PHX_HIP_CHECK(hipDeviceSynchronize());
+    for (u32 b = 0; b < nb; ++b) {
+        if (!fresh[b]) continue;
+        if (members[b].size() == 1) {
+            members[b][0]->arena_ = fresh[b];
+        } else {
+            p->branch_arenas_.push_back(fresh[b]);
+            for (auto *q : members[b]) {
+                (void)hipFree(q->arena_);
+                q->arena_ = nullptr;
+            }
+        }
+        for (auto *q : members[b]) {
+            q->bind(fresh[b]);
+            q->upload_tables(nullptr, nullptr);
+        }
+    }
+    for (size_t i = 0; i < parts.size(); ++i) {
+        BatchPlan *q = parts[i];
         if (q->exec_) (void)hipGraphExecDestroy(q->exec_);
         if (q->graph_) (void)hipGraphDestroy(q->graph_);
         q->exec_ = nullptr;
         q->graph_ = nullptr;
+        p->joined_items_ += q->size();
+        p->branch_.push_back(br[i]);
         p->parts_.emplace_back(q);
     }
+    p->nbranch_ = nb;
+    p->scratch_bytes_ = total;
     return p.release();
 }
 
@@ -436,8 +561,7 @@ BatchPlan *BatchPlan::rotate(const Context &ctx, const std::vector<Ciphertext *>
         p->items_.push_back(it);
         p->cts_.push_back(cts[i]);
     }
-    p->layout();
-    return p.release();
+    return finish(std::move(p));
 }
 
 BatchPlan *BatchPlan::rescale(const Context &ctx, const std::vector<const Ciphertext *> &ins,
@@ -464,8 +588,7 @@ BatchPlan *BatchPlan::rescale(const Context &ctx, const std::vector<const Cipher
         p->ins_.push_back(in);
         p->cts_.push_back(out);
     }
-    p->layout();
-    return p.release();
+    return finish(std::move(p));
 }
 
 size_t BatchPlan::launches() const {
@@ -518,8 +641,22 @@ void BatchPlan::record_keyswitch(hipStream_t s) {
 }
 
 void BatchPlan::record(hipStream_t s) {
-    for (const auto &q : parts_) q->record(s);
-    if (kind_ == kJoin) return;
+    if (kind_ == kJoin) {
+        if (nbranch_ <= 1) {
+            for (const auto &q : parts_) q->record(s);
+            return;
+        }
+        // fork: every branch stream waits for the work queued on s so far
+        PHX_HIP_CHECK(hipEventRecord(fork_, s));
+        for (hipStream_t st : aux_) PHX_HIP_CHECK(hipStreamWaitEvent(st, fork_, 0));
+        for (size_t i = 0; i < parts_.size(); ++i) parts_[i]->record(branch_[i] == 0 ? s : aux_[branch_[i] - 1]);
+        // join: s waits for every branch
+        for (size_t b = 0; b < aux_.size(); ++b) {
+            PHX_HIP_CHECK(hipEventRecord(joins_[b], aux_[b]));
+            PHX_HIP_CHECK(hipStreamWaitEvent(s, joins_[b], 0));
+        }
+        return;
+    }
     const Context &ctx = ctx_;
     const u64 N = ctx.N();
     const NttTables &T = ctx.tables();

@@ -1134,6 +1134,14 @@ int phx_batch_launches(const phx_batch *b, size_t *launches) {
     });
 }
 
+int phx_batch_branches(const phx_batch *b, size_t *branches) {
+    return guard([&] {
+        nn(b, "batch");
+        nn(branches, "branches");
+        *branches = b->plan->branches();
+    });
+}
+
 int phx_batch_destroy(phx_batch *b) {
     return guard([&] {
         if (!b) return;

@@ -84,8 +84,8 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
         fwd_round<1, 8>(v, baseC, logN, a);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            st_stream(data + baseC[b], A::to_bits(v[2 * b]));
-            st_stream(data + baseC[b] + 256, A::to_bits(v[2 * b + 1]));
+            st_mid(data + baseC[b], A::to_bits(v[2 * b]));
+            st_mid(data + baseC[b] + 256, A::to_bits(v[2 * b + 1]));
         }
         return;
     } else {
@@ -135,13 +135,13 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
         for (int b = 0; b < BB; ++b)
 #pragma unroll
             for (int e = 0; e < (1 << NB); ++e)
-                st_stream(data + baseB[b] + ((u32)e << 8), A::to_bits(v[b * (1 << NB) + e]));
+                st_mid(data + baseB[b] + ((u32)e << 8), A::to_bits(v[b * (1 << NB) + e]));
     } else {
 #pragma unroll
         for (int b = 0; b < BA; ++b)
 #pragma unroll
             for (int e = 0; e < (1 << NA); ++e)
-                st_stream(data + baseA[b] + ((u32)e << 8), A::to_bits(v[b * (1 << NA) + e]));
+                st_mid(data + baseA[b] + ((u32)e << 8), A::to_bits(v[b * (1 << NA) + e]));
     }
     }
 }
@@ -157,7 +157,7 @@ __device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *d
     T v[16];
     u32 baseA[1] = {(r0 + u) * 256 + l};
 #pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = A::from_bits(ld_stream(data + baseA[0] + ((u32)e << 4)));
+    for (int e = 0; e < 16; ++e) v[e] = A::from_bits(ld_mid(data + baseA[0] + ((u32)e << 4)));
     fwd_round<4, 4>(v, baseA, logN, a);
 #pragma unroll
     for (int e = 0; e < 16; ++e) lds[row_addr(u, ((u32)e << 4) + l)] = A::to_bits(v[e]);
@@ -214,7 +214,7 @@ __device__ __forceinline__ void inv_rows_body(const A &a, u64 *lds, u64 *data, c
     for (int e = 0; e < 16; ++e) v[e] = A::from_bits(lds[row_addr(u, ((u32)e << 4) + l)]);
     inv_round<4, 4>(v, baseA, logN, a);
 #pragma unroll
-    for (int e = 0; e < 16; ++e) st_stream(data + baseA[0] + ((u32)e << 4), A::to_bits(v[e]));
+    for (int e = 0; e < 16; ++e) st_mid(data + baseA[0] + ((u32)e << 4), A::to_bits(v[e]));
 }
 
 // Inverse pass 2 body: the remaining SR stages (distances 256 .. N/2) on
@@ -237,8 +237,8 @@ __device__ __forceinline__ void inv_cols_body(const A &a, u64 *lds, u64 *data, u
         for (int b = 0; b < 8; ++b) {
             const u32 r2 = 2 * ((u32)b * 32 + rest);
             baseC[b] = r2 * 256 + c0 + uc;
-            v[2 * b] = A::from_bits(ld_stream(data + baseC[b]));
-            v[2 * b + 1] = A::from_bits(ld_stream(data + baseC[b] + 256));
+            v[2 * b] = A::from_bits(ld_mid(data + baseC[b]));
+            v[2 * b + 1] = A::from_bits(ld_mid(data + baseC[b] + 256));
         }
         inv_round<1, 8>(v, baseC, logN, a);
 #pragma unroll
@@ -291,7 +291,7 @@ __device__ __forceinline__ void inv_cols_body(const A &a, u64 *lds, u64 *data, u
         for (int b = 0; b < BB; ++b)
 #pragma unroll
             for (int e = 0; e < (1 << NB); ++e)
-                v[b * (1 << NB) + e] = A::from_bits(ld_stream(data + baseB[b] + ((u32)e << 8)));
+                v[b * (1 << NB) + e] = A::from_bits(ld_mid(data + baseB[b] + ((u32)e << 8)));
         inv_round<NB, 8>(v, baseB, logN, a);
 #pragma unroll
         for (int b = 0; b < BB; ++b)
@@ -309,7 +309,7 @@ __device__ __forceinline__ void inv_cols_body(const A &a, u64 *lds, u64 *data, u
         for (int b = 0; b < BA; ++b)
 #pragma unroll
             for (int e = 0; e < (1 << NA); ++e)
-                v[b * (1 << NA) + e] = A::from_bits(ld_stream(data + baseA[b] + ((u32)e << 8)));
+                v[b * (1 << NA) + e] = A::from_bits(ld_mid(data + baseA[b] + ((u32)e << 8)));
     }
     inv_round_last<NA>(v, baseA, 8 + NB, logN, a, s0, s0s, s1, s1s);
 #pragma unroll

@@ -14,6 +14,10 @@
 //                128, 256 (default) or 512
 //   bconv_wgs_per_cu  FastBConv on the matrix cores: workgroups per CU of its grid
 //                (1, 2, 4; 0 = as many as the LDS holds, the default)
+//   batch_branches  concurrent branches of a persistent batch plan (1..8, default
+//                2): plans of at least twice as many items are split into that many
+//                parts that run on forked streams, and joined plans spread their
+//                parts over them; 1 runs everything on the caller's stream
 // Environment defaults: PHX_BCONV_VALU=1 -> bconv_mfma=0, PHX_NTT_INT_ONLY=1 -> ntt_fp64=0.
 #include <atomic>
 #include <cstdlib>
@@ -36,6 +40,7 @@ std::atomic<int> g_ntt_chunk{0};
 std::atomic<int> g_ntt_cols{256};
 std::atomic<int> g_ntt_rows{256};
 std::atomic<int> g_bconv_wgs{0};
+std::atomic<int> g_batch_branches{2};
 
 std::atomic<int> *slot(const char *key) {
     if (!key) return nullptr;
@@ -47,6 +52,7 @@ std::atomic<int> *slot(const char *key) {
     if (!std::strcmp(key, "ntt_cols_threads")) return &g_ntt_cols;
     if (!std::strcmp(key, "ntt_rows_threads")) return &g_ntt_rows;
     if (!std::strcmp(key, "bconv_wgs_per_cu")) return &g_bconv_wgs;
+    if (!std::strcmp(key, "batch_branches")) return &g_batch_branches;
     return nullptr;
 }
 }  // namespace

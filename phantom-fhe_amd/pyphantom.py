@@ -123,6 +123,7 @@ _SIGS = {
     "phx_multiply_and_relin_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "phx_multiply_and_relin_inplace_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "phx_batch_launches": (ctypes.c_int, [_vp, ctypes.POINTER(_sz)]),
+    "phx_batch_branches": (ctypes.c_int, [_vp, ctypes.POINTER(_sz)]),
     "phx_ciphertext_clone": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
     "phx_multiply_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "phx_relinearize_inplace": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
@@ -832,7 +833,8 @@ class batch:
 
     @classmethod
     def join(cls, parts):
-        """one batch running `parts` (any contexts) in order; the parts are consumed"""
+        """one batch running `parts` (any contexts; joined batches are flattened) on
+        concurrent branches; the parts are consumed"""
         arr = (_vp * len(parts))(*[p._h for p in parts])
         h = _vp()
         _check(lib().phx_batch_join(arr, len(parts), ctypes.byref(h)))
@@ -855,6 +857,12 @@ class batch:
         """kernel launches of one run (kernel nodes of the captured graph)"""
         n = _sz()
         _check(lib().phx_batch_launches(self._h, ctypes.byref(n)))
+        return n.value
+
+    def branches(self):
+        """concurrent branches the batch runs on (1: the caller's stream alone)"""
+        n = _sz()
+        _check(lib().phx_batch_branches(self._h, ctypes.byref(n)))
         return n.value
 
     def scratch_bytes(self):

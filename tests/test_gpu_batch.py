@@ -144,8 +144,10 @@ def test_batch_one_launch_per_stage_across_levels(small):
     c2[2] = c1[2]
     bt = ph.batch.multiply_and_relin(ctx, c1, c2, rk)
     one = ph.batch.multiply_and_relin(ctx, [upload_ct(ctx, data[0][2], 1)], [upload_ct(ctx, data[0][3], 1)], rk)
-    # 11 launches: the tensor product is the first load of the modup INTT
-    assert bt.launches() == one.launches() == 11
+    # 11 launches per branch: the tensor product is the first load of the modup INTT;
+    # four items run as two concurrent branches of two
+    assert one.branches() == 1 and one.launches() == 11
+    assert bt.branches() == 2 and bt.launches() == 11 * 2
     bt.run()
     ph.device_synchronize()
     for i, (c, Ql, a, b) in enumerate(data):
@@ -156,7 +158,7 @@ def test_batch_one_launch_per_stage_across_levels(small):
     x, y = rand_ct(rng, mods[:5], N), rand_ct(rng, mods[:2], N)
     cx = [upload_ct(ctx, x, 3), upload_ct(ctx, y, 6)]
     bt = ph.batch.rotate(ctx, cx, [2, 2], gk)
-    assert bt.launches() == 12
+    assert bt.branches() == 1 and bt.launches() == 12
     bt.run()
     ph.device_synchronize()
     assert np.array_equal(cx[0].download(N), octx.apply_galois(5, x, elt, key))
@@ -192,7 +194,8 @@ def test_128_ciphertexts_batched_rotations_bit_exact():
         cts.append(upload_ct(ctx, a, chains[i]))
     stream = ph.stream_create()
     bt = ph.batch.rotate(ctx, cts, [ROTATION_STEPS[k] for k in steps_idx], gk)
-    assert bt.launches() == 12      # one launch per stage over all 31 levels
+    # one launch per stage and branch over all 31 levels
+    assert bt.branches() == 2 and bt.launches() == 12 * 2
     bt.capture(stream)
     bt.replay(stream)
     ph.synchronize(stream)
@@ -224,10 +227,13 @@ def test_joined_batch_two_ring_degrees():
         c2 = [upload_ct(ctx, b, c) for c, _, _, b in data]
         parts.append(ph.batch.multiply_and_relin(ctx, c1, c2, rk))
         checks.append((ctx, octx, key, data, c1))
-    single = max(p.scratch_bytes() for p in parts)
+    # three items per ring degree: not split; the two parts run as two branches,
+    # each in its own arena
+    assert [p.branches() for p in parts] == [1, 1]
+    both = sum(p.scratch_bytes() for p in parts)
     jb = ph.batch.join(parts)
     assert all(p._h is None for p in parts)
-    assert jb.scratch_bytes() == single
+    assert jb.branches() == 2 and jb.scratch_bytes() == both
     stream = ph.stream_create()
 
     def verify():
@@ -263,3 +269,76 @@ def test_join_rejects_repeated_part(small):
         ph.batch.join([bt, bt])
     bt.run()   # untouched by the failed join
     ph.device_synchronize()
+
+
+@pytest.fixture
+def branches_switch():
+    old = ph.tune_get("batch_branches")
+    yield lambda v: ph.tune_set("batch_branches", v)
+    ph.tune_set("batch_branches", old)
+
+
+def test_batch_branches_split_and_join_bit_exact(small, branches_switch):
+    """the same items as one branch, split into 2 and 3 concurrent branches of
+    about equal work (mixed levels, a squaring), and two split batches joined
+    (flattened: four parts over two branches sharing arenas): every item
+    bit-exact against the oracle on eager runs and on graph replays with eager
+    work between them"""
+    ctx, octx, mods, key = small
+    N = ctx.N
+    rng = np.random.default_rng(31)
+    rk = ph.relin_key(ctx, host=key)
+    chains = [1, 7, 1, 3, 2, 1, 5, 1]
+    data = [(c, ctx.limbs_at(c), rand_ct(rng, mods[:ctx.limbs_at(c)], N),
+             rand_ct(rng, mods[:ctx.limbs_at(c)], N)) for c in chains]
+    want = [octx.multiply_and_relin(Ql, a, a if i == 3 else b, key) for i, (c, Ql, a, b) in enumerate(data)]
+    stream = ph.stream_create()
+
+    def operands(lo, hi):
+        c1 = [upload_ct(ctx, data[i][2], data[i][0]) for i in range(lo, hi)]
+        c2 = [c1[i - lo] if i == 3 else upload_ct(ctx, data[i][3], data[i][0]) for i in range(lo, hi)]
+        return c1, c2
+
+    def check(c1, lo, tag):
+        for j, c in enumerate(c1):
+            assert np.array_equal(c.download(N), want[lo + j]), (tag, lo + j)
+
+    def reset(c1, lo):
+        for j, c in enumerate(c1):
+            c.upload(data[lo + j][2])
+
+    for nb in (1, 2, 3):
+        branches_switch(nb)
+        say(f"case: batch branches={nb}")
+        c1, c2 = operands(0, len(data))
+        bt = ph.batch.multiply_and_relin(ctx, c1, c2, rk)
+        assert bt.branches() == nb and bt.launches() == 11 * nb
+        bt.run(stream)
+        ph.synchronize(stream)
+        check(c1, 0, ("eager", nb))
+        bt.capture(stream)
+        for rep in range(2):
+            reset(c1, 0)
+            ph.device_synchronize()
+            bt.replay(stream)
+            ph.synchronize(stream)
+            check(c1, 0, ("replay", nb, rep))
+    branches_switch(2)
+    say("case: join of two split batches")
+    a1, a2 = operands(0, 4)
+    b1, b2 = operands(4, 8)
+    p = ph.batch.multiply_and_relin(ctx, a1, a2, rk)
+    q = ph.batch.multiply_and_relin(ctx, b1, b2, rk)
+    assert p.branches() == q.branches() == 2
+    jb = ph.batch.join([p, q])
+    assert p._h is None and q._h is None
+    assert jb.branches() == 2 and jb.launches() == 4 * 11
+    jb.capture(stream)
+    for rep in range(2):
+        reset(a1, 0)
+        reset(b1, 4)
+        ph.device_synchronize()
+        jb.replay(stream)
+        ph.synchronize(stream)
+        check(a1, 0, ("join", rep))
+        check(b1, 4, ("join", rep))
