@@ -365,7 +365,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--ntt-polys", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -491,7 +491,8 @@ def main():
         "config": {"workload": "CKKS HMult+relin, N=65536, 35 data limbs (60+34x50 bit) + 12 special (60 bit), "
                                "dnum=3; batched path: one batch plan over all pairs of a step, replayed as a "
                                "captured hipGraph",
-                   "batch_per_step": args.batch, "mode": "batched", "tune": tune},
+                   "batch_per_step": args.batch, "batch_branches": ph.tune_get("batch_branches"),
+                   "mode": "batched", "tune": tune},
         "autotune": autotune,
         "plain_value": round(plain, 2),
         "plain_ms_per_step": round(ms_plain / args.steps, 4),
