@@ -27,9 +27,10 @@ __global__ __launch_bounds__(kThreads) void k_permute_items(const PreItem *items
     const u32 k = blockIdx.x * kThreads + threadIdx.x;
     if (limb >= it.Ql || k >= N) return;
     const u64 LN = (u64)it.Ql * N, o = (u64)limb * N;
-    const u32 pk = it.perm[k];
-    it.base0[o + k] = it.a[o + pk];
-    it.c2[o + k] = it.a[LN + o + pk];
+    const u32 pk = gmem(it.perm)[k];
+    const auto a = gmem(it.a);
+    gmem(it.base0)[o + k] = a[o + pk];
+    gmem(it.c2)[o + k] = a[LN + o + pk];
 }
 
 // key inner product over the items of a plan (key_switch_inner_prod_c2_and_evk,
@@ -63,15 +64,15 @@ __global__ __launch_bounds__(kThreads) void k_inner_product_items(const IpItem *
 #pragma unroll
             for (u32 j = 0; j < MB; ++j)
                 if (j < it.beta) {
-                    const u64 *p = it.key + (u64)j * 2 * QPN + (u64)r * N + k;
+                    const auto p = gmem(it.key + (u64)j * 2 * QPN + (u64)r * N + k);
                     kb[j] = p[0];
                     ka[j] = p[QPN];
                 }
         }
-        const u64 *ti = it.t + (u64)pos * N + k;
+        const auto ti = gmem(it.t + (u64)pos * N + k);
         const u64 LPN = (u64)it.QlP * N;
-        const u64 xo = own != ~0u ? it.c2[(u64)pos * N + k] : 0;
-        u64 *co = it.cx + (u64)pos * N + k;
+        const u64 xo = own != ~0u ? gmem(it.c2)[(u64)pos * N + k] : 0;
+        const auto co = gmem(it.cx + (u64)pos * N + k);
         if (fp) {
             // q < 2^50: exact products on the FP64 pipe (FpArith::mulmod, |r| < q), the
             // partial sums reduced every 4 digits (|sum| < 4.5 q < 2^53), then canonical
@@ -109,7 +110,7 @@ template <class PreItem>
 __global__ __launch_bounds__(kThreads) void k_gather_items(const PreItem *items, int which, u64 *__restrict__ dst,
                                                            u64 words, u64 dst_stride) {
     const PreItem it = items[blockIdx.y];
-    const u64 *a = which ? it.b : it.a;
+    const auto a = gmem(which ? it.b : it.a);
     u64 *d = dst + blockIdx.y * dst_stride;
     for (u64 i = (u64)blockIdx.x * kThreads + threadIdx.x; i < words; i += (u64)gridDim.x * kThreads) d[i] = a[i];
 }
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_items(const PreItem *items,
 template <class PreItem>
 __global__ __launch_bounds__(kThreads) void k_scatter_items(const PreItem *items, const u64 *__restrict__ src,
                                                             u64 words, u64 src_stride) {
-    u64 *d = items[blockIdx.y].out;
+    const auto d = gmem(items[blockIdx.y].out);
     const u64 *a = src + blockIdx.y * src_stride;
     for (u64 i = (u64)blockIdx.x * kThreads + threadIdx.x; i < words; i += (u64)gridDim.x * kThreads) d[i] = a[i];
 }

@@ -62,6 +62,17 @@ struct alignas(16) TwPair {
     u64 ws;
 };
 
+// A device buffer seen through the global address space. Pointers that kernels
+// read from device work lists are generic to the compiler, which then emits flat
+// memory instructions; a flat access also counts against the LDS wait counter, so
+// every LDS wait in a kernel would wait for the flat loads and stores in flight as
+// well. Every pointer the kernels dereference this way is device memory.
+#define PHX_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ PHX_GLOBAL T *gmem(T *p) {
+    return (PHX_GLOBAL T *)p;
+}
+
 __device__ __forceinline__ u64 umulhi(u64 a, u64 b) { return __umul64hi(a, b); }
 
 // conditional subtract: x in [0, 2m) -> [0, m)

@@ -56,8 +56,8 @@ __global__ __launch_bounds__(kThreads) void k_bconv(BconvArgs A) {
     const u32 N = A.N;
     const u32 k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= N) return;
-    const u64 *inj = inb + (u64)d.in_first * N + k;
-    u64 *outj = outb + k;
+    const auto inj = gmem(inb + (u64)d.in_first * N + k);
+    const auto outj = gmem(outb + k);
     const u64 *mat = A.cbuf + d.mat_off;
     const u32 *pos = reinterpret_cast<const u32 *>(A.cbuf) + d.pos_off;
     const u32 *rows = reinterpret_cast<const u32 *>(A.cbuf) + d.row_off;
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kThreads) void k_bconv(BconvArgs A) {
         outj[(u64)pos[o] * N] = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
     }
     if (A.copy_src) {
-        const u64 *cs = A.copy_src + bt * A.copy_batch_stride + j * A.copy_job_stride + (u64)d.in_first * N + k;
+        const auto cs = gmem(A.copy_src + bt * A.copy_batch_stride + j * A.copy_job_stride + (u64)d.in_first * N + k);
         for (u32 i = 0; i < d.ni; ++i) outj[(u64)(d.own_first + i) * N] = cs[(u64)i * N];
     }
 }
@@ -137,8 +137,8 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
     }
     __syncthreads();
 
-    const u64 *inj = inb + (u64)d.in_first * N;
-    u64 *outj = outb;
+    const auto inj = gmem(inb + (u64)d.in_first * N);
+    const auto outj = gmem(outb);
     const u32 wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
     const u32 ntiles = N / kBconvTileCols;
     const u32 groups = rt / 3;
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
         }
     }
     if (A.copy_src) {
-        const u64 *cs = A.copy_src + bt * A.copy_batch_stride + j * A.copy_job_stride + (u64)d.in_first * N;
+        const auto cs = gmem(A.copy_src + bt * A.copy_batch_stride + j * A.copy_job_stride + (u64)d.in_first * N);
         const u64 total = (u64)ni * N;
         for (u64 idx = (u64)blockIdx.x * kBconvThreads + tid; idx < total; idx += (u64)gridDim.x * kBconvThreads) {
             const u64 i = idx / N, kk = idx % N;

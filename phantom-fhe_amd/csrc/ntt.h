@@ -35,10 +35,10 @@ struct StoreModDown {
                                                const DevModulus &m) const {
         const u64 q = m.value;
         const u64 off = (u64)limb * N + i;
-        const u64 d = sub_mod(cx[poly * cx_stride + off], v, q);
+        const u64 d = sub_mod(gmem(cx)[poly * cx_stride + off], v, q);
         const u64 r = mul_shoup(d, pinv[2 * limb], pinv[2 * limb + 1], q);
         const u64 *base = poly == 0 ? base0 : base1;
-        ct[poly * ct_stride + off] = base ? add_mod(base[off], r, q) : r;
+        gmem(ct)[poly * ct_stride + off] = base ? add_mod(gmem(base)[off], r, q) : r;
     }
 };
 
@@ -52,14 +52,15 @@ struct LoadTensor {
     u32 N;
     __device__ __forceinline__ u64 operator()(const u64 *, u32 i, u32 limb, u32, const DevModulus &m) const {
         const u64 o = (u64)limb * N + i, q = m.value;
-        const u64 a0 = a[o], a1 = a[LN + o], b0 = b[o], b1 = b[LN + o];
+        const auto ga = gmem(a), gb = gmem(b);
+        const u64 a0 = ga[o], a1 = ga[LN + o], b0 = gb[o], b1 = gb[LN + o];
         const u64 r0 = mul_mod(a0, b0, m);
         const u64 r2 = mul_mod(a1, b1, m);
         u64 r1 = mul_mod(add_mod(a0, a1, q), add_mod(b0, b1, q), m);
         r1 = sub_mod(sub_mod(r1, r0, q), r2, q);
-        out[o] = r0;
-        out[LN + o] = r1;
-        c2[o] = r2;
+        gmem(out)[o] = r0;
+        gmem(out)[LN + o] = r1;
+        gmem(c2)[o] = r2;
         return r2;
     }
 };
@@ -71,7 +72,7 @@ struct LoadRescale {
     const u64 *last;     // [poly][N] coefficient form, canonical mod q_last
     u64 last_stride;
     __device__ __forceinline__ u64 operator()(const u64 *, u32 i, u32, u32 poly, const DevModulus &m) const {
-        return barrett64(last[poly * last_stride + i], m.value, m.ratio_hi);
+        return barrett64(gmem(last)[poly * last_stride + i], m.value, m.ratio_hi);
     }
 };
 struct StoreRescale {
@@ -84,8 +85,8 @@ struct StoreRescale {
                                                const DevModulus &m) const {
         const u64 q = m.value;
         const u64 off = (u64)limb * N + i;
-        const u64 d = sub_mod(in[poly * in_stride + off], v, q);
-        out[poly * out_stride + off] = mul_shoup(d, qinv[2 * limb], qinv[2 * limb + 1], q);
+        const u64 d = sub_mod(gmem(in)[poly * in_stride + off], v, q);
+        gmem(out)[poly * out_stride + off] = mul_shoup(d, qinv[2 * limb], qinv[2 * limb + 1], q);
     }
 };
 
@@ -103,13 +104,13 @@ struct StoreModDownJob {
         u32 add_out;
         __device__ __forceinline__ void operator()(u64 *, u32 i, u64 v, u32, u32, const DevModulus &m) const {
             const u64 q = m.value;
-            const u64 r = mul_shoup(sub_mod(cx[i], v, q), p, ps, q);
-            const u64 b = base ? base[i] : (add_out ? out[i] : 0);
-            out[i] = add_mod(b, r, q);
+            const u64 r = mul_shoup(sub_mod(gmem(cx)[i], v, q), p, ps, q);
+            const u64 b = base ? gmem(base)[i] : (add_out ? gmem(out)[i] : 0);
+            gmem(out)[i] = add_mod(b, r, q);
         }
     };
     __device__ __forceinline__ Bound bind(const NttJob &J) const {
-        return Bound{J.aux0, J.aux1, J.aux2, J.scale[0], J.scale[1], J.flags & 1u};
+        return Bound{J.aux0, J.aux1, J.aux2, gmem(J.scale)[0], gmem(J.scale)[1], J.flags & 1u};
     }
 };
 // Rescale: the NTT input is [c_last]_q from aux2 (the coefficient-form last limb of
@@ -120,7 +121,7 @@ struct LoadRescaleJob {
     struct Bound {
         const u64 *last;
         __device__ __forceinline__ u64 operator()(const u64 *, u32 i, u32, u32, const DevModulus &m) const {
-            return barrett64(last[i], m.value, m.ratio_hi);
+            return barrett64(gmem(last)[i], m.value, m.ratio_hi);
         }
     };
     __device__ __forceinline__ Bound bind(const NttJob &J) const { return Bound{J.aux2}; }
@@ -133,10 +134,12 @@ struct StoreRescaleJob {
         u64 p, ps;
         __device__ __forceinline__ void operator()(u64 *, u32 i, u64 v, u32, u32, const DevModulus &m) const {
             const u64 q = m.value;
-            out[i] = mul_shoup(sub_mod(in[i], v, q), p, ps, q);
+            gmem(out)[i] = mul_shoup(sub_mod(gmem(in)[i], v, q), p, ps, q);
         }
     };
-    __device__ __forceinline__ Bound bind(const NttJob &J) const { return Bound{J.aux0, J.aux1, J.scale[0], J.scale[1]}; }
+    __device__ __forceinline__ Bound bind(const NttJob &J) const {
+        return Bound{J.aux0, J.aux1, gmem(J.scale)[0], gmem(J.scale)[1]};
+    }
 };
 
 // Tensor product as the load of the modup INTT's first pass (tensor_prod_2x2_rns_poly,
@@ -153,14 +156,15 @@ struct LoadTensorJob {
         u64 LN;
         __device__ __forceinline__ u64 operator()(const u64 *, u32 i, u32, u32, const DevModulus &m) const {
             const u64 q = m.value;
-            const u64 a0 = a[i], a1 = a[LN + i], b0 = b[i], b1 = b[LN + i];
+            const auto ga = gmem(a), gb = gmem(b);
+            const u64 a0 = ga[i], a1 = ga[LN + i], b0 = gb[i], b1 = gb[LN + i];
             const u64 r0 = mul_mod(a0, b0, m);
             const u64 r2 = mul_mod(a1, b1, m);
             u64 r1 = mul_mod(add_mod(a0, a1, q), add_mod(b0, b1, q), m);
             r1 = sub_mod(sub_mod(r1, r0, q), r2, q);
-            out[i] = r0;
-            out[LN + i] = r1;
-            c2[i] = r2;
+            gmem(out)[i] = r0;
+            gmem(out)[LN + i] = r1;
+            gmem(c2)[i] = r2;
             return r2;
         }
     };

@@ -26,7 +26,7 @@ __device__ __forceinline__ u32 row_addr(u32 u, u32 j) { return u * 272 + j + (j 
 
 __device__ __forceinline__ void limb_scale(const NttTables &T, const LimbMap &M, u32 limb, u32 row,
                                            u64 &s0, u64 &s0s, u64 &s1, u64 &s1s) {
-    const u64 *c = M.scale ? M.scale + 4 * (u64)limb : T.inv_consts + 4 * (u64)row;
+    const auto c = gmem(M.scale ? M.scale + 4 * (u64)limb : T.inv_consts + 4 * (u64)row);
     s0 = c[0];
     s0s = c[1];
     s1 = c[2];

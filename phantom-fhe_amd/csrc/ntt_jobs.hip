@@ -61,7 +61,7 @@ __global__ __launch_bounds__(NT) void k_inv_cols_jobs(NttTables T, const NttJob 
     __shared__ u64 lds[R * (TC + 1)];
     const NttJob J = jobs[blockIdx.y];
     const DevModulus md = T.mod[J.row];
-    const u64 *c = J.scale ? J.scale : T.inv_consts + 4 * (u64)J.row;
+    const auto c = gmem(J.scale ? J.scale : T.inv_consts + 4 * (u64)J.row);
     const u64 s0 = c[0], s0s = c[1], s1 = c[2], s1s = c[3];
     const u32 c0 = blockIdx.x * TC;
     if (md.pad)

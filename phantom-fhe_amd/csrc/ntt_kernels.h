@@ -29,20 +29,13 @@ namespace phx {
 // Every NTT pass streams its tiles through HBM once (the working set of a
 // launch is far above the L2), so data and intermediate accesses are
 // nontemporal: they do not displace the twiddle tables from L2.
-__device__ __forceinline__ u64 ld_stream(const u64 *p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ void st_stream(u64 *p, u64 v) { __builtin_nontemporal_store(v, p); }
+// (An A/B with cached intermediates, so that a launch slice's intermediate could
+// stay in the last-level cache, measured no difference.)
+__device__ __forceinline__ u64 ld_stream(const u64 *p) { return __builtin_nontemporal_load(gmem(p)); }
+__device__ __forceinline__ void st_stream(u64 *p, u64 v) { __builtin_nontemporal_store(v, gmem(p)); }
 // the intermediate between the two passes
-#ifndef PHX_NTT_MID_TEMPORAL
-#define PHX_NTT_MID_TEMPORAL 0
-#endif
-__device__ __forceinline__ u64 ld_mid(const u64 *p) {
-    if constexpr (PHX_NTT_MID_TEMPORAL) return *p;
-    else return __builtin_nontemporal_load(p);
-}
-__device__ __forceinline__ void st_mid(u64 *p, u64 v) {
-    if constexpr (PHX_NTT_MID_TEMPORAL) *p = v;
-    else __builtin_nontemporal_store(v, p);
-}
+__device__ __forceinline__ u64 ld_mid(const u64 *p) { return ld_stream(p); }
+__device__ __forceinline__ void st_mid(u64 *p, u64 v) { st_stream(p, v); }
 
 constexpr int kNttThreads = 256;
 constexpr int kNttRegs = 16;
