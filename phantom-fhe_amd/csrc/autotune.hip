@@ -167,7 +167,7 @@ TuneResult autotune(Context &ctx, hipStream_t s) {
         }
         const u64 tiles = N / kNttTile;
         for (int g : {1, 2, 4}) {
-            if (tiles % (u64)g) continue;
+            if (tiles < (u64)g || tiles % (u64)g) continue;
             for (int min_limbs : {0, 64, 256}) {
                 TuneChoice cand;
                 cand.ntt_fused = g;

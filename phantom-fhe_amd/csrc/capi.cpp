@@ -159,7 +159,7 @@ int phx_coeff_modulus_create(uint64_t N, const int *bits, size_t n, uint64_t *ou
     return guard([&] {
         nn(bits, "bit_sizes");
         nn(out, "out");
-        require(N >= 4096 && N <= 131072 && (N & (N - 1)) == 0, "poly_modulus_degree is invalid");
+        require(N >= 2048 && N <= 131072 && (N & (N - 1)) == 0, "poly_modulus_degree is invalid");
         auto v = hm::coeff_modulus_create(N, std::vector<int>(bits, bits + n));
         std::memcpy(out, v.data(), n * 8);
     });

@@ -300,8 +300,8 @@ Context::Context(const Params &p) : params_(p) {
     require(p.special_modulus_size >= 1 && p.special_modulus_size < QP, "invalid special_modulus_size");
     N_ = p.poly_degree;
     logN_ = (u32)hm::bit_count(N_) - 1;
-    require(N_ == ((u64)1 << logN_) && logN_ >= 12 && logN_ <= 17,
-            "poly_modulus_degree must be a power of two in [2^12, 2^17]");
+    require(N_ == ((u64)1 << logN_) && logN_ >= 11 && logN_ <= 17,
+            "poly_modulus_degree must be a power of two in [2^11, 2^17]");
     size_QP_ = (u32)QP;
     size_P_ = (u32)p.special_modulus_size;
     size_Q_ = size_QP_ - size_P_;

@@ -16,7 +16,8 @@ namespace phx {
 
 namespace {
 constexpr int kThreads = 256;
-constexpr int kMaxDigitLimbs = 16;   // FastBConv input limbs per digit (alpha)
+constexpr int kMaxDigitLimbs = 16;   // FastBConv input limbs held in registers
+constexpr int kMaxBconvInputs = 63;  // FastBConv input limbs a 128-bit sum takes
 
 inline dim3 grid_for(u64 N, u32 y, u32 z = 1) { return dim3((u32)((N + kThreads - 1) / kThreads), y, z); }
 }  // namespace
@@ -62,17 +63,30 @@ __global__ __launch_bounds__(kThreads) void k_bconv(BconvArgs A) {
     const u32 *pos = reinterpret_cast<const u32 *>(A.cbuf) + d.pos_off;
     const u32 *rows = reinterpret_cast<const u32 *>(A.cbuf) + d.row_off;
     const DevModulus *mod = A.mod;
-    u64 x[kMaxDigitLimbs];
+    if (d.ni <= (u32)kMaxDigitLimbs) {
+        u64 x[kMaxDigitLimbs];
 #pragma unroll
-    for (int i = 0; i < kMaxDigitLimbs; ++i) x[i] = (u32)i < d.ni ? inj[(u64)i * N] : 0;
-    for (u32 o = 0; o < d.no; ++o) {
-        const u64 *mo = mat + (u64)o * d.ni;
-        u128 acc = 0;
+        for (int i = 0; i < kMaxDigitLimbs; ++i) x[i] = (u32)i < d.ni ? inj[(u64)i * N] : 0;
+        for (u32 o = 0; o < d.no; ++o) {
+            const u64 *mo = mat + (u64)o * d.ni;
+            u128 acc = 0;
 #pragma unroll
-        for (int i = 0; i < kMaxDigitLimbs; ++i)
-            if ((u32)i < d.ni) acc += (u128)x[i] * mo[i];
-        const DevModulus m = mod[rows[o]];
-        outj[(u64)pos[o] * N] = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
+            for (int i = 0; i < kMaxDigitLimbs; ++i)
+                if ((u32)i < d.ni) acc += (u128)x[i] * mo[i];
+            const DevModulus m = mod[rows[o]];
+            outj[(u64)pos[o] * N] = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
+        }
+    } else {
+        // more input limbs than registers (special_modulus_size above 16): every
+        // output reads its inputs again, from the cache; below 64 inputs of 61 bits
+        // the 128-bit sum cannot overflow
+        for (u32 o = 0; o < d.no; ++o) {
+            const u64 *mo = mat + (u64)o * d.ni;
+            u128 acc = 0;
+            for (u32 i = 0; i < d.ni; ++i) acc += (u128)inj[(u64)i * N] * mo[i];
+            const DevModulus m = mod[rows[o]];
+            outj[(u64)pos[o] * N] = barrett128(acc, m.value, m.ratio_lo, m.ratio_hi);
+        }
     }
     if (A.copy_src) {
         const auto cs = gmem(A.copy_src + bt * A.copy_batch_stride + j * A.copy_job_stride + (u64)d.in_first * N + k);
@@ -634,7 +648,7 @@ void modup_ws(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, u64 *
     const LevelConsts &L = ctx.level(chain);
     const u64 N = ctx.N();
     const u32 Ql = L.size_Ql, QlP = L.size_QlP, beta = L.beta, alpha = ctx.size_P();
-    require(alpha <= (u32)kMaxDigitLimbs, "special_modulus_size above 16 is not supported");
+    require(alpha <= (u32)kMaxBconvInputs, "special_modulus_size above 63 is not supported");
     // INTT(c2) * partQlHatInv (out of place); with `tensor`, c2 is the tensor
     // product computed (and written) by the first pass's load
     LimbMap mi = limb_map(t_cks, 0, Ql);

@@ -108,7 +108,7 @@ u32 fused_groups(const NttTables &T, u32 nlimbs, u32 npolys) {
     const int g = T.fused_g >= 0 ? T.fused_g : tune_value("ntt_fused");
     if (g != 1 && g != 2 && g != 4) return 0;
     const u32 tiles = T.N / kNttTile;
-    if (tiles % (u32)g) return 0;
+    if (tiles < (u32)g || tiles % (u32)g) return 0;   // N = 2^11 has no 4096-coefficient tile
     const int min_limbs = T.fused_min >= 0 ? T.fused_min : tune_value("ntt_fused_min_limbs");
     if ((u64)nlimbs * npolys < (u64)std::max(0, min_limbs)) return 0;
     return (u32)g;
@@ -132,13 +132,11 @@ void launch_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys
         LimbMap Mc = M;
         Mc.poly0 = M.poly0 + p0;
         const u32 np = std::min(step, npolys - p0);
-        dispatch_sr(T.logN, [&](auto sr) {
+        dispatch_cols(T.logN, ntc, [&](auto sr, auto nt) {
             constexpr int SR = decltype(sr)::value;
-            dispatch_cols_nt(ntc, [&](auto nt) {
-                constexpr int NT = decltype(nt)::value;
-                hipLaunchKernelGGL((k_fwd_cols<SR, NT, Load>), dim3(T.N / (NT * kNttRegs), np, nlimbs), dim3(NT), 0,
-                                   s, T, Mc, ld);
-            });
+            constexpr int NT = decltype(nt)::value;
+            hipLaunchKernelGGL((k_fwd_cols<SR, NT, Load>), dim3(T.N / (NT * kNttRegs), np, nlimbs), dim3(NT), 0,
+                               s, T, Mc, ld);
         });
         dispatch_rows_nt(ntr, [&](auto nt) {
             constexpr int NT = decltype(nt)::value;
@@ -169,13 +167,11 @@ void launch_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys
             hipLaunchKernelGGL((k_inv_rows<NT, Load>), dim3(T.N / (NT * kNttRegs), np, nlimbs), dim3(NT), 0, s, T,
                                Mc, ld);
         });
-        dispatch_sr(T.logN, [&](auto sr) {
+        dispatch_cols(T.logN, ntc, [&](auto sr, auto nt) {
             constexpr int SR = decltype(sr)::value;
-            dispatch_cols_nt(ntc, [&](auto nt) {
-                constexpr int NT = decltype(nt)::value;
-                hipLaunchKernelGGL((k_inv_cols<SR, NT, StorePlain>), dim3(T.N / (NT * kNttRegs), np, nlimbs),
-                                   dim3(NT), 0, s, T, Mc, StorePlain{});
-            });
+            constexpr int NT = decltype(nt)::value;
+            hipLaunchKernelGGL((k_inv_cols<SR, NT, StorePlain>), dim3(T.N / (NT * kNttRegs), np, nlimbs),
+                               dim3(NT), 0, s, T, Mc, StorePlain{});
         });
     }
     PHX_HIP_CHECK(hipGetLastError());

@@ -330,7 +330,7 @@ void dispatch_sr(u32 logN, F &&f) {
         case 15: f(std::integral_constant<int, 7>{}); break;
         case 16: f(std::integral_constant<int, 8>{}); break;
         case 17: f(std::integral_constant<int, 9>{}); break;
-        default: fail(kInvalidArgument, "unsupported polynomial degree for the NTT (2^12 .. 2^17)");
+        default: fail(kInvalidArgument, "unsupported polynomial degree for the NTT (2^11 .. 2^17)");
     }
 }
 
@@ -359,16 +359,25 @@ inline u32 pick_threads(int ctx_choice, const char *key, u32 N) {
 }
 inline u32 cols_threads(const NttTables &T) {
     const u32 nt = pick_threads(T.cols_nt, "ntt_cols_threads", T.N);
-    return nt == 128 ? 256 : nt;   // column passes run on 256 or 512 threads
+    // column passes run on 256 or 512 threads, on 128 only where N = 2^11 needs it
+    return nt == 128 && 256 * kNttRegs <= T.N ? 256 : nt;
 }
 inline u32 rows_threads(const NttTables &T) { return pick_threads(T.rows_nt, "ntt_rows_threads", T.N); }
 
+// column passes: (SR, NT) for log2 N and the tile's threads. N = 2^11 (8 rows of
+// 256) runs one 128-thread tile per limb: 16 registers x 128 threads = the limb
 template <class F>
-void dispatch_cols_nt(u32 nt, F &&f) {
-    if (nt == 512)
-        f(std::integral_constant<int, 512>{});
-    else
-        f(std::integral_constant<int, 256>{});
+void dispatch_cols(u32 logN, u32 nt, F &&f) {
+    if (logN == 11) {
+        f(std::integral_constant<int, 3>{}, std::integral_constant<int, 128>{});
+        return;
+    }
+    dispatch_sr(logN, [&](auto sr) {
+        if (nt == 512)
+            f(sr, std::integral_constant<int, 512>{});
+        else
+            f(sr, std::integral_constant<int, 256>{});
+    });
 }
 template <class F>
 void dispatch_rows_nt(u32 nt, F &&f) {

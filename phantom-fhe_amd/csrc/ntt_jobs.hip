@@ -82,13 +82,11 @@ void launch_forward_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, cons
     const u32 ntc = cols_threads(T), ntr = rows_threads(T);
     for (u32 j0 = 0; j0 < njobs; j0 += kMaxJobsPerLaunch) {
         const u32 nj = std::min(kMaxJobsPerLaunch, njobs - j0);
-        dispatch_sr(T.logN, [&](auto sr) {
+        dispatch_cols(T.logN, ntc, [&](auto sr, auto nt) {
             constexpr int SR = decltype(sr)::value;
-            dispatch_cols_nt(ntc, [&](auto nt) {
-                constexpr int NT = decltype(nt)::value;
-                hipLaunchKernelGGL((k_fwd_cols_jobs<SR, NT, Load>), dim3(T.N / (NT * kNttRegs), nj), dim3(NT), 0, s,
-                                   T, jobs + j0, ld);
-            });
+            constexpr int NT = decltype(nt)::value;
+            hipLaunchKernelGGL((k_fwd_cols_jobs<SR, NT, Load>), dim3(T.N / (NT * kNttRegs), nj), dim3(NT), 0, s,
+                               T, jobs + j0, ld);
         });
         dispatch_rows_nt(ntr, [&](auto nt) {
             constexpr int NT = decltype(nt)::value;
@@ -123,13 +121,11 @@ void launch_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, cons
             hipLaunchKernelGGL((k_inv_rows_jobs<NT, Load>), dim3(T.N / (NT * kNttRegs), nj), dim3(NT), 0, s, T,
                                jobs + j0, ld);
         });
-        dispatch_sr(T.logN, [&](auto sr) {
+        dispatch_cols(T.logN, ntc, [&](auto sr, auto nt) {
             constexpr int SR = decltype(sr)::value;
-            dispatch_cols_nt(ntc, [&](auto nt) {
-                constexpr int NT = decltype(nt)::value;
-                hipLaunchKernelGGL((k_inv_cols_jobs<SR, NT>), dim3(T.N / (NT * kNttRegs), nj), dim3(NT), 0, s, T,
-                                   jobs + j0);
-            });
+            constexpr int NT = decltype(nt)::value;
+            hipLaunchKernelGGL((k_inv_cols_jobs<SR, NT>), dim3(T.N / (NT * kNttRegs), nj), dim3(NT), 0, s, T,
+                               jobs + j0);
         });
     }
     PHX_HIP_CHECK(hipGetLastError());

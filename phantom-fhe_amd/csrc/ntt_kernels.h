@@ -115,7 +115,7 @@ struct NttTables {
     int rows_nt;
 };
 
-// Storage slot of table entry k (N >= 4096): entries below N/16 in place; the
+// Storage slot of table entry k (N >= 2048): entries below N/16 in place; the
 // block [m, 2m) of the stage at distance 2^jj (m = N >> (jj+1), jj < 4) holds
 // per row r of 256 coefficients a 16 x c matrix (c = 8 >> jj; entry
 // m + r*16c + h*c + i for thread column h and i < c) stored as c x 16.

@@ -107,7 +107,7 @@ def test_per_context_choice_bit_exact(choice):
     assert {k: ph.tune_get(k) for k in before} == before
 
 
-@pytest.mark.parametrize("logN", [12, 13, 15, 16, 17])
+@pytest.mark.parametrize("logN", [11, 12, 13, 15, 16, 17])
 @pytest.mark.parametrize("cols,rows", [(256, 128), (512, 256), (512, 512)])
 def test_tile_shapes_every_degree(logN, cols, rows):
     """every tile shape at every ring degree (shapes whose tile exceeds a limb fall

@@ -32,7 +32,7 @@ def test_n12_three_60bit_ntt_then_intt_bit_exact():
     assert np.array_equal(inv, a)
 
 
-@pytest.mark.parametrize("logN", [12, 13, 14, 15, 16, 17])
+@pytest.mark.parametrize("logN", [11, 12, 13, 14, 15, 16, 17])
 def test_ntt_parity_all_degrees(logN):
     N = 1 << logN
     say(f"case: ntt parity N=2^{logN}")
