@@ -1,8 +1,8 @@
 """Two shapes the reference accepts at the edges of this build's kernels, bit-exact
 against the oracle: ring degree N = 2^11 (the smallest of the reference's 2-D NTT,
-include/ntt.cuh:134-137; one 128-thread tile per limb here) and a special modulus
-of more than 16 primes (the FastBConv inputs no longer fit the registers of the
-VALU kernel's fast path, nor the matrix-core form)."""
+include/ntt.cuh:134-137; one 128-thread tile per limb here), for CKKS and BFV, and
+a special modulus of more than 16 primes (the FastBConv inputs no longer fit the
+registers of the VALU kernel's fast path, nor the matrix-core form)."""
 import numpy as np
 import pytest
 
@@ -148,3 +148,36 @@ def test_special_modulus_20_batch(wide_p):
     ph.device_synchronize()
     for i, (c, Ql, a, b) in enumerate(data):
         assert np.array_equal(c1[i].download(N), octx.multiply_and_relin(Ql, a, b, key)), i
+
+
+@pytest.mark.parametrize("tech", [ph.MUL_TECH_BEHZ, ph.MUL_TECH_HPS])
+def test_n11_bfv_multiply_relinearize(tech):
+    """BFV at N = 2^11 (t = 65537, two 54-bit primes + one special): multiply with
+    BEHZ or HPS, then relinearise, each against the oracle"""
+    ob.set_threads(8)
+    N = 1 << 11
+    say(f"case: N=2^11 bfv tech={tech}")
+    mods = ph.coeff_modulus_create(N, [54, 54, 60])
+    parms = ph.EncryptionParameters(ph.SCHEME_BFV)
+    parms.set_poly_modulus_degree(N)
+    parms.set_coeff_modulus(mods)
+    parms.set_special_modulus_size(1)
+    parms.set_plain_modulus(65537)
+    parms.set_mul_tech(tech)
+    ctx = ph.context(parms)
+    octx = ob.Ctx(11, mods, 1)
+    tool = ob.Behz(octx, 65537) if tech == ph.MUL_TECH_BEHZ else ob.Hps(octx, 65537, overq=False)
+    rng = np.random.default_rng(tech)
+    a, b = rand_ct(rng, mods[:2], N), rand_ct(rng, mods[:2], N)
+    c1 = ph.ciphertext(ctx, 1, 2)
+    c1.upload(a)
+    c2 = ph.ciphertext(ctx, 1, 2)
+    c2.upload(b)
+    ph.multiply_inplace(ctx, c1, c2)
+    ph.device_synchronize()
+    prod = tool.multiply(a, b)
+    assert np.array_equal(c1.download(N), prod)
+    key = rand_key(rng, mods, N, ctx.dnum)
+    rk = ph.relin_key(ctx, host=key)
+    ph.relinearize_inplace(ctx, c1, rk)
+    assert np.array_equal(c1.download(N), ob.bfv_relinearize(octx, prod, key))
