@@ -15,9 +15,9 @@
  * and Q -> Ql) are pinned bit for bit by an exact-rational model of fused
  * multiply-adds in input order (tests/test_oracle_hps_fp64.py), which holds if
  * nvcc contracts the reference's `nu += (double) x * frac` that way (its
- * default). Not pinned:
- *   - the FP64 sum of the HPS forms of BFV decryption (rounding boundaries;
- *     the closed form holds within the FP64 bound): parity unpinned;
+ * default); so is the HPS scale-and-round of BFV decryption, in all four of its
+ * forms (tests/test_oracle_bfv_decrypt.py, on phases at rounding ties).
+ * Not pinned:
  *   - orc_hps_levels_to_drop (FindLevelsToDrop): invariants only, its value is
  *     parity unpinned;
  *   - orc_sample_uniform_seeded: the Salsa20 core is pinned by the

@@ -107,3 +107,63 @@ def scale_round(xq, xr, qs, rs, num, which_frac, t_over):
     X = crt(list(xq) + list(xr), ss)
     val = math.floor(t_over(X) + F(1, 2)) + (int(nu) - math.floor(nu_ex))
     return val, nu, nu_ex
+
+
+def llround(x):
+    """C llround of a double: nearest integer, halves away from zero (exact)"""
+    v = F(x)
+    n = math.floor(abs(v) + F(1, 2))
+    return n if v >= 0 else -n
+
+
+def decrypt_hps(phase, qs, q_max, t, fused=True, reverse=False):
+    """The reference's HPS decryption scale-and-round (src/rns.cu:1519-1632, forms
+    chosen as src/rns.cu:1634-1698 chooses them, constants as src/rns.cu:583-678)
+    of phase = [k][N] residues modulo the level's primes qs; q_max is the largest
+    prime of the whole chain. Every double operation is rounded once, exactly as
+    IEEE defines it: the fma chain as above, the other sums, products, the
+    division of the constants and 1/t by Python's correctly rounded floats; the
+    64-bit integer sum wraps as a uint64_t does. fused=False (a rounded product
+    added with a second rounding) and reverse=True (the primes summed in reverse
+    order) are the variants the tests must tell apart."""
+    k, N = len(qs), len(phase[0])
+    qmsb, sizeqmsb, tmsb = int(q_max).bit_length(), k.bit_length(), int(t).bit_length()
+    large = qmsb + sizeqmsb >= 52
+    h = qmsb >> 1
+    lazy = (h + tmsb + sizeqmsb < 52) if large else (qmsb + tmsb + sizeqmsb < 52)
+    Q = prod(qs)
+    om, fr, omb, frb = [], [], [], []
+    for q in qs:
+        v = pow((Q // q) % q, -1, q)
+        big = t * v
+        om.append((big // q) % t)
+        fr.append(fl(big % q) / fl(q))
+        vb = (v << h) % q
+        bigb = t * vb
+        omb.append((bigb // q) % t)
+        frb.append(fl(bigb % q) / fl(q))
+    tinv = 1.0 / fl(t)
+    mask = (1 << 64) - 1
+    out = []
+    def mac(x, w, acc):
+        return fma(fl(x), w, acc) if fused else acc + fl(x) * w
+
+    order = list(range(k))[::-1] if reverse else list(range(k))
+    for n in range(N):
+        fs, isum = 0.0, 0
+        for i in order:
+            x = int(phase[i][n])
+            if not large:
+                fs = mac(x, fr[i], fs)
+                isum = (isum + (x * om[i] if lazy else x * om[i] % t)) & mask
+            else:
+                hi, lo = x >> h, x & ((1 << h) - 1)
+                fs = mac(lo, fr[i], fs)
+                fs = mac(hi, frb[i], fs)
+                isum = (isum + (lo * om[i] if lazy else lo * om[i] % t)) & mask
+                isum = (isum + (hi * omb[i] if lazy else hi * omb[i] % t)) & mask
+        fs = fs + fl(isum)
+        quot = int(fs * tinv)
+        fs = fs - fl((t * quot) & mask)
+        out.append(llround(fs) & mask)
+    return out

@@ -66,3 +66,42 @@ def test_uniform_phase_rounds_to_nearest(name, bits, k, tbits, behz):
     bad = [d for d in diff if d != 0]
     assert all(d in (1, t - 1) for d in bad), bad[:5]
     assert len(bad) <= N // 200
+
+
+@pytest.mark.parametrize("name,bits,k,tbits,behz", [c for c in CASES if not c[4]])
+def test_hps_forms_bit_exact_fp64_model(name, bits, k, tbits, behz):
+    """the HPS forms' FP64 arithmetic, pinned bit for bit: the oracle equals
+    tests/hps_model.py's exact-rational evaluation of the reference kernel (fused
+    multiply-adds in input order, every other double operation rounded once,
+    the integer sum wrapping at 2^64, llround halves away from zero) at every
+    coefficient, on uniform phases and on phases a few units from a rounding tie
+    of t X / Q, where the floating-point evaluation alone decides the result.
+    In the non-lazy small form (a small integer sum) the products' roundings
+    decide such coefficients: an unfused or a reordered evaluation gives other
+    results there, so the check tells those apart. In the other forms the 64-bit
+    integer sum added last is large enough that the double's resolution (2^-3 ..
+    2^-19) hides those roundings: every variant gives the same results."""
+    import hps_model as hm
+
+    octx, data, Q, t = setup(bits, k, tbits)
+    q_max = max(octx.moduli[:len(data)])
+    rng = np.random.default_rng(11 * bits + k)
+    uniform = np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in data])
+    m = [int(v) for v in rng.integers(0, t, N)]
+    d = [int(v) for v in rng.integers(-3, 4, N)]
+    X = [((2 * mi + 1) * Q) // (2 * t) + di for mi, di in zip(m, d)]
+    ties = residues(X, data)
+    for phase in (uniform, ties):
+        got = [int(v) for v in ob.bfv_decrypt(octx, t, phase, behz)]
+        rows = [[int(v) for v in row] for row in phase]
+        assert got == hm.decrypt_hps(rows, data, q_max, t)
+    unfused = hm.decrypt_hps(rows, data, q_max, t, fused=False)
+    reordered = hm.decrypt_hps(rows, data, q_max, t, reverse=True)
+    n_unf = sum(a != b for a, b in zip(got, unfused))
+    n_rev = sum(a != b for a, b in zip(got, reordered))
+    print(f"{name}: {2 * N} coefficients bit-exact; near ties an unfused evaluation differs at {n_unf}, "
+          f"a reversed order at {n_rev} of {N}")
+    if name == "hps_small":
+        assert n_unf > 0 and n_rev > 0
+    else:
+        assert n_unf == 0 and n_rev == 0
