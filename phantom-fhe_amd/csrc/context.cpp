@@ -222,31 +222,40 @@ HpsConsts resolve_hps(const uint8_t *cbb, const HpsOffs &ho, u32 k, u32 nr, bool
 
 std::vector<uint8_t> bconv_mfma_operands(const std::vector<u64> &mat, u32 ni, u32 no, const std::vector<u64> &omod,
                                          u32 &rt) {
-    // 4 outputs per group of 3 tiles; tile 3T+s, row 4g+i holds output 4T+g, slice b = 4s+i
-    rt = 3 * ((no + 3) / 4);
+    // 4 outputs per group of 2 tiles; tile 2T+s, row 4g+i holds output 4T+g, digit b = 4s+i
+    rt = 2 * ((no + 3) / 4);
     const size_t a_bytes = (size_t)rt * 2 * 64 * 16;
     std::vector<uint8_t> blob(a_bytes + (size_t)no * 16, 0);
-    // mp[o][i][a] = 2^(8a) * mat[o][i] mod p_o (a indexes the input's bytes)
+    // mp[o][i][a] = 2^(8a) * mat[o][i] mod p_o (a indexes the input's bytes), written
+    // in balanced base 256: mp = sum_b d_b 256^b with digits d_b in [-128, 127], b < 8
+    // (eight such digits reach 2^63, above every p_o < 2^61)
+    std::vector<int8_t> dg((size_t)no * ni * 8 * 8);
     std::vector<u64> mp((size_t)no * ni * 8);
     for (u32 o = 0; o < no; ++o) {
         const u64 p = omod[o];
         for (u32 i = 0; i < ni; ++i) {
             u64 v = mat[(size_t)o * ni + i] % p;
             for (u32 a = 0; a < 8; ++a) {
-                mp[((size_t)o * ni + i) * 8 + a] = v;
+                const size_t e = ((size_t)o * ni + i) * 8 + a;
+                mp[e] = v;
+                long long w = (long long)v;
+                for (u32 b = 0; b < 8; ++b) {
+                    long long r = w & 255;
+                    if (r >= 128) r -= 256;
+                    dg[e * 8 + b] = (int8_t)r;
+                    w = (w - r) >> 8;
+                }
+                if (w != 0) fail(kLogicError, "FastBConv constant outside the balanced digit range");
                 v = hm::mulmod(v, 256, p);
             }
         }
     }
-    auto slice = [&](u32 o, u32 b, u32 i, u32 a) -> uint8_t {
-        return (uint8_t)((mp[((size_t)o * ni + i) * 8 + a] >> (7 * b)) & 127);
-    };
-    // A[row][K]: row -> (output, 7-bit slice) as above, K = i*8 + a
+    // A[row][K]: row -> (output, digit) as above, K = i*8 + a
     auto A = [&](u32 row, u32 K) -> uint8_t {
         const u32 t = row / 16, rr = row % 16;
-        const u32 o = 4 * (t / 3) + rr / 4, b = 4 * (t % 3) + rr % 4, i = K / 8, a = K % 8;
-        if (o >= no || b >= 9 || i >= ni) return 0;
-        return slice(o, b, i, a);
+        const u32 o = 4 * (t / 2) + rr / 4, b = 4 * (t % 2) + rr % 4, i = K / 8, a = K % 8;
+        if (o >= no || i >= ni) return 0;
+        return (uint8_t)dg[(((size_t)o * ni + i) * 8 + a) * 8 + b];
     };
     for (u32 t = 0; t < rt; ++t)
         for (u32 h = 0; h < 2; ++h)
@@ -254,16 +263,12 @@ std::vector<uint8_t> bconv_mfma_operands(const std::vector<u64> &mat, u32 ni, u3
                 for (u32 j = 0; j < 16; ++j)
                     blob[(((size_t)t * 2 + h) * 64 + l) * 16 + j] = A(16 * t + l % 16, 64 * h + 16 * (l / 16) + j);
     // bias of the signed input bytes: the kernel feeds s = byte - 128, so output o
-    // adds back C_o = 128 * sum_b 2^(7b) sum_(i,a) slice_b(mp[o][i][a])  (< 2^76);
-    // stored as (C_o mod 2^64, fl(C_o) as double bits)
+    // adds back C_o = 128 * sum_b 256^b sum_(i,a) d_b(mp[o][i][a]) = 128 sum_(i,a)
+    // mp[o][i][a] (< 2^75); stored as (C_o mod 2^64, fl(C_o) as double bits)
     for (u32 o = 0; o < no; ++o) {
         hm::u128 c = 0;
-        for (u32 b = 0; b < 9; ++b) {
-            u64 r = 0;
-            for (u32 i = 0; i < ni; ++i)
-                for (u32 a = 0; a < 8; ++a) r += slice(o, b, i, a);
-            c += ((hm::u128)r * 128) << (7 * b);
-        }
+        for (u32 i = 0; i < ni; ++i)
+            for (u32 a = 0; a < 8; ++a) c += (hm::u128)mp[((size_t)o * ni + i) * 8 + a] * 128;
         const u64 lo = (u64)c;
         const double cd = (double)(u64)(c >> 64) * 0x1p64 + (double)lo;
         std::memcpy(blob.data() + a_bytes + (size_t)o * 16, &lo, 8);

@@ -40,15 +40,15 @@ struct BconvDesc {
     // MFMA form (ni <= 16): int8 operand tiles of the sliced matrix, see bconv_mfma
     u64 a_off;           // offset (u64 units) of rt x 2 x 64 lanes x 16 bytes, then no x (lo, fl) bias
     u64 pinv_off;        // offset (u64 units) of fl(1/p_o) as double bits, [no]
-    u32 rt;              // 16-row operand tiles: 3 * ceil(no / 4); 0 = no MFMA form
+    u32 rt;              // 16-row operand tiles: 2 * ceil(no / 4); 0 = no MFMA form
     u32 pad;
 };
 
 // MFMA operand blob of one FastBConv matrix (host side). Row 4g+i of tile
-// 3T+s stands for output o = 4T+g, slice b = 4s+i (b < 9, other rows zero), so
-// the 9 partial sums of an output end up in one lane's accumulators; K =
-// (i, a) = i*8 + a indexes byte a of input i; A[row][K] = bits [7b, 7b+7) of
-// (2^(8a) * mat[o][i] mod p_o). Lane l of tile t, half h holds bytes
+// 2T+s stands for output o = 4T+g, digit b = 4s+i (b < 8), so the 8 partial
+// sums of an output end up in one lane's accumulators; K = (i, a) = i*8 + a
+// indexes byte a of input i; A[row][K] = digit b of the balanced base-256
+// expansion (digits in [-128, 127]) of 2^(8a) * mat[o][i] mod p_o. Lane l of tile t, half h holds bytes
 // A[16t + l%16][64h + 16(l/16) + j], j < 16. After the tiles: per output the
 // bias C_o that restores the unsigned input bytes, as (C_o mod 2^64, fl(C_o)).
 std::vector<uint8_t> bconv_mfma_operands(const std::vector<u64> &mat, u32 ni, u32 no,

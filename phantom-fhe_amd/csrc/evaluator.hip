@@ -98,17 +98,17 @@ __global__ __launch_bounds__(kThreads) void k_bconv(BconvArgs A) {
 // FastBConv phase 2 on the matrix cores, exact (same outputs as k_bconv).
 //
 // With x_i = sum_a x_ia 2^(8a) (its bytes, a < 8) and
-// M'_oia = 2^(8a) M[o][i] mod p_o = sum_b m_oiab 2^(7b) (7-bit slices, b < 9):
-//   sum_i x_i M[o][i] = sum_b 2^(7b) W[o,b]  (mod p_o),
-//   W[o,b] = sum_(i,a) m_oiab x_ia.
+// M'_oia = 2^(8a) M[o][i] mod p_o = sum_b d_oiab 256^b (balanced base-256
+// digits d in [-128, 127], b < 8):
+//   T_o = sum_(i,a) x_ia M'_oia = sum_i x_i M[o][i]  (mod p_o),  T_o < 2^15 p_o.
 // The matrix cores multiply signed bytes, so the inputs enter as
-// s_ia = x_ia - 128 (bit 7 flipped) and Z[o,b] = sum m_oiab s_ia is an int8
-// GEMM with int32 sums (|Z| < 128*127*128 < 2^21, exact); the constant
-// 128 * sum_b 2^(7b) sum_(i,a) m_oiab is precomputed per output and added
-// back. K = (i,a) = 8i + a <= 128 -> two v_mfma_i32_16x16x64_i8 per 16x16
-// tile; rows are permuted so the 9 sums of an output sit in one lane's
-// accumulators. V = sum_b 2^(7b) Z_b + C_o < 2^79 is reduced with an FP64
-// quotient estimate (off by at most one) and one signed correction.
+// s_ia = x_ia - 128 (bit 7 flipped) and Z[o,b] = sum d_oiab s_ia is an int8
+// GEMM with int32 sums (|Z| <= 128*128*128 = 2^21, exact); then
+//   T_o = sum_b 256^b Z[o,b] + C_o,  C_o = 128 sum_(i,a) M'_oia,
+// with C_o precomputed per output. K = (i,a) = 8i + a <= 128 -> two
+// v_mfma_i32_16x16x64_i8 per 16x16 tile; rows are permuted so the 8 sums of an
+// output sit in one lane's accumulators. T_o is reduced with an FP64 quotient
+// estimate (off by at most one) and one correction.
 // ---------------------------------------------------------------------------
 typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr u32 kBconvTileCols = 16;   // coefficients per MFMA column tile
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
     const auto outj = gmem(outb);
     const u32 wave = tid >> 6, lane = tid & 63, ln = lane & 15, lg = lane >> 4;
     const u32 ntiles = N / kBconvTileCols;
-    const u32 groups = rt / 3;
+    const u32 groups = rt / 2;
 
     // each wave walks its own column tiles (no workgroup barriers below); the
     // inputs of the next tile are loaded before the current tile is computed
@@ -184,11 +184,11 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
             b[h] = v4i{(int)((u32)xn[h][0] ^ 0x80808080u), (int)((u32)(xn[h][0] >> 32) ^ 0x80808080u),
                        (int)((u32)xn[h][1] ^ 0x80808080u), (int)((u32)(xn[h][1] >> 32) ^ 0x80808080u)};
         if (ct + ct_step < ntiles) load_x(ct + ct_step);
-        // 3 tiles per group of 4 outputs; lane (lg, ln) owns output 4T + lg
-        auto products = [&](u32 T, v4i(&acc)[3]) {
+        // 2 tiles per group of 4 outputs; lane (lg, ln) owns output 4T + lg
+        auto products = [&](u32 T, v4i(&acc)[2]) {
 #pragma unroll
-            for (int sgrp = 0; sgrp < 3; ++sgrp) {
-                const u32 t = 3 * T + sgrp;
+            for (int sgrp = 0; sgrp < 2; ++sgrp) {
+                const u32 t = 2 * T + sgrp;
                 const uint4 ua0 = As[(t * 2 + 0) * 64 + lane];
                 const uint4 ua1 = As[(t * 2 + 1) * 64 + lane];
                 const v4i a0 = {(int)ua0.x, (int)ua0.y, (int)ua0.z, (int)ua0.w};
@@ -198,24 +198,24 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
                 acc[sgrp] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b[1], c, 0, 0, 0);
             }
         };
-        // Z_b = acc[b / 4][b % 4] (signed, |Z_b| < 2^21), b < 9;
-        // V = sum_b 2^(7b) Z_b + C_o is in [0, 2^15 p): its low 64 bits exactly,
-        // its value in FP64 to within 2^-33 p, then r = V - floor(V/p - 2^-30) p
-        // lies in [0, 2p)
-        auto reduce = [&](u32 T, const v4i(&acc)[3]) {
+        // Z_b = acc[b / 4][b % 4] (signed, |Z_b| <= 2^21), b < 8, paired as
+        // y_k = Z_2k + 256 Z_2k+1 (|y_k| < 2^30): T = sum_k 2^(16k) y_k + C_o is in
+        // [0, 2^15 p). Its low 64 bits come exactly from 64-bit integer sums; in
+        // FP64 every partial sum stays below 2^17 p (a balanced-digit expansion
+        // has sum_b 256^b |d_b| < 3.02 M': the top digit is at least 1 and the
+        // lower ones sum to at most 0.502 256^top), so the four roundings leave T
+        // within 2^-34 p, and r = T - floor(T/p - 2^-30) p lies in [0, 2p)
+        auto reduce = [&](u32 T, const v4i(&acc)[2]) {
             const u32 o = 4 * T + lg;
             if (o >= no) return;
             const OutConst c = oc[o];
-            const int y0 = acc[0][0] + acc[0][1] * 128, y1 = acc[0][2] + acc[0][3] * 128;
-            const int y2 = acc[1][0] + acc[1][1] * 128, y3 = acc[1][2] + acc[1][3] * 128;
-            const int y4 = acc[2][0];
-            const u64 lo = c.bias_lo + (u64)((long long)y0 + (long long)y1 * (1ll << 14) +
-                                             (long long)y2 * (1ll << 28)) +
-                           (u64)(long long)y3 * (1ull << 42) + (u64)(long long)y4 * (1ull << 56);
-            double v = __builtin_fma((double)y4, 0x1p56, c.bias);
-            v = __builtin_fma((double)y3, 0x1p42, v);
-            v = __builtin_fma((double)y2, 0x1p28, v);
-            v = __builtin_fma((double)y1, 0x1p14, v);
+            const int y0 = acc[0][0] + acc[0][1] * 256, y1 = acc[0][2] + acc[0][3] * 256;
+            const int y2 = acc[1][0] + acc[1][1] * 256, y3 = acc[1][2] + acc[1][3] * 256;
+            const u64 lo = c.bias_lo + (u64)((long long)y0 + (long long)y1 * (1ll << 16)) +
+                           (u64)(long long)y2 * (1ull << 32) + (u64)(long long)y3 * (1ull << 48);
+            double v = __builtin_fma((double)y3, 0x1p48, c.bias);
+            v = __builtin_fma((double)y2, 0x1p32, v);
+            v = __builtin_fma((double)y1, 0x1p16, v);
             v += (double)y0;
             const u32 kq = (u32)__builtin_fmax(__builtin_floor(__builtin_fma(v, c.pinv, -0x1p-30)), 0.0);
             u64 r = lo - (u64)kq * c.p;
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kBconvThreads) void k_bconv_mfma(BconvArgs A) {
             outj[c.off + k] = r;
         };
         // the next group's products are issued before this group's reduction
-        v4i acc0[3], acc1[3];
+        v4i acc0[2], acc1[2];
         products(0, acc0);
         for (u32 T = 0; T < groups; T += 2) {
             const bool odd = T + 1 < groups;
