@@ -2,6 +2,7 @@
 // lists (see batch.h).
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <numeric>
 
@@ -306,7 +307,7 @@ void BatchPlan::bind(u64 *arena) {
     }
     // items sharing a key next to each other, the most digits first (IpItem.pad holds the chain)
     std::stable_sort(ip_.begin(), ip_.end(), [](const IpItem &x, const IpItem &y) {
-        return x.key != y.key ? x.key < y.key : x.pad < y.pad;
+        return x.key != y.key ? std::less<const u64 *>{}(x.key, y.key) : x.pad < y.pad;
     });
     sort_by_row(mu_inv_);
     sort_by_row(mu_fwd_);
@@ -392,6 +393,25 @@ BatchPlan *BatchPlan::finish(std::unique_ptr<BatchPlan> p) {
     if (nb <= 1) {
         p->layout();
         return p.release();
+    }
+    // items sharing a key-switching key go to the same run where possible, so each
+    // branch's key inner product reads its keys once (rotations by many steps);
+    // the items' order within the plan does not matter to their results
+    {
+        std::vector<size_t> order(n);
+        std::iota(order.begin(), order.end(), size_t{0});
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t x, size_t y) { return std::less<const u64 *>{}(p->items_[x].key, p->items_[y].key); });
+        auto permute = [&](auto &v) {
+            if (v.empty()) return;
+            auto copy = v;
+            for (size_t k = 0; k < n; ++k) v[k] = copy[order[k]];
+        };
+        permute(p->items_);
+        permute(p->cts_);
+        permute(p->ins_);
+        permute(p->scale_mul_);
+        permute(p->bfv_square_);
     }
     // contiguous runs of items of about equal work (scratch words as its measure):
     // an item joins the run when its midpoint lies below the run's share of the
