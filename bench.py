@@ -371,6 +371,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ntt-only", action="store_true",
                     help="only the forward NTT over 47 x 16 limbs and the copy kernel (for counter runs)")
+    ap.add_argument("--rotations-only", action="store_true",
+                    help="only the batched-rotation comparison (128 ciphertexts, 16 keys); one JSON line")
     ap.add_argument("--hmult-only", action="store_true",
                     help="only the plain and batched HMult+relin (for kernel-trace runs)")
     ap.add_argument("--serve-qps", type=float, default=400.0)
@@ -398,9 +400,12 @@ def main():
     ctx, mods = make_context(ph, N, BITS_N16_Q35_P12, 12)
     stream = ph.stream_create()
     autotune = None
-    if not args.no_autotune and not args.tune and not args.ntt_only:
+    if not args.no_autotune and not args.tune and not args.ntt_only and not args.rotations_only:
         autotune = ctx.autotune()
         log(f"bench: autotune {autotune}")
+    if args.rotations_only:
+        print(json.dumps(bench_rotations(ph, ctx, mods, N, stream)), flush=True)
+        return
     if args.ntt_only:
         L = ctx.size_Q + ctx.size_P
         ntt_ms, ntt_min = bench_ntt(ph, ctx, N, L, args.ntt_polys, 30, stream)
