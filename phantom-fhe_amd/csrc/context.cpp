@@ -381,13 +381,15 @@ Context::Context(const Params &p) : params_(p) {
     const size_t twdb = R * N_ * sizeof(double);
     const size_t modb = R * sizeof(DevModulus);
     const size_t icb = R * 4 * sizeof(u64);
-    std::vector<uint8_t> h(2 * twb + 2 * twdb + modb + icb);
+    const size_t ncb = R * 3 * sizeof(u64);
+    std::vector<uint8_t> h(2 * twb + 2 * twdb + modb + icb + ncb);
     TwPair *htw = reinterpret_cast<TwPair *>(h.data());
     TwPair *hitw = reinterpret_cast<TwPair *>(h.data() + twb);
     double *htwd = reinterpret_cast<double *>(h.data() + 2 * twb);
     double *hitwd = reinterpret_cast<double *>(h.data() + 2 * twb + twdb);
     DevModulus *hmod = reinterpret_cast<DevModulus *>(h.data() + 2 * twb + 2 * twdb);
     u64 *hic = reinterpret_cast<u64 *>(h.data() + 2 * twb + 2 * twdb + modb);
+    u64 *hnc = reinterpret_cast<u64 *>(h.data() + 2 * twb + 2 * twdb + modb + icb);
     for (size_t r = 0; r < R; ++r) {
         const auto &t = host_tables_[r];
         for (u64 k = 0; k < N_; ++k) {
@@ -410,6 +412,9 @@ Context::Context(const Params &p) : params_(p) {
         hic[4 * r + 1] = hm::shoup(t.ninv, t.q);
         hic[4 * r + 2] = s1;
         hic[4 * r + 3] = hm::shoup(s1, t.q);
+        hnc[3 * r + 0] = 0 - t.q;
+        hnc[3 * r + 1] = 0 - 2 * t.q;
+        hnc[3 * r + 2] = 2 * t.q + 1;
     }
     PHX_HIP_CHECK(hipMalloc(&tw_buf_, h.size()));
     PHX_HIP_CHECK(hipMemcpy(tw_buf_, h.data(), h.size(), hipMemcpyHostToDevice));
@@ -420,6 +425,7 @@ Context::Context(const Params &p) : params_(p) {
     tables_.itwd = reinterpret_cast<const double *>(db + 2 * twb + twdb);
     tables_.mod = reinterpret_cast<const DevModulus *>(db + 2 * twb + 2 * twdb);
     tables_.inv_consts = reinterpret_cast<const u64 *>(db + 2 * twb + 2 * twdb + modb);
+    tables_.int_consts = reinterpret_cast<const u64 *>(db + 2 * twb + 2 * twdb + modb + icb);
     tables_.logN = logN_;
     tables_.N = (u32)N_;
     set_tuning(TuneChoice{});

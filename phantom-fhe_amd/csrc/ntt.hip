@@ -29,8 +29,7 @@ __global__ __launch_bounds__(NT) void k_fwd_cols(NttTables T, LimbMap M, Load ld
         fwd_cols_body<SR, NT>(fp_arith(T.twd + (u64)row * T.N, md), lds, data, T.logN, c0, limb, z, md, ld,
                               threadIdx.x);
     else
-        fwd_cols_body<SR, NT>(int_arith(T.tw + (u64)row * T.N, md.value), lds, data, T.logN, c0, limb, z, md,
-                              ld, threadIdx.x);
+        fwd_cols_body<SR, NT>(int_fwd(T, row, md.value), lds, data, T.logN, c0, limb, z, md, ld, threadIdx.x);
 }
 
 template <int NT, class Store>
@@ -47,8 +46,7 @@ __global__ __launch_bounds__(NT) void k_fwd_rows(NttTables T, LimbMap M, Store s
         fwd_rows_body<NT>(fp_arith(T.twd + (u64)row * T.N, md), lds, data, T.logN, r0, limb, z, md, st,
                           threadIdx.x);
     else
-        fwd_rows_body<NT>(int_arith(T.tw + (u64)row * T.N, md.value), lds, data, T.logN, r0, limb, z, md, st,
-                          threadIdx.x);
+        fwd_rows_body<NT>(int_fwd(T, row, md.value), lds, data, T.logN, r0, limb, z, md, st, threadIdx.x);
 }
 
 template <int NT, class Load>
@@ -66,8 +64,7 @@ __global__ __launch_bounds__(NT) void k_inv_rows(NttTables T, LimbMap M, Load ld
         inv_rows_body<NT>(fp_arith(T.itwd + (u64)row * T.N, md), lds, data, src, T.logN, r0, limb, z, md, ld,
                           threadIdx.x);
     else
-        inv_rows_body<NT>(int_arith(T.itw + (u64)row * T.N, md.value), lds, data, src, T.logN, r0, limb, z, md,
-                          ld, threadIdx.x);
+        inv_rows_body<NT>(int_inv(T, row, md.value), lds, data, src, T.logN, r0, limb, z, md, ld, threadIdx.x);
 }
 
 template <int SR, int NT, class Store>
@@ -88,7 +85,7 @@ __global__ __launch_bounds__(NT) void k_inv_cols(NttTables T, LimbMap M, Store s
         inv_cols_body<SR, NT>(fp_arith(T.itwd + (u64)row * T.N, md), lds, data, T.logN, c0, limb, z, md, st,
                               s0, s0s, s1, s1s, threadIdx.x);
     else
-        inv_cols_body<SR, NT>(int_arith(T.itw + (u64)row * T.N, md.value), lds, data, T.logN, c0, limb, z, md,
+        inv_cols_body<SR, NT>(int_inv(T, row, md.value), lds, data, T.logN, c0, limb, z, md,
                               st, s0, s0s, s1, s1s, threadIdx.x);
 }
 

@@ -18,24 +18,44 @@
 
 namespace phx {
 
+// x in [0, 2m) -> [0, m) for m < 2^63, given nm = -m mod 2^64: t = x - m as an
+// add, and m added back where t went negative. Unlike a compare and select (or a
+// 64-bit subtract), this needs no lane mask, whose write-then-read the gfx950
+// VALU pads with wait states.
+__device__ __forceinline__ u64 csub_n(u64 x, u64 m, u64 nm) {
+    const u64 t = x + nm;
+    return t + (m & (u64)((long long)t >> 63));
+}
+
 struct IntArith {
     using T = u64;
     const TwPair *tw;
     u64 q, q2;
+    // -q, -2q and 2q + 1 mod 2^64, read from the device tables: the subtractions
+    // below are additions of them (y - t = y + 2q + 1 + ~t - 2q), and the
+    // optimiser, which does not see these values, keeps them additions
+    u64 nq, nq2, q2p1;
 
     __device__ __forceinline__ static T from_canon(u64 v) { return v; }
     __device__ __forceinline__ static T from_bits(u64 b) { return b; }
     __device__ __forceinline__ static u64 to_bits(T v) { return v; }
     // forward output [0, 4q) -> [0, q)
-    __device__ __forceinline__ u64 fwd_canon(T v) const { return csub(csub(v, q2), q); }
+    __device__ __forceinline__ u64 fwd_canon(T v) const { return csub_n(csub_n(v, q2, nq2), q, nq); }
 
+    // Harvey's lazy butterflies (the arithmetic of ct_bfly / gs_bfly in common.h):
+    // t = y w - floor(y w' / 2^64) q in [0, 2q) as y w + hi (-q)
     __device__ __forceinline__ void ct(T &x, T &y, u32 idx) const {
         const TwPair w = tw[idx];
-        ct_bfly(x, y, w.w, w.ws, q, q2);
+        const u64 t = y * w.w + umulhi(y, w.ws) * nq;
+        const u64 xx = csub_n(x, q2, nq2);
+        y = xx + q2p1 + ~t;   // xx + 2q - t
+        x = xx + t;
     }
     __device__ __forceinline__ void gs(T &x, T &y, u32 idx) const {
         const TwPair w = tw[idx];
-        gs_bfly(x, y, w.w, w.ws, q, q2);
+        const u64 d = x + q2p1 + ~y;   // x + 2q - y
+        x = csub_n(x + y, q2, nq2);
+        y = d * w.w + umulhi(d, w.ws) * nq;
     }
     // top inverse stage with both outputs scaled to canonical form
     __device__ __forceinline__ void gs_last(T &x, T &y, u64 s0, u64 s0s, u64 s1, u64 s1s) const {

@@ -33,7 +33,13 @@ __device__ __forceinline__ void limb_scale(const NttTables &T, const LimbMap &M,
     s1s = c[3];
 }
 
-__device__ __forceinline__ IntArith int_arith(const TwPair *tw, u64 q) { return IntArith{tw, q, q << 1}; }
+// the integer arithmetic of table row `row`: forward (tw) or inverse (itw) twiddles
+__device__ __forceinline__ IntArith int_arith(const NttTables &T, const TwPair *tw, u32 row, u64 q) {
+    const u64 *c = T.int_consts + 3 * (u64)row;
+    return IntArith{tw + (u64)row * T.N, q, q << 1, c[0], c[1], c[2]};
+}
+__device__ __forceinline__ IntArith int_fwd(const NttTables &T, u32 row, u64 q) { return int_arith(T, T.tw, row, q); }
+__device__ __forceinline__ IntArith int_inv(const NttTables &T, u32 row, u64 q) { return int_arith(T, T.itw, row, q); }
 __device__ __forceinline__ FpArith fp_arith(const double *tw, const DevModulus &m) {
     return FpArith{tw, (double)m.value, __longlong_as_double((long long)m.pad)};
 }

@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kNttThreads *G) void k_fwd_fused(NttTables T, LimbM
     if (md.pad)
         run(fp_arith(T.twd + (u64)row * T.N, md));
     else
-        run(int_arith(T.tw + (u64)row * T.N, md.value));
+        run(int_fwd(T, row, md.value));
 }
 
 template <int SR, int G, class Load, class Store>
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kNttThreads *G) void k_inv_fused(NttTables T, LimbM
     if (md.pad)
         run(fp_arith(T.itwd + (u64)row * T.N, md));
     else
-        run(int_arith(T.itw + (u64)row * T.N, md.value));
+        run(int_inv(T, row, md.value));
 }
 
 namespace {

@@ -18,8 +18,7 @@ __global__ __launch_bounds__(NT) void k_fwd_cols_jobs(NttTables T, const NttJob 
         fwd_cols_body<SR, NT>(fp_arith(T.twd + (u64)J.row * T.N, md), lds, J.data, T.logN, c0, 0u, 0u, md, lb,
                               threadIdx.x);
     else
-        fwd_cols_body<SR, NT>(int_arith(T.tw + (u64)J.row * T.N, md.value), lds, J.data, T.logN, c0, 0u, 0u, md, lb,
-                              threadIdx.x);
+        fwd_cols_body<SR, NT>(int_fwd(T, J.row, md.value), lds, J.data, T.logN, c0, 0u, 0u, md, lb, threadIdx.x);
 }
 
 template <int NT, class Store>
@@ -33,8 +32,7 @@ __global__ __launch_bounds__(NT) void k_fwd_rows_jobs(NttTables T, const NttJob 
         fwd_rows_body<NT>(fp_arith(T.twd + (u64)J.row * T.N, md), lds, J.data, T.logN, r0, 0u, 0u, md, sb,
                           threadIdx.x);
     else
-        fwd_rows_body<NT>(int_arith(T.tw + (u64)J.row * T.N, md.value), lds, J.data, T.logN, r0, 0u, 0u, md, sb,
-                          threadIdx.x);
+        fwd_rows_body<NT>(int_fwd(T, J.row, md.value), lds, J.data, T.logN, r0, 0u, 0u, md, sb, threadIdx.x);
 }
 
 template <int NT, class Load>
@@ -50,8 +48,7 @@ __global__ __launch_bounds__(NT) void k_inv_rows_jobs(NttTables T, const NttJob 
         inv_rows_body<NT>(fp_arith(T.itwd + (u64)J.row * T.N, md), lds, J.data, src, T.logN, r0, 0u, 0u, md, lb,
                           threadIdx.x);
     else
-        inv_rows_body<NT>(int_arith(T.itw + (u64)J.row * T.N, md.value), lds, J.data, src, T.logN, r0, 0u, 0u, md,
-                          lb, threadIdx.x);
+        inv_rows_body<NT>(int_inv(T, J.row, md.value), lds, J.data, src, T.logN, r0, 0u, 0u, md, lb, threadIdx.x);
 }
 
 template <int SR, int NT>
@@ -68,7 +65,7 @@ __global__ __launch_bounds__(NT) void k_inv_cols_jobs(NttTables T, const NttJob 
         inv_cols_body<SR, NT>(fp_arith(T.itwd + (u64)J.row * T.N, md), lds, J.data, T.logN, c0, 0u, 0u, md,
                               StorePlain{}, s0, s0s, s1, s1s, threadIdx.x);
     else
-        inv_cols_body<SR, NT>(int_arith(T.itw + (u64)J.row * T.N, md.value), lds, J.data, T.logN, c0, 0u, 0u, md,
+        inv_cols_body<SR, NT>(int_inv(T, J.row, md.value), lds, J.data, T.logN, c0, 0u, 0u, md,
                               StorePlain{}, s0, s0s, s1, s1s, threadIdx.x);
 }
 
