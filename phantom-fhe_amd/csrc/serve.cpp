@@ -23,6 +23,8 @@ Server::~Server() {
     }
     for (auto &kv : reqs_) kv.second->ct.release(s_);
     (void)hipStreamSynchronize(s_);
+    for (auto &t : inflight_)
+        for (auto &c : t.retired) c.release(s_);
     (void)hipFree(arena_);
     (void)hipFree(tables_);
     for (auto &t : inflight_)
@@ -138,6 +140,7 @@ void Server::harvest(bool wait_oldest) {
         }
         (void)hipEventDestroy(t.ev);
         for (auto &b : t.staging) staging_free_.push_back(b);
+        for (auto &c : t.retired) c.release(s_);
         inflight_.pop_front();
     }
 }
@@ -182,7 +185,9 @@ u32 Server::step() {
         }
         if (!res.empty()) {
             // every rescale of the lane in one batched launch set: outputs at the next
-            // level from the pool, the inputs released after the plan's work (same stream)
+            // level from the pool; the inputs are freed when the tick is harvested
+            // (its event has completed), not in stream order on s_: the caller's
+            // thread allocates request copies from the same pool on its own stream
             std::vector<Ciphertext> outs(res.size());
             std::vector<const Ciphertext *> ins;
             std::vector<Ciphertext *> outp;
@@ -199,7 +204,7 @@ u32 Server::step() {
                 throw;
             }
             for (size_t i = 0; i < res.size(); ++i) {
-                res[i]->ct.release(s_);
+                tick.retired.push_back(res[i]->ct);
                 res[i]->ct = outs[i];
             }
         }

@@ -78,6 +78,10 @@ private:
         hipEvent_t ev = nullptr;
         std::vector<std::unique_ptr<BatchPlan>> plans;
         std::vector<Staging> staging;   // returned to the free list when the tick is harvested
+        // ciphertexts the tick's rescales replaced: freed once the tick has run, so
+        // the pool cannot hand their memory to another stream (the caller's uploads)
+        // while this tick's kernels may still read it
+        std::vector<Ciphertext> retired;
         std::vector<u64> finishing;
     };
     void harvest(bool wait_oldest);
