@@ -370,15 +370,16 @@ int phx_device_count(int *n);
  *   "ntt_fp64"   1 = exact FP64 butterflies for primes < 2^50 (default), 0 = 64-bit integer
  *                only; takes effect for contexts created afterwards
  *   "ntt_fused", "ntt_fused_min_limbs", "ntt_chunk_limbs", "ntt_cols_threads",
- *   "ntt_rows_threads", "bconv_wgs_per_cu": the process-wide defaults of the
- *                per-context variants below
+ *   "ntt_rows_threads", "bconv_wgs_per_cu", "bconv_fused": the process-wide defaults
+ *                of the per-context variants below
  * unknown keys return PHX_EINVAL */
 int phx_tune_set(const char *key, int value);
 int phx_tune_get(const char *key, int *value);
 /* Per-context kernel variants (no reference counterpart: the reference fixes its
  * launch shapes at compile time). choice[PHX_TUNE_AXES] = {ntt_fused,
  * ntt_fused_min_limbs, ntt_chunk_limbs, bconv_mfma, ntt_cols_threads,
- * ntt_rows_threads, bconv_wgs_per_cu}; -1 = follow the process-wide switch of that name.
+ * ntt_rows_threads, bconv_wgs_per_cu, bconv_fused}; -1 = follow the process-wide switch of
+ * that name.
  *   ntt_fused           0 = two launches per NTT; 1, 2, 4 = one workgroup of that
  *                       many x 256 threads per limb (N / 4096 a multiple of it)
  *   ntt_fused_min_limbs launches over fewer limbs keep two launches
@@ -390,8 +391,11 @@ int phx_tune_get(const char *key, int *value);
  *   ntt_rows_threads    tile shape of the row passes: 128, 256 or 512 threads
  *                       (8, 16 or 32 rows of 256 coefficients)
  *   bconv_wgs_per_cu    matrix-core FastBConv workgroups per CU: 1, 2, 4 (0 = as many
- *                       as the LDS holds) */
-#define PHX_TUNE_AXES 7
+ *                       as the LDS holds)
+ *   bconv_fused         1 = the matrix-core FastBConv of modup and moddown fused with the
+ *                       column pass of the forward NTT of its outputs (N = 2^13 .. 2^16,
+ *                       special_modulus_size up to 16), 0 = separate launches */
+#define PHX_TUNE_AXES 8
 typedef struct {
     int choice[PHX_TUNE_AXES];   /* the variants applied */
     double default_us;   /* objective with every switch at -1: one HMult+relin at the top level plus

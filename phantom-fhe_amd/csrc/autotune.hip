@@ -100,7 +100,7 @@ std::string describe(const TuneChoice &c) {
     o << "ntt_fused=" << c.ntt_fused << ",ntt_fused_min_limbs=" << c.ntt_fused_min_limbs
       << ",ntt_chunk_limbs=" << c.ntt_chunk_limbs << ",bconv_mfma=" << c.bconv_mfma
       << ",ntt_cols_threads=" << c.ntt_cols_threads << ",ntt_rows_threads=" << c.ntt_rows_threads
-      << ",bconv_wgs_per_cu=" << c.bconv_wgs_per_cu;
+      << ",bconv_wgs_per_cu=" << c.bconv_wgs_per_cu << ",bconv_fused=" << c.bconv_fused;
     return o.str();
 }
 }  // namespace
@@ -218,6 +218,8 @@ TuneResult autotune(Context &ctx, hipStream_t s) {
         best_us = va_us < kMargin * mf_us ? va_us : mf_us;
         // 4. the matrix-core FastBConv's workgroups per CU
         if (best.bconv_mfma == 1) axis(&TuneChoice::bconv_wgs_per_cu, {0, 1, 2}, 0);
+        // 5. the matrix-core FastBConv fused with the forward NTT's column pass, or separate
+        if (best.bconv_mfma == 1) axis(&TuneChoice::bconv_fused, {1, 0}, 1);
         res.choice = best;
         res.best_us = best_us;
     } catch (...) {
@@ -271,7 +273,8 @@ void tune_cache_store(const std::string &path, const std::string &key, const Tun
     e << "  {" << tag << ", \"ntt_fused\": " << c.ntt_fused << ", \"ntt_fused_min_limbs\": " << c.ntt_fused_min_limbs
       << ", \"ntt_chunk_limbs\": " << c.ntt_chunk_limbs << ", \"bconv_mfma\": " << c.bconv_mfma
       << ", \"ntt_cols_threads\": " << c.ntt_cols_threads << ", \"ntt_rows_threads\": " << c.ntt_rows_threads
-      << ", \"bconv_wgs_per_cu\": " << c.bconv_wgs_per_cu << ", \"hmult_relin_us\": " << us << "}";
+      << ", \"bconv_wgs_per_cu\": " << c.bconv_wgs_per_cu << ", \"bconv_fused\": " << c.bconv_fused
+      << ", \"hmult_relin_us\": " << us << "}";
     kept.push_back(e.str());
     std::ofstream out(path, std::ios::trunc);
     require((bool)out, "cannot write the tuning cache " + path);
@@ -298,6 +301,7 @@ bool tune_cache_lookup(const std::string &path, const std::string &key, TuneChoi
         entry_field(line, "ntt_cols_threads", &c.ntt_cols_threads);
         entry_field(line, "ntt_rows_threads", &c.ntt_rows_threads);
         entry_field(line, "bconv_wgs_per_cu", &c.bconv_wgs_per_cu);
+        entry_field(line, "bconv_fused", &c.bconv_fused);
         *out = c;
         return true;
     }

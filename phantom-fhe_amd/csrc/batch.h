@@ -139,6 +139,7 @@ private:
     std::vector<NttJob> mu_inv_, mu_fwd_, md_inv_, md_fwd_;
     std::vector<BconvRow> mu_bc_, md_bc_;
     std::vector<BconvDesc> mu_desc_host_, md_desc_host_;   // every descriptor the rows use
+    std::vector<BconvDesc> mu_fdesc_host_, md_fdesc_host_; // and their fused forms
     u32 max_Ql_ = 0, max_beta_ = 0;
     std::vector<uint8_t> host_tables_;
     uint8_t *tables_ = nullptr;      // device copy (persistent plans)

@@ -212,6 +212,8 @@ void BatchPlan::bind(u64 *arena) {
     md_bc_.clear();
     mu_desc_host_.clear();
     md_desc_host_.clear();
+    mu_fdesc_host_.clear();
+    md_fdesc_host_.clear();
     max_Ql_ = max_beta_ = 0;
     if (kind_ == kBfvMulRelin) {
         const u64 B = items_.size(), LN = (u64)items_[0].Ql * N;
@@ -275,7 +277,7 @@ void BatchPlan::bind(u64 *arena) {
                 mu_inv_.push_back(NttJob{t_cks + l * N, c2 + l * N, nullptr, nullptr, L.modup_scale + 4 * l, l, 0});
         }
         for (u32 j = 0; j < beta; ++j) {
-            mu_bc_.push_back(BconvRow{L.modup_bconv + j, t_cks, t_mod_up + (u64)j * LPN});
+            mu_bc_.push_back(BconvRow{L.modup_bconv + j, t_cks, t_mod_up + (u64)j * LPN, L.modup_fbconv + j});
             const u32 lo = alpha * j, hi = std::min(alpha * (j + 1), Ql);
             for (u32 pos = 0; pos < QlP; ++pos) {
                 if (pos >= lo && pos < hi) continue;
@@ -288,6 +290,8 @@ void BatchPlan::bind(u64 *arena) {
             seen_levels.push_back(it.chain);
             mu_desc_host_.insert(mu_desc_host_.end(), L.h_modup_bconv.begin(), L.h_modup_bconv.end());
             md_desc_host_.push_back(L.h_moddown_bconv);
+            mu_fdesc_host_.insert(mu_fdesc_host_.end(), L.h_modup_fbconv.begin(), L.h_modup_fbconv.end());
+            md_fdesc_host_.push_back(L.h_moddown_fbconv);
         }
         ip_.push_back(IpItem{t_mod_up, c2, cx, it.key, Ql, QlP, beta, it.chain});
         // moddown of both halves: INTT of the P limbs * PHatInv, FastBConv P -> Ql,
@@ -296,7 +300,7 @@ void BatchPlan::bind(u64 *arena) {
             for (u32 pj = 0; pj < P; ++pj)
                 md_inv_.push_back(NttJob{cx + (u64)h * LPN + (u64)(Ql + pj) * N, nullptr, nullptr, nullptr,
                                          L.moddown_scale + 4 * pj, size_Q + pj, 0});
-            md_bc_.push_back(BconvRow{L.moddown_bconv, cx + (u64)h * LPN, delta + (u64)h * LN});
+            md_bc_.push_back(BconvRow{L.moddown_bconv, cx + (u64)h * LPN, delta + (u64)h * LN, L.moddown_fbconv});
             for (u32 l = 0; l < Ql; ++l) {
                 const u64 *base = kind_ == kRotate && h == 0 ? base0 + (u64)l * N : nullptr;
                 md_fwd_.push_back(NttJob{delta + (u64)h * LN + (u64)l * N, cx + (u64)h * LPN + (u64)l * N,
@@ -641,8 +645,14 @@ void BatchPlan::record_keyswitch(hipStream_t s) {
     ba.cbuf = ctx.const_buf();
     ba.mod = ctx.dev_mod();
     ba.N = (u32)N;
-    bconv_run(ctx, mu_desc_host_, ba, (u32)mu_bc_.size(), s);
-    ntt_forward_jobs(T, reinterpret_cast<const NttJob *>(tab(off_mu_fwd_)), (u32)mu_fwd_.size(), s);
+    // FastBConv and the forward NTT of its outputs: fused with the column pass, or separate
+    if (bconv_fused_ok(ctx, mu_fdesc_host_)) {
+        bconv_fwd_cols_run(ctx, mu_fdesc_host_, ba, (u32)mu_bc_.size(), s);
+        ntt_forward_rows_jobs(T, reinterpret_cast<const NttJob *>(tab(off_mu_fwd_)), (u32)mu_fwd_.size(), s);
+    } else {
+        bconv_run(ctx, mu_desc_host_, ba, (u32)mu_bc_.size(), s);
+        ntt_forward_jobs(T, reinterpret_cast<const NttJob *>(tab(off_mu_fwd_)), (u32)mu_fwd_.size(), s);
+    }
     // key inner product
     auto ip = max_beta_ <= 8 ? k_inner_product_items<8, IpItem>
                              : max_beta_ <= 16 ? k_inner_product_items<16, IpItem> : k_inner_product_items<32, IpItem>;
@@ -657,8 +667,13 @@ void BatchPlan::record_keyswitch(hipStream_t s) {
     bd.cbuf = ctx.const_buf();
     bd.mod = ctx.dev_mod();
     bd.N = (u32)N;
-    bconv_run(ctx, md_desc_host_, bd, (u32)md_bc_.size(), s);
-    ntt_forward_jobs_moddown(T, reinterpret_cast<const NttJob *>(tab(off_md_fwd_)), (u32)md_fwd_.size(), s);
+    if (bconv_fused_ok(ctx, md_fdesc_host_)) {
+        bconv_fwd_cols_run(ctx, md_fdesc_host_, bd, (u32)md_bc_.size(), s);
+        ntt_forward_rows_jobs_moddown(T, reinterpret_cast<const NttJob *>(tab(off_md_fwd_)), (u32)md_fwd_.size(), s);
+    } else {
+        bconv_run(ctx, md_desc_host_, bd, (u32)md_bc_.size(), s);
+        ntt_forward_jobs_moddown(T, reinterpret_cast<const NttJob *>(tab(off_md_fwd_)), (u32)md_fwd_.size(), s);
+    }
 }
 
 void BatchPlan::record(hipStream_t s) {

@@ -90,7 +90,7 @@ inline void nn(const void *p, const char *what) {
 
 namespace {
 // tuning choice <-> the C array of PHX_TUNE_AXES entries {ntt_fused, ntt_fused_min_limbs,
-// ntt_chunk_limbs, bconv_mfma, ntt_cols_threads, ntt_rows_threads, bconv_wgs_per_cu}
+// ntt_chunk_limbs, bconv_mfma, ntt_cols_threads, ntt_rows_threads, bconv_wgs_per_cu, bconv_fused}
 TuneChoice choice_of(const int c[PHX_TUNE_AXES]) {
     TuneChoice t;
     t.ntt_fused = c[0];
@@ -100,6 +100,7 @@ TuneChoice choice_of(const int c[PHX_TUNE_AXES]) {
     t.ntt_cols_threads = c[4];
     t.ntt_rows_threads = c[5];
     t.bconv_wgs_per_cu = c[6];
+    t.bconv_fused = c[7];
     return t;
 }
 void put_choice(const TuneChoice &t, int c[PHX_TUNE_AXES]) {
@@ -110,6 +111,7 @@ void put_choice(const TuneChoice &t, int c[PHX_TUNE_AXES]) {
     c[4] = t.ntt_cols_threads;
     c[5] = t.ntt_rows_threads;
     c[6] = t.bconv_wgs_per_cu;
+    c[7] = t.bconv_fused;
 }
 // A ciphertext handed to an evaluator entry point must belong to the context:
 // its ring degree and limb count are the context's at its chain index (the

@@ -437,7 +437,7 @@ Context::Context(const Params &p) : params_(p) {
     const u32 alpha = size_P_;
     ConstBuilder cb;
     struct Offs {
-        size_t modup_scale, modup_bconv, moddown_scale, moddown_bconv, pinv, rescale;
+        size_t modup_scale, modup_bconv, moddown_scale, moddown_bconv, pinv, rescale, modup_fbconv, moddown_fbconv;
     };
     std::vector<Offs> offs(size_Q_ + 1);
     levels_.resize(size_Q_ + 1);
@@ -459,6 +459,40 @@ Context::Context(const Params &p) : params_(p) {
         d.pinv_off = cb.push64(pinv) / 8;
         d.rt = rt;
     };
+    // The same conversion for the launch fused with the forward NTT's column pass
+    // (bconv_ntt.hip): outputs on 64-bit integer butterflies first, then those on
+    // FP64 butterflies, each class padded to a multiple of four with outputs that
+    // are computed but not stored (pos ~0, a zero matrix row, the class's first
+    // prime), so that the four outputs of one workgroup share one arithmetic.
+    auto row_fp = [&](u32 row) { return !int_only && host_tables_[row].q < (1ull << 50); };
+    auto add_fused = [&](const BconvDesc &d, const std::vector<u64> &mat, const std::vector<u32> &pos,
+                         const std::vector<u32> &rows, const std::vector<u64> &om) {
+        BconvDesc f{};
+        if (d.ni > 16) return f;
+        std::vector<u32> order[2];
+        for (u32 o = 0; o < d.no; ++o) order[row_fp(rows[o]) ? 1 : 0].push_back(o);
+        std::vector<u64> fmat, fom;
+        std::vector<u32> fpos, frows;
+        for (const auto &ord : order) {
+            const size_t padded = (ord.size() + 3) / 4 * 4;
+            for (size_t k = 0; k < padded; ++k) {
+                const bool real = k < ord.size();
+                const u32 o = real ? ord[k] : ord[0];
+                for (u32 i = 0; i < d.ni; ++i) fmat.push_back(real ? mat[(size_t)o * d.ni + i] : 0);
+                fom.push_back(om[o]);
+                fpos.push_back(real ? pos[o] : 0xffffffffu);
+                frows.push_back(rows[o]);
+            }
+        }
+        f.ni = d.ni;
+        f.no = (u32)fpos.size();
+        f.in_first = d.in_first;
+        f.own_first = d.own_first;
+        f.pos_off = cb.push32(fpos) / 4;
+        f.row_off = cb.push32(frows) / 4;
+        add_mfma(f, fmat, fom);
+        return f;
+    };
     auto scale_consts = [&](u32 row, u64 scale) {
         const auto &t = host_tables_[row];
         const u64 s0 = hm::mulmod(t.ninv, scale, t.q);
@@ -478,7 +512,7 @@ Context::Context(const Params &p) : params_(p) {
 
         // modup: INTT scale partQlHatInv, then per digit FastBConv to the complement
         std::vector<u64> msc;
-        std::vector<BconvDesc> descs;
+        std::vector<BconvDesc> descs, fdescs;
         for (u32 j = 0; j < beta; ++j) {
             const u32 st = alpha * j, sz = (j == beta - 1) ? Ql - alpha * (beta - 1) : alpha;
             for (u32 i = st; i < st + sz; ++i) {
@@ -506,10 +540,13 @@ Context::Context(const Params &p) : params_(p) {
             for (u32 o = 0; o < no; ++o) om[o] = qlp[pos[o]];
             add_mfma(d, mat, om);
             descs.push_back(d);
+            fdescs.push_back(add_fused(d, mat, pos, rows, om));
         }
         offs[c].modup_scale = cb.push64(msc);
         offs[c].modup_bconv = cb.push_struct(descs);
+        offs[c].modup_fbconv = cb.push_struct(fdescs);
         L.h_modup_bconv = descs;
+        L.h_modup_fbconv = fdescs;
 
         // moddown: INTT of the P limbs scaled by PHatInv, FastBConv P -> Ql, P^-1
         std::vector<u64> dsc;
@@ -535,9 +572,12 @@ Context::Context(const Params &p) : params_(p) {
             d.mat_off = cb.push64(mat) / 8;
             d.pos_off = cb.push32(pos) / 4;
             d.row_off = cb.push32(rows) / 4;
-            add_mfma(d, mat, std::vector<u64>(qp, qp + Ql));
+            const std::vector<u64> om(qp, qp + Ql);
+            add_mfma(d, mat, om);
             offs[c].moddown_bconv = cb.push_struct(std::vector<BconvDesc>{d});
             L.h_moddown_bconv = d;
+            L.h_moddown_fbconv = add_fused(d, mat, pos, rows, om);
+            offs[c].moddown_fbconv = cb.push_struct(std::vector<BconvDesc>{L.h_moddown_fbconv});
         }
         std::vector<u64> pinv;
         for (u32 i = 0; i < Ql; ++i) {
@@ -761,6 +801,8 @@ Context::Context(const Params &p) : params_(p) {
         L.modup_bconv = reinterpret_cast<const BconvDesc *>(cbb + offs[c].modup_bconv);
         L.moddown_scale = reinterpret_cast<const u64 *>(cbb + offs[c].moddown_scale);
         L.moddown_bconv = reinterpret_cast<const BconvDesc *>(cbb + offs[c].moddown_bconv);
+        L.modup_fbconv = reinterpret_cast<const BconvDesc *>(cbb + offs[c].modup_fbconv);
+        L.moddown_fbconv = reinterpret_cast<const BconvDesc *>(cbb + offs[c].moddown_fbconv);
         L.pinv = reinterpret_cast<const u64 *>(cbb + offs[c].pinv);
         L.rescale_qinv = reinterpret_cast<const u64 *>(cbb + offs[c].rescale);
     }
@@ -823,6 +865,7 @@ void Context::set_tuning(const TuneChoice &t) {
     require(t.ntt_fused == -1 || t.ntt_fused == 0 || t.ntt_fused == 1 || t.ntt_fused == 2 || t.ntt_fused == 4,
             "ntt_fused must be -1, 0, 1, 2 or 4");
     require(t.bconv_mfma >= -1 && t.bconv_mfma <= 1, "bconv_mfma must be -1, 0 or 1");
+    require(t.bconv_fused >= -1 && t.bconv_fused <= 1, "bconv_fused must be -1, 0 or 1");
     require(t.ntt_chunk_limbs >= -1 && t.ntt_fused_min_limbs >= -1, "limb counts must be -1 or non-negative");
     require(t.ntt_cols_threads == -1 || t.ntt_cols_threads == 256 || t.ntt_cols_threads == 512,
             "ntt_cols_threads must be -1, 256 or 512");

@@ -66,6 +66,12 @@ struct LevelConsts {
     const u64 *rescale_qinv = nullptr;    // [Ql-1][2] q_last^-1 mod q_j (Shoup)
     std::vector<BconvDesc> h_modup_bconv; // host copies
     BconvDesc h_moddown_bconv{};
+    // the same conversions for the launch fused with the forward NTT's column pass
+    // (outputs grouped by butterfly arithmetic, see bconv_ntt.hip; rt = 0: none)
+    const BconvDesc *modup_fbconv = nullptr;   // [beta]
+    const BconvDesc *moddown_fbconv = nullptr; // [1]
+    std::vector<BconvDesc> h_modup_fbconv;
+    BconvDesc h_moddown_fbconv{};
 };
 
 // BEHZ constants of the top data level (BFV multiply; reference DRNSTool ctor,
@@ -150,6 +156,7 @@ struct TuneChoice {
     int ntt_cols_threads = -1;   // tile shape of the column passes: 256 or 512 threads
     int ntt_rows_threads = -1;   // tile shape of the row passes: 128, 256 or 512 threads
     int bconv_wgs_per_cu = -1;   // FastBConv (matrix cores) workgroups per CU: 1, 2, 4 (0 = as LDS allows)
+    int bconv_fused = -1;        // 1: FastBConv fused with the forward NTT's column pass, 0: separate
 };
 
 class Context {

@@ -58,6 +58,7 @@ struct BconvRow {
     const BconvDesc *desc;
     const u64 *in;
     u64 *out;
+    const BconvDesc *fdesc;   // the same conversion's fused form (bconv_ntt.hip)
 };
 
 // FastBConv launch arguments. Grid row y = batch * jobs_per_batch + job; job j
@@ -82,6 +83,14 @@ struct BconvArgs {
 // an MFMA form (and the bconv_mfma switch is on), the VALU kernel otherwise
 void bconv_run(const Context &ctx, const std::vector<BconvDesc> &descs_host, const BconvArgs &args, u32 jobs,
                hipStream_t s);
+// FastBConv fused with the column pass of the forward NTT of its outputs
+// (bconv_ntt.hip): fdescs are the fused descriptors the launch uses (args.descs /
+// args.rows[].fdesc); the outputs are left as the column pass leaves them, for the
+// row pass (ntt_forward_rows*). bconv_fused_ok says whether the fused launch serves
+// these conversions on this context; copy_src is not supported.
+bool bconv_fused_ok(const Context &ctx, const std::vector<BconvDesc> &fdescs);
+void bconv_fwd_cols_run(const Context &ctx, const std::vector<BconvDesc> &fdescs, const BconvArgs &args, u32 jobs,
+                        hipStream_t s);
 
 // ---- hot path ----
 void tensor_2x2(const Context &ctx, u32 Ql, const u64 *a, const u64 *b, u64 *d0, u64 *d1, u64 *d2, hipStream_t s);

@@ -302,9 +302,8 @@ void bconv_run(const Context &ctx, const std::vector<BconvDesc> &descs_host, con
 }
 
 namespace {
-void bconv_launch(const Context &ctx, const BconvDesc *descs_dev, const std::vector<BconvDesc> &descs_host,
-                  u32 desc_stride, u32 jobs, const u64 *in, u64 in_job_stride, u64 *out, u64 out_job_stride,
-                  const u64 *copy_src, u64 copy_job_stride, hipStream_t s) {
+BconvArgs bconv_args(const Context &ctx, const BconvDesc *descs_dev, u32 desc_stride, u32 jobs, const u64 *in,
+                     u64 in_job_stride, u64 *out, u64 out_job_stride, const u64 *copy_src, u64 copy_job_stride) {
     BconvArgs a{};
     a.cbuf = ctx.const_buf();
     a.descs = descs_dev;
@@ -318,7 +317,15 @@ void bconv_launch(const Context &ctx, const BconvDesc *descs_dev, const std::vec
     a.copy_job_stride = copy_job_stride;
     a.mod = ctx.dev_mod();
     a.N = (u32)ctx.N();
-    bconv_run(ctx, descs_host, a, jobs, s);
+    return a;
+}
+void bconv_launch(const Context &ctx, const BconvDesc *descs_dev, const std::vector<BconvDesc> &descs_host,
+                  u32 desc_stride, u32 jobs, const u64 *in, u64 in_job_stride, u64 *out, u64 out_job_stride,
+                  const u64 *copy_src, u64 copy_job_stride, hipStream_t s) {
+    bconv_run(ctx, descs_host,
+              bconv_args(ctx, descs_dev, desc_stride, jobs, in, in_job_stride, out, out_job_stride, copy_src,
+                         copy_job_stride),
+              jobs, s);
 }
 }  // namespace
 
@@ -658,10 +665,7 @@ void modup_ws(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, u64 *
         ntt_inverse_tensor(ctx.tables(), mi, Ql, *tensor, s);
     else
         ntt_inverse(ctx.tables(), mi, Ql, 1, s);
-    // FastBConv of every digit to its complement, own limbs copied
-    bconv_launch(ctx, L.modup_bconv, L.h_modup_bconv, 1u, beta, t_cks, 0, t_mod_up, (u64)QlP * N,
-                 copy_own ? c2 : nullptr, 0, s);
-    // forward NTT of the complement limbs of every digit
+    // the complement limbs of every digit in t_mod_up
     LimbMap mf = limb_map(t_mod_up, 0, QlP, (u64)QlP * N);
     mf.split = Ql;
     mf.special_row = ctx.size_Q();
@@ -669,6 +673,19 @@ void modup_ws(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, u64 *
     mf.skip_cap = Ql;
     mf.nlimbs = QlP;
     const u32 last_sz = Ql - alpha * (beta - 1);
+    if (!copy_own && bconv_fused_ok(ctx, L.h_modup_fbconv)) {
+        // FastBConv of every digit to its complement fused with the NTT's column
+        // pass, then the row pass
+        bconv_fwd_cols_run(ctx, L.h_modup_fbconv,
+                           bconv_args(ctx, L.modup_fbconv, 1u, beta, t_cks, 0, t_mod_up, (u64)QlP * N, nullptr, 0),
+                           beta, s);
+        ntt_forward_rows(ctx.tables(), mf, QlP - last_sz, beta, s);
+        return;
+    }
+    // FastBConv of every digit to its complement, own limbs copied, then the
+    // forward NTT of the complements
+    bconv_launch(ctx, L.modup_bconv, L.h_modup_bconv, 1u, beta, t_cks, 0, t_mod_up, (u64)QlP * N,
+                 copy_own ? c2 : nullptr, 0, s);
     ntt_forward(ctx.tables(), mf, QlP - last_sz, beta, s);
 }
 
@@ -700,8 +717,15 @@ void moddown_ws(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base
     LimbMap mi = limb_map(cx + (u64)Ql * N, ctx.size_Q(), P, (u64)QlP * N);
     mi.scale = L.moddown_scale;
     ntt_inverse(ctx.tables(), mi, P, npolys, s);
-    bconv_launch(ctx, L.moddown_bconv, {L.h_moddown_bconv}, 0u, npolys, cx, (u64)QlP * N, delta, (u64)Ql * N,
-                 nullptr, 0, s);
+    const bool fused = bconv_fused_ok(ctx, {L.h_moddown_fbconv});
+    if (fused)
+        bconv_fwd_cols_run(ctx, {L.h_moddown_fbconv},
+                           bconv_args(ctx, L.moddown_fbconv, 0u, npolys, cx, (u64)QlP * N, delta, (u64)Ql * N,
+                                      nullptr, 0),
+                           npolys, s);
+    else
+        bconv_launch(ctx, L.moddown_bconv, {L.h_moddown_bconv}, 0u, npolys, cx, (u64)QlP * N, delta, (u64)Ql * N,
+                     nullptr, 0, s);
     StoreModDown st{};
     st.ct = ct;
     st.base0 = base0;
@@ -711,7 +735,10 @@ void moddown_ws(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base
     st.ct_stride = (u64)Ql * N;
     st.cx_stride = (u64)QlP * N;
     st.N = (u32)N;
-    ntt_forward_store(ctx.tables(), limb_map(delta, 0, Ql, (u64)Ql * N), Ql, npolys, st, s);
+    if (fused)
+        ntt_forward_rows_store(ctx.tables(), limb_map(delta, 0, Ql, (u64)Ql * N), Ql, npolys, st, s);
+    else
+        ntt_forward_store(ctx.tables(), limb_map(delta, 0, Ql, (u64)Ql * N), Ql, npolys, st, s);
 }
 
 void moddown(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base0, const u64 *base1, u32 npolys,

@@ -179,6 +179,9 @@ struct LoadTensorJob {
 void ntt_forward_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
 void ntt_forward_jobs_moddown(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
 void ntt_forward_jobs_rescale(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
+// the row pass alone, over entries whose column pass ran fused with FastBConv
+void ntt_forward_rows_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
+void ntt_forward_rows_jobs_moddown(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
 // inverse: entries with aux0 read their first pass from aux0 (out of place)
 void ntt_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
 // inverse over entries whose first pass loads the tensor product (LoadTensorJob)
@@ -188,6 +191,10 @@ void ntt_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, h
 void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s);
 void ntt_forward_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const StoreModDown &st,
                        hipStream_t s);
+// the row pass alone, over limbs whose column pass ran fused with FastBConv
+void ntt_forward_rows(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s);
+void ntt_forward_rows_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const StoreModDown &st,
+                            hipStream_t s);
 void ntt_forward_rescale(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const LoadRescale &ld,
                          const StoreRescale &st, hipStream_t s);
 // inverse of one polynomial whose first pass loads the tensor product (always two passes)

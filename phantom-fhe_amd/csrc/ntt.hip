@@ -49,6 +49,24 @@ __global__ __launch_bounds__(NT) void k_fwd_rows(NttTables T, LimbMap M, Store s
         fwd_rows_body<NT>(int_fwd(T, row, md.value), lds, data, T.logN, r0, limb, z, md, st, threadIdx.x);
 }
 
+// row pass after the FastBConv-fused column pass (its remaining column stages first)
+template <int SR, int NT, class Store>
+__global__ __launch_bounds__(NT) void k_fwd_rows_rb(NttTables T, LimbMap M, Store st) {
+    __shared__ u64 lds[(NT / 16) * 272];
+    const u32 z = blockIdx.y + M.poly0;
+    const u32 limb = M.limb(blockIdx.z, z);
+    if (limb >= M.nlimbs) return;
+    const u32 row = M.row(limb);
+    u64 *data = M.data + z * M.poly_stride + (u64)limb * T.N;
+    const DevModulus md = T.mod[row];
+    const u32 r0 = blockIdx.x * (NT / 16);
+    if (md.pad)
+        fwd_rows_body_rb<SR, NT>(fp_arith(T.twd + (u64)row * T.N, md), lds, data, T.logN, r0, limb, z, md, st,
+                                 threadIdx.x);
+    else
+        fwd_rows_body_rb<SR, NT>(int_fwd(T, row, md.value), lds, data, T.logN, r0, limb, z, md, st, threadIdx.x);
+}
+
 template <int NT, class Load>
 __global__ __launch_bounds__(NT) void k_inv_rows(NttTables T, LimbMap M, Load ld) {
     __shared__ u64 lds[(NT / 16) * 272];
@@ -147,6 +165,33 @@ void launch_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys
 
 void ntt_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s) {
     launch_forward(T, M, nlimbs, npolys, LoadPlain{}, StorePlain{}, s);
+}
+
+namespace {
+// the row pass alone, over limbs whose column pass ran fused with FastBConv
+// (bconv_ntt.hip: its first register round only; the rest runs here)
+template <class Store>
+void launch_forward_rows(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const Store &st,
+                         hipStream_t s) {
+    if (nlimbs == 0 || npolys == 0) return;
+    const u32 nt = rows_threads(T) == 512 ? 512 : 256;
+    dispatch_rows_rb(T.logN, nt, [&](auto sr, auto ntc) {
+        constexpr int SR = decltype(sr)::value;
+        constexpr int NT = decltype(ntc)::value;
+        hipLaunchKernelGGL((k_fwd_rows_rb<SR, NT, Store>), dim3(T.N / (NT * kNttRegs), npolys, nlimbs), dim3(NT), 0,
+                           s, T, M, st);
+    });
+    PHX_HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+void ntt_forward_rows(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s) {
+    launch_forward_rows(T, M, nlimbs, npolys, StorePlain{}, s);
+}
+
+void ntt_forward_rows_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const StoreModDown &st,
+                            hipStream_t s) {
+    launch_forward_rows(T, M, nlimbs, npolys, st, s);
 }
 
 namespace {

@@ -155,7 +155,8 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
 // Forward pass 2 core: the last 8 stages (distances 128 .. 1) on 16 rows;
 // returns the canonical outputs at tile index t + 256 k in out[k]. The LDS
 // region is free again after the caller's next barrier.
-template <int NT, class A>
+// FromLds: the tile's values are in LDS (row_addr layout) instead of in data.
+template <int NT, class A, bool FromLds = false>
 __device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *data, u32 logN, u32 r0, const u32 t,
                                               u64 (&out)[16]) {
     using T = typename A::T;
@@ -163,7 +164,8 @@ __device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *d
     T v[16];
     u32 baseA[1] = {(r0 + u) * 256 + l};
 #pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = A::from_bits(ld_mid(data + baseA[0] + ((u32)e << 4)));
+    for (int e = 0; e < 16; ++e)
+        v[e] = A::from_bits(FromLds ? lds[row_addr(u, ((u32)e << 4) + l)] : ld_mid(data + baseA[0] + ((u32)e << 4)));
     fwd_round<4, 4>(v, baseA, logN, a);
 #pragma unroll
     for (int e = 0; e < 16; ++e) lds[row_addr(u, ((u32)e << 4) + l)] = A::to_bits(v[e]);
@@ -182,6 +184,42 @@ __device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *d
         const u32 idx = t + NT * k;
         out[k] = lds[row_addr(idx >> 8, idx & 255)];
     }
+}
+
+// Forward pass 2 over a tile whose column pass ran only its first register
+// round (the column pass fused with FastBConv, bconv_ntt.hip): the column
+// pass's remaining SR - 4 stages (distances 2^(SR-5+8) .. 2^8, i.e. within
+// groups of 2^(SR-4) consecutive rows) on the tile's columns, then the last 8
+// stages as fwd_rows_core. Thread t holds column t % 256 of rows
+// r0 + 16 (t / 256) + e (NT = 256 or 512: 16 rows per 256 threads).
+template <int SR, int NT, class A>
+__device__ __forceinline__ void fwd_rows_core_rb(const A &a, u64 *lds, const u64 *data, u32 logN, u32 r0,
+                                                 const u32 t, u64 (&out)[16]) {
+    static_assert(NT % 256 == 0 && SR >= 5 && SR <= 8, "column-first row pass: 256 or 512 threads, N = 2^13 .. 2^16");
+    using T = typename A::T;
+    constexpr int NB = SR - 4;
+    constexpr int BB = 16 >> NB;
+    const u32 cc = t & 255, rb = r0 + 16 * (t >> 8);
+    T v[16];
+    u32 base[BB];
+#pragma unroll
+    for (int b = 0; b < BB; ++b) base[b] = (rb + ((u32)b << NB)) * 256 + cc;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = A::from_bits(ld_mid(data + (rb + (u32)e) * 256 + cc));
+    fwd_round<NB, 8>(v, base, logN, a);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) lds[row_addr(16 * (t >> 8) + (u32)e, cc)] = A::to_bits(v[e]);
+    __syncthreads();
+    fwd_rows_core<NT, A, true>(a, lds, data, logN, r0, t, out);
+}
+
+template <int SR, int NT, class A, class Store>
+__device__ __forceinline__ void fwd_rows_body_rb(const A &a, u64 *lds, u64 *data, u32 logN, u32 r0, u32 limb, u32 z,
+                                                 const DevModulus &md, const Store &st, const u32 t) {
+    u64 out[16];
+    fwd_rows_core_rb<SR, NT>(a, lds, data, logN, r0, t, out);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) st(data, r0 * 256 + t + NT * k, out[k], limb, z, md);
 }
 
 // Forward pass 2 body: the core, then the store policy per output.
@@ -385,6 +423,25 @@ void dispatch_cols(u32 logN, u32 nt, F &&f) {
             f(sr, std::integral_constant<int, 256>{});
     });
 }
+// row passes after the FastBConv-fused column pass: (SR, NT) for N = 2^13 .. 2^16
+// and 256 or 512 threads
+template <class F>
+void dispatch_rows_rb(u32 logN, u32 nt, F &&f) {
+    auto go = [&](auto sr) {
+        if (nt == 512)
+            f(sr, std::integral_constant<int, 512>{});
+        else
+            f(sr, std::integral_constant<int, 256>{});
+    };
+    switch (logN) {
+        case 13: go(std::integral_constant<int, 5>{}); break;
+        case 14: go(std::integral_constant<int, 6>{}); break;
+        case 15: go(std::integral_constant<int, 7>{}); break;
+        case 16: go(std::integral_constant<int, 8>{}); break;
+        default: fail(kInvalidArgument, "fused FastBConv row pass: unsupported polynomial degree");
+    }
+}
+
 template <class F>
 void dispatch_rows_nt(u32 nt, F &&f) {
     if (nt == 512)

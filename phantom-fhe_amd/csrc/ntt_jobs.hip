@@ -35,6 +35,22 @@ __global__ __launch_bounds__(NT) void k_fwd_rows_jobs(NttTables T, const NttJob 
         fwd_rows_body<NT>(int_fwd(T, J.row, md.value), lds, J.data, T.logN, r0, 0u, 0u, md, sb, threadIdx.x);
 }
 
+// row pass after the FastBConv-fused column pass (its remaining column stages first)
+template <int SR, int NT, class Store>
+__global__ __launch_bounds__(NT) void k_fwd_rows_rb_jobs(NttTables T, const NttJob *jobs, Store st) {
+    __shared__ u64 lds[(NT / 16) * 272];
+    const NttJob J = jobs[blockIdx.y];
+    const DevModulus md = T.mod[J.row];
+    const auto sb = bind_job(st, J);
+    const u32 r0 = blockIdx.x * (NT / 16);
+    if (md.pad)
+        fwd_rows_body_rb<SR, NT>(fp_arith(T.twd + (u64)J.row * T.N, md), lds, J.data, T.logN, r0, 0u, 0u, md, sb,
+                                 threadIdx.x);
+    else
+        fwd_rows_body_rb<SR, NT>(int_fwd(T, J.row, md.value), lds, J.data, T.logN, r0, 0u, 0u, md, sb,
+                                 threadIdx.x);
+}
+
 template <int NT, class Load>
 __global__ __launch_bounds__(NT) void k_inv_rows_jobs(NttTables T, const NttJob *jobs, Load ld) {
     __shared__ u64 lds[(NT / 16) * 272];
@@ -105,6 +121,33 @@ void ntt_forward_jobs_moddown(const NttTables &T, const NttJob *jobs, u32 njobs,
 
 void ntt_forward_jobs_rescale(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s) {
     launch_forward_jobs(T, jobs, njobs, LoadRescaleJob{}, StoreRescaleJob{}, s);
+}
+
+namespace {
+// the row pass alone, over entries whose column pass ran fused with FastBConv
+// (bconv_ntt.hip: its first register round only; the rest runs here)
+template <class Store>
+void launch_forward_rows_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, const Store &st, hipStream_t s) {
+    const u32 nt = rows_threads(T) == 512 ? 512 : 256;
+    for (u32 j0 = 0; j0 < njobs; j0 += kMaxJobsPerLaunch) {
+        const u32 nj = std::min(kMaxJobsPerLaunch, njobs - j0);
+        dispatch_rows_rb(T.logN, nt, [&](auto sr, auto ntc) {
+            constexpr int SR = decltype(sr)::value;
+            constexpr int NT = decltype(ntc)::value;
+            hipLaunchKernelGGL((k_fwd_rows_rb_jobs<SR, NT, Store>), dim3(T.N / (NT * kNttRegs), nj), dim3(NT), 0, s,
+                               T, jobs + j0, st);
+        });
+    }
+    PHX_HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+void ntt_forward_rows_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s) {
+    launch_forward_rows_jobs(T, jobs, njobs, StorePlain{}, s);
+}
+
+void ntt_forward_rows_jobs_moddown(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s) {
+    launch_forward_rows_jobs(T, jobs, njobs, StoreModDownJob{}, s);
 }
 
 namespace {

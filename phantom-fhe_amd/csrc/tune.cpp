@@ -14,6 +14,9 @@
 //                128, 256 (default) or 512
 //   bconv_wgs_per_cu  FastBConv on the matrix cores: workgroups per CU of its grid
 //                (1, 2, 4; 0 = as many as the LDS holds, the default)
+//   bconv_fused  1: FastBConv on the matrix cores fused with the column pass of the
+//                forward NTT of its outputs (default, N = 2^13 .. 2^16), 0: separate
+//                launches
 //   batch_branches  concurrent branches of a persistent batch plan (1..8, default
 //                2): plans of at least twice as many items are split into that many
 //                parts that run on forked streams, and joined plans spread their
@@ -41,6 +44,7 @@ std::atomic<int> g_ntt_cols{256};
 std::atomic<int> g_ntt_rows{256};
 std::atomic<int> g_bconv_wgs{0};
 std::atomic<int> g_batch_branches{2};
+std::atomic<int> g_bconv_fused{1};
 
 std::atomic<int> *slot(const char *key) {
     if (!key) return nullptr;
@@ -53,6 +57,7 @@ std::atomic<int> *slot(const char *key) {
     if (!std::strcmp(key, "ntt_rows_threads")) return &g_ntt_rows;
     if (!std::strcmp(key, "bconv_wgs_per_cu")) return &g_bconv_wgs;
     if (!std::strcmp(key, "batch_branches")) return &g_batch_branches;
+    if (!std::strcmp(key, "bconv_fused")) return &g_bconv_fused;
     return nullptr;
 }
 }  // namespace

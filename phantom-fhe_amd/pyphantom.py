@@ -971,7 +971,7 @@ class Event:
 
 
 TUNE_KEYS = ("ntt_fused", "ntt_fused_min_limbs", "ntt_chunk_limbs", "bconv_mfma", "ntt_cols_threads",
-             "ntt_rows_threads", "bconv_wgs_per_cu")
+             "ntt_rows_threads", "bconv_wgs_per_cu", "bconv_fused")
 
 
 class _TuneReport(ctypes.Structure):

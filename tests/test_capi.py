@@ -69,9 +69,9 @@ def test_tune_cache_roundtrip(tmp_path):
     path = str(tmp_path / "tune.json")
     assert ph.tune_cache_lookup(path, ph.SCHEME_CKKS, 65536, 35, 12) is None
     a = dict(ntt_fused=0, ntt_fused_min_limbs=0, ntt_chunk_limbs=256, bconv_mfma=1, ntt_cols_threads=512,
-             ntt_rows_threads=128, bconv_wgs_per_cu=2)
+             ntt_rows_threads=128, bconv_wgs_per_cu=2, bconv_fused=0)
     b = dict(ntt_fused=2, ntt_fused_min_limbs=64, ntt_chunk_limbs=0, bconv_mfma=0, ntt_cols_threads=-1,
-             ntt_rows_threads=256, bconv_wgs_per_cu=0)
+             ntt_rows_threads=256, bconv_wgs_per_cu=0, bconv_fused=1)
     ph.tune_cache_store(path, ph.SCHEME_CKKS, 65536, 35, 12, a, 340.5)
     ph.tune_cache_store(path, ph.SCHEME_BFV, 8192, 4, 1, b, 12.0)
     assert ph.tune_cache_lookup(path, ph.SCHEME_CKKS, 65536, 35, 12) == a
@@ -91,6 +91,7 @@ def test_tune_cache_roundtrip(tmp_path):
                 '"ntt_chunk_limbs": 0, "bconv_mfma": 1, "hmult_relin_us": 10}\n]}\n')
     old = ph.tune_cache_lookup(path, ph.SCHEME_CKKS, 4096, 2, 1)
     assert old["ntt_fused"] == 1 and old["ntt_cols_threads"] == old["ntt_rows_threads"] == -1
+    assert old["bconv_fused"] == -1
     with open(path, "w") as f:
         f.write('{"entries": [\n  {"key": "scheme=ckks,N=4096,Q=2,P=1", "ntt_fused": 1}\n]}\n')
     with pytest.raises(ValueError):

@@ -45,11 +45,12 @@ def test_autotune_applies_and_caches(tmp_path):
     assert ch["ntt_chunk_limbs"] >= 0 and ch["ntt_fused_min_limbs"] >= 0
     assert ch["ntt_cols_threads"] in (256, 512) and ch["ntt_rows_threads"] in (128, 256, 512)
     assert ch["bconv_wgs_per_cu"] in (-1, 0, 1, 2)
+    assert ch["bconv_fused"] in (-1, 0, 1)
     assert rep["trials"] >= 12 and rep["best_us"] > 0 and rep["default_us"] > 0
     # the report lists every candidate with its timing, each axis among them
     assert len(rep["timings"]) == rep["trials"]
     for axis in ("ntt_rows_threads=128", "ntt_rows_threads=512", "ntt_cols_threads=512", "bconv_mfma=0",
-                 "bconv_wgs_per_cu=1", "ntt_fused=2", "ntt_chunk_limbs=128"):
+                 "bconv_wgs_per_cu=1", "ntt_fused=2", "ntt_chunk_limbs=128", "bconv_fused=0"):
         assert any(axis in t["choice"] for t in rep["timings"]), axis
     # the winner is no slower than the default beyond timing noise
     assert rep["best_us"] <= 1.1 * rep["default_us"]
@@ -74,6 +75,9 @@ def test_autotune_applies_and_caches(tmp_path):
     dict(ntt_fused=4, ntt_fused_min_limbs=0, ntt_chunk_limbs=0, bconv_mfma=1),
     dict(ntt_fused=0, ntt_cols_threads=512, ntt_rows_threads=128, bconv_mfma=1, bconv_wgs_per_cu=1),
     dict(ntt_fused=0, ntt_cols_threads=256, ntt_rows_threads=512, bconv_mfma=1, bconv_wgs_per_cu=2),
+    dict(ntt_fused=0, ntt_rows_threads=128, bconv_mfma=1, bconv_fused=1),
+    dict(ntt_fused=0, ntt_rows_threads=512, bconv_mfma=1, bconv_fused=1),
+    dict(ntt_fused=0, bconv_mfma=1, bconv_fused=0),
 ])
 def test_per_context_choice_bit_exact(choice):
     """a context's own choice drives its launches (process-wide switches untouched),
@@ -140,6 +144,8 @@ def test_set_tuning_rejects():
         ctx.set_tuning(ntt_rows_threads=1024)
     with pytest.raises(ValueError):
         ctx.set_tuning(bconv_wgs_per_cu=3)
+    with pytest.raises(ValueError):
+        ctx.set_tuning(bconv_fused=2)
 
 
 def test_autotune_bfv_leveled_context():
