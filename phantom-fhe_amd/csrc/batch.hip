@@ -635,10 +635,13 @@ void BatchPlan::record_keyswitch(hipStream_t s) {
     const NttTables &T = ctx.tables();
     auto tab = [&](size_t off) { return tables_bound_ + off; };
     // modup (for a multiplication the tensor product is the INTT's first load)
+    // the fused FastBConv takes its inputs byte-flipped from the INTT's store
+    const bool mu_fused = bconv_fused_ok(ctx, mu_fdesc_host_), md_fused = bconv_fused_ok(ctx, md_fdesc_host_);
     if (kind_ == kMulRelin)
-        ntt_inverse_jobs_tensor(T, reinterpret_cast<const NttJob *>(tab(off_mu_inv_)), (u32)mu_inv_.size(), s);
+        ntt_inverse_jobs_tensor(T, reinterpret_cast<const NttJob *>(tab(off_mu_inv_)), (u32)mu_inv_.size(), s,
+                                mu_fused);
     else
-        ntt_inverse_jobs(T, reinterpret_cast<const NttJob *>(tab(off_mu_inv_)), (u32)mu_inv_.size(), s);
+        ntt_inverse_jobs(T, reinterpret_cast<const NttJob *>(tab(off_mu_inv_)), (u32)mu_inv_.size(), s, mu_fused);
     require(mu_bc_.size() <= kMaxGridY && md_bc_.size() <= kMaxGridY, "too many base conversions in one batch");
     BconvArgs ba{};
     ba.rows = reinterpret_cast<const BconvRow *>(tab(off_mu_bc_));
@@ -646,7 +649,7 @@ void BatchPlan::record_keyswitch(hipStream_t s) {
     ba.mod = ctx.dev_mod();
     ba.N = (u32)N;
     // FastBConv and the forward NTT of its outputs: fused with the column pass, or separate
-    if (bconv_fused_ok(ctx, mu_fdesc_host_)) {
+    if (mu_fused) {
         bconv_fwd_cols_run(ctx, mu_fdesc_host_, ba, (u32)mu_bc_.size(), s);
         ntt_forward_rows_jobs(T, reinterpret_cast<const NttJob *>(tab(off_mu_fwd_)), (u32)mu_fwd_.size(), s);
     } else {
@@ -661,13 +664,13 @@ void BatchPlan::record_keyswitch(hipStream_t s) {
                        ctx.size_Q(), ctx.size_QP(), ctx.size_P());
     PHX_HIP_CHECK(hipGetLastError());
     // moddown
-    ntt_inverse_jobs(T, reinterpret_cast<const NttJob *>(tab(off_md_inv_)), (u32)md_inv_.size(), s);
+    ntt_inverse_jobs(T, reinterpret_cast<const NttJob *>(tab(off_md_inv_)), (u32)md_inv_.size(), s, md_fused);
     BconvArgs bd{};
     bd.rows = reinterpret_cast<const BconvRow *>(tab(off_md_bc_));
     bd.cbuf = ctx.const_buf();
     bd.mod = ctx.dev_mod();
     bd.N = (u32)N;
-    if (bconv_fused_ok(ctx, md_fdesc_host_)) {
+    if (md_fused) {
         bconv_fwd_cols_run(ctx, md_fdesc_host_, bd, (u32)md_bc_.size(), s);
         ntt_forward_rows_jobs_moddown(T, reinterpret_cast<const NttJob *>(tab(off_md_fwd_)), (u32)md_fwd_.size(), s);
     } else {

@@ -43,7 +43,7 @@ struct FcShape {
 };
 
 // the column pass's first register round on one lane's 16 values (rows
-// low + e R/16 of one column, canonical), stored as the intermediate
+// low + e R/16 of one column, residues below 2q), stored as the intermediate
 template <int SR, class Ar>
 __device__ __forceinline__ void fc_round_a(const Ar &a, const u64 (&raw)[16], u64 *outp, bool store, u32 logN,
                                            u32 base) {
@@ -111,34 +111,36 @@ __global__ __launch_bounds__(FcShape<SR>::NT, 4) void k_bconv_fwd_cols(BconvArgs
             a[s][h] = v4i{(int)t.x, (int)t.y, (int)t.z, (int)t.w};
         }
     // lane (lg, ln) feeds K = 64h + 16 lg + j: the bytes of inputs 8h + 2 lg + r
-    // (r < 2) at column c0 + ln, row low + e R/16 in step e. Inputs beyond ni
-    // enter as zero, read from input ni - 1 so that every lane issues the same loads.
+    // (r < 2) at column c0 + ln, row low + e R/16 in step e. An input beyond ni
+    // meets zero operand digits (its products vanish whatever its bytes), so such
+    // a lane reads input ni - 1 instead. Per lane the input offsets are fixed and
+    // per step the base advances by R/16 rows (a uniform address, so the step
+    // costs no vector instructions).
     const u32 base = low * 256 + c0 + ln;
-    const PHX_GLOBAL u64 *src[2][2];
-    bool have[2][2];
+    const u64 *in0 = inb + (u64)d.in_first * N;
+    u32 voff[2][2];
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             const u32 i = 8 * h + 2 * lg + r;
-            have[h][r] = i < ni;
-            src[h][r] = gmem(inb + (u64)(d.in_first + (i < ni ? i : ni - 1)) * N + base);
+            voff[h][r] = ((i < ni ? i : ni - 1) * N + base) * 8;   // bytes, below 2^32
         }
     u64 xb[2][2][2];   // inputs of two steps in flight
     auto load_x = [&](u64(&x)[2][2], int e) {
+        const auto sb = reinterpret_cast<const PHX_GLOBAL uint8_t *>(gmem(in0 + (u64)e * (S::LOWS * 256)));
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int r = 0; r < 2; ++r) x[h][r] = src[h][r][(u32)e * (S::LOWS * 256)];
+            for (int r = 0; r < 2; ++r) x[h][r] = *reinterpret_cast<const PHX_GLOBAL u64 *>(sb + voff[h][r]);
     };
+    // the inputs arrive with bit 7 of every byte flipped (StoreFlipped): their bytes
+    // are the signed int8 operands s = byte - 128 as they are
     auto products = [&](const u64(&x)[2][2], v4i(&acc)[2]) {
         v4i b[2];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const u64 x0 = have[h][0] ? x[h][0] : 0, x1 = have[h][1] ? x[h][1] : 0;
-            b[h] = v4i{(int)((u32)x0 ^ 0x80808080u), (int)((u32)(x0 >> 32) ^ 0x80808080u),
-                       (int)((u32)x1 ^ 0x80808080u), (int)((u32)(x1 >> 32) ^ 0x80808080u)};
-        }
+        for (int h = 0; h < 2; ++h)
+            b[h] = v4i{(int)(u32)x[h][0], (int)(u32)(x[h][0] >> 32), (int)(u32)x[h][1], (int)(u32)(x[h][1] >> 32)};
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             v4i c = {0, 0, 0, 0};
@@ -146,7 +148,10 @@ __global__ __launch_bounds__(FcShape<SR>::NT, 4) void k_bconv_fwd_cols(BconvArgs
             acc[s] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[s][1], b[1], c, 0, 0, 0);
         }
     };
-    // T = sum_b 256^b Z_b + C_o in [0, 2^15 p) -> [T]_p, as k_bconv_mfma's reduction
+    // T = sum_b 256^b Z_b + C_o in [0, 2^15 p) -> r = T - floor(T/p - 2^-30) p in
+    // [0, 2p), as k_bconv_mfma's reduction without its final subtraction: the
+    // butterflies take inputs below 2p (the integer ones below 4p, the FP64 ones
+    // below 2^52, and 2p < 2^51 there)
     auto reduce = [&](const v4i(&acc)[2]) -> u64 {
         const int y0 = acc[0][0] + acc[0][1] * 256, y1 = acc[0][2] + acc[0][3] * 256;
         const int y2 = acc[1][0] + acc[1][1] * 256, y3 = acc[1][2] + acc[1][3] * 256;
@@ -157,8 +162,7 @@ __global__ __launch_bounds__(FcShape<SR>::NT, 4) void k_bconv_fwd_cols(BconvArgs
         v = __builtin_fma((double)y1, 0x1p16, v);
         v += (double)y0;
         const u32 kq = (u32)__builtin_fmax(__builtin_floor(__builtin_fma(v, pinv, -0x1p-30)), 0.0);
-        const u64 r = lo - (u64)kq * p;
-        return r >= p ? r - p : r;
+        return lo - (u64)kq * p;
     };
     // step e + 1's products are issued before step e's reduction, and the inputs
     // of step e + 2 are loaded then

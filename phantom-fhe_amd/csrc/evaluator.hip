@@ -431,12 +431,8 @@ __global__ __launch_bounds__(kThreads) void k_tensor(const u64 *a, const u64 *b,
     if (k >= N) return;
     const u64 o = (u64)limb * N + k;
     const DevModulus m = mod[limb];
-    const u64 q = m.value;
-    const u64 a0 = a[o], a1 = a[LN + o], b0 = b[o], b1 = b[LN + o];
-    const u64 r0 = mul_mod(a0, b0, m);
-    const u64 r2 = mul_mod(a1, b1, m);
-    u64 r1 = mul_mod(add_mod(a0, a1, q), add_mod(b0, b1, q), m);
-    r1 = sub_mod(sub_mod(r1, r0, q), r2, q);
+    u64 r0, r1, r2;
+    tensor_coeff(a[o], a[LN + o], b[o], b[LN + o], m, r0, r1, r2);
     d0[o] = r0;
     d1[o] = r1;
     d2[o] = r2;
@@ -658,13 +654,15 @@ void modup_ws(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, u64 *
     require(alpha <= (u32)kMaxBconvInputs, "special_modulus_size above 63 is not supported");
     // INTT(c2) * partQlHatInv (out of place); with `tensor`, c2 is the tensor
     // product computed (and written) by the first pass's load
+    // the fused FastBConv takes its inputs byte-flipped from the INTT's store
+    const bool fused = !copy_own && bconv_fused_ok(ctx, L.h_modup_fbconv);
     LimbMap mi = limb_map(t_cks, 0, Ql);
     mi.src = c2;
     mi.scale = L.modup_scale;
     if (tensor)
-        ntt_inverse_tensor(ctx.tables(), mi, Ql, *tensor, s);
+        ntt_inverse_tensor(ctx.tables(), mi, Ql, *tensor, s, fused);
     else
-        ntt_inverse(ctx.tables(), mi, Ql, 1, s);
+        ntt_inverse(ctx.tables(), mi, Ql, 1, s, fused);
     // the complement limbs of every digit in t_mod_up
     LimbMap mf = limb_map(t_mod_up, 0, QlP, (u64)QlP * N);
     mf.split = Ql;
@@ -673,7 +671,7 @@ void modup_ws(const Context &ctx, u32 chain, const u64 *c2, u64 *t_mod_up, u64 *
     mf.skip_cap = Ql;
     mf.nlimbs = QlP;
     const u32 last_sz = Ql - alpha * (beta - 1);
-    if (!copy_own && bconv_fused_ok(ctx, L.h_modup_fbconv)) {
+    if (fused) {
         // FastBConv of every digit to its complement fused with the NTT's column
         // pass, then the row pass
         bconv_fwd_cols_run(ctx, L.h_modup_fbconv,
@@ -716,8 +714,8 @@ void moddown_ws(const Context &ctx, u32 chain, u64 *cx, u64 *ct, const u64 *base
     // INTT of the P limbs, scaled by PHatInv (FastBConv phase 1)
     LimbMap mi = limb_map(cx + (u64)Ql * N, ctx.size_Q(), P, (u64)QlP * N);
     mi.scale = L.moddown_scale;
-    ntt_inverse(ctx.tables(), mi, P, npolys, s);
     const bool fused = bconv_fused_ok(ctx, {L.h_moddown_fbconv});
+    ntt_inverse(ctx.tables(), mi, P, npolys, s, fused);
     if (fused)
         bconv_fwd_cols_run(ctx, {L.h_moddown_fbconv},
                            bconv_args(ctx, L.moddown_fbconv, 0u, npolys, cx, (u64)QlP * N, delta, (u64)Ql * N,

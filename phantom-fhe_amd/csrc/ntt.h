@@ -18,6 +18,16 @@ struct StorePlain {
     }
 };
 
+// The inverse transforms whose outputs only the FastBConv fused with the forward
+// column pass reads (bconv_ntt.hip) store them with bit 7 of every byte flipped:
+// the matrix cores take those bytes as signed int8 (byte - 128) as they are.
+constexpr u64 kByteFlip = 0x8080808080808080ull;
+struct StoreFlipped {
+    __device__ __forceinline__ void operator()(u64 *d, u32 i, u64 v, u32, u32, const DevModulus &) const {
+        st_stream(d + i, v ^ kByteFlip);
+    }
+};
+
 // Moddown tail fused into the forward NTT of delta (CKKS):
 //   ct[poly][limb] += (cx[poly][limb] - NTT(delta)) * P^-1   (mod q_limb)
 // (reference: inplace_fnwt_radix8_phase2_fuse_moddown, src/ntt/ntt_moddown.cu:106-216,
@@ -51,13 +61,10 @@ struct LoadTensor {
     u64 LN;
     u32 N;
     __device__ __forceinline__ u64 operator()(const u64 *, u32 i, u32 limb, u32, const DevModulus &m) const {
-        const u64 o = (u64)limb * N + i, q = m.value;
+        const u64 o = (u64)limb * N + i;
         const auto ga = gmem(a), gb = gmem(b);
-        const u64 a0 = ga[o], a1 = ga[LN + o], b0 = gb[o], b1 = gb[LN + o];
-        const u64 r0 = mul_mod(a0, b0, m);
-        const u64 r2 = mul_mod(a1, b1, m);
-        u64 r1 = mul_mod(add_mod(a0, a1, q), add_mod(b0, b1, q), m);
-        r1 = sub_mod(sub_mod(r1, r0, q), r2, q);
+        u64 r0, r1, r2;
+        tensor_coeff(ga[o], ga[LN + o], gb[o], gb[LN + o], m, r0, r1, r2);
         gmem(out)[o] = r0;
         gmem(out)[LN + o] = r1;
         gmem(c2)[o] = r2;
@@ -155,13 +162,9 @@ struct LoadTensorJob {
         u64 *out, *c2;
         u64 LN;
         __device__ __forceinline__ u64 operator()(const u64 *, u32 i, u32, u32, const DevModulus &m) const {
-            const u64 q = m.value;
             const auto ga = gmem(a), gb = gmem(b);
-            const u64 a0 = ga[i], a1 = ga[LN + i], b0 = gb[i], b1 = gb[LN + i];
-            const u64 r0 = mul_mod(a0, b0, m);
-            const u64 r2 = mul_mod(a1, b1, m);
-            u64 r1 = mul_mod(add_mod(a0, a1, q), add_mod(b0, b1, q), m);
-            r1 = sub_mod(sub_mod(r1, r0, q), r2, q);
+            u64 r0, r1, r2;
+            tensor_coeff(ga[i], ga[LN + i], gb[i], gb[LN + i], m, r0, r1, r2);
             gmem(out)[i] = r0;
             gmem(out)[LN + i] = r1;
             gmem(c2)[i] = r2;
@@ -182,13 +185,14 @@ void ntt_forward_jobs_rescale(const NttTables &T, const NttJob *jobs, u32 njobs,
 // the row pass alone, over entries whose column pass ran fused with FastBConv
 void ntt_forward_rows_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
 void ntt_forward_rows_jobs_moddown(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
-// inverse: entries with aux0 read their first pass from aux0 (out of place)
-void ntt_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
+// inverse: entries with aux0 read their first pass from aux0 (out of place);
+// flip: outputs stored byte-flipped (StoreFlipped)
+void ntt_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s, bool flip = false);
 // inverse over entries whose first pass loads the tensor product (LoadTensorJob)
-void ntt_inverse_jobs_tensor(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s);
+void ntt_inverse_jobs_tensor(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s, bool flip = false);
 
 void ntt_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s);
-void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s);
+void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s, bool flip = false);
 void ntt_forward_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const StoreModDown &st,
                        hipStream_t s);
 // the row pass alone, over limbs whose column pass ran fused with FastBConv
@@ -198,7 +202,8 @@ void ntt_forward_rows_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u3
 void ntt_forward_rescale(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const LoadRescale &ld,
                          const StoreRescale &st, hipStream_t s);
 // inverse of one polynomial whose first pass loads the tensor product (always two passes)
-void ntt_inverse_tensor(const NttTables &T, const LimbMap &M, u32 nlimbs, const LoadTensor &ld, hipStream_t s);
+void ntt_inverse_tensor(const NttTables &T, const LimbMap &M, u32 nlimbs, const LoadTensor &ld, hipStream_t s,
+                        bool flip = false);
 
 // contiguous limbs [0, L) of one polynomial with rows first_row + i
 inline LimbMap limb_map(u64 *data, u32 first_row, u32 L, u64 poly_stride = 0) {

@@ -195,7 +195,7 @@ void ntt_forward_rows_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u3
 }
 
 namespace {
-template <class Load>
+template <class Load, class Store = StorePlain>
 void launch_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const Load &ld, hipStream_t s) {
     const u32 ntc = cols_threads(T), ntr = rows_threads(T);
     // launch order and slicing as in launch_forward
@@ -212,16 +212,21 @@ void launch_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys
         dispatch_cols(T.logN, ntc, [&](auto sr, auto nt) {
             constexpr int SR = decltype(sr)::value;
             constexpr int NT = decltype(nt)::value;
-            hipLaunchKernelGGL((k_inv_cols<SR, NT, StorePlain>), dim3(T.N / (NT * kNttRegs), np, nlimbs),
-                               dim3(NT), 0, s, T, Mc, StorePlain{});
+            hipLaunchKernelGGL((k_inv_cols<SR, NT, Store>), dim3(T.N / (NT * kNttRegs), np, nlimbs),
+                               dim3(NT), 0, s, T, Mc, Store{});
         });
     }
     PHX_HIP_CHECK(hipGetLastError());
 }
 }  // namespace
 
-void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s) {
+void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, hipStream_t s, bool flip) {
     if (nlimbs == 0 || npolys == 0) return;
+    if (flip) {
+        // byte-flipped outputs for the fused FastBConv: always the two-launch form
+        launch_inverse<LoadPlain, StoreFlipped>(T, M, nlimbs, npolys, LoadPlain{}, s);
+        return;
+    }
     if (const u32 G = fused_groups(T, nlimbs, npolys)) {
         fused_inverse(G, T, M, nlimbs, npolys, s);
         return;
@@ -229,9 +234,13 @@ void ntt_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, h
     launch_inverse(T, M, nlimbs, npolys, LoadPlain{}, s);
 }
 
-void ntt_inverse_tensor(const NttTables &T, const LimbMap &M, u32 nlimbs, const LoadTensor &ld, hipStream_t s) {
+void ntt_inverse_tensor(const NttTables &T, const LimbMap &M, u32 nlimbs, const LoadTensor &ld, hipStream_t s,
+                        bool flip) {
     if (nlimbs == 0) return;
-    launch_inverse(T, M, nlimbs, 1, ld, s);
+    if (flip)
+        launch_inverse<LoadTensor, StoreFlipped>(T, M, nlimbs, 1, ld, s);
+    else
+        launch_inverse(T, M, nlimbs, 1, ld, s);
 }
 
 void ntt_forward_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const StoreModDown &st,

@@ -10,8 +10,13 @@
 //   h = fl(y*w), l = fma(y, w, -h); the quotient k = rint(h / q) comes from
 //   the precomputed reciprocal (a Barrett reduction) and r = fma(-k, q, h) + l
 //   is exact because |h - k q| and |r| are integers below 2^53.
-//   Bounds (q < 2^50): reduced values |v| <= q/2 (+ eps); mulmod outputs
-//   |r| < 2q for |y| < 2^52; butterfly outputs |v| < 2.5q < 2^52.
+//   Bounds (q < 2^50): reduced values |v| <= q/2 (+ eps). mulmod(y, w) for
+//   |y| <= 5.3q: |h/q - k| <= 1/2 + |y| 2^-52 <= 1.83 and |l| <= |y w| 2^-53 <=
+//   0.66q, so |r| <= 2.5q, and for |y| <= 3q, |r| <= 1.63q. The forward
+//   butterfly reduces x on every other stage only (ct_lazy skips it): a reducing
+//   stage outputs |v| <= q/2 + 2.24q < 3q (its inputs are below 4.63q), a lazy
+//   one |v| <= 3q + 1.63q = 4.63q; every value stays below 2^52.3, and the last
+//   stage of a register round always reduces.
 #pragma once
 
 #include "common.h"
@@ -51,6 +56,8 @@ struct IntArith {
         y = xx + q2p1 + ~t;   // xx + 2q - t
         x = xx + t;
     }
+    // the lazy integer butterfly keeps its range by the reduction above on every stage
+    __device__ __forceinline__ void ct_lazy(T &x, T &y, u32 idx) const { ct(x, y, idx); }
     __device__ __forceinline__ void gs(T &x, T &y, u32 idx) const {
         const TwPair w = tw[idx];
         const u64 d = x + q2p1 + ~y;   // x + 2q - y
@@ -108,6 +115,14 @@ struct FpArith {
         x = xr + t;
         y = xr - t;
     }
+    // the forward butterfly without the reduction of x (see the bounds above)
+    __device__ __forceinline__ void ct_lazy(T &x, T &y, u32 idx) const {
+#pragma clang fp contract(off)
+        const T t = mulmod(y, tw[idx]);
+        const T xr = x;
+        x = xr + t;
+        y = xr - t;
+    }
     __device__ __forceinline__ void gs(T &x, T &y, u32 idx) const {
 #pragma clang fp contract(off)
         const T s = x + y, d = x - y;
@@ -122,5 +137,32 @@ struct FpArith {
     }
     __device__ __forceinline__ u64 inv_canon(T v) const { return canon(v); }
 };
+
+// Karatsuba tensor of one coefficient pair (tensor_prod_2x2_rns_poly,
+// src/polymath.cu:463-498): r0 = a0 b0, r2 = a1 b1, r1 = (a0 + a1)(b0 + b1) - r0 - r2
+// mod q, canonical, from canonical inputs. A row whose DevModulus carries fl(1/q)
+// (q < 2^50) multiplies exactly on the FP64 pipe: operands below 2q give |h/q - k|
+// <= 1/2 + 4q 2^-52 <= 1.5 and |l| <= 4q^2 2^-53 <= q/2, so each product is below
+// 2q and the difference below 4.5q, all exact; other rows use 128-bit Barrett.
+__device__ __forceinline__ void tensor_coeff(u64 a0, u64 a1, u64 b0, u64 b1, const DevModulus &m, u64 &r0, u64 &r1,
+                                             u64 &r2) {
+    if (m.pad) {
+#pragma clang fp contract(off)
+        const FpArith f{nullptr, (double)m.value, __longlong_as_double((long long)m.pad)};
+        const double x0 = FpArith::from_canon(a0), x1 = FpArith::from_canon(a1);
+        const double y0 = FpArith::from_canon(b0), y1 = FpArith::from_canon(b1);
+        const double p0 = f.mulmod(x0, y0), p2 = f.mulmod(x1, y1);
+        const double p1 = f.mulmod(x0 + x1, y0 + y1);
+        r0 = f.canon(p0);
+        r2 = f.canon(p2);
+        r1 = f.canon(p1 - p0 - p2);
+        return;
+    }
+    const u64 q = m.value;
+    r0 = mul_mod(a0, b0, m);
+    r2 = mul_mod(a1, b1, m);
+    r1 = mul_mod(add_mod(a0, a1, q), add_mod(b0, b1, q), m);
+    r1 = sub_mod(sub_mod(r1, r0, q), r2, q);
+}
 
 }  // namespace phx

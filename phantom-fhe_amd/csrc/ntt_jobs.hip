@@ -67,7 +67,7 @@ __global__ __launch_bounds__(NT) void k_inv_rows_jobs(NttTables T, const NttJob 
         inv_rows_body<NT>(int_inv(T, J.row, md.value), lds, J.data, src, T.logN, r0, 0u, 0u, md, lb, threadIdx.x);
 }
 
-template <int SR, int NT>
+template <int SR, int NT, class Store>
 __global__ __launch_bounds__(NT) void k_inv_cols_jobs(NttTables T, const NttJob *jobs) {
     constexpr int R = 1 << SR;
     constexpr int TC = NT * kNttRegs / R;
@@ -79,10 +79,10 @@ __global__ __launch_bounds__(NT) void k_inv_cols_jobs(NttTables T, const NttJob 
     const u32 c0 = blockIdx.x * TC;
     if (md.pad)
         inv_cols_body<SR, NT>(fp_arith(T.itwd + (u64)J.row * T.N, md), lds, J.data, T.logN, c0, 0u, 0u, md,
-                              StorePlain{}, s0, s0s, s1, s1s, threadIdx.x);
+                              Store{}, s0, s0s, s1, s1s, threadIdx.x);
     else
         inv_cols_body<SR, NT>(int_inv(T, J.row, md.value), lds, J.data, T.logN, c0, 0u, 0u, md,
-                              StorePlain{}, s0, s0s, s1, s1s, threadIdx.x);
+                              Store{}, s0, s0s, s1, s1s, threadIdx.x);
 }
 
 namespace {
@@ -151,7 +151,7 @@ void ntt_forward_rows_jobs_moddown(const NttTables &T, const NttJob *jobs, u32 n
 }
 
 namespace {
-template <class Load>
+template <class Load, class Store = StorePlain>
 void launch_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, const Load &ld, hipStream_t s) {
     const u32 ntc = cols_threads(T), ntr = rows_threads(T);
     for (u32 j0 = 0; j0 < njobs; j0 += kMaxJobsPerLaunch) {
@@ -164,20 +164,26 @@ void launch_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, cons
         dispatch_cols(T.logN, ntc, [&](auto sr, auto nt) {
             constexpr int SR = decltype(sr)::value;
             constexpr int NT = decltype(nt)::value;
-            hipLaunchKernelGGL((k_inv_cols_jobs<SR, NT>), dim3(T.N / (NT * kNttRegs), nj), dim3(NT), 0, s, T,
-                               jobs + j0);
+            hipLaunchKernelGGL((k_inv_cols_jobs<SR, NT, Store>), dim3(T.N / (NT * kNttRegs), nj), dim3(NT), 0, s,
+                               T, jobs + j0);
         });
     }
     PHX_HIP_CHECK(hipGetLastError());
 }
 }  // namespace
 
-void ntt_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s) {
-    launch_inverse_jobs(T, jobs, njobs, LoadPlain{}, s);
+void ntt_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s, bool flip) {
+    if (flip)
+        launch_inverse_jobs<LoadPlain, StoreFlipped>(T, jobs, njobs, LoadPlain{}, s);
+    else
+        launch_inverse_jobs(T, jobs, njobs, LoadPlain{}, s);
 }
 
-void ntt_inverse_jobs_tensor(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s) {
-    launch_inverse_jobs(T, jobs, njobs, LoadTensorJob{T.N}, s);
+void ntt_inverse_jobs_tensor(const NttTables &T, const NttJob *jobs, u32 njobs, hipStream_t s, bool flip) {
+    if (flip)
+        launch_inverse_jobs<LoadTensorJob, StoreFlipped>(T, jobs, njobs, LoadTensorJob{T.N}, s);
+    else
+        launch_inverse_jobs(T, jobs, njobs, LoadTensorJob{T.N}, s);
 }
 
 }  // namespace phx

@@ -166,7 +166,12 @@ __device__ __forceinline__ void fwd_round(typename A::T (&v)[16], const u32 (&ba
             for (int e = 0; e < (1 << NB); ++e) {
                 if (e & (1 << jj)) continue;
                 const u32 idx = Top ? m + ((u32)e >> (jj + 1)) : tw_index<P, NB>(m, base[b], jj, e);
-                a.ct(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], idx);
+                // odd stages skip the FP64 arithmetic's reduction of x; stage 0,
+                // the round's last, always reduces (FpArith's bounds)
+                if (jj & 1)
+                    a.ct_lazy(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], idx);
+                else
+                    a.ct(v[b * (1 << NB) + e], v[b * (1 << NB) + e + (1 << jj)], idx);
             }
         }
     }
