@@ -393,7 +393,7 @@ int phx_tune_get(const char *key, int *value);
  *   bconv_wgs_per_cu    matrix-core FastBConv workgroups per CU: 1, 2, 4 (0 = as many
  *                       as the LDS holds)
  *   bconv_fused         1 = the matrix-core FastBConv of modup and moddown fused with the
- *                       column pass of the forward NTT of its outputs (N = 2^13 .. 2^16,
+ *                       column pass of the forward NTT of its outputs (N = 2^12 .. 2^16,
  *                       special_modulus_size up to 16), 0 = separate launches */
 #define PHX_TUNE_AXES 8
 typedef struct {

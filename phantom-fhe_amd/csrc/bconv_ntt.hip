@@ -188,11 +188,11 @@ __global__ __launch_bounds__(FcShape<SR>::NT, 4) void k_bconv_fwd_cols(BconvArgs
         fc_round_a<SR>(int_fwd(T, row, p), raw, outp, store, T.logN, base);
 }
 
-// true when the fused launch serves these conversions: ring degree 2^13 .. 2^16,
+// true when the fused launch serves these conversions: ring degree 2^12 .. 2^16,
 // every descriptor with a fused matrix-core form, the bconv_fused switch on
 bool bconv_fused_ok(const Context &ctx, const std::vector<BconvDesc> &fdescs) {
     const u32 logN = ctx.logN();
-    if (logN < 13 || logN > 16) return false;
+    if (logN < 12 || logN > 16) return false;
     const int sw = ctx.tuning().bconv_fused >= 0 ? ctx.tuning().bconv_fused : tune_value("bconv_fused");
     if (sw == 0) return false;
     const int mf = ctx.tuning().bconv_mfma >= 0 ? ctx.tuning().bconv_mfma : tune_value("bconv_mfma");
@@ -216,6 +216,7 @@ void bconv_fwd_cols_run(const Context &ctx, const std::vector<BconvDesc> &fdescs
                            groups);
     };
     switch (ctx.logN()) {
+        case 12: launch(std::integral_constant<int, 4>{}); break;
         case 13: launch(std::integral_constant<int, 5>{}); break;
         case 14: launch(std::integral_constant<int, 6>{}); break;
         case 15: launch(std::integral_constant<int, 7>{}); break;

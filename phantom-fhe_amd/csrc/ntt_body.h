@@ -195,9 +195,14 @@ __device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *d
 template <int SR, int NT, class A>
 __device__ __forceinline__ void fwd_rows_core_rb(const A &a, u64 *lds, const u64 *data, u32 logN, u32 r0,
                                                  const u32 t, u64 (&out)[16]) {
-    static_assert(NT % 256 == 0 && SR >= 5 && SR <= 8, "column-first row pass: 256 or 512 threads, N = 2^13 .. 2^16");
+    static_assert(NT % 256 == 0 && SR >= 4 && SR <= 8, "column-first row pass: 256 or 512 threads, N = 2^12 .. 2^16");
     using T = typename A::T;
     constexpr int NB = SR - 4;
+    if constexpr (NB == 0) {
+        // N = 2^12: the column pass has only the round the fused launch ran
+        fwd_rows_core<NT, A>(a, lds, data, logN, r0, t, out);
+        return;
+    }
     constexpr int BB = 16 >> NB;
     const u32 cc = t & 255, rb = r0 + 16 * (t >> 8);
     T v[16];
@@ -423,7 +428,7 @@ void dispatch_cols(u32 logN, u32 nt, F &&f) {
             f(sr, std::integral_constant<int, 256>{});
     });
 }
-// row passes after the FastBConv-fused column pass: (SR, NT) for N = 2^13 .. 2^16
+// row passes after the FastBConv-fused column pass: (SR, NT) for N = 2^12 .. 2^16
 // and 256 or 512 threads
 template <class F>
 void dispatch_rows_rb(u32 logN, u32 nt, F &&f) {
@@ -434,6 +439,7 @@ void dispatch_rows_rb(u32 logN, u32 nt, F &&f) {
             f(sr, std::integral_constant<int, 256>{});
     };
     switch (logN) {
+        case 12: go(std::integral_constant<int, 4>{}); break;
         case 13: go(std::integral_constant<int, 5>{}); break;
         case 14: go(std::integral_constant<int, 6>{}); break;
         case 15: go(std::integral_constant<int, 7>{}); break;

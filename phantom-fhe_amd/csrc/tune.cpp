@@ -15,7 +15,7 @@
 //   bconv_wgs_per_cu  FastBConv on the matrix cores: workgroups per CU of its grid
 //                (1, 2, 4; 0 = as many as the LDS holds, the default)
 //   bconv_fused  1: FastBConv on the matrix cores fused with the column pass of the
-//                forward NTT of its outputs (default, N = 2^13 .. 2^16), 0: separate
+//                forward NTT of its outputs (default, N = 2^12 .. 2^16), 0: separate
 //                launches
 //   batch_branches  concurrent branches of a persistent batch plan (1..8, default
 //                2): plans of at least twice as many items are split into that many

@@ -1,7 +1,7 @@
 """FastBConv fused with the forward NTT's column pass (bconv_ntt.hip): modup,
 moddown, multiply_and_relin, rotation and batch plans bit-exact against the
 oracle with the fused launch on and off, at every ring degree it serves
-(2^13 .. 2^16), over prime sets whose 64-bit-integer and FP64 outputs are not
+(2^12 .. 2^16), over prime sets whose 64-bit-integer and FP64 outputs are not
 multiples of four (padded groups)."""
 import numpy as np
 import pytest
@@ -14,6 +14,7 @@ pytestmark = pytest.mark.gpu
 # (N, bit sizes, special primes): 60-bit primes take the integer butterflies,
 # 50-bit ones the FP64 butterflies; class sizes per conversion vary by digit
 SHAPES = [
+    (1 << 12, [60, 50, 50, 50, 50, 50, 60, 60], 2),
     (1 << 13, [60] + [50] * 6 + [60] * 3, 3),
     (1 << 14, [60, 60] + [50] * 8 + [60] * 5, 5),
     (1 << 15, [60] + [50] * 10 + [60] * 4, 4),
