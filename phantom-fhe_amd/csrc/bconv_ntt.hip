@@ -18,8 +18,10 @@
 // low, low + R/16, ..., low + 15 R/16 of its output's column: the 16 values of
 // a column-pass thread's first register round. The four outputs of a workgroup
 // share one butterfly arithmetic: the fused descriptors list the 64-bit-integer
-// outputs first and the FP64 ones after, each class padded to a multiple of four
-// with outputs that are computed but not stored.
+// outputs first and the FP64 ones after (padded to a multiple of four with
+// outputs that are computed but not stored); a group that holds both runs the
+// integer butterflies, exact for any prime below 2^61, and stores its FP64
+// outputs in the FP64 form their row pass reads.
 //
 // Workgroups over the same (conversion, column tile, rows) read the same inputs;
 // they are numbered so that they run on one XCD next to each other, and the
@@ -43,10 +45,12 @@ struct FcShape {
 };
 
 // the column pass's first register round on one lane's 16 values (rows
-// low + e R/16 of one column, residues below 2q), stored as the intermediate
+// low + e R/16 of one column, residues below 2q), stored as the intermediate.
+// fp_form: the integer round ran for a prime whose row pass takes the FP64
+// arithmetic, so its outputs are stored canonical as doubles.
 template <int SR, class Ar>
 __device__ __forceinline__ void fc_round_a(const Ar &a, const u64 (&raw)[16], u64 *outp, bool store, u32 logN,
-                                           u32 base) {
+                                           u32 base, bool fp_form = false) {
     using Tv = typename Ar::T;
     constexpr int NB = SR - 4;
     Tv v[16];
@@ -56,7 +60,12 @@ __device__ __forceinline__ void fc_round_a(const Ar &a, const u64 (&raw)[16], u6
     fwd_round<4, 8 + NB, Ar, true>(v, baseA, logN, a);
     if (store) {
 #pragma unroll
-        for (int e = 0; e < 16; ++e) st_mid(outp + base + ((u32)e << (8 + NB)), Ar::to_bits(v[e]));
+        for (int e = 0; e < 16; ++e) {
+            u64 w = Ar::to_bits(v[e]);
+            if constexpr (std::is_same<Ar, IntArith>::value)
+                if (fp_form) w = FpArith::to_bits(FpArith::from_canon(a.fwd_canon(v[e])));
+            st_mid(outp + base + ((u32)e << (8 + NB)), w);
+        }
     }
 }
 }  // namespace
@@ -181,11 +190,13 @@ __global__ __launch_bounds__(FcShape<SR>::NT, 4) void k_bconv_fwd_cols(BconvArgs
     const u32 opos = (reinterpret_cast<const u32 *>(A.cbuf) + d.pos_off)[o];
     u64 *outp = outb + (u64)(opos == 0xffffffffu ? 0 : opos) * N;
     const bool store = opos != 0xffffffffu;
-    // the group's four outputs share one arithmetic (uniform branch)
-    if (md.pad)
+    // the group's four outputs share one arithmetic (a uniform branch): the integer
+    // one when the group's first output needs it (integer outputs come first)
+    const bool grp_fp = T.mod[(reinterpret_cast<const u32 *>(A.cbuf) + d.row_off)[4 * grp]].pad != 0;
+    if (grp_fp)
         fc_round_a<SR>(fp_arith(T.twd + (u64)row * N, md), raw, outp, store, T.logN, base);
     else
-        fc_round_a<SR>(int_fwd(T, row, p), raw, outp, store, T.logN, base);
+        fc_round_a<SR>(int_fwd(T, row, p), raw, outp, store, T.logN, base, md.pad != 0);
 }
 
 // true when the fused launch serves these conversions: ring degree 2^12 .. 2^16,

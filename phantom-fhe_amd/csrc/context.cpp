@@ -461,9 +461,10 @@ Context::Context(const Params &p) : params_(p) {
     };
     // The same conversion for the launch fused with the forward NTT's column pass
     // (bconv_ntt.hip): outputs on 64-bit integer butterflies first, then those on
-    // FP64 butterflies, each class padded to a multiple of four with outputs that
-    // are computed but not stored (pos ~0, a zero matrix row, the class's first
-    // prime), so that the four outputs of one workgroup share one arithmetic.
+    // FP64 butterflies, padded to a multiple of four with outputs that are computed
+    // but not stored (pos ~0, a zero matrix row, the last output's prime). A group
+    // of four outputs runs one arithmetic: the integer one when its first output
+    // needs it (the FP64 outputs of such a group are stored in FP64 form).
     auto row_fp = [&](u32 row) { return !int_only && host_tables_[row].q < (1ull << 50); };
     auto add_fused = [&](const BconvDesc &d, const std::vector<u64> &mat, const std::vector<u32> &pos,
                          const std::vector<u32> &rows, const std::vector<u64> &om) {
@@ -473,16 +474,16 @@ Context::Context(const Params &p) : params_(p) {
         for (u32 o = 0; o < d.no; ++o) order[row_fp(rows[o]) ? 1 : 0].push_back(o);
         std::vector<u64> fmat, fom;
         std::vector<u32> fpos, frows;
-        for (const auto &ord : order) {
-            const size_t padded = (ord.size() + 3) / 4 * 4;
-            for (size_t k = 0; k < padded; ++k) {
-                const bool real = k < ord.size();
-                const u32 o = real ? ord[k] : ord[0];
-                for (u32 i = 0; i < d.ni; ++i) fmat.push_back(real ? mat[(size_t)o * d.ni + i] : 0);
-                fom.push_back(om[o]);
-                fpos.push_back(real ? pos[o] : 0xffffffffu);
-                frows.push_back(rows[o]);
-            }
+        std::vector<u32> ord = order[0];
+        ord.insert(ord.end(), order[1].begin(), order[1].end());
+        const size_t padded = (ord.size() + 3) / 4 * 4;
+        for (size_t k = 0; k < padded; ++k) {
+            const bool real = k < ord.size();
+            const u32 o = real ? ord[k] : ord.back();
+            for (u32 i = 0; i < d.ni; ++i) fmat.push_back(real ? mat[(size_t)o * d.ni + i] : 0);
+            fom.push_back(om[o]);
+            fpos.push_back(real ? pos[o] : 0xffffffffu);
+            frows.push_back(rows[o]);
         }
         f.ni = d.ni;
         f.no = (u32)fpos.size();
