@@ -116,3 +116,24 @@ def test_extreme_inputs_fused():
         assert np.array_equal(ob_.to_array((Ql, N)), octx.moddown(Ql, cx))
 
     _with_fused(1, run)
+
+
+@pytest.mark.parametrize("N", [1 << 12, 1 << 14])
+@pytest.mark.parametrize("P", [1, 2, 5, 8, 13, 16])
+def test_special_modulus_sizes(N, P):
+    """Digits of every width the fused launch takes (1 .. 16 limbs), with data and
+    special primes of both arithmetics: multiply_and_relin at the top level and two
+    levels down, bit-exact against the oracle."""
+    say(f"case: bconv fused N={N} P={P}")
+    L = 2 * P + 3
+    bits = [60] + [50] * (L - 1) + [60] * P
+    ctx, octx, mods = make_ctx(N, bits, P)
+    rng = np.random.default_rng(N + 31 * P)
+    key = rand_key(rng, mods, N, ctx.dnum)
+    rk = ph.relin_key(ctx, host=key)
+    for chain in (1, 3):
+        Ql = ctx.limbs_at(chain)
+        a, b = rand_ct(rng, mods[:Ql], N), rand_ct(rng, mods[:Ql], N)
+        c1, c2 = upload_ct(ctx, a, chain), upload_ct(ctx, b, chain)
+        ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
+        assert np.array_equal(c1.download(N), octx.multiply_and_relin(Ql, a, b, key)), chain
