@@ -287,17 +287,31 @@ def ckks_error_report(ph, ctx, scale=2.0 ** 50, step=5, seed=20):
     return {k: (float(f"{v:.3e}") if isinstance(v, float) else v) for k, v in out.items()}
 
 
-def bench_ntt(ph, ctx, N, L, npolys, iters, stream):
-    """Forward NTT over npolys x L limbs in one launch pair (pass 1 + pass 2)."""
+def bench_ntt(ph, ctx, N, L, npolys, iters, stream, warmup=80):
+    """Forward NTT over npolys x L limbs in one launch pair (pass 1 + pass 2):
+    mean over iters pairs back to back after warmup pairs, and the fastest single pair."""
     rng = np.random.default_rng(7)
     mods = ctx.coeff_modulus[:L]
     host = np.concatenate([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in mods])
                            for _ in range(npolys)])
     buf = ph.DeviceBuffer.from_array(host)
     del host
-    for _ in range(3):
+    # warm-up of about 30 ms of transforms: in the first milliseconds of sustained
+    # load the launch pairs run up to 20% slower for a while (a transient of the
+    # GPU's clock and power management, seen in kernel traces), then settle
+    for _ in range(warmup):
         ph.ntt_forward_batch(ctx, buf, L, 0, npolys, L * N, stream)
     ph.synchronize(stream)
+    # mean: iters launch pairs back to back between two events (an event recorded
+    # between launches leaves a gap of a few microseconds before the next one)
+    e0, e1 = ph.Event(), ph.Event()
+    e0.record(stream)
+    for _ in range(iters):
+        ph.ntt_forward_batch(ctx, buf, L, 0, npolys, L * N, stream)
+    e1.record(stream)
+    ph.synchronize(stream)
+    mean = e0.elapsed_ms(e1) / iters
+    # fastest single pair, each bracketed by its own events
     evs = [ph.Event() for _ in range(iters + 1)]
     evs[0].record(stream)
     for i in range(iters):
@@ -305,7 +319,7 @@ def bench_ntt(ph, ctx, N, L, npolys, iters, stream):
         evs[i + 1].record(stream)
     ph.synchronize(stream)
     per = [evs[i].elapsed_ms(evs[i + 1]) for i in range(iters)]
-    return float(np.mean(per)), float(np.min(per))
+    return float(mean), float(np.min(per))
 
 
 def bench_copy(ph, n_u64, iters, stream):
