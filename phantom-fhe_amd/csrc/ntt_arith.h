@@ -1,7 +1,8 @@
 // ntt_arith.h -- the two butterfly arithmetics of the NTT kernels.
 //
 // IntArith: 64-bit Shoup/Harvey lazy butterflies on u64 values (any q < 2^61),
-//   values in [0, 4q) (forward) / [0, 2q) (inverse).
+//   values in [0, 6q) (forward) / [0, 3q) (inverse): the Shoup quotient skips
+//   the low partial product (shoup_quot_lo), see ct and gs.
 //
 // FpArith: exact modular arithmetic on the FP64 pipe for q < 2^50 (the
 //   MI355X VALU runs FP64 FMA at full rate, 32-bit integer multiplies at a
@@ -32,6 +33,18 @@ __device__ __forceinline__ u64 csub_n(u64 x, u64 m, u64 nm) {
     return t + (m & (u64)((long long)t >> 63));
 }
 
+// floor(a b / 2^64) without the low partial product a0 b0: with a = a1 2^32 + a0
+// and b = b1 2^32 + b0, a b - a0 b0 = (a1 b1 + hi(m) + hi(n)) 2^64 + lo(n) 2^32
+// for m = a0 b1 and n = a1 b0 + lo(m), and a0 b0 < 2^64 adds at most one carry,
+// so the result is the exact high word or one less (three 32 x 32 -> 64
+// multiplies instead of four)
+__device__ __forceinline__ u64 shoup_quot_lo(u64 a, u64 b) {
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    const u64 m = (u64)a0 * b1;
+    const u64 n = (u64)a1 * b0 + (u32)m;
+    return (u64)a1 * b1 + (m >> 32) + (n >> 32);
+}
+
 struct IntArith {
     using T = u64;
     const TwPair *tw;
@@ -40,33 +53,45 @@ struct IntArith {
     // below are additions of them (y - t = y + 2q + 1 + ~t - 2q), and the
     // optimiser, which does not see these values, keeps them additions
     u64 nq, nq2, q2p1;
+    // 3q, and -3q and 3q + 1 mod 2^64 from the tables (the butterflies below)
+    u64 q3, nq3, q3p1;
 
     __device__ __forceinline__ static T from_canon(u64 v) { return v; }
     __device__ __forceinline__ static T from_bits(u64 b) { return b; }
     __device__ __forceinline__ static u64 to_bits(T v) { return v; }
-    // forward output [0, 4q) -> [0, q)
-    __device__ __forceinline__ u64 fwd_canon(T v) const { return csub_n(csub_n(v, q2, nq2), q, nq); }
+    // forward output [0, 6q) -> [0, q)
+    __device__ __forceinline__ u64 fwd_canon(T v) const {
+        return csub_n(csub_n(csub_n(v, q3, nq3), q2, nq2), q, nq);
+    }
 
-    // Harvey's lazy butterflies (the arithmetic of ct_bfly / gs_bfly in common.h):
-    // t = y w - floor(y w' / 2^64) q in [0, 2q) as y w + hi (-q)
+    // Harvey's lazy forward butterfly (ct_bfly in common.h) with a cheaper Shoup
+    // quotient: shoup_quot_lo omits the low partial product, so it is
+    // floor(y w' / 2^64) or one less, and t = y w - quot q (as y w + quot (-q)) is
+    // in [0, 3q) instead of [0, 2q) for any 64-bit y. x is reduced below 3q, so
+    // both outputs stay below 6q < 2^64 (q < 2^61): one 32-bit multiply fewer
+    // per butterfly.
     __device__ __forceinline__ void ct(T &x, T &y, u32 idx) const {
         const TwPair w = tw[idx];
-        const u64 t = y * w.w + umulhi(y, w.ws) * nq;
-        const u64 xx = csub_n(x, q2, nq2);
-        y = xx + q2p1 + ~t;   // xx + 2q - t
+        const u64 t = y * w.w + shoup_quot_lo(y, w.ws) * nq;
+        const u64 xx = csub_n(x, q3, nq3);
+        y = xx + q3p1 + ~t;   // xx + 3q - t
         x = xx + t;
     }
     // the lazy integer butterfly keeps its range by the reduction above on every stage
     __device__ __forceinline__ void ct_lazy(T &x, T &y, u32 idx) const { ct(x, y, idx); }
+    // the inverse butterfly (gs_bfly in common.h) on values below 3q: the sum is
+    // reduced by 3q, and the product of x + 3q - y takes shoup_quot_lo, which
+    // leaves it in [0, 3q) (as in ct)
     __device__ __forceinline__ void gs(T &x, T &y, u32 idx) const {
         const TwPair w = tw[idx];
-        const u64 d = x + q2p1 + ~y;   // x + 2q - y
-        x = csub_n(x + y, q2, nq2);
-        y = d * w.w + umulhi(d, w.ws) * nq;
+        const u64 d = x + q3p1 + ~y;   // x + 3q - y
+        x = csub_n(x + y, q3, nq3);
+        y = d * w.w + shoup_quot_lo(d, w.ws) * nq;
     }
-    // top inverse stage with both outputs scaled to canonical form
+    // top inverse stage with both outputs scaled to canonical form (mul_shoup
+    // takes any 64-bit input)
     __device__ __forceinline__ void gs_last(T &x, T &y, u64 s0, u64 s0s, u64 s1, u64 s1s) const {
-        const u64 sum = x + y, dif = x + q2 - y;
+        const u64 sum = x + y, dif = x + q3 - y;
         x = mul_shoup(sum, s0, s0s, q);
         y = mul_shoup(dif, s1, s1s, q);
     }
