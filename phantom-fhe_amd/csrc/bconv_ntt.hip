@@ -159,7 +159,7 @@ __global__ __launch_bounds__(FcShape<SR>::NT, 4) void k_bconv_fwd_cols(BconvArgs
     };
     // T = sum_b 256^b Z_b + C_o in [0, 2^15 p) -> r = T - floor(T/p - 2^-30) p in
     // [0, 2p), as k_bconv_mfma's reduction without its final subtraction: the
-    // butterflies take inputs below 2p (the integer ones below 4p, the FP64 ones
+    // butterflies take inputs below 2p (the integer ones below 6p, the FP64 ones
     // below 2^52, and 2p < 2^51 there)
     auto reduce = [&](const v4i(&acc)[2]) -> u64 {
         const int y0 = acc[0][0] + acc[0][1] * 256, y1 = acc[0][2] + acc[0][3] * 256;

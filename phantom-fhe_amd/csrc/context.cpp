@@ -381,7 +381,7 @@ Context::Context(const Params &p) : params_(p) {
     const size_t twdb = R * N_ * sizeof(double);
     const size_t modb = R * sizeof(DevModulus);
     const size_t icb = R * 4 * sizeof(u64);
-    const size_t ncb = R * 5 * sizeof(u64);
+    const size_t ncb = R * 4 * sizeof(u64);
     std::vector<uint8_t> h(2 * twb + 2 * twdb + modb + icb + ncb);
     TwPair *htw = reinterpret_cast<TwPair *>(h.data());
     TwPair *hitw = reinterpret_cast<TwPair *>(h.data() + twb);
@@ -412,11 +412,10 @@ Context::Context(const Params &p) : params_(p) {
         hic[4 * r + 1] = hm::shoup(t.ninv, t.q);
         hic[4 * r + 2] = s1;
         hic[4 * r + 3] = hm::shoup(s1, t.q);
-        hnc[5 * r + 0] = 0 - t.q;
-        hnc[5 * r + 1] = 0 - 2 * t.q;
-        hnc[5 * r + 2] = 2 * t.q + 1;
-        hnc[5 * r + 3] = 0 - 3 * t.q;
-        hnc[5 * r + 4] = 3 * t.q + 1;
+        hnc[4 * r + 0] = 0 - t.q;
+        hnc[4 * r + 1] = 0 - 2 * t.q;
+        hnc[4 * r + 2] = 0 - 3 * t.q;
+        hnc[4 * r + 3] = 3 * t.q + 1;
     }
     PHX_HIP_CHECK(hipMalloc(&tw_buf_, h.size()));
     PHX_HIP_CHECK(hipMemcpy(tw_buf_, h.data(), h.size(), hipMemcpyHostToDevice));

@@ -48,13 +48,11 @@ __device__ __forceinline__ u64 shoup_quot_lo(u64 a, u64 b) {
 struct IntArith {
     using T = u64;
     const TwPair *tw;
-    u64 q, q2;
-    // -q, -2q and 2q + 1 mod 2^64, read from the device tables: the subtractions
-    // below are additions of them (y - t = y + 2q + 1 + ~t - 2q), and the
-    // optimiser, which does not see these values, keeps them additions
-    u64 nq, nq2, q2p1;
-    // 3q, and -3q and 3q + 1 mod 2^64 from the tables (the butterflies below)
-    u64 q3, nq3, q3p1;
+    u64 q, q2, q3;
+    // -q, -2q, -3q and 3q + 1 mod 2^64, read from the device tables: the
+    // subtractions below are additions of them (x - y = x + 3q + 1 + ~y - 3q), and
+    // the optimiser, which does not see these values, keeps them additions
+    u64 nq, nq2, nq3, q3p1;
 
     __device__ __forceinline__ static T from_canon(u64 v) { return v; }
     __device__ __forceinline__ static T from_bits(u64 b) { return b; }

@@ -35,8 +35,8 @@ __device__ __forceinline__ void limb_scale(const NttTables &T, const LimbMap &M,
 
 // the integer arithmetic of table row `row`: forward (tw) or inverse (itw) twiddles
 __device__ __forceinline__ IntArith int_arith(const NttTables &T, const TwPair *tw, u32 row, u64 q) {
-    const u64 *c = T.int_consts + 5 * (u64)row;
-    return IntArith{tw + (u64)row * T.N, q, q << 1, c[0], c[1], c[2], 3 * q, c[3], c[4]};
+    const u64 *c = T.int_consts + 4 * (u64)row;
+    return IntArith{tw + (u64)row * T.N, q, q << 1, 3 * q, c[0], c[1], c[2], c[3]};
 }
 __device__ __forceinline__ IntArith int_fwd(const NttTables &T, u32 row, u64 q) { return int_arith(T, T.tw, row, q); }
 __device__ __forceinline__ IntArith int_inv(const NttTables &T, u32 row, u64 q) { return int_arith(T, T.itw, row, q); }

@@ -101,7 +101,7 @@ struct NttTables {
     const double *itwd;      // [rows][N]
     const DevModulus *mod;   // [rows]
     const u64 *inv_consts;   // [rows][4]: n^-1, shoup, n^-1 * itw[1], shoup
-    const u64 *int_consts;   // [rows][5]: -q, -2q, 2q + 1, -3q, 3q + 1 mod 2^64 (the integer butterflies)
+    const u64 *int_consts;   // [rows][4]: -q, -2q, -3q, 3q + 1 mod 2^64 (the integer butterflies)
     u32 logN;
     u32 N;
     // launch shape chosen for this context by the autotuner; -1 = follow the
