@@ -2,6 +2,8 @@
 import numpy as np
 import pytest
 
+import bigint_ref as br
+import oracle_bind as ob
 import pyphantom as ph
 from gpu_helpers import make_ctx, rand_limbs, say
 
@@ -104,3 +106,42 @@ def test_ntt_max_degree_edge_values():
         f = run_ntt(ctx, a, 0)
         assert np.array_equal(f, octx.ntt_forward(a, 0))
         assert np.array_equal(run_ntt(ctx, f, 0, inverse=True), a)
+
+
+@pytest.mark.parametrize("logN", [12, 16])
+def test_ntt_61bit_edge_values(logN):
+    """Primes just below 2^61, the largest a context takes: the integer butterflies'
+    lazy ranges are widest there (forward values below 6q < 2^64, inverse below 3q).
+    All q - 1, alternating q - 1 / 0 and uniform inputs, forward and inverse against
+    the oracle and round trips; then multiply_and_relin with such special primes and
+    every input at q - 1."""
+    N = 1 << logN
+    say(f"case: ntt 61-bit edge values N=2^{logN}")
+    p61 = br.primes_below(61, N, 5)   # coeff_modulus_create stops at 60 bits
+    mods = p61[:3] + ph.coeff_modulus_create(N, [50]) + p61[3:]
+    assert all(q >> 60 == 1 for q in p61)
+    parms = ph.EncryptionParameters(ph.SCHEME_CKKS)
+    parms.set_poly_modulus_degree(N)
+    parms.set_coeff_modulus(mods)
+    parms.set_special_modulus_size(2)
+    ctx = ph.context(parms)
+    octx = ob.Ctx(logN, mods, 2)
+    qs = np.array(mods, np.uint64)[:, None]
+    L = len(mods)
+    rng = np.random.default_rng(61 + logN)
+    alt = np.where(np.arange(N) % 2 == 0, qs - 1, 0).astype(np.uint64)
+    for a in (np.broadcast_to(qs - 1, (L, N)).copy(), alt, rand_limbs(rng, mods, N)):
+        f = run_ntt(ctx, a, 0)
+        assert np.array_equal(f, octx.ntt_forward(a, 0))
+        assert np.array_equal(run_ntt(ctx, f, 0, inverse=True), a)
+        assert np.array_equal(run_ntt(ctx, a, 0, inverse=True), octx.ntt_inverse(a, 0))
+    Ql = ctx.size_Q
+    key = np.stack([np.stack([np.broadcast_to(qs - 1, (L, N)).copy() for _ in range(2)]) for _ in range(ctx.dnum)])
+    rk = ph.relin_key(ctx, host=key)
+    a = np.stack([np.broadcast_to(qs[:Ql] - 1, (Ql, N)).copy() for _ in range(2)])
+    c1 = ph.ciphertext(ctx, 1, 2)
+    c1.upload(a)
+    c2 = ph.ciphertext(ctx, 1, 2)
+    c2.upload(a)
+    ph.multiply_and_relin_inplace(ctx, c1, c2, rk)
+    assert np.array_equal(c1.download(N), octx.multiply_and_relin(Ql, a, a, key))
