@@ -359,6 +359,29 @@ static void ntt_rows(const orc_ctx *c, u64 *data, const size_t *rows, size_t L, 
     parallel_for(L, ntt_rows_task, &j);
 }
 
+/* as ntt_rows over limbs that lie anywhere: limb i at ptrs[i] */
+typedef struct {
+    const orc_ctx *c;
+    u64 **ptrs;
+    const size_t *rows;
+    int inverse;
+} ntt_ptrs_job;
+
+static void ntt_ptrs_task(void *arg, size_t i) {
+    ntt_ptrs_job *j = (ntt_ptrs_job *)arg;
+    const orc_ctx *c = j->c;
+    size_t p = j->rows[i];
+    if (j->inverse)
+        ntt_inv_limb(j->ptrs[i], c->N, c->q[p], c->irp + p * c->N, c->irps + p * c->N, c->ninv[p]);
+    else
+        ntt_fwd_limb(j->ptrs[i], c->N, c->q[p], c->rp + p * c->N, c->rps + p * c->N);
+}
+
+static void ntt_ptrs(const orc_ctx *c, u64 **ptrs, const size_t *rows, size_t L, int inverse) {
+    ntt_ptrs_job j = {c, ptrs, rows, inverse};
+    parallel_for(L, ntt_ptrs_task, &j);
+}
+
 /* ------------------------------------------------------------------ */
 /* RNS base conversion                                                 */
 /* ------------------------------------------------------------------ */
@@ -406,15 +429,47 @@ static void bconv_task(void *arg, size_t o) {
 /* FastBConv (phase 1 + phase 2): y_i = [x_i * qHatInv_i]_{q_i}
  * (src/rns_bconv.cu:40-60), then phase 2 as above; DBaseConverter::bConv_BEHZ
  * src/rns_bconv.cu:212-229 */
+typedef struct {
+    size_t N, ni;
+    const u64 *ibase, *in;
+    u64 *y;
+} bconv_pre_job;
+
+/* FastBConv phase 1 for input i: y_i = [x_i * qHatInv_i]_{q_i} */
+static void bconv_pre_task(void *arg, size_t i) {
+    bconv_pre_job *j = (bconv_pre_job *)arg;
+    u64 q = j->ibase[i];
+    u64 hinv = invmod(punctured_mod(j->ibase, j->ni, i, q), q);
+    for (size_t k = 0; k < j->N; k++) j->y[i * j->N + k] = mulmod(j->in[i * j->N + k], hinv, q);
+}
+
+/* several FastBConv phase-2 jobs as one pool of (job, output) tasks, so that
+ * the digits of a modup or the polynomials of a moddown share the threads */
+typedef struct {
+    bconv_job *jobs;
+    size_t njobs;
+} bconv_multi;
+
+static void bconv_multi_task(void *arg, size_t t) {
+    bconv_multi *m = (bconv_multi *)arg;
+    size_t j = 0;
+    while (t >= m->jobs[j].no) t -= m->jobs[j++].no;
+    bconv_task(&m->jobs[j], t);
+}
+
+static void bconv_multi_run(bconv_job *jobs, size_t njobs) {
+    size_t n = 0;
+    for (size_t j = 0; j < njobs; j++) n += jobs[j].no;
+    bconv_multi m = {jobs, njobs};
+    parallel_for(n, bconv_multi_task, &m);
+}
+
 int orc_fastbconv(size_t N, const u64 *ibase, size_t ni, const u64 *obase, size_t no,
                   const u64 *in, u64 *out) {
     u64 *y = (u64 *)malloc(ni * N * 8);
     u64 *mat = (u64 *)malloc(no * ni * 8);
-    for (size_t i = 0; i < ni; i++) {
-        u64 q = ibase[i];
-        u64 hinv = invmod(punctured_mod(ibase, ni, i, q), q);
-        for (size_t k = 0; k < N; k++) y[i * N + k] = mulmod(in[i * N + k], hinv, q);
-    }
+    bconv_pre_job pj = {N, ni, ibase, in, y};
+    parallel_for(ni, bconv_pre_task, &pj);
     for (size_t o = 0; o < no; o++)
         for (size_t i = 0; i < ni; i++) mat[o * ni + i] = punctured_mod(ibase, ni, i, obase[o]);
     bconv_job j = {N, ni, no, ibase, obase, y, mat, out, NULL};
@@ -447,6 +502,23 @@ static size_t qlp_row(const orc_ctx *c, size_t size_Ql, size_t pos) {
  *     src/rns_bconv.cu:522-528); complement limbs = FastBConv phase 2 of the
  *     digit, written at their QlP positions (bconv_matmul_padded, 455-485)
  *  3. NTT of the complement limbs (exclude-range NTT, src/ntt/ntt_modup.cu:610-657) */
+typedef struct {
+    size_t N, alpha, size_Ql, beta;
+    const u64 *qlp;
+    u64 *t_cks;
+} scale_job;
+
+/* limb i of digit j = i / alpha: t_cks[i] *= [(Q_j / q_i)^-1]_{q_i} (partQlHatInv) */
+static void modup_scale_task(void *arg, size_t i) {
+    scale_job *sj = (scale_job *)arg;
+    size_t N = sj->N, j = i / sj->alpha;
+    size_t st = sj->alpha * j;
+    size_t sz = (j == sj->beta - 1) ? sj->size_Ql - sj->alpha * (sj->beta - 1) : sj->alpha;
+    u64 q = sj->qlp[i];
+    u64 hinv = invmod(punctured_mod(sj->qlp + st, sz, i - st, q), q);
+    for (size_t k = 0; k < N; k++) sj->t_cks[i * N + k] = mulmod(sj->t_cks[i * N + k], hinv, q);
+}
+
 int orc_modup(const orc_ctx *c, size_t size_Ql, const u64 *c2, u64 *t_mod_up) {
     size_t N = c->N, alpha = c->size_P, size_QlP = size_Ql + c->size_P;
     size_t beta = orc_beta(c, size_Ql);
@@ -459,44 +531,46 @@ int orc_modup(const orc_ctx *c, size_t size_Ql, const u64 *c2, u64 *t_mod_up) {
     u64 *t_cks = (u64 *)malloc(size_Ql * N * 8);
     memcpy(t_cks, c2, size_Ql * N * 8);
     orc_ntt_inverse(c, t_cks, size_Ql, 0);
-    for (size_t j = 0; j < beta; j++) {
-        size_t st = alpha * j;
-        size_t sz = (j == beta - 1) ? size_Ql - alpha * (beta - 1) : alpha;
-        for (size_t i = st; i < st + sz; i++) {
-            u64 q = qlp[i];
-            u64 hinv = invmod(punctured_mod(qlp + st, sz, i - st, q), q);
-            for (size_t k = 0; k < N; k++) t_cks[i * N + k] = mulmod(t_cks[i * N + k], hinv, q);
-        }
-    }
+    scale_job sj = {N, alpha, size_Ql, beta, qlp, t_cks};
+    parallel_for(size_Ql, modup_scale_task, &sj);
+    /* per digit j: own limbs copied, complement limbs converted (all digits'
+     * conversions as one task pool), then one NTT over every complement limb */
+    bconv_job *jobs = (bconv_job *)malloc(beta * sizeof(bconv_job));
+    u64 **obases = (u64 **)malloc(beta * sizeof(u64 *));
+    size_t **orows = (size_t **)malloc(beta * sizeof(size_t *));
+    u64 **mats = (u64 **)malloc(beta * sizeof(u64 *));
+    size_t total = 0;
     for (size_t j = 0; j < beta; j++) {
         size_t st = alpha * j;
         size_t sz = (j == beta - 1) ? size_Ql - alpha * (beta - 1) : alpha;
         u64 *dst = t_mod_up + j * size_QlP * N;
         memcpy(dst + st * N, c2 + st * N, sz * N * 8);
         size_t no = size_QlP - sz;
-        u64 *obase = (u64 *)malloc(no * 8);
-        size_t *orow = (size_t *)malloc(no * sizeof(size_t));
-        size_t *ntt_rows_v = (size_t *)malloc(no * sizeof(size_t));
+        obases[j] = (u64 *)malloc(no * 8);
+        orows[j] = (size_t *)malloc(no * sizeof(size_t));
         for (size_t o = 0; o < no; o++) {
             size_t pos = o < st ? o : o + sz;
-            obase[o] = qlp[pos];
-            orow[o] = pos;
+            obases[j][o] = qlp[pos];
+            orows[j][o] = pos;
         }
-        u64 *mat = (u64 *)malloc(no * sz * 8);
+        mats[j] = (u64 *)malloc(no * sz * 8);
         for (size_t o = 0; o < no; o++)
-            for (size_t i = 0; i < sz; i++) mat[o * sz + i] = punctured_mod(qlp + st, sz, i, obase[o]);
-        bconv_job bj = {N, sz, no, qlp + st, obase, t_cks + st * N, mat, dst, orow};
-        parallel_for(no, bconv_task, &bj);
-        /* forward NTT of every complement limb with its own table row */
-        u64 *tmp = (u64 *)malloc(no * N * 8);
-        for (size_t o = 0; o < no; o++) {
-            memcpy(tmp + o * N, dst + orow[o] * N, N * 8);
-            ntt_rows_v[o] = rows[orow[o]];
-        }
-        ntt_rows(c, tmp, ntt_rows_v, no, 0);
-        for (size_t o = 0; o < no; o++) memcpy(dst + orow[o] * N, tmp + o * N, N * 8);
-        free(tmp); free(mat); free(obase); free(orow); free(ntt_rows_v);
+            for (size_t i = 0; i < sz; i++) mats[j][o * sz + i] = punctured_mod(qlp + st, sz, i, obases[j][o]);
+        jobs[j] = (bconv_job){N, sz, no, qlp + st, obases[j], t_cks + st * N, mats[j], dst, orows[j]};
+        total += no;
     }
+    bconv_multi_run(jobs, beta);
+    /* forward NTT of every complement limb, in place, with its own table row */
+    u64 **ptrs = (u64 **)malloc(total * sizeof(u64 *));
+    size_t *ntt_rows_v = (size_t *)malloc(total * sizeof(size_t));
+    for (size_t j = 0, t = 0; j < beta; j++)
+        for (size_t o = 0; o < jobs[j].no; o++, t++) {
+            ptrs[t] = jobs[j].out + orows[j][o] * N;
+            ntt_rows_v[t] = rows[orows[j][o]];
+        }
+    ntt_ptrs(c, ptrs, ntt_rows_v, total, 0);
+    for (size_t j = 0; j < beta; j++) { free(obases[j]); free(orows[j]); free(mats[j]); }
+    free(jobs); free(obases); free(orows); free(mats); free(ptrs); free(ntt_rows_v);
     free(t_cks); free(qlp); free(rows);
     return 0;
 }
@@ -546,6 +620,35 @@ int orc_inner_product(const orc_ctx *c, size_t size_Ql, const u64 *t_mod_up, con
  *  3. out = (cx_i[Ql] - NTT(delta)) * P^-1 mod q_i
  *     (fused phase 2, src/ntt/ntt_moddown.cu:106-216; sub_negate_const_mult
  *      include/uintmodmath.cuh:234-242) */
+typedef struct {
+    const orc_ctx *c;
+    const u64 *cx_i, *delta;
+    u64 *out;
+} moddown_tail_job;
+
+/* limb i: out = (cx_i - delta) * P^-1 mod q_i */
+static void moddown_tail_task(void *arg, size_t i) {
+    moddown_tail_job *tj = (moddown_tail_job *)arg;
+    const orc_ctx *c = tj->c;
+    size_t N = c->N;
+    u64 q = c->q[i];
+    u64 pinv = invmod(prod_mod(c->q + c->size_Q, c->size_P, q), q);
+    for (size_t k = 0; k < N; k++) {
+        u64 d = submod(tj->cx_i[i * N + k], tj->delta[i * N + k], q);
+        tj->out[i * N + k] = mulmod(d, pinv, q);
+    }
+}
+
+typedef struct {
+    moddown_tail_job *jobs;
+    size_t size_Ql;
+} moddown_tail_pair;
+
+static void moddown_tail_pair_task(void *arg, size_t t) {
+    moddown_tail_pair *tp = (moddown_tail_pair *)arg;
+    moddown_tail_task(&tp->jobs[t / tp->size_Ql], t % tp->size_Ql);
+}
+
 int orc_moddown(const orc_ctx *c, size_t size_Ql, const u64 *cx_i, u64 *out) {
     size_t N = c->N, P = c->size_P;
     u64 *xp = (u64 *)malloc(P * N * 8);
@@ -554,17 +657,49 @@ int orc_moddown(const orc_ctx *c, size_t size_Ql, const u64 *cx_i, u64 *out) {
     u64 *delta = (u64 *)malloc(size_Ql * N * 8);
     orc_fastbconv(N, c->q + c->size_Q, P, c->q, size_Ql, xp, delta);
     orc_ntt_forward(c, delta, size_Ql, 0);
-    for (size_t i = 0; i < size_Ql; i++) {
-        u64 q = c->q[i];
-        u64 pinv = invmod(prod_mod(c->q + c->size_Q, P, q), q);
-        for (size_t k = 0; k < N; k++) {
-            u64 d = submod(cx_i[i * N + k], delta[i * N + k], q);
-            out[i * N + k] = mulmod(d, pinv, q);
-        }
-    }
+    moddown_tail_job tj = {c, cx_i, delta, out};
+    parallel_for(size_Ql, moddown_tail_task, &tj);
     free(xp);
     free(delta);
     return 0;
+}
+
+/* orc_moddown of both halves cx[0], cx[1] (each [QlP][N]) into out2[0], out2[1]
+ * (each [Ql][N]), their limbs pooled in every step so that they share the
+ * threads: INTT of the 2P special limbs, FastBConv P -> Ql of both, NTT of the
+ * 2 Ql delta limbs, the tail of every limb. Same values as two orc_moddown calls. */
+static void moddown_pair(const orc_ctx *c, size_t size_Ql, const u64 *cx, u64 *out2) {
+    size_t N = c->N, P = c->size_P, size_QlP = size_Ql + P;
+    u64 *xp = (u64 *)malloc(2 * P * N * 8);
+    size_t *prow = (size_t *)malloc(2 * P * sizeof(size_t));
+    for (size_t h = 0; h < 2; h++) {
+        memcpy(xp + h * P * N, cx + h * size_QlP * N + size_Ql * N, P * N * 8);
+        for (size_t j = 0; j < P; j++) prow[h * P + j] = c->size_Q + j;
+    }
+    ntt_rows(c, xp, prow, 2 * P, 1);
+    /* FastBConv phase 1 (both halves share the scales) and phase 2 */
+    const u64 *ibase = c->q + c->size_Q;
+    u64 *y = (u64 *)malloc(2 * P * N * 8);
+    bconv_pre_job pj = {N, P, ibase, xp, y};
+    bconv_pre_job pj2 = {N, P, ibase, xp + P * N, y + P * N};
+    parallel_for(P, bconv_pre_task, &pj);
+    parallel_for(P, bconv_pre_task, &pj2);
+    u64 *mat = (u64 *)malloc(size_Ql * P * 8);
+    for (size_t o = 0; o < size_Ql; o++)
+        for (size_t i = 0; i < P; i++) mat[o * P + i] = punctured_mod(ibase, P, i, c->q[o]);
+    u64 *delta = (u64 *)malloc(2 * size_Ql * N * 8);
+    bconv_job jobs[2] = {{N, P, size_Ql, ibase, c->q, y, mat, delta, NULL},
+                         {N, P, size_Ql, ibase, c->q, y + P * N, mat, delta + size_Ql * N, NULL}};
+    bconv_multi_run(jobs, 2);
+    size_t *drow = (size_t *)malloc(2 * size_Ql * sizeof(size_t));
+    for (size_t h = 0; h < 2; h++)
+        for (size_t i = 0; i < size_Ql; i++) drow[h * size_Ql + i] = i;
+    ntt_rows(c, delta, drow, 2 * size_Ql, 0);
+    moddown_tail_job tj[2] = {{c, cx, delta, out2},
+                              {c, cx + size_QlP * N, delta + size_Ql * N, out2 + size_Ql * N}};
+    moddown_tail_pair tp = {tj, size_Ql};
+    parallel_for(2 * size_Ql, moddown_tail_pair_task, &tp);
+    free(xp); free(prow); free(y); free(mat); free(delta); free(drow);
 }
 
 /* keyswitch_inplace core (src/eval_key_switch.cu:95-182): modup, inner
@@ -579,8 +714,7 @@ int orc_keyswitch(const orc_ctx *c, size_t size_Ql, const u64 *c2, const u64 *ke
     u64 *cx = (u64 *)malloc(2 * size_QlP * N * 8);
     orc_modup(c, size_Ql, c2, t);
     orc_inner_product(c, size_Ql, t, key, dnum, cx);
-    orc_moddown(c, size_Ql, cx, out2);
-    orc_moddown(c, size_Ql, cx + size_QlP * N, out2 + size_Ql * N);
+    moddown_pair(c, size_Ql, cx, out2);
     free(t);
     free(cx);
     return 0;
@@ -592,25 +726,55 @@ int orc_keyswitch(const orc_ctx *c, size_t size_Ql, const u64 *c2, const u64 *ke
 
 /* tensor_prod_2x2_rns_poly (src/polymath.cu:463-498): d0 = c0 c0',
  * d2 = c1 c1', d1 = (c0+c1)(c0'+c1') - d0 - d2 (= c0 c1' + c1 c0') */
-void orc_tensor(const orc_ctx *c, size_t size_Ql, const u64 *ct1, const u64 *ct2, u64 *out3) {
-    size_t N = c->N, LN = size_Ql * N;
-    for (size_t i = 0; i < size_Ql; i++) {
-        u64 q = c->q[i];
-        for (size_t k = 0; k < N; k++) {
-            size_t o = i * N + k;
-            u64 a0 = ct1[o], a1 = ct1[LN + o], b0 = ct2[o], b1 = ct2[LN + o];
-            u64 d0 = mulmod(a0, b0, q), d2 = mulmod(a1, b1, q);
-            u64 d1 = mulmod(addmod(a0, a1, q), addmod(b0, b1, q), q);
-            d1 = submod(submod(d1, d0, q), d2, q);
-            out3[o] = d0;
-            out3[LN + o] = d1;
-            out3[2 * LN + o] = d2;
-        }
+typedef struct {
+    const orc_ctx *c;
+    size_t LN;
+    const u64 *ct1, *ct2;
+    u64 *out3;
+} tensor_job;
+
+static void tensor_task(void *arg, size_t i) {
+    tensor_job *j = (tensor_job *)arg;
+    size_t N = j->c->N, LN = j->LN;
+    u64 q = j->c->q[i];
+    for (size_t k = 0; k < N; k++) {
+        size_t o = i * N + k;
+        u64 a0 = j->ct1[o], a1 = j->ct1[LN + o], b0 = j->ct2[o], b1 = j->ct2[LN + o];
+        u64 d0 = mulmod(a0, b0, q), d2 = mulmod(a1, b1, q);
+        u64 d1 = mulmod(addmod(a0, a1, q), addmod(b0, b1, q), q);
+        d1 = submod(submod(d1, d0, q), d2, q);
+        j->out3[o] = d0;
+        j->out3[LN + o] = d1;
+        j->out3[2 * LN + o] = d2;
     }
+}
+
+void orc_tensor(const orc_ctx *c, size_t size_Ql, const u64 *ct1, const u64 *ct2, u64 *out3) {
+    tensor_job j = {c, size_Ql * c->N, ct1, ct2, out3};
+    parallel_for(size_Ql, tensor_task, &j);
 }
 
 /* multiply_and_relin_inplace for CKKS (src/evaluate.cu:1061-1104):
  * bgv_ckks_multiply (345-397) then relinearize_inplace (1342-1374) */
+typedef struct {
+    const orc_ctx *c;
+    size_t LN;
+    const u64 *d, *ks;
+    u64 *out2;
+} add_job;
+
+/* limb i of both output polynomials: d + ks (add_to_ct_kernel) */
+static void add_ks_task(void *arg, size_t i) {
+    add_job *aj = (add_job *)arg;
+    size_t N = aj->c->N, LN = aj->LN;
+    u64 q = aj->c->q[i];
+    for (size_t k = 0; k < N; k++) {
+        size_t o = i * N + k;
+        aj->out2[o] = addmod(aj->d[o], aj->ks[o], q);
+        aj->out2[LN + o] = addmod(aj->d[LN + o], aj->ks[LN + o], q);
+    }
+}
+
 int orc_multiply_and_relin(const orc_ctx *c, size_t size_Ql, const u64 *ct1, const u64 *ct2,
                            const u64 *key, size_t dnum, u64 *out2) {
     size_t N = c->N, LN = size_Ql * N;
@@ -619,14 +783,8 @@ int orc_multiply_and_relin(const orc_ctx *c, size_t size_Ql, const u64 *ct1, con
     orc_tensor(c, size_Ql, ct1, ct2, d);
     int rc = orc_keyswitch(c, size_Ql, d + 2 * LN, key, dnum, ks);
     if (rc == 0) {
-        for (size_t i = 0; i < size_Ql; i++) {
-            u64 q = c->q[i];
-            for (size_t k = 0; k < N; k++) {
-                size_t o = i * N + k;
-                out2[o] = addmod(d[o], ks[o], q);
-                out2[LN + o] = addmod(d[LN + o], ks[LN + o], q);
-            }
-        }
+        add_job aj = {c, LN, d, ks, out2};
+        parallel_for(size_Ql, add_ks_task, &aj);
     }
     free(d);
     free(ks);
