@@ -41,14 +41,25 @@ struct StoreModDown {
     u64 ct_stride, cx_stride;
     u32 N;
     u32 pad;
-    __device__ __forceinline__ void operator()(u64 *, u32 i, u64 v, u32 limb, u32 poly,
+    __device__ __forceinline__ void operator()(u64 *d, u32 i, u64 v, u32 limb, u32 poly,
                                                const DevModulus &m) const {
+        apply(d, i, v, prefetch(i, limb, poly), limb, poly, m);
+    }
+    // the tail's operands, loaded by the row pass while its outputs wait in LDS
+    struct Pre {
+        u64 cx, base;
+    };
+    __device__ __forceinline__ Pre prefetch(u32 i, u32 limb, u32 poly) const {
+        const u64 off = (u64)limb * N + i;
+        const u64 *base = poly == 0 ? base0 : base1;
+        return Pre{gmem(cx)[poly * cx_stride + off], base ? gmem(base)[off] : 0};
+    }
+    __device__ __forceinline__ void apply(u64 *, u32 i, u64 v, const Pre &pf, u32 limb, u32 poly,
+                                          const DevModulus &m) const {
         const u64 q = m.value;
         const u64 off = (u64)limb * N + i;
-        const u64 d = sub_mod(gmem(cx)[poly * cx_stride + off], v, q);
-        const u64 r = mul_shoup(d, pinv[2 * limb], pinv[2 * limb + 1], q);
-        const u64 *base = poly == 0 ? base0 : base1;
-        gmem(ct)[poly * ct_stride + off] = base ? add_mod(gmem(base)[off], r, q) : r;
+        const u64 r = mul_shoup(sub_mod(pf.cx, v, q), pinv[2 * limb], pinv[2 * limb + 1], q);
+        gmem(ct)[poly * ct_stride + off] = add_mod(pf.base, r, q);
     }
 };
 
@@ -88,12 +99,22 @@ struct StoreRescale {
     const u64 *qinv;     // [Ql-1][2] q_last^-1 mod q_j, Shoup
     u64 in_stride, out_stride;
     u32 N;
-    __device__ __forceinline__ void operator()(u64 *, u32 i, u64 v, u32 limb, u32 poly,
+    __device__ __forceinline__ void operator()(u64 *d, u32 i, u64 v, u32 limb, u32 poly,
                                                const DevModulus &m) const {
+        apply(d, i, v, prefetch(i, limb, poly), limb, poly, m);
+    }
+    // the input limb, loaded by the row pass while its outputs wait in LDS
+    struct Pre {
+        u64 x;
+    };
+    __device__ __forceinline__ Pre prefetch(u32 i, u32 limb, u32 poly) const {
+        return Pre{gmem(in)[poly * in_stride + (u64)limb * N + i]};
+    }
+    __device__ __forceinline__ void apply(u64 *, u32 i, u64 v, const Pre &pf, u32 limb, u32 poly,
+                                          const DevModulus &m) const {
         const u64 q = m.value;
         const u64 off = (u64)limb * N + i;
-        const u64 d = sub_mod(gmem(in)[poly * in_stride + off], v, q);
-        gmem(out)[poly * out_stride + off] = mul_shoup(d, qinv[2 * limb], qinv[2 * limb + 1], q);
+        gmem(out)[poly * out_stride + off] = mul_shoup(sub_mod(pf.x, v, q), qinv[2 * limb], qinv[2 * limb + 1], q);
     }
 };
 
@@ -109,11 +130,21 @@ struct StoreModDownJob {
         const u64 *base;
         u64 p, ps;
         u32 add_out;
-        __device__ __forceinline__ void operator()(u64 *, u32 i, u64 v, u32, u32, const DevModulus &m) const {
+        __device__ __forceinline__ void operator()(u64 *d, u32 i, u64 v, u32 limb, u32 z,
+                                                   const DevModulus &m) const {
+            apply(d, i, v, prefetch(i, limb, z), limb, z, m);
+        }
+        // the tail's operands, loaded by the row pass while its outputs wait in LDS
+        struct Pre {
+            u64 cx, base;
+        };
+        __device__ __forceinline__ Pre prefetch(u32 i, u32, u32) const {
+            return Pre{gmem(cx)[i], base ? gmem(base)[i] : (add_out ? gmem(out)[i] : 0)};
+        }
+        __device__ __forceinline__ void apply(u64 *, u32 i, u64 v, const Pre &pf, u32, u32,
+                                              const DevModulus &m) const {
             const u64 q = m.value;
-            const u64 r = mul_shoup(sub_mod(gmem(cx)[i], v, q), p, ps, q);
-            const u64 b = base ? gmem(base)[i] : (add_out ? gmem(out)[i] : 0);
-            gmem(out)[i] = add_mod(b, r, q);
+            gmem(out)[i] = add_mod(pf.base, mul_shoup(sub_mod(pf.cx, v, q), p, ps, q), q);
         }
     };
     __device__ __forceinline__ Bound bind(const NttJob &J) const {
@@ -139,9 +170,19 @@ struct StoreRescaleJob {
         const u64 *in;
         u64 *out;
         u64 p, ps;
-        __device__ __forceinline__ void operator()(u64 *, u32 i, u64 v, u32, u32, const DevModulus &m) const {
+        __device__ __forceinline__ void operator()(u64 *d, u32 i, u64 v, u32 limb, u32 z,
+                                                   const DevModulus &m) const {
+            apply(d, i, v, prefetch(i, limb, z), limb, z, m);
+        }
+        // the input limb, loaded by the row pass while its outputs wait in LDS
+        struct Pre {
+            u64 x;
+        };
+        __device__ __forceinline__ Pre prefetch(u32 i, u32, u32) const { return Pre{gmem(in)[i]}; }
+        __device__ __forceinline__ void apply(u64 *, u32 i, u64 v, const Pre &pf, u32, u32,
+                                              const DevModulus &m) const {
             const u64 q = m.value;
-            gmem(out)[i] = mul_shoup(sub_mod(gmem(in)[i], v, q), p, ps, q);
+            gmem(out)[i] = mul_shoup(sub_mod(pf.x, v, q), p, ps, q);
         }
     };
     __device__ __forceinline__ Bound bind(const NttJob &J) const {

@@ -17,6 +17,12 @@
 namespace phx {
 namespace {
 
+// a store policy that loads operands of its own (Pre, prefetch, apply)
+template <class S, class = void>
+struct HasPrefetch : std::false_type {};
+template <class S>
+struct HasPrefetch<S, std::void_t<typename S::Pre>> : std::true_type {};
+
 // LDS address helpers (u64 units). Column tiles pad each row of TC columns
 // by one slot; row tiles pad every 16 coefficients by one slot so the
 // 16-apart accesses of the radix-16 exchange fall in distinct banks.
@@ -156,9 +162,13 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
 // returns the canonical outputs at tile index t + 256 k in out[k]. The LDS
 // region is free again after the caller's next barrier.
 // FromLds: the tile's values are in LDS (row_addr layout) instead of in data.
-template <int NT, class A, bool FromLds = false>
+struct NoPrefetch {
+    __device__ __forceinline__ void operator()() const {}
+};
+
+template <int NT, class A, bool FromLds = false, class Pre = NoPrefetch>
 __device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *data, u32 logN, u32 r0, const u32 t,
-                                              u64 (&out)[16]) {
+                                              u64 (&out)[16], const Pre &pre = Pre{}) {
     using T = typename A::T;
     const u32 l = t & 15, u = t >> 4;
     T v[16];
@@ -178,6 +188,7 @@ __device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *d
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; ++e) lds[row_addr(u, (hp << 4) + e)] = a.fwd_canon(v[e]);
+    pre();   // the store policy's loads, in flight across the barrier
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
@@ -192,15 +203,15 @@ __device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *d
 // groups of 2^(SR-4) consecutive rows) on the tile's columns, then the last 8
 // stages as fwd_rows_core. Thread t holds column t % 256 of rows
 // r0 + 16 (t / 256) + e (NT = 256 or 512: 16 rows per 256 threads).
-template <int SR, int NT, class A>
+template <int SR, int NT, class A, class Pre = NoPrefetch>
 __device__ __forceinline__ void fwd_rows_core_rb(const A &a, u64 *lds, const u64 *data, u32 logN, u32 r0,
-                                                 const u32 t, u64 (&out)[16]) {
+                                                 const u32 t, u64 (&out)[16], const Pre &pre = Pre{}) {
     static_assert(NT % 256 == 0 && SR >= 4 && SR <= 8, "column-first row pass: 256 or 512 threads, N = 2^12 .. 2^16");
     using T = typename A::T;
     constexpr int NB = SR - 4;
     if constexpr (NB == 0) {
         // N = 2^12: the column pass has only the round the fused launch ran
-        fwd_rows_core<NT, A>(a, lds, data, logN, r0, t, out);
+        fwd_rows_core<NT, A, false, Pre>(a, lds, data, logN, r0, t, out, pre);
         return;
     }
     constexpr int BB = 16 >> NB;
@@ -215,26 +226,49 @@ __device__ __forceinline__ void fwd_rows_core_rb(const A &a, u64 *lds, const u64
 #pragma unroll
     for (int e = 0; e < 16; ++e) lds[row_addr(16 * (t >> 8) + (u32)e, cc)] = A::to_bits(v[e]);
     __syncthreads();
-    fwd_rows_core<NT, A, true>(a, lds, data, logN, r0, t, out);
+    fwd_rows_core<NT, A, true, Pre>(a, lds, data, logN, r0, t, out, pre);
 }
 
 template <int SR, int NT, class A, class Store>
 __device__ __forceinline__ void fwd_rows_body_rb(const A &a, u64 *lds, u64 *data, u32 logN, u32 r0, u32 limb, u32 z,
                                                  const DevModulus &md, const Store &st, const u32 t) {
     u64 out[16];
-    fwd_rows_core_rb<SR, NT>(a, lds, data, logN, r0, t, out);
+    if constexpr (HasPrefetch<Store>::value) {
+        typename Store::Pre pf[16];
+        const auto pre = [&] {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) st(data, r0 * 256 + t + NT * k, out[k], limb, z, md);
+            for (int k = 0; k < 16; ++k) pf[k] = st.prefetch(r0 * 256 + t + NT * k, limb, z);
+        };
+        fwd_rows_core_rb<SR, NT>(a, lds, data, logN, r0, t, out, pre);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) st.apply(data, r0 * 256 + t + NT * k, out[k], pf[k], limb, z, md);
+    } else {
+        fwd_rows_core_rb<SR, NT>(a, lds, data, logN, r0, t, out);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) st(data, r0 * 256 + t + NT * k, out[k], limb, z, md);
+    }
 }
 
-// Forward pass 2 body: the core, then the store policy per output.
+// Forward pass 2 body: the core, then the store policy per output (a policy
+// with a prefetch has its operands loaded while the outputs wait in LDS).
 template <int NT, class A, class Store>
 __device__ __forceinline__ void fwd_rows_body(const A &a, u64 *lds, u64 *data, u32 logN, u32 r0, u32 limb, u32 z,
                                               const DevModulus &md, const Store &st, const u32 t) {
     u64 out[16];
-    fwd_rows_core<NT>(a, lds, data, logN, r0, t, out);
+    if constexpr (HasPrefetch<Store>::value) {
+        typename Store::Pre pf[16];
+        const auto pre = [&] {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) st(data, r0 * 256 + t + NT * k, out[k], limb, z, md);
+            for (int k = 0; k < 16; ++k) pf[k] = st.prefetch(r0 * 256 + t + NT * k, limb, z);
+        };
+        fwd_rows_core<NT, A, false>(a, lds, data, logN, r0, t, out, pre);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) st.apply(data, r0 * 256 + t + NT * k, out[k], pf[k], limb, z, md);
+    } else {
+        fwd_rows_core<NT>(a, lds, data, logN, r0, t, out);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) st(data, r0 * 256 + t + NT * k, out[k], limb, z, md);
+    }
 }
 
 // Inverse pass 1 body: the first 8 stages (distances 1 .. 128) on 16 rows.
