@@ -125,6 +125,28 @@ __global__ __launch_bounds__(FcShape<SR>::NT, 4) void k_bconv_fwd_cols(BconvArgs
     // a lane reads input ni - 1 instead. Per lane the input offsets are fixed and
     // per step the base advances by R/16 rows (a uniform address, so the step
     // costs no vector instructions).
+    // the first column round's twiddles of the group's four outputs (entries
+    // 1 .. 15 of their table rows: with uniform indices the round reads only
+    // those) are staged in LDS, and the output position read, before the loop, so
+    // that the round after it waits on no global load
+    // (every wave writes the same 64 entries: no branch on the lane index, whose
+    // divergence would cost the loop loads below their uniform-base form)
+    __shared__ __align__(16) u64 s_tw[4][32];
+    const u32 *crow = reinterpret_cast<const u32 *>(A.cbuf) + d.row_off;
+    const bool grp_fp = T.mod[crow[4 * grp]].pad != 0;
+    {
+        const u32 g = (tid >> 4) & 3, k = tid & 15;
+        const u64 rg = crow[4 * grp + g];
+        if (grp_fp) {
+            s_tw[g][k] = (u64)__double_as_longlong(gmem(T.twd)[rg * N + k]);
+        } else {
+            const auto w = gmem(reinterpret_cast<const u64 *>(T.tw + rg * N + k));
+            s_tw[g][2 * k] = w[0];
+            s_tw[g][2 * k + 1] = w[1];
+        }
+    }
+    const u32 opos = (reinterpret_cast<const u32 *>(A.cbuf) + d.pos_off)[o];
+    __syncthreads();
     const u32 base = low * 256 + c0 + ln;
     const u64 *in0 = inb + (u64)d.in_first * N;
     u32 voff[2][2];
@@ -187,16 +209,19 @@ __global__ __launch_bounds__(FcShape<SR>::NT, 4) void k_bconv_fwd_cols(BconvArgs
         raw[e] = reduce(acc[e & 1]);
     }
 
-    const u32 opos = (reinterpret_cast<const u32 *>(A.cbuf) + d.pos_off)[o];
     u64 *outp = outb + (u64)(opos == 0xffffffffu ? 0 : opos) * N;
     const bool store = opos != 0xffffffffu;
     // the group's four outputs share one arithmetic (a uniform branch): the integer
     // one when the group's first output needs it (integer outputs come first)
-    const bool grp_fp = T.mod[(reinterpret_cast<const u32 *>(A.cbuf) + d.row_off)[4 * grp]].pad != 0;
-    if (grp_fp)
-        fc_round_a<SR>(fp_arith(T.twd + (u64)row * N, md), raw, outp, store, T.logN, base);
-    else
-        fc_round_a<SR>(int_fwd(T, row, p), raw, outp, store, T.logN, base, md.pad != 0);
+    if (grp_fp) {
+        FpArith fa = fp_arith(T.twd + (u64)row * N, md);
+        fa.tw = reinterpret_cast<const double *>(&s_tw[lg][0]);
+        fc_round_a<SR>(fa, raw, outp, store, T.logN, base);
+    } else {
+        IntArith ia = int_fwd(T, row, p);
+        ia.tw = reinterpret_cast<const TwPair *>(&s_tw[lg][0]);
+        fc_round_a<SR>(ia, raw, outp, store, T.logN, base, md.pad != 0);
+    }
 }
 
 // true when the fused launch serves these conversions: ring degree 2^12 .. 2^16,
