@@ -68,15 +68,18 @@ struct IntArith {
     // in [0, 3q) instead of [0, 2q) for any 64-bit y. x is reduced below 3q, so
     // both outputs stay below 6q < 2^64 (q < 2^61): one 32-bit multiply fewer
     // per butterfly.
-    __device__ __forceinline__ void ct(T &x, T &y, u32 idx) const {
-        const TwPair w = tw[idx];
+    using W = TwPair;   // a twiddle as the butterflies take it
+    __device__ __forceinline__ W twiddle(u32 idx) const { return tw[idx]; }
+    __device__ __forceinline__ void ct_w(T &x, T &y, const W &w) const {
         const u64 t = y * w.w + shoup_quot_lo(y, w.ws) * nq;
         const u64 xx = csub_n(x, q3, nq3);
         y = xx + q3p1 + ~t;   // xx + 3q - t
         x = xx + t;
     }
+    __device__ __forceinline__ void ct(T &x, T &y, u32 idx) const { ct_w(x, y, tw[idx]); }
     // the lazy integer butterfly keeps its range by the reduction above on every stage
     __device__ __forceinline__ void ct_lazy(T &x, T &y, u32 idx) const { ct(x, y, idx); }
+    __device__ __forceinline__ void ct_lazy_w(T &x, T &y, const W &w) const { ct_w(x, y, w); }
     // the inverse butterfly (gs_bfly in common.h) on values below 3q: the sum is
     // reduced by 3q, and the product of x + 3q - y takes shoup_quot_lo, which
     // leaves it in [0, 3q) (as in ct)
@@ -131,21 +134,25 @@ struct FpArith {
     }
     __device__ __forceinline__ u64 fwd_canon(T v) const { return canon(v); }
 
-    __device__ __forceinline__ void ct(T &x, T &y, u32 idx) const {
+    using W = double;
+    __device__ __forceinline__ W twiddle(u32 idx) const { return tw[idx]; }
+    __device__ __forceinline__ void ct_w(T &x, T &y, W w) const {
 #pragma clang fp contract(off)
-        const T t = mulmod(y, tw[idx]);
+        const T t = mulmod(y, w);
         const T xr = red(x);
         x = xr + t;
         y = xr - t;
     }
+    __device__ __forceinline__ void ct(T &x, T &y, u32 idx) const { ct_w(x, y, tw[idx]); }
     // the forward butterfly without the reduction of x (see the bounds above)
-    __device__ __forceinline__ void ct_lazy(T &x, T &y, u32 idx) const {
+    __device__ __forceinline__ void ct_lazy_w(T &x, T &y, W w) const {
 #pragma clang fp contract(off)
-        const T t = mulmod(y, tw[idx]);
+        const T t = mulmod(y, w);
         const T xr = x;
         x = xr + t;
         y = xr - t;
     }
+    __device__ __forceinline__ void ct_lazy(T &x, T &y, u32 idx) const { ct_lazy_w(x, y, tw[idx]); }
     __device__ __forceinline__ void gs(T &x, T &y, u32 idx) const {
 #pragma clang fp contract(off)
         const T s = x + y, d = x - y;

@@ -166,9 +166,11 @@ struct NoPrefetch {
     __device__ __forceinline__ void operator()() const {}
 };
 
+// FromLds: the caller loaded round 1's twiddles (w1) across its barrier.
 template <int NT, class A, bool FromLds = false, class Pre = NoPrefetch>
 __device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *data, u32 logN, u32 r0, const u32 t,
-                                              u64 (&out)[16], const Pre &pre = Pre{}) {
+                                              u64 (&out)[16], const Pre &pre = Pre{},
+                                              const typename A::W *w1 = nullptr) {
     using T = typename A::T;
     const u32 l = t & 15, u = t >> 4;
     T v[16];
@@ -176,15 +178,21 @@ __device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *d
 #pragma unroll
     for (int e = 0; e < 16; ++e)
         v[e] = A::from_bits(FromLds ? lds[row_addr(u, ((u32)e << 4) + l)] : ld_mid(data + baseA[0] + ((u32)e << 4)));
-    fwd_round<4, 4>(v, baseA, logN, a);
+    if constexpr (FromLds)
+        fwd_round_w(v, w1, a);
+    else
+        fwd_round<4, 4>(v, baseA, logN, a);
 #pragma unroll
     for (int e = 0; e < 16; ++e) lds[row_addr(u, ((u32)e << 4) + l)] = A::to_bits(v[e]);
-    __syncthreads();
+    // round 2's twiddles load while the values cross the barrier
     const u32 hp = l;
     u32 baseB[1] = {(r0 + u) * 256 + (hp << 4)};
+    typename A::W w2[15];
+    load_round_tw<0>(a, baseB[0], logN, w2);
+    __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = A::from_bits(lds[row_addr(u, (hp << 4) + e)]);
-    fwd_round<4, 0>(v, baseB, logN, a);
+    fwd_round_w(v, w2, a);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; ++e) lds[row_addr(u, (hp << 4) + e)] = a.fwd_canon(v[e]);
@@ -225,8 +233,11 @@ __device__ __forceinline__ void fwd_rows_core_rb(const A &a, u64 *lds, const u64
     fwd_round<NB, 8>(v, base, logN, a);
 #pragma unroll
     for (int e = 0; e < 16; ++e) lds[row_addr(16 * (t >> 8) + (u32)e, cc)] = A::to_bits(v[e]);
+    // the row pass's round 1 twiddles load while the values cross the barrier
+    typename A::W w1[15];
+    load_round_tw<4>(a, (r0 + (t >> 4)) * 256 + (t & 15), logN, w1);
     __syncthreads();
-    fwd_rows_core<NT, A, true, Pre>(a, lds, data, logN, r0, t, out, pre);
+    fwd_rows_core<NT, A, true, Pre>(a, lds, data, logN, r0, t, out, pre, w1);
 }
 
 template <int SR, int NT, class A, class Store>

@@ -177,6 +177,37 @@ __device__ __forceinline__ void fwd_round(typename A::T (&v)[16], const u32 (&ba
     }
 }
 
+// The 15 twiddles of a radix-16 forward round (NB = 4, one base per thread) in the
+// order fwd_round_w takes them: stage jj's 8 >> jj entries k = 0 .. at slot
+// (8 >> jj) - 1 + k. A pass loads them while its values wait in LDS across a
+// barrier, so the round after the barrier waits on no global load.
+template <int P, class A>
+__device__ __forceinline__ void load_round_tw(const A &a, u32 base, u32 logN, typename A::W (&w)[15]) {
+#pragma unroll
+    for (int jj = 3; jj >= 0; --jj) {
+        const u32 m = 1u << (logN - (P + jj + 1));
+#pragma unroll
+        for (int k = 0; k < (8 >> jj); ++k) w[(8 >> jj) - 1 + k] = a.twiddle(tw_index<P, 4>(m, base, jj, k << (jj + 1)));
+    }
+}
+
+// fwd_round<4, P> with its twiddles already in registers (load_round_tw)
+template <class A>
+__device__ __forceinline__ void fwd_round_w(typename A::T (&v)[16], const typename A::W *w, const A &a) {
+#pragma unroll
+    for (int jj = 3; jj >= 0; --jj) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            if (e & (1 << jj)) continue;
+            const auto &tw = w[(8 >> jj) - 1 + (e >> (jj + 1))];
+            if (jj & 1)
+                a.ct_lazy_w(v[e], v[e + (1 << jj)], tw);
+            else
+                a.ct_w(v[e], v[e + (1 << jj)], tw);
+        }
+    }
+}
+
 template <int NB, int P, class A>
 __device__ __forceinline__ void inv_round(typename A::T (&v)[16], const u32 (&base)[16 >> NB], u32 logN,
                                           const A &a) {
