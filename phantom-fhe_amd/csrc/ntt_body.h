@@ -127,7 +127,6 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
 #pragma unroll
             for (int e = 0; e < (1 << NA); ++e)
                 lds[col_addr<TC>(((u32)e << NB) + lowA[b], uA[b])] = A::to_bits(v[b * (1 << NA) + e]);
-        __syncthreads();
         constexpr int BB = 16 >> NB;
         u32 baseB[BB], uB[BB], hpB[BB];
 #pragma unroll
@@ -137,12 +136,20 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
             hpB[b] = kap / TC;
             baseB[b] = (hpB[b] << NB) * 256 + c0 + uB[b];
         }
+        // a full radix-16 second round loads its twiddles while the values cross
+        // the barrier
+        typename A::W wB[15];
+        if constexpr (NB == 4) load_round_tw<8>(a, baseB[0], logN, wB);
+        __syncthreads();
 #pragma unroll
         for (int b = 0; b < BB; ++b)
 #pragma unroll
             for (int e = 0; e < (1 << NB); ++e)
                 v[b * (1 << NB) + e] = A::from_bits(lds[col_addr<TC>((hpB[b] << NB) + e, uB[b])]);
-        fwd_round<NB, 8>(v, baseB, logN, a);
+        if constexpr (NB == 4)
+            fwd_round_w(v, wB, a);
+        else
+            fwd_round<NB, 8>(v, baseB, logN, a);
 #pragma unroll
         for (int b = 0; b < BB; ++b)
 #pragma unroll
