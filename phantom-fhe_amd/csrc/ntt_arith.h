@@ -83,12 +83,12 @@ struct IntArith {
     // the inverse butterfly (gs_bfly in common.h) on values below 3q: the sum is
     // reduced by 3q, and the product of x + 3q - y takes shoup_quot_lo, which
     // leaves it in [0, 3q) (as in ct)
-    __device__ __forceinline__ void gs(T &x, T &y, u32 idx) const {
-        const TwPair w = tw[idx];
+    __device__ __forceinline__ void gs_w(T &x, T &y, const W &w) const {
         const u64 d = x + q3p1 + ~y;   // x + 3q - y
         x = csub_n(x + y, q3, nq3);
         y = d * w.w + shoup_quot_lo(d, w.ws) * nq;
     }
+    __device__ __forceinline__ void gs(T &x, T &y, u32 idx) const { gs_w(x, y, tw[idx]); }
     // top inverse stage with both outputs scaled to canonical form (mul_shoup
     // takes any 64-bit input)
     __device__ __forceinline__ void gs_last(T &x, T &y, u64 s0, u64 s0s, u64 s1, u64 s1s) const {
@@ -153,12 +153,13 @@ struct FpArith {
         y = xr - t;
     }
     __device__ __forceinline__ void ct_lazy(T &x, T &y, u32 idx) const { ct_lazy_w(x, y, tw[idx]); }
-    __device__ __forceinline__ void gs(T &x, T &y, u32 idx) const {
+    __device__ __forceinline__ void gs_w(T &x, T &y, W w) const {
 #pragma clang fp contract(off)
         const T s = x + y, d = x - y;
         x = red(s);
-        y = mulmod(d, tw[idx]);
+        y = mulmod(d, w);
     }
+    __device__ __forceinline__ void gs(T &x, T &y, u32 idx) const { gs_w(x, y, tw[idx]); }
     __device__ __forceinline__ void gs_last(T &x, T &y, u64 s0, u64, u64 s1, u64) const {
 #pragma clang fp contract(off)
         const T s = x + y, d = x - y;

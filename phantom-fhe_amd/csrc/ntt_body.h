@@ -299,21 +299,26 @@ __device__ __forceinline__ void inv_rows_body(const A &a, u64 *lds, u64 *data, c
         const u32 idx = t + NT * k;
         lds[row_addr(idx >> 8, idx & 255)] = ld(src, r0 * 256 + idx, limb, z, md);
     }
-    __syncthreads();
+    // each register round's twiddles load while the values cross the barrier before it
     const u32 l = t & 15, u = t >> 4;
     T v[16];
     u32 baseB[1] = {(r0 + u) * 256 + (l << 4)};
+    typename A::W w1[15];
+    load_round_tw<0>(a, baseB[0], logN, w1);
+    __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = A::from_canon(lds[row_addr(u, (l << 4) + e)]);
-    inv_round<4, 0>(v, baseB, logN, a);
+    inv_round_w(v, w1, a);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; ++e) lds[row_addr(u, (l << 4) + e)] = A::to_bits(v[e]);
-    __syncthreads();
     u32 baseA[1] = {(r0 + u) * 256 + l};
+    typename A::W w2[15];
+    load_round_tw<4>(a, baseA[0], logN, w2);
+    __syncthreads();
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = A::from_bits(lds[row_addr(u, ((u32)e << 4) + l)]);
-    inv_round<4, 4>(v, baseA, logN, a);
+    inv_round_w(v, w2, a);
 #pragma unroll
     for (int e = 0; e < 16; ++e) st_mid(data + baseA[0] + ((u32)e << 4), A::to_bits(v[e]));
 }

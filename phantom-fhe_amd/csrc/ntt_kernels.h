@@ -208,6 +208,20 @@ __device__ __forceinline__ void fwd_round_w(typename A::T (&v)[16], const typena
     }
 }
 
+// inv_round<4, P> with its twiddles already in registers (load_round_tw over the
+// inverse table: the same entries and slots)
+template <class A>
+__device__ __forceinline__ void inv_round_w(typename A::T (&v)[16], const typename A::W *w, const A &a) {
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            if (e & (1 << jj)) continue;
+            a.gs_w(v[e], v[e + (1 << jj)], w[(8 >> jj) - 1 + (e >> (jj + 1))]);
+        }
+    }
+}
+
 template <int NB, int P, class A>
 __device__ __forceinline__ void inv_round(typename A::T (&v)[16], const u32 (&base)[16 >> NB], u32 logN,
                                           const A &a) {
