@@ -185,6 +185,8 @@ TuneResult autotune(Context &ctx, hipStream_t s) {
         std::vector<int> rows, cols;
         for (int t : {128, 256, 512})
             if ((u64)t * kNttRegs <= N) rows.push_back(t);
+        // column passes: 0 = by launch size (the default), or a fixed shape
+        cols.push_back(0);
         for (int t : {256, 512})
             if ((u64)t * kNttRegs <= N) cols.push_back(t);
         auto axis = [&](int TuneChoice::*field, const std::vector<int> &v, int inc) {
@@ -206,7 +208,7 @@ TuneResult autotune(Context &ctx, hipStream_t s) {
             best_us = inc_us;
         };
         axis(&TuneChoice::ntt_rows_threads, rows, 256);
-        axis(&TuneChoice::ntt_cols_threads, cols, 256);
+        axis(&TuneChoice::ntt_cols_threads, cols, 0);
         // 3. FastBConv kernel: the matrix-core form stays unless the VALU form wins
         TuneChoice mf = best;
         mf.bconv_mfma = 1;

@@ -869,8 +869,9 @@ void Context::set_tuning(const TuneChoice &t) {
     require(t.bconv_mfma >= -1 && t.bconv_mfma <= 1, "bconv_mfma must be -1, 0 or 1");
     require(t.bconv_fused >= -1 && t.bconv_fused <= 1, "bconv_fused must be -1, 0 or 1");
     require(t.ntt_chunk_limbs >= -1 && t.ntt_fused_min_limbs >= -1, "limb counts must be -1 or non-negative");
-    require(t.ntt_cols_threads == -1 || t.ntt_cols_threads == 256 || t.ntt_cols_threads == 512,
-            "ntt_cols_threads must be -1, 256 or 512");
+    require(t.ntt_cols_threads == -1 || t.ntt_cols_threads == 0 || t.ntt_cols_threads == 256 ||
+                t.ntt_cols_threads == 512,
+            "ntt_cols_threads must be -1, 0, 256 or 512");
     require(t.ntt_rows_threads == -1 || t.ntt_rows_threads == 128 || t.ntt_rows_threads == 256 ||
                 t.ntt_rows_threads == 512,
             "ntt_rows_threads must be -1, 128, 256 or 512");

@@ -153,7 +153,7 @@ struct TuneChoice {
     int ntt_fused_min_limbs = -1;
     int ntt_chunk_limbs = -1;    // limbs per two-pass launch pair (0 = one launch)
     int bconv_mfma = -1;         // 1: FastBConv on the matrix cores, 0: VALU
-    int ntt_cols_threads = -1;   // tile shape of the column passes: 256 or 512 threads
+    int ntt_cols_threads = -1;   // tile shape of the column passes: 256 or 512 threads, 0 by launch size
     int ntt_rows_threads = -1;   // tile shape of the row passes: 128, 256 or 512 threads
     int bconv_wgs_per_cu = -1;   // FastBConv (matrix cores) workgroups per CU: 1, 2, 4 (0 = as LDS allows)
     int bconv_fused = -1;        // 1: FastBConv fused with the forward NTT's column pass, 0: separate

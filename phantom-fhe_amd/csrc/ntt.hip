@@ -137,7 +137,7 @@ void launch_forward(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys
         fused_forward(G, T, M, nlimbs, npolys, ld, st, s);
         return;
     }
-    const u32 ntc = cols_threads(T), ntr = rows_threads(T);
+    const u32 ntc = cols_threads(T, (u64)nlimbs * npolys), ntr = rows_threads(T);
     // polynomials sharing a prime run next to each other (and, with 8 or 16
     // tiles per limb, on the same XCD), so the prime's twiddles stay in L2;
     // large launches go in slices of whole polynomials so that the pass-1
@@ -197,7 +197,7 @@ void ntt_forward_rows_store(const NttTables &T, const LimbMap &M, u32 nlimbs, u3
 namespace {
 template <class Load, class Store = StorePlain>
 void launch_inverse(const NttTables &T, const LimbMap &M, u32 nlimbs, u32 npolys, const Load &ld, hipStream_t s) {
-    const u32 ntc = cols_threads(T), ntr = rows_threads(T);
+    const u32 ntc = cols_threads(T, (u64)nlimbs * npolys), ntr = rows_threads(T);
     // launch order and slicing as in launch_forward
     const u32 step = chunk_polys(T, nlimbs, npolys);
     for (u32 p0 = 0; p0 < npolys; p0 += step) {

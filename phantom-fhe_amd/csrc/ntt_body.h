@@ -463,8 +463,17 @@ inline u32 pick_threads(int ctx_choice, const char *key, u32 N) {
     while ((u32)nt * kNttRegs > N && nt > 128) nt >>= 1;
     return (u32)nt;
 }
-inline u32 cols_threads(const NttTables &T) {
-    const u32 nt = pick_threads(T.cols_nt, "ntt_cols_threads", T.N);
+// Column tiles of 256 threads hold 16 columns at N = 2^16, so the pass reads and
+// writes its limb in 128-byte pieces 2 KiB apart; 512 threads (32 columns, 256-byte
+// pieces) move the same bytes faster (752-limb forward pair 0.340 -> 0.331 ms) but
+// halve the workgroups, which costs launches over few limbs (the plain HMult+relin's
+// 24..106-limb passes: -1%). Choice 0 takes 512 threads at N = 2^16 for launches of
+// at least kWideColsMin limb-transforms, else 256.
+constexpr u64 kWideColsMin = 256;
+inline u32 cols_threads(const NttTables &T, u64 transforms) {
+    int choice = T.cols_nt >= 0 ? T.cols_nt : tune_value("ntt_cols_threads");
+    if (choice == 0) choice = T.logN == 16 && transforms >= kWideColsMin ? 512 : 256;
+    const u32 nt = pick_threads(choice, "ntt_cols_threads", T.N);
     // column passes run on 256 or 512 threads, on 128 only where N = 2^11 needs it
     return nt == 128 && 256 * kNttRegs <= T.N ? 256 : nt;
 }

@@ -92,7 +92,7 @@ constexpr u32 kMaxJobsPerLaunch = 65535;
 template <class Load, class Store>
 void launch_forward_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, const Load &ld, const Store &st,
                          hipStream_t s) {
-    const u32 ntc = cols_threads(T), ntr = rows_threads(T);
+    const u32 ntc = cols_threads(T, njobs), ntr = rows_threads(T);
     for (u32 j0 = 0; j0 < njobs; j0 += kMaxJobsPerLaunch) {
         const u32 nj = std::min(kMaxJobsPerLaunch, njobs - j0);
         dispatch_cols(T.logN, ntc, [&](auto sr, auto nt) {
@@ -153,7 +153,7 @@ void ntt_forward_rows_jobs_moddown(const NttTables &T, const NttJob *jobs, u32 n
 namespace {
 template <class Load, class Store = StorePlain>
 void launch_inverse_jobs(const NttTables &T, const NttJob *jobs, u32 njobs, const Load &ld, hipStream_t s) {
-    const u32 ntc = cols_threads(T), ntr = rows_threads(T);
+    const u32 ntc = cols_threads(T, njobs), ntr = rows_threads(T);
     for (u32 j0 = 0; j0 < njobs; j0 += kMaxJobsPerLaunch) {
         const u32 nj = std::min(kMaxJobsPerLaunch, njobs - j0);
         dispatch_rows_nt(ntr, [&](auto nt) {

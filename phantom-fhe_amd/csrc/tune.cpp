@@ -10,8 +10,9 @@
 //   ntt_chunk_limbs  two-pass launches over more limbs than this are split into
 //                slices of whole polynomials of about this many limbs (0 = never)
 //   ntt_cols_threads / ntt_rows_threads  tile shape of the two passes: threads per
-//                workgroup (16 registers each): columns 256 (default) or 512, rows
-//                128, 256 (default) or 512
+//                workgroup (16 registers each): columns 256 or 512, or 0 (default)
+//                for 512 at N = 2^16 in launches of at least 256 limb-transforms and
+//                256 otherwise; rows 128, 256 (default) or 512
 //   bconv_wgs_per_cu  FastBConv on the matrix cores: workgroups per CU of its grid
 //                (1, 2, 4; 0 = as many as the LDS holds, the default)
 //   bconv_fused  1: FastBConv on the matrix cores fused with the column pass of the
@@ -40,7 +41,7 @@ std::atomic<int> g_ntt_fp64{env_is_one("PHX_NTT_INT_ONLY") ? 0 : 1};
 std::atomic<int> g_ntt_fused{0};
 std::atomic<int> g_ntt_fused_min{512};
 std::atomic<int> g_ntt_chunk{0};
-std::atomic<int> g_ntt_cols{256};
+std::atomic<int> g_ntt_cols{0};
 std::atomic<int> g_ntt_rows{256};
 std::atomic<int> g_bconv_wgs{0};
 std::atomic<int> g_batch_branches{2};
