@@ -585,23 +585,33 @@ __global__ void k_dot_secret(const u64 *ct, const u64 *s, const u64 *s2, u64 *ou
     out[o] = v;
 }
 
-// 16-byte streaming copy (nontemporal), grid-stride over a grid of four
-// workgroups per CU: the fastest plain copy form measured on MI355X
-// (~6.0 TB/s for 1 GiB each way, against ~4.6 TB/s for one-shot grids)
+// 16-byte streaming copy (nontemporal): each workgroup moves one 32 KiB chunk,
+// every thread issuing its eight loads before its stores (~6.5 TB/s for 1 GiB
+// each way on MI355X, against ~6.0 TB/s for a grid-stride loop over four
+// workgroups per CU and ~4.6 TB/s for one vector per thread)
 typedef unsigned int copy_vec __attribute__((ext_vector_type(4)));
+constexpr u32 kCopyVecs = 8;   // vectors per thread
 __global__ __launch_bounds__(kThreads) void k_copy(const copy_vec *__restrict__ src, copy_vec *__restrict__ dst,
                                                    u64 n16) {
-    for (u64 i = (u64)blockIdx.x * kThreads + threadIdx.x; i < n16; i += (u64)gridDim.x * kThreads)
-        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+    const u64 base = (u64)blockIdx.x * (kThreads * kCopyVecs) + threadIdx.x;
+    copy_vec v[kCopyVecs];
+#pragma unroll
+    for (u32 e = 0; e < kCopyVecs; ++e)
+        if (base + e * kThreads < n16) v[e] = __builtin_nontemporal_load(src + base + e * kThreads);
+#pragma unroll
+    for (u32 e = 0; e < kCopyVecs; ++e)
+        if (base + e * kThreads < n16) __builtin_nontemporal_store(v[e], dst + base + e * kThreads);
 }
 
 void copy_u64(u64 *dst, const u64 *src, size_t n, hipStream_t s) {
     require(n % 2 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0,
             "copy_u64 needs an even count and 16-byte aligned buffers");
     const u64 n16 = n / 2;
-    const u32 blocks = (u32)std::min<u64>((n16 + kThreads - 1) / kThreads, 256 * 4);
+    const u64 per_block = (u64)kThreads * kCopyVecs;
+    const u64 blocks = (n16 + per_block - 1) / per_block;
     if (blocks == 0) return;
-    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kThreads), 0, s, reinterpret_cast<const copy_vec *>(src),
+    require(blocks <= 0x7fffffffu, "copy_u64: too many words");
+    hipLaunchKernelGGL(k_copy, dim3((u32)blocks), dim3(kThreads), 0, s, reinterpret_cast<const copy_vec *>(src),
                        reinterpret_cast<copy_vec *>(dst), n16);
     PHX_HIP_CHECK(hipGetLastError());
 }

@@ -149,3 +149,20 @@ def test_leveled_below_first_level_refused(leveled):
     ph.mod_switch_to_next_inplace(ctx, f)
     ph.apply_galois_inplace(ctx, f, elt, gk)
     assert f.chain_index() == 2
+
+
+@pytest.mark.parametrize("n", [2, 4096, 4096 + 6, 3 * 4096 + 2, 1 << 20])
+def test_copy_kernel_sizes(n):
+    """the streaming copy the bench calibrates its counters with: whole 32 KiB
+    chunks and a partial last chunk copy every word and nothing beyond"""
+    say(f"case: copy_u64 n={n}")
+    rng = np.random.default_rng(n)
+    src = rng.integers(0, 1 << 63, n, dtype=np.uint64)
+    guard = np.full(8, 0x5A5A5A5A5A5A5A5A, dtype=np.uint64)
+    a = ph.DeviceBuffer.from_array(src)
+    b = ph.DeviceBuffer.from_array(np.concatenate([np.zeros(n, dtype=np.uint64), guard]))
+    ph.copy_u64(b, a, n)
+    ph.device_synchronize()
+    out = b.to_array((n + 8,))
+    assert np.array_equal(out[:n], src)
+    assert np.array_equal(out[n:], guard)
