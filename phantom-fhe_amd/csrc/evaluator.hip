@@ -586,21 +586,27 @@ __global__ void k_dot_secret(const u64 *ct, const u64 *s, const u64 *s2, u64 *ou
 }
 
 // 16-byte streaming copy (nontemporal): each workgroup moves one 32 KiB chunk,
-// every thread issuing its eight loads before its stores (~6.5 TB/s for 1 GiB
+// every thread issuing its eight loads before its stores (~6.2 TB/s for 1 GiB
 // each way on MI355X, against ~6.0 TB/s for a grid-stride loop over four
 // workgroups per CU and ~4.6 TB/s for one vector per thread)
 typedef unsigned int copy_vec __attribute__((ext_vector_type(4)));
 constexpr u32 kCopyVecs = 8;   // vectors per thread
 __global__ __launch_bounds__(kThreads) void k_copy(const copy_vec *__restrict__ src, copy_vec *__restrict__ dst,
                                                    u64 n16) {
-    const u64 base = (u64)blockIdx.x * (kThreads * kCopyVecs) + threadIdx.x;
-    copy_vec v[kCopyVecs];
+    const u64 first = (u64)blockIdx.x * (kThreads * kCopyVecs);
+    if (first + kThreads * kCopyVecs <= n16) {
+        // a whole chunk: no bounds checks, which would serialise the loads
+        const copy_vec *s = src + first + threadIdx.x;
+        copy_vec *d = dst + first + threadIdx.x;
+        copy_vec v[kCopyVecs];
 #pragma unroll
-    for (u32 e = 0; e < kCopyVecs; ++e)
-        if (base + e * kThreads < n16) v[e] = __builtin_nontemporal_load(src + base + e * kThreads);
+        for (u32 e = 0; e < kCopyVecs; ++e) v[e] = __builtin_nontemporal_load(s + e * kThreads);
 #pragma unroll
-    for (u32 e = 0; e < kCopyVecs; ++e)
-        if (base + e * kThreads < n16) __builtin_nontemporal_store(v[e], dst + base + e * kThreads);
+        for (u32 e = 0; e < kCopyVecs; ++e) __builtin_nontemporal_store(v[e], d + e * kThreads);
+    } else {
+        for (u64 i = first + threadIdx.x; i < n16; i += kThreads)
+            __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+    }
 }
 
 void copy_u64(u64 *dst, const u64 *src, size_t n, hipStream_t s) {
