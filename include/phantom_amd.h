@@ -388,7 +388,8 @@ int phx_tune_get(const char *key, int *value);
  *   bconv_mfma          1 = FastBConv on the matrix cores, 0 = VALU kernel
  *   ntt_cols_threads    tile shape of the column passes: 256 or 512 threads x 16
  *                       coefficients (16 or 32 columns at N = 2^16); 0 = 512 at
- *                       N = 2^16 for launches of at least 256 limb-transforms, else 256
+ *                       N = 2^17, and at N = 2^16 for launches of at least 256
+ *                       limb-transforms, else 256
  *   ntt_rows_threads    tile shape of the row passes: 128, 256 or 512 threads
  *                       (8, 16 or 32 rows of 256 coefficients)
  *   bconv_wgs_per_cu    matrix-core FastBConv workgroups per CU: 1, 2, 4 (0 = as many

@@ -61,7 +61,8 @@ __device__ __forceinline__ void fwd_cols_body(const A &a, u64 *lds, u64 *data, u
     constexpr int R = 1 << SR;
     constexpr int TC = (NT * kNttRegs) / R;
     if constexpr (SR > 8) {
-        // N = 2^17: three rounds of 4 + 4 + 1 stages over 512 rows x 8 columns.
+        // N = 2^17: three rounds of 4 + 4 + 1 stages over 512 rows x TC columns
+        // (8 at 256 threads, 16 at 512).
         // Row bits (high to low): 4 for round A, 4 for round B, 1 for round C.
         static_assert(SR == 9, "the three-round column pass covers 512 rows");
         T v[16];
@@ -467,12 +468,14 @@ inline u32 pick_threads(int ctx_choice, const char *key, u32 N) {
 // writes its limb in 128-byte pieces 2 KiB apart; 512 threads (32 columns, 256-byte
 // pieces) move the same bytes faster (752-limb forward pair 0.340 -> 0.331 ms) but
 // halve the workgroups, which costs launches over few limbs (the plain HMult+relin's
-// 24..106-limb passes: -1%). Choice 0 takes 512 threads at N = 2^16 for launches of
-// at least kWideColsMin limb-transforms, else 256.
+// 24..106-limb passes: -1%). At N = 2^17 a 256-thread tile holds 8 columns (64-byte
+// pieces) and 512 threads are faster for any launch (384 limbs: 0.490 -> 0.378 ms).
+// Choice 0 takes 512 threads at N = 2^17, and at N = 2^16 for launches of at least
+// kWideColsMin limb-transforms; 256 otherwise.
 constexpr u64 kWideColsMin = 256;
 inline u32 cols_threads(const NttTables &T, u64 transforms) {
     int choice = T.cols_nt >= 0 ? T.cols_nt : tune_value("ntt_cols_threads");
-    if (choice == 0) choice = T.logN == 16 && transforms >= kWideColsMin ? 512 : 256;
+    if (choice == 0) choice = T.logN >= 17 || (T.logN == 16 && transforms >= kWideColsMin) ? 512 : 256;
     const u32 nt = pick_threads(choice, "ntt_cols_threads", T.N);
     // column passes run on 256 or 512 threads, on 128 only where N = 2^11 needs it
     return nt == 128 && 256 * kNttRegs <= T.N ? 256 : nt;

@@ -11,8 +11,8 @@
 //                slices of whole polynomials of about this many limbs (0 = never)
 //   ntt_cols_threads / ntt_rows_threads  tile shape of the two passes: threads per
 //                workgroup (16 registers each): columns 256 or 512, or 0 (default)
-//                for 512 at N = 2^16 in launches of at least 256 limb-transforms and
-//                256 otherwise; rows 128, 256 (default) or 512
+//                for 512 at N = 2^17, and at N = 2^16 in launches of at least 256
+//                limb-transforms, 256 otherwise; rows 128, 256 (default) or 512
 //   bconv_wgs_per_cu  FastBConv on the matrix cores: workgroups per CU of its grid
 //                (1, 2, 4; 0 = as many as the LDS holds, the default)
 //   bconv_fused  1: FastBConv on the matrix cores fused with the column pass of the
