@@ -182,14 +182,19 @@ __device__ __forceinline__ void fwd_rows_core(const A &a, u64 *lds, const u64 *d
     using T = typename A::T;
     const u32 l = t & 15, u = t >> 4;
     T v[16];
-    u32 baseA[1] = {(r0 + u) * 256 + l};
+    const u32 baseA = (r0 + u) * 256 + l;
+    // round 1's twiddles load ahead of the values (from L2, they arrive first; the
+    // waits complete in issue order, so the round's first stages can start while
+    // the values' later loads are still in flight)
+    typename A::W w1g[15];
+    if constexpr (!FromLds) load_round_tw<4>(a, baseA, logN, w1g);
 #pragma unroll
     for (int e = 0; e < 16; ++e)
-        v[e] = A::from_bits(FromLds ? lds[row_addr(u, ((u32)e << 4) + l)] : ld_mid(data + baseA[0] + ((u32)e << 4)));
+        v[e] = A::from_bits(FromLds ? lds[row_addr(u, ((u32)e << 4) + l)] : ld_mid(data + baseA + ((u32)e << 4)));
     if constexpr (FromLds)
         fwd_round_w(v, w1, a);
     else
-        fwd_round<4, 4>(v, baseA, logN, a);
+        fwd_round_w(v, w1g, a);
 #pragma unroll
     for (int e = 0; e < 16; ++e) lds[row_addr(u, ((u32)e << 4) + l)] = A::to_bits(v[e]);
     // round 2's twiddles load while the values cross the barrier
@@ -236,9 +241,15 @@ __device__ __forceinline__ void fwd_rows_core_rb(const A &a, u64 *lds, const u64
     u32 base[BB];
 #pragma unroll
     for (int b = 0; b < BB; ++b) base[b] = (rb + ((u32)b << NB)) * 256 + cc;
+    // a full radix-16 column round (N = 2^16) loads its twiddles ahead of the values
+    typename A::W wB[15];
+    if constexpr (NB == 4) load_round_tw<8>(a, base[0], logN, wB);
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = A::from_bits(ld_mid(data + (rb + (u32)e) * 256 + cc));
-    fwd_round<NB, 8>(v, base, logN, a);
+    if constexpr (NB == 4)
+        fwd_round_w(v, wB, a);
+    else
+        fwd_round<NB, 8>(v, base, logN, a);
 #pragma unroll
     for (int e = 0; e < 16; ++e) lds[row_addr(16 * (t >> 8) + (u32)e, cc)] = A::to_bits(v[e]);
     // the row pass's round 1 twiddles load while the values cross the barrier
@@ -394,12 +405,19 @@ __device__ __forceinline__ void inv_cols_body(const A &a, u64 *lds, u64 *data, u
             hpB[b] = kap / TC;
             baseB[b] = (hpB[b] << NB) * 256 + c0 + uB[b];
         }
+        // a full radix-16 first round (N = 2^16) loads its twiddles ahead of the
+        // values (from L2, they arrive first, and waits complete in issue order)
+        typename A::W wB[15];
+        if constexpr (NB == 4) load_round_tw<8>(a, baseB[0], logN, wB);
 #pragma unroll
         for (int b = 0; b < BB; ++b)
 #pragma unroll
             for (int e = 0; e < (1 << NB); ++e)
                 v[b * (1 << NB) + e] = A::from_bits(ld_mid(data + baseB[b] + ((u32)e << 8)));
-        inv_round<NB, 8>(v, baseB, logN, a);
+        if constexpr (NB == 4)
+            inv_round_w(v, wB, a);
+        else
+            inv_round<NB, 8>(v, baseB, logN, a);
 #pragma unroll
         for (int b = 0; b < BB; ++b)
 #pragma unroll
