@@ -5,7 +5,8 @@
 // A body runs on NT threads x 16 registers, a tile of 16 NT coefficients: the
 // column passes see TC = 16 NT / R columns of all R rows, the row passes
 // NT / 16 rows of 256 coefficients. NT is the tile shape the autotuner picks
-// (256 is the default: 16 columns and 16 rows at N = 2^16).
+// (by default 256 for the row passes, 16 rows; the column passes take 256 or 512
+// by ring degree and launch size, cols_threads below).
 #pragma once
 
 #include <algorithm>
